@@ -1,0 +1,219 @@
+// Shared device helpers of the MI355X Navier-Stokes kernels: complex arithmetic,
+// LDS Stockham FFTs of arbitrary 2^a 3^b 5^c length, real transforms built on them
+// (DCT-II by Makhoul's reordering, discrete Hartley), deterministic block reductions
+// and order-independent max atomics.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+namespace aero {
+
+// ------------------------------------------------------------------ errors (host)
+void set_error(const std::string &msg);
+
+#define AERO_HIP_CHECK(expr)                                                        \
+  do {                                                                              \
+    hipError_t e_ = (expr);                                                         \
+    if (e_ != hipSuccess) {                                                         \
+      ::aero::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));         \
+      return AERO_EHIP;                                                             \
+    }                                                                               \
+  } while (0)
+
+// ------------------------------------------------------------------ complex
+template <typename T>
+struct cplx {
+  T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cadd(cplx<T> a, cplx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T> b) { return {a.x - b.x, a.y - b.y}; }
+
+// ------------------------------------------------------------------ FFT plan
+// Radix sequence of an N-point transform and the twiddle table exp(-2 pi i m / N).
+constexpr int kMaxRadices = 24;
+struct FftPlan {
+  int n;
+  int nrad;
+  int radix[kMaxRadices];
+};
+
+// Forward radix-R butterflies (sign -1).
+template <typename T>
+__device__ __forceinline__ void dft2(cplx<T> *v) {
+  cplx<T> a = v[0], b = v[1];
+  v[0] = cadd(a, b);
+  v[1] = csub(a, b);
+}
+template <typename T>
+__device__ __forceinline__ void dft3(cplx<T> *v) {
+  const T s = (T)0.86602540378443864676;  // sin(2 pi / 3)
+  cplx<T> a = v[0], b = v[1], c = v[2];
+  cplx<T> p = cadd(b, c), z = csub(b, c);
+  cplx<T> t = {a.x - (T)0.5 * p.x, a.y - (T)0.5 * p.y};
+  v[0] = cadd(a, p);
+  v[1] = {t.x + s * z.y, t.y - s * z.x};
+  v[2] = {t.x - s * z.y, t.y + s * z.x};
+}
+template <typename T>
+__device__ __forceinline__ void dft4(cplx<T> *v) {
+  cplx<T> s0 = cadd(v[0], v[2]), s1 = csub(v[0], v[2]);
+  cplx<T> s2 = cadd(v[1], v[3]), s3 = csub(v[1], v[3]);
+  v[0] = cadd(s0, s2);
+  v[2] = csub(s0, s2);
+  v[1] = {s1.x + s3.y, s1.y - s3.x};
+  v[3] = {s1.x - s3.y, s1.y + s3.x};
+}
+template <typename T>
+__device__ __forceinline__ void dft5(cplx<T> *v) {
+  const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;
+  const T s1 = (T)0.95105651629515357212, s2 = (T)0.58778525229247312917;
+  cplx<T> a0 = v[0];
+  cplx<T> b1 = cadd(v[1], v[4]), b2 = cadd(v[2], v[3]);
+  cplx<T> d1 = csub(v[1], v[4]), d2 = csub(v[2], v[3]);
+  cplx<T> t1 = {a0.x + c1 * b1.x + c2 * b2.x, a0.y + c1 * b1.y + c2 * b2.y};
+  cplx<T> t2 = {a0.x + c2 * b1.x + c1 * b2.x, a0.y + c2 * b1.y + c1 * b2.y};
+  cplx<T> u1 = {s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y};
+  cplx<T> u2 = {s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y};
+  v[0] = {a0.x + b1.x + b2.x, a0.y + b1.y + b2.y};
+  v[1] = {t1.x + u1.y, t1.y - u1.x};
+  v[4] = {t1.x - u1.y, t1.y + u1.x};
+  v[2] = {t2.x + u2.y, t2.y - u2.x};
+  v[3] = {t2.x - u2.y, t2.y + u2.x};
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void stockham_stage(const cplx<T> *__restrict__ src, cplx<T> *__restrict__ dst,
+                                               int n, int p, const cplx<T> *__restrict__ tw, int nslot,
+                                               int slot_stride, int tid, int nthr) {
+  const int nr = n / R;
+  const int stride = n / (p * R);
+  for (int w = tid; w < nslot * nr; w += nthr) {
+    const int f = w / nr;
+    const int k = w - f * nr;
+    const cplx<T> *s = src + f * slot_stride;
+    cplx<T> *d = dst + f * slot_stride;
+    const int i = k % p;
+    cplx<T> v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = s[k + r * nr];
+    if (p > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[r * i * stride]);
+    }
+    if constexpr (R == 2) dft2(v);
+    if constexpr (R == 3) dft3(v);
+    if constexpr (R == 4) dft4(v);
+    if constexpr (R == 5) dft5(v);
+    const int j = (k - i) * R + i;
+#pragma unroll
+    for (int r = 0; r < R; ++r) d[j + r * p] = v[r];
+  }
+}
+
+// Forward complex FFT of `nslot` length-n sequences stored slot_stride apart in LDS
+// buffer a (b is a scratch buffer of the same size). Returns the buffer holding the
+// result in natural order. All threads of the block must call it.
+template <typename T>
+__device__ cplx<T> *fft_lds(cplx<T> *a, cplx<T> *b, const FftPlan &pl, const cplx<T> *__restrict__ tw,
+                            int nslot, int slot_stride) {
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  const int nthr = blockDim.x * blockDim.y;
+  cplx<T> *src = a, *dst = b;
+  int p = 1;
+  for (int s = 0; s < pl.nrad; ++s) {
+    const int R = pl.radix[s];
+    switch (R) {
+      case 4: stockham_stage<T, 4>(src, dst, pl.n, p, tw, nslot, slot_stride, tid, nthr); break;
+      case 2: stockham_stage<T, 2>(src, dst, pl.n, p, tw, nslot, slot_stride, tid, nthr); break;
+      case 3: stockham_stage<T, 3>(src, dst, pl.n, p, tw, nslot, slot_stride, tid, nthr); break;
+      default: stockham_stage<T, 5>(src, dst, pl.n, p, tw, nslot, slot_stride, tid, nthr); break;
+    }
+    __syncthreads();
+    cplx<T> *t = src;
+    src = dst;
+    dst = t;
+    p *= R;
+  }
+  return src;
+}
+
+// ------------------------------------------------------------------ real transforms of pairs
+// Two real sequences xa, xb of length n are packed into one complex sequence
+// z = xa + i xb; one complex FFT gives both spectra (Hermitian split).
+//
+// DCT-II (no factor 2):  X_k = sum_n x_n cos(pi k (2n+1) / 2n)
+//   Makhoul: v_m = x_{2m}, v_{n-1-m} = x_{2m+1}; V = FFT(v); X_k = Re(e^{-i pi k / 2n} V_k).
+// DHT:  H_k = sum_n x_n cas(2 pi k n / n) = Re F_k - Im F_k.
+// Inverses: DCT-III scaled so that idct(dct(x)) = x; DHT is its own inverse up to 1/n.
+//
+// Index of the reordered position m of the natural index q (Makhoul order).
+__device__ __forceinline__ int makhoul_pos(int q, int n) { return (q & 1) ? (n - 1 - (q >> 1)) : (q >> 1); }
+
+// After fft_lds of packed (xa + i xb): spectra of the two real inputs at k.
+template <typename T>
+__device__ __forceinline__ void split_pair(const cplx<T> *Z, int n, int k, cplx<T> &va, cplx<T> &vb) {
+  const cplx<T> z = Z[k];
+  const cplx<T> zc = Z[k == 0 ? 0 : n - k];
+  va = {(T)0.5 * (z.x + zc.x), (T)0.5 * (z.y - zc.y)};
+  vb = {(T)0.5 * (z.y + zc.y), (T)0.5 * (zc.x - z.x)};
+}
+
+// ------------------------------------------------------------------ reductions
+// Block-wide sum (deterministic order for a fixed block size); every thread gets the result.
+template <typename T>
+__device__ T block_sum(T v, T *lds) {
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  const int nthr = blockDim.x * blockDim.y;
+  __syncthreads();
+  lds[tid] = v;
+  __syncthreads();
+  for (int s = 1; s < nthr; s <<= 1) {
+    // pairwise tree over the power-of-two block
+    if ((tid % (2 * s)) == 0 && tid + s < nthr) lds[tid] = lds[tid] + lds[tid + s];
+    __syncthreads();
+  }
+  T r = lds[0];
+  __syncthreads();
+  return r;
+}
+
+template <typename T>
+__device__ T block_max(T v, T *lds) {
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  const int nthr = blockDim.x * blockDim.y;
+  __syncthreads();
+  lds[tid] = v;
+  __syncthreads();
+  for (int s = nthr / 2; s > 0; s >>= 1) {
+    if (tid < s) lds[tid] = lds[tid] > lds[tid + s] ? lds[tid] : lds[tid + s];
+    __syncthreads();
+  }
+  T r = lds[0];
+  __syncthreads();
+  return r;
+}
+
+// Max of non-negative values as an unsigned-integer atomic (order independent).
+__device__ __forceinline__ void atomic_max_nonneg(float *addr, float v) {
+  atomicMax(reinterpret_cast<unsigned int *>(addr), __float_as_uint(v));
+}
+__device__ __forceinline__ void atomic_max_nonneg(double *addr, double v) {
+  atomicMax(reinterpret_cast<unsigned long long *>(addr), (unsigned long long)__double_as_longlong(v));
+}
+
+template <typename T>
+__device__ __forceinline__ T load_relaxed(const T *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+}  // namespace aero

@@ -1,0 +1,144 @@
+/*
+ * aerolume_amd - C ABI of the MI355X Navier-Stokes step (libaerolume_amd.so).
+ *
+ * Plain C types only: device pointers are `void *`, the HIP stream is passed as
+ * `void *` (a hipStream_t; NULL = the null stream). The caller owns every field
+ * buffer; a plan owns only its scratch (stage buffers, spectral work arrays,
+ * twiddle and eigenvalue tables, reduction slots), allocated once at creation.
+ * Nothing inside a step allocates, frees or synchronises with the host, so a
+ * step can be captured in a hipGraph. Return codes: 0 = OK, negative = error;
+ * aero_last_error() gives the message of the calling thread's last failure.
+ *
+ * Reference interfaces these entry points replace (TanmayPasula/Aerolume,
+ * src/aerolume/solvers/...):
+ *   aero2d_plan_create   - ns2d/core.py:179-202 make_operators (static operators,
+ *                          PoissonSolver construction ns2d/poisson.py:107-126)
+ *   aero2d_step          - ns2d/core.py:815-969 step_full (one SSP-RK3 step with
+ *                          projection, penalisation, forces, temperature, smoke),
+ *                          called as ns2d/solver.py:384-400 _step_static does
+ *   aero2d_project       - ns2d/core.py:626-642 project
+ *   aero2d_poisson_solve - ns2d/poisson.py:128-134 PoissonSolver.solve
+ */
+#ifndef AEROLUME_AMD_H
+#define AEROLUME_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AERO_OK 0
+#define AERO_EINVAL (-1)        /* bad descriptor or argument */
+#define AERO_EUNSUPPORTED (-2)  /* feature combination the step does not carry */
+#define AERO_EHIP (-3)          /* a HIP runtime call failed */
+
+/* boundary kinds */
+#define AERO_BC_SLIP 0
+#define AERO_BC_NOSLIP 1
+#define AERO_BC_MOVING 2
+/* temperature boundary modes (left, right, bottom, top) */
+#define AERO_TH_ADIABATIC 0
+#define AERO_TH_TEMPERATURE 1
+#define AERO_TH_HEAT_FLUX 2
+#define AERO_TH_INLET 3
+#define AERO_TH_OUTLET 4
+#define AERO_TH_PERIODIC 5
+
+#define AERO2D_MAX_GROUPS 8
+
+typedef struct aero2d_plan aero2d_plan;
+
+/* Static description of a 2D solve (ns2d/config.py:369-516 after set-up). */
+typedef struct {
+  int32_t nx, ny;          /* cells */
+  int32_t dtype;           /* 0 = float32, 1 = float64 */
+  int32_t periodic;        /* doubly periodic box */
+  int32_t closed;          /* walls on all four sides */
+  int32_t advection;       /* 0 = central, 1 = upwind3 */
+  int32_t turbulence;      /* 0 none, 1 smagorinsky, 2 wale, 3 vreman */
+  int32_t bc_bottom, bc_top, bc_left, bc_right;   /* AERO_BC_* */
+  int32_t fixed_dt;        /* 1: dt from params.dt_fixed, else CFL rule */
+  int32_t win_j0, win_j1, win_i0, win_i1;         /* body cell window (empty: j1 <= j0) */
+  int32_t nbody;
+  int32_t thermal;         /* temperature carried */
+  int32_t buoyant;         /* Ri != 0 */
+  int32_t wall_modes[4];   /* AERO_TH_* for left, right, bottom, top */
+  int32_t n_temp;          /* temperature bodies */
+  int32_t has_heat;        /* volumetric heater field present */
+  int32_t insulated;       /* thermal masks on the window (diffusivity x ins) */
+  int32_t dye_mode;        /* 0 none, 1 inlet comb (single channel), 2 channels + emitters */
+  int32_t n_dye;           /* smoke channels */
+  int32_t n_groups;        /* emitter groups */
+  int32_t group_channel[AERO2D_MAX_GROUPS];
+  double h;
+  double ground_speed;     /* moving bottom wall speed */
+  double cfl;
+  double force_ref;
+  double les_c;            /* model constant */
+  double gravity_x, gravity_y;
+  double prt;              /* turbulent Prandtl number */
+  double diff_coef;        /* largest temperature diffusion coefficient / (kappa / h^2) */
+} aero2d_desc;
+
+/* Evolving fields, caller-owned device buffers of the plan's dtype.
+ * u (ny, nx+1), v (ny+1, nx), p (ny, nx), theta (ny, nx), dye (n_dye, ny, nx),
+ * t and dt 0-d. in == out (in place) is allowed for every field. */
+typedef struct {
+  void *u, *v, *p, *theta, *dye, *t, *dt;
+} aero2d_fields;
+
+/* Window masks (ns2d/geometry.py:160-197, 120-157); wy = j1-j0, wx = i1-i0.
+ * chi_u, a_u (wy, wx+1); chi_v, a_v (wy+1, wx); chi_c (wy, wx);
+ * k_u (nbody, 2, wy, wx+1); k_v (nbody, 2, wy+1, wx);
+ * chi_t, a_t (wy, wx); w_t (n_temp, wy, wx); vol_u (wy, wx+1); vol_v (wy+1, wx);
+ * ins_u (wy, wx+1); ins_v (wy+1, wx). */
+typedef struct {
+  const void *chi_u, *chi_v, *chi_c, *a_u, *a_v, *k_u, *k_v;
+  const void *chi_t, *a_t, *w_t, *vol_u, *vol_v, *ins_u, *ins_v;
+} aero2d_geom;
+
+/* Live values as device scalars / arrays (edited in place between steps, so a
+ * captured graph stays valid): nu, eta, dt_fixed, kappa, ri, eta_t, theta_in,
+ * heat_max are 0-d; u_inlet (ny); wall_values (4); heat (ny, nx);
+ * dye_inflow (n_dye, ny); group_weight, group_target (n_groups, ny, nx);
+ * group_timing (n_groups, 4) = [start, stop, period, duty]. */
+typedef struct {
+  const void *nu, *eta, *u_inlet, *dt_fixed;
+  const void *kappa, *ri, *eta_t, *theta_in, *wall_values, *heat, *heat_max;
+  const void *dye_inflow, *group_weight, *group_target, *group_timing;
+} aero2d_params;
+
+/* Per-step outputs (device; any may be NULL): coeffs (nbody, 3) = [Cd, Cl, Cm];
+ * heat (n_temp); wall_heat (4); heat_storage (1); flow_heat (2).
+ * hist: ring of hist_cap rows [t, Cd, Cl, Cm] (summed over bodies) written at
+ * row (*hist_count mod hist_cap), then *hist_count += 1 (int64 device counter).
+ * aux: ring of aux_cap rows [t, heat.., wall_heat(4), heat_storage, flow_heat(2)]
+ * with counter aux_count. */
+typedef struct {
+  void *coeffs, *heat, *wall_heat, *heat_storage, *flow_heat;
+  void *hist, *hist_count;
+  int64_t hist_cap;
+  void *aux, *aux_count;
+  int64_t aux_cap;
+} aero2d_outputs;
+
+int aero2d_plan_create(const aero2d_desc *desc, aero2d_plan **plan);
+int aero2d_plan_destroy(aero2d_plan *plan);
+size_t aero2d_plan_workspace_bytes(const aero2d_plan *plan);
+int aero2d_step(aero2d_plan *plan, void *stream, const aero2d_fields *in, aero2d_fields *out,
+                const aero2d_geom *geom, const aero2d_params *params, aero2d_outputs *outputs);
+/* u, v in place: u - grad psi with lap psi = div u (no inlet/outlet change); psi (ny, nx) optional. */
+int aero2d_project(aero2d_plan *plan, void *stream, void *u, void *v, void *psi);
+/* psi = L^-1 f with zero mean (f, psi (ny, nx); may alias). */
+int aero2d_poisson_solve(aero2d_plan *plan, void *stream, const void *f, void *psi);
+
+const char *aero_last_error(void);
+int aero_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AEROLUME_AMD_H */

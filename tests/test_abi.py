@@ -1,0 +1,55 @@
+"""The C-ABI library loads and exports every function ``include/aerolume_amd.h`` declares;
+plan creation validates its descriptor (no GPU needed for either)."""
+from __future__ import annotations
+
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def _declared():
+    names = set()
+    for hdr in (ROOT / "include").glob("*.h"):
+        text = re.sub(r"/\*.*?\*/", "", hdr.read_text(), flags=re.S)
+        names |= set(re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\**\s*(aero[a-z0-9_]*)\s*\(", text, flags=re.M))
+    return sorted(names)
+
+
+def test_header_declares_entry_points():
+    names = _declared()
+    for n in ("aero2d_plan_create", "aero2d_step", "aero2d_project", "aero2d_poisson_solve", "aero_last_error"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    from aerolume_amd import _lib
+    lib = _lib.load()
+    missing = [n for n in _declared() if not hasattr(lib, n)]
+    assert not missing, missing
+    assert lib.aero_version() >= 1
+
+
+def test_plan_create_rejects_bad_descriptors():
+    from aerolume_amd import _lib
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    d = _lib.Desc2D()
+    d.nx, d.ny, d.h = 2, 2, 0.1
+    assert lib.aero2d_plan_create(ctypes.byref(d), ctypes.byref(h)) == _lib.AERO_EINVAL
+    assert b"4 x 4" in lib.aero_last_error()
+    d.nx, d.ny = 14, 8   # 14 = 2 * 7: not 5-smooth
+    rc = lib.aero2d_plan_create(ctypes.byref(d), ctypes.byref(h))
+    assert rc == _lib.AERO_EUNSUPPORTED
+    with pytest.raises(NotImplementedError):
+        _lib.check(rc, "plan")
+
+
+def test_solver_refuses_cpu_device():
+    from aerolume_amd.ns2d import solver as S
+    from tests import cases
+    with pytest.raises(RuntimeError, match="GPU"):
+        S.make_operators(cases.trace_cfg("float64", device="cpu"))

@@ -1,0 +1,199 @@
+"""GPU parity of the 2D step: HIP kernels (through the C ABI) against the CPU oracle and
+the reference's golden trajectories. Bars: float64 to round-off (1e-10 or tighter), float32
+within the tolerances written in each test."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from tests import cases
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device("cuda:0")
+
+
+def _need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def _S():
+    from aerolume_amd.ns2d import solver as S
+    return S
+
+
+def _setup(cfg, window=None, live=False):
+    from aerolume_amd.ns2d import setup as su
+    S = _S()
+    ops = S.make_operators(cfg, device=DEV)
+    if live:
+        window = su.live_window(cfg.bodies, ops)
+    geom = su.build_geometry(cfg.bodies, ops, window=window)
+    return ops, geom
+
+
+def _state(S, ops, u, v, theta=None, dye=None):
+    z = torch.zeros((), device=DEV, dtype=ops.dtype)
+    to = lambda a: None if a is None else torch.as_tensor(a, device=DEV, dtype=ops.dtype)  # noqa: E731
+    return S.State(to(u), to(v), torch.zeros(ops.ny, ops.nx, device=DEV, dtype=ops.dtype), to(dye), z.clone(),
+                   z.clone(), to(theta))
+
+
+def _err(a, b):
+    return float(np.abs(np.asarray(a.detach().cpu() if torch.is_tensor(a) else a, dtype=np.float64)
+                        - np.asarray(b, dtype=np.float64)).max())
+
+
+@pytest.mark.parametrize("nx,ny,periodic,dtype", [(64, 32, False, "float64"), (30, 20, False, "float64"),
+                                                  (48, 40, True, "float64"), (64, 32, False, "float32"),
+                                                  (30, 20, True, "float32")])
+def test_poisson_solve_matches_oracle(nx, ny, periodic, dtype):
+    """Exact DCT/DHT solve on the GPU equals the oracle's and satisfies L psi = f - mean."""
+    _need_gpu()
+    from aerolume_amd.ns2d.config import SolverConfig
+    from oracle import ns2d as O
+    S = _S()
+    print(f"poisson {nx}x{ny} periodic={periodic} {dtype}", flush=True)
+    cfg = SolverConfig(nx=nx, ny=ny, h=0.1, x0=0, y0=0, re=100, dtype=dtype, device="cpu", dye=False,
+                       perturb=False, periodic=periodic)
+    ops, geom = _setup(cfg)
+    tdt = torch.float64 if dtype == "float64" else torch.float32
+    f = torch.as_tensor(np.random.default_rng(3).standard_normal((ny, nx)), dtype=tdt)
+    f = f - f.mean()
+    oracle = O.PoissonSolver(nx, ny, 0.1, periodic=periodic, dtype=tdt)
+    ref = oracle.solve(f)
+    pl = S.get_plan(ops, geom)
+    psi = torch.empty(ny, nx, device=DEV, dtype=tdt)
+    pl.poisson(f.to(DEV), psi)
+    torch.cuda.synchronize()
+    scale = float(ref.abs().max())
+    tol = 1e-12 if dtype == "float64" else 2e-5
+    assert _err(psi, ref) <= tol * scale
+    res = oracle.laplacian(psi.cpu().to(torch.float64)) - f.to(torch.float64)
+    assert float(res.abs().max()) <= (1e-9 if dtype == "float64" else 1e-3) * float(f.abs().max())
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_projection_removes_divergence(dtype):
+    """Projected white noise is divergence-free to round-off; boundary-normal faces untouched."""
+    _need_gpu()
+    S = _S()
+    cfg = cases.trace_cfg(dtype)
+    ops, geom = _setup(cfg)
+    tdt = ops.dtype
+    rng = np.random.default_rng(11)
+    u = torch.as_tensor(rng.standard_normal((ops.ny, ops.nx + 1)), dtype=tdt, device=DEV)
+    v = torch.as_tensor(rng.standard_normal((ops.ny + 1, ops.nx)), dtype=tdt, device=DEV)
+    v[0] = 0
+    v[-1] = 0
+    u, v = S.apply_boundary_values(u, v, ops)
+    un, vn, _ = S.project(u, v, ops)
+    div = S.divergence(un, vn, ops.h)
+    lim = 1e-10 if dtype == "float64" else 1e-4
+    assert float(div.abs().max()) * ops.h < lim
+    assert torch.equal(un[:, 0], u[:, 0]) and torch.equal(un[:, -1], u[:, -1])
+    assert torch.equal(vn[0], v[0]) and torch.equal(vn[-1], v[-1])
+
+
+def test_traced_step_matches_reference(golden):
+    """One float64 step of the 64x32 disc case and 20 more equal the reference's to 1e-10."""
+    _need_gpu()
+    S = _S()
+    F = golden("ns2d_trace_64x32.npz")
+    ops, geom = _setup(cases.trace_cfg("float64"))
+    st = _state(S, ops, F["u0"], F["v0"])
+    new, c = S.step_full(st, geom, ops)
+    print("trace step 1", flush=True)
+    assert _err(new.u, F["u1"]) < 1e-12
+    assert _err(new.v, F["v1"]) < 1e-12
+    assert _err(new.p, F["p1"]) < 1e-10
+    assert _err(new.dt, F["dt1"]) < 1e-15
+    assert _err(c, F["coeffs1"]) < 1e-12
+    cur = new
+    for _ in range(20):
+        cur, c = S.step_full(cur, geom, ops)
+    assert _err(cur.u, F["u21"]) < 1e-11
+    assert _err(cur.p, F["p21"]) < 1e-9
+    assert _err(c, F["coeffs_hist"][-1]) < 1e-10
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_c1_500_steps_match_reference(golden, dtype):
+    """256x128 disc case, 500 steps from the seeded noisy start, against the reference's
+    trajectory. float64: 1e-9; float32: |du|,|dv| <= 5e-5 and |dp| <= 2e-4 (the reference's
+    own FP32-vs-FP64 spread at step 500 is of that order); divergence no higher than the
+    reference's float32 state."""
+    _need_gpu()
+    S = _S()
+    tag = "f64" if dtype == "float64" else "f32"
+    F = golden(f"ns2d_c1_{tag}.npz")
+    ops, geom = _setup(cases.c1_cfg(dtype))
+    cur = _state(S, ops, F["u0"], F["v0"])
+    tol_u = 1e-9 if dtype == "float64" else 5e-5
+    tol_p = 1e-8 if dtype == "float64" else 2e-4
+    for n in range(1, 501):
+        cur, c = S.step_full(cur, geom, ops)
+        if f"u{n}" in F.files:
+            print(f"c1 {dtype} step {n}", flush=True)
+            assert _err(cur.u, F[f"u{n}"]) < tol_u, n
+            assert _err(cur.v, F[f"v{n}"]) < tol_u, n
+            assert _err(cur.p, F[f"p{n}"]) < tol_p, n
+    assert abs(float(cur.t) - float(F["t500"])) < (1e-10 if dtype == "float64" else 1e-4)
+    assert _err(c[0], F["coeffs_hist"][-1]) < (1e-8 if dtype == "float64" else 2e-3)
+    div = float(S.divergence(cur.u, cur.v, ops.h).abs().max())
+    ref = F["u500"], F["v500"]
+    ref_div = float(np.abs((ref[0][:, 1:] - ref[0][:, :-1] + ref[1][1:] - ref[1][:-1]) / ops.h).max())
+    assert div <= max(1.5 * ref_div, 1e-12 if dtype == "float64" else 1e-5)
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_heated_airfoil_miniature_matches_reference(golden, dtype):
+    """128x64 heated-airfoil miniature (Smagorinsky, temperature body, Boussinesq, line
+    emitter), 20 steps: velocity, pressure, theta, smoke and heat outputs against the
+    reference. float64 to 1e-10; float32: 5e-5 on u, v, theta, dye; 1e-3 on p."""
+    _need_gpu()
+    S = _S()
+    tag = "f64" if dtype == "float64" else "f32"
+    F = golden(f"ns2d_c2mini_{tag}.npz")
+    cfg = cases.c2_cfg(128, 64, dtype)
+    ops, geom = _setup(cfg, live=True)
+    assert tuple(geom.window) == tuple(F["g_window"])
+    cur = _state(S, ops, F["u0"], F["v0"], theta=F["theta0"], dye=F["dye0"])
+    f64 = dtype == "float64"
+    for n in range(1, 21):
+        out = {}
+        cur, c = S.step_full(cur, geom, ops, out=out)
+        assert abs(float(cur.dt) - F["hist_dt"][n - 1]) < (1e-15 if f64 else 1e-8)
+        assert _err(c[0], F["hist_coeffs"][n - 1]) < (1e-9 if f64 else 2e-3)
+        assert _err(out["heat"], F["hist_heat"][n - 1]) < (1e-10 if f64 else 1e-4)
+        assert _err(out["wall_heat"], F["hist_wall_heat"][n - 1]) < (1e-12 if f64 else 1e-6)
+        assert _err(out["flow_heat"], F["hist_flow_heat"][n - 1]) < (1e-12 if f64 else 1e-5)
+        assert _err(out["heat_storage"], F["hist_heat_storage"][n - 1]) < (1e-9 if f64 else 2e-2)
+        if n in (1, 20):
+            print(f"c2mini {dtype} step {n}", flush=True)
+            for k in ("u", "v", "theta", "dye"):
+                assert _err(getattr(cur, k), F[f"{k}{n}"]) < (1e-11 if f64 else 5e-5), (k, n)
+            assert _err(cur.p, F[f"p{n}"]) < (1e-9 if f64 else 1e-3), n
+
+
+def test_flowsolver_graphs_equal_eager():
+    """Graph-replayed steps give bitwise the same state as eager steps."""
+    _need_gpu()
+    import dataclasses
+    S = _S()
+    cfg = cases.c2_cfg(128, 64, "float32")
+    a = S.FlowSolver(cfg)
+    b = S.FlowSolver(dataclasses.replace(cfg, use_cuda_graphs=False))
+    assert a.uses_cuda_graphs and not b.uses_cuda_graphs
+    a.advance(25)
+    b.advance(25)
+    torch.cuda.synchronize()
+    for k in ("u", "v", "p", "theta", "dye"):
+        assert torch.equal(getattr(a.state, k), getattr(b.state, k)), k
+    ha, hb = a.history(), b.history()
+    assert len(ha["t"]) == 25 and np.array_equal(ha["cd"], hb["cd"])
+    ph = a.physics_history()
+    assert len(ph["t"]) == 25 and np.isfinite(ph["heat:0"]).all()
+    assert a.graph_error is None

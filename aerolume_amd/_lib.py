@@ -81,8 +81,14 @@ def load() -> ctypes.CDLL:
     lib.aero2d_plan_workspace_bytes.restype = ctypes.c_size_t
     lib.aero2d_step.argtypes = [P, P, ctypes.POINTER(Fields2D), ctypes.POINTER(Fields2D), ctypes.POINTER(Geom2D),
                                 ctypes.POINTER(Params2D), ctypes.POINTER(Outputs2D)]
+    lib.aero2d_step_chained.argtypes = lib.aero2d_step.argtypes
+    lib.aero2d_step_chained.restype = ctypes.c_int
     lib.aero2d_project.argtypes = [P, P, P, P, P]
     lib.aero2d_poisson_solve.argtypes = [P, P, P, P]
+    lib.aero2d_time_stage.argtypes = [P, P, ctypes.POINTER(Fields2D), ctypes.POINTER(Geom2D),
+                                      ctypes.POINTER(Params2D), ctypes.c_int, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_double)]
+    lib.aero2d_time_stage.restype = ctypes.c_int
     for name in ("aero2d_plan_create", "aero2d_plan_destroy", "aero2d_step", "aero2d_project",
                  "aero2d_poisson_solve"):
         getattr(lib, name).restype = ctypes.c_int

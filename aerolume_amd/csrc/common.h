@@ -203,6 +203,43 @@ __device__ T block_max(T v, T *lds) {
   return r;
 }
 
+// Wavefront (64-lane) reductions by butterfly shuffles: every lane gets the result, and the
+// combination order is fixed, so results are deterministic.
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const T w = __shfl_xor(v, o, 64);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+// Block sum of `n` values per thread over a 256-thread block (4 wavefronts): wave sums, then
+// the four wave partials added in order. `lds` holds 4 * n doubles. Thread 0 gets the result.
+template <int N>
+__device__ __forceinline__ void block_sum4(double (&v)[N], double *lds) {
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  const int w = tid >> 6, lane = tid & 63;
+#pragma unroll
+  for (int q = 0; q < N; ++q) v[q] = wave_sum(v[q]);
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < N; ++q) lds[w * N + q] = v[q];
+  }
+  __syncthreads();
+  if (tid == 0) {
+#pragma unroll
+    for (int q = 0; q < N; ++q) v[q] = ((lds[q] + lds[N + q]) + lds[2 * N + q]) + lds[3 * N + q];
+  }
+}
+
 // Max of non-negative values as an unsigned-integer atomic (order independent).
 __device__ __forceinline__ void atomic_max_nonneg(float *addr, float v) {
   atomicMax(reinterpret_cast<unsigned int *>(addr), __float_as_uint(v));

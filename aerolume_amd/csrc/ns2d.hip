@@ -3,24 +3,28 @@
 // One SSP-RK3 step (reference solvers/ns2d/core.py:815-969) is a fixed sequence of
 // launches on the caller's stream, with no host synchronisation and no allocation:
 //
-//   [nut0]  reduce0            CFL / thermal dt limits from the state (core.py:685-695,
-//                              physics.py:851-888); eddy viscosity max when LES is on
-//   3 x  [nut] stage           upwind3/central advection + diffusion + LES stress +
-//                              RK update + inlet/outlet + penalisation + buoyancy +
-//                              temperature stage (core.py:571-614, 855-935)
-//        poisA                 divergence (outlet mass balance folded in) -> DCT/DHT along x
-//        poisB                 DCT/DHT along y, x 1/lambda (FP64-built), inverse along y
-//        poisC                 inverse along x -> psi, step-averaged pressure
-//        correct               u -= grad psi (core.py:645-660)
-//   dye                        smoke transport with the end-of-step velocity
-//   finalize                   forces, heat rates, t, dt, history rings (one block)
+//   [pre, dt]              dt inputs from the state: max |u|, |v|, |theta|, nu_t and
+//                          sum theta (skipped in chained mode: the previous step's last two
+//                          kernels left them) - core.py:685-695, physics.py:851-888
+//   3 x stage              64x8-cell LDS tiles: the previous stage's projection applied on
+//                          the fly (u = u^ - grad psi, outlet mass balance), eddy viscosity
+//                          on the tile + 1 ring, upwind3/central advection, diffusion, LES
+//                          stress, RK update, inlet/outlet, window penalisation with force
+//                          increments, buoyancy, temperature stage (core.py:571-614, 855-935)
+//       poisA              divergence (outlet correction folded in) -> DCT/DHT along x
+//       poisB              DCT/DHT along y, x 1/lambda (FP64-built), inverse along y
+//       poisC              inverse along x -> psi; step-averaged pressure (core.py:935-942)
+//   final                  u = u^ - grad psi written to the state, smoke transport with it,
+//                          per-block partials of the next step's dt inputs, forces, heat
+//   finalize               one block: forces, heat rates, t, dt, next dt, history rings
 //
 // All arithmetic is in the plan's dtype T (float or double); the Poisson eigenvalue
-// inverse is formed in double, and the heat-balance sums accumulate in double.
+// inverse is formed in double, and the heat-balance sums accumulate in double. No
+// atomics: every global reduction goes through per-block partials summed in fixed order.
 #include <hip/hip_runtime.h>
 
-#include <cmath>
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -35,35 +39,34 @@ void set_error(const std::string &msg) { g_error = msg; }
 
 namespace ns2d {
 
-constexpr int kRedBlocks = 1024;      // blocks of the state reduction
 constexpr int kFftThreads = 256;
+constexpr int TX = 64, TY = 8;              // cells per tile (block of 64 x 4 threads, 2 rows each)
+constexpr int HW = TX + 4, HH = TY + 4;     // halo-2 tiles of u, v, scalar
+constexpr int NW = TX + 2, NH = TY + 2;     // eddy-viscosity tile (halo 1)
+constexpr int kMaxWin = 16;                 // force/heat partial slots per block (4 per body + temps)
 constexpr double kSsp[3][2] = {{0.0, 1.0}, {0.75, 0.25}, {1.0 / 3.0, 2.0 / 3.0}};
 constexpr double kForceW[3] = {1.0 / 6.0, 2.0 / 3.0, 1.0};
 
-// Reduction slots shared by the kernels of one plan (device memory, zeroed at creation;
-// every step leaves the max slots and tickets at zero again).
+// Reduction slots of one plan (device memory, zeroed at creation).
 template <typename T>
 struct Red {
-  T umax, vmax, thmax, nutmax;   // max |u0|, |v0|, |theta0|, nu_t (order-free atomics)
-  T dt;                          // this step's dt
-  T corr;                        // outlet correction of the current stage
-  unsigned int ticket;           // last-block detection of reduce0
-  unsigned int pad;
-  double th_old;                 // sum of theta at step start
+  T dt;          // this step's dt
+  T corr;        // outlet correction of the current stage
+  double th_old; // sum of theta at step start
 };
 
 // Everything a kernel needs, passed by value.
 template <typename T>
 struct Dev {
-  int nx, ny;
-  int periodic, closed, upwind, turb;
+  int nx, ny, nbx, nby;
+  int periodic, closed, upwind, turb, les;
   int bcb, bct, bcl, bcr;
   int j0, j1, i0, i1, wx, wy, win;
   int nbody, thermal, buoyant, n_temp, has_heat, insulated;
   int wm[4];
   int dye_mode, n_dye, n_groups;
   int gch[AERO2D_MAX_GROUPS];
-  int fixed_dt, les;
+  int fixed_dt;
   double hd, cfl, force_ref, prt, diff_coef, les_c, gx, gy;
   T h, inv_h, kadv, inv_h2, ground;
   // geometry
@@ -73,27 +76,38 @@ struct Dev {
   const T *nu, *eta, *u_inlet, *dt_fixed, *kappa, *ri, *eta_t, *theta_in, *wall_values, *heat, *heat_max;
   const T *dye_inflow, *g_weight, *g_target, *g_timing;
   // scratch
-  T *nut, *F, *psi, *acc_u, *acc_v, *acc_t, *outcol;
-  double *partial;     // kRedBlocks doubles (reduce0)
-  double *partial2;    // per-block theta sums of the last stage
+  T *F, *psi, *acc_u, *acc_v, *acc_t, *outcol;
+  double *pmax;    // per block: max |u|, |v|, |theta|, nu_t, sum theta
+  double *pforce;  // per block: 4 force sums per body, heat per temperature body
   Red<T> *red;
   const cplx<T> *tw_x, *tw_y, *cs_x, *cs_y;
   const double *lam_x, *lam_y;
   FftPlan fx, fy;
-  int dct_x, dct_y;    // 1: Neumann (DCT-II), 0: periodic (DHT)
+  int dct_x, dct_y;   // 1: Neumann (DCT-II), 0: periodic (DHT)
 };
 
-// Per-stage pointers and coefficients.
 template <typename T>
 struct Stage {
   int s;
   double a, b, w;
-  const T *up, *vp, *thp;   // stage input
-  const T *u0, *v0, *th0;   // step start
-  T *uh, *vh, *thh;         // stage output (before projection)
+  const T *up, *vp, *psi_p;  // stage input velocity (+ the previous stage's potential)
+  const T *thp;              // stage input temperature
+  const T *u0, *v0, *th0;    // step start
+  T *uh, *vh, *thh;          // stage output (before projection)
 };
 
-// ------------------------------------------------------------------ ghost-aware reads
+template <typename T>
+struct Lds {
+  T u[HH][HW];
+  T v[HH][HW];
+  T c[HH][HW];
+  T nu[NH][NW];
+  double red[4 * kMaxWin];
+};
+
+__device__ __forceinline__ int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+__device__ __forceinline__ int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
+
 template <typename T>
 __device__ __forceinline__ T wall_ghost(int kind, T val, T speed) {
   if (kind == AERO_BC_SLIP) return val;
@@ -101,29 +115,63 @@ __device__ __forceinline__ T wall_ghost(int kind, T val, T speed) {
   return (T)2 * speed - val;
 }
 
-__device__ __forceinline__ int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
-
-// u at row j (-2..ny+1), face i (-2..nx+2) with the ghost rules of core.py:287-305.
+// ------------------------------------------------------------------ in-domain reads
+// u at face (j, i) of the stage input: u^ - grad psi when psi is given (core.py:645-660),
+// with the outlet face shifted by the mass correction (core.py:337-357).
 template <typename T>
-__device__ __forceinline__ T ug(const Dev<T> &d, const T *__restrict__ u, int j, int i) {
-  const int nx = d.nx, ny = d.ny, ld = nx + 1;
-  if (d.periodic) return u[wrap(j, ny) * ld + wrap(i, nx)];
+__device__ __forceinline__ T u_in(const Dev<T> &d, const T *__restrict__ u, const T *__restrict__ psi, T corr, int j,
+                                  int i) {
+  const int nx = d.nx;
+  T val = u[j * (nx + 1) + i];
+  if (psi) {
+    if (d.periodic) {
+      const int ie = i == nx ? 0 : i;
+      val = val - (psi[j * nx + ie] - psi[j * nx + wrap(ie - 1, nx)]) * d.inv_h;
+    } else if (i > 0 && i < nx) {
+      val = val - (psi[j * nx + i] - psi[j * nx + i - 1]) * d.inv_h;
+    } else if (i == nx && !d.closed) {
+      val = val + corr;
+    }
+  }
+  return val;
+}
+template <typename T>
+__device__ __forceinline__ T v_in(const Dev<T> &d, const T *__restrict__ v, const T *__restrict__ psi, int j, int i) {
+  const int nx = d.nx, ny = d.ny;
+  T val = v[j * nx + i];
+  if (psi) {
+    if (d.periodic) {
+      const int je = j == ny ? 0 : j;
+      val = val - (psi[je * nx + i] - psi[wrap(je - 1, ny) * nx + i]) * d.inv_h;
+    } else if (j > 0 && j < ny) {
+      val = val - (psi[j * nx + i] - psi[(j - 1) * nx + i]) * d.inv_h;
+    }
+  }
+  return val;
+}
+
+// u at row j (-2..ny+1), face i (-2..nx+2), ghost rules of core.py:287-305. Positions
+// outside that range are clamped to valid memory; no output reads them.
+template <typename T>
+__device__ __forceinline__ T u_ghost(const Dev<T> &d, const T *u, const T *psi, T corr, int j, int i) {
+  const int nx = d.nx, ny = d.ny;
+  if (d.periodic) return u_in(d, u, psi, corr, clampi(wrap(j, ny), 0, ny - 1), clampi(wrap(i, nx), 0, nx));
   T sx = (T)1;
   if (i < 0) {
     if (d.closed) { i = -i; sx = (T)-1; } else i = 0;
   } else if (i > nx) {
     if (d.closed) { i = 2 * nx - i; sx = (T)-1; } else i = nx;
   }
-  if (j < 0) return wall_ghost(d.bcb, sx * u[(-1 - j) * ld + i], d.ground);
-  if (j >= ny) return wall_ghost(d.bct, sx * u[(2 * ny - 1 - j) * ld + i], (T)0);
-  return sx * u[j * ld + i];
+  i = clampi(i, 0, nx);
+  if (j < 0) return wall_ghost(d.bcb, sx * u_in(d, u, psi, corr, clampi(-1 - j, 0, ny - 1), i), d.ground);
+  if (j >= ny) return wall_ghost(d.bct, sx * u_in(d, u, psi, corr, clampi(2 * ny - 1 - j, 0, ny - 1), i), (T)0);
+  return sx * u_in(d, u, psi, corr, j, i);
 }
-
-// v at face row j (-2..ny+2), column i (-2..nx+1) with the rules of core.py:308-325.
+// v at face row j (-2..ny+2), column i (-2..nx+1), ghost rules of core.py:308-325.
 template <typename T>
-__device__ __forceinline__ T vg(const Dev<T> &d, const T *__restrict__ v, int j, int i) {
+__device__ __forceinline__ T v_ghost(const Dev<T> &d, const T *v, const T *psi, int j, int i) {
   const int nx = d.nx, ny = d.ny;
-  if (d.periodic) return v[wrap(j, ny) * nx + wrap(i, nx)];
+  if (d.periodic) return v_in(d, v, psi, clampi(wrap(j, ny), 0, ny), clampi(wrap(i, nx), 0, nx - 1));
   T sy = (T)1, sx = (T)1;
   if (j < 0) { j = -j; sy = (T)-1; } else if (j > ny) { j = 2 * ny - j; sy = (T)-1; }
   if (i < 0) {
@@ -132,67 +180,46 @@ __device__ __forceinline__ T vg(const Dev<T> &d, const T *__restrict__ v, int j,
   } else if (i >= nx) {
     if (d.closed) { i = 2 * nx - 1 - i; sx = d.bcr != AERO_BC_NOSLIP ? (T)1 : (T)-1; } else i = nx - 1;
   }
-  return sx * (sy * v[j * nx + i]);
+  return sx * (sy * v_in(d, v, psi, clampi(j, 0, ny), clampi(i, 0, nx - 1)));
+}
+// cell scalar with the TVD ghost rules of scalar.py:42-60: inflow (or replicate) on the
+// left, replicate elsewhere, wrap when periodic. use_inflow 2: the smoke comb's rule
+// (inflow also in a closed box, core.py:718-723).
+template <typename T>
+__device__ __forceinline__ T c_ghost(const Dev<T> &d, const T *c, int j, int i, T inflow, int use_inflow) {
+  const int nx = d.nx, ny = d.ny;
+  if (d.periodic) return c[clampi(wrap(j, ny), 0, ny - 1) * nx + clampi(wrap(i, nx), 0, nx - 1)];
+  const int jj = clampi(j, 0, ny - 1);
+  if (i < 0 && (use_inflow == 2 || (use_inflow == 1 && !d.closed))) return inflow;
+  return c[jj * nx + clampi(i, 0, nx - 1)];
 }
 
-// eddy viscosity at cell (j, c), j, c in -1..ny / -1..nx (written by k_nut)
+// ------------------------------------------------------------------ tile loads
+// U, V tiles of the stage-input velocity around the tile at (tj0, ti0).
 template <typename T>
-__device__ __forceinline__ T nut_at(const Dev<T> &d, int j, int c) {
-  return d.nut[(j + 1) * (d.nx + 2) + (c + 1)];
-}
-
-// ------------------------------------------------------------------ stencil pieces
-// Scaled face flux of core.py:364-388 (central: vsum (b+c); upwind3: 24/7 x flux).
-template <typename T>
-__device__ __forceinline__ T face_flux(T vsum, T a, T b, T c, T dd, int upwind) {
-  const T s = b + c;
-  if (!upwind) return vsum * s;
-  const T cen = s + (T)(-1.0 / 7.0) * (a + dd);
-  const T dis = (b - c) + (T)(-1.0 / 3.0) * (a - dd);
-  return vsum * cen + (T)(3.0 / 7.0) * fabs(vsum) * dis;
-}
-
-// u-momentum x-flux at cell centre c (between faces c and c+1) of row j
-template <typename T>
-__device__ __forceinline__ T flux_ux(const Dev<T> &d, const T *u, int j, int c) {
-  const T a = ug(d, u, j, c - 1), b = ug(d, u, j, c), cc = ug(d, u, j, c + 1), e = ug(d, u, j, c + 2);
-  return face_flux(b + cc, a, b, cc, e, d.upwind);
-}
-// u-momentum y-flux at corner (jc, i)
-template <typename T>
-__device__ __forceinline__ T flux_uy(const Dev<T> &d, const T *u, const T *v, int jc, int i) {
-  const T vs = vg(d, v, jc, i - 1) + vg(d, v, jc, i);
-  return face_flux(vs, ug(d, u, jc - 2, i), ug(d, u, jc - 1, i), ug(d, u, jc, i), ug(d, u, jc + 1, i), d.upwind);
-}
-// v-momentum x-flux at corner (j, ic)
-template <typename T>
-__device__ __forceinline__ T flux_vx(const Dev<T> &d, const T *u, const T *v, int j, int ic) {
-  const T us = ug(d, u, j - 1, ic) + ug(d, u, j, ic);
-  return face_flux(us, vg(d, v, j, ic - 2), vg(d, v, j, ic - 1), vg(d, v, j, ic), vg(d, v, j, ic + 1), d.upwind);
-}
-// v-momentum y-flux at cell centre row cm (between v faces cm and cm+1)
-template <typename T>
-__device__ __forceinline__ T flux_vy(const Dev<T> &d, const T *v, int cm, int i) {
-  const T a = vg(d, v, cm - 1, i), b = vg(d, v, cm, i), c = vg(d, v, cm + 1, i), e = vg(d, v, cm + 2, i);
-  return face_flux(b + c, a, b, c, e, d.upwind);
-}
-
-// shear du/dy + dv/dx at corner (jc, ic)
-template <typename T>
-__device__ __forceinline__ T corner_shear(const Dev<T> &d, const T *u, const T *v, int jc, int ic) {
-  return (ug(d, u, jc, ic) - ug(d, u, jc - 1, ic)) * d.inv_h + (vg(d, v, jc, ic) - vg(d, v, jc, ic - 1)) * d.inv_h;
+__device__ void load_uv(const Dev<T> &d, Lds<T> &L, int tj0, int ti0, const T *u, const T *v, const T *psi, T corr) {
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  for (int e = tid; e < HH * HW; e += 256) {
+    const int r = e / HW, c = e - r * HW;
+    L.u[r][c] = u_ghost(d, u, psi, corr, tj0 - 2 + r, ti0 - 2 + c);
+    L.v[r][c] = v_ghost(d, v, psi, tj0 - 2 + r, ti0 - 2 + c);
+  }
 }
 template <typename T>
-__device__ __forceinline__ T corner_nut(const Dev<T> &d, int jc, int ic) {
-  return (((nut_at(d, jc - 1, ic - 1) + nut_at(d, jc, ic - 1)) + nut_at(d, jc - 1, ic)) + nut_at(d, jc, ic)) *
-         (T)0.25;
+__device__ void load_c(const Dev<T> &d, Lds<T> &L, int tj0, int ti0, const T *c, const T *inflow_col, T inflow0,
+                       int use_inflow) {
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  for (int e = tid; e < HH * HW; e += 256) {
+    const int r = e / HW, cc = e - r * HW;
+    const int j = tj0 - 2 + r;
+    const T inflow = inflow_col ? inflow_col[clampi(j, 0, d.ny - 1)] : inflow0;
+    L.c[r][cc] = c_ghost(d, c, j, ti0 - 2 + cc, inflow, use_inflow);
+  }
 }
 
 // Eddy viscosity of core.py:444-478 from the gradient tensor.
 template <typename T>
 __device__ __forceinline__ T eddy(const Dev<T> &d, T g11, T g12, T g21, T g22) {
-  const T delta2 = d.h * d.h;
-  const T c = (T)d.les_c;
   const T s12 = (T)0.5 * (g12 + g21);
   const T ss = (g11 * g11 + g22 * g22) + (T)2 * s12 * s12;
   if (d.turb == 1) return (T)(d.les_c * d.les_c * d.hd * d.hd) * sqrt((T)2 * ss + (T)1e-30);
@@ -214,9 +241,46 @@ __device__ __forceinline__ T eddy(const Dev<T> &d, T g11, T g12, T g21, T g22) {
   T bb = b11 * b22 - b12 * b12;
   bb = bb > (T)0 ? bb : (T)0;
   const T aa = ((g11 * g11 + g12 * g12) + g21 * g21) + g22 * g22;
-  (void)delta2;
-  (void)c;
   return (T)(d.les_c * d.hd * d.hd) * sqrt(bb / (aa + (T)1e-30) + (T)1e-30);
+}
+
+// nu_t on the tile's cells and their ring (core.py:421-503), switched off in solids.
+template <typename T>
+__device__ void tile_nut(const Dev<T> &d, Lds<T> &L, int tj0, int ti0) {
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  const T ih = d.inv_h;
+  for (int e = tid; e < NH * NW; e += 256) {
+    const int r = e / NW, c = e - r * NW;
+    const int j = tj0 - 1 + r, cc = ti0 - 1 + c;   // cell
+    const int ur = r + 1, uc = c + 1;               // u/v tile index of (j, cc)
+    const T g11 = (L.u[ur][uc + 1] - L.u[ur][uc]) * ih;
+    const T g22 = (L.v[ur + 1][uc] - L.v[ur][uc]) * ih;
+    const T du00 = (L.u[ur][uc] - L.u[ur - 1][uc]) * ih;
+    const T du10 = (L.u[ur + 1][uc] - L.u[ur][uc]) * ih;
+    const T du01 = (L.u[ur][uc + 1] - L.u[ur - 1][uc + 1]) * ih;
+    const T du11 = (L.u[ur + 1][uc + 1] - L.u[ur][uc + 1]) * ih;
+    const T dv00 = (L.v[ur][uc] - L.v[ur][uc - 1]) * ih;
+    const T dv10 = (L.v[ur + 1][uc] - L.v[ur + 1][uc - 1]) * ih;
+    const T dv01 = (L.v[ur][uc + 1] - L.v[ur][uc]) * ih;
+    const T dv11 = (L.v[ur + 1][uc + 1] - L.v[ur + 1][uc]) * ih;
+    const T g12 = (((du00 + du10) + du01) + du11) * (T)0.25;
+    const T g21 = (((dv00 + dv10) + dv01) + dv11) * (T)0.25;
+    T val = eddy(d, g11, g12, g21, g22);
+    if (d.win && j >= d.j0 && j < d.j1 && cc >= d.i0 && cc < d.i1)
+      val = val * ((T)1 - d.chi_c[(j - d.j0) * d.wx + (cc - d.i0)]);
+    L.nu[r][c] = val;
+  }
+}
+
+// ------------------------------------------------------------------ stencil pieces
+// Scaled face flux of core.py:364-388 (central: vsum (b+c); upwind3: 24/7 x flux).
+template <typename T>
+__device__ __forceinline__ T face_flux(T vsum, T a, T b, T c, T dd, int upwind) {
+  const T s = b + c;
+  if (!upwind) return vsum * s;
+  const T cen = s + (T)(-1.0 / 7.0) * (a + dd);
+  const T dis = (b - c) + (T)(-1.0 / 3.0) * (a - dd);
+  return vsum * cen + (T)(3.0 / 7.0) * fabs(vsum) * dis;
 }
 
 // torch.lerp(start, end, w)
@@ -225,68 +289,45 @@ __device__ __forceinline__ T lerp_t(T start, T end, T w) {
   return w < (T)0.5 ? start + w * (end - start) : end - (end - start) * ((T)1 - w);
 }
 
-// ------------------------------------------------------------------ scalar transport pieces
-// cell value with the TVD ghost rules of scalar.py:42-60 (x: inflow / replicate / wrap)
-template <typename T>
-__device__ __forceinline__ T cx_ghost(const Dev<T> &d, const T *__restrict__ c, int j, int i, T inflow, int use_inflow) {
-  const int nx = d.nx;
-  if (d.periodic) return c[j * nx + wrap(i, nx)];
-  if (i < 0) return (use_inflow == 2 || (use_inflow && !d.closed)) ? inflow : c[j * nx];
-  if (i >= nx) return c[j * nx + nx - 1];
-  return c[j * nx + i];
-}
-template <typename T>
-__device__ __forceinline__ T cy_ghost(const Dev<T> &d, const T *__restrict__ c, int j, int i) {
-  const int nx = d.nx, ny = d.ny;
-  if (d.periodic) return c[wrap(j, ny) * nx + i];
-  if (j < 0) j = 0;
-  if (j >= ny) j = ny - 1;
-  return c[j * nx + i];
-}
 template <typename T>
 __device__ __forceinline__ T van_leer(T a, T b) {
   const T ab = a * b;
   return ab > (T)0 ? (T)2 * ab / (a + b) : (T)0;
 }
-// MUSCL / van Leer advective flux at x-face f of row j (scalar.py:63-83)
+
+// MUSCL / van Leer flux at the x-face between tile cells (r, c-1) and (r, c) (scalar.py:63-83)
 template <typename T>
-__device__ __forceinline__ T tvd_fx(const Dev<T> &d, const T *c, const T *u, int j, int f, T inflow, int use_inflow) {
-  const T cm2 = cx_ghost(d, c, j, f - 2, inflow, use_inflow), cm1 = cx_ghost(d, c, j, f - 1, inflow, use_inflow);
-  const T c0 = cx_ghost(d, c, j, f, inflow, use_inflow), c1 = cx_ghost(d, c, j, f + 1, inflow, use_inflow);
-  const T sl = van_leer(cm1 - cm2, c0 - cm1);
-  const T sr = van_leer(c0 - cm1, c1 - c0);
-  const T left = cm1 + (T)0.5 * sl;
-  const T right = c0 - (T)0.5 * sr;
-  const T uf = u[j * (d.nx + 1) + f];
+__device__ __forceinline__ T tvd_x(const Lds<T> &L, int r, int c, T uf) {
+  const T cm2 = L.c[r][c - 2], cm1 = L.c[r][c - 1], c0 = L.c[r][c], c1 = L.c[r][c + 1];
+  const T left = cm1 + (T)0.5 * van_leer(cm1 - cm2, c0 - cm1);
+  const T right = c0 - (T)0.5 * van_leer(c0 - cm1, c1 - c0);
   return uf * (uf > (T)0 ? left : right);
 }
 template <typename T>
-__device__ __forceinline__ T tvd_fy(const Dev<T> &d, const T *c, const T *v, int f, int i) {
-  const T cm2 = cy_ghost(d, c, f - 2, i), cm1 = cy_ghost(d, c, f - 1, i);
-  const T c0 = cy_ghost(d, c, f, i), c1 = cy_ghost(d, c, f + 1, i);
-  const T sl = van_leer(cm1 - cm2, c0 - cm1);
-  const T sr = van_leer(c0 - cm1, c1 - c0);
-  const T low = cm1 + (T)0.5 * sl;
-  const T high = c0 - (T)0.5 * sr;
-  const T vf = v[f * d.nx + i];
+__device__ __forceinline__ T tvd_y(const Lds<T> &L, int r, int c, T vf) {
+  const T cm2 = L.c[r - 2][c], cm1 = L.c[r - 1][c], c0 = L.c[r][c], c1 = L.c[r + 1][c];
+  const T low = cm1 + (T)0.5 * van_leer(cm1 - cm2, c0 - cm1);
+  const T high = c0 - (T)0.5 * van_leer(c0 - cm1, c1 - c0);
   return vf * (vf > (T)0 ? low : high);
 }
 
 // temperature diffusivity at x-face f of row j / y-face f of column i (physics.py:713-724)
 template <typename T>
-__device__ __forceinline__ T kappa_x(const Dev<T> &d, T kap, int j, int f) {
+__device__ __forceinline__ T kappa_x(const Dev<T> &d, const Lds<T> &L, int tj0, int ti0, T kap, int j, int f) {
   T k = kap;
   if (d.les) {
     const T ip = (T)(1.0 / d.prt);
+    const int r = j - tj0 + 1;
+    auto N = [&](int c) { return L.nu[r][c - ti0 + 1]; };
     T fa;
     if (d.periodic && (f == 0 || f == d.nx))
-      fa = (T)0.5 * (nut_at(d, j, 0) * ip + nut_at(d, j, d.nx - 1) * ip);
+      fa = (T)0.5 * (N(f == 0 ? 0 : d.nx) * ip + N(f == 0 ? -1 : d.nx - 1) * ip);
     else if (f == 0)
-      fa = nut_at(d, j, 0) * ip;
+      fa = N(0) * ip;
     else if (f == d.nx)
-      fa = nut_at(d, j, d.nx - 1) * ip;
+      fa = N(d.nx - 1) * ip;
     else
-      fa = (T)0.5 * (nut_at(d, j, f) * ip + nut_at(d, j, f - 1) * ip);
+      fa = (T)0.5 * (N(f) * ip + N(f - 1) * ip);
     k = kap + fa;
   }
   if (d.insulated && d.win && j >= d.j0 && j < d.j1 && f >= d.i0 && f <= d.i1)
@@ -294,19 +335,21 @@ __device__ __forceinline__ T kappa_x(const Dev<T> &d, T kap, int j, int f) {
   return k;
 }
 template <typename T>
-__device__ __forceinline__ T kappa_y(const Dev<T> &d, T kap, int f, int i) {
+__device__ __forceinline__ T kappa_y(const Dev<T> &d, const Lds<T> &L, int tj0, int ti0, T kap, int f, int i) {
   T k = kap;
   if (d.les) {
     const T ip = (T)(1.0 / d.prt);
+    const int c = i - ti0 + 1;
+    auto N = [&](int j) { return L.nu[j - tj0 + 1][c]; };
     T fa;
     if (d.periodic && (f == 0 || f == d.ny))
-      fa = (T)0.5 * (nut_at(d, 0, i) * ip + nut_at(d, d.ny - 1, i) * ip);
+      fa = (T)0.5 * (N(f == 0 ? 0 : d.ny) * ip + N(f == 0 ? -1 : d.ny - 1) * ip);
     else if (f == 0)
-      fa = nut_at(d, 0, i) * ip;
+      fa = N(0) * ip;
     else if (f == d.ny)
-      fa = nut_at(d, d.ny - 1, i) * ip;
+      fa = N(d.ny - 1) * ip;
     else
-      fa = (T)0.5 * (nut_at(d, f, i) * ip + nut_at(d, f - 1, i) * ip);
+      fa = (T)0.5 * (N(f) * ip + N(f - 1) * ip);
     k = kap + fa;
   }
   if (d.insulated && d.win && f >= d.j0 && f <= d.j1 && i >= d.i0 && i < d.i1)
@@ -322,76 +365,17 @@ __device__ __forceinline__ T wall_dflux(const Dev<T> &d, int mode, T k, T inner,
   return (T)0;
 }
 
-// diffusive flux -kappa dtheta/dx at x-face f of row j (scalar.py:99-141)
+// emitter clock (physics.py:798-802)
 template <typename T>
-__device__ __forceinline__ T diff_fx(const Dev<T> &d, const T *th, T kap, int j, int f) {
-  const int nx = d.nx;
-  if (f > 0 && f < nx) return -(kappa_x(d, kap, j, f) * (th[j * nx + f] - th[j * nx + f - 1])) * d.inv_h;
-  if (d.periodic) return -(kappa_x(d, kap, j, 0) * (th[j * nx] - th[j * nx + nx - 1])) * d.inv_h;
-  if (f == 0) return wall_dflux(d, d.wm[0], kappa_x(d, kap, j, 0), th[j * nx], d.wall_values[0], 1.0);
-  return wall_dflux(d, d.wm[1], kappa_x(d, kap, j, nx), th[j * nx + nx - 1], d.wall_values[1], -1.0);
-}
-template <typename T>
-__device__ __forceinline__ T diff_fy(const Dev<T> &d, const T *th, T kap, int f, int i) {
-  const int nx = d.nx, ny = d.ny;
-  if (f > 0 && f < ny) return -(kappa_y(d, kap, f, i) * (th[f * nx + i] - th[(f - 1) * nx + i])) * d.inv_h;
-  if (d.periodic) return -(kappa_y(d, kap, 0, i) * (th[i] - th[(ny - 1) * nx + i])) * d.inv_h;
-  if (f == 0) return wall_dflux(d, d.wm[2], kappa_y(d, kap, 0, i), th[i], d.wall_values[2], 1.0);
-  return wall_dflux(d, d.wm[3], kappa_y(d, kap, ny, i), th[(ny - 1) * nx + i], d.wall_values[3], -1.0);
-}
-
-// Boussinesq force per unit Ri*(-g) component: face average of theta, zero on boundary faces
-template <typename T>
-__device__ __forceinline__ T theta_face_x(const Dev<T> &d, const T *th, int j, int f) {
-  const int nx = d.nx;
-  if (f == 0 || f == nx) {
-    if (!d.periodic) return (T)0;
-    return (T)0.5 * (th[j * nx] + th[j * nx + nx - 1]);
-  }
-  return (T)0.5 * (th[j * nx + f] + th[j * nx + f - 1]);
-}
-template <typename T>
-__device__ __forceinline__ T theta_face_y(const Dev<T> &d, const T *th, int f, int i) {
-  const int nx = d.nx, ny = d.ny;
-  if (f == 0 || f == ny) {
-    if (!d.periodic) return (T)0;
-    return (T)0.5 * (th[i] + th[(ny - 1) * nx + i]);
-  }
-  return (T)0.5 * (th[f * nx + i] + th[(f - 1) * nx + i]);
-}
-
-// ================================================================== kernels
-// Eddy viscosity at cells -1..ny x -1..nx of the stage velocity, switched off inside
-// solids (core.py:421-503); the max goes to red->nutmax on stage 0.
-template <typename T>
-__global__ void __launch_bounds__(256) k_nut(Dev<T> d, const T *__restrict__ u, const T *__restrict__ v, int want_max) {
-  __shared__ T lds[256];
-  const int c = blockIdx.x * blockDim.x + threadIdx.x - 1;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y - 1;
-  T val = (T)0;
-  if (c <= d.nx && j <= d.ny) {
-    const T ih = d.inv_h;
-    const T g11 = (ug(d, u, j, c + 1) - ug(d, u, j, c)) * ih;
-    const T g22 = (vg(d, v, j + 1, c) - vg(d, v, j, c)) * ih;
-    const T du00 = (ug(d, u, j, c) - ug(d, u, j - 1, c)) * ih;
-    const T du10 = (ug(d, u, j + 1, c) - ug(d, u, j, c)) * ih;
-    const T du01 = (ug(d, u, j, c + 1) - ug(d, u, j - 1, c + 1)) * ih;
-    const T du11 = (ug(d, u, j + 1, c + 1) - ug(d, u, j, c + 1)) * ih;
-    const T dv00 = (vg(d, v, j, c) - vg(d, v, j, c - 1)) * ih;
-    const T dv10 = (vg(d, v, j + 1, c) - vg(d, v, j + 1, c - 1)) * ih;
-    const T dv01 = (vg(d, v, j, c + 1) - vg(d, v, j, c)) * ih;
-    const T dv11 = (vg(d, v, j + 1, c + 1) - vg(d, v, j + 1, c)) * ih;
-    const T g12 = (((du00 + du10) + du01) + du11) * (T)0.25;
-    const T g21 = (((dv00 + dv10) + dv01) + dv11) * (T)0.25;
-    val = eddy(d, g11, g12, g21, g22);
-    if (d.win && j >= d.j0 && j < d.j1 && c >= d.i0 && c < d.i1)
-      val = val * ((T)1 - d.chi_c[(j - d.j0) * d.wx + (c - d.i0)]);
-    d.nut[(j + 1) * (d.nx + 2) + (c + 1)] = val;
-  }
-  if (want_max) {
-    const T m = block_max(val, lds);
-    if (threadIdx.x == 0 && threadIdx.y == 0) atomic_max_nonneg(&d.red->nutmax, m);
-  }
+__device__ __forceinline__ T emitter_on(T t, const T *tm) {
+  const T start = tm[0], stop = tm[1], period = tm[2], duty = tm[3];
+  const bool active = (t >= start) && (t < stop);
+  const T safe = period > (T)0 ? period : (T)1;
+  T r = fmod(t - start, safe);
+  if (r != (T)0 && ((r < (T)0) != (safe < (T)0))) r += safe;
+  const T phase = r / safe;
+  const bool on = active && ((period <= (T)0) || (phase < duty));
+  return on ? (T)1 : (T)0;
 }
 
 // dt of the step (core.py:685-695 + physics.py:851-888) from the maxima of the state.
@@ -399,7 +383,7 @@ template <typename T>
 __device__ T compute_dt(const Dev<T> &d, T umax, T vmax, T thmax, T nutmax) {
   if (d.fixed_dt) return *d.dt_fixed;
   const T nu = *d.nu;
-  T us = umax + vmax;
+  const T us = umax + vmax;
   const T uc = us > (T)1e-6 ? us : (T)1e-6;
   T dt = (T)(d.cfl * d.hd) / uc;
   const T nue = d.les ? nu + nutmax : nu;
@@ -407,8 +391,7 @@ __device__ T compute_dt(const Dev<T> &d, T umax, T vmax, T thmax, T nutmax) {
   dt = vis < dt ? vis : dt;
   if (d.thermal) {
     const T kap = d.les ? *d.kappa + nutmax * (T)(1.0 / d.prt) : *d.kappa;
-    const T speed = us;   // u_floor is 0 without ports and fans
-    const T rate = speed * (T)(2.0 / d.hd) + kap * (T)(d.diff_coef / (d.hd * d.hd));
+    const T rate = us * (T)(2.0 / d.hd) + kap * (T)(d.diff_coef / (d.hd * d.hd));
     const T lim = (T)1 / rate;
     dt = lim < dt ? lim : dt;
     T acc = fabs(*d.ri) * thmax;
@@ -429,205 +412,192 @@ __device__ T compute_dt(const Dev<T> &d, T umax, T vmax, T thmax, T nutmax) {
   return dt;
 }
 
-// max |u0|, |v0|, |theta0| and sum theta0 over the state; the last block sets dt.
-template <typename T>
-__global__ void __launch_bounds__(256) k_reduce0(Dev<T> d, const T *__restrict__ u, const T *__restrict__ v,
-                                                 const T *__restrict__ th) {
-  __shared__ T lds[256];
-  __shared__ double ldsd[256];
-  __shared__ bool last;
-  const int nu_ = d.ny * (d.nx + 1), nv_ = (d.ny + 1) * d.nx, nc = d.ny * d.nx;
-  const int n = nu_ > nv_ ? nu_ : nv_;
-  T mu = 0, mv = 0, mt = 0;
-  double st = 0.0;
-  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
-    if (e < nu_) { T a = fabs(u[e]); mu = a > mu ? a : mu; }
-    if (e < nv_) { T a = fabs(v[e]); mv = a > mv ? a : mv; }
-    if (d.thermal && e < nc) { T a = fabs(th[e]); mt = a > mt ? a : mt; st += (double)th[e]; }
-  }
-  mu = block_max(mu, lds);
-  mv = block_max(mv, lds);
-  if (d.thermal) {
-    mt = block_max(mt, lds);
-    st = block_sum(st, ldsd);
-  }
-  if (threadIdx.x == 0) {
-    atomic_max_nonneg(&d.red->umax, mu);
-    atomic_max_nonneg(&d.red->vmax, mv);
-    if (d.thermal) atomic_max_nonneg(&d.red->thmax, mt);
-    d.partial[blockIdx.x] = st;
-    __threadfence();
-    const unsigned t = atomicAdd(&d.red->ticket, 1u);
-    last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  // deterministic sum of the per-block partials
-  double s = 0.0;
-  for (int b = threadIdx.x; b < gridDim.x; b += blockDim.x) s += load_relaxed(&d.partial[b]);
-  s = block_sum(s, ldsd);
-  if (threadIdx.x == 0) {
-    Red<T> *r = d.red;
-    const T umax = load_relaxed(&r->umax), vmax = load_relaxed(&r->vmax);
-    const T thmax = load_relaxed(&r->thmax), nutmax = load_relaxed(&r->nutmax);
-    r->dt = compute_dt(d, umax, vmax, thmax, nutmax);
-    r->th_old = s;
-    r->umax = (T)0;
-    r->vmax = (T)0;
-    r->thmax = (T)0;
-    r->nutmax = (T)0;
-    r->ticket = 0u;
-  }
-}
-
-// The explicit stage (core.py:855-935 per face / cell): tendencies, RK update,
-// inlet/outlet, penalisation with force increments, buoyancy, temperature stage.
+// ================================================================== kernels
+// The explicit stage on one 64x8 tile (core.py:855-935 per face / cell).
 template <typename T>
 __global__ void __launch_bounds__(256) k_stage(Dev<T> d, Stage<T> s) {
-  __shared__ double ldsd[256];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  __shared__ Lds<T> L;
+  const int tj0 = blockIdx.y * TY, ti0 = blockIdx.x * TX;
   const int nx = d.nx, ny = d.ny;
+  const T corr = s.psi_p ? d.red->corr : (T)0;
+  load_uv(d, L, tj0, ti0, s.up, s.vp, s.psi_p, corr);
+  if (d.thermal) load_c(d, L, tj0, ti0, s.thp, (const T *)nullptr, *d.theta_in, 1);
+  __syncthreads();
+  if (d.les) {
+    tile_nut(d, L, tj0, ti0);
+    __syncthreads();
+  }
   const T dt = d.red->dt;
   const T bdt = dt * (T)s.b;
   const T a = (T)s.a;
   const T wgt = (T)s.w;
   const T nu_h2 = *d.nu * d.inv_h2;
   const T kpen = bdt / *d.eta;
-  const T *up = s.up, *vp = s.vp;
   const bool first = s.s == 0;
-  T ri_g = 0;
-  if (d.thermal && d.buoyant) ri_g = *d.ri;
+  const T ih = d.inv_h;
+  const T ri_g = (d.thermal && d.buoyant) ? *d.ri : (T)0;
+  const T kap = d.thermal ? *d.kappa : (T)0;
+  // tile accessors: absolute (j, i) -> LDS
+  auto U = [&](int j, int i) { return L.u[j - tj0 + 2][i - ti0 + 2]; };
+  auto V = [&](int j, int i) { return L.v[j - tj0 + 2][i - ti0 + 2]; };
+  auto N = [&](int j, int c) { return L.nu[j - tj0 + 1][c - ti0 + 1]; };
+  auto C = [&](int j, int i) { return L.c[j - tj0 + 2][i - ti0 + 2]; };
+  auto flux_ux = [&](int j, int c) {   // u-momentum x-flux at cell centre c
+    const T a_ = U(j, c - 1), b_ = U(j, c), c_ = U(j, c + 1), e_ = U(j, c + 2);
+    return face_flux(b_ + c_, a_, b_, c_, e_, d.upwind);
+  };
+  auto flux_uy = [&](int jc, int i) {  // u-momentum y-flux at corner (jc, i)
+    return face_flux(V(jc, i - 1) + V(jc, i), U(jc - 2, i), U(jc - 1, i), U(jc, i), U(jc + 1, i), d.upwind);
+  };
+  auto flux_vx = [&](int j, int ic) {  // v-momentum x-flux at corner (j, ic)
+    return face_flux(U(j - 1, ic) + U(j, ic), V(j, ic - 2), V(j, ic - 1), V(j, ic), V(j, ic + 1), d.upwind);
+  };
+  auto flux_vy = [&](int cm, int i) {  // v-momentum y-flux at cell row cm
+    const T a_ = V(cm - 1, i), b_ = V(cm, i), c_ = V(cm + 1, i), e_ = V(cm + 2, i);
+    return face_flux(b_ + c_, a_, b_, c_, e_, d.upwind);
+  };
+  auto shear = [&](int jc, int ic) {
+    return (U(jc, ic) - U(jc - 1, ic)) * ih + (V(jc, ic) - V(jc, ic - 1)) * ih;
+  };
+  auto nutk = [&](int jc, int ic) {
+    return (((N(jc - 1, ic - 1) + N(jc, ic - 1)) + N(jc - 1, ic)) + N(jc, ic)) * (T)0.25;
+  };
+  // penalise a u-face value in the window, book the increment (+ buoyancy share)
+  auto pen_u = [&](int j, int i, T uh, T fu) -> T {
+    const int wo = (j - d.j0) * (d.wx + 1) + (i - d.i0);
+    const T un = (uh + kpen * d.a_u[wo]) / ((T)1 + kpen * d.chi_u[wo]);
+    T acc = (un - uh) * wgt;
+    if (!first) acc = d.acc_u[wo] + acc;
+    if (fu != (T)0) acc = acc + (fu * (d.vol_u ? d.vol_u[wo] : d.chi_u[wo])) * (bdt * wgt);
+    d.acc_u[wo] = acc;
+    return un;
+  };
+  auto pen_v = [&](int j, int i, T vh, T fv) -> T {
+    const int wo = (j - d.j0) * d.wx + (i - d.i0);
+    const T vn = (vh + kpen * d.a_v[wo]) / ((T)1 + kpen * d.chi_v[wo]);
+    T acc = (vn - vh) * wgt;
+    if (!first) acc = d.acc_v[wo] + acc;
+    if (fv != (T)0) acc = acc + (fv * (d.vol_v ? d.vol_v[wo] : d.chi_v[wo])) * (bdt * wgt);
+    d.acc_v[wo] = acc;
+    return vn;
+  };
+  auto in_win_u = [&](int j, int i) { return d.win && j >= d.j0 && j < d.j1 && i >= d.i0 && i <= d.i1; };
+  auto in_win_v = [&](int j, int i) { return d.win && j >= d.j0 && j <= d.j1 && i >= d.i0 && i < d.i1; };
+  auto theta_fx = [&](int j, int f) -> T {   // face average of theta (zero on boundary faces)
+    if (f == 0 || f == nx) return d.periodic ? (T)0.5 * (C(j, 0) + C(j, -1)) : (T)0;
+    return (T)0.5 * (C(j, f) + C(j, f - 1));
+  };
+  auto theta_fy = [&](int f, int i) -> T {
+    if (f == 0 || f == ny) return d.periodic ? (T)0.5 * (C(0, i) + C(-1, i)) : (T)0;
+    return (T)0.5 * (C(f, i) + C(f - 1, i));
+  };
 
-  // ---------------- u face (j, i)
-  if (j < ny && i <= nx) {
-    const bool bnd = !d.periodic && (i == 0 || i == nx);
-    const int ie = (d.periodic && i == nx) ? 0 : i;   // periodic duplicate face computes as face 0
-    T lu;
-    if (bnd) {
-      lu = (d.closed || i == 0) ? (T)0 : (up[j * (nx + 1) + nx - 1] - up[j * (nx + 1) + nx]) * d.inv_h;
-    } else {
-      const T fxl = flux_ux(d, up, j, ie - 1), fxr = flux_ux(d, up, j, ie);
-      const T gyb = flux_uy(d, up, vp, j, ie), gyt = flux_uy(d, up, vp, j + 1, ie);
-      const T au = (((fxl - fxr) + gyb) - gyt) * d.kadv;
-      const T uc = ug(d, up, j, ie);
-      const T ring = ((ug(d, up, j, ie + 1) + ug(d, up, j, ie - 1)) + ug(d, up, j + 1, ie)) + ug(d, up, j - 1, ie);
-      lu = au + (ring + (T)-4 * uc) * nu_h2;
-      if (d.les) {
-        const T txx_r = (T)2 * nut_at(d, j, ie) * ((ug(d, up, j, ie + 1) - ug(d, up, j, ie)) * d.inv_h);
-        const T txx_l = (T)2 * nut_at(d, j, ie - 1) * ((ug(d, up, j, ie) - ug(d, up, j, ie - 1)) * d.inv_h);
-        const T txy_t = corner_nut(d, j + 1, ie) * corner_shear(d, up, vp, j + 1, ie);
-        const T txy_b = corner_nut(d, j, ie) * corner_shear(d, up, vp, j, ie);
-        lu = lu + (((txx_r - txx_l) + txy_t) - txy_b) * d.inv_h;
+  const int i = ti0 + threadIdx.x;
+#pragma unroll 1
+  for (int k = 0; k < 2; ++k) {
+    const int j = tj0 + threadIdx.y + 4 * k;
+    if (j >= ny || i >= nx) continue;
+    // ---------------- u face (j, i) [+ (j, nx) by the last column]
+    {
+      const bool bnd = !d.periodic && i == 0;
+      T lu;
+      if (bnd) {
+        lu = (T)0;
+      } else {
+        const T au = (((flux_ux(j, i - 1) - flux_ux(j, i)) + flux_uy(j, i)) - flux_uy(j + 1, i)) * d.kadv;
+        const T ring = ((U(j, i + 1) + U(j, i - 1)) + U(j + 1, i)) + U(j - 1, i);
+        lu = au + (ring + (T)-4 * U(j, i)) * nu_h2;
+        if (d.les) {
+          const T txx_r = (T)2 * N(j, i) * ((U(j, i + 1) - U(j, i)) * ih);
+          const T txx_l = (T)2 * N(j, i - 1) * ((U(j, i) - U(j, i - 1)) * ih);
+          lu = lu + (((txx_r - txx_l) + nutk(j + 1, i) * shear(j + 1, i)) - nutk(j, i) * shear(j, i)) * ih;
+        }
+      }
+      const int o = j * (nx + 1) + i;
+      const T upv = U(j, i);
+      T uh = (s.a == 0.0 ? upv : lerp_t(upv, s.u0[o], a)) + lu * bdt;
+      if (!d.periodic && i == 0) uh = d.closed ? (T)0 : d.u_inlet[j];
+      T fu = 0;
+      if (ri_g != (T)0 && d.gx != 0.0) fu = theta_fx(j, i) * (ri_g * (T)(-d.gx));
+      T ust = in_win_u(j, i) ? pen_u(j, i, uh, fu) : uh;
+      if (fu != (T)0) ust = ust + bdt * fu;
+      s.uh[o] = ust;
+      if (d.periodic && i == 0) {
+        // duplicate face nx: face 0's value (core.py:912-913); its own mask books its increment
+        if (in_win_u(j, nx)) (void)pen_u(j, nx, uh, fu);
+        s.uh[j * (nx + 1) + nx] = ust;
+      }
+      if (!d.periodic && i == nx - 1) {
+        // outlet face: convective condition u_t + u_x = 0 (closed: wall)
+        const int on = j * (nx + 1) + nx;
+        const T lo = d.closed ? (T)0 : (U(j, nx - 1) - U(j, nx)) * ih;
+        const T upo = U(j, nx);
+        T uo = (s.a == 0.0 ? upo : lerp_t(upo, s.u0[on], a)) + lo * bdt;
+        if (d.closed) uo = (T)0;
+        else d.outcol[j] = uo;   // shifted by the mass correction in the projection passes
+        if (in_win_u(j, nx)) uo = pen_u(j, nx, uo, (T)0);
+        s.uh[on] = uo;
       }
     }
-    const int o = j * (nx + 1) + i;
-    const T upv = up[o];
-    T uh = (s.a == 0.0 ? upv : lerp_t(upv, s.u0[o], a)) + lu * bdt;
-    if (!d.periodic) {
-      if (d.closed) {
-        if (i == 0 || i == nx) uh = (T)0;
-      } else if (i == 0) {
-        uh = d.u_inlet[j];
-      } else if (i == nx) {
-        d.outcol[j] = uh;     // outlet mass balance applied by the projection passes
+    // ---------------- v face (j, i) [+ (ny, i) by the last row]
+    {
+      const bool bnd = !d.periodic && j == 0;
+      T lv;
+      if (bnd) {
+        lv = (T)0;
+      } else {
+        const T av = (((flux_vx(j, i) - flux_vx(j, i + 1)) + flux_vy(j - 1, i)) - flux_vy(j, i)) * d.kadv;
+        const T ring = ((V(j, i + 1) + V(j, i - 1)) + V(j + 1, i)) + V(j - 1, i);
+        lv = av + (ring + (T)-4 * V(j, i)) * nu_h2;
+        if (d.les) {
+          const T tyy_t = (T)2 * N(j, i) * ((V(j + 1, i) - V(j, i)) * ih);
+          const T tyy_b = (T)2 * N(j - 1, i) * ((V(j, i) - V(j - 1, i)) * ih);
+          lv = lv + (((nutk(j, i + 1) * shear(j, i + 1) - nutk(j, i) * shear(j, i)) + tyy_t) - tyy_b) * ih;
+        }
       }
-    }
-    T fu = 0;
-    if (ri_g != (T)0 && d.gx != 0.0) fu = theta_face_x(d, s.thp, j, i) * (ri_g * (T)(-d.gx));
-    const bool inwin = d.win && j >= d.j0 && j < d.j1 && i >= d.i0 && i <= d.i1;
-    T ustore = uh;
-    if (inwin) {
-      const int wo = (j - d.j0) * (d.wx + 1) + (i - d.i0);
-      const T un = (uh + kpen * d.a_u[wo]) / ((T)1 + kpen * d.chi_u[wo]);
-      T acc = (un - uh) * wgt;
-      if (!first) acc = d.acc_u[wo] + acc;
-      if (fu != (T)0) {
-        const T vu = d.vol_u ? d.vol_u[wo] : d.chi_u[wo];
-        acc = acc + (fu * vu) * (bdt * wgt);
-      }
-      d.acc_u[wo] = acc;
-      ustore = un;
-    }
-    if (d.periodic && i == nx) {
-      // duplicate face: the value of face 0 (penalised there), core.py:912-913
-      ustore = uh;
-      if (d.win && j >= d.j0 && j < d.j1 && d.i0 == 0) {
-        const int wo0 = (j - d.j0) * (d.wx + 1);
-        ustore = (uh + kpen * d.a_u[wo0]) / ((T)1 + kpen * d.chi_u[wo0]);
-      }
-    }
-    if (fu != (T)0) ustore = ustore + bdt * fu;
-    s.uh[o] = ustore;
-  }
-
-  // ---------------- v face (j, i)
-  if (j <= ny && i < nx) {
-    const bool bnd = !d.periodic && (j == 0 || j == ny);
-    const int je = (d.periodic && j == ny) ? 0 : j;
-    T lv;
-    if (bnd) {
-      lv = (T)0;
-    } else {
-      const T fxl = flux_vx(d, up, vp, je, i), fxr = flux_vx(d, up, vp, je, i + 1);
-      const T gyb = flux_vy(d, vp, je - 1, i), gyt = flux_vy(d, vp, je, i);
-      const T av = (((fxl - fxr) + gyb) - gyt) * d.kadv;
-      const T vc = vg(d, vp, je, i);
-      const T ring = ((vg(d, vp, je, i + 1) + vg(d, vp, je, i - 1)) + vg(d, vp, je + 1, i)) + vg(d, vp, je - 1, i);
-      lv = av + (ring + (T)-4 * vc) * nu_h2;
-      if (d.les) {
-        const T txy_r = corner_nut(d, je, i + 1) * corner_shear(d, up, vp, je, i + 1);
-        const T txy_l = corner_nut(d, je, i) * corner_shear(d, up, vp, je, i);
-        const T tyy_t = (T)2 * nut_at(d, je, i) * ((vg(d, vp, je + 1, i) - vg(d, vp, je, i)) * d.inv_h);
-        const T tyy_b = (T)2 * nut_at(d, je - 1, i) * ((vg(d, vp, je, i) - vg(d, vp, je - 1, i)) * d.inv_h);
-        lv = lv + (((txy_r - txy_l) + tyy_t) - tyy_b) * d.inv_h;
-      }
-    }
-    const int o = j * nx + i;
-    const T vpv = vp[o];
-    T vh = (s.a == 0.0 ? vpv : lerp_t(vpv, s.v0[o], a)) + lv * bdt;
-    T fv = 0;
-    if (ri_g != (T)0 && d.gy != 0.0) fv = theta_face_y(d, s.thp, j, i) * (ri_g * (T)(-d.gy));
-    const bool inwin = d.win && j >= d.j0 && j <= d.j1 && i >= d.i0 && i < d.i1;
-    T vstore = vh;
-    if (inwin) {
-      const int wo = (j - d.j0) * d.wx + (i - d.i0);
-      const T vn = (vh + kpen * d.a_v[wo]) / ((T)1 + kpen * d.chi_v[wo]);
-      T acc = (vn - vh) * wgt;
-      if (!first) acc = d.acc_v[wo] + acc;
-      if (fv != (T)0) {
-        const T vv = d.vol_v ? d.vol_v[wo] : d.chi_v[wo];
-        acc = acc + (fv * vv) * (bdt * wgt);
-      }
-      d.acc_v[wo] = acc;
-      vstore = vn;
-    }
-    if (d.periodic && j == ny) {
-      vstore = vh;
-      if (d.win && d.j0 == 0 && i >= d.i0 && i < d.i1) {
-        const int wo0 = i - d.i0;
-        vstore = (vh + kpen * d.a_v[wo0]) / ((T)1 + kpen * d.chi_v[wo0]);
-      }
-    }
-    if (fv != (T)0) vstore = vstore + bdt * fv;
-    s.vh[o] = vstore;
-  }
-
-  // ---------------- temperature cell (j, i) (physics.py:696-757)
-  if (d.thermal) {
-    double part = 0.0;
-    if (j < ny && i < nx) {
-      const T kap = *d.kappa;
-      const T tin = *d.theta_in;
-      const T *th = s.thp;
-      T lt = (((tvd_fx(d, th, up, j, i, tin, 1) - tvd_fx(d, th, up, j, i + 1, tin, 1)) + tvd_fy(d, th, vp, j, i)) -
-              tvd_fy(d, th, vp, j + 1, i)) * d.inv_h;
-      lt = lt + (((diff_fx(d, th, kap, j, i) - diff_fx(d, th, kap, j, i + 1)) + diff_fy(d, th, kap, j, i)) -
-                 diff_fy(d, th, kap, j + 1, i)) * d.inv_h;
-      if (d.has_heat) lt = lt + d.heat[j * nx + i];
       const int o = j * nx + i;
-      const T tp = th[o];
+      const T vpv = V(j, i);
+      T vh = (s.a == 0.0 ? vpv : lerp_t(vpv, s.v0[o], a)) + lv * bdt;
+      T fv = 0;
+      if (ri_g != (T)0 && d.gy != 0.0) fv = theta_fy(j, i) * (ri_g * (T)(-d.gy));
+      T vst = in_win_v(j, i) ? pen_v(j, i, vh, fv) : vh;
+      if (fv != (T)0) vst = vst + bdt * fv;
+      s.vh[o] = vst;
+      if (d.periodic && j == 0) {
+        if (in_win_v(ny, i)) (void)pen_v(ny, i, vh, fv);
+        s.vh[ny * nx + i] = vst;
+      }
+      if (!d.periodic && j == ny - 1) {
+        const int on = ny * nx + i;
+        const T vpo = V(ny, i);
+        T vo = (s.a == 0.0 ? vpo : lerp_t(vpo, s.v0[on], a));
+        if (in_win_v(ny, i)) vo = pen_v(ny, i, vo, (T)0);
+        s.vh[on] = vo;
+      }
+    }
+    // ---------------- temperature cell (j, i) (physics.py:696-757)
+    if (d.thermal) {
+      const int r = j - tj0 + 2, c = i - ti0 + 2;
+      const T fxl = tvd_x(L, r, c, U(j, i)), fxr = tvd_x(L, r, c + 1, U(j, i + 1));
+      const T fyb = tvd_y(L, r, c, V(j, i)), fyt = tvd_y(L, r + 1, c, V(j + 1, i));
+      T lt = (((fxl - fxr) + fyb) - fyt) * ih;
+      auto dfx = [&](int f) -> T {
+        // periodic boundary faces: the wrapped difference (scalar.py:133-134) read from the tile
+        if ((f > 0 && f < nx) || d.periodic)
+          return -(kappa_x(d, L, tj0, ti0, kap, j, f) * (C(j, f) - C(j, f - 1))) * ih;
+        if (f == 0) return wall_dflux(d, d.wm[0], kappa_x(d, L, tj0, ti0, kap, j, 0), C(j, 0), d.wall_values[0], 1.0);
+        return wall_dflux(d, d.wm[1], kappa_x(d, L, tj0, ti0, kap, j, nx), C(j, nx - 1), d.wall_values[1], -1.0);
+      };
+      auto dfy = [&](int f) -> T {
+        if ((f > 0 && f < ny) || d.periodic)
+          return -(kappa_y(d, L, tj0, ti0, kap, f, i) * (C(f, i) - C(f - 1, i))) * ih;
+        if (f == 0) return wall_dflux(d, d.wm[2], kappa_y(d, L, tj0, ti0, kap, 0, i), C(0, i), d.wall_values[2], 1.0);
+        return wall_dflux(d, d.wm[3], kappa_y(d, L, tj0, ti0, kap, ny, i), C(ny - 1, i), d.wall_values[3], -1.0);
+      };
+      lt = lt + (((dfx(i) - dfx(i + 1)) + dfy(j)) - dfy(j + 1)) * ih;
+      const int o = j * nx + i;
+      if (d.has_heat) lt = lt + d.heat[o];
+      const T tp = C(j, i);
       T thh = (s.a == 0.0 ? tp : lerp_t(tp, s.th0[o], a)) + lt * bdt;
       if (d.n_temp > 0 && d.win && j >= d.j0 && j < d.j1 && i >= d.i0 && i < d.i1) {
         const int wo = (j - d.j0) * d.wx + (i - d.i0);
@@ -639,13 +609,113 @@ __global__ void __launch_bounds__(256) k_stage(Dev<T> d, Stage<T> s) {
         thh = tn;
       }
       s.thh[o] = thh;
-      part = (double)thh;
     }
-    if (s.s == 2) {
-      // per-block partial of sum theta_new (heat storage), fixed order
-      part = block_sum(part, ldsd);
-      if (threadIdx.x == 0 && threadIdx.y == 0) d.partial2[blockIdx.y * gridDim.x + blockIdx.x] = part;
+  }
+}
+
+// Per-block maxima of a velocity tile (+ theta, nu_t) and sum theta: the dt inputs.
+// Each tile also covers the ghost ring of nu_t cells at the domain edge (core.py:503).
+template <typename T>
+__device__ void tile_maxima(const Dev<T> &d, Lds<T> &L, int tj0, int ti0, const T *th) {
+  const int nx = d.nx, ny = d.ny;
+  double mu = 0, mv = 0, mt = 0, mn = 0, st = 0;
+  const int i = ti0 + threadIdx.x;
+  for (int k = 0; k < 2; ++k) {
+    const int j = tj0 + threadIdx.y + 4 * k;
+    if (j >= ny || i >= nx) continue;
+    const int r = j - tj0 + 2, c = i - ti0 + 2;
+    mu = fmax(mu, (double)fabs(L.u[r][c]));
+    mv = fmax(mv, (double)fabs(L.v[r][c]));
+    if (i == nx - 1) mu = fmax(mu, (double)fabs(L.u[r][c + 1]));
+    if (j == ny - 1) mv = fmax(mv, (double)fabs(L.v[r + 1][c]));
+    if (d.thermal) {
+      const T tv = th[j * nx + i];
+      mt = fmax(mt, (double)fabs(tv));
+      st += (double)tv;
     }
+  }
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  if (d.les) {
+    for (int e = tid; e < NH * NW; e += 256) {
+      const int r = e / NW, c = e - r * NW;
+      const int j = tj0 - 1 + r, cc = ti0 - 1 + c;
+      if (j < -1 || j > ny || cc < -1 || cc > nx) continue;
+      mn = fmax(mn, (double)L.nu[r][c]);
+    }
+  }
+  mu = wave_max(mu);
+  mv = wave_max(mv);
+  mt = wave_max(mt);
+  mn = wave_max(mn);
+  st = wave_sum(st);
+  const int w = tid >> 6;
+  __syncthreads();
+  if ((tid & 63) == 0) {
+    L.red[w * 5 + 0] = mu;
+    L.red[w * 5 + 1] = mv;
+    L.red[w * 5 + 2] = mt;
+    L.red[w * 5 + 3] = mn;
+    L.red[w * 5 + 4] = st;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double *p = d.pmax + 5 * (blockIdx.y * gridDim.x + blockIdx.x);
+    for (int q = 0; q < 4; ++q)
+      p[q] = fmax(fmax(L.red[q], L.red[5 + q]), fmax(L.red[10 + q], L.red[15 + q]));
+    p[4] = ((L.red[4] + L.red[9]) + L.red[14]) + L.red[19];
+  }
+  __syncthreads();
+}
+
+// dt inputs of the step-start state (non-chained steps).
+template <typename T>
+__global__ void __launch_bounds__(256) k_pre(Dev<T> d, const T *u0, const T *v0, const T *th0) {
+  __shared__ Lds<T> L;
+  const int tj0 = blockIdx.y * TY, ti0 = blockIdx.x * TX;
+  load_uv(d, L, tj0, ti0, u0, v0, (const T *)nullptr, (T)0);
+  __syncthreads();
+  if (d.les) {
+    tile_nut(d, L, tj0, ti0);
+    __syncthreads();
+  }
+  tile_maxima(d, L, tj0, ti0, th0);
+}
+
+// Reduce the per-block dt inputs in a fixed order (one block; every thread gets out[]).
+template <typename T>
+__device__ void reduce_pmax(const Dev<T> &d, int nblocks, double *lds, double out[5]) {
+  const int tid = threadIdx.x;
+  double m[4] = {0, 0, 0, 0}, s = 0;
+  for (int b = tid; b < nblocks; b += blockDim.x) {
+    const double *p = d.pmax + 5 * b;
+    for (int q = 0; q < 4; ++q) m[q] = fmax(m[q], p[q]);
+    s += p[4];
+  }
+  for (int q = 0; q < 4; ++q) m[q] = wave_max(m[q]);
+  s = wave_sum(s);
+  const int w = tid >> 6, nw = (blockDim.x + 63) >> 6;
+  __syncthreads();
+  if ((tid & 63) == 0) {
+    for (int q = 0; q < 4; ++q) lds[w * 5 + q] = m[q];
+    lds[w * 5 + 4] = s;
+  }
+  __syncthreads();
+  for (int q = 0; q < 5; ++q) out[q] = 0;
+  for (int ww = 0; ww < nw; ++ww) {
+    for (int q = 0; q < 4; ++q) out[q] = fmax(out[q], lds[ww * 5 + q]);
+    out[4] += lds[ww * 5 + 4];
+  }
+  __syncthreads();
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024) k_dt(Dev<T> d) {
+  __shared__ double lds[16 * 5];
+  double r[5];
+  reduce_pmax(d, d.nbx * d.nby, lds, r);
+  if (threadIdx.x == 0) {
+    d.red->dt = compute_dt(d, (T)r[0], (T)r[1], (T)r[2], (T)r[3]);
+    d.red->th_old = r[4];
   }
 }
 
@@ -654,16 +724,23 @@ __device__ __forceinline__ int makhoul_src(int n, int len) { return n < (len + 1
 
 // Outlet correction of core.py:337-357, computed identically by every block.
 template <typename T>
-__device__ T outlet_corr(const Dev<T> &d, T *red) {
-  T si = 0, so = 0;
+__device__ T outlet_corr(const Dev<T> &d, double *lds) {
+  double v[2] = {0, 0};
   const int tid = threadIdx.x;
   for (int j = tid; j < d.ny; j += blockDim.x) {
-    si += d.u_inlet[j];
-    so += d.outcol[j];
+    v[0] += (double)d.u_inlet[j];
+    v[1] += (double)d.outcol[j];
   }
-  si = block_sum(si, red);
-  so = block_sum(so, red);
-  return (si - so) / (T)d.ny;
+  block_sum4<2>(v, lds);
+  __syncthreads();
+  if (tid == 0) {
+    lds[8] = v[0];
+    lds[9] = v[1];
+  }
+  __syncthreads();
+  const T r = ((T)lds[8] - (T)lds[9]) / (T)d.ny;
+  __syncthreads();
+  return r;
 }
 
 // divergence at cell (j, i) with the outlet face shifted by corr (core.py:621-623)
@@ -675,18 +752,25 @@ __device__ __forceinline__ T div_at(const Dev<T> &d, const T *u, const T *v, int
   return (((ur - u[j * (nx + 1) + i]) + v[(j + 1) * nx + i]) - v[j * nx + i]) * d.inv_h;
 }
 
+template <typename T>
+__device__ __forceinline__ void load_twiddles(cplx<T> *dst, const cplx<T> *__restrict__ src, int n) {
+  for (int m = threadIdx.x; m < n; m += blockDim.x) dst[m] = src[m];
+}
+
 // Pass A: divergence of rows (2b, 2b+1) -> forward transform along x -> F rows.
 template <typename T>
 __global__ void __launch_bounds__(kFftThreads) k_poisA(Dev<T> d, const T *__restrict__ u, const T *__restrict__ v,
                                                        const T *__restrict__ f_in, int mode) {
   // mode 0: f_in is the right-hand side itself; 1: divergence of (u, v); 2: + outlet correction
   extern __shared__ unsigned char smem[];
-  cplx<T> *A = reinterpret_cast<cplx<T> *>(smem);
-  cplx<T> *B = A + d.nx;
-  T *red = reinterpret_cast<T *>(B + d.nx);
   const int nx = d.nx;
+  cplx<T> *A = reinterpret_cast<cplx<T> *>(smem);
+  cplx<T> *B = A + nx;
+  cplx<T> *TW = B + nx;
+  double *red = reinterpret_cast<double *>(TW + nx);
   const int ja = 2 * blockIdx.x, jb = ja + 1;
   const bool hasb = jb < d.ny;
+  load_twiddles(TW, d.tw_x, nx);
   T corr = 0;
   if (mode == 2) {
     corr = outlet_corr(d, red);
@@ -705,7 +789,7 @@ __global__ void __launch_bounds__(kFftThreads) k_poisA(Dev<T> d, const T *__rest
     A[n] = {xa, xb};
   }
   __syncthreads();
-  cplx<T> *R = fft_lds(A, B, d.fx, d.tw_x, 1, nx);
+  cplx<T> *R = fft_lds(A, B, d.fx, TW, 1, nx);
   for (int k = threadIdx.x; k < nx; k += blockDim.x) {
     cplx<T> va, vb;
     split_pair(R, nx, k, va, vb);
@@ -723,39 +807,33 @@ __global__ void __launch_bounds__(kFftThreads) k_poisA(Dev<T> d, const T *__rest
   }
 }
 
-// Inverse real transform of two spectra held in LDS real arrays xa, xb (length n) into
-// the complex buffer Zbuf (packed for the FFT); after fft_lds the caller reads
-// v_a[m] = R[m].x * inv_n, v_b[m] = -R[m].y * inv_n (DCT: at natural index makhoul_src).
+// Inverse real transform packing (see common.h): conj(Z) for the DCT-III, plain pair for DHT.
 template <typename T>
-__device__ __forceinline__ void pack_inverse(const T *xa, const T *xb, cplx<T> *Z, int n, int dct,
-                                             const cplx<T> *__restrict__ cs, int k) {
+__device__ __forceinline__ void pack_inverse(const T *xa, const T *xb, cplx<T> *Z, int n, int dct, cplx<T> w, int k) {
   if (dct) {
-    const cplx<T> w = cs[k];
     const T xa_k = xa[k], xb_k = xb[k];
     const T xa_r = k == 0 ? (T)0 : xa[n - k];
     const T xb_r = k == 0 ? (T)0 : xb[n - k];
-    // V = e^{+i pi k / 2n} (X_k - i X_{n-k})
     const cplx<T> va = {w.x * xa_k + w.y * xa_r, w.y * xa_k - w.x * xa_r};
     const cplx<T> vb = {w.x * xb_k + w.y * xb_r, w.y * xb_k - w.x * xb_r};
-    // Z = va + i vb, stored conjugated (inverse FFT via the forward one)
     Z[k] = {va.x - vb.y, -(va.y + vb.x)};
   } else {
-    // DHT is its own inverse: pack the two real spectra and transform forward
     Z[k] = {xa[k], xb[k]};
   }
 }
 
-// Pass B: columns (2c, 2c+1) of `nslot` column pairs per block: forward along y,
-// multiply by the inverse eigenvalues (FP64-built), inverse along y, in place in F.
+// Pass B: column pairs, nslot per block: forward along y, x 1/lambda (FP64), inverse along y.
 template <typename T>
 __global__ void __launch_bounds__(kFftThreads) k_poisB(Dev<T> d, int nslot) {
   extern __shared__ unsigned char smem[];
   const int ny = d.ny, nx = d.nx;
   cplx<T> *A = reinterpret_cast<cplx<T> *>(smem);
   cplx<T> *B = A + nslot * ny;
-  T *X = reinterpret_cast<T *>(B + nslot * ny);   // 2 * nslot * ny reals
+  cplx<T> *TW = B + nslot * ny;
+  T *X = reinterpret_cast<T *>(TW + ny);   // 2 * nslot * ny reals
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int c0 = 2 * nslot * blockIdx.x;
+  load_twiddles(TW, d.tw_y, ny);
   for (int w = tid; w < nslot * ny; w += nthr) {
     const int f = w / ny, n = w - f * ny;
     const int ca = c0 + 2 * f, cb = ca + 1;
@@ -765,7 +843,7 @@ __global__ void __launch_bounds__(kFftThreads) k_poisB(Dev<T> d, int nslot) {
     A[f * ny + n] = {xa, xb};
   }
   __syncthreads();
-  cplx<T> *R = fft_lds(A, B, d.fy, d.tw_y, nslot, ny);
+  cplx<T> *R = fft_lds(A, B, d.fy, TW, nslot, ny);
   for (int w = tid; w < nslot * ny; w += nthr) {
     const int f = w / ny, k = w - f * ny;
     const int ca = c0 + 2 * f, cb = ca + 1;
@@ -780,10 +858,9 @@ __global__ void __launch_bounds__(kFftThreads) k_poisB(Dev<T> d, int nslot) {
       xa = va.x - va.y;
       xb = vb.x - vb.y;
     }
-    // 1 / (lambda_x + lambda_y) in double, zero for the constant mode (poisson.py:119-126)
     const double ly = d.lam_y[k];
-    double la = d.lam_x[ca < nx ? ca : 0] + ly;
-    double lb = d.lam_x[cb < nx ? cb : 0] + ly;
+    const double la = d.lam_x[ca < nx ? ca : 0] + ly;
+    const double lb = d.lam_x[cb < nx ? cb : 0] + ly;
     const double ia = (la != 0.0 && !(ca == 0 && k == 0)) ? 1.0 / la : 0.0;
     const double ib = (lb != 0.0) ? 1.0 / lb : 0.0;
     X[(2 * f) * ny + k] = xa * (T)ia;
@@ -793,10 +870,11 @@ __global__ void __launch_bounds__(kFftThreads) k_poisB(Dev<T> d, int nslot) {
   cplx<T> *Z = (R == A) ? B : A;
   for (int w = tid; w < nslot * ny; w += nthr) {
     const int f = w / ny, k = w - f * ny;
-    pack_inverse(X + (2 * f) * ny, X + (2 * f + 1) * ny, Z + f * ny, ny, d.dct_y, d.cs_y, k);
+    pack_inverse(X + (2 * f) * ny, X + (2 * f + 1) * ny, Z + f * ny, ny, d.dct_y, d.dct_y ? d.cs_y[k] : cplx<T>{0, 0},
+                 k);
   }
   __syncthreads();
-  cplx<T> *R2 = fft_lds(Z, (Z == A) ? B : A, d.fy, d.tw_y, nslot, ny);
+  cplx<T> *R2 = fft_lds(Z, (Z == A) ? B : A, d.fy, TW, nslot, ny);
   const T inv_n = (T)(1.0 / (double)ny);
   for (int w = tid; w < nslot * ny; w += nthr) {
     const int f = w / ny, m = w - f * ny;
@@ -817,29 +895,32 @@ __global__ void __launch_bounds__(kFftThreads) k_poisB(Dev<T> d, int nslot) {
   }
 }
 
-// Pass C: inverse along x of rows (2b, 2b+1) -> psi; accumulate the step-averaged
-// pressure p (core.py:935-942): stage 0 p = w psi, stage 1 p += w psi, stage 2
-// p = (p + psi) / dt.
+// Pass C: inverse along x of rows (2b, 2b+1) -> psi; accumulate the step-averaged pressure
+// p (core.py:935-942): stage 0 p = w psi, stage 1 p += w psi, stage 2 p = (p + psi) / dt.
 template <typename T>
 __global__ void __launch_bounds__(kFftThreads) k_poisC(Dev<T> d, T *__restrict__ psi, T *__restrict__ p, int stage) {
   extern __shared__ unsigned char smem[];
   const int nx = d.nx;
   cplx<T> *A = reinterpret_cast<cplx<T> *>(smem);
   cplx<T> *B = A + nx;
-  T *X = reinterpret_cast<T *>(B + nx);
+  cplx<T> *TW = B + nx;
+  T *X = reinterpret_cast<T *>(TW + nx);
   const int ja = 2 * blockIdx.x, jb = ja + 1;
   const bool hasb = jb < d.ny;
+  load_twiddles(TW, d.tw_x, nx);
   for (int k = threadIdx.x; k < nx; k += blockDim.x) {
     X[k] = d.F[ja * nx + k];
     X[nx + k] = hasb ? d.F[jb * nx + k] : (T)0;
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < nx; k += blockDim.x) pack_inverse(X, X + nx, A, nx, d.dct_x, d.cs_x, k);
+  for (int k = threadIdx.x; k < nx; k += blockDim.x)
+    pack_inverse(X, X + nx, A, nx, d.dct_x, d.dct_x ? d.cs_x[k] : cplx<T>{0, 0}, k);
   __syncthreads();
-  cplx<T> *R = fft_lds(A, B, d.fx, d.tw_x, 1, nx);
+  cplx<T> *R = fft_lds(A, B, d.fx, TW, 1, nx);
   const T inv_n = (T)(1.0 / (double)nx);
   T rdt = 0;
   if (p != nullptr && stage == 2) rdt = (T)1 / d.red->dt;
+  const T w = (T)kForceW[stage];
   for (int m = threadIdx.x; m < nx; m += blockDim.x) {
     T va, vb;
     if (d.dct_x) {
@@ -855,7 +936,6 @@ __global__ void __launch_bounds__(kFftThreads) k_poisC(Dev<T> d, T *__restrict__
     psi[ja * nx + q] = va;
     if (hasb) psi[jb * nx + q] = vb;
     if (p != nullptr) {
-      const T w = (T)kForceW[stage];
       if (stage == 0) {
         p[ja * nx + q] = va * w;
         if (hasb) p[jb * nx + q] = vb * w;
@@ -870,10 +950,9 @@ __global__ void __launch_bounds__(kFftThreads) k_poisC(Dev<T> d, T *__restrict__
   }
 }
 
-// u -= grad psi on interior faces (core.py:645-660); the outlet gets the mass correction.
+// u -= grad psi (standalone projection, core.py:626-660).
 template <typename T>
-__global__ void __launch_bounds__(256) k_correct(Dev<T> d, T *__restrict__ u, T *__restrict__ v,
-                                                 const T *__restrict__ psi, int use_corr) {
+__global__ void __launch_bounds__(256) k_correct(Dev<T> d, T *__restrict__ u, T *__restrict__ v, const T *__restrict__ psi) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   const int nx = d.nx, ny = d.ny;
@@ -884,8 +963,6 @@ __global__ void __launch_bounds__(256) k_correct(Dev<T> d, T *__restrict__ u, T 
       u[o] = u[o] - (psi[j * nx + ie] - psi[j * nx + wrap(ie - 1, nx)]) * d.inv_h;
     } else if (i > 0 && i < nx) {
       u[o] = u[o] - (psi[j * nx + i] - psi[j * nx + i - 1]) * d.inv_h;
-    } else if (i == nx && use_corr && !d.closed) {
-      u[o] = u[o] + d.red->corr;
     }
   }
   if (j <= ny && i < nx) {
@@ -899,58 +976,124 @@ __global__ void __launch_bounds__(256) k_correct(Dev<T> d, T *__restrict__ u, T 
   }
 }
 
-// emitter clock (physics.py:798-802)
+// End of step on one tile: write u = u^ - grad psi, smoke transport with it (physics.py:
+// 805-830 / core.py:709-771), next-step dt inputs and the window force / heat partials.
 template <typename T>
-__device__ __forceinline__ T emitter_on(T t, const T *tm) {
-  const T start = tm[0], stop = tm[1], period = tm[2], duty = tm[3];
-  const bool active = (t >= start) && (t < stop);
-  const T safe = period > (T)0 ? period : (T)1;
-  const T x = t - start;
-  T r = fmod(x, safe);
-  if (r != (T)0 && ((r < (T)0) != (safe < (T)0))) r += safe;
-  const T phase = r / safe;
-  const bool on = active && ((period <= (T)0) || (phase < duty));
-  return on ? (T)1 : (T)0;
-}
-
-// Smoke step with the end-of-step velocity: channels + emitters (physics.py:805-830)
-// or the inlet comb alone (core.py:709-733, 758-771).
-template <typename T>
-__global__ void __launch_bounds__(256) k_dye(Dev<T> d, const T *__restrict__ cin, T *__restrict__ cout,
-                                             const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ t_in) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int ch = blockIdx.z;
+__global__ void __launch_bounds__(256) k_final(Dev<T> d, const T *uh, const T *vh, T *u_out, T *v_out,
+                                              const T *th_new, const T *cin, T *cout, const T *t_in) {
+  __shared__ Lds<T> L;
+  const int tj0 = blockIdx.y * TY, ti0 = blockIdx.x * TX;
   const int nx = d.nx, ny = d.ny;
-  if (j >= ny || i >= nx) return;
-  const T dt = d.red->dt;
-  const T *c = cin + (size_t)ch * ny * nx;
-  const int o = j * nx + i;
-  const T inflow = d.dye_inflow ? d.dye_inflow[ch * ny + j] : (T)0;
-  const int mode_in = d.dye_mode == 1 ? 2 : 1;   // the comb path always uses its inlet column
-  const T fxl = tvd_fx(d, c, u, j, i, inflow, mode_in), fxr = tvd_fx(d, c, u, j, i + 1, inflow, mode_in);
-  const T fyb = tvd_fy(d, c, v, j, i), fyt = tvd_fy(d, c, v, j + 1, i);
-  T val;
-  if (d.dye_mode == 2) {
-    val = c[o] + dt * ((((fxl - fxr) + fyb) - fyt) * d.inv_h);
-    const T tnew = *t_in + dt;
-    for (int g = 0; g < d.n_groups; ++g) {
-      if (d.gch[g] != ch) continue;
-      const T wg = d.g_weight[(size_t)g * ny * nx + o] * emitter_on(tnew, d.g_timing + 4 * g);
-      val = val + wg * (d.g_target[(size_t)g * ny * nx + o] - val);
-    }
-    if (d.win && j >= d.j0 && j < d.j1 && i >= d.i0 && i < d.i1)
-      val = val * (d.chi_c[(j - d.j0) * d.wx + (i - d.i0)] <= (T)0.5 ? (T)1 : (T)0);
-    val = val > (T)1e-6 ? val : (T)0;
-    val = val < (T)1 ? val : (T)1;
-  } else {
-    val = c[o] - (dt * d.inv_h) * (((fxr - fxl) + fyt) - fyb);
-    val = val > (T)1e-6 ? val : (T)0;
-    val = val < (T)1 ? val : (T)1;
-    if (d.win && j >= d.j0 && j < d.j1 && i >= d.i0 && i < d.i1)
-      val = val * (d.chi_c[(j - d.j0) * d.wx + (i - d.i0)] <= (T)0.5 ? (T)1 : (T)0);
+  const T corr = d.red->corr;
+  load_uv(d, L, tj0, ti0, uh, vh, (const T *)d.psi, corr);
+  __syncthreads();
+  const int i = ti0 + threadIdx.x;
+  for (int k = 0; k < 2; ++k) {
+    const int j = tj0 + threadIdx.y + 4 * k;
+    if (j >= ny || i >= nx) continue;
+    const int r = j - tj0 + 2, c = i - ti0 + 2;
+    u_out[j * (nx + 1) + i] = L.u[r][c];
+    v_out[j * nx + i] = L.v[r][c];
+    if (i == nx - 1) u_out[j * (nx + 1) + nx] = L.u[r][c + 1];
+    if (j == ny - 1) v_out[ny * nx + i] = L.v[r + 1][c];
   }
-  cout[(size_t)ch * ny * nx + o] = val;
+  if (d.les) {
+    tile_nut(d, L, tj0, ti0);
+    __syncthreads();
+  }
+  tile_maxima(d, L, tj0, ti0, th_new);
+  // force / heat partials of the window faces this tile owns
+  const bool touch = d.win && ti0 <= d.i1 && (ti0 + TX > d.i0 || d.periodic) && tj0 <= d.j1 &&
+                     (tj0 + TY > d.j0 || d.periodic);
+  if (touch) {
+    double acc[kMaxWin];
+    for (int q = 0; q < kMaxWin; ++q) acc[q] = 0;
+    const int nuw = d.wy * (d.wx + 1), nvw = (d.wy + 1) * d.wx;
+    auto add_u = [&](int j, int fi) {
+      if (j >= d.j0 && j < d.j1 && fi >= d.i0 && fi <= d.i1) {
+        const int wo = (j - d.j0) * (d.wx + 1) + (fi - d.i0);
+        const double a = (double)d.acc_u[wo];
+        for (int b = 0; b < d.nbody; ++b) {
+          acc[4 * b] += (double)d.k_u[(size_t)b * 2 * nuw + wo] * a;
+          acc[4 * b + 1] += (double)d.k_u[(size_t)b * 2 * nuw + nuw + wo] * a;
+        }
+      }
+    };
+    auto add_v = [&](int fj, int i_) {
+      if (fj >= d.j0 && fj <= d.j1 && i_ >= d.i0 && i_ < d.i1) {
+        const int wo = (fj - d.j0) * d.wx + (i_ - d.i0);
+        const double a = (double)d.acc_v[wo];
+        for (int b = 0; b < d.nbody; ++b) {
+          acc[4 * b + 2] += (double)d.k_v[(size_t)b * 2 * nvw + wo] * a;
+          acc[4 * b + 3] += (double)d.k_v[(size_t)b * 2 * nvw + nvw + wo] * a;
+        }
+      }
+    };
+    for (int k = 0; k < 2; ++k) {
+      const int j = tj0 + threadIdx.y + 4 * k;
+      if (j >= ny || i >= nx) continue;
+      add_u(j, i);
+      add_v(j, i);
+      // the extra faces nx / ny belong to the last column / row (periodic: to face 0)
+      if (i == nx - 1 && !d.periodic) add_u(j, nx);
+      if (j == ny - 1 && !d.periodic) add_v(ny, i);
+      if (i == 0 && d.periodic) add_u(j, nx);
+      if (j == 0 && d.periodic) add_v(ny, i);
+      if (d.thermal && d.n_temp > 0 && j >= d.j0 && j < d.j1 && i >= d.i0 && i < d.i1) {
+        const int wo = (j - d.j0) * d.wx + (i - d.i0);
+        for (int b = 0; b < d.n_temp; ++b)
+          acc[4 * d.nbody + b] += (double)d.w_t[(size_t)b * d.wy * d.wx + wo] * (double)d.acc_t[wo];
+      }
+    }
+    block_sum4<kMaxWin>(acc, L.red);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+      double *p = d.pforce + kMaxWin * (blockIdx.y * gridDim.x + blockIdx.x);
+      for (int q = 0; q < kMaxWin; ++q) p[q] = acc[q];
+    }
+  }
+  // smoke
+  if (d.dye_mode != 0 && cin != nullptr) {
+    const T dt = d.red->dt;
+    const T tnew = *t_in + dt;
+    const T ih = d.inv_h;
+    for (int ch = 0; ch < d.n_dye; ++ch) {
+      const T *c = cin + (size_t)ch * ny * nx;
+      __syncthreads();
+      if (d.dye_mode == 2)
+        load_c(d, L, tj0, ti0, c, d.dye_inflow + (size_t)ch * ny, (T)0, 1);
+      else
+        load_c(d, L, tj0, ti0, c, d.dye_inflow, (T)0, 2);
+      __syncthreads();
+      for (int k = 0; k < 2; ++k) {
+        const int j = tj0 + threadIdx.y + 4 * k;
+        if (j >= ny || i >= nx) continue;
+        const int r = j - tj0 + 2, cc = i - ti0 + 2;
+        const T fxl = tvd_x(L, r, cc, L.u[r][cc]), fxr = tvd_x(L, r, cc + 1, L.u[r][cc + 1]);
+        const T fyb = tvd_y(L, r, cc, L.v[r][cc]), fyt = tvd_y(L, r + 1, cc, L.v[r + 1][cc]);
+        const int o = j * nx + i;
+        T val;
+        const bool solid = d.win && j >= d.j0 && j < d.j1 && i >= d.i0 && i < d.i1 &&
+                           !(d.chi_c[(j - d.j0) * d.wx + (i - d.i0)] <= (T)0.5);
+        if (d.dye_mode == 2) {
+          val = L.c[r][cc] + dt * ((((fxl - fxr) + fyb) - fyt) * ih);
+          for (int g = 0; g < d.n_groups; ++g) {
+            if (d.gch[g] != ch) continue;
+            const T wg = d.g_weight[(size_t)g * ny * nx + o] * emitter_on(tnew, d.g_timing + 4 * g);
+            val = val + wg * (d.g_target[(size_t)g * ny * nx + o] - val);
+          }
+          if (solid) val = val * (T)0;
+          val = val > (T)1e-6 ? val : (T)0;
+          val = val < (T)1 ? val : (T)1;
+        } else {
+          val = L.c[r][cc] - (dt * ih) * (((fxr - fxl) + fyt) - fyb);
+          val = val > (T)1e-6 ? val : (T)0;
+          val = val < (T)1 ? val : (T)1;
+          if (solid) val = val * (T)0;
+        }
+        cout[(size_t)ch * ny * nx + o] = val;
+      }
+    }
+  }
 }
 
 template <typename T>
@@ -959,39 +1102,81 @@ __global__ void k_copy(T *__restrict__ dst, const T *__restrict__ src, size_t n)
     dst[e] = src[e];
 }
 
-// Per-step outputs (core.py:942-1004, physics.py:760-795): forces, heat rates, t, dt,
-// history rings. One block; sums in double, fixed order.
+// Per-step outputs (core.py:942-1004, physics.py:760-795) and the next step's dt.
 template <typename T>
 __global__ void __launch_bounds__(1024) k_finalize(Dev<T> d, const T *__restrict__ t_in, T *__restrict__ t_out,
                                                    T *__restrict__ dt_out, const T *__restrict__ u_new,
-                                                   const T *__restrict__ th_new, aero2d_outputs o, int n_stage_blocks) {
-  __shared__ double ldsd[1024];
-  __shared__ double res[64];
+                                                   const T *__restrict__ th_new, aero2d_outputs o) {
+  __shared__ double lds[16 * kMaxWin];
+  __shared__ double res[kMaxWin];
   const int tid = threadIdx.x, nthr = blockDim.x;
   const T dt = d.red->dt;
   const T tnew = *t_in + dt;
-  const int wy = d.wy, wx = d.wx;
-  const int nuw = wy * (wx + 1), nvw = (wy + 1) * wx;
-  // forces
+  const int nx = d.nx, ny = d.ny, nxp = nx + 1;
+  const int w = tid >> 6, nw = nthr >> 6;
+  // window partials of the blocks that own window faces
+  double acc[kMaxWin];
+  for (int q = 0; q < kMaxWin; ++q) acc[q] = 0;
+  if (d.win) {
+    int bx0 = d.i0 / TX, bx1 = (d.i1 < nx ? d.i1 : nx - 1) / TX;
+    int by0 = d.j0 / TY, by1 = (d.j1 < ny ? d.j1 : ny - 1) / TY;
+    if (d.periodic) { bx0 = 0; by0 = 0; }
+    const int nbx = bx1 - bx0 + 1, nb = nbx * (by1 - by0 + 1);
+    for (int e = tid; e < nb; e += nthr) {
+      const int bx = bx0 + e % nbx, by = by0 + e / nbx;
+      const double *p = d.pforce + kMaxWin * (by * d.nbx + bx);
+      for (int q = 0; q < kMaxWin; ++q) acc[q] += p[q];
+    }
+  }
+  for (int q = 0; q < kMaxWin; ++q) acc[q] = wave_sum(acc[q]);
+  if ((tid & 63) == 0)
+    for (int q = 0; q < kMaxWin; ++q) lds[w * kMaxWin + q] = acc[q];
+  __syncthreads();
+  if (tid == 0) {
+    for (int q = 0; q < kMaxWin; ++q) {
+      double sacc = 0;
+      for (int ww = 0; ww < nw; ++ww) sacc += lds[ww * kMaxWin + q];
+      res[q] = sacc;
+    }
+  }
+  __syncthreads();
+  // next step's dt inputs
+  double mx[5];
+  reduce_pmax(d, d.nbx * d.nby, lds, mx);
+  // heat outputs: wall sums, in/outflow
+  double ws[6] = {0, 0, 0, 0, 0, 0};
+  if (d.thermal) {
+    for (int jj = tid; jj < ny; jj += nthr) {
+      ws[0] += (double)(d.wall_values[0] - th_new[jj * nx]);
+      ws[1] += (double)(d.wall_values[1] - th_new[jj * nx + nx - 1]);
+      ws[4] += (double)(u_new[jj * nxp] * *d.theta_in);
+      ws[5] += (double)(u_new[jj * nxp + nx] * th_new[jj * nx + nx - 1]);
+    }
+    for (int ii = tid; ii < nx; ii += nthr) {
+      ws[2] += (double)(d.wall_values[2] - th_new[ii]);
+      ws[3] += (double)(d.wall_values[3] - th_new[(ny - 1) * nx + ii]);
+    }
+    for (int q = 0; q < 6; ++q) ws[q] = wave_sum(ws[q]);
+    __syncthreads();
+    if ((tid & 63) == 0)
+      for (int q = 0; q < 6; ++q) lds[w * 6 + q] = ws[q];
+    __syncthreads();
+    if (tid == 0)
+      for (int q = 0; q < 6; ++q) {
+        double sacc = 0;
+        for (int ww = 0; ww < nw; ++ww) sacc += lds[ww * 6 + q];
+        ws[q] = sacc;
+      }
+  }
+  if (tid != 0) return;
+  *t_out = tnew;
+  *dt_out = dt;
+  const T scale = (T)(-(d.hd * d.hd)) / ((dt * (T)0.5) * (T)d.force_ref);
   T sumc[3] = {0, 0, 0};
   for (int b = 0; b < d.nbody; ++b) {
-    double r[4] = {0, 0, 0, 0};
-    if (d.win) {
-      const T *ku = d.k_u + (size_t)b * 2 * nuw;
-      const T *kv = d.k_v + (size_t)b * 2 * nvw;
-      for (int e = tid; e < nuw; e += nthr) {
-        r[0] += (double)ku[e] * (double)d.acc_u[e];
-        r[1] += (double)ku[nuw + e] * (double)d.acc_u[e];
-      }
-      for (int e = tid; e < nvw; e += nthr) {
-        r[2] += (double)kv[e] * (double)d.acc_v[e];
-        r[3] += (double)kv[nvw + e] * (double)d.acc_v[e];
-      }
-    }
-    for (int q = 0; q < 4; ++q) r[q] = block_sum(r[q], ldsd);
-    const T scale = (T)(-(d.hd * d.hd)) / ((dt * (T)0.5) * (T)d.force_ref);
-    const T cd = (T)r[0] * scale, cl = (T)r[2] * scale, cm = ((T)r[1] + (T)r[3]) * scale;
-    if (tid == 0 && o.coeffs) {
+    const T cd = (T)res[4 * b] * scale, cl = (T)res[4 * b + 2] * scale;
+    const T cm = ((T)res[4 * b + 1] + (T)res[4 * b + 3]) * scale;
+    if (o.coeffs) {
       T *cf = reinterpret_cast<T *>(o.coeffs);
       cf[3 * b] = cd;
       cf[3 * b + 1] = cl;
@@ -1001,85 +1186,54 @@ __global__ void __launch_bounds__(1024) k_finalize(Dev<T> d, const T *__restrict
     sumc[1] += cl;
     sumc[2] += cm;
   }
-  // heat outputs
-  const int nxp = d.nx + 1, nx = d.nx, ny = d.ny;
-  const T h2dt = (T)(d.hd * d.hd) / dt;
-  if (d.thermal) {
-    for (int b = 0; b < d.n_temp; ++b) {
-      double r = 0;
-      if (d.win)
-        for (int e = tid; e < wy * wx; e += nthr) r += (double)d.w_t[(size_t)b * wy * wx + e] * (double)d.acc_t[e];
-      r = block_sum(r, ldsd);
-      if (tid == 0) res[b] = (double)((T)r * h2dt);
-    }
-    // wall heat (left, right, bottom, top), inflow / outflow heat, storage
-    const T kap = *d.kappa;
-    double wsum[4] = {0, 0, 0, 0}, fl[2] = {0, 0}, st = 0;
-    for (int jj = tid; jj < ny; jj += nthr) {
-      wsum[0] += (double)(d.wall_values[0] - th_new[jj * nx]);
-      wsum[1] += (double)(d.wall_values[1] - th_new[jj * nx + nx - 1]);
-      fl[0] += (double)(u_new[jj * nxp] * *d.theta_in);
-      fl[1] += (double)(u_new[jj * nxp + nx] * th_new[jj * nx + nx - 1]);
-    }
-    for (int ii = tid; ii < nx; ii += nthr) {
-      wsum[2] += (double)(d.wall_values[2] - th_new[ii]);
-      wsum[3] += (double)(d.wall_values[3] - th_new[(ny - 1) * nx + ii]);
-    }
-    for (int b = tid; b < n_stage_blocks; b += nthr) st += d.partial2[b];
-    for (int q = 0; q < 4; ++q) wsum[q] = block_sum(wsum[q], ldsd);
-    fl[0] = block_sum(fl[0], ldsd);
-    fl[1] = block_sum(fl[1], ldsd);
-    st = block_sum(st, ldsd);
-    if (tid == 0) {
-      const int nt = d.n_temp;
-      const int len[4] = {ny, ny, nx, nx};
-      for (int q = 0; q < 4; ++q) {
-        const int m = d.wm[q];
-        double val = 0;
-        if (m == AERO_TH_TEMPERATURE || m == AERO_TH_INLET) val = (double)(((T)2 * kap) * (T)wsum[q]);
-        else if (m == AERO_TH_HEAT_FLUX) val = (double)(kap * d.wall_values[q] * (T)(len[q] * d.hd));
-        res[nt + q] = val;
-      }
-      res[nt + 4] = (double)((T)(st - d.red->th_old) * h2dt);
-      const bool tunnel = !d.periodic && !d.closed;
-      res[nt + 5] = tunnel ? (double)((T)fl[0] * (T)d.hd) : 0.0;
-      res[nt + 6] = tunnel ? (double)((T)fl[1] * (T)d.hd) : 0.0;
-    }
+  if (o.hist) {
+    long long *cnt = reinterpret_cast<long long *>(o.hist_count);
+    const long long row = *cnt % o.hist_cap;
+    T *hrow = reinterpret_cast<T *>(o.hist) + row * 4;
+    hrow[0] = tnew;
+    hrow[1] = sumc[0];
+    hrow[2] = sumc[1];
+    hrow[3] = sumc[2];
+    *cnt = *cnt + 1;
   }
-  __syncthreads();
-  if (tid == 0) {
-    *t_out = tnew;
-    *dt_out = dt;
-    if (o.hist) {
-      long long *cnt = reinterpret_cast<long long *>(o.hist_count);
-      const long long row = *cnt % o.hist_cap;
-      T *hrow = reinterpret_cast<T *>(o.hist) + row * 4;
-      hrow[0] = tnew;
-      hrow[1] = sumc[0];
-      hrow[2] = sumc[1];
-      hrow[3] = sumc[2];
+  if (d.thermal) {
+    const int nt = d.n_temp;
+    const T h2dt = (T)(d.hd * d.hd) / dt;
+    const T kap = *d.kappa;
+    T vals[kMaxWin + 8];
+    for (int b = 0; b < nt; ++b) vals[b] = (T)res[4 * d.nbody + b] * h2dt;
+    const int len[4] = {ny, ny, nx, nx};
+    for (int q = 0; q < 4; ++q) {
+      const int m = d.wm[q];
+      T val = 0;
+      if (m == AERO_TH_TEMPERATURE || m == AERO_TH_INLET) val = ((T)2 * kap) * (T)ws[q];
+      else if (m == AERO_TH_HEAT_FLUX) val = kap * d.wall_values[q] * (T)(len[q] * d.hd);
+      vals[nt + q] = val;
+    }
+    vals[nt + 4] = (T)(mx[4] - d.red->th_old) * h2dt;
+    const bool tunnel = !d.periodic && !d.closed;
+    vals[nt + 5] = tunnel ? (T)ws[4] * (T)d.hd : (T)0;
+    vals[nt + 6] = tunnel ? (T)ws[5] * (T)d.hd : (T)0;
+    if (o.heat) for (int b = 0; b < nt; ++b) reinterpret_cast<T *>(o.heat)[b] = vals[b];
+    if (o.wall_heat) for (int q = 0; q < 4; ++q) reinterpret_cast<T *>(o.wall_heat)[q] = vals[nt + q];
+    if (o.heat_storage) reinterpret_cast<T *>(o.heat_storage)[0] = vals[nt + 4];
+    if (o.flow_heat) {
+      reinterpret_cast<T *>(o.flow_heat)[0] = vals[nt + 5];
+      reinterpret_cast<T *>(o.flow_heat)[1] = vals[nt + 6];
+    }
+    if (o.aux) {
+      long long *cnt = reinterpret_cast<long long *>(o.aux_count);
+      const long long row = *cnt % o.aux_cap;
+      const int ncol = 1 + nt + 7;
+      T *arow = reinterpret_cast<T *>(o.aux) + row * ncol;
+      arow[0] = tnew;
+      for (int q = 0; q < nt + 7; ++q) arow[1 + q] = vals[q];
       *cnt = *cnt + 1;
     }
-    if (d.thermal) {
-      const int nt = d.n_temp;
-      if (o.heat) for (int b = 0; b < nt; ++b) reinterpret_cast<T *>(o.heat)[b] = (T)res[b];
-      if (o.wall_heat) for (int q = 0; q < 4; ++q) reinterpret_cast<T *>(o.wall_heat)[q] = (T)res[nt + q];
-      if (o.heat_storage) reinterpret_cast<T *>(o.heat_storage)[0] = (T)res[nt + 4];
-      if (o.flow_heat) {
-        reinterpret_cast<T *>(o.flow_heat)[0] = (T)res[nt + 5];
-        reinterpret_cast<T *>(o.flow_heat)[1] = (T)res[nt + 6];
-      }
-      if (o.aux) {
-        long long *cnt = reinterpret_cast<long long *>(o.aux_count);
-        const long long row = *cnt % o.aux_cap;
-        const int ncol = 1 + nt + 7;
-        T *arow = reinterpret_cast<T *>(o.aux) + row * ncol;
-        arow[0] = tnew;
-        for (int q = 0; q < nt + 7; ++q) arow[1 + q] = (T)res[q];
-        *cnt = *cnt + 1;
-      }
-    }
   }
+  // the next step (chained) starts from this state
+  d.red->dt = compute_dt(d, (T)mx[0], (T)mx[1], (T)mx[2], (T)mx[3]);
+  d.red->th_old = mx[4];
 }
 
 }  // namespace ns2d
@@ -1110,13 +1264,12 @@ struct aero2d_plan {
   size_t esize;
   size_t bytes = 0;
   std::vector<void *> allocs;
-  // scratch
-  void *uA, *vA, *uB, *vB, *thA, *thB, *nut, *F, *psi, *acc_u, *acc_v, *acc_t, *outcol, *dye_tmp;
+  void *uA, *vA, *uB, *vB, *uC, *vC, *thA, *thB, *F, *psi, *acc_u, *acc_v, *acc_t, *outcol, *dye_tmp;
   void *tw_x, *tw_y, *cs_x, *cs_y;
-  double *lam_x, *lam_y, *partial, *partial2;
+  double *lam_x, *lam_y, *pmax, *pforce;
   void *red;
   FftPlan fx, fy;
-  int stage_blocks;
+  int nbx, nby;
   int nslotB;
 };
 
@@ -1174,6 +1327,8 @@ Dev<T> make_dev(const aero2d_plan *p, const aero2d_geom *g, const aero2d_params 
   std::memset(&d, 0, sizeof(d));
   d.nx = s.nx;
   d.ny = s.ny;
+  d.nbx = p->nbx;
+  d.nby = p->nby;
   d.periodic = s.periodic;
   d.closed = s.closed;
   d.upwind = s.advection == 1;
@@ -1230,9 +1385,9 @@ Dev<T> make_dev(const aero2d_plan *p, const aero2d_geom *g, const aero2d_params 
     d.dye_inflow = (const T *)pr->dye_inflow; d.g_weight = (const T *)pr->group_weight;
     d.g_target = (const T *)pr->group_target; d.g_timing = (const T *)pr->group_timing;
   }
-  d.nut = (T *)p->nut; d.F = (T *)p->F; d.psi = (T *)p->psi;
+  d.F = (T *)p->F; d.psi = (T *)p->psi;
   d.acc_u = (T *)p->acc_u; d.acc_v = (T *)p->acc_v; d.acc_t = (T *)p->acc_t; d.outcol = (T *)p->outcol;
-  d.partial = p->partial; d.partial2 = p->partial2;
+  d.pmax = p->pmax; d.pforce = p->pforce;
   d.red = (Red<T> *)p->red;
   d.tw_x = (const cplx<T> *)p->tw_x; d.tw_y = (const cplx<T> *)p->tw_y;
   d.cs_x = (const cplx<T> *)p->cs_x; d.cs_y = (const cplx<T> *)p->cs_y;
@@ -1244,77 +1399,71 @@ Dev<T> make_dev(const aero2d_plan *p, const aero2d_geom *g, const aero2d_params 
 }
 
 template <typename T>
-size_t lds_A(const aero2d_plan *p) { return 2 * p->desc.nx * sizeof(cplx<T>) + 2 * p->desc.nx * sizeof(T) + 256 * sizeof(T); }
+size_t lds_A(const aero2d_plan *p) { return 3 * p->desc.nx * sizeof(cplx<T>) + 2 * p->desc.nx * sizeof(T) + 64 * 8; }
 template <typename T>
-size_t lds_B(const aero2d_plan *p) { return p->nslotB * p->desc.ny * (2 * sizeof(cplx<T>) + 2 * sizeof(T)); }
+size_t lds_B(const aero2d_plan *p) {
+  return p->nslotB * p->desc.ny * (2 * sizeof(cplx<T>) + 2 * sizeof(T)) + p->desc.ny * sizeof(cplx<T>);
+}
 
-// divergence / rhs -> psi (passes A, B, C) on `stream`
+// right-hand side -> psi (passes A, B, C) on `stream`
 template <typename T>
-int solve_passes(aero2d_plan *p, hipStream_t st, Dev<T> &d, const T *u, const T *v, const T *f, int mode, T *psi,
-                 T *pacc, int stage) {
+void solve_passes(aero2d_plan *p, hipStream_t st, Dev<T> &d, const T *u, const T *v, const T *f, int mode, T *psi,
+                  T *pacc, int stage) {
   const int ny = p->desc.ny, nx = p->desc.nx;
   hipLaunchKernelGGL(k_poisA<T>, dim3((ny + 1) / 2), dim3(kFftThreads), lds_A<T>(p), st, d, u, v, f, mode);
   const int cols_per_block = 2 * p->nslotB;
-  hipLaunchKernelGGL(k_poisB<T>, dim3((nx + cols_per_block - 1) / cols_per_block), dim3(kFftThreads), lds_B<T>(p), st,
-                     d, p->nslotB);
+  hipLaunchKernelGGL(k_poisB<T>, dim3((nx + cols_per_block - 1) / cols_per_block), dim3(kFftThreads), lds_B<T>(p),
+                     st, d, p->nslotB);
   hipLaunchKernelGGL(k_poisC<T>, dim3((ny + 1) / 2), dim3(kFftThreads), lds_A<T>(p), st, d, psi, pacc, stage);
-  AERO_HIP_CHECK(hipGetLastError());
-  return AERO_OK;
 }
 
+// one step; chained: the dt inputs come from the previous step of this plan
 template <typename T>
 int step_impl(aero2d_plan *p, hipStream_t st, const aero2d_fields *in, aero2d_fields *out, const aero2d_geom *g,
-              const aero2d_params *pr, aero2d_outputs *o) {
+              const aero2d_params *pr, aero2d_outputs *o, bool chained) {
   const aero2d_desc &s = p->desc;
   Dev<T> d = make_dev<T>(p, g, pr);
   const int nx = s.nx, ny = s.ny;
   const dim3 blk(64, 4);
-  const dim3 grid_faces((nx + 1 + 63) / 64, (ny + 1 + 3) / 4);
-  const dim3 grid_nut((nx + 2 + 63) / 64, (ny + 2 + 3) / 4);
+  const dim3 tiles(p->nbx, p->nby);
   const T *u0 = (const T *)in->u, *v0 = (const T *)in->v, *th0 = (const T *)in->theta;
-  // state reductions -> dt
-  if (d.les) hipLaunchKernelGGL(k_nut<T>, grid_nut, blk, 0, st, d, u0, v0, 1);
-  const int nmax = std::max(ny * (nx + 1), (ny + 1) * nx);
-  const int rb = std::min(kRedBlocks, (nmax + 255) / 256);
-  hipLaunchKernelGGL(k_reduce0<T>, dim3(rb), dim3(256), 0, st, d, u0, v0, th0);
-  T *ubuf[3] = {(T *)p->uA, (T *)p->uB, (T *)out->u};
-  T *vbuf[3] = {(T *)p->vA, (T *)p->vB, (T *)out->v};
+  if (!chained) {
+    hipLaunchKernelGGL(k_pre<T>, tiles, blk, 0, st, d, u0, v0, th0);
+    hipLaunchKernelGGL(k_dt<T>, dim3(1), dim3(1024), 0, st, d);
+  }
+  T *ubuf[3] = {(T *)p->uA, (T *)p->uB, (T *)p->uC};
+  T *vbuf[3] = {(T *)p->vA, (T *)p->vB, (T *)p->vC};
   T *tbuf[3] = {(T *)p->thA, (T *)p->thB, (T *)out->theta};
-  const T *up = u0, *vp = v0, *thp = th0;
+  const T *up = u0, *vp = v0, *thp = th0, *psi_p = nullptr;
+  const int mode = (s.periodic || s.closed) ? 1 : 2;
   for (int k = 0; k < 3; ++k) {
-    if (d.les && k > 0) hipLaunchKernelGGL(k_nut<T>, grid_nut, blk, 0, st, d, up, vp, 0);
     Stage<T> sg;
     sg.s = k;
     sg.a = kSsp[k][0];
     sg.b = kSsp[k][1];
     sg.w = kForceW[k];
-    sg.up = up; sg.vp = vp; sg.thp = thp;
+    sg.up = up; sg.vp = vp; sg.psi_p = psi_p; sg.thp = thp;
     sg.u0 = u0; sg.v0 = v0; sg.th0 = th0;
     sg.uh = ubuf[k]; sg.vh = vbuf[k]; sg.thh = s.thermal ? tbuf[k] : nullptr;
-    hipLaunchKernelGGL(k_stage<T>, grid_faces, blk, 0, st, d, sg);
-    int rc = solve_passes<T>(p, st, d, sg.uh, sg.vh, nullptr, (s.periodic || s.closed) ? 1 : 2, d.psi, (T *)out->p, k);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_correct<T>, grid_faces, blk, 0, st, d, sg.uh, sg.vh, (const T *)d.psi,
-                       (s.periodic || s.closed) ? 0 : 1);
-    up = sg.uh; vp = sg.vh; thp = sg.thh;
+    hipLaunchKernelGGL(k_stage<T>, tiles, blk, 0, st, d, sg);
+    solve_passes<T>(p, st, d, sg.uh, sg.vh, nullptr, mode, d.psi, (T *)out->p, k);
+    up = sg.uh; vp = sg.vh; thp = sg.thh; psi_p = d.psi;
   }
-  // smoke with the end-of-step velocity
-  if (s.dye_mode != 0 && in->dye && out->dye) {
+  const bool dye = s.dye_mode != 0 && in->dye && out->dye;
+  const bool inplace = dye && in->dye == out->dye;
+  T *dye_dst = dye ? (inplace ? (T *)p->dye_tmp : (T *)out->dye) : nullptr;
+  hipLaunchKernelGGL(k_final<T>, tiles, blk, 0, st, d, (const T *)p->uC, (const T *)p->vC, (T *)out->u, (T *)out->v,
+                     (const T *)out->theta, dye ? (const T *)in->dye : (const T *)nullptr, dye_dst, (const T *)in->t);
+  if (inplace) {
     const size_t n = (size_t)s.n_dye * ny * nx;
-    const bool inplace = in->dye == out->dye;
-    T *dst = inplace ? (T *)p->dye_tmp : (T *)out->dye;
-    hipLaunchKernelGGL(k_dye<T>, dim3((nx + 63) / 64, (ny + 3) / 4, s.n_dye), blk, 0, st, d, (const T *)in->dye, dst,
-                       (const T *)out->u, (const T *)out->v, (const T *)in->t);
-    if (inplace) {
-      const int cb = (int)std::min<size_t>(4096, (n + 255) / 256);
-      hipLaunchKernelGGL(k_copy<T>, dim3(cb), dim3(256), 0, st, (T *)out->dye, (const T *)dst, n);
-    }
+    const int cb = (int)std::min<size_t>(4096, (n + 255) / 256);
+    hipLaunchKernelGGL(k_copy<T>, dim3(cb), dim3(256), 0, st, (T *)out->dye, (const T *)dye_dst, n);
   }
   aero2d_outputs oo;
   std::memset(&oo, 0, sizeof(oo));
   if (o) oo = *o;
   hipLaunchKernelGGL(k_finalize<T>, dim3(1), dim3(1024), 0, st, d, (const T *)in->t, (T *)out->t, (T *)out->dt,
-                     (const T *)out->u, (const T *)out->theta, oo, (int)(grid_faces.x * grid_faces.y));
+                     (const T *)out->u, (const T *)out->theta, oo);
   AERO_HIP_CHECK(hipGetLastError());
   return AERO_OK;
 }
@@ -1323,11 +1472,10 @@ template <typename T>
 int project_impl(aero2d_plan *p, hipStream_t st, void *u, void *v, void *psi_out) {
   Dev<T> d = make_dev<T>(p, nullptr, nullptr);
   T *psi = psi_out ? (T *)psi_out : d.psi;
-  int rc = solve_passes<T>(p, st, d, (const T *)u, (const T *)v, nullptr, 1, psi, nullptr, 0);
-  if (rc) return rc;
+  solve_passes<T>(p, st, d, (const T *)u, (const T *)v, nullptr, 1, psi, nullptr, 0);
   const dim3 blk(64, 4);
   const dim3 grid_faces((p->desc.nx + 1 + 63) / 64, (p->desc.ny + 1 + 3) / 4);
-  hipLaunchKernelGGL(k_correct<T>, grid_faces, blk, 0, st, d, (T *)u, (T *)v, (const T *)psi, 0);
+  hipLaunchKernelGGL(k_correct<T>, grid_faces, blk, 0, st, d, (T *)u, (T *)v, (const T *)psi);
   AERO_HIP_CHECK(hipGetLastError());
   return AERO_OK;
 }
@@ -1335,7 +1483,44 @@ int project_impl(aero2d_plan *p, hipStream_t st, void *u, void *v, void *psi_out
 template <typename T>
 int poisson_impl(aero2d_plan *p, hipStream_t st, const void *f, void *psi) {
   Dev<T> d = make_dev<T>(p, nullptr, nullptr);
-  return solve_passes<T>(p, st, d, nullptr, nullptr, (const T *)f, 0, (T *)psi, nullptr, 0);
+  solve_passes<T>(p, st, d, nullptr, nullptr, (const T *)f, 0, (T *)psi, nullptr, 0);
+  AERO_HIP_CHECK(hipGetLastError());
+  return AERO_OK;
+}
+
+template <typename T>
+int time_stage_impl(aero2d_plan *p, hipStream_t st, const aero2d_fields *f, const aero2d_geom *g,
+                    const aero2d_params *pr, int stage, int iters, double *avg_ms) {
+  const aero2d_desc &s = p->desc;
+  Dev<T> d = make_dev<T>(p, g, pr);
+  Stage<T> sg;
+  sg.s = stage;
+  sg.a = kSsp[stage][0];
+  sg.b = kSsp[stage][1];
+  sg.w = kForceW[stage];
+  // stage input = the previous stage's unprojected buffer and potential (as in a step)
+  sg.up = stage > 0 ? (const T *)p->uB : (const T *)f->u;
+  sg.vp = stage > 0 ? (const T *)p->vB : (const T *)f->v;
+  sg.psi_p = stage > 0 ? (const T *)p->psi : nullptr;
+  sg.thp = (const T *)f->theta;
+  sg.u0 = (const T *)f->u; sg.v0 = (const T *)f->v; sg.th0 = (const T *)f->theta;
+  sg.uh = (T *)p->uA; sg.vh = (T *)p->vA; sg.thh = s.thermal ? (T *)p->thA : nullptr;
+  const dim3 blk(64, 4);
+  const dim3 tiles(p->nbx, p->nby);
+  hipEvent_t e0, e1;
+  AERO_HIP_CHECK(hipEventCreate(&e0));
+  AERO_HIP_CHECK(hipEventCreate(&e1));
+  hipLaunchKernelGGL(k_stage<T>, tiles, blk, 0, st, d, sg);   // warm
+  AERO_HIP_CHECK(hipEventRecord(e0, st));
+  for (int it = 0; it < iters; ++it) hipLaunchKernelGGL(k_stage<T>, tiles, blk, 0, st, d, sg);
+  AERO_HIP_CHECK(hipEventRecord(e1, st));
+  AERO_HIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  AERO_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *avg_ms = (double)ms / (double)iters;
+  return AERO_OK;
 }
 
 int check_desc(const aero2d_desc *d) {
@@ -1347,8 +1532,8 @@ int check_desc(const aero2d_desc *d) {
   if (d->advection != 0 && d->advection != 1) { set_error("unknown advection scheme"); return AERO_EINVAL; }
   if (d->periodic && d->closed) { set_error("periodic and closed are exclusive"); return AERO_EINVAL; }
   const size_t es = d->dtype ? 8 : 4;
-  const size_t ldsx = 2 * d->nx * 2 * es + 2 * d->nx * es + 256 * es;
-  const size_t ldsy = d->ny * (2 * 2 * es + 2 * es);
+  const size_t ldsx = 3 * d->nx * 2 * es + 2 * d->nx * es + 512;
+  const size_t ldsy = d->ny * (2 * 2 * es + 2 * es) + d->ny * 2 * es;
   if (ldsx > 65536 || ldsy > 65536) { set_error("grid too large for the LDS-resident transforms"); return AERO_EUNSUPPORTED; }
   if (d->n_groups > AERO2D_MAX_GROUPS || d->n_groups < 0) { set_error("too many emitter groups"); return AERO_EUNSUPPORTED; }
   if (d->dye_mode < 0 || d->dye_mode > 2) { set_error("unknown dye mode"); return AERO_EINVAL; }
@@ -1357,75 +1542,15 @@ int check_desc(const aero2d_desc *d) {
     set_error("body window outside the grid");
     return AERO_EINVAL;
   }
-  if (d->n_temp > 58) { set_error("too many temperature bodies"); return AERO_EUNSUPPORTED; }
-  return AERO_OK;
-}
-
-}  // namespace
-
-extern "C" {
-
-const char *aero_last_error(void) { return g_error.c_str(); }
-int aero_version(void) { return 1; }
-
-int aero2d_plan_create(const aero2d_desc *desc, aero2d_plan **out) {
-  if (!out) { set_error("null plan pointer"); return AERO_EINVAL; }
-  *out = nullptr;
-  int rc = check_desc(desc);
-  if (rc) return rc;
-  aero2d_plan *p = new aero2d_plan();
-  p->desc = *desc;
-  p->dbl = desc->dtype == 1;
-  p->esize = p->dbl ? 8 : 4;
-  if (!factor_radices(desc->nx, p->fx) || !factor_radices(desc->ny, p->fy)) {
-    delete p;
-    set_error("nx and ny must be products of 2, 3 and 5");
+  if (4 * d->nbody + (d->thermal ? d->n_temp : 0) > kMaxWin) {
+    set_error("too many bodies for the per-block force partials (4 per body + 1 per temperature body <= 16)");
     return AERO_EUNSUPPORTED;
   }
-  const size_t es = p->esize;
-  const size_t nx = desc->nx, ny = desc->ny;
-  const size_t nu_ = ny * (nx + 1), nv_ = (ny + 1) * nx, nc = ny * nx;
-  const bool win = desc->win_j1 > desc->win_j0 && desc->win_i1 > desc->win_i0;
-  const size_t wy = win ? desc->win_j1 - desc->win_j0 : 0, wx = win ? desc->win_i1 - desc->win_i0 : 0;
-  const size_t blocks = ((nx + 1 + 63) / 64) * ((ny + 1 + 3) / 4);
-  p->stage_blocks = (int)blocks;
-  p->nslotB = 2;
-  while (p->nslotB > 1 && ny * p->nslotB * (2 * 2 * es + 2 * es) > 65536) p->nslotB--;
-  struct { void **ptr; size_t n; } tab[] = {
-      {&p->uA, nu_ * es}, {&p->vA, nv_ * es}, {&p->uB, nu_ * es}, {&p->vB, nv_ * es},
-      {&p->thA, desc->thermal ? nc * es : 0}, {&p->thB, desc->thermal ? nc * es : 0},
-      {&p->nut, desc->turbulence ? (ny + 2) * (nx + 2) * es : 0}, {&p->F, nc * es}, {&p->psi, nc * es},
-      {&p->acc_u, wy * (wx + 1) * es}, {&p->acc_v, (wy + 1) * wx * es}, {&p->acc_t, wy * wx * es},
-      {&p->outcol, ny * es}, {&p->dye_tmp, desc->dye_mode ? (size_t)desc->n_dye * nc * es : 0},
-      {&p->tw_x, nx * 2 * es}, {&p->tw_y, ny * 2 * es}, {&p->cs_x, nx * 2 * es}, {&p->cs_y, ny * 2 * es},
-      {(void **)&p->lam_x, nx * 8}, {(void **)&p->lam_y, ny * 8}, {(void **)&p->partial, kRedBlocks * 8},
-      {(void **)&p->partial2, blocks * 8}, {&p->red, 256}};
-  for (auto &t : tab) {
-    if ((rc = alloc(p, t.ptr, t.n))) {
-      aero2d_plan_destroy(p);
-      return rc;
-    }
-  }
-  rc = p->dbl ? upload_tables<double>(p) : upload_tables<float>(p);
-  if (rc) {
-    aero2d_plan_destroy(p);
-    return rc;
-  }
-  *out = p;
   return AERO_OK;
 }
 
-int aero2d_plan_destroy(aero2d_plan *p) {
-  if (!p) return AERO_OK;
-  for (void *a : p->allocs) (void)hipFree(a);
-  delete p;
-  return AERO_OK;
-}
-
-size_t aero2d_plan_workspace_bytes(const aero2d_plan *p) { return p ? p->bytes : 0; }
-
-int aero2d_step(aero2d_plan *p, void *stream, const aero2d_fields *in, aero2d_fields *out, const aero2d_geom *g,
-                const aero2d_params *pr, aero2d_outputs *o) {
+int check_step_args(aero2d_plan *p, const aero2d_fields *in, aero2d_fields *out, const aero2d_geom *g,
+                    const aero2d_params *pr, aero2d_outputs *o) {
   if (!p || !in || !out || !pr) { set_error("null argument"); return AERO_EINVAL; }
   if (!in->u || !in->v || !out->u || !out->v || !out->p || !in->t || !out->t || !out->dt || !pr->nu || !pr->eta) {
     set_error("missing field or parameter pointer");
@@ -1453,14 +1578,106 @@ int aero2d_step(aero2d_plan *p, void *stream, const aero2d_fields *in, aero2d_fi
     return AERO_EINVAL;
   }
   if (s.has_heat && (!pr->heat || !pr->heat_max)) { set_error("heaters need heat and heat_max"); return AERO_EINVAL; }
+  if (s.dye_mode != 0 && in->dye && !pr->dye_inflow) { set_error("smoke needs dye_inflow"); return AERO_EINVAL; }
   if (s.dye_mode == 2 && s.n_groups > 0 && (!pr->group_weight || !pr->group_target || !pr->group_timing)) {
     set_error("emitter groups need weight, target and timing");
     return AERO_EINVAL;
   }
   if (o && o->hist && (!o->hist_count || o->hist_cap <= 0)) { set_error("history ring needs count and capacity"); return AERO_EINVAL; }
   if (o && o->aux && (!o->aux_count || o->aux_cap <= 0)) { set_error("aux ring needs count and capacity"); return AERO_EINVAL; }
+  return AERO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *aero_last_error(void) { return g_error.c_str(); }
+int aero_version(void) { return 2; }
+
+int aero2d_plan_create(const aero2d_desc *desc, aero2d_plan **out) {
+  if (!out) { set_error("null plan pointer"); return AERO_EINVAL; }
+  *out = nullptr;
+  int rc = check_desc(desc);
+  if (rc) return rc;
+  aero2d_plan *p = new aero2d_plan();
+  p->desc = *desc;
+  p->dbl = desc->dtype == 1;
+  p->esize = p->dbl ? 8 : 4;
+  if (!factor_radices(desc->nx, p->fx) || !factor_radices(desc->ny, p->fy)) {
+    delete p;
+    set_error("nx and ny must be products of 2, 3 and 5");
+    return AERO_EUNSUPPORTED;
+  }
+  const size_t es = p->esize;
+  const size_t nx = desc->nx, ny = desc->ny;
+  const size_t nu_ = ny * (nx + 1), nv_ = (ny + 1) * nx, nc = ny * nx;
+  const bool win = desc->win_j1 > desc->win_j0 && desc->win_i1 > desc->win_i0;
+  const size_t wy = win ? desc->win_j1 - desc->win_j0 : 0, wx = win ? desc->win_i1 - desc->win_i0 : 0;
+  p->nbx = (int)((nx + TX - 1) / TX);
+  p->nby = (int)((ny + TY - 1) / TY);
+  const size_t blocks = (size_t)p->nbx * p->nby;
+  p->nslotB = 2;
+  while (p->nslotB > 1 && ny * p->nslotB * (2 * 2 * es + 2 * es) + ny * 2 * es > 65536) p->nslotB--;
+  struct { void **ptr; size_t n; } tab[] = {
+      {&p->uA, nu_ * es}, {&p->vA, nv_ * es}, {&p->uB, nu_ * es}, {&p->vB, nv_ * es}, {&p->uC, nu_ * es},
+      {&p->vC, nv_ * es}, {&p->thA, desc->thermal ? nc * es : 0}, {&p->thB, desc->thermal ? nc * es : 0},
+      {&p->F, nc * es}, {&p->psi, nc * es},
+      {&p->acc_u, wy * (wx + 1) * es}, {&p->acc_v, (wy + 1) * wx * es}, {&p->acc_t, wy * wx * es},
+      {&p->outcol, ny * es}, {&p->dye_tmp, desc->dye_mode ? (size_t)desc->n_dye * nc * es : 0},
+      {&p->tw_x, nx * 2 * es}, {&p->tw_y, ny * 2 * es}, {&p->cs_x, nx * 2 * es}, {&p->cs_y, ny * 2 * es},
+      {(void **)&p->lam_x, nx * 8}, {(void **)&p->lam_y, ny * 8}, {(void **)&p->pmax, blocks * 5 * 8},
+      {(void **)&p->pforce, blocks * kMaxWin * 8}, {&p->red, 256}};
+  for (auto &t : tab) {
+    if ((rc = alloc(p, t.ptr, t.n))) {
+      aero2d_plan_destroy(p);
+      return rc;
+    }
+  }
+  rc = p->dbl ? upload_tables<double>(p) : upload_tables<float>(p);
+  if (rc) {
+    aero2d_plan_destroy(p);
+    return rc;
+  }
+  *out = p;
+  return AERO_OK;
+}
+
+int aero2d_plan_destroy(aero2d_plan *p) {
+  if (!p) return AERO_OK;
+  for (void *a : p->allocs) (void)hipFree(a);
+  delete p;
+  return AERO_OK;
+}
+
+size_t aero2d_plan_workspace_bytes(const aero2d_plan *p) { return p ? p->bytes : 0; }
+
+int aero2d_step(aero2d_plan *p, void *stream, const aero2d_fields *in, aero2d_fields *out, const aero2d_geom *g,
+                const aero2d_params *pr, aero2d_outputs *o) {
+  int rc = check_step_args(p, in, out, g, pr, o);
+  if (rc) return rc;
   hipStream_t st = (hipStream_t)stream;
-  return p->dbl ? step_impl<double>(p, st, in, out, g, pr, o) : step_impl<float>(p, st, in, out, g, pr, o);
+  return p->dbl ? step_impl<double>(p, st, in, out, g, pr, o, false) : step_impl<float>(p, st, in, out, g, pr, o, false);
+}
+
+int aero2d_step_chained(aero2d_plan *p, void *stream, const aero2d_fields *in, aero2d_fields *out,
+                        const aero2d_geom *g, const aero2d_params *pr, aero2d_outputs *o) {
+  int rc = check_step_args(p, in, out, g, pr, o);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  return p->dbl ? step_impl<double>(p, st, in, out, g, pr, o, true) : step_impl<float>(p, st, in, out, g, pr, o, true);
+}
+
+int aero2d_time_stage(aero2d_plan *p, void *stream, const aero2d_fields *f, const aero2d_geom *g,
+                      const aero2d_params *pr, int stage, int iters, double *avg_ms) {
+  if (!p || !f || !pr || !avg_ms || !f->u || !f->v || iters <= 0 || stage < 0 || stage > 2) {
+    set_error("bad argument");
+    return AERO_EINVAL;
+  }
+  if (p->desc.thermal && !f->theta) { set_error("thermal plan needs theta"); return AERO_EINVAL; }
+  hipStream_t st = (hipStream_t)stream;
+  return p->dbl ? time_stage_impl<double>(p, st, f, g, pr, stage, iters, avg_ms)
+                : time_stage_impl<float>(p, st, f, g, pr, stage, iters, avg_ms);
 }
 
 int aero2d_project(aero2d_plan *p, void *stream, void *u, void *v, void *psi) {

@@ -1,0 +1,185 @@
+"""Benchmark of the MI355X Navier-Stokes step (one JSON line on stdout).
+
+Workload at N=1: the 2D heated-airfoil case at 1024x512 cells, FP32 - NACA 0012 at 8 deg held
+at theta = 1 (dT 10 K), Re 1e5, Smagorinsky Cs 0.17, upwind3 + SSP-RK3 at CFL 0.5, exact DCT
+projection, Boussinesq coupling (Pr 0.71, Ri 0.1), one passive smoke scalar from a line
+emitter; seeded 1 % Gaussian velocity noise on the inflow state (synthetic data, no files).
+A step is one full SSP-RK3 time step of that case through ``FlowSolver.advance`` (HIP graphs
+of 10 steps). ``value`` is Mcell-steps/s = 524288 cells x steps / s / 1e6.
+
+Also reported: ``roofline`` of the dominant kernel (the fused stage kernel; algorithmic bytes
+per launch / its average duration timed with HIP events on the solver's stream) and
+``cpu_baseline`` (the CPU oracle on the host's cores, a bounded sample of the same case).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+
+METRIC = "Mcell-steps/s at 2D 1024×512 & 3D 4.2M cells; %HBM peak; ensemble solves/s"
+HBM_PEAK_GBS = 8000.0
+NX, NY = 1024, 512
+SEED = 202
+# algorithmic bytes per cell of one stage-kernel launch at stages 2-3 (FP32): read u, v, nu_t,
+# theta of the stage input and u0, v0, theta0 of the step start (7 x 4 B), write u, v, theta
+# of the stage (3 x 4 B)
+STAGE_BYTES_PER_CELL = 40
+STEP_BYTES_PER_CELL = 300     # whole fused step of this case (byte model of the design notes)
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def c2_config(dtype="float32", device="cuda"):
+    import math
+
+    import numpy as np
+
+    from aerolume_amd.ns2d.config import EmitterSpec, ImmersedBody, SolverConfig, ThermalConfig
+    t = 0.12
+    beta = np.linspace(0.0, np.pi, 201)
+    x = 0.5 * (1 - np.cos(beta))
+    yt = 5 * t * (0.2969 * np.sqrt(x) - 0.1260 * x - 0.3516 * x**2 + 0.2843 * x**3 - 0.1036 * x**4)
+    foil = np.vstack([np.c_[x, yt][::-1], np.c_[x, -yt][1:-1]])
+    a = math.radians(-8.0)
+    c, s = math.cos(a), math.sin(a)
+    p = foil - (0.25, 0.0)
+    foil = np.c_[c * p[:, 0] - s * p[:, 1], s * p[:, 0] + c * p[:, 1]] + (0.25, 0.0)
+    return SolverConfig(nx=NX, ny=NY, h=8.0 / NX, x0=-2.0, y0=-2.0, re=1e5,
+                        bodies=[ImmersedBody(foil, thermal="temperature", thermal_value=1.0)],
+                        advection="upwind3", turbulence="smagorinsky", cfl=0.5, dtype=dtype, device=device,
+                        dye=True, dye_inlet=False, perturb=False,
+                        thermal=ThermalConfig(prandtl=0.71, richardson=0.1),
+                        emitters=[EmitterSpec(kind="line", position=(-1.0, -0.2), end=(-1.0, 0.2), channel=0,
+                                              rate=1.0)])
+
+
+def noisy_velocity(ops, seed):
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    u = 1.0 + 0.01 * rng.standard_normal((ops.ny, ops.nx + 1))
+    v = 0.01 * rng.standard_normal((ops.ny + 1, ops.nx))
+    return u, v
+
+
+def gpu_run(steps: int, warmup: int):
+    import torch
+
+    from aerolume_amd.ns2d import solver as S
+    from aerolume_amd import _lib
+    cfg = c2_config()
+    solver = S.FlowSolver(cfg)
+    ops = solver.ops
+    u, v = noisy_velocity(ops, SEED)
+    u = torch.as_tensor(u, device=ops.device, dtype=ops.dtype)
+    v = torch.as_tensor(v, device=ops.device, dtype=ops.dtype)
+    u, v = S.apply_boundary_values(u, v, ops)
+    u, v, _ = S.project(u, v, ops)
+    solver.set_velocity(u, v)
+    log(f"bench: workspace {solver.workspace_bytes / 2**20:.1f} MiB, window {solver.geometry.window}")
+    solver.advance(warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solver.advance(steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    if not all(bool(torch.isfinite(t).all()) for t in (solver.state.u, solver.state.v, solver.state.theta)):
+        raise RuntimeError("non-finite state after the timed steps")
+    # dominant kernel: the stage kernel, timed alone with HIP events on the solver's stream
+    lib = _lib.load()
+    ms = ctypes.c_double()
+    pl = solver._plan
+    iters = 200
+    st = _lib.stream_handle(ops.device)
+    rc = lib.aero2d_time_stage(pl.handle, ctypes.c_void_p(st), ctypes.byref(solver._fields),
+                               ctypes.byref(pl.geom_ptrs), ctypes.byref(pl.params), 1, iters, ctypes.byref(ms))
+    _lib.check(rc, "aero2d_time_stage")
+    torch.cuda.synchronize()
+    return solver, wall, ms.value
+
+
+def cpu_baseline(seconds: float = 15.0):
+    """The CPU oracle (torch, FP32, all host threads, flush-to-zero) on the same case:
+    warm-up step, then as many steps as fit in about ``seconds``, best per-step time."""
+    import torch
+
+    from aerolume_amd.ns2d import setup as su
+    from oracle import ns2d as O
+    cfg = c2_config(device="cpu")
+    ops = su.make_operators(cfg, device="cpu")
+    geom = su.build_geometry(cfg.bodies, ops, window=su.live_window(cfg.bodies, ops))
+    u, v = noisy_velocity(ops, SEED)
+    u, v = O.apply_boundary_values(torch.as_tensor(u, dtype=ops.dtype), torch.as_tensor(v, dtype=ops.dtype), ops)
+    u, v, _, _ = O.project(u, v, ops)
+    z = torch.zeros((), dtype=ops.dtype)
+    st = O.State(u, v, torch.zeros(ops.ny, ops.nx, dtype=ops.dtype), torch.zeros(ops.ny, ops.nx, dtype=ops.dtype),
+                 z.clone(), z.clone(), torch.zeros(ops.ny, ops.nx, dtype=ops.dtype))
+    threads = torch.get_num_threads()
+    torch.set_flush_denormal(True)
+    st, _ = O.step_full(st, geom, ops)
+    n, t_total, best = 0, 0.0, float("inf")
+    while t_total < seconds and n < 200:
+        t0 = time.perf_counter()
+        st, _ = O.step_full(st, geom, ops)
+        dt = time.perf_counter() - t0
+        best = min(best, dt)
+        t_total += dt
+        n += 1
+    return {"value": round(NX * NY / best / 1e6, 3), "unit": "Mcell-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{n} steps of the 1024x512 heated-airfoil case (FP32) after 1 warm-up step, best step "
+                      f"of {t_total:.1f} s, torch CPU oracle, {threads} threads, flush-to-zero"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (every GPU box has one "
+              "MI355X); run with --gpus 1", file=sys.stderr)
+        return 2
+    solver, wall, stage_ms = gpu_run(args.steps, args.warmup)
+    cells = NX * NY
+    value = cells * args.steps / wall / 1e6
+    ms_step = wall / args.steps * 1e3
+    stage_bytes = STAGE_BYTES_PER_CELL * cells
+    achieved = stage_bytes / (stage_ms * 1e-3) / 1e9
+    step_gbs = STEP_BYTES_PER_CELL * cells / (wall / args.steps) / 1e9
+    line = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Mcell-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_step, 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "2D 1024x512 heated NACA 0012 at 8 deg, Re 1e5, Smagorinsky, Boussinesq Pr 0.71 "
+                               "Ri 0.1, line smoke emitter, upwind3 + SSP-RK3 CFL 0.5, DCT projection",
+                   "nx": NX, "ny": NY, "cells": cells, "seed": SEED, "graph_steps": 10},
+        "roofline": {"bound": "hbm", "kernel": "k_stage (stage 2 of 3)", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "avg_launch_us": round(stage_ms * 1e3, 3),
+                     "algorithmic_bytes_per_launch": stage_bytes},
+        "step_hbm": {"algorithmic_bytes_per_step": STEP_BYTES_PER_CELL * cells, "achieved_gbs": round(step_gbs, 1),
+                     "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+        line["gpu_over_cpu"] = round(value / line["cpu_baseline"]["value"], 1)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

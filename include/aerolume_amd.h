@@ -129,10 +129,24 @@ int aero2d_plan_destroy(aero2d_plan *plan);
 size_t aero2d_plan_workspace_bytes(const aero2d_plan *plan);
 int aero2d_step(aero2d_plan *plan, void *stream, const aero2d_fields *in, aero2d_fields *out,
                 const aero2d_geom *geom, const aero2d_params *params, aero2d_outputs *outputs);
+/* Same step as aero2d_step, without re-reading the state for its dt inputs: it reuses the
+ * maxima (|u|, |v|, |theta|, nu_t) and sum theta that the previous aero2d_step /
+ * aero2d_step_chained call of this plan computed from its output state. Valid only when
+ * `in` is that output, unmodified (the stateful solver's hot loop); saves two launches. */
+int aero2d_step_chained(aero2d_plan *plan, void *stream, const aero2d_fields *in, aero2d_fields *out,
+                        const aero2d_geom *geom, const aero2d_params *params, aero2d_outputs *outputs);
 /* u, v in place: u - grad psi with lap psi = div u (no inlet/outlet change); psi (ny, nx) optional. */
 int aero2d_project(aero2d_plan *plan, void *stream, void *u, void *v, void *psi);
 /* psi = L^-1 f with zero mean (f, psi (ny, nx); may alias). */
 int aero2d_poisson_solve(aero2d_plan *plan, void *stream, const void *f, void *psi);
+
+/* Timing helper (no reference counterpart; used by bench.py to price the dominant kernel
+ * against the HBM roofline): launches the fused stage kernel of RK stage `stage` (0..2)
+ * `iters` times on `stream`, reading `fields` as both stage input and step start and
+ * writing only plan scratch, between two HIP events; returns the average launch time in ms.
+ * Synchronises the stream. Call after at least one aero2d_step (it reuses that step's dt). */
+int aero2d_time_stage(aero2d_plan *plan, void *stream, const aero2d_fields *fields, const aero2d_geom *geom,
+                      const aero2d_params *params, int stage, int iters, double *avg_ms);
 
 const char *aero_last_error(void);
 int aero_version(void);

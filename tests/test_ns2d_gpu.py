@@ -183,7 +183,7 @@ def test_flowsolver_graphs_equal_eager():
     _need_gpu()
     import dataclasses
     S = _S()
-    cfg = cases.c2_cfg(128, 64, "float32")
+    cfg = cases.c2_cfg(128, 64, "float32", device="cuda")
     a = S.FlowSolver(cfg)
     b = S.FlowSolver(dataclasses.replace(cfg, use_cuda_graphs=False))
     assert a.uses_cuda_graphs and not b.uses_cuda_graphs
@@ -197,3 +197,51 @@ def test_flowsolver_graphs_equal_eager():
     ph = a.physics_history()
     assert len(ph["t"]) == 25 and np.isfinite(ph["heat:0"]).all()
     assert a.graph_error is None
+
+
+def test_chained_steps_equal_plain_steps():
+    """FlowSolver's chained steps (dt inputs left by the previous step) equal functional
+    steps that re-read the state every time, bitwise."""
+    _need_gpu()
+    S = _S()
+    cfg = cases.c2_cfg(128, 64, "float32", device="cuda")
+    fs = S.FlowSolver(cfg)
+    u, v = np.random.default_rng(1).standard_normal((2, 1))  # noqa: F841 (shape probe)
+    st0 = fs.state
+    ref = S.State(st0.u.clone(), st0.v.clone(), st0.p.clone(), st0.dye.clone(), st0.t.clone(), st0.dt.clone(),
+                  st0.theta.clone())
+    fs.advance(15)
+    for _ in range(15):
+        ref, _ = S.step_full(ref, fs.geometry, fs.ops)
+    torch.cuda.synchronize()
+    for k in ("u", "v", "p", "theta", "dye", "t", "dt"):
+        assert torch.equal(getattr(fs.state, k), getattr(ref, k)), k
+
+
+def test_periodic_taylor_green_matches_oracle():
+    """Doubly periodic Taylor-Green vortex, float64, 30 steps: GPU equals the oracle to 1e-12."""
+    _need_gpu()
+    import math
+    from aerolume_amd.ns2d.config import SolverConfig
+    from oracle import ns2d as O
+    S = _S()
+    n = 40
+    cfg = SolverConfig(nx=n, ny=n, h=2 * math.pi / n, x0=0, y0=0, re=50.0, periodic=True, dtype="float64",
+                       device="cpu", dye=False, perturb=False, turbulence="wale")
+    from aerolume_amd.ns2d import setup as su
+    ops_c = su.make_operators(cfg, device="cpu")
+    geom_c = su.build_geometry([], ops_c)
+    h = ops_c.h
+    xu, yu = np.meshgrid(np.arange(n + 1) * h, (np.arange(n) + 0.5) * h)
+    xv, yv = np.meshgrid((np.arange(n) + 0.5) * h, np.arange(n + 1) * h)
+    u = np.sin(xu) * np.cos(yu) + 0.01 * np.cos(3 * yu)
+    v = -np.cos(xv) * np.sin(yv)
+    z = torch.zeros((), dtype=torch.float64)
+    ref = O.State(torch.as_tensor(u), torch.as_tensor(v), torch.zeros(n, n, dtype=torch.float64), None, z, z)
+    ops, geom = _setup(cfg)
+    cur = _state(S, ops, u, v)
+    for _ in range(30):
+        ref, _ = O.step_full(ref, geom_c, ops_c)
+        cur, _ = S.step_full(cur, geom, ops)
+    assert _err(cur.u, ref.u) < 1e-12
+    assert _err(cur.p, ref.p) < 1e-10

@@ -40,6 +40,7 @@ void set_error(const std::string &msg) { g_error = msg; }
 namespace ns2d {
 
 constexpr int kFftThreads = 256;
+constexpr int kMaxPer = 16;                // row elements per FFT thread (n <= 4096)
 constexpr int TX = 64, TY = 8;              // cells per tile (block of 64 x 4 threads, 2 rows each)
 constexpr int HW = TX + 4, HH = TY + 4;     // halo-2 tiles of u, v, scalar
 constexpr int NW = TX + 2, NH = TY + 2;     // eddy-viscosity tile (halo 1)
@@ -150,70 +151,188 @@ __device__ __forceinline__ T v_in(const Dev<T> &d, const T *__restrict__ v, cons
   return val;
 }
 
-// u at row j (-2..ny+1), face i (-2..nx+2), ghost rules of core.py:287-305. Positions
-// outside that range are clamped to valid memory; no output reads them.
+// ------------------------------------------------------------------ tile loads
+// The tile loaders first map every tile position to an in-domain address and a transform
+// (pure integer work), then issue all loads at once, then apply the transforms: the loads
+// of a thread are independent, so the block waits for one memory latency, not one per
+// element. The transforms reproduce the ghost rules of core.py:287-325 and scalar.py:42-60.
+constexpr int NE = (HH * HW + 255) / 256;   // tile elements per thread
+
+// flag bits: 1 gradient correction, 2 outlet correction, 4 negate (mirror), 8/16/24 wall
+// kind slip/noslip/moving (u rows outside the domain)
 template <typename T>
-__device__ __forceinline__ T u_ghost(const Dev<T> &d, const T *u, const T *psi, T corr, int j, int i) {
+__device__ __forceinline__ void map_u(const Dev<T> &d, int j, int i, bool has_psi, int &idx, int &p1, int &p0,
+                                      int &flag) {
   const int nx = d.nx, ny = d.ny;
-  if (d.periodic) return u_in(d, u, psi, corr, clampi(wrap(j, ny), 0, ny - 1), clampi(wrap(i, nx), 0, nx));
-  T sx = (T)1;
-  if (i < 0) {
-    if (d.closed) { i = -i; sx = (T)-1; } else i = 0;
-  } else if (i > nx) {
-    if (d.closed) { i = 2 * nx - i; sx = (T)-1; } else i = nx;
+  flag = 0;
+  int jj, ii;
+  if (d.periodic) {
+    jj = clampi(wrap(j, ny), 0, ny - 1);
+    ii = clampi(wrap(i, nx), 0, nx);
+    const int ie = ii == nx ? 0 : ii;
+    p1 = jj * nx + ie;
+    p0 = jj * nx + wrap(ie - 1, nx);
+    if (has_psi) flag |= 1;
+  } else {
+    if (i < 0) {
+      if (d.closed) { ii = -i; flag |= 4; } else ii = 0;
+    } else if (i > nx) {
+      if (d.closed) { ii = 2 * nx - i; flag |= 4; } else ii = nx;
+    } else {
+      ii = i;
+    }
+    ii = clampi(ii, 0, nx);
+    if (j < 0) {
+      jj = clampi(-1 - j, 0, ny - 1);
+      flag |= 8 * (d.bcb + 1);
+    } else if (j >= ny) {
+      jj = clampi(2 * ny - 1 - j, 0, ny - 1);
+      flag |= 8 * (d.bct + 1);
+    } else {
+      jj = j;
+    }
+    p1 = jj * nx + (ii < nx ? ii : nx - 1);
+    p0 = ii > 0 ? p1 - 1 : p1;
+    if (has_psi) {
+      if (ii > 0 && ii < nx) flag |= 1;
+      else if (ii == nx && !d.closed) flag |= 2;
+    }
   }
-  i = clampi(i, 0, nx);
-  if (j < 0) return wall_ghost(d.bcb, sx * u_in(d, u, psi, corr, clampi(-1 - j, 0, ny - 1), i), d.ground);
-  if (j >= ny) return wall_ghost(d.bct, sx * u_in(d, u, psi, corr, clampi(2 * ny - 1 - j, 0, ny - 1), i), (T)0);
-  return sx * u_in(d, u, psi, corr, j, i);
+  idx = jj * (nx + 1) + ii;
 }
-// v at face row j (-2..ny+2), column i (-2..nx+1), ghost rules of core.py:308-325.
 template <typename T>
-__device__ __forceinline__ T v_ghost(const Dev<T> &d, const T *v, const T *psi, int j, int i) {
+__device__ __forceinline__ T finish_u(const Dev<T> &d, T x, T q1, T q0, T corr, int flag, int is_bottom) {
+  if (flag & 1) x = x - (q1 - q0) * d.inv_h;
+  else if (flag & 2) x = x + corr;
+  if (flag & 4) x = -x;
+  const int wk = flag >> 3;   // 0 interior row, else kind + 1
+  if (wk == 2) x = -x;
+  else if (wk == 3) x = (T)2 * (is_bottom ? d.ground : (T)0) - x;
+  return x;
+}
+template <typename T>
+__device__ __forceinline__ void map_v(const Dev<T> &d, int j, int i, bool has_psi, int &idx, int &p1, int &p0,
+                                      int &flag) {
   const int nx = d.nx, ny = d.ny;
-  if (d.periodic) return v_in(d, v, psi, clampi(wrap(j, ny), 0, ny), clampi(wrap(i, nx), 0, nx - 1));
-  T sy = (T)1, sx = (T)1;
-  if (j < 0) { j = -j; sy = (T)-1; } else if (j > ny) { j = 2 * ny - j; sy = (T)-1; }
-  if (i < 0) {
-    i = -1 - i;
-    sx = (d.closed && d.bcl != AERO_BC_NOSLIP) ? (T)1 : (T)-1;
-  } else if (i >= nx) {
-    if (d.closed) { i = 2 * nx - 1 - i; sx = d.bcr != AERO_BC_NOSLIP ? (T)1 : (T)-1; } else i = nx - 1;
+  flag = 0;
+  int jj, ii;
+  if (d.periodic) {
+    jj = clampi(wrap(j, ny), 0, ny);
+    ii = clampi(wrap(i, nx), 0, nx - 1);
+    const int je = jj == ny ? 0 : jj;
+    p1 = je * nx + ii;
+    p0 = wrap(je - 1, ny) * nx + ii;
+    if (has_psi) flag |= 1;
+  } else {
+    bool neg = false;
+    if (j < 0) { jj = -j; neg = !neg; } else if (j > ny) { jj = 2 * ny - j; neg = !neg; } else jj = j;
+    if (i < 0) {
+      ii = -1 - i;
+      if (!(d.closed && d.bcl != AERO_BC_NOSLIP)) neg = !neg;
+    } else if (i >= nx) {
+      if (d.closed) {
+        ii = 2 * nx - 1 - i;
+        if (d.bcr == AERO_BC_NOSLIP) neg = !neg;
+      } else {
+        ii = nx - 1;
+      }
+    } else {
+      ii = i;
+    }
+    jj = clampi(jj, 0, ny);
+    ii = clampi(ii, 0, nx - 1);
+    p1 = (jj < ny ? jj : ny - 1) * nx + ii;
+    p0 = jj > 0 ? p1 - nx : p1;
+    if (neg) flag |= 4;
+    if (has_psi && jj > 0 && jj < ny) flag |= 1;
   }
-  return sx * (sy * v_in(d, v, psi, clampi(j, 0, ny), clampi(i, 0, nx - 1)));
+  idx = jj * nx + ii;
 }
-// cell scalar with the TVD ghost rules of scalar.py:42-60: inflow (or replicate) on the
-// left, replicate elsewhere, wrap when periodic. use_inflow 2: the smoke comb's rule
-// (inflow also in a closed box, core.py:718-723).
 template <typename T>
-__device__ __forceinline__ T c_ghost(const Dev<T> &d, const T *c, int j, int i, T inflow, int use_inflow) {
-  const int nx = d.nx, ny = d.ny;
-  if (d.periodic) return c[clampi(wrap(j, ny), 0, ny - 1) * nx + clampi(wrap(i, nx), 0, nx - 1)];
-  const int jj = clampi(j, 0, ny - 1);
-  if (i < 0 && (use_inflow == 2 || (use_inflow == 1 && !d.closed))) return inflow;
-  return c[jj * nx + clampi(i, 0, nx - 1)];
+__device__ __forceinline__ T finish_v(const Dev<T> &d, T x, T q1, T q0, int flag) {
+  if (flag & 1) x = x - (q1 - q0) * d.inv_h;
+  if (flag & 4) x = -x;
+  return x;
 }
 
-// ------------------------------------------------------------------ tile loads
 // U, V tiles of the stage-input velocity around the tile at (tj0, ti0).
 template <typename T>
 __device__ void load_uv(const Dev<T> &d, Lds<T> &L, int tj0, int ti0, const T *u, const T *v, const T *psi, T corr) {
   const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-  for (int e = tid; e < HH * HW; e += 256) {
-    const int r = e / HW, c = e - r * HW;
-    L.u[r][c] = u_ghost(d, u, psi, corr, tj0 - 2 + r, ti0 - 2 + c);
-    L.v[r][c] = v_ghost(d, v, psi, tj0 - 2 + r, ti0 - 2 + c);
+  const bool hp = psi != nullptr;
+  int ui[NE], u1[NE], u0[NE], uf[NE], vi[NE], v1[NE], v0[NE], vf[NE], rr[NE];
+#pragma unroll
+  for (int q = 0; q < NE; ++q) {
+    const int e = tid + 256 * q;
+    const int ee = e < HH * HW ? e : 0;
+    const int r = ee / HW, c = ee - r * HW;
+    rr[q] = r;
+    map_u(d, tj0 - 2 + r, ti0 - 2 + c, hp, ui[q], u1[q], u0[q], uf[q]);
+    map_v(d, tj0 - 2 + r, ti0 - 2 + c, hp, vi[q], v1[q], v0[q], vf[q]);
+  }
+  T xu[NE], xv[NE], a1[NE], a0[NE], b1[NE], b0[NE];
+#pragma unroll
+  for (int q = 0; q < NE; ++q) {
+    xu[q] = u[ui[q]];
+    xv[q] = v[vi[q]];
+    if (hp) {
+      a1[q] = psi[u1[q]];
+      a0[q] = psi[u0[q]];
+      b1[q] = psi[v1[q]];
+      b0[q] = psi[v0[q]];
+    } else {
+      a1[q] = a0[q] = b1[q] = b0[q] = (T)0;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NE; ++q) {
+    const int e = tid + 256 * q;
+    if (e < HH * HW) {
+      const int r = rr[q], c = e - r * HW;
+      L.u[r][c] = finish_u(d, xu[q], a1[q], a0[q], corr, uf[q], tj0 - 2 + r < 0);
+      L.v[r][c] = finish_v(d, xv[q], b1[q], b0[q], vf[q]);
+    }
   }
 }
+// scalar tile with the TVD ghost rules of scalar.py:42-60: inflow (or replicate) on the left,
+// replicate elsewhere, wrap when periodic; use_inflow 2 is the smoke comb's rule (inflow also
+// in a closed box, core.py:718-723)
 template <typename T>
 __device__ void load_c(const Dev<T> &d, Lds<T> &L, int tj0, int ti0, const T *c, const T *inflow_col, T inflow0,
                        int use_inflow) {
   const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-  for (int e = tid; e < HH * HW; e += 256) {
-    const int r = e / HW, cc = e - r * HW;
-    const int j = tj0 - 2 + r;
-    const T inflow = inflow_col ? inflow_col[clampi(j, 0, d.ny - 1)] : inflow0;
-    L.c[r][cc] = c_ghost(d, c, j, ti0 - 2 + cc, inflow, use_inflow);
+  const int nx = d.nx, ny = d.ny;
+  int ci[NE], rr[NE], jr[NE];
+  bool inf[NE];
+#pragma unroll
+  for (int q = 0; q < NE; ++q) {
+    const int e = tid + 256 * q;
+    const int ee = e < HH * HW ? e : 0;
+    const int r = ee / HW, cc = ee - r * HW;
+    const int j = tj0 - 2 + r, i = ti0 - 2 + cc;
+    rr[q] = r;
+    jr[q] = clampi(j, 0, ny - 1);
+    if (d.periodic) {
+      ci[q] = clampi(wrap(j, ny), 0, ny - 1) * nx + clampi(wrap(i, nx), 0, nx - 1);
+      inf[q] = false;
+    } else {
+      ci[q] = jr[q] * nx + clampi(i, 0, nx - 1);
+      inf[q] = i < 0 && (use_inflow == 2 || (use_inflow == 1 && !d.closed));
+    }
+  }
+  T x[NE], fl[NE];
+#pragma unroll
+  for (int q = 0; q < NE; ++q) {
+    x[q] = c[ci[q]];
+    fl[q] = inflow_col ? inflow_col[jr[q]] : inflow0;
+  }
+#pragma unroll
+  for (int q = 0; q < NE; ++q) {
+    const int e = tid + 256 * q;
+    if (e < HH * HW) {
+      const int r = rr[q], cc = e - r * HW;
+      L.c[r][cc] = inf[q] ? fl[q] : x[q];
+    }
   }
 }
 
@@ -378,11 +497,27 @@ __device__ __forceinline__ T emitter_on(T t, const T *tm) {
   return on ? (T)1 : (T)0;
 }
 
+// Live scalars the dt rule reads, loaded once (all loads issued together).
+template <typename T>
+struct DtIn {
+  T nu, kappa, ri, dt_fixed, heat_max;
+};
+template <typename T>
+__device__ __forceinline__ DtIn<T> load_dt_in(const Dev<T> &d) {
+  DtIn<T> r;
+  r.nu = *d.nu;
+  r.dt_fixed = d.fixed_dt ? *d.dt_fixed : (T)0;
+  r.kappa = d.thermal ? *d.kappa : (T)0;
+  r.ri = d.thermal ? *d.ri : (T)0;
+  r.heat_max = d.has_heat ? *d.heat_max : (T)0;
+  return r;
+}
+
 // dt of the step (core.py:685-695 + physics.py:851-888) from the maxima of the state.
 template <typename T>
-__device__ T compute_dt(const Dev<T> &d, T umax, T vmax, T thmax, T nutmax) {
-  if (d.fixed_dt) return *d.dt_fixed;
-  const T nu = *d.nu;
+__device__ T compute_dt(const Dev<T> &d, const DtIn<T> &in, T umax, T vmax, T thmax, T nutmax) {
+  if (d.fixed_dt) return in.dt_fixed;
+  const T nu = in.nu;
   const T us = umax + vmax;
   const T uc = us > (T)1e-6 ? us : (T)1e-6;
   T dt = (T)(d.cfl * d.hd) / uc;
@@ -390,20 +525,20 @@ __device__ T compute_dt(const Dev<T> &d, T umax, T vmax, T thmax, T nutmax) {
   const T vis = (T)(0.25 * d.hd * d.hd) / nue;
   dt = vis < dt ? vis : dt;
   if (d.thermal) {
-    const T kap = d.les ? *d.kappa + nutmax * (T)(1.0 / d.prt) : *d.kappa;
+    const T kap = d.les ? in.kappa + nutmax * (T)(1.0 / d.prt) : in.kappa;
     const T rate = us * (T)(2.0 / d.hd) + kap * (T)(d.diff_coef / (d.hd * d.hd));
     const T lim = (T)1 / rate;
     dt = lim < dt ? lim : dt;
-    T acc = fabs(*d.ri) * thmax;
+    T acc = fabs(in.ri) * thmax;
     acc = acc > (T)1e-12 ? acc : (T)1e-12;
     const T ff = (T)d.cfl * sqrt((T)d.hd / acc);
     dt = ff < dt ? ff : dt;
     if (d.has_heat) {
-      T smax = *d.heat_max;
+      T smax = in.heat_max;
       smax = smax > (T)1e-12 ? smax : (T)1e-12;
       const T l1 = (T)1 / smax;
       dt = l1 < dt ? l1 : dt;
-      T q = fabs(*d.ri) * smax;
+      T q = fabs(in.ri) * smax;
       q = q > (T)1e-12 ? q : (T)1e-12;
       const T l2 = pow((T)(6.0 * d.cfl * d.hd) / q, (T)(1.0 / 3.0));
       dt = l2 < dt ? l2 : dt;
@@ -686,6 +821,7 @@ template <typename T>
 __device__ void reduce_pmax(const Dev<T> &d, int nblocks, double *lds, double out[5]) {
   const int tid = threadIdx.x;
   double m[4] = {0, 0, 0, 0}, s = 0;
+#pragma unroll 4
   for (int b = tid; b < nblocks; b += blockDim.x) {
     const double *p = d.pmax + 5 * b;
     for (int q = 0; q < 4; ++q) m[q] = fmax(m[q], p[q]);
@@ -712,9 +848,11 @@ template <typename T>
 __global__ void __launch_bounds__(1024) k_dt(Dev<T> d) {
   __shared__ double lds[16 * 5];
   double r[5];
+  DtIn<T> din;
+  if (threadIdx.x == 0) din = load_dt_in(d);
   reduce_pmax(d, d.nbx * d.nby, lds, r);
   if (threadIdx.x == 0) {
-    d.red->dt = compute_dt(d, (T)r[0], (T)r[1], (T)r[2], (T)r[3]);
+    d.red->dt = compute_dt(d, din, (T)r[0], (T)r[1], (T)r[2], (T)r[3]);
     d.red->th_old = r[4];
   }
 }
@@ -776,26 +914,36 @@ __global__ void __launch_bounds__(kFftThreads) k_poisA(Dev<T> d, const T *__rest
     corr = outlet_corr(d, red);
     if (blockIdx.x == 0 && threadIdx.x == 0) d.red->corr = corr;
   }
-  for (int n = threadIdx.x; n < nx; n += blockDim.x) {
-    const int q = d.dct_x ? makhoul_src(n, nx) : n;
-    T xa, xb = 0;
-    if (mode == 0) {
-      xa = f_in[ja * nx + q];
-      if (hasb) xb = f_in[jb * nx + q];
-    } else {
-      xa = div_at(d, u, v, ja, q, corr);
-      if (hasb) xb = div_at(d, u, v, jb, q, corr);
+  // all loads of this thread first (row elements and DCT twiddles), then the FFT
+  cplx<T> csr[kMaxPer];
+#pragma unroll
+  for (int r = 0; r < kMaxPer; ++r) {
+    const int n = threadIdx.x + kFftThreads * r;
+    if (n < nx) {
+      const int q = d.dct_x ? makhoul_src(n, nx) : n;
+      T xa, xb = 0;
+      if (mode == 0) {
+        xa = f_in[ja * nx + q];
+        if (hasb) xb = f_in[jb * nx + q];
+      } else {
+        xa = div_at(d, u, v, ja, q, corr);
+        if (hasb) xb = div_at(d, u, v, jb, q, corr);
+      }
+      A[n] = {xa, xb};
+      if (d.dct_x) csr[r] = d.cs_x[n];
     }
-    A[n] = {xa, xb};
   }
   __syncthreads();
   cplx<T> *R = fft_lds(A, B, d.fx, TW, 1, nx);
-  for (int k = threadIdx.x; k < nx; k += blockDim.x) {
+#pragma unroll
+  for (int rr = 0; rr < kMaxPer; ++rr) {
+    const int k = threadIdx.x + kFftThreads * rr;
+    if (k >= nx) continue;
     cplx<T> va, vb;
     split_pair(R, nx, k, va, vb);
     T xa, xb;
     if (d.dct_x) {
-      const cplx<T> cs = d.cs_x[k];
+      const cplx<T> cs = csr[rr];
       xa = cs.x * va.x + cs.y * va.y;
       xb = cs.x * vb.x + cs.y * vb.y;
     } else {
@@ -834,33 +982,46 @@ __global__ void __launch_bounds__(kFftThreads) k_poisB(Dev<T> d, int nslot) {
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int c0 = 2 * nslot * blockIdx.x;
   load_twiddles(TW, d.tw_y, ny);
-  for (int w = tid; w < nslot * ny; w += nthr) {
-    const int f = w / ny, n = w - f * ny;
-    const int ca = c0 + 2 * f, cb = ca + 1;
-    const int q = d.dct_y ? makhoul_src(n, ny) : n;
-    const T xa = ca < nx ? d.F[q * nx + ca] : (T)0;
-    const T xb = cb < nx ? d.F[q * nx + cb] : (T)0;
-    A[f * ny + n] = {xa, xb};
+  cplx<T> csr[kMaxPer];
+  double lyr[kMaxPer], lxa[kMaxPer], lxb[kMaxPer];
+#pragma unroll
+  for (int r = 0; r < kMaxPer; ++r) {
+    const int w = tid + nthr * r;
+    if (w < nslot * ny) {
+      const int f = w / ny, n = w - f * ny;
+      const int ca = c0 + 2 * f, cb = ca + 1;
+      const int q = d.dct_y ? makhoul_src(n, ny) : n;
+      const T xa = ca < nx ? d.F[q * nx + ca] : (T)0;
+      const T xb = cb < nx ? d.F[q * nx + cb] : (T)0;
+      A[f * ny + n] = {xa, xb};
+      csr[r] = d.cs_y[n];
+      lyr[r] = d.lam_y[n];
+      lxa[r] = d.lam_x[ca < nx ? ca : 0];
+      lxb[r] = d.lam_x[cb < nx ? cb : 0];
+    }
   }
   __syncthreads();
   cplx<T> *R = fft_lds(A, B, d.fy, TW, nslot, ny);
-  for (int w = tid; w < nslot * ny; w += nthr) {
+#pragma unroll
+  for (int rr = 0; rr < kMaxPer; ++rr) {
+    const int w = tid + nthr * rr;
+    if (w >= nslot * ny) continue;
     const int f = w / ny, k = w - f * ny;
     const int ca = c0 + 2 * f, cb = ca + 1;
     cplx<T> va, vb;
     split_pair(R + f * ny, ny, k, va, vb);
     T xa, xb;
     if (d.dct_y) {
-      const cplx<T> cs = d.cs_y[k];
+      const cplx<T> cs = csr[rr];
       xa = cs.x * va.x + cs.y * va.y;
       xb = cs.x * vb.x + cs.y * vb.y;
     } else {
       xa = va.x - va.y;
       xb = vb.x - vb.y;
     }
-    const double ly = d.lam_y[k];
-    const double la = d.lam_x[ca < nx ? ca : 0] + ly;
-    const double lb = d.lam_x[cb < nx ? cb : 0] + ly;
+    const double ly = lyr[rr];
+    const double la = lxa[rr] + ly;
+    const double lb = lxb[rr] + ly;
     const double ia = (la != 0.0 && !(ca == 0 && k == 0)) ? 1.0 / la : 0.0;
     const double ib = (lb != 0.0) ? 1.0 / lb : 0.0;
     X[(2 * f) * ny + k] = xa * (T)ia;
@@ -868,10 +1029,12 @@ __global__ void __launch_bounds__(kFftThreads) k_poisB(Dev<T> d, int nslot) {
   }
   __syncthreads();
   cplx<T> *Z = (R == A) ? B : A;
-  for (int w = tid; w < nslot * ny; w += nthr) {
+#pragma unroll
+  for (int rr = 0; rr < kMaxPer; ++rr) {
+    const int w = tid + nthr * rr;
+    if (w >= nslot * ny) continue;
     const int f = w / ny, k = w - f * ny;
-    pack_inverse(X + (2 * f) * ny, X + (2 * f + 1) * ny, Z + f * ny, ny, d.dct_y, d.dct_y ? d.cs_y[k] : cplx<T>{0, 0},
-                 k);
+    pack_inverse(X + (2 * f) * ny, X + (2 * f + 1) * ny, Z + f * ny, ny, d.dct_y, csr[rr], k);
   }
   __syncthreads();
   cplx<T> *R2 = fft_lds(Z, (Z == A) ? B : A, d.fy, TW, nslot, ny);
@@ -908,20 +1071,45 @@ __global__ void __launch_bounds__(kFftThreads) k_poisC(Dev<T> d, T *__restrict__
   const int ja = 2 * blockIdx.x, jb = ja + 1;
   const bool hasb = jb < d.ny;
   load_twiddles(TW, d.tw_x, nx);
-  for (int k = threadIdx.x; k < nx; k += blockDim.x) {
-    X[k] = d.F[ja * nx + k];
-    X[nx + k] = hasb ? d.F[jb * nx + k] : (T)0;
+  cplx<T> csr[kMaxPer];
+#pragma unroll
+  for (int r = 0; r < kMaxPer; ++r) {
+    const int k = threadIdx.x + kFftThreads * r;
+    if (k < nx) {
+      X[k] = d.F[ja * nx + k];
+      X[nx + k] = hasb ? d.F[jb * nx + k] : (T)0;
+      csr[r] = d.dct_x ? d.cs_x[k] : cplx<T>{0, 0};
+    }
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < nx; k += blockDim.x)
-    pack_inverse(X, X + nx, A, nx, d.dct_x, d.dct_x ? d.cs_x[k] : cplx<T>{0, 0}, k);
+#pragma unroll
+  for (int r = 0; r < kMaxPer; ++r) {
+    const int k = threadIdx.x + kFftThreads * r;
+    if (k < nx) pack_inverse(X, X + nx, A, nx, d.dct_x, csr[r], k);
+  }
   __syncthreads();
   cplx<T> *R = fft_lds(A, B, d.fx, TW, 1, nx);
   const T inv_n = (T)(1.0 / (double)nx);
   T rdt = 0;
   if (p != nullptr && stage == 2) rdt = (T)1 / d.red->dt;
   const T w = (T)kForceW[stage];
-  for (int m = threadIdx.x; m < nx; m += blockDim.x) {
+  // previous pressure accumulator values, loaded ahead of the FFT result
+  T pa[kMaxPer], pb[kMaxPer];
+  if (p != nullptr && stage > 0) {
+#pragma unroll
+    for (int r = 0; r < kMaxPer; ++r) {
+      const int m = threadIdx.x + kFftThreads * r;
+      if (m < nx) {
+        const int q = d.dct_x ? makhoul_src(m, nx) : m;
+        pa[r] = p[ja * nx + q];
+        pb[r] = hasb ? p[jb * nx + q] : (T)0;
+      }
+    }
+  }
+#pragma unroll
+  for (int rr = 0; rr < kMaxPer; ++rr) {
+    const int m = threadIdx.x + kFftThreads * rr;
+    if (m >= nx) continue;
     T va, vb;
     if (d.dct_x) {
       va = R[m].x * inv_n;
@@ -940,11 +1128,11 @@ __global__ void __launch_bounds__(kFftThreads) k_poisC(Dev<T> d, T *__restrict__
         p[ja * nx + q] = va * w;
         if (hasb) p[jb * nx + q] = vb * w;
       } else if (stage == 1) {
-        p[ja * nx + q] = p[ja * nx + q] + w * va;
-        if (hasb) p[jb * nx + q] = p[jb * nx + q] + w * vb;
+        p[ja * nx + q] = pa[rr] + w * va;
+        if (hasb) p[jb * nx + q] = pb[rr] + w * vb;
       } else {
-        p[ja * nx + q] = (p[ja * nx + q] + va) * rdt;
-        if (hasb) p[jb * nx + q] = (p[jb * nx + q] + vb) * rdt;
+        p[ja * nx + q] = (pa[rr] + va) * rdt;
+        if (hasb) p[jb * nx + q] = (pb[rr] + vb) * rdt;
       }
     }
   }
@@ -1110,10 +1298,43 @@ __global__ void __launch_bounds__(1024) k_finalize(Dev<T> d, const T *__restrict
   __shared__ double lds[16 * kMaxWin];
   __shared__ double res[kMaxWin];
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const T dt = d.red->dt;
-  const T tnew = *t_in + dt;
   const int nx = d.nx, ny = d.ny, nxp = nx + 1;
   const int w = tid >> 6, nw = nthr >> 6;
+  // every load this block needs is issued here, before any reduction waits on it
+  T dt = 0, tin = 0, kap = 0, tin_th = 0, wv[4] = {0, 0, 0, 0};
+  double th_old = 0;
+  DtIn<T> din;
+  if (tid == 0) {
+    dt = d.red->dt;
+    tin = *t_in;
+    th_old = d.red->th_old;
+    din = load_dt_in(d);
+    if (d.thermal) {
+      kap = *d.kappa;
+      tin_th = *d.theta_in;
+      for (int q = 0; q < 4; ++q) wv[q] = d.wall_values[q];
+    }
+  }
+  double ws[6] = {0, 0, 0, 0, 0, 0};
+  if (d.thermal) {
+    const T tin_all = *d.theta_in;
+    T w0 = d.wall_values[0], w1 = d.wall_values[1], w2 = d.wall_values[2], w3 = d.wall_values[3];
+#pragma unroll 2
+    for (int jj = tid; jj < ny; jj += nthr) {
+      const T a0 = th_new[jj * nx], a1 = th_new[jj * nx + nx - 1];
+      const T b0 = u_new[jj * nxp], b1 = u_new[jj * nxp + nx];
+      ws[0] += (double)(w0 - a0);
+      ws[1] += (double)(w1 - a1);
+      ws[4] += (double)(b0 * tin_all);
+      ws[5] += (double)(b1 * a1);
+    }
+#pragma unroll 2
+    for (int ii = tid; ii < nx; ii += nthr) {
+      const T c0 = th_new[ii], c1 = th_new[(ny - 1) * nx + ii];
+      ws[2] += (double)(w2 - c0);
+      ws[3] += (double)(w3 - c1);
+    }
+  }
   // window partials of the blocks that own window faces
   double acc[kMaxWin];
   for (int q = 0; q < kMaxWin; ++q) acc[q] = 0;
@@ -1122,6 +1343,7 @@ __global__ void __launch_bounds__(1024) k_finalize(Dev<T> d, const T *__restrict
     int by0 = d.j0 / TY, by1 = (d.j1 < ny ? d.j1 : ny - 1) / TY;
     if (d.periodic) { bx0 = 0; by0 = 0; }
     const int nbx = bx1 - bx0 + 1, nb = nbx * (by1 - by0 + 1);
+#pragma unroll 2
     for (int e = tid; e < nb; e += nthr) {
       const int bx = bx0 + e % nbx, by = by0 + e / nbx;
       const double *p = d.pforce + kMaxWin * (by * d.nbx + bx);
@@ -1144,18 +1366,7 @@ __global__ void __launch_bounds__(1024) k_finalize(Dev<T> d, const T *__restrict
   double mx[5];
   reduce_pmax(d, d.nbx * d.nby, lds, mx);
   // heat outputs: wall sums, in/outflow
-  double ws[6] = {0, 0, 0, 0, 0, 0};
   if (d.thermal) {
-    for (int jj = tid; jj < ny; jj += nthr) {
-      ws[0] += (double)(d.wall_values[0] - th_new[jj * nx]);
-      ws[1] += (double)(d.wall_values[1] - th_new[jj * nx + nx - 1]);
-      ws[4] += (double)(u_new[jj * nxp] * *d.theta_in);
-      ws[5] += (double)(u_new[jj * nxp + nx] * th_new[jj * nx + nx - 1]);
-    }
-    for (int ii = tid; ii < nx; ii += nthr) {
-      ws[2] += (double)(d.wall_values[2] - th_new[ii]);
-      ws[3] += (double)(d.wall_values[3] - th_new[(ny - 1) * nx + ii]);
-    }
     for (int q = 0; q < 6; ++q) ws[q] = wave_sum(ws[q]);
     __syncthreads();
     if ((tid & 63) == 0)
@@ -1169,6 +1380,7 @@ __global__ void __launch_bounds__(1024) k_finalize(Dev<T> d, const T *__restrict
       }
   }
   if (tid != 0) return;
+  const T tnew = tin + dt;
   *t_out = tnew;
   *dt_out = dt;
   const T scale = (T)(-(d.hd * d.hd)) / ((dt * (T)0.5) * (T)d.force_ref);
@@ -1199,7 +1411,6 @@ __global__ void __launch_bounds__(1024) k_finalize(Dev<T> d, const T *__restrict
   if (d.thermal) {
     const int nt = d.n_temp;
     const T h2dt = (T)(d.hd * d.hd) / dt;
-    const T kap = *d.kappa;
     T vals[kMaxWin + 8];
     for (int b = 0; b < nt; ++b) vals[b] = (T)res[4 * d.nbody + b] * h2dt;
     const int len[4] = {ny, ny, nx, nx};
@@ -1207,10 +1418,10 @@ __global__ void __launch_bounds__(1024) k_finalize(Dev<T> d, const T *__restrict
       const int m = d.wm[q];
       T val = 0;
       if (m == AERO_TH_TEMPERATURE || m == AERO_TH_INLET) val = ((T)2 * kap) * (T)ws[q];
-      else if (m == AERO_TH_HEAT_FLUX) val = kap * d.wall_values[q] * (T)(len[q] * d.hd);
+      else if (m == AERO_TH_HEAT_FLUX) val = kap * wv[q] * (T)(len[q] * d.hd);
       vals[nt + q] = val;
     }
-    vals[nt + 4] = (T)(mx[4] - d.red->th_old) * h2dt;
+    vals[nt + 4] = (T)(mx[4] - th_old) * h2dt;
     const bool tunnel = !d.periodic && !d.closed;
     vals[nt + 5] = tunnel ? (T)ws[4] * (T)d.hd : (T)0;
     vals[nt + 6] = tunnel ? (T)ws[5] * (T)d.hd : (T)0;
@@ -1232,7 +1443,7 @@ __global__ void __launch_bounds__(1024) k_finalize(Dev<T> d, const T *__restrict
     }
   }
   // the next step (chained) starts from this state
-  d.red->dt = compute_dt(d, (T)mx[0], (T)mx[1], (T)mx[2], (T)mx[3]);
+  d.red->dt = compute_dt(d, din, (T)mx[0], (T)mx[1], (T)mx[2], (T)mx[3]);
   d.red->th_old = mx[4];
 }
 

@@ -101,7 +101,7 @@ def gpu_run(steps: int, warmup: int):
     pl = solver._plan
     iters = 200
     st = _lib.stream_handle(ops.device)
-    rc = lib.aero2d_time_stage(pl.handle, ctypes.c_void_p(st), ctypes.byref(solver._fields),
+    rc = lib.aero2d_time_stage(pl.handle, ctypes.c_void_p(st), ctypes.byref(solver._fields[solver._parity][0]),
                                ctypes.byref(pl.geom_ptrs), ctypes.byref(pl.params), 1, iters, ctypes.byref(ms))
     _lib.check(rc, "aero2d_time_stage")
     torch.cuda.synchronize()

@@ -1,0 +1,463 @@
+"""Set-up of the 3D solver: static operators, ghost-rule table, body masks.
+
+Follows the reference's set-up (``solvers/ns3d/core.py:61-203``, ``geometry.py:265-508``):
+``Ops3D`` holds the grid, the boundary rules and the live device scalars (nu, eta);
+``Geometry3D`` holds the window arrays of the immersed bodies - solid fractions on the
+u, v, w faces and cell centres, the rigid-rotation solid velocity pre-multiplied by the
+fraction, and the force / moment weights. Everything here runs once per solver on the host
+(float64) and is copied to the device in the solver's dtype.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import NamedTuple
+
+import numpy as np
+import torch
+
+from .config import AnalyticBody, ImmersedMesh, SolverConfig3D
+
+SSP_RK3 = ((0.0, 1.0), (0.75, 0.25), (1.0 / 3.0, 2.0 / 3.0))
+FORCE_WEIGHTS = (1.0 / 6.0, 2.0 / 3.0, 1.0)
+LES_CONSTANTS = {"smagorinsky": 0.17, "wale": 0.5, "vreman": 0.07}
+ETA_PER_H = 0.05
+STIFFNESS_MIN = 60.0
+MASK_SHIFT = 0.32
+VISCOUS_DT = 1.0 / 6.0
+COEFF_NAMES = ("cd", "cs", "cl", "c_roll", "cm", "cn")
+
+# ghost-rule kinds of the C ABI (one per side of every axis of every component)
+RULE_CODES = {"edge": 0, "even": 1, "odd": 2, "affine": 3, "face_odd": 4, "periodic_cell": 5,
+              "periodic_face": 6}
+
+
+def auto_eta(h: float, re: float) -> float:
+    """Brinkman permeability rule: min(0.05 h, h^2 Re / 60) (reference ``ns3d/core.py:135-137``)."""
+    return min(ETA_PER_H * h, h * h * re / STIFFNESS_MIN)
+
+
+@dataclass
+class Ops3D:
+    """Static operators of one grid (reference ``ns3d/core.py:61-108``): Python scalars are
+    static, the 0-d tensors ``nu`` and ``eta`` live on the device and may be edited in place."""
+    cfg: SolverConfig3D
+    nx: int
+    ny: int
+    nz: int
+    h: float
+    x0: float
+    y0: float
+    z0: float
+    device: torch.device
+    dtype: torch.dtype
+    periodic: bool
+    periodic_axes: tuple
+    symmetry: bool
+    advection: str
+    turbulence: str
+    bc_x: str
+    bc_y: str
+    bc_zmin: str
+    bc_zmax: str
+    ground_speed: float
+    cfl: float
+    dt_fixed: float | None
+    ref_area: float
+    length_ref: float
+    span_ref: float
+    moment_ref: tuple
+    mask_width: float
+    mask_shift: float
+    eta_fixed: float | None
+    les_c: float
+    nu: torch.Tensor
+    eta: torch.Tensor
+    nu_value: float
+    u_inlet: torch.Tensor            # (nz, ny, 1)
+    ghost: dict
+    closed: bool = False
+    extras: dict = field(default_factory=dict)
+
+
+def resolve_device_dtype(cfg: SolverConfig3D, device=None) -> tuple[torch.device, torch.dtype]:
+    dev = device if device is not None else cfg.device
+    if dev in ("auto", None, ""):
+        dev = "cuda" if torch.cuda.is_available() else "cpu"
+    return torch.device(dev), (torch.float64 if cfg.dtype == "float64" else torch.float32)
+
+
+def check_supported(cfg: SolverConfig3D) -> None:
+    """Raise ``NotImplementedError`` for configuration parts the MI355X 3D step does not carry."""
+    missing = []
+    if cfg.thermal is not None or cfg.heat_sources:
+        missing.append("heat transfer (thermal / heat_sources)")
+    if cfg.fans or cfg.ports:
+        missing.append("fans / ports")
+    if cfg.emitters:
+        missing.append("emitters")
+    if cfg.particle_capacity:
+        missing.append("tracer particles")
+    if cfg.diagnostics:
+        missing.append("the diagnostics history")
+    if cfg.adaptive_dt != "cfl":
+        missing.append("adaptive_dt='pid'")
+    if cfg.dye:
+        missing.append("smoke (set dye=False)")
+    for b in cfg.bodies:
+        if b.thermal != "adiabatic":
+            missing.append("heated bodies")
+            break
+    if missing:
+        raise NotImplementedError("the MI355X 3D step does not carry: " + ", ".join(missing))
+
+
+def ghost_rules(cfg: SolverConfig3D) -> dict:
+    """Ghost rule per velocity component and axis: ``{comp: (x, y, z)}``, each a (lo, hi) pair
+    of (kind, value) (reference ``ns3d/core.py:147-181``). ``edge`` repeats the boundary value
+    (inlet / zero-gradient outlet); ``even`` / ``odd`` mirror a cell-centred quantity across a
+    wall half a cell away (zero gradient / zero value); ``affine`` mirrors to a prescribed
+    value (moving floor); ``face_odd`` mirrors a wall-normal component across a wall on that
+    face; ``periodic_cell`` / ``periodic_face`` wrap (face arrays carry a duplicate last face)."""
+    px, py, pz = cfg.periodic_axes
+    pc, pf = ("periodic_cell", 0.0), ("periodic_face", 0.0)
+    even, odd, edge, fodd = ("even", 0.0), ("odd", 0.0), ("edge", 0.0), ("face_odd", 0.0)
+    zlo_tan = {"slip": even, "noslip": odd, "moving": ("affine", float(cfg.ground_speed))}[cfg.bc_zmin]
+    zlo_n = {"slip": even, "noslip": odd, "moving": odd}[cfg.bc_zmin]
+    zhi = {"slip": even, "noslip": odd}[cfg.bc_zmax]
+    y_tan = odd if cfg.bc_y == "noslip" else even
+    if cfg.closed:
+        x_tan = odd if cfg.bc_x == "noslip" else even
+        ux, vx = (fodd, fodd), (x_tan, x_tan)
+    else:
+        ux, vx = (edge, edge), (odd, edge)
+    u = ((pf, pf) if px else ux, (pc, pc) if py else (y_tan, y_tan), (pc, pc) if pz else (zlo_tan, zhi))
+    v = ((pc, pc) if px else vx, (pf, pf) if py else (fodd, fodd), (pc, pc) if pz else (zlo_n, zhi))
+    w = ((pc, pc) if px else vx, (pc, pc) if py else (y_tan, y_tan), (pf, pf) if pz else (fodd, fodd))
+    return {"u": u, "v": v, "w": w}
+
+
+def rule_table(ghost: dict) -> tuple[list[int], list[float]]:
+    """Flatten the ghost rules to 18 codes and 18 values, ordered comp (u, v, w), axis
+    (x, y, z), side (lo, hi)."""
+    codes, vals = [], []
+    for comp in ("u", "v", "w"):
+        for axis in ghost[comp]:
+            for kind, val in axis:
+                codes.append(RULE_CODES[kind])
+                vals.append(float(val))
+    return codes, vals
+
+
+def make_operators3d(cfg: SolverConfig3D, device=None) -> Ops3D:
+    """Static operators for a config (reference ``ns3d/core.py:184-203``)."""
+    dev, dtype = resolve_device_dtype(cfg, device)
+    nu = torch.tensor(1.0 / cfg.re, device=dev, dtype=dtype)
+    eta = torch.tensor(cfg.eta if cfg.eta is not None else auto_eta(cfg.h, cfg.re), device=dev, dtype=dtype)
+    u_in = torch.ones(cfg.nz, cfg.ny, 1, device=dev, dtype=dtype)
+    les_c = cfg.les_constant if cfg.les_constant is not None else LES_CONSTANTS.get(cfg.turbulence, 0.0)
+    ops = Ops3D(cfg=cfg, nx=cfg.nx, ny=cfg.ny, nz=cfg.nz, h=cfg.h, x0=cfg.x0, y0=cfg.y0, z0=cfg.z0, device=dev,
+                dtype=dtype, periodic=cfg.periodic, periodic_axes=cfg.periodic_axes, symmetry=cfg.symmetry,
+                advection=cfg.advection, turbulence=cfg.turbulence, bc_x=cfg.bc_x, bc_y=cfg.bc_y,
+                bc_zmin=cfg.bc_zmin, bc_zmax=cfg.bc_zmax, ground_speed=float(cfg.ground_speed), cfl=cfg.cfl,
+                dt_fixed=cfg.dt, ref_area=cfg.ref_area, length_ref=cfg.length_ref, span_ref=cfg.span_ref,
+                moment_ref=cfg.moment_ref, mask_width=cfg.mask_width, mask_shift=MASK_SHIFT, eta_fixed=cfg.eta,
+                les_c=les_c, nu=nu, eta=eta, nu_value=1.0 / cfg.re, u_inlet=u_in, ghost=ghost_rules(cfg),
+                closed=cfg.closed)
+    if cfg.dt is not None:
+        ops.extras["dt_fixed"] = torch.tensor(float(cfg.dt), device=dev, dtype=dtype)
+    return ops
+
+
+def set_reynolds(ops: Ops3D, re: float) -> None:
+    """Change Re in place (nu, eta and the launch value), reference ``ns3d/core.py:140-144``."""
+    ops.nu.fill_(1.0 / re)
+    ops.nu_value = 1.0 / re
+    ops.eta.fill_(ops.eta_fixed if ops.eta_fixed is not None else auto_eta(ops.h, re))
+
+
+# ---------------------------------------------------------------------------
+# signed distances
+# ---------------------------------------------------------------------------
+
+def smooth_step(d: torch.Tensor, w: float) -> torch.Tensor:
+    """C1 solid fraction of a signed distance: 1 inside beyond w, 0 outside beyond w."""
+    s = (d / w).clamp(-1.0, 1.0)
+    return 0.5 - 0.5 * torch.sin(0.5 * math.pi * s)
+
+
+def rounded_box_sdf(center, size, radius):
+    """Signed distance of a box of full extents ``size`` whose edges are rounded by ``radius``."""
+    cx, cy, cz = center
+    hx, hy, hz = (0.5 * s - radius for s in size)
+
+    def sdf(x, y, z):
+        qx, qy, qz = (x - cx).abs() - hx, (y - cy).abs() - hy, (z - cz).abs() - hz
+        out = torch.sqrt(qx.clamp_min(0) ** 2 + qy.clamp_min(0) ** 2 + qz.clamp_min(0) ** 2)
+        inside = torch.maximum(torch.maximum(qx, qy), qz).clamp_max(0.0)
+        return out + inside - radius
+    return sdf
+
+
+def cylinder_y_sdf(center, radius, width):
+    """Signed distance of a cylinder of ``radius`` with its axis along y, ``width`` long."""
+    cx, cy, cz = center
+
+    def sdf(x, y, z):
+        dr = torch.sqrt((x - cx) ** 2 + (z - cz) ** 2) - radius
+        da = (y - cy).abs() - 0.5 * width
+        out = torch.sqrt(dr.clamp_min(0) ** 2 + da.clamp_min(0) ** 2)
+        return out + torch.maximum(dr, da).clamp_max(0.0)
+    return sdf
+
+
+def car_bodies(center=(0.0, 0.0, 0.0), length=4.0, width=1.6, height=1.2, corner=0.2, wheel_radius=0.3,
+               wheel_width=0.3, ground_speed=1.0, clearance=0.2):
+    """A car-like body on the floor z = center[2]: a rounded box (length x width x height)
+    above four wheel cylinders that roll at ``ground_speed`` (omega = -U / r about +y)."""
+    cx, cy, z0 = center
+    zc = z0 + clearance + 0.5 * height
+    box = AnalyticBody(rounded_box_sdf((cx, cy, zc), (length, width, height), corner),
+                       ((cx - 0.5 * length, cy - 0.5 * width, zc - 0.5 * height),
+                        (cx + 0.5 * length, cy + 0.5 * width, zc + 0.5 * height)), name="body")
+    out = [box]
+    zw = z0 + wheel_radius
+    xs = (cx - 0.3 * length, cx + 0.3 * length)
+    ys = (cy - 0.5 * width + 0.5 * wheel_width, cy + 0.5 * width - 0.5 * wheel_width)
+    for i, x in enumerate(xs):
+        for j, y in enumerate(ys):
+            c = (x, y, zw)
+            out.append(AnalyticBody(cylinder_y_sdf(c, wheel_radius, wheel_width),
+                                    ((x - wheel_radius, y - 0.5 * wheel_width, zw - wheel_radius),
+                                     (x + wheel_radius, y + 0.5 * wheel_width, zw + wheel_radius)),
+                                    name=f"wheel{2 * i + j}", role="wheel", omega=-ground_speed / wheel_radius,
+                                    center=c, axis=(0.0, 1.0, 0.0)))
+    return out
+
+
+def _tri_dist2(p, a, b, c):
+    """Squared distance of points p (P, 3) to triangles a, b, c (T, 3): the foot of the
+    perpendicular when it falls inside the triangle, else the nearest point of its edges."""
+    p = p[:, None, :]
+    a, b, c = a[None], b[None], c[None]
+    ab, bc, ca = b - a, c - b, a - c
+    n = torch.linalg.cross(ab, -ca)
+    nn = (n * n).sum(-1)
+
+    def inner(e, v0):
+        return (n * torch.linalg.cross(e.expand_as(p - v0), p - v0)).sum(-1) >= 0
+
+    inside = inner(ab, a) & inner(bc, b) & inner(ca, c) & (nn > 1e-300)
+    pd = (n * (p - a)).sum(-1)
+    plane = pd * pd / nn.clamp_min(1e-300)
+
+    def seg(v0, e):
+        w = p - v0
+        t = ((w * e).sum(-1) / (e * e).sum(-1).clamp_min(1e-300)).clamp(0.0, 1.0)
+        r = w - t[..., None] * e
+        return (r * r).sum(-1)
+
+    edge = torch.minimum(torch.minimum(seg(a, ab), seg(b, bc)), seg(c, ca))
+    return torch.where(inside, plane, edge)
+
+
+def _solid_angles(p, a, b, c):
+    """Signed solid angle of each triangle seen from each point (van Oosterom & Strackee)."""
+    ra, rb, rc = a[None] - p[:, None], b[None] - p[:, None], c[None] - p[:, None]
+    la, lb, lc = (torch.sqrt((r * r).sum(-1)) for r in (ra, rb, rc))
+    det = (ra * torch.linalg.cross(rb, rc, dim=-1)).sum(-1)
+    den = la * lb * lc + (ra * rb).sum(-1) * lc + (rb * rc).sum(-1) * la + (rc * ra).sum(-1) * lb
+    return 2.0 * torch.atan2(det, den)
+
+
+def mesh_sdf(vertices: np.ndarray, faces: np.ndarray):
+    """Signed distance of a triangle soup (negative where the generalised winding number
+    exceeds 1/2), brute force over triangles in float64 (set-up only)."""
+    v = torch.as_tensor(np.asarray(vertices, dtype=np.float64))
+    f = torch.as_tensor(np.asarray(faces, dtype=np.int64))
+    a, b, c = v[f[:, 0]], v[f[:, 1]], v[f[:, 2]]
+
+    def sdf(x, y, z):
+        shape = x.shape
+        pts = torch.stack([x.reshape(-1), y.reshape(-1), z.reshape(-1)], 1).to(torch.float64)
+        out = torch.empty(pts.shape[0], dtype=torch.float64)
+        step = max(1, (1 << 21) // max(1, f.shape[0]))
+        for s in range(0, pts.shape[0], step):
+            p = pts[s:s + step]
+            d2 = _tri_dist2(p, a, b, c).amin(1)
+            wn = _solid_angles(p, a, b, c).sum(1)
+            out[s:s + step] = torch.where(wn.abs() > 2.0 * math.pi, -1.0, 1.0) * torch.sqrt(d2)
+        return out.reshape(shape)
+    return sdf
+
+
+def body_sdf(body):
+    if isinstance(body, AnalyticBody):
+        return body.sdf
+    if isinstance(body, ImmersedMesh):
+        return mesh_sdf(body.vertices, body.faces)
+    raise TypeError(f"unsupported body {type(body).__name__}")
+
+
+# ---------------------------------------------------------------------------
+# geometry
+# ---------------------------------------------------------------------------
+
+class Geometry3D(NamedTuple):
+    """Window arrays of all bodies (reference ``ns3d/geometry.py:275-303``).
+
+    ``window = (k0, k1, j0, j1, i0, i1)`` is a block of cells. u-face arrays cover layers
+    k0..k1-1, rows j0..j1-1, faces i0..i1 (wz, wy, wx+1); v-face arrays (wz, wy+1, wx);
+    w-face arrays (wz+1, wy, wx); chi_c (wz, wy, wx). ``k_u`` (nb, 3, ...) maps u increments
+    to [Fx, My, Mz] parts, ``k_v`` v increments to [Fy, Mx, Mz], ``k_w`` w increments to
+    [Fz, Mx, My] (lever arms about the moment reference).
+    """
+    window: tuple
+    chi_u: torch.Tensor
+    chi_v: torch.Tensor
+    chi_w: torch.Tensor
+    chi_c: torch.Tensor
+    a_u: torch.Tensor
+    a_v: torch.Tensor
+    a_w: torch.Tensor
+    k_u: torch.Tensor
+    k_v: torch.Tensor
+    k_w: torch.Tensor
+    nbody: int
+
+    @property
+    def empty(self) -> bool:
+        k0, k1, j0, j1, i0, i1 = self.window
+        return k1 <= k0 or j1 <= j0 or i1 <= i0
+
+    def to(self, device=None, dtype=None) -> "Geometry3D":
+        def cv(t):
+            return t.to(device=device, dtype=dtype).contiguous()
+        return Geometry3D(self.window, *(cv(t) for t in self[1:11]), self.nbody)
+
+
+def rotation_matrix(pitch_deg: float, yaw_deg: float) -> np.ndarray:
+    """Attitude: yaw about +z by -yaw, then pitch about +y by +pitch (reference
+    ``ns3d/geometry.py:315-320``)."""
+    a, b = math.radians(pitch_deg), math.radians(-yaw_deg)
+    ry = np.array([[math.cos(a), 0.0, math.sin(a)], [0.0, 1.0, 0.0], [-math.sin(a), 0.0, math.cos(a)]])
+    rz = np.array([[math.cos(b), -math.sin(b), 0.0], [math.sin(b), math.cos(b), 0.0], [0.0, 0.0, 1.0]])
+    return ry @ rz
+
+
+def body_window3d(vertex_sets, ops: Ops3D, margin_cells: float | None = None, *, pitch_deg: float = 0.0,
+                  yaw_deg: float = 0.0) -> tuple:
+    """Cell window covering every vertex plus a margin of mask_width + 3 + |shift| cells,
+    clipped to the domain (reference ``ns3d/geometry.py:323-356``)."""
+    sets = [np.asarray(v, dtype=np.float64).reshape(-1, 3) for v in vertex_sets if len(v)]
+    if not sets:
+        return (0, 0, 0, 0, 0, 0)
+    pts = np.vstack(sets)
+    if pitch_deg > 0 or yaw_deg > 0:
+        ref = np.asarray(ops.moment_ref, dtype=np.float64)
+        rot = [(pts - ref) @ rotation_matrix(pa, ya).T + ref
+               for pa in np.linspace(-pitch_deg, pitch_deg, 7 if pitch_deg > 0 else 1)
+               for ya in np.linspace(-yaw_deg, yaw_deg, 7 if yaw_deg > 0 else 1)]
+        pts = np.vstack(rot)
+    margin = (ops.mask_width + 3.0 + abs(ops.mask_shift)) if margin_cells is None else margin_cells
+    h = ops.h
+    lo, hi = pts.min(axis=0), pts.max(axis=0)
+    org = (ops.x0, ops.y0, ops.z0)
+    n = (ops.nx, ops.ny, ops.nz)
+    box = []
+    for a in (2, 1, 0):
+        c0 = max(math.floor((lo[a] - org[a]) / h - margin), 0)
+        c1 = min(math.ceil((hi[a] - org[a]) / h + margin), n[a])
+        if c1 <= c0:
+            return (0, 0, 0, 0, 0, 0)
+        box += [c0, c1]
+    return tuple(box)
+
+
+def empty_geometry(nb: int, device=None, dtype=torch.float64) -> Geometry3D:
+    z = torch.zeros(0, 0, 0, device=device, dtype=dtype)
+    k = torch.zeros(0, 3, 0, 0, 0, device=device, dtype=dtype)
+    return Geometry3D((0, 0, 0, 0, 0, 0), z, z, z, z, z, z, z, k, k, k, nb)
+
+
+def build_geometry3d(bodies, ops: Ops3D, *, window=None) -> Geometry3D:
+    """Masks, solid velocities and force weights on the window, built in float64 on the host
+    and returned in the solver's dtype on its device (semantics of the reference's
+    ``ns3d/geometry.py:385-508``: per-body fraction ``smooth_step(d + 0.32 h, mask_width h)``
+    on each staggered grid, zero on domain-boundary faces, union clamped at 1; rigid rotation
+    pre-multiplied by each body's fraction; each body's share of a face where bodies overlap)."""
+    nb = len(bodies)
+    if window is None:
+        window = body_window3d([b.vertices_numpy() for b in bodies], ops,
+                               pitch_deg=ops.cfg.window_pitch_deg, yaw_deg=ops.cfg.window_yaw_deg)
+    k0, k1, j0, j1, i0, i1 = window
+    wz, wy, wx = k1 - k0, j1 - j0, i1 - i0
+    if nb == 0 or wz <= 0 or wy <= 0 or wx <= 0:
+        return empty_geometry(nb, ops.device, ops.dtype)
+    f64 = torch.float64
+    h = ops.h
+    w = ops.mask_width * h
+    shift = ops.mask_shift * h
+    X0, Y0, Z0 = ops.x0 + i0 * h, ops.y0 + j0 * h, ops.z0 + k0 * h
+
+    def axis(n, start):
+        return start + torch.arange(n, dtype=f64) * h
+
+    # sample positions of each staggered grid: (z, y, x) 1D coordinate vectors
+    grids = {
+        "u": (axis(wz, Z0 + 0.5 * h), axis(wy, Y0 + 0.5 * h), axis(wx + 1, X0)),
+        "v": (axis(wz, Z0 + 0.5 * h), axis(wy + 1, Y0), axis(wx, X0 + 0.5 * h)),
+        "w": (axis(wz + 1, Z0), axis(wy, Y0 + 0.5 * h), axis(wx, X0 + 0.5 * h)),
+        "c": (axis(wz, Z0 + 0.5 * h), axis(wy, Y0 + 0.5 * h), axis(wx, X0 + 0.5 * h)),
+    }
+    chis = {k: [] for k in grids}
+    for b in bodies:
+        sdf = body_sdf(b)
+        for key, (zs, ys, xs) in grids.items():
+            zz, yy, xx = torch.meshgrid(zs, ys, xs, indexing="ij")
+            chis[key].append(smooth_step(sdf(xx, yy, zz) + shift, w))
+    cu, cv, cw, cc = (torch.stack(chis[k]) for k in ("u", "v", "w", "c"))
+    px, py, pz = ops.periodic_axes
+    if not px:
+        if i0 == 0:
+            cu[..., 0] = 0.0
+        if i1 == ops.nx:
+            cu[..., -1] = 0.0
+    if not py:
+        if j0 == 0:
+            cv[:, :, 0] = 0.0
+        if j1 == ops.ny:
+            cv[:, :, -1] = 0.0
+    if not pz:
+        if k0 == 0:
+            cw[:, 0] = 0.0
+        if k1 == ops.nz:
+            cw[:, -1] = 0.0
+    chi_u, chi_v, chi_w, chi_c = (c.sum(0).clamp(max=1.0) for c in (cu, cv, cw, cc))
+
+    zu, yu = grids["u"][0][:, None, None], grids["u"][1][None, :, None]
+    zv, xv = grids["v"][0][:, None, None], grids["v"][2][None, None, :]
+    yw, xw = grids["w"][1][None, :, None], grids["w"][2][None, None, :]
+    a_u, a_v, a_w = torch.zeros_like(chi_u), torch.zeros_like(chi_v), torch.zeros_like(chi_w)
+    for i, b in enumerate(bodies):
+        om = b.omega
+        if om == 0.0:
+            continue
+        (cx, cy, cz), (ax, ay, az) = b.center, b.axis
+        a_u = a_u + cu[i] * (om * (ay * (zu - cz) - az * (yu - cy)))
+        a_v = a_v + cv[i] * (om * (az * (xv - cx) - ax * (zv - cz)))
+        a_w = a_w + cw[i] * (om * (ax * (yw - cy) - ay * (xw - cx)))
+    if nb == 1:
+        wu, wv, ww = torch.ones_like(cu), torch.ones_like(cv), torch.ones_like(cw)
+    else:
+        wu = cu / (cu.sum(0, keepdim=True) + 1e-30)
+        wv = cv / (cv.sum(0, keepdim=True) + 1e-30)
+        ww = cw / (cw.sum(0, keepdim=True) + 1e-30)
+    xr, yr, zr = ops.moment_ref
+    # M = r x F: Fx at u-faces gives My += (z - zr) Fx, Mz -= (y - yr) Fx; Fy at v-faces gives
+    # Mx -= (z - zr) Fy, Mz += (x - xr) Fy; Fz at w-faces gives Mx += (y - yr) Fz, My -= (x - xr) Fz
+    k_u = torch.stack([wu, wu * (zu - zr), -wu * (yu - yr)], dim=1)
+    k_v = torch.stack([wv, -wv * (zv - zr), wv * (xv - xr)], dim=1)
+    k_w = torch.stack([ww, ww * (yw - yr), -ww * (xw - xr)], dim=1)
+    g = Geometry3D(tuple(int(c) for c in window), chi_u, chi_v, chi_w, chi_c, a_u, a_v, a_w, k_u, k_v, k_w, nb)
+    return g.to(ops.device, ops.dtype)

@@ -1,0 +1,286 @@
+"""CPU checks of the 3D oracle and set-up code against the reference's golden vectors and
+against what the mathematics fixes (closed forms, invariants, brute force)."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import ns3d as O
+from tests import cases
+
+
+def _setup(name, dtype="float64"):
+    from aerolume_amd.ns3d import setup as su
+    cfg = cases.cfg3d(name, dtype)
+    ops = su.make_operators3d(cfg, device="cpu")
+    return cfg, ops, su.build_geometry3d(cfg.bodies, ops)
+
+
+def _state(u, v, w, ops):
+    z = torch.zeros((), dtype=ops.dtype)
+    t = lambda a: torch.as_tensor(a, dtype=ops.dtype)  # noqa: E731
+    return O.State3D(t(u), t(v), t(w), torch.zeros(ops.nz, ops.ny, ops.nx, dtype=ops.dtype), z.clone(), z.clone())
+
+
+GEOM = ("chi_u", "chi_v", "chi_w", "chi_c", "a_u", "a_v", "a_w", "k_u", "k_v", "k_w")
+
+
+# ----------------------------------------------------------------- reference trajectories
+@pytest.mark.parametrize("name", list(cases.CASES3D))
+def test_steps_match_reference(name, golden):
+    """Five float64 steps from the reference's seeded start: fields, dt, t and the six
+    coefficients of every body agree with the reference to round-off."""
+    fx = golden(f"ns3d_{name}_f64.npz")
+    cfg, ops, g = _setup(name)
+    if not g.empty:
+        for k in GEOM:
+            assert np.abs(getattr(g, k).numpy() - fx["g_" + k]).max() < 1e-13, k
+    st = _state(fx["u0"], fx["v0"], fx["w0"], ops)
+    for n in range(1, 6):
+        st, c = O.step3d_full(st, g, ops)
+        assert abs(float(st.dt) - fx["dt"][n - 1]) <= 1e-14 * fx["dt"][n - 1]
+        if c.numel():
+            assert np.abs(c.numpy() - fx["coeffs"][n - 1]).max() < 1e-10
+        if n in (1, 5):
+            for f in "uvwp":
+                ref = fx[f"{f}_{n}"]
+                assert np.abs(getattr(st, f).numpy() - ref).max() < 1e-11 * max(1.0, np.abs(ref).max()), (f, n)
+    assert abs(float(st.t) - fx["t"][-1]) < 1e-14
+
+
+def test_car_miniature_float32_matches_reference(golden):
+    """The car miniature (WALE, moving floor, periodic y, five bodies) in float32, 20 steps."""
+    fx = golden("ns3d_wale_moving_py_f32.npz")
+    cfg, ops, g = _setup("wale_moving_py", "float32")
+    st = _state(fx["u0"], fx["v0"], fx["w0"], ops)
+    for n in range(20):
+        st, c = O.step3d_full(st, g, ops)
+    for f in "uvwp":
+        assert np.abs(getattr(st, f).numpy() - fx[f"{f}_20"]).max() < 2e-5, f
+    assert np.abs(c.numpy() - fx["coeffs"][-1]).max() < 1e-3
+
+
+def test_mask_builder_matches_reference_on_meshes(golden):
+    """The reference's build_geometry3d on two overlapping triangulated boxes (one spinning)
+    against this package's builder fed the same meshes, and fed the exact box SDFs."""
+    from aerolume_amd.ns3d import setup as su
+    from aerolume_amd.ns3d.config import AnalyticBody, ImmersedMesh
+    from tests.golden.make_ns3d_fixtures import box_mesh
+    fx = golden("ns3d_masks.npz")
+    cfg = cases.cfg3d("laminar")
+    ops = su.make_operators3d(cfg, device="cpu")
+    for kind in ("mesh", "analytic"):
+        bodies = []
+        for i in range(2):
+            c, s = tuple(fx[f"center_{i}"]), tuple(fx[f"size_{i}"])
+            rot = dict(omega=float(fx[f"omega_{i}"]), center=tuple(fx[f"rot_center_{i}"]))
+            if kind == "mesh":
+                bodies.append(ImmersedMesh(*box_mesh(c, s), **rot))
+            else:
+                lo = tuple(a - b / 2 for a, b in zip(c, s))
+                hi = tuple(a + b / 2 for a, b in zip(c, s))
+                bodies.append(AnalyticBody(su.rounded_box_sdf(c, s, 0.0), (lo, hi), **rot))
+        g = su.build_geometry3d(bodies, ops)
+        assert tuple(g.window) == tuple(fx["window"])
+        for k in GEOM:
+            assert np.abs(getattr(g, k).numpy() - fx[k]).max() < 1e-14, (kind, k)
+
+
+# ----------------------------------------------------------------- Poisson / projection
+@pytest.mark.parametrize("shape", [(6, 8, 10), (5, 9, 12), (8, 8, 8)])
+@pytest.mark.parametrize("periodic", [(False, False, False), (False, True, False), (True, True, True)])
+def test_poisson_exact_residual(shape, periodic):
+    """L psi = f - mean(f) to 1e-10 relative with zero mean, for every periodicity mix."""
+    nz, ny, nx = shape
+    f = torch.as_tensor(np.random.default_rng(7).standard_normal(shape))
+    ps = O.PoissonSolver3D(nx, ny, nz, 0.25, periodic=periodic, dtype=torch.float64)
+    psi = ps.solve(f)
+    r = ps.laplacian(psi) - (f - f.mean())
+    assert float(r.abs().max()) < 1e-10 * float(f.abs().max())
+    assert abs(float(psi.mean())) < 1e-12
+
+
+def test_poisson_matches_dense_least_squares():
+    """Brute force: the dense 7-point operator (Neumann x, z; periodic y) by least squares."""
+    nx, ny, nz, h = 4, 5, 3, 0.5
+    n = nx * ny * nz
+    L = np.zeros((n, n))
+    idx = lambda k, j, i: (k * ny + j) * nx + i  # noqa: E731
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                c = idx(k, j, i)
+                for dk, dj, di in ((0, 0, 1), (0, 0, -1), (0, 1, 0), (0, -1, 0), (1, 0, 0), (-1, 0, 0)):
+                    kk, jj, ii = k + dk, (j + dj) % ny, i + di
+                    if 0 <= kk < nz and 0 <= ii < nx:
+                        L[c, idx(kk, jj, ii)] += 1 / h**2
+                        L[c, c] -= 1 / h**2
+    f = np.random.default_rng(9).standard_normal(n)
+    f -= f.mean()
+    ref = np.linalg.lstsq(L, f, rcond=None)[0]
+    psi = O.PoissonSolver3D(nx, ny, nz, h, periodic=(False, True, False), dtype=torch.float64).solve(
+        torch.as_tensor(f.reshape(nz, ny, nx)))
+    assert np.abs(psi.numpy().ravel() - (ref - ref.mean())).max() < 1e-10
+
+
+def test_bases_match_scipy_orthonormal_dct():
+    """The Neumann basis is scipy's orthonormal DCT-II; the periodic one is the DHT / sqrt(n)."""
+    from scipy.fft import dct, fft
+    x = np.random.default_rng(3).standard_normal(12)
+    q, _ = O.neumann_basis(12)
+    assert np.abs(q.numpy() @ x - dct(x, type=2, norm="ortho")).max() < 1e-12
+    q, _ = O.periodic_basis(12)
+    F = fft(x)
+    assert np.abs(q.numpy() @ x - (F.real - F.imag) / math.sqrt(12)).max() < 1e-12
+
+
+@pytest.mark.parametrize("name", ["laminar", "wale_moving_py", "closed_slip", "periodic_wale"])
+def test_projection_of_noise_is_divergence_free(name):
+    cfg, ops, _ = _setup(name)
+    u, v, w = O.noisy_start3d(ops, 11, amplitude=0.3)
+    assert float(O.divergence3d(u, v, w, ops.h).abs().max()) * ops.h < 1e-10
+
+
+def test_inlet_outlet_balances_mass():
+    """Inlet plane held at the profile, outlet shifted so sum(out) == sum(in)."""
+    cfg, ops, _ = _setup("laminar")
+    u = torch.as_tensor(np.random.default_rng(1).standard_normal((ops.nz, ops.ny, ops.nx + 1)))
+    u2 = O.inlet_outlet(u, ops)
+    assert torch.equal(u2[..., 0], ops.u_inlet[..., 0].to(u2.dtype))
+    assert abs(float(u2[..., -1].sum() - u2[..., 0].sum())) < 1e-12
+    assert torch.equal(u2[..., 1:-1], u[..., 1:-1])
+
+
+def test_ghost_rules_give_wall_values():
+    """Padded arrays: no-slip walls average to zero, the moving floor to the ground speed,
+    slip walls have zero gradient, periodic copies wrap."""
+    cfg, ops, _ = _setup("smag_central_noslip")
+    rng = np.random.default_rng(2)
+    u = torch.as_tensor(rng.standard_normal((ops.nz, ops.ny, ops.nx + 1)))
+    U = O.pad3(u, O.ghost_rules(ops)["u"])
+    assert float((U[1, 2:-2, 2:-2] + U[2, 2:-2, 2:-2]).abs().max()) < 1e-15       # z = z0 no-slip
+    assert float((U[2:-2, 1, 2:-2] + U[2:-2, 2, 2:-2]).abs().max()) < 1e-15       # y = y0 no-slip
+    cfg, ops, _ = _setup("wale_moving_py")
+    U = O.pad3(u, O.ghost_rules(ops)["u"])
+    assert float((0.5 * (U[1, 2:-2, 2:-2] + U[2, 2:-2, 2:-2]) - 1.0).abs().max()) < 1e-14
+    assert torch.equal(U[2:-2, 0:2, 2:-2], U[2:-2, ops.ny:ops.ny + 2, 2:-2])      # periodic y
+    cfg, ops, _ = _setup("laminar")
+    U = O.pad3(u, O.ghost_rules(ops)["u"])
+    assert torch.equal(U[1, 2:-2, 2:-2], U[2, 2:-2, 2:-2])                        # slip floor
+
+
+# ----------------------------------------------------------------- closed forms
+def _taylor_green(ops, amp=1.0):
+    """u = sin x cos y cos z, v = -cos x sin y cos z, w = 0 on the [0, 2 pi)^3 periodic box."""
+    n, h = ops.nx, ops.h
+    xf = torch.arange(n + 1, dtype=torch.float64) * h
+    xc = (torch.arange(n, dtype=torch.float64) + 0.5) * h
+    u = amp * torch.sin(xf)[None, None, :] * torch.cos(xc)[None, :, None] * torch.cos(xc)[:, None, None]
+    v = -amp * torch.cos(xc)[None, None, :] * torch.sin(xf)[None, :, None] * torch.cos(xc)[:, None, None]
+    w = torch.zeros(n + 1, n, n, dtype=torch.float64)
+    return u, v, w
+
+
+@pytest.mark.parametrize("n", [16, 32])
+def test_taylor_green_decay_second_order(n):
+    """Central scheme, periodic box, Re 100: kinetic energy decays as exp(-6 t / Re) to
+    leading order at small amplitude; the error shrinks about 4x per grid doubling."""
+    from aerolume_amd.ns3d.config import SolverConfig3D
+    from aerolume_amd.ns3d import setup as su
+    errs = []
+    for m in (n, 2 * n):
+        cfg = SolverConfig3D(nx=m, ny=m, nz=m, h=2 * math.pi / m, x0=0, y0=0, z0=0, re=100.0, periodic=True,
+                             advection="central", dtype="float64", device="cpu", dye=False, dt=0.02)
+        ops = su.make_operators3d(cfg, device="cpu")
+        u, v, w = _taylor_green(ops, 1e-3)
+        st = O.State3D(u, v, w, torch.zeros(m, m, m, dtype=torch.float64), torch.zeros((), dtype=torch.float64),
+                       torch.zeros((), dtype=torch.float64))
+        g = su.empty_geometry(0, dtype=torch.float64)
+        e0 = float((u[..., :-1] ** 2).sum() + (v[:, :-1] ** 2).sum())
+        for _ in range(10):
+            st, _ = O.step3d_full(st, g, ops, dt=0.02)
+        e1 = float((st.u[..., :-1] ** 2).sum() + (st.v[:, :-1] ** 2).sum() + (st.w[:-1] ** 2).sum())
+        exact = math.exp(-2 * 3 * 0.2 / 100.0)
+        errs.append(abs(e1 / e0 - exact))
+    if n == 16:
+        assert errs[1] < errs[0] / 3.0
+    assert errs[-1] < 2e-4
+
+
+def test_les_closed_forms_and_special_cases():
+    """Random gradients: Smagorinsky, WALE and Vreman against their textbook formulas in
+    float64 numpy; pure shear gives zero for WALE and Vreman and > 0 for Smagorinsky."""
+    rng = np.random.default_rng(5)
+    G = rng.standard_normal((3, 3, 7))
+    G[2, 2] = -(G[0, 0] + G[1, 1])
+    ops = type("o", (), {"h": 0.1, "les_c": 0.3})()
+    g = tuple(tuple(torch.as_tensor(G[i, j]) for j in range(3)) for i in range(3))
+    d2 = 0.01
+    for model in ("smagorinsky", "wale", "vreman"):
+        ops.turbulence = model
+        nut = O.eddy_viscosity3d(g, ops).numpy()
+        for p in range(7):
+            a = G[:, :, p]
+            S = 0.5 * (a + a.T)
+            if model == "smagorinsky":
+                ref = 0.09 * d2 * math.sqrt(2 * (S * S).sum() + 1e-30)
+            elif model == "wale":
+                a2 = a @ a
+                Sd = 0.5 * (a2 + a2.T) - np.eye(3) * np.trace(a2) / 3
+                sd = (Sd * Sd).sum()
+                ref = 0.09 * d2 * sd ** 1.5 / ((S * S).sum() ** 2.5 + sd ** 1.25 + 1e-30)
+            else:
+                B = a @ a.T          # beta_ij = sum_m g_im g_jm with g_ij = du_i / dx_j
+                bb = B[0, 0] * B[1, 1] - B[0, 1] ** 2 + B[0, 0] * B[2, 2] - B[0, 2] ** 2 + B[1, 1] * B[2, 2] - B[1, 2] ** 2
+                ref = 0.3 * d2 * math.sqrt(max(bb, 0) / ((a * a).sum() + 1e-30) + 1e-30)
+            assert abs(nut[p] - ref) < 1e-12 * max(1.0, abs(ref)), model
+    z = torch.zeros(1, dtype=torch.float64)
+    shear = ((z, z + 2.0, z), (z, z, z), (z, z, z))
+    for model, positive in (("smagorinsky", True), ("wale", False), ("vreman", False)):
+        ops.turbulence = model
+        nut = float(O.eddy_viscosity3d(shear, ops)[0])
+        assert (nut > 1e-6) if positive else (abs(nut) < 1e-12), model
+
+
+def test_solid_rotation_velocity_and_force_weights():
+    """A spinning wheel's solid velocity is omega x (x - c) on its support, and the moment
+    weights transfer: k[1] = (z - zr) k[0] on u-faces (lever arm of Fx about y)."""
+    cfg, ops, g = _setup("laminar")
+    k0, k1, j0, j1, i0, i1 = g.window
+    h = ops.h
+    z = ops.z0 + (torch.arange(k0, k1, dtype=torch.float64) + 0.5) * h
+    wheel = cfg.bodies[1]
+    # a_u = chi_wheel * omega * (ay (z - cz) - az (y - cy)) with axis +y
+    inside = g.chi_u > 0.999
+    expect = wheel.omega * (z[:, None, None] - wheel.center[2]).expand_as(g.chi_u)
+    sel = inside & (g.k_u[1, 0] > 0.999)
+    assert int(sel.sum()) > 0
+    # the wheel's own fraction there is within 1e-3 of 1
+    assert float((g.a_u[sel] - expect[sel]).abs().max()) < 1e-3 * float(expect[sel].abs().max())
+    zr = ops.moment_ref[2]
+    lever = (z[:, None, None] - zr).expand_as(g.chi_u)
+    assert float((g.k_u[:, 1] - g.k_u[:, 0] * lever).abs().max()) < 1e-14
+
+
+def test_symmetry_plane_doubles_forces():
+    """With a symmetry plane, the reported CD is twice the half-body's and CS = 0."""
+    cfg, ops, g = _setup("vreman_symmetry")
+    acc = tuple(torch.ones_like(t) for t in (g.chi_u, g.chi_v, g.chi_w))
+    c = O.force_coefficients(acc, g, ops, torch.tensor(0.1, dtype=torch.float64))
+    ops.symmetry = False
+    c1 = O.force_coefficients(acc, g, ops, torch.tensor(0.1, dtype=torch.float64))
+    assert abs(float(c[0, 0]) - 2 * float(c1[0, 0])) < 1e-12
+    assert float(c[0, 1]) == 0.0
+
+
+def test_planted_stage_weight_mistake_is_caught(golden, monkeypatch):
+    """A plausible mistake planted in the oracle (swapped RK stage weights) fails the
+    reference comparison, so the comparison has teeth."""
+    fx = golden("ns3d_laminar_f64.npz")
+    cfg, ops, g = _setup("laminar")
+    monkeypatch.setattr(O, "SSP_RK3", ((0.0, 1.0), (0.25, 0.75), (1.0 / 3.0, 2.0 / 3.0)))
+    st, _ = O.step3d_full(_state(fx["u0"], fx["v0"], fx["w0"], ops), g, ops)
+    assert np.abs(st.u.numpy() - fx["u_1"]).max() > 1e-8

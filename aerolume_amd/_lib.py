@@ -92,18 +92,10 @@ def load() -> ctypes.CDLL:
     for name in ("aero2d_plan_create", "aero2d_plan_destroy", "aero2d_step", "aero2d_project",
                  "aero2d_poisson_solve"):
         getattr(lib, name).restype = ctypes.c_int
-    _bind_optional(lib)
+    from . import _lib3d
+    _lib3d.bind(lib)
     _LIB = lib
     return lib
-
-
-def _bind_optional(lib) -> None:
-    """Entry points of later solver families (bound when the library has them)."""
-    try:
-        from . import _lib3d
-        _lib3d.bind(lib)
-    except (ImportError, AttributeError):
-        pass
 
 
 def check(rc: int, what: str) -> None:

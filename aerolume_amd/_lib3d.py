@@ -1,0 +1,55 @@
+"""ctypes structures and bindings of the 3D entry points of ``libaerolume_amd.so``
+(``aero3d_*`` in ``include/aerolume_amd.h``)."""
+from __future__ import annotations
+
+import ctypes
+
+i32 = ctypes.c_int32
+f64 = ctypes.c_double
+vp = ctypes.c_void_p
+
+MAX_BODIES = 8
+
+
+class Desc3D(ctypes.Structure):
+    _fields_ = [("nx", i32), ("ny", i32), ("nz", i32), ("dtype", i32), ("periodic_x", i32), ("periodic_y", i32),
+                ("periodic_z", i32), ("closed", i32), ("advection", i32), ("turbulence", i32), ("fixed_dt", i32),
+                ("win_k0", i32), ("win_k1", i32), ("win_j0", i32), ("win_j1", i32), ("win_i0", i32),
+                ("win_i1", i32), ("nbody", i32), ("symmetry", i32), ("ghost_kind", i32 * 18),
+                ("ghost_value", f64 * 18), ("h", f64), ("y0", f64), ("cfl", f64), ("les_c", f64),
+                ("ref_area", f64), ("length_ref", f64), ("span_ref", f64), ("moment_ref", f64 * 3)]
+
+
+class Fields3D(ctypes.Structure):
+    _fields_ = [(k, vp) for k in ("u", "v", "w", "p", "t", "dt")]
+
+
+class Geom3D(ctypes.Structure):
+    _fields_ = [(k, vp) for k in ("chi_u", "chi_v", "chi_w", "chi_c", "a_u", "a_v", "a_w", "k_u", "k_v", "k_w")]
+
+
+class Params3D(ctypes.Structure):
+    _fields_ = [(k, vp) for k in ("nu", "eta", "u_inlet", "dt_fixed")]
+
+
+class Outputs3D(ctypes.Structure):
+    _fields_ = [("coeffs", vp), ("hist", vp), ("hist_count", vp), ("hist_cap", ctypes.c_int64)]
+
+
+def bind(lib) -> None:
+    P = ctypes.c_void_p
+    lib.aero3d_plan_create.argtypes = [ctypes.POINTER(Desc3D), ctypes.POINTER(P)]
+    lib.aero3d_plan_destroy.argtypes = [P]
+    lib.aero3d_plan_workspace_bytes.argtypes = [P]
+    lib.aero3d_plan_workspace_bytes.restype = ctypes.c_size_t
+    lib.aero3d_step.argtypes = [P, P, ctypes.POINTER(Fields3D), ctypes.POINTER(Fields3D), ctypes.POINTER(Geom3D),
+                                ctypes.POINTER(Params3D), ctypes.POINTER(Outputs3D)]
+    lib.aero3d_step_chained.argtypes = lib.aero3d_step.argtypes
+    lib.aero3d_project.argtypes = [P, P, P, P, P, P]
+    lib.aero3d_poisson_solve.argtypes = [P, P, P, P]
+    lib.aero3d_time_stage.argtypes = [P, P, ctypes.POINTER(Fields3D), ctypes.POINTER(Geom3D),
+                                      ctypes.POINTER(Params3D), ctypes.c_int, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_double)]
+    for name in ("aero3d_plan_create", "aero3d_plan_destroy", "aero3d_step", "aero3d_step_chained",
+                 "aero3d_project", "aero3d_poisson_solve", "aero3d_time_stage"):
+        getattr(lib, name).restype = ctypes.c_int

@@ -47,6 +47,19 @@ struct FftPlan {
   int radix[kMaxRadices];
 };
 
+// Radix sequence of an n-point transform (4s first, then 2, 3, 5); false when n has
+// another prime factor.
+inline bool factor_radices(int n, FftPlan &pl) {
+  pl.n = n;
+  pl.nrad = 0;
+  int m = n;
+  while (m % 4 == 0 && pl.nrad < kMaxRadices) { pl.radix[pl.nrad++] = 4; m /= 4; }
+  while (m % 2 == 0 && pl.nrad < kMaxRadices) { pl.radix[pl.nrad++] = 2; m /= 2; }
+  while (m % 3 == 0 && pl.nrad < kMaxRadices) { pl.radix[pl.nrad++] = 3; m /= 3; }
+  while (m % 5 == 0 && pl.nrad < kMaxRadices) { pl.radix[pl.nrad++] = 5; m /= 5; }
+  return m == 1;
+}
+
 // Forward radix-R butterflies (sign -1).
 template <typename T>
 __device__ __forceinline__ void dft2(cplx<T> *v) {

@@ -1454,21 +1454,6 @@ __global__ void __launch_bounds__(1024) k_finalize(Dev<T> d, const T *__restrict
 using namespace aero;
 using namespace aero::ns2d;
 
-namespace {
-
-bool factor_radices(int n, FftPlan &pl) {
-  pl.n = n;
-  pl.nrad = 0;
-  int m = n;
-  while (m % 4 == 0 && pl.nrad < kMaxRadices) { pl.radix[pl.nrad++] = 4; m /= 4; }
-  while (m % 2 == 0 && pl.nrad < kMaxRadices) { pl.radix[pl.nrad++] = 2; m /= 2; }
-  while (m % 3 == 0 && pl.nrad < kMaxRadices) { pl.radix[pl.nrad++] = 3; m /= 3; }
-  while (m % 5 == 0 && pl.nrad < kMaxRadices) { pl.radix[pl.nrad++] = 5; m /= 5; }
-  return m == 1;
-}
-
-}  // namespace
-
 struct aero2d_plan {
   aero2d_desc desc;
   int dbl;

@@ -1,0 +1,1600 @@
+// MI355X (gfx950) kernels of the 3D incompressible Navier-Stokes step and its C ABI.
+//
+// One SSP-RK3 step (reference solvers/ns3d/core.py:753-883 without placed physics) is a
+// fixed sequence of launches on the caller's stream, with no host synchronisation and no
+// allocation:
+//
+//   [max, dt]   dt inputs from the state: max |u|, |v|, |w| and max nu_t over the cells
+//               -1..n (skipped in chained mode: the previous step's last two kernels left
+//               them) - core.py:637-646
+//   3 x stage   LDS tiles with a 2-cell halo: the previous stage's projection applied on
+//               the fly (u = u^ - grad psi, outlet mass balance), ghost rules per
+//               component and axis, eddy viscosity on the tile + 1 ring, upwind3 / central
+//               advection, 7-point diffusion, LES stress divergence, RK update,
+//               inlet / outlet, window penalisation with force increments (core.py:331-634)
+//       p1      divergence (outlet correction folded in) -> DCT / DHT along x, row pairs
+//       p2      DCT / DHT along y, column chunks
+//       p3      DCT / DHT along z, x 1/lambda (FP64-built), inverse along z
+//       p4      inverse along y
+//       p5      inverse along x -> psi; step-averaged pressure (core.py:874-875)
+//   final       u = u^ - grad psi written to the state; per-block partials of the next
+//               step's dt inputs and of the force sums
+//   finalize    one block: six coefficients per body, t, dt, next dt, history ring
+//
+// Arithmetic in the plan's dtype T (float or double); the Poisson eigenvalue inverse is
+// formed in double; the outlet and force sums accumulate in double. No atomics: every
+// global reduction goes through per-block partials summed in a fixed order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/aerolume_amd.h"
+#include "common.h"
+
+namespace aero {
+namespace ns3d {
+
+constexpr int kThreads = 256;
+constexpr int kFinThreads = 1024;
+constexpr int kMaxBodies = 8;
+constexpr int kMaxQ = 9 * kMaxBodies;      // force sums per body: u (Fx, My, Mz), v (Fy, Mx, Mz), w (Fz, Mx, My)
+constexpr double kSsp[3][2] = {{0.0, 1.0}, {0.75, 0.25}, {1.0 / 3.0, 2.0 / 3.0}};
+constexpr double kForceW[3] = {1.0 / 6.0, 2.0 / 3.0, 1.0};
+constexpr double kViscousDt = 1.0 / 6.0;   // ns3d/core.py:51
+
+// cells per stage tile (x, y, z); 256 threads per tile
+template <typename T>
+struct Tile;
+template <>
+struct Tile<float> {
+  static constexpr int X = 32, Y = 8, Z = 4;
+};
+template <>
+struct Tile<double> {
+  static constexpr int X = 16, Y = 8, Z = 4;
+};
+
+template <typename T>
+struct Red3 {
+  T dt;          // dt of the step being taken
+  T corr[3];     // outlet correction of each stage's u^
+};
+
+// Static description of a plan, resident in device memory.
+template <typename T>
+struct Dev3 {
+  int nx, ny, nz;
+  int per[3];
+  int closed, outlet;
+  int upwind, turb, fixed_dt;
+  int nbx, nby, nbz, nblk, xcd_chunk;
+  int win, wk0, wk1, wj0, wj1, wi0, wi1, wz, wy, wx;
+  int wbx0, wby0, wbz0, nwbx, nwby, nwbz, nwt;
+  int nbody, symmetry, nq;
+  int rule[18];
+  T rval2[18];
+  T h, inv_h, inv_h2, kadv, les_k, cfl_h, visc_h2, corr_scale;
+  T neg_h3, q_a, q_a_span, q_a_len, dy;
+  T *F, *psi, *acc_u, *acc_v, *acc_w, *pmax;
+  double *pout, *pforce;
+  Red3<T> *red;
+  const cplx<T> *tw[3];
+  const cplx<T> *cs[3];
+  const double *lam[3];
+  FftPlan fp[3];
+  int dct[3];
+  int ns1, ns2, ns3, ns5;
+};
+
+// Per-launch pointers (by value).
+template <typename T>
+struct Call3 {
+  const T *u0, *v0, *w0;     // step start
+  const T *up, *vp, *wp;     // stage input: the previous stage's u^ (with psi) or a velocity
+  const T *psi;              // previous stage's psi; null: up is used as it is
+  T *uo, *vo, *wo;           // outputs
+  const T *chi_u, *chi_v, *chi_w, *chi_c, *a_u, *a_v, *a_w, *k_u, *k_v, *k_w;
+  const T *nu, *eta, *u_inlet, *dt_fixed;
+  int stage;
+  int corr_stage;            // red->corr slot added at the outlet face of up (-1: none)
+};
+
+template <typename T>
+struct Lds3 {
+  static constexpr int TX = Tile<T>::X, TY = Tile<T>::Y, TZ = Tile<T>::Z;
+  static constexpr int UX = TX + 5, UY = TY + 4, UZ = TZ + 4;
+  static constexpr int VX = TX + 4, VY = TY + 5, VZ = TZ + 4;
+  static constexpr int WX = TX + 4, WY = TY + 4, WZ = TZ + 5;
+  static constexpr int NX = TX + 2, NY = TY + 2, NZ = TZ + 2;
+  T u[UZ][UY][UX];
+  T v[VZ][VY][VX];
+  T w[WZ][WY][WX];
+  T nu[NZ][NY][NX];
+  double red[4 * 16];
+};
+
+// local (tile-relative) accessors: faces / cells from -2 (halo) on
+#define LU(k, j, i) L.u[(k) + 2][(j) + 2][(i) + 2]
+#define LV(k, j, i) L.v[(k) + 2][(j) + 2][(i) + 2]
+#define LW(k, j, i) L.w[(k) + 2][(j) + 2][(i) + 2]
+#define LN(k, j, i) L.nu[(k) + 1][(j) + 1][(i) + 1]
+
+template <typename T>
+__device__ __forceinline__ T step_dt(const Dev3<T> &d, const Call3<T> &c) {
+  return d.fixed_dt ? *c.dt_fixed : d.red->dt;
+}
+
+// Tile of block b, consecutive tiles of a chunk on one XCD (blocks are dealt to the 8 XCDs
+// round-robin, so the chunk's halos are shared through that XCD's L2).
+template <typename T>
+__device__ __forceinline__ void tile_of(const Dev3<T> &d, int &bx, int &by, int &bz, int &tile) {
+  const int b = blockIdx.x;
+  tile = d.xcd_chunk ? (b & 7) * d.xcd_chunk + (b >> 3) : b;
+  bx = tile % d.nbx;
+  const int t = tile / d.nbx;
+  by = t % d.nby;
+  bz = t / d.nby;
+}
+
+// ------------------------------------------------------------------ ghost rules
+// One axis of a ghost rule (ns3d/core.py:223-272): index g (outside 0..n-1) of an axis of
+// extent n -> in-range source; op 0 copy, 1 negate, 2 affine (2 value - x).
+__device__ __forceinline__ int ghost_axis(int code, int g, int n, int &op) {
+  const bool lo = g < 0;
+  switch (code) {
+    case 0: op = 0; return lo ? 0 : n - 1;                       // edge
+    case 1: op = 0; return lo ? -1 - g : 2 * n - 1 - g;          // even
+    case 2: op = 1; return lo ? -1 - g : 2 * n - 1 - g;          // odd
+    case 3: op = 2; return lo ? -1 - g : 2 * n - 1 - g;          // affine
+    case 4: op = 1; return lo ? -g : 2 * n - 2 - g;              // face_odd
+    case 5: op = 0; return lo ? g + n : g - n;                   // periodic_cell
+    default: op = 0; return lo ? g + n - 1 : g - n + 1;          // periodic_face
+  }
+}
+
+// Stage-input velocity of component comp at an in-range position: u^ - grad psi with the
+// outlet shift (the previous stage's projected field, core.py:281-297, 582-612).
+template <typename T>
+__device__ __forceinline__ T load_up(const Dev3<T> &d, const Call3<T> &c, int comp, int k, int j, int i, T corr) {
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  if (comp == 0) {
+    T x = c.up[((size_t)k * ny + j) * (nx + 1) + i];
+    if (c.psi) {
+      const T *ps = c.psi + ((size_t)k * ny + j) * nx;
+      if (d.per[0]) {
+        const int ii = i == nx ? 0 : i, im = ii == 0 ? nx - 1 : ii - 1;
+        x = x - (ps[ii] - ps[im]) * d.inv_h;
+      } else if (i > 0 && i < nx) {
+        x = x - (ps[i] - ps[i - 1]) * d.inv_h;
+      }
+    }
+    if (c.corr_stage >= 0 && d.outlet && i == nx) x = x + corr;
+    return x;
+  }
+  if (comp == 1) {
+    T x = c.vp[((size_t)k * (ny + 1) + j) * nx + i];
+    if (c.psi) {
+      const T *ps = c.psi + (size_t)k * ny * nx + i;
+      if (d.per[1]) {
+        const int jj = j == ny ? 0 : j, jm = jj == 0 ? ny - 1 : jj - 1;
+        x = x - (ps[(size_t)jj * nx] - ps[(size_t)jm * nx]) * d.inv_h;
+      } else if (j > 0 && j < ny) {
+        x = x - (ps[(size_t)j * nx] - ps[(size_t)(j - 1) * nx]) * d.inv_h;
+      }
+    }
+    return x;
+  }
+  T x = c.wp[((size_t)k * ny + j) * nx + i];
+  if (c.psi) {
+    const size_t pl = (size_t)ny * nx;
+    const T *ps = c.psi + (size_t)j * nx + i;
+    if (d.per[2]) {
+      const int kk = k == nz ? 0 : k, km = kk == 0 ? nz - 1 : kk - 1;
+      x = x - (ps[kk * pl] - ps[km * pl]) * d.inv_h;
+    } else if (k > 0 && k < nz) {
+      x = x - (ps[k * pl] - ps[(k - 1) * pl]) * d.inv_h;
+    }
+  }
+  return x;
+}
+
+// Ghost-padded value (pad3: the x rule, then y on the x-padded array, then z).
+template <typename T>
+__device__ T padded(const Dev3<T> &d, const Call3<T> &c, int comp, int gk, int gj, int gi, T corr) {
+  const int ex = d.nx + (comp == 0), ey = d.ny + (comp == 1), ez = d.nz + (comp == 2);
+  gi = gi > ex + 1 ? ex + 1 : gi;   // beyond the two ghost layers: never used, kept in bounds
+  gj = gj > ey + 1 ? ey + 1 : gj;
+  gk = gk > ez + 1 ? ez + 1 : gk;
+  const int *rule = d.rule + comp * 6;
+  const T *rv = d.rval2 + comp * 6;
+  int ox = 0, oy = 0, oz = 0, sx = 0, sy = 0, sz = 0;
+  if (gi < 0 || gi >= ex) { sx = gi < 0 ? 0 : 1; gi = ghost_axis(rule[sx], gi, ex, ox); ox += 1; }
+  if (gj < 0 || gj >= ey) { sy = gj < 0 ? 0 : 1; gj = ghost_axis(rule[2 + sy], gj, ey, oy); oy += 1; }
+  if (gk < 0 || gk >= ez) { sz = gk < 0 ? 0 : 1; gk = ghost_axis(rule[4 + sz], gk, ez, oz); oz += 1; }
+  T x = load_up(d, c, comp, gk, gj, gi, corr);
+  if (ox == 2) x = -x;
+  else if (ox == 3) x = rv[sx] - x;
+  if (oy == 2) x = -x;
+  else if (oy == 3) x = rv[2 + sy] - x;
+  if (oz == 2) x = -x;
+  else if (oz == 3) x = rv[4 + sz] - x;
+  return x;
+}
+
+template <typename T>
+__device__ void load_tile(const Dev3<T> &d, const Call3<T> &c, Lds3<T> &L, int k0, int j0, int i0, T corr) {
+  using S = Lds3<T>;
+  for (int e = threadIdx.x; e < S::UZ * S::UY * S::UX; e += kThreads) {
+    const int x = e % S::UX, t = e / S::UX, y = t % S::UY, z = t / S::UY;
+    (&L.u[0][0][0])[e] = padded(d, c, 0, k0 - 2 + z, j0 - 2 + y, i0 - 2 + x, corr);
+  }
+  for (int e = threadIdx.x; e < S::VZ * S::VY * S::VX; e += kThreads) {
+    const int x = e % S::VX, t = e / S::VX, y = t % S::VY, z = t / S::VY;
+    (&L.v[0][0][0])[e] = padded(d, c, 1, k0 - 2 + z, j0 - 2 + y, i0 - 2 + x, corr);
+  }
+  for (int e = threadIdx.x; e < S::WZ * S::WY * S::WX; e += kThreads) {
+    const int x = e % S::WX, t = e / S::WX, y = t % S::WY, z = t / S::WY;
+    (&L.w[0][0][0])[e] = padded(d, c, 2, k0 - 2 + z, j0 - 2 + y, i0 - 2 + x, corr);
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------------------------ eddy viscosity
+// Cell-centred gradient g[i][j] = du_i/dx_j at local cell (k, j, i): diagonal from the
+// cell's faces, off-diagonal as averages of the four surrounding edge values
+// (core.py:400-434).
+template <typename T>
+__device__ __forceinline__ void cell_grad(const Lds3<T> &L, int k, int j, int i, T inv, T g[3][3]) {
+  g[0][0] = (LU(k, j, i + 1) - LU(k, j, i)) * inv;
+  g[1][1] = (LV(k, j + 1, i) - LV(k, j, i)) * inv;
+  g[2][2] = (LW(k + 1, j, i) - LW(k, j, i)) * inv;
+#define DUDY(kk, jj, ii) ((LU(kk, jj, ii) - LU(kk, (jj) - 1, ii)) * inv)
+#define DVDX(kk, jj, ii) ((LV(kk, jj, ii) - LV(kk, jj, (ii) - 1)) * inv)
+#define DUDZ(kk, jj, ii) ((LU(kk, jj, ii) - LU((kk) - 1, jj, ii)) * inv)
+#define DWDX(kk, jj, ii) ((LW(kk, jj, ii) - LW(kk, jj, (ii) - 1)) * inv)
+#define DVDZ(kk, jj, ii) ((LV(kk, jj, ii) - LV((kk) - 1, jj, ii)) * inv)
+#define DWDY(kk, jj, ii) ((LW(kk, jj, ii) - LW(kk, (jj) - 1, ii)) * inv)
+  const T q = (T)0.25;
+  g[0][1] = (((DUDY(k, j, i) + DUDY(k, j + 1, i)) + DUDY(k, j, i + 1)) + DUDY(k, j + 1, i + 1)) * q;
+  g[0][2] = (((DUDZ(k, j, i) + DUDZ(k + 1, j, i)) + DUDZ(k, j, i + 1)) + DUDZ(k + 1, j, i + 1)) * q;
+  g[1][0] = (((DVDX(k, j, i) + DVDX(k, j + 1, i)) + DVDX(k, j, i + 1)) + DVDX(k, j + 1, i + 1)) * q;
+  g[1][2] = (((DVDZ(k, j, i) + DVDZ(k + 1, j, i)) + DVDZ(k, j + 1, i)) + DVDZ(k + 1, j + 1, i)) * q;
+  g[2][0] = (((DWDX(k, j, i) + DWDX(k + 1, j, i)) + DWDX(k, j, i + 1)) + DWDX(k + 1, j, i + 1)) * q;
+  g[2][1] = (((DWDY(k, j, i) + DWDY(k + 1, j, i)) + DWDY(k, j + 1, i)) + DWDY(k + 1, j + 1, i)) * q;
+}
+
+// Smagorinsky / WALE / Vreman on the traceless gradient (core.py:437-482).
+template <typename T>
+__device__ __forceinline__ T eddy3(int turb, T les_k, T g[3][3]) {
+  g[2][2] = -(g[0][0] + g[1][1]);
+  T s[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) s[a][b] = (T)0.5 * (g[a][b] + g[b][a]);
+  T ss = s[0][0] * s[0][0];
+#pragma unroll
+  for (int q = 1; q < 9; ++q) ss = ss + s[q / 3][q % 3] * s[q / 3][q % 3];
+  if (turb == 1) return les_k * sqrt((T)2 * ss + (T)1e-30);
+  if (turb == 2) {
+    T g2[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) g2[a][b] = (g[a][0] * g[0][b] + g[a][1] * g[1][b]) + g[a][2] * g[2][b];
+    const T tr3 = ((g2[0][0] + g2[1][1]) + g2[2][2]) * (T)(1.0 / 3.0);
+    T sd = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        T x = (T)0.5 * (g2[a][b] + g2[b][a]);
+        if (a == b) x = x - tr3;
+        sd = sd + x * x;
+      }
+    const T sdc = sd > (T)0 ? sd : (T)0, ssc = ss > (T)0 ? ss : (T)0;
+    const T rs = sqrt(sdc);
+    const T num = sdc * rs;                                     // sd^1.5
+    const T den = (ssc * ssc * sqrt(ssc) + rs * sqrt(rs)) + (T)1e-30;   // ss^2.5 + sd^1.25
+    return (les_k * num) / den;
+  }
+  T b[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) b[a][c] = (g[a][0] * g[c][0] + g[a][1] * g[c][1]) + g[a][2] * g[c][2];
+  T bb = ((((b[0][0] * b[1][1] - b[0][1] * b[0][1]) + b[0][0] * b[2][2]) - b[0][2] * b[0][2]) + b[1][1] * b[2][2]) -
+         b[1][2] * b[1][2];
+  bb = bb > (T)0 ? bb : (T)0;
+  T aa = g[0][0] * g[0][0];
+#pragma unroll
+  for (int q = 1; q < 9; ++q) aa = aa + g[q / 3][q % 3] * g[q / 3][q % 3];
+  return les_k * sqrt(bb / (aa + (T)1e-30) + (T)1e-30);
+}
+
+// nu_t on the tile's cells and their one-cell ring, zero inside solids (x (1 - chi_c) on
+// the window, core.py:491-503); returns this thread's max over the cells the tile owns
+// (its own cells, plus the ghost ring at domain boundaries).
+template <typename T>
+__device__ T nu_tile(const Dev3<T> &d, const Call3<T> &c, Lds3<T> &L, int k0, int j0, int i0) {
+  using S = Lds3<T>;
+  T mx = 0;
+  const int ie = min(i0 + S::TX, d.nx), je = min(j0 + S::TY, d.ny), ke = min(k0 + S::TZ, d.nz);
+  for (int e = threadIdx.x; e < S::NZ * S::NY * S::NX; e += kThreads) {
+    const int x = e % S::NX, t = e / S::NX, y = t % S::NY, z = t / S::NY;
+    const int i = x - 1, j = y - 1, k = z - 1;
+    T g[3][3];
+    cell_grad(L, k, j, i, d.inv_h, g);
+    T nu = eddy3(d.turb, d.les_k, g);
+    const int gi = i0 + i, gj = j0 + j, gk = k0 + k;
+    if (d.win && gk >= d.wk0 && gk < d.wk1 && gj >= d.wj0 && gj < d.wj1 && gi >= d.wi0 && gi < d.wi1)
+      nu = nu * ((T)1 - c.chi_c[((size_t)(gk - d.wk0) * d.wy + (gj - d.wj0)) * d.wx + (gi - d.wi0)]);
+    L.nu[z][y][x] = nu;
+    const bool oi = (gi >= i0 && gi < ie) || gi == -1 || gi == d.nx;
+    const bool oj = (gj >= j0 && gj < je) || gj == -1 || gj == d.ny;
+    const bool ok = (gk >= k0 && gk < ke) || gk == -1 || gk == d.nz;
+    if (oi && oj && ok && gi <= ie && gj <= je && gk <= ke) mx = nu > mx ? nu : mx;
+  }
+  __syncthreads();
+  return mx;
+}
+
+// ------------------------------------------------------------------ right-hand side
+// upwind3 / central face flux (core.py:331-346).
+template <typename T>
+__device__ __forceinline__ T flux(T vs, T a, T b, T c, T dd, int upwind) {
+  const T s = b + c;
+  if (!upwind) return vs * s;
+  const T cen = s + (T)(-1.0 / 7.0) * (a + dd);
+  const T dis = (b - c) + (T)(-1.0 / 3.0) * (a - dd);
+  return vs * cen + ((T)(3.0 / 7.0) * fabs(vs)) * dis;
+}
+
+template <typename T>
+__device__ __forceinline__ T lerp_t(T start, T end, T w) {
+  return w < (T)0.5 ? start + w * (end - start) : end - (end - start) * ((T)1 - w);
+}
+
+// SGS stress components on edges / cells (core.py:505-524), from LDS
+#define NXY(k, j, i) ((((LN(k, (j) - 1, (i) - 1) + LN(k, j, (i) - 1)) + LN(k, (j) - 1, i)) + LN(k, j, i)) * (T)0.25)
+#define NXZ(k, j, i) ((((LN((k) - 1, j, (i) - 1) + LN(k, j, (i) - 1)) + LN((k) - 1, j, i)) + LN(k, j, i)) * (T)0.25)
+#define NYZ(k, j, i) ((((LN((k) - 1, (j) - 1, i) + LN(k, (j) - 1, i)) + LN((k) - 1, j, i)) + LN(k, j, i)) * (T)0.25)
+#define T12(k, j, i) (NXY(k, j, i) * (DUDY(k, j, i) + DVDX(k, j, i)))
+#define T13(k, j, i) (NXZ(k, j, i) * (DUDZ(k, j, i) + DWDX(k, j, i)))
+#define T23(k, j, i) (NYZ(k, j, i) * (DVDZ(k, j, i) + DWDY(k, j, i)))
+#define T11(k, j, i) (((T)2 * LN(k, j, i)) * ((LU(k, j, (i) + 1) - LU(k, j, i)) * inv))
+#define T22(k, j, i) (((T)2 * LN(k, j, i)) * ((LV(k, (j) + 1, i) - LV(k, j, i)) * inv))
+#define T33(k, j, i) (((T)2 * LN(k, j, i)) * ((LW((k) + 1, j, i) - LW(k, j, i)) * inv))
+
+// explicit tendency at a u-face (local k, j, i): advection + diffusion (+ LES)
+template <typename T>
+__device__ __forceinline__ T rhs_u(const Dev3<T> &d, const Lds3<T> &L, int k, int j, int i, T nu_h2) {
+  const int up = d.upwind;
+  const T fl = flux(LU(k, j, i - 1) + LU(k, j, i), LU(k, j, i - 2), LU(k, j, i - 1), LU(k, j, i), LU(k, j, i + 1), up);
+  const T fr = flux(LU(k, j, i) + LU(k, j, i + 1), LU(k, j, i - 1), LU(k, j, i), LU(k, j, i + 1), LU(k, j, i + 2), up);
+  const T gd = flux(LV(k, j, i - 1) + LV(k, j, i), LU(k, j - 2, i), LU(k, j - 1, i), LU(k, j, i), LU(k, j + 1, i), up);
+  const T gu =
+      flux(LV(k, j + 1, i - 1) + LV(k, j + 1, i), LU(k, j - 1, i), LU(k, j, i), LU(k, j + 1, i), LU(k, j + 2, i), up);
+  const T zd = flux(LW(k, j, i - 1) + LW(k, j, i), LU(k - 2, j, i), LU(k - 1, j, i), LU(k, j, i), LU(k + 1, j, i), up);
+  const T zu =
+      flux(LW(k + 1, j, i - 1) + LW(k + 1, j, i), LU(k - 1, j, i), LU(k, j, i), LU(k + 1, j, i), LU(k + 2, j, i), up);
+  const T a = (((fl - fr) + (gd - gu)) + (zd - zu)) * d.kadv;
+  const T ring = ((LU(k, j, i + 1) + LU(k, j, i - 1)) + (LU(k, j + 1, i) + LU(k, j - 1, i))) +
+                 (LU(k + 1, j, i) + LU(k - 1, j, i));
+  T r = a + (ring - (T)6 * LU(k, j, i)) * nu_h2;
+  if (d.turb) {
+    const T inv = d.inv_h;
+    r = r + (((((T11(k, j, i) - T11(k, j, i - 1)) + T12(k, j + 1, i)) - T12(k, j, i)) + T13(k + 1, j, i)) -
+             T13(k, j, i)) * inv;
+  }
+  return r;
+}
+
+template <typename T>
+__device__ __forceinline__ T rhs_v(const Dev3<T> &d, const Lds3<T> &L, int k, int j, int i, T nu_h2) {
+  const int up = d.upwind;
+  const T gl = flux(LU(k, j - 1, i) + LU(k, j, i), LV(k, j, i - 2), LV(k, j, i - 1), LV(k, j, i), LV(k, j, i + 1), up);
+  const T gr =
+      flux(LU(k, j - 1, i + 1) + LU(k, j, i + 1), LV(k, j, i - 1), LV(k, j, i), LV(k, j, i + 1), LV(k, j, i + 2), up);
+  const T fd = flux(LV(k, j - 1, i) + LV(k, j, i), LV(k, j - 2, i), LV(k, j - 1, i), LV(k, j, i), LV(k, j + 1, i), up);
+  const T fu = flux(LV(k, j, i) + LV(k, j + 1, i), LV(k, j - 1, i), LV(k, j, i), LV(k, j + 1, i), LV(k, j + 2, i), up);
+  const T zd = flux(LW(k, j - 1, i) + LW(k, j, i), LV(k - 2, j, i), LV(k - 1, j, i), LV(k, j, i), LV(k + 1, j, i), up);
+  const T zu =
+      flux(LW(k + 1, j - 1, i) + LW(k + 1, j, i), LV(k - 1, j, i), LV(k, j, i), LV(k + 1, j, i), LV(k + 2, j, i), up);
+  const T a = (((gl - gr) + (fd - fu)) + (zd - zu)) * d.kadv;
+  const T ring = ((LV(k, j, i + 1) + LV(k, j, i - 1)) + (LV(k, j + 1, i) + LV(k, j - 1, i))) +
+                 (LV(k + 1, j, i) + LV(k - 1, j, i));
+  T r = a + (ring - (T)6 * LV(k, j, i)) * nu_h2;
+  if (d.turb) {
+    const T inv = d.inv_h;
+    r = r + (((((T12(k, j, i + 1) - T12(k, j, i)) + T22(k, j, i)) - T22(k, j - 1, i)) + T23(k + 1, j, i)) -
+             T23(k, j, i)) * inv;
+  }
+  return r;
+}
+
+template <typename T>
+__device__ __forceinline__ T rhs_w(const Dev3<T> &d, const Lds3<T> &L, int k, int j, int i, T nu_h2) {
+  const int up = d.upwind;
+  const T gl = flux(LU(k - 1, j, i) + LU(k, j, i), LW(k, j, i - 2), LW(k, j, i - 1), LW(k, j, i), LW(k, j, i + 1), up);
+  const T gr =
+      flux(LU(k - 1, j, i + 1) + LU(k, j, i + 1), LW(k, j, i - 1), LW(k, j, i), LW(k, j, i + 1), LW(k, j, i + 2), up);
+  const T gd = flux(LV(k - 1, j, i) + LV(k, j, i), LW(k, j - 2, i), LW(k, j - 1, i), LW(k, j, i), LW(k, j + 1, i), up);
+  const T gu =
+      flux(LV(k - 1, j + 1, i) + LV(k, j + 1, i), LW(k, j - 1, i), LW(k, j, i), LW(k, j + 1, i), LW(k, j + 2, i), up);
+  const T fd = flux(LW(k - 1, j, i) + LW(k, j, i), LW(k - 2, j, i), LW(k - 1, j, i), LW(k, j, i), LW(k + 1, j, i), up);
+  const T fu = flux(LW(k, j, i) + LW(k + 1, j, i), LW(k - 1, j, i), LW(k, j, i), LW(k + 1, j, i), LW(k + 2, j, i), up);
+  const T a = (((gl - gr) + (gd - gu)) + (fd - fu)) * d.kadv;
+  const T ring = ((LW(k, j, i + 1) + LW(k, j, i - 1)) + (LW(k, j + 1, i) + LW(k, j - 1, i))) +
+                 (LW(k + 1, j, i) + LW(k - 1, j, i));
+  T r = a + (ring - (T)6 * LW(k, j, i)) * nu_h2;
+  if (d.turb) {
+    const T inv = d.inv_h;
+    r = r + (((((T13(k, j, i + 1) - T13(k, j, i)) + T23(k, j + 1, i)) - T23(k, j, i)) + T33(k, j, i)) -
+             T33(k - 1, j, i)) * inv;
+  }
+  return r;
+}
+
+// faces a tile writes along one axis: [c0, end); the last tile also writes the last face
+__device__ __forceinline__ int face_end(int c0, int t, int n) { return c0 + t >= n ? n + 1 : c0 + t; }
+
+// ------------------------------------------------------------------ stage kernel
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_stage(const Dev3<T> *__restrict__ dp, Call3<T> c) {
+  using S = Lds3<T>;
+  __shared__ S L;
+  const Dev3<T> &d = *dp;
+  int bx, by, bz, tile;
+  tile_of(d, bx, by, bz, tile);
+  const int i0 = bx * S::TX, j0 = by * S::TY, k0 = bz * S::TZ;
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  const int s = c.stage;
+  const T corr = c.corr_stage >= 0 ? d.red->corr[c.corr_stage] : (T)0;
+  load_tile(d, c, L, k0, j0, i0, corr);
+  if (d.turb) (void)nu_tile(d, c, L, k0, j0, i0);
+  const T dt = step_dt(d, c);
+  const T a = (T)kSsp[s][0];
+  const T bdt = dt * (T)kSsp[s][1];
+  const T ws = (T)kForceW[s];
+  const T nu_h2 = *c.nu * d.inv_h2;
+  const T kpen = d.win ? bdt / *c.eta : (T)0;
+  const int ie = min(i0 + S::TX, nx), je = min(j0 + S::TY, ny), ke = min(k0 + S::TZ, nz);
+  double osum = 0.0, isum = 0.0;
+  // u faces
+  {
+    const int fe = face_end(i0, S::TX, nx), nfx = fe - i0, nfy = je - j0, nfz = ke - k0;
+    for (int e = threadIdx.x; e < nfx * nfy * nfz; e += kThreads) {
+      const int li = e % nfx, t = e / nfx, lj = t % nfy, lk = t / nfy;
+      const int gi = i0 + li, gj = j0 + lj, gk = k0 + lk;
+      const size_t idx = ((size_t)gk * ny + gj) * (nx + 1) + gi;
+      T lu;
+      const bool xb = !d.per[0] && (gi == 0 || gi == nx);
+      if (xb) lu = (gi == nx && d.outlet) ? (LU(lk, lj, li - 1) - LU(lk, lj, li)) * d.inv_h : (T)0;
+      else lu = rhs_u(d, L, lk, lj, li, nu_h2);
+      const T upv = LU(lk, lj, li);
+      T uh = (s == 0 ? upv : lerp_t(upv, c.u0[idx], a)) + lu * bdt;
+      if (xb && gi == 0) uh = d.outlet ? c.u_inlet[(size_t)gk * ny + gj] : (T)0;
+      if (xb && gi == nx && d.closed) uh = (T)0;
+      if (d.win && gk >= d.wk0 && gk < d.wk1 && gj >= d.wj0 && gj < d.wj1 && gi >= d.wi0 && gi <= d.wi1) {
+        const size_t wi = ((size_t)(gk - d.wk0) * d.wy + (gj - d.wj0)) * (d.wx + 1) + (gi - d.wi0);
+        const T un = (uh + kpen * c.a_u[wi]) / ((T)1 + kpen * c.chi_u[wi]);
+        const T du = un - uh;
+        d.acc_u[wi] = s == 0 ? du * ws : d.acc_u[wi] + du * ws;
+        uh = un;
+      }
+      c.uo[idx] = uh;
+      if (gi == nx && d.outlet) {
+        osum += (double)uh;
+        isum += (double)c.u_inlet[(size_t)gk * ny + gj];
+      }
+    }
+  }
+  // v faces
+  {
+    const int fe = face_end(j0, S::TY, ny), nfx = ie - i0, nfy = fe - j0, nfz = ke - k0;
+    for (int e = threadIdx.x; e < nfx * nfy * nfz; e += kThreads) {
+      const int li = e % nfx, t = e / nfx, lj = t % nfy, lk = t / nfy;
+      const int gi = i0 + li, gj = j0 + lj, gk = k0 + lk;
+      const size_t idx = ((size_t)gk * (ny + 1) + gj) * nx + gi;
+      const bool yb = !d.per[1] && (gj == 0 || gj == ny);
+      const T lv = yb ? (T)0 : rhs_v(d, L, lk, lj, li, nu_h2);
+      const T vpv = LV(lk, lj, li);
+      T vh = (s == 0 ? vpv : lerp_t(vpv, c.v0[idx], a)) + lv * bdt;
+      if (d.win && gk >= d.wk0 && gk < d.wk1 && gj >= d.wj0 && gj <= d.wj1 && gi >= d.wi0 && gi < d.wi1) {
+        const size_t wi = ((size_t)(gk - d.wk0) * (d.wy + 1) + (gj - d.wj0)) * d.wx + (gi - d.wi0);
+        const T vn = (vh + kpen * c.a_v[wi]) / ((T)1 + kpen * c.chi_v[wi]);
+        const T dv = vn - vh;
+        d.acc_v[wi] = s == 0 ? dv * ws : d.acc_v[wi] + dv * ws;
+        vh = vn;
+      }
+      c.vo[idx] = vh;
+    }
+  }
+  // w faces
+  {
+    const int fe = face_end(k0, S::TZ, nz), nfx = ie - i0, nfy = je - j0, nfz = fe - k0;
+    for (int e = threadIdx.x; e < nfx * nfy * nfz; e += kThreads) {
+      const int li = e % nfx, t = e / nfx, lj = t % nfy, lk = t / nfy;
+      const int gi = i0 + li, gj = j0 + lj, gk = k0 + lk;
+      const size_t idx = ((size_t)gk * ny + gj) * nx + gi;
+      const bool zb = !d.per[2] && (gk == 0 || gk == nz);
+      const T lw = zb ? (T)0 : rhs_w(d, L, lk, lj, li, nu_h2);
+      const T wpv = LW(lk, lj, li);
+      T wh = (s == 0 ? wpv : lerp_t(wpv, c.w0[idx], a)) + lw * bdt;
+      if (d.win && gk >= d.wk0 && gk <= d.wk1 && gj >= d.wj0 && gj < d.wj1 && gi >= d.wi0 && gi < d.wi1) {
+        const size_t wi = ((size_t)(gk - d.wk0) * d.wy + (gj - d.wj0)) * d.wx + (gi - d.wi0);
+        const T wn = (wh + kpen * c.a_w[wi]) / ((T)1 + kpen * c.chi_w[wi]);
+        const T dw = wn - wh;
+        d.acc_w[wi] = s == 0 ? dw * ws : d.acc_w[wi] + dw * ws;
+        wh = wn;
+      }
+      c.wo[idx] = wh;
+    }
+  }
+  // outlet plane sums of this tile (the outlet correction needs them all)
+  if (d.outlet && i0 + S::TX >= nx) {
+    double v[2] = {isum, osum};
+    block_sum4<2>(v, L.red);
+    if (threadIdx.x == 0) {
+      d.pout[2 * (bz * d.nby + by)] = v[0];
+      d.pout[2 * (bz * d.nby + by) + 1] = v[1];
+    }
+  }
+}
+
+// ------------------------------------------------------------------ maxima / final
+// mode 0: dt inputs of a state (max |u|, |v|, |w|, nu_t), no writes.
+// mode 1: the last stage's projection applied (out = u^ - grad psi), written to the state,
+//         plus the dt inputs of that state and the force sums of window tiles.
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_final(const Dev3<T> *__restrict__ dp, Call3<T> c, int mode) {
+  using S = Lds3<T>;
+  __shared__ S L;
+  const Dev3<T> &d = *dp;
+  int bx, by, bz, tile;
+  tile_of(d, bx, by, bz, tile);
+  const int i0 = bx * S::TX, j0 = by * S::TY, k0 = bz * S::TZ;
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  const T corr = c.corr_stage >= 0 ? d.red->corr[c.corr_stage] : (T)0;
+  load_tile(d, c, L, k0, j0, i0, corr);
+  T mnu = 0;
+  if (d.turb) mnu = nu_tile(d, c, L, k0, j0, i0);
+  const int ie = min(i0 + S::TX, nx), je = min(j0 + S::TY, ny), ke = min(k0 + S::TZ, nz);
+  T mu = 0, mv = 0, mw = 0;
+  {
+    const int fe = face_end(i0, S::TX, nx), nfx = fe - i0, nfy = je - j0, nfz = ke - k0;
+    for (int e = threadIdx.x; e < nfx * nfy * nfz; e += kThreads) {
+      const int li = e % nfx, t = e / nfx, lj = t % nfy, lk = t / nfy;
+      const T x = LU(lk, lj, li);
+      if (mode) c.uo[((size_t)(k0 + lk) * ny + (j0 + lj)) * (nx + 1) + i0 + li] = x;
+      mu = fabs(x) > mu ? fabs(x) : mu;
+    }
+  }
+  {
+    const int fe = face_end(j0, S::TY, ny), nfx = ie - i0, nfy = fe - j0, nfz = ke - k0;
+    for (int e = threadIdx.x; e < nfx * nfy * nfz; e += kThreads) {
+      const int li = e % nfx, t = e / nfx, lj = t % nfy, lk = t / nfy;
+      const T x = LV(lk, lj, li);
+      if (mode) c.vo[((size_t)(k0 + lk) * (ny + 1) + (j0 + lj)) * nx + i0 + li] = x;
+      mv = fabs(x) > mv ? fabs(x) : mv;
+    }
+  }
+  {
+    const int fe = face_end(k0, S::TZ, nz), nfx = ie - i0, nfy = je - j0, nfz = fe - k0;
+    for (int e = threadIdx.x; e < nfx * nfy * nfz; e += kThreads) {
+      const int li = e % nfx, t = e / nfx, lj = t % nfy, lk = t / nfy;
+      const T x = LW(lk, lj, li);
+      if (mode) c.wo[((size_t)(k0 + lk) * ny + (j0 + lj)) * nx + i0 + li] = x;
+      mw = fabs(x) > mw ? fabs(x) : mw;
+    }
+  }
+  mu = wave_max(mu);
+  mv = wave_max(mv);
+  mw = wave_max(mw);
+  mnu = wave_max(mnu);
+  T *red = reinterpret_cast<T *>(L.red);
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __syncthreads();
+  if (lane == 0) {
+    red[wv * 4 + 0] = mu;
+    red[wv * 4 + 1] = mv;
+    red[wv * 4 + 2] = mw;
+    red[wv * 4 + 3] = mnu;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    T m = red[threadIdx.x];
+    for (int q = 1; q < 4; ++q) m = red[q * 4 + threadIdx.x] > m ? red[q * 4 + threadIdx.x] : m;
+    d.pmax[(size_t)tile * 4 + threadIdx.x] = m;
+  }
+  // force sums of this tile's window faces: sum_faces k[b, m] * acc (core.py:653-660)
+  if (mode && d.win && bx >= d.wbx0 && bx < d.wbx0 + d.nwbx && by >= d.wby0 && by < d.wby0 + d.nwby &&
+      bz >= d.wbz0 && bz < d.wbz0 + d.nwbz) {
+    const int wt = ((bz - d.wbz0) * d.nwby + (by - d.wby0)) * d.nwbx + (bx - d.wbx0);
+    __syncthreads();
+    for (int q = 0; q < d.nq; ++q) {
+      const int b = q / 9, r = q % 9, comp = r / 3, m = r % 3;
+      double acc = 0.0;
+      // window faces of component comp inside this tile
+      int f0[3] = {max(i0, d.wi0), max(j0, d.wj0), max(k0, d.wk0)};
+      int f1[3] = {min(comp == 0 ? face_end(i0, S::TX, nx) : ie, d.wi1 + (comp == 0)),
+                   min(comp == 1 ? face_end(j0, S::TY, ny) : je, d.wj1 + (comp == 1)),
+                   min(comp == 2 ? face_end(k0, S::TZ, nz) : ke, d.wk1 + (comp == 2))};
+      const int n0 = f1[0] - f0[0], n1 = f1[1] - f0[1], n2 = f1[2] - f0[2];
+      if (n0 > 0 && n1 > 0 && n2 > 0) {
+        const int ex = d.wx + (comp == 0), ey = d.wy + (comp == 1), ez = d.wz + (comp == 2);
+        const size_t wn = (size_t)ex * ey * ez;
+        const T *kk = (comp == 0 ? c.k_u : comp == 1 ? c.k_v : c.k_w) + ((size_t)b * 3 + m) * wn;
+        const T *ac = comp == 0 ? d.acc_u : comp == 1 ? d.acc_v : d.acc_w;
+        for (int e = threadIdx.x; e < n0 * n1 * n2; e += kThreads) {
+          const int x = e % n0, t = e / n0, y = t % n1, z = t / n1;
+          const size_t wi = ((size_t)(f0[2] + z - d.wk0) * ey + (f0[1] + y - d.wj0)) * ex + (f0[0] + x - d.wi0);
+          acc += (double)kk[wi] * (double)ac[wi];
+        }
+      }
+      acc = wave_sum(acc);
+      if (lane == 0) L.red[wv * 16 + (q & 15)] = acc;
+      if ((q & 15) == 15 || q == d.nq - 1) {
+        __syncthreads();
+        const int base = q & ~15;
+        if (threadIdx.x <= q - base)
+          d.pforce[(size_t)wt * d.nq + base + threadIdx.x] =
+              ((L.red[threadIdx.x] + L.red[16 + threadIdx.x]) + L.red[32 + threadIdx.x]) + L.red[48 + threadIdx.x];
+        __syncthreads();
+      }
+    }
+  }
+}
+
+// dt rule: min(cfl h / (max|u| + max|v| + max|w|), h^2 / 6 / (nu + nu_t,max)) (core.py:637-646)
+template <typename T>
+__device__ T cfl_dt(const Dev3<T> &d, const Call3<T> &c, T mu, T mv, T mw, T mnu) {
+  T umax = (mu + mv) + mw;
+  umax = umax > (T)1e-6 ? umax : (T)1e-6;
+  const T dta = d.cfl_h / umax;
+  const T nu = d.turb ? *c.nu + mnu : *c.nu;
+  const T dtb = d.visc_h2 / nu;
+  return dta < dtb ? dta : dtb;
+}
+
+// Max of the per-tile partials (fixed order), by all threads of a 1024-thread block.
+template <typename T>
+__device__ void reduce_pmax(const Dev3<T> &d, T out[4], T *lds) {
+  T m[4] = {0, 0, 0, 0};
+  for (int b = threadIdx.x; b < d.nblk; b += blockDim.x)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) m[q] = d.pmax[(size_t)b * 4 + q] > m[q] ? d.pmax[(size_t)b * 4 + q] : m[q];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) m[q] = wave_max(m[q]);
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) lds[wv * 4 + q] = m[q];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    T r = lds[q];
+    for (int w = 1; w < nw; ++w) r = lds[w * 4 + q] > r ? lds[w * 4 + q] : r;
+    out[q] = r;
+  }
+  __syncthreads();
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kFinThreads) k_dt(const Dev3<T> *__restrict__ dp, Call3<T> c) {
+  __shared__ T lds[64];
+  const Dev3<T> &d = *dp;
+  T m[4];
+  reduce_pmax(d, m, lds);
+  if (threadIdx.x == 0) d.red->dt = cfl_dt(d, c, m[0], m[1], m[2], m[3]);
+}
+
+// Outputs of the finalize kernel.
+template <typename T>
+struct Fin3 {
+  const T *t_in;
+  T *t_out, *dt_out, *coeffs, *hist;
+  long long *hist_count;
+  long long hist_cap;
+  int next_dt;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(kFinThreads) k_finalize(const Dev3<T> *__restrict__ dp, Call3<T> c, Fin3<T> f) {
+  __shared__ T lds[64];
+  __shared__ double fs[kMaxQ];
+  __shared__ double part[kFinThreads];
+  const Dev3<T> &d = *dp;
+  const T dt = step_dt(d, c);
+  // force sums over window tiles: thread (q, slice) sums tiles slice, slice + P, ... in order
+  const int nq = d.nq;
+  if (nq > 0) {
+    const int P = kFinThreads / nq;
+    const int q = threadIdx.x % nq, sl = threadIdx.x / nq;
+    double a = 0.0;
+    if (sl < P)
+      for (int t = sl; t < d.nwt; t += P) a += d.pforce[(size_t)t * nq + q];
+    part[threadIdx.x] = a;
+    __syncthreads();
+    if (threadIdx.x < nq) {
+      double r = 0.0;
+      for (int s2 = 0; s2 < P; ++s2) r += part[s2 * nq + threadIdx.x];
+      fs[threadIdx.x] = r;
+    }
+    __syncthreads();
+  }
+  T mx[4] = {0, 0, 0, 0};
+  if (f.next_dt && !d.fixed_dt) reduce_pmax(d, mx, lds);
+  if (threadIdx.x != 0) return;
+  const T scale = d.neg_h3 / dt;
+  T tot[6] = {0, 0, 0, 0, 0, 0};
+  for (int b = 0; b < d.nbody; ++b) {
+    T ru[3], rv[3], rw[3];
+    for (int m = 0; m < 3; ++m) {
+      ru[m] = (T)fs[b * 9 + m] * scale;
+      rv[m] = (T)fs[b * 9 + 3 + m] * scale;
+      rw[m] = (T)fs[b * 9 + 6 + m] * scale;
+    }
+    T fx = ru[0], fy = rv[0], fz = rw[0];
+    T mxm = rv[1] + rw[1], my = ru[1] + rw[2], mz = ru[2] + rv[2];
+    if (d.symmetry) {
+      fx = (T)2 * fx;
+      fz = (T)2 * fz;
+      my = (T)2 * my;
+      fy = (T)0;
+      mxm = -d.dy * fz;
+      mz = d.dy * fx;
+    }
+    const T cf[6] = {fx / d.q_a, fy / d.q_a, fz / d.q_a, mxm / d.q_a_span, my / d.q_a_len, mz / d.q_a_span};
+    for (int m = 0; m < 6; ++m) {
+      if (f.coeffs) f.coeffs[b * 6 + m] = cf[m];
+      tot[m] = tot[m] + cf[m];
+    }
+  }
+  const T tnew = *f.t_in + dt;
+  if (f.t_out) *f.t_out = tnew;
+  if (f.dt_out) *f.dt_out = dt;
+  if (f.hist && f.hist_count && f.hist_cap > 0) {
+    long long n = *f.hist_count;
+    T *row = f.hist + (size_t)(n % f.hist_cap) * 7;
+    row[0] = tnew;
+    for (int m = 0; m < 6; ++m) row[1 + m] = tot[m];
+    *f.hist_count = n + 1;
+  }
+  if (f.next_dt && !d.fixed_dt) d.red->dt = cfl_dt(d, c, mx[0], mx[1], mx[2], mx[3]);
+}
+
+// ------------------------------------------------------------------ Poisson passes
+__device__ __forceinline__ int makhoul_src(int n, int len) { return n < (len + 1) / 2 ? 2 * n : 2 * (len - 1 - n) + 1; }
+
+// Inverse real-transform packing: conj-weighted pair for the DCT-III, plain pair for the DHT.
+template <typename T>
+__device__ __forceinline__ cplx<T> pack_inv(const T *xa, const T *xb, int n, int dct, cplx<T> w, int k) {
+  if (dct) {
+    const T xa_k = xa[k], xb_k = xb[k];
+    const T xa_r = k == 0 ? (T)0 : xa[n - k];
+    const T xb_r = k == 0 ? (T)0 : xb[n - k];
+    const cplx<T> va = {w.x * xa_k + w.y * xa_r, w.y * xa_k - w.x * xa_r};
+    const cplx<T> vb = {w.x * xb_k + w.y * xb_r, w.y * xb_k - w.x * xb_r};
+    return {va.x - vb.y, -(va.y + vb.x)};
+  }
+  return {xa[k], xb[k]};
+}
+
+// forward real transform values of slot f at spectral index k (after fft of packed pairs)
+template <typename T>
+__device__ __forceinline__ void fwd_split(const cplx<T> *R, int n, int k, int dct, cplx<T> cs, T &xa, T &xb) {
+  cplx<T> va, vb;
+  split_pair(R, n, k, va, vb);
+  if (dct) {
+    xa = cs.x * va.x + cs.y * va.y;
+    xb = cs.x * vb.x + cs.y * vb.y;
+  } else {
+    xa = va.x - va.y;
+    xb = vb.x - vb.y;
+  }
+}
+
+// inverse real transform values at natural position q(m) of slot f (after fft of pack_inv)
+template <typename T>
+__device__ __forceinline__ void inv_unpack(const cplx<T> *R, int n, int m, int dct, T inv_n, T &va, T &vb) {
+  if (dct) {
+    va = R[m].x * inv_n;
+    vb = -R[m].y * inv_n;
+  } else {
+    cplx<T> ha, hb;
+    split_pair(R, n, m, ha, hb);
+    va = (ha.x - ha.y) * inv_n;
+    vb = (hb.x - hb.y) * inv_n;
+  }
+}
+
+// divergence at cell (k, j, i) of (u^, v^, w^) with the outlet face shifted by corr
+// (core.py:577-579)
+template <typename T>
+__device__ __forceinline__ T div_at(const Dev3<T> &d, const T *u, const T *v, const T *w, int k, int j, int i, T corr) {
+  const int nx = d.nx, ny = d.ny;
+  const T *ur = u + ((size_t)k * ny + j) * (nx + 1);
+  T u1 = ur[i + 1];
+  if (i + 1 == nx) u1 = u1 + corr;
+  const T *vr = v + ((size_t)k * (ny + 1) + j) * nx + i;
+  const size_t pl = (size_t)ny * nx;
+  const T *wr = w + ((size_t)k * ny + j) * nx + i;
+  return (((u1 - ur[i]) + (vr[nx] - vr[0])) + (wr[pl] - wr[0])) * d.inv_h;
+}
+
+// P1: rows (pairs) along x, forward. mode 0: f is the right-hand side; 1: divergence of
+// (u, v, w); 2: divergence with the outlet correction of stage `stage` (computed here).
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_p1(const Dev3<T> *__restrict__ dp, const T *__restrict__ u,
+                                                 const T *__restrict__ v, const T *__restrict__ w,
+                                                 const T *__restrict__ f, int mode, int stage) {
+  extern __shared__ unsigned char smem[];
+  const Dev3<T> &d = *dp;
+  const int nx = d.nx, ns = d.ns1, nrows = d.ny * d.nz;
+  cplx<T> *A = reinterpret_cast<cplx<T> *>(smem);
+  cplx<T> *B = A + ns * nx;
+  cplx<T> *TW = B + ns * nx;
+  double *red = reinterpret_cast<double *>(TW + nx);
+  for (int m = threadIdx.x; m < nx; m += kThreads) TW[m] = d.tw[0][m];
+  T corr = 0;
+  if (mode == 2) {
+    double v2[2] = {0.0, 0.0};
+    const int np = d.nby * d.nbz;
+    for (int q = threadIdx.x; q < np; q += kThreads) {
+      v2[0] += d.pout[2 * q];
+      v2[1] += d.pout[2 * q + 1];
+    }
+    block_sum4<2>(v2, red);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      red[8] = v2[0];
+      red[9] = v2[1];
+    }
+    __syncthreads();
+    corr = ((T)red[8] - (T)red[9]) * d.corr_scale;
+    if (blockIdx.x == 0 && threadIdx.x == 0) d.red->corr[stage] = corr;
+  }
+  const int r0 = 2 * ns * blockIdx.x;
+  for (int e = threadIdx.x; e < ns * nx; e += kThreads) {
+    const int sl = e / nx, n = e - sl * nx;
+    const int ra = r0 + 2 * sl, rb = ra + 1;
+    const int q = d.dct[0] ? makhoul_src(n, nx) : n;
+    T xa = 0, xb = 0;
+    if (mode == 0) {
+      if (ra < nrows) xa = f[(size_t)ra * nx + q];
+      if (rb < nrows) xb = f[(size_t)rb * nx + q];
+    } else {
+      if (ra < nrows) xa = div_at(d, u, v, w, ra / d.ny, ra % d.ny, q, corr);
+      if (rb < nrows) xb = div_at(d, u, v, w, rb / d.ny, rb % d.ny, q, corr);
+    }
+    A[e] = {xa, xb};
+  }
+  __syncthreads();
+  const cplx<T> *R = fft_lds(A, B, d.fp[0], TW, ns, nx);
+  for (int e = threadIdx.x; e < ns * nx; e += kThreads) {
+    const int sl = e / nx, k = e - sl * nx;
+    const int ra = r0 + 2 * sl, rb = ra + 1;
+    T xa, xb;
+    fwd_split(R + sl * nx, nx, k, d.dct[0], d.dct[0] ? d.cs[0][k] : cplx<T>{0, 0}, xa, xb);
+    if (ra < nrows) d.F[(size_t)ra * nx + k] = xa;
+    if (rb < nrows) d.F[(size_t)rb * nx + k] = xb;
+  }
+}
+
+// P2 / P4: column chunks along y (stride nx) of layer blockIdx.y, forward or inverse.
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_py(const Dev3<T> *__restrict__ dp, int inverse) {
+  extern __shared__ unsigned char smem[];
+  const Dev3<T> &d = *dp;
+  const int nx = d.nx, ny = d.ny, ns = d.ns2, ncol = 2 * ns, st = ny + 1;
+  cplx<T> *A = reinterpret_cast<cplx<T> *>(smem);
+  cplx<T> *B = A + ns * st;
+  cplx<T> *TW = B + ns * st;
+  T *X = reinterpret_cast<T *>(TW + ny);   // ncol rows of st reals
+  for (int m = threadIdx.x; m < ny; m += kThreads) TW[m] = d.tw[1][m];
+  const int c0 = ncol * blockIdx.x, k = blockIdx.y;
+  T *base = d.F + (size_t)k * ny * nx + c0;
+  const int dct = d.dct[1];
+  if (!inverse) {
+    for (int e = threadIdx.x; e < ncol * ny; e += kThreads) {
+      const int n = e / ncol, col = e - n * ncol;
+      const int q = dct ? makhoul_src(n, ny) : n;
+      const T val = c0 + col < nx ? base[(size_t)q * nx + col] : (T)0;
+      reinterpret_cast<T *>(&A[(col >> 1) * st + n])[col & 1] = val;
+    }
+    __syncthreads();
+    const cplx<T> *R = fft_lds(A, B, d.fp[1], TW, ns, st);
+    for (int e = threadIdx.x; e < ns * ny; e += kThreads) {
+      const int sl = e / ny, kk = e - sl * ny;
+      T xa, xb;
+      fwd_split(R + sl * st, ny, kk, dct, dct ? d.cs[1][kk] : cplx<T>{0, 0}, xa, xb);
+      X[(2 * sl) * st + kk] = xa;
+      X[(2 * sl + 1) * st + kk] = xb;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < ncol * ny; e += kThreads) {
+      const int kk = e / ncol, col = e - kk * ncol;
+      if (c0 + col < nx) base[(size_t)kk * nx + col] = X[col * st + kk];
+    }
+    return;
+  }
+  for (int e = threadIdx.x; e < ncol * ny; e += kThreads) {
+    const int kk = e / ncol, col = e - kk * ncol;
+    X[col * st + kk] = c0 + col < nx ? base[(size_t)kk * nx + col] : (T)0;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < ns * ny; e += kThreads) {
+    const int sl = e / ny, kk = e - sl * ny;
+    A[sl * st + kk] = pack_inv(X + (2 * sl) * st, X + (2 * sl + 1) * st, ny, dct, dct ? d.cs[1][kk] : cplx<T>{0, 0}, kk);
+  }
+  __syncthreads();
+  const cplx<T> *R = fft_lds(A, B, d.fp[1], TW, ns, st);
+  const T inv_n = (T)(1.0 / (double)ny);
+  for (int e = threadIdx.x; e < ns * ny; e += kThreads) {
+    const int sl = e / ny, m = e - sl * ny;
+    T va, vb;
+    inv_unpack(R + sl * st, ny, m, dct, inv_n, va, vb);
+    const int q = dct ? makhoul_src(m, ny) : m;
+    X[(2 * sl) * st + q] = va;
+    X[(2 * sl + 1) * st + q] = vb;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < ncol * ny; e += kThreads) {
+    const int q = e / ncol, col = e - q * ncol;
+    if (c0 + col < nx) base[(size_t)q * nx + col] = X[col * st + q];
+  }
+}
+
+// P3: column chunks along z (stride ny nx) of row blockIdx.y: forward, x 1/lambda, inverse.
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_pz(const Dev3<T> *__restrict__ dp) {
+  extern __shared__ unsigned char smem[];
+  const Dev3<T> &d = *dp;
+  const int nx = d.nx, ny = d.ny, nz = d.nz, ns = d.ns3, ncol = 2 * ns, st = nz + 1;
+  cplx<T> *A = reinterpret_cast<cplx<T> *>(smem);
+  cplx<T> *B = A + ns * st;
+  cplx<T> *TW = B + ns * st;
+  T *X = reinterpret_cast<T *>(TW + nz);
+  for (int m = threadIdx.x; m < nz; m += kThreads) TW[m] = d.tw[2][m];
+  const int c0 = ncol * blockIdx.x, j = blockIdx.y;
+  const size_t pl = (size_t)ny * nx;
+  T *base = d.F + (size_t)j * nx + c0;
+  const int dct = d.dct[2];
+  for (int e = threadIdx.x; e < ncol * nz; e += kThreads) {
+    const int n = e / ncol, col = e - n * ncol;
+    const int q = dct ? makhoul_src(n, nz) : n;
+    const T val = c0 + col < nx ? base[(size_t)q * pl + col] : (T)0;
+    reinterpret_cast<T *>(&A[(col >> 1) * st + n])[col & 1] = val;
+  }
+  __syncthreads();
+  const cplx<T> *R = fft_lds(A, B, d.fp[2], TW, ns, st);
+  const double ly = d.lam[1][j];
+  for (int e = threadIdx.x; e < ns * nz; e += kThreads) {
+    const int sl = e / nz, kz = e - sl * nz;
+    T xa, xb;
+    fwd_split(R + sl * st, nz, kz, dct, dct ? d.cs[2][kz] : cplx<T>{0, 0}, xa, xb);
+    const int ca = c0 + 2 * sl, cb = ca + 1;
+    const double lz = d.lam[2][kz] + ly;
+    const double la = lz + d.lam[0][ca < nx ? ca : 0];
+    const double lb = lz + d.lam[0][cb < nx ? cb : 0];
+    const double ia = (la != 0.0 && !(ca == 0 && j == 0 && kz == 0)) ? 1.0 / la : 0.0;
+    const double ib = lb != 0.0 ? 1.0 / lb : 0.0;
+    X[(2 * sl) * st + kz] = xa * (T)ia;
+    X[(2 * sl + 1) * st + kz] = xb * (T)ib;
+  }
+  __syncthreads();
+  cplx<T> *Z = (R == A) ? B : A;
+  for (int e = threadIdx.x; e < ns * nz; e += kThreads) {
+    const int sl = e / nz, kz = e - sl * nz;
+    Z[sl * st + kz] = pack_inv(X + (2 * sl) * st, X + (2 * sl + 1) * st, nz, dct, dct ? d.cs[2][kz] : cplx<T>{0, 0}, kz);
+  }
+  __syncthreads();
+  const cplx<T> *R2 = fft_lds(Z, (Z == A) ? B : A, d.fp[2], TW, ns, st);
+  const T inv_n = (T)(1.0 / (double)nz);
+  for (int e = threadIdx.x; e < ns * nz; e += kThreads) {
+    const int sl = e / nz, m = e - sl * nz;
+    T va, vb;
+    inv_unpack(R2 + sl * st, nz, m, dct, inv_n, va, vb);
+    const int q = dct ? makhoul_src(m, nz) : m;
+    X[(2 * sl) * st + q] = va;
+    X[(2 * sl + 1) * st + q] = vb;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < ncol * nz; e += kThreads) {
+    const int q = e / ncol, col = e - q * ncol;
+    if (c0 + col < nx) base[(size_t)q * pl + col] = X[col * st + q];
+  }
+}
+
+// P5: rows (pairs) along x, inverse -> psi; the step-averaged pressure p (core.py:874-875):
+// stage 0 p = w psi, stage 1 p = p + w psi, stage 2 p = (p + psi) (1 / dt).
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_p5(const Dev3<T> *__restrict__ dp, Call3<T> c, T *__restrict__ psi,
+                                                 T *__restrict__ p, int stage) {
+  extern __shared__ unsigned char smem[];
+  const Dev3<T> &d = *dp;
+  const int nx = d.nx, ns = d.ns5, nrows = d.ny * d.nz, st = nx + 1;
+  cplx<T> *A = reinterpret_cast<cplx<T> *>(smem);
+  cplx<T> *B = A + ns * st;
+  cplx<T> *TW = B + ns * st;
+  T *X = reinterpret_cast<T *>(TW + nx);
+  for (int m = threadIdx.x; m < nx; m += kThreads) TW[m] = d.tw[0][m];
+  const int r0 = 2 * ns * blockIdx.x;
+  const int dct = d.dct[0];
+  for (int e = threadIdx.x; e < 2 * ns * nx; e += kThreads) {
+    const int rr = e / nx, k = e - rr * nx;
+    X[rr * st + k] = r0 + rr < nrows ? d.F[(size_t)(r0 + rr) * nx + k] : (T)0;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < ns * nx; e += kThreads) {
+    const int sl = e / nx, k = e - sl * nx;
+    A[sl * st + k] = pack_inv(X + (2 * sl) * st, X + (2 * sl + 1) * st, nx, dct, dct ? d.cs[0][k] : cplx<T>{0, 0}, k);
+  }
+  __syncthreads();
+  const cplx<T> *R = fft_lds(A, B, d.fp[0], TW, ns, st);
+  const T inv_n = (T)(1.0 / (double)nx);
+  T rdt = 0;
+  if (p && stage == 2) rdt = (T)1 / step_dt(d, c);
+  const T w = (T)kForceW[stage];
+  for (int e = threadIdx.x; e < ns * nx; e += kThreads) {
+    const int sl = e / nx, m = e - sl * nx;
+    T va, vb;
+    inv_unpack(R + sl * st, nx, m, dct, inv_n, va, vb);
+    const int q = dct ? makhoul_src(m, nx) : m;
+    X[(2 * sl) * st + q] = va;
+    X[(2 * sl + 1) * st + q] = vb;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 2 * ns * nx; e += kThreads) {
+    const int rr = e / nx, q = e - rr * nx;
+    if (r0 + rr >= nrows) continue;
+    const size_t idx = (size_t)(r0 + rr) * nx + q;
+    const T ps = X[rr * st + q];
+    psi[idx] = ps;
+    if (p) {
+      if (stage == 0) p[idx] = ps * w;
+      else if (stage == 1) p[idx] = p[idx] + w * ps;
+      else p[idx] = (p[idx] + w * ps) * rdt;
+    }
+  }
+}
+
+// u -= grad psi in place, boundary-normal faces untouched (core.py:582-612).
+template <typename T>
+__global__ void k_correct(const Dev3<T> *__restrict__ dp, T *u, T *v, T *w, const T *__restrict__ psi) {
+  const Dev3<T> &d = *dp;
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  const size_t nu = (size_t)nz * ny * (nx + 1), nv = (size_t)nz * (ny + 1) * nx, nw = (size_t)(nz + 1) * ny * nx;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nu + nv + nw; e += (size_t)gridDim.x * blockDim.x) {
+    if (e < nu) {
+      const int i = e % (nx + 1);
+      const size_t r = e / (nx + 1);
+      const T *ps = psi + r * nx;
+      if (d.per[0]) {
+        const int ii = i == nx ? 0 : i, im = ii == 0 ? nx - 1 : ii - 1;
+        u[e] = u[e] - (ps[ii] - ps[im]) * d.inv_h;
+      } else if (i > 0 && i < nx) {
+        u[e] = u[e] - (ps[i] - ps[i - 1]) * d.inv_h;
+      }
+    } else if (e < nu + nv) {
+      const size_t f = e - nu;
+      const int i = f % nx;
+      const int j = (f / nx) % (ny + 1);
+      const size_t k = f / ((size_t)nx * (ny + 1));
+      const T *ps = psi + k * ny * nx + i;
+      if (d.per[1]) {
+        const int jj = j == ny ? 0 : j, jm = jj == 0 ? ny - 1 : jj - 1;
+        v[f] = v[f] - (ps[(size_t)jj * nx] - ps[(size_t)jm * nx]) * d.inv_h;
+      } else if (j > 0 && j < ny) {
+        v[f] = v[f] - (ps[(size_t)j * nx] - ps[(size_t)(j - 1) * nx]) * d.inv_h;
+      }
+    } else {
+      const size_t f = e - nu - nv;
+      const size_t pl = (size_t)ny * nx;
+      const int k = (int)(f / pl);
+      const size_t r = f % pl;
+      const T *ps = psi + r;
+      if (d.per[2]) {
+        const int kk = k == nz ? 0 : k, km = kk == 0 ? nz - 1 : kk - 1;
+        w[f] = w[f] - (ps[kk * pl] - ps[km * pl]) * d.inv_h;
+      } else if (k > 0 && k < nz) {
+        w[f] = w[f] - (ps[k * pl] - ps[(k - 1) * pl]) * d.inv_h;
+      }
+    }
+  }
+}
+
+}  // namespace ns3d
+}  // namespace aero
+
+// ====================================================================== host side
+using namespace aero;
+using namespace aero::ns3d;
+
+struct aero3d_plan {
+  aero3d_desc desc;
+  int dbl;
+  size_t esize;
+  size_t bytes = 0;
+  std::vector<void *> allocs;
+  void *dev = nullptr;
+  void *uA, *vA, *wA, *uB, *vB, *wB, *F, *psi, *acc_u, *acc_v, *acc_w, *pmax, *pout, *pforce, *red;
+  void *tw[3], *cs[3];
+  double *lam[3];
+  FftPlan fp[3];
+  int nbx, nby, nbz, nblk, nwt, nq;
+  int ns1, ns2, ns3, ns5;
+  size_t lds1, lds2, lds3, lds5;
+};
+
+namespace {
+
+int alloc3(aero3d_plan *p, void **ptr, size_t n) {
+  *ptr = nullptr;
+  if (n == 0) return AERO_OK;
+  AERO_HIP_CHECK(hipMalloc(ptr, n));
+  p->allocs.push_back(*ptr);
+  p->bytes += n;
+  AERO_HIP_CHECK(hipMemset(*ptr, 0, n));
+  return AERO_OK;
+}
+
+template <typename T>
+int tile_dims(int &tx, int &ty, int &tz) {
+  tx = Tile<T>::X;
+  ty = Tile<T>::Y;
+  tz = Tile<T>::Z;
+  return 0;
+}
+
+template <typename T>
+int upload3(aero3d_plan *p) {
+  const aero3d_desc &s = p->desc;
+  const int n3[3] = {s.nx, s.ny, s.nz};
+  const int per[3] = {s.periodic_x, s.periodic_y, s.periodic_z};
+  for (int a = 0; a < 3; ++a) {
+    const int n = n3[a];
+    std::vector<cplx<T>> tw(n), cs(n);
+    std::vector<double> lam(n);
+    const double hh = s.h * s.h;
+    for (int m = 0; m < n; ++m) {
+      const double ang = -2.0 * M_PI * (double)m / (double)n;
+      tw[m] = {(T)std::cos(ang), (T)std::sin(ang)};
+      const double th = M_PI * (double)m / (2.0 * (double)n);
+      cs[m] = {(T)std::cos(th), (T)std::sin(th)};
+      lam[m] = per[a] ? (2.0 * std::cos(2.0 * M_PI * (double)m / (double)n) - 2.0) / hh
+                      : (2.0 * std::cos(M_PI * (double)m / (double)n) - 2.0) / hh;
+    }
+    AERO_HIP_CHECK(hipMemcpy(p->tw[a], tw.data(), n * sizeof(cplx<T>), hipMemcpyHostToDevice));
+    AERO_HIP_CHECK(hipMemcpy(p->cs[a], cs.data(), n * sizeof(cplx<T>), hipMemcpyHostToDevice));
+    AERO_HIP_CHECK(hipMemcpy(p->lam[a], lam.data(), n * sizeof(double), hipMemcpyHostToDevice));
+  }
+  Dev3<T> d;
+  std::memset(&d, 0, sizeof(d));
+  d.nx = s.nx;
+  d.ny = s.ny;
+  d.nz = s.nz;
+  for (int a = 0; a < 3; ++a) d.per[a] = per[a];
+  d.closed = s.closed;
+  d.outlet = !s.periodic_x && !s.closed;
+  d.upwind = s.advection == 1;
+  d.turb = s.turbulence;
+  d.fixed_dt = s.fixed_dt;
+  d.nbx = p->nbx;
+  d.nby = p->nby;
+  d.nbz = p->nbz;
+  d.nblk = p->nblk;
+  d.xcd_chunk = (p->nblk % 8 == 0 && p->nblk >= 64) ? p->nblk / 8 : 0;
+  d.win = s.win_k1 > s.win_k0 && s.win_j1 > s.win_j0 && s.win_i1 > s.win_i0 && s.nbody > 0;
+  d.wk0 = s.win_k0;
+  d.wk1 = s.win_k1;
+  d.wj0 = s.win_j0;
+  d.wj1 = s.win_j1;
+  d.wi0 = s.win_i0;
+  d.wi1 = s.win_i1;
+  d.wz = d.win ? s.win_k1 - s.win_k0 : 0;
+  d.wy = d.win ? s.win_j1 - s.win_j0 : 0;
+  d.wx = d.win ? s.win_i1 - s.win_i0 : 0;
+  int tx, ty, tz;
+  tile_dims<T>(tx, ty, tz);
+  if (d.win) {
+    d.wbx0 = s.win_i0 / tx;
+    d.wby0 = s.win_j0 / ty;
+    d.wbz0 = s.win_k0 / tz;
+    d.nwbx = std::min(s.win_i1 / tx, p->nbx - 1) - d.wbx0 + 1;
+    d.nwby = std::min(s.win_j1 / ty, p->nby - 1) - d.wby0 + 1;
+    d.nwbz = std::min(s.win_k1 / tz, p->nbz - 1) - d.wbz0 + 1;
+    d.nwt = d.nwbx * d.nwby * d.nwbz;
+    d.nq = 9 * s.nbody;
+  }
+  d.nbody = d.win ? s.nbody : 0;
+  d.symmetry = s.symmetry;
+  for (int q = 0; q < 18; ++q) {
+    d.rule[q] = s.ghost_kind[q];
+    d.rval2[q] = (T)(2.0 * s.ghost_value[q]);
+  }
+  const double h = s.h;
+  d.h = (T)h;
+  d.inv_h = (T)(1.0 / h);
+  d.inv_h2 = (T)(1.0 / (h * h));
+  d.kadv = (T)((s.advection == 1 ? 7.0 / 24.0 : 0.25) / h);
+  const double delta2 = h * h;
+  d.les_k = (T)(s.turbulence == 3 ? s.les_c * delta2 : s.les_c * s.les_c * delta2);
+  d.cfl_h = (T)(s.cfl * h);
+  d.visc_h2 = (T)(kViscousDt * h * h);
+  d.corr_scale = (T)(1.0 / ((double)s.nz * (double)s.ny));
+  d.neg_h3 = (T)(-(h * h * h));
+  const double q_a = 0.5 * s.ref_area;
+  d.q_a = (T)q_a;
+  d.q_a_span = (T)(q_a * s.span_ref);
+  d.q_a_len = (T)(q_a * s.length_ref);
+  d.dy = (T)(s.moment_ref[1] - s.y0);
+  d.F = (T *)p->F;
+  d.psi = (T *)p->psi;
+  d.acc_u = (T *)p->acc_u;
+  d.acc_v = (T *)p->acc_v;
+  d.acc_w = (T *)p->acc_w;
+  d.pmax = (T *)p->pmax;
+  d.pout = (double *)p->pout;
+  d.pforce = (double *)p->pforce;
+  d.red = (Red3<T> *)p->red;
+  for (int a = 0; a < 3; ++a) {
+    d.tw[a] = (const cplx<T> *)p->tw[a];
+    d.cs[a] = (const cplx<T> *)p->cs[a];
+    d.lam[a] = p->lam[a];
+    d.fp[a] = p->fp[a];
+    d.dct[a] = !per[a];
+  }
+  d.ns1 = p->ns1;
+  d.ns2 = p->ns2;
+  d.ns3 = p->ns3;
+  d.ns5 = p->ns5;
+  AERO_HIP_CHECK(hipMemcpy(p->dev, &d, sizeof(d), hipMemcpyHostToDevice));
+  return AERO_OK;
+}
+
+// largest slot count (<= cap) whose LDS use stays within 48 KB
+size_t pick_slots(int n, size_t es, int extra_reals, int cap, int &ns) {
+  const size_t c = 2 * es;   // sizeof(cplx<T>)
+  ns = cap;
+  auto need = [&](int k) { return 2 * (size_t)k * (n + 1) * c + (size_t)n * c + (size_t)extra_reals * k * (n + 1) * es + 128; };
+  while (ns > 1 && need(ns) > 48 * 1024) --ns;
+  return need(ns);
+}
+
+template <typename T>
+Call3<T> make_call(const aero3d_fields *in, const aero3d_geom *g, const aero3d_params *pr) {
+  Call3<T> c;
+  std::memset(&c, 0, sizeof(c));
+  c.u0 = (const T *)in->u;
+  c.v0 = (const T *)in->v;
+  c.w0 = (const T *)in->w;
+  c.up = c.u0;
+  c.vp = c.v0;
+  c.wp = c.w0;
+  if (g) {
+    c.chi_u = (const T *)g->chi_u;
+    c.chi_v = (const T *)g->chi_v;
+    c.chi_w = (const T *)g->chi_w;
+    c.chi_c = (const T *)g->chi_c;
+    c.a_u = (const T *)g->a_u;
+    c.a_v = (const T *)g->a_v;
+    c.a_w = (const T *)g->a_w;
+    c.k_u = (const T *)g->k_u;
+    c.k_v = (const T *)g->k_v;
+    c.k_w = (const T *)g->k_w;
+  }
+  if (pr) {
+    c.nu = (const T *)pr->nu;
+    c.eta = (const T *)pr->eta;
+    c.u_inlet = (const T *)pr->u_inlet;
+    c.dt_fixed = (const T *)pr->dt_fixed;
+  }
+  c.corr_stage = -1;
+  return c;
+}
+
+template <typename T>
+int poisson_passes(aero3d_plan *p, hipStream_t st, const Dev3<T> *dev, Call3<T> c, const T *f, int mode, int stage,
+                   T *psi, T *pr) {
+  const aero3d_desc &s = p->desc;
+  const int rows = s.ny * s.nz;
+  const int g1 = (rows + 2 * p->ns1 - 1) / (2 * p->ns1);
+  hipLaunchKernelGGL(k_p1<T>, dim3(g1), dim3(kThreads), p->lds1, st, dev, c.uo, c.vo, c.wo, f, mode, stage);
+  const dim3 g2((s.nx + 2 * p->ns2 - 1) / (2 * p->ns2), s.nz);
+  hipLaunchKernelGGL(k_py<T>, g2, dim3(kThreads), p->lds2, st, dev, 0);
+  const dim3 g3((s.nx + 2 * p->ns3 - 1) / (2 * p->ns3), s.ny);
+  hipLaunchKernelGGL(k_pz<T>, g3, dim3(kThreads), p->lds3, st, dev);
+  hipLaunchKernelGGL(k_py<T>, g2, dim3(kThreads), p->lds2, st, dev, 1);
+  const int g5 = (rows + 2 * p->ns5 - 1) / (2 * p->ns5);
+  hipLaunchKernelGGL(k_p5<T>, dim3(g5), dim3(kThreads), p->lds5, st, dev, c, psi, pr, stage);
+  AERO_HIP_CHECK(hipGetLastError());
+  return AERO_OK;
+}
+
+template <typename T>
+int step_impl(aero3d_plan *p, hipStream_t st, const aero3d_fields *in, aero3d_fields *out, const aero3d_geom *g,
+              const aero3d_params *pr, aero3d_outputs *o, bool chained) {
+  const aero3d_desc &s = p->desc;
+  const Dev3<T> *dev = (const Dev3<T> *)p->dev;
+  Call3<T> c = make_call<T>(in, g, pr);
+  const dim3 grid(p->nblk), blk(kThreads);
+  if (!s.fixed_dt && !chained) {
+    hipLaunchKernelGGL(k_final<T>, grid, blk, 0, st, dev, c, 0);
+    hipLaunchKernelGGL(k_dt<T>, dim3(1), dim3(kFinThreads), 0, st, dev, c);
+  }
+  T *ub[2] = {(T *)p->uA, (T *)p->uB}, *vb[2] = {(T *)p->vA, (T *)p->vB}, *wb[2] = {(T *)p->wA, (T *)p->wB};
+  for (int stage = 0; stage < 3; ++stage) {
+    c.stage = stage;
+    c.up = stage == 0 ? (const T *)in->u : ub[(stage - 1) & 1];
+    c.vp = stage == 0 ? (const T *)in->v : vb[(stage - 1) & 1];
+    c.wp = stage == 0 ? (const T *)in->w : wb[(stage - 1) & 1];
+    c.psi = stage == 0 ? nullptr : (const T *)p->psi;
+    c.corr_stage = stage == 0 ? -1 : stage - 1;
+    c.uo = ub[stage & 1];
+    c.vo = vb[stage & 1];
+    c.wo = wb[stage & 1];
+    hipLaunchKernelGGL(k_stage<T>, grid, blk, 0, st, dev, c);
+    const int mode = (!s.periodic_x && !s.closed) ? 2 : 1;
+    int rc = poisson_passes<T>(p, st, dev, c, nullptr, mode, stage, (T *)p->psi, (T *)out->p);
+    if (rc) return rc;
+  }
+  c.stage = 3;
+  c.up = ub[0];
+  c.vp = vb[0];
+  c.wp = wb[0];
+  c.psi = (const T *)p->psi;
+  c.corr_stage = (!s.periodic_x && !s.closed) ? 2 : -1;
+  c.uo = (T *)out->u;
+  c.vo = (T *)out->v;
+  c.wo = (T *)out->w;
+  hipLaunchKernelGGL(k_final<T>, grid, blk, 0, st, dev, c, 1);
+  Fin3<T> f;
+  std::memset(&f, 0, sizeof(f));
+  f.t_in = (const T *)in->t;
+  f.t_out = (T *)out->t;
+  f.dt_out = (T *)out->dt;
+  if (o) {
+    f.coeffs = (T *)o->coeffs;
+    f.hist = (T *)o->hist;
+    f.hist_count = (long long *)o->hist_count;
+    f.hist_cap = o->hist_cap;
+  }
+  f.next_dt = 1;
+  hipLaunchKernelGGL(k_finalize<T>, dim3(1), dim3(kFinThreads), 0, st, dev, c, f);
+  AERO_HIP_CHECK(hipGetLastError());
+  return AERO_OK;
+}
+
+template <typename T>
+int project_impl(aero3d_plan *p, hipStream_t st, void *u, void *v, void *w, void *psi_out) {
+  const Dev3<T> *dev = (const Dev3<T> *)p->dev;
+  Call3<T> c;
+  std::memset(&c, 0, sizeof(c));
+  c.uo = (T *)u;
+  c.vo = (T *)v;
+  c.wo = (T *)w;
+  c.corr_stage = -1;
+  int rc = poisson_passes<T>(p, st, dev, c, nullptr, 1, 0, (T *)p->psi, nullptr);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_correct<T>, dim3(1024), dim3(256), 0, st, dev, (T *)u, (T *)v, (T *)w, (const T *)p->psi);
+  if (psi_out) {
+    const aero3d_desc &s = p->desc;
+    AERO_HIP_CHECK(hipMemcpyAsync(psi_out, p->psi, (size_t)s.nx * s.ny * s.nz * sizeof(T), hipMemcpyDeviceToDevice, st));
+  }
+  AERO_HIP_CHECK(hipGetLastError());
+  return AERO_OK;
+}
+
+template <typename T>
+int poisson_impl(aero3d_plan *p, hipStream_t st, const void *f, void *psi) {
+  const Dev3<T> *dev = (const Dev3<T> *)p->dev;
+  Call3<T> c;
+  std::memset(&c, 0, sizeof(c));
+  c.corr_stage = -1;
+  return poisson_passes<T>(p, st, dev, c, (const T *)f, 0, 0, (T *)psi, nullptr);
+}
+
+template <typename T>
+int time_stage_impl(aero3d_plan *p, hipStream_t st, const aero3d_fields *fl, const aero3d_geom *g,
+                    const aero3d_params *pr, int stage, int iters, double *avg_ms) {
+  const Dev3<T> *dev = (const Dev3<T> *)p->dev;
+  Call3<T> c = make_call<T>(fl, g, pr);
+  c.stage = stage;
+  c.psi = stage == 0 ? nullptr : (const T *)p->psi;
+  c.corr_stage = stage == 0 ? -1 : stage - 1;
+  c.uo = (T *)p->uB;
+  c.vo = (T *)p->vB;
+  c.wo = (T *)p->wB;
+  hipEvent_t e0, e1;
+  AERO_HIP_CHECK(hipEventCreate(&e0));
+  AERO_HIP_CHECK(hipEventCreate(&e1));
+  hipLaunchKernelGGL(k_stage<T>, dim3(p->nblk), dim3(kThreads), 0, st, dev, c);
+  AERO_HIP_CHECK(hipEventRecord(e0, st));
+  for (int it = 0; it < iters; ++it) hipLaunchKernelGGL(k_stage<T>, dim3(p->nblk), dim3(kThreads), 0, st, dev, c);
+  AERO_HIP_CHECK(hipEventRecord(e1, st));
+  AERO_HIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  AERO_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  *avg_ms = (double)ms / (double)iters;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  AERO_HIP_CHECK(hipGetLastError());
+  return AERO_OK;
+}
+
+int check3_desc(const aero3d_desc *d) {
+  if (!d) { set_error("null descriptor"); return AERO_EINVAL; }
+  if (d->nx < 4 || d->ny < 4 || d->nz < 4) { set_error("the grid needs at least 4 cells per axis"); return AERO_EINVAL; }
+  if (d->nx > 4096 || d->ny > 4096 || d->nz > 4096) { set_error("at most 4096 cells per axis"); return AERO_EUNSUPPORTED; }
+  if (d->dtype != 0 && d->dtype != 1) { set_error("dtype must be 0 (float32) or 1 (float64)"); return AERO_EINVAL; }
+  if (!(d->h > 0)) { set_error("h must be positive"); return AERO_EINVAL; }
+  if (d->turbulence < 0 || d->turbulence > 3 || d->advection < 0 || d->advection > 1) {
+    set_error("unknown advection or turbulence code");
+    return AERO_EINVAL;
+  }
+  if (d->closed && d->periodic_x) { set_error("a closed domain cannot be periodic in x"); return AERO_EINVAL; }
+  for (int q = 0; q < 18; ++q)
+    if (d->ghost_kind[q] < 0 || d->ghost_kind[q] > 6) { set_error("unknown ghost rule"); return AERO_EINVAL; }
+  if (d->nbody < 0 || d->nbody > kMaxBodies) { set_error("at most 8 bodies"); return AERO_EUNSUPPORTED; }
+  const bool win = d->win_k1 > d->win_k0 && d->win_j1 > d->win_j0 && d->win_i1 > d->win_i0;
+  if (win) {
+    if (d->win_i0 < 0 || d->win_j0 < 0 || d->win_k0 < 0 || d->win_i1 > d->nx || d->win_j1 > d->ny || d->win_k1 > d->nz) {
+      set_error("body window outside the grid");
+      return AERO_EINVAL;
+    }
+    if ((d->periodic_x && (d->win_i0 == 0 || d->win_i1 == d->nx)) ||
+        (d->periodic_y && (d->win_j0 == 0 || d->win_j1 == d->ny)) ||
+        (d->periodic_z && (d->win_k0 == 0 || d->win_k1 == d->nz))) {
+      set_error("the body window reaches a periodic boundary");
+      return AERO_EUNSUPPORTED;
+    }
+  }
+  return AERO_OK;
+}
+
+int check3_step(const aero3d_plan *p, const aero3d_fields *in, const aero3d_fields *out, const aero3d_geom *g,
+                const aero3d_params *pr) {
+  if (!p || !in || !out || !pr) { set_error("null plan or argument"); return AERO_EINVAL; }
+  if (!in->u || !in->v || !in->w || !in->t || !out->u || !out->v || !out->w || !out->p || !out->t || !out->dt) {
+    set_error("missing field pointer");
+    return AERO_EINVAL;
+  }
+  if (!pr->nu || !pr->eta || !pr->u_inlet || (p->desc.fixed_dt && !pr->dt_fixed)) {
+    set_error("missing parameter pointer");
+    return AERO_EINVAL;
+  }
+  const aero3d_desc &s = p->desc;
+  const bool win = s.win_k1 > s.win_k0 && s.win_j1 > s.win_j0 && s.win_i1 > s.win_i0 && s.nbody > 0;
+  if (win && (!g || !g->chi_u || !g->chi_v || !g->chi_w || !g->chi_c || !g->a_u || !g->a_v || !g->a_w || !g->k_u ||
+              !g->k_v || !g->k_w)) {
+    set_error("missing geometry pointer");
+    return AERO_EINVAL;
+  }
+  return AERO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int aero3d_plan_create(const aero3d_desc *desc, aero3d_plan **out) {
+  if (!out) { set_error("null output pointer"); return AERO_EINVAL; }
+  *out = nullptr;
+  int rc = check3_desc(desc);
+  if (rc) return rc;
+  aero3d_plan *p = new aero3d_plan();
+  p->desc = *desc;
+  p->dbl = desc->dtype == 1;
+  p->esize = p->dbl ? 8 : 4;
+  if (!factor_radices(desc->nx, p->fp[0]) || !factor_radices(desc->ny, p->fp[1]) || !factor_radices(desc->nz, p->fp[2])) {
+    delete p;
+    set_error("nx, ny and nz must be products of 2, 3 and 5");
+    return AERO_EUNSUPPORTED;
+  }
+  const size_t es = p->esize;
+  const size_t nx = desc->nx, ny = desc->ny, nz = desc->nz;
+  int tx, ty, tz;
+  if (p->dbl) tile_dims<double>(tx, ty, tz);
+  else tile_dims<float>(tx, ty, tz);
+  p->nbx = (int)((nx + tx - 1) / tx);
+  p->nby = (int)((ny + ty - 1) / ty);
+  p->nbz = (int)((nz + tz - 1) / tz);
+  p->nblk = p->nbx * p->nby * p->nbz;
+  const bool win = desc->win_k1 > desc->win_k0 && desc->win_j1 > desc->win_j0 && desc->win_i1 > desc->win_i0 &&
+                   desc->nbody > 0;
+  const size_t wz = win ? desc->win_k1 - desc->win_k0 : 0, wy = win ? desc->win_j1 - desc->win_j0 : 0,
+               wx = win ? desc->win_i1 - desc->win_i0 : 0;
+  p->nwt = 0;
+  p->nq = win ? 9 * desc->nbody : 0;
+  if (win) {
+    const int nwbx = std::min(desc->win_i1 / tx, p->nbx - 1) - desc->win_i0 / tx + 1;
+    const int nwby = std::min(desc->win_j1 / ty, p->nby - 1) - desc->win_j0 / ty + 1;
+    const int nwbz = std::min(desc->win_k1 / tz, p->nbz - 1) - desc->win_k0 / tz + 1;
+    p->nwt = nwbx * nwby * nwbz;
+  }
+  p->lds1 = pick_slots((int)nx, es, 0, 4, p->ns1);
+  p->lds2 = pick_slots((int)ny, es, 2, 16, p->ns2);
+  p->lds3 = pick_slots((int)nz, es, 2, 16, p->ns3);
+  p->lds5 = pick_slots((int)nx, es, 2, 4, p->ns5);
+  const size_t nu_ = nz * ny * (nx + 1), nv_ = nz * (ny + 1) * nx, nw_ = (nz + 1) * ny * nx, nc = nz * ny * nx;
+  struct { void **ptr; size_t n; } tab[] = {
+      {&p->dev, sizeof(Dev3<double>)},
+      {&p->uA, nu_ * es}, {&p->vA, nv_ * es}, {&p->wA, nw_ * es}, {&p->uB, nu_ * es}, {&p->vB, nv_ * es},
+      {&p->wB, nw_ * es}, {&p->F, nc * es}, {&p->psi, nc * es},
+      {&p->acc_u, wz * wy * (wx + 1) * es}, {&p->acc_v, wz * (wy + 1) * wx * es}, {&p->acc_w, (wz + 1) * wy * wx * es},
+      {&p->pmax, (size_t)p->nblk * 4 * es}, {&p->pout, (size_t)p->nby * p->nbz * 2 * 8},
+      {&p->pforce, (size_t)std::max(1, p->nwt) * std::max(1, p->nq) * 8}, {&p->red, 256},
+      {&p->tw[0], nx * 2 * es}, {&p->tw[1], ny * 2 * es}, {&p->tw[2], nz * 2 * es},
+      {&p->cs[0], nx * 2 * es}, {&p->cs[1], ny * 2 * es}, {&p->cs[2], nz * 2 * es},
+      {(void **)&p->lam[0], nx * 8}, {(void **)&p->lam[1], ny * 8}, {(void **)&p->lam[2], nz * 8}};
+  for (auto &t : tab) {
+    if ((rc = alloc3(p, t.ptr, t.n))) {
+      aero3d_plan_destroy(p);
+      return rc;
+    }
+  }
+  rc = p->dbl ? upload3<double>(p) : upload3<float>(p);
+  if (rc) {
+    aero3d_plan_destroy(p);
+    return rc;
+  }
+  *out = p;
+  return AERO_OK;
+}
+
+int aero3d_plan_destroy(aero3d_plan *p) {
+  if (!p) return AERO_OK;
+  for (void *a : p->allocs) (void)hipFree(a);
+  delete p;
+  return AERO_OK;
+}
+
+size_t aero3d_plan_workspace_bytes(const aero3d_plan *p) { return p ? p->bytes : 0; }
+
+int aero3d_step(aero3d_plan *p, void *stream, const aero3d_fields *in, aero3d_fields *out, const aero3d_geom *g,
+                const aero3d_params *pr, aero3d_outputs *o) {
+  int rc = check3_step(p, in, out, g, pr);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  return p->dbl ? step_impl<double>(p, st, in, out, g, pr, o, false) : step_impl<float>(p, st, in, out, g, pr, o, false);
+}
+
+int aero3d_step_chained(aero3d_plan *p, void *stream, const aero3d_fields *in, aero3d_fields *out,
+                        const aero3d_geom *g, const aero3d_params *pr, aero3d_outputs *o) {
+  int rc = check3_step(p, in, out, g, pr);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  return p->dbl ? step_impl<double>(p, st, in, out, g, pr, o, true) : step_impl<float>(p, st, in, out, g, pr, o, true);
+}
+
+int aero3d_project(aero3d_plan *p, void *stream, void *u, void *v, void *w, void *psi) {
+  if (!p || !u || !v || !w) { set_error("null plan or field"); return AERO_EINVAL; }
+  hipStream_t st = (hipStream_t)stream;
+  return p->dbl ? project_impl<double>(p, st, u, v, w, psi) : project_impl<float>(p, st, u, v, w, psi);
+}
+
+int aero3d_poisson_solve(aero3d_plan *p, void *stream, const void *f, void *psi) {
+  if (!p || !f || !psi) { set_error("null plan or field"); return AERO_EINVAL; }
+  hipStream_t st = (hipStream_t)stream;
+  return p->dbl ? poisson_impl<double>(p, st, f, psi) : poisson_impl<float>(p, st, f, psi);
+}
+
+int aero3d_time_stage(aero3d_plan *p, void *stream, const aero3d_fields *fields, const aero3d_geom *g,
+                      const aero3d_params *pr, int stage, int iters, double *avg_ms) {
+  if (!p || !fields || !pr || !avg_ms || stage < 0 || stage > 2 || iters < 1) {
+    set_error("bad aero3d_time_stage arguments");
+    return AERO_EINVAL;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  return p->dbl ? time_stage_impl<double>(p, st, fields, g, pr, stage, iters, avg_ms)
+                : time_stage_impl<float>(p, st, fields, g, pr, stage, iters, avg_ms);
+}
+
+}  // extern "C"

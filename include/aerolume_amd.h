@@ -18,6 +18,14 @@
  *                          called as ns2d/solver.py:384-400 _step_static does
  *   aero2d_project       - ns2d/core.py:626-642 project
  *   aero2d_poisson_solve - ns2d/poisson.py:128-134 PoissonSolver.solve
+ *   aero3d_plan_create   - ns3d/core.py:184-203 make_operators3d (static operators,
+ *                          ghost-rule table core.py:147-181, PoissonSolver3D
+ *                          ns3d/poisson.py:107-126)
+ *   aero3d_step          - ns3d/core.py:753-762 step3d_full (one SSP-RK3 step with
+ *                          projection, penalisation and the six coefficients), called
+ *                          as ns3d/solver.py:583-595 _step_fn dispatches the fused path
+ *   aero3d_project       - ns3d/core.py:604-612 project3d
+ *   aero3d_poisson_solve - ns3d/poisson.py:128-157 PoissonSolver3D.solve
  */
 #ifndef AEROLUME_AMD_H
 #define AEROLUME_AMD_H
@@ -147,6 +155,87 @@ int aero2d_poisson_solve(aero2d_plan *plan, void *stream, const void *f, void *p
  * Synchronises the stream. Call after at least one aero2d_step (it reuses that step's dt). */
 int aero2d_time_stage(aero2d_plan *plan, void *stream, const aero2d_fields *fields, const aero2d_geom *geom,
                       const aero2d_params *params, int stage, int iters, double *avg_ms);
+
+/* ------------------------------------------------------------------ 3D */
+/* ghost-rule kinds (ns3d/core.py:147-181, 223-272) */
+#define AERO3D_GHOST_EDGE 0          /* repeat the boundary value (inlet, outlet) */
+#define AERO3D_GHOST_EVEN 1          /* mirror across a wall half a cell away (zero gradient) */
+#define AERO3D_GHOST_ODD 2           /* mirror with sign change (zero value) */
+#define AERO3D_GHOST_AFFINE 3        /* mirror to a prescribed value: 2 value - x */
+#define AERO3D_GHOST_FACE_ODD 4      /* wall-normal component across a wall on the face */
+#define AERO3D_GHOST_PERIODIC_CELL 5
+#define AERO3D_GHOST_PERIODIC_FACE 6 /* face arrays carry a duplicate last face */
+#define AERO3D_MAX_BODIES 8
+
+typedef struct aero3d_plan aero3d_plan;
+
+/* Static description of a 3D solve (ns3d/config.py:528-731 after set-up). */
+typedef struct {
+  int32_t nx, ny, nz;       /* cells; each a product of 2, 3 and 5, at most 4096 */
+  int32_t dtype;            /* 0 = float32, 1 = float64 */
+  int32_t periodic_x, periodic_y, periodic_z;
+  int32_t closed;           /* walls at x0 and x1 (no inflow) */
+  int32_t advection;        /* 0 = central, 1 = upwind3 */
+  int32_t turbulence;       /* 0 none, 1 smagorinsky, 2 wale, 3 vreman */
+  int32_t fixed_dt;         /* 1: dt from params.dt_fixed, else the CFL / viscous rule */
+  int32_t win_k0, win_k1, win_j0, win_j1, win_i0, win_i1;   /* body cell window (empty: k1 <= k0) */
+  int32_t nbody;
+  int32_t symmetry;         /* y = y0 symmetry plane: full-body coefficients */
+  int32_t ghost_kind[18];   /* AERO3D_GHOST_* per component (u, v, w), axis (x, y, z), side (lo, hi) */
+  double ghost_value[18];   /* value of the AFFINE rule (moving floor speed) */
+  double h, y0;
+  double cfl, les_c;
+  double ref_area, length_ref, span_ref;
+  double moment_ref[3];
+} aero3d_desc;
+
+/* Caller-owned device buffers of the plan's dtype: u (nz, ny, nx+1), v (nz, ny+1, nx),
+ * w (nz+1, ny, nx), p (nz, ny, nx), t and dt 0-d. in == out is allowed. */
+typedef struct {
+  void *u, *v, *w, *p, *t, *dt;
+} aero3d_fields;
+
+/* Window arrays (ns3d/geometry.py:275-303), wz = k1-k0, wy = j1-j0, wx = i1-i0:
+ * chi_u, a_u (wz, wy, wx+1); chi_v, a_v (wz, wy+1, wx); chi_w, a_w (wz+1, wy, wx);
+ * chi_c (wz, wy, wx); k_u (nbody, 3, wz, wy, wx+1) = [Fx, My, Mz] weights;
+ * k_v (nbody, 3, ...) = [Fy, Mx, Mz]; k_w (nbody, 3, ...) = [Fz, Mx, My]. */
+typedef struct {
+  const void *chi_u, *chi_v, *chi_w, *chi_c, *a_u, *a_v, *a_w, *k_u, *k_v, *k_w;
+} aero3d_geom;
+
+/* Live device scalars / arrays: nu, eta, dt_fixed 0-d; u_inlet (nz, ny). */
+typedef struct {
+  const void *nu, *eta, *u_inlet, *dt_fixed;
+} aero3d_params;
+
+/* Per-step outputs (device; any may be NULL): coeffs (nbody, 6) = [CD, CS, CL, C_roll, Cm, Cn];
+ * hist: ring of hist_cap rows [t, CD, CS, CL, C_roll, Cm, Cn] (summed over bodies) written at
+ * row (*hist_count mod hist_cap), then *hist_count += 1 (int64 device counter). */
+typedef struct {
+  void *coeffs;
+  void *hist, *hist_count;
+  int64_t hist_cap;
+} aero3d_outputs;
+
+int aero3d_plan_create(const aero3d_desc *desc, aero3d_plan **plan);
+int aero3d_plan_destroy(aero3d_plan *plan);
+size_t aero3d_plan_workspace_bytes(const aero3d_plan *plan);
+int aero3d_step(aero3d_plan *plan, void *stream, const aero3d_fields *in, aero3d_fields *out,
+                const aero3d_geom *geom, const aero3d_params *params, aero3d_outputs *outputs);
+/* Same step, reusing the dt inputs (max |u|, |v|, |w|, nu_t) that the previous step of this
+ * plan computed from its output state; valid only when `in` is that output, unmodified. */
+int aero3d_step_chained(aero3d_plan *plan, void *stream, const aero3d_fields *in, aero3d_fields *out,
+                        const aero3d_geom *geom, const aero3d_params *params, aero3d_outputs *outputs);
+/* u, v, w in place: u - grad psi with lap psi = div u; psi (nz, ny, nx) optional. */
+int aero3d_project(aero3d_plan *plan, void *stream, void *u, void *v, void *w, void *psi);
+/* psi = L^-1 f with zero mean (f, psi (nz, ny, nx); may alias). */
+int aero3d_poisson_solve(aero3d_plan *plan, void *stream, const void *f, void *psi);
+/* Timing helper (no reference counterpart; bench.py prices the dominant kernel with it):
+ * launches the stage kernel of RK stage `stage` `iters` times on `stream`, reading `fields`
+ * as stage input and step start and writing only plan scratch, between two HIP events;
+ * returns the average launch time in ms. Synchronises the stream. Call after one step. */
+int aero3d_time_stage(aero3d_plan *plan, void *stream, const aero3d_fields *fields, const aero3d_geom *geom,
+                      const aero3d_params *params, int stage, int iters, double *avg_ms);
 
 const char *aero_last_error(void);
 int aero_version(void);

@@ -48,8 +48,39 @@ def test_plan_create_rejects_bad_descriptors():
         _lib.check(rc, "plan")
 
 
+def test_plan3d_create_rejects_bad_descriptors():
+    from aerolume_amd import _lib
+    from aerolume_amd._lib3d import Desc3D
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    d = Desc3D()
+    d.nx, d.ny, d.nz, d.h = 8, 8, 2, 0.1
+    assert lib.aero3d_plan_create(ctypes.byref(d), ctypes.byref(h)) == _lib.AERO_EINVAL
+    d.nz = 14                # 2 * 7: not 5-smooth
+    assert lib.aero3d_plan_create(ctypes.byref(d), ctypes.byref(h)) == _lib.AERO_EUNSUPPORTED
+    d.nz = 8
+    d.ghost_kind[3] = 9      # unknown ghost rule
+    assert lib.aero3d_plan_create(ctypes.byref(d), ctypes.byref(h)) == _lib.AERO_EINVAL
+    d.ghost_kind[3] = 1
+    d.periodic_y = 1         # a body window on the periodic boundary
+    d.win_k0, d.win_k1, d.win_j0, d.win_j1, d.win_i0, d.win_i1, d.nbody = 1, 4, 0, 4, 1, 4, 1
+    assert lib.aero3d_plan_create(ctypes.byref(d), ctypes.byref(h)) == _lib.AERO_EUNSUPPORTED
+    assert b"periodic" in lib.aero_last_error()
+
+
 def test_solver_refuses_cpu_device():
     from aerolume_amd.ns2d import solver as S
     from tests import cases
     with pytest.raises(RuntimeError, match="GPU"):
         S.make_operators(cases.trace_cfg("float64", device="cpu"))
+    from aerolume_amd.ns3d import solver as S3
+    with pytest.raises(RuntimeError, match="GPU"):
+        S3.make_operators3d(cases.cfg3d("laminar", device="cpu"))
+
+
+def test_3d_unsupported_features_fail_loudly():
+    from aerolume_amd.ns3d import setup as su
+    from aerolume_amd.ns3d.config import SolverConfig3D
+    cfg = SolverConfig3D(nx=8, ny=8, nz=8, h=0.1, x0=0, y0=0, z0=0, re=100.0)    # smoke on by default
+    with pytest.raises(NotImplementedError, match="smoke"):
+        su.check_supported(cfg)

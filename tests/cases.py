@@ -116,3 +116,16 @@ def c3_cfg(dtype="float32", device="cuda", nx=256, ny=128, nz=128):
                           ground_speed=1.0, bc_y="periodic", bc_zmax="slip", advection="upwind3", cfl=0.5,
                           ref_area=1.92, length_ref=4.0, span_ref=1.6, moment_ref=(0.0, 0.0, 0.0),
                           dtype=dtype, device=device, dye=False, perturb=False)
+
+
+def box_mesh(center, size):
+    """Triangulated axis-aligned box (12 outward-wound triangles)."""
+    cx, cy, cz = center
+    hx, hy, hz = (0.5 * s for s in size)
+    v = np.array([[cx + sx * hx, cy + sy * hy, cz + sz * hz] for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)])
+    # quads of the corners (index = 4 sx + 2 sy + sz with s in {0, 1})
+    quads = [(0, 1, 3, 2), (4, 6, 7, 5), (0, 4, 5, 1), (2, 3, 7, 6), (0, 2, 6, 4), (1, 5, 7, 3)]
+    f = []
+    for a, b, c, d in quads:
+        f += [(a, b, c), (a, c, d)]
+    return v, np.array(f, dtype=np.int64)

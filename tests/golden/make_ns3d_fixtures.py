@@ -40,7 +40,7 @@ from aerolume.solvers.ns3d.geometry import Geometry3D as RGeometry  # noqa: E402
 from aerolume.solvers.ns3d.geometry import build_geometry3d as rbuild  # noqa: E402
 
 from aerolume_amd.ns3d import setup as su  # noqa: E402
-from tests.cases import CASES3D, GRID3D, SEED3D, cfg3d  # noqa: E402
+from tests.cases import CASES3D, GRID3D, SEED3D, box_mesh, cfg3d  # noqa: E402
 
 GEOM_KEYS = ("chi_u", "chi_v", "chi_w", "chi_c", "a_u", "a_v", "a_w", "k_u", "k_v", "k_w")
 
@@ -103,18 +103,6 @@ def run_case(name, dtype, steps, keep):
     out["t"] = np.array(ts)
     out["coeffs"] = np.stack(coeffs)
     return out
-
-
-def box_mesh(center, size):
-    cx, cy, cz = center
-    hx, hy, hz = (0.5 * s for s in size)
-    v = np.array([[cx + sx * hx, cy + sy * hy, cz + sz * hz] for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)])
-    # outward-wound quads of the unit cube corners (index = 4 sx + 2 sy + sz, s in {0, 1})
-    quads = [(0, 1, 3, 2), (4, 6, 7, 5), (0, 4, 5, 1), (2, 3, 7, 6), (0, 2, 6, 4), (1, 5, 7, 3)]
-    f = []
-    for a, b, c, d in quads:
-        f += [(a, b, c), (a, c, d)]
-    return v, np.array(f, dtype=np.int64)
 
 
 MASK_BOXES = (((0.2, 0.0, 0.9), (0.8, 0.6, 0.5), 0.0, (0.0, 0.0, 0.0)),

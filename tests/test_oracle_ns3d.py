@@ -68,7 +68,7 @@ def test_mask_builder_matches_reference_on_meshes(golden):
     against this package's builder fed the same meshes, and fed the exact box SDFs."""
     from aerolume_amd.ns3d import setup as su
     from aerolume_amd.ns3d.config import AnalyticBody, ImmersedMesh
-    from tests.golden.make_ns3d_fixtures import box_mesh
+    from tests.cases import box_mesh
     fx = golden("ns3d_masks.npz")
     cfg = cases.cfg3d("laminar")
     ops = su.make_operators3d(cfg, device="cpu")

@@ -299,7 +299,7 @@ __device__ __forceinline__ T eddy3(int turb, T les_k, T g[3][3]) {
     const T sdc = sd > (T)0 ? sd : (T)0, ssc = ss > (T)0 ? ss : (T)0;
     const T rs = sqrt(sdc);
     const T num = sdc * rs;                                     // sd^1.5
-    const T den = (ssc * ssc * sqrt(ssc) + rs * sqrt(rs)) + (T)1e-30;   // ss^2.5 + sd^1.25
+    const T den = (ssc * ssc * sqrt(ssc) + sdc * sqrt(rs)) + (T)1e-30;   // ss^2.5 + sd^1.25
     return (les_k * num) / den;
   }
   T b[3][3];

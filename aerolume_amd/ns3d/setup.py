@@ -84,7 +84,10 @@ def resolve_device_dtype(cfg: SolverConfig3D, device=None) -> tuple[torch.device
     dev = device if device is not None else cfg.device
     if dev in ("auto", None, ""):
         dev = "cuda" if torch.cuda.is_available() else "cpu"
-    return torch.device(dev), (torch.float64 if cfg.dtype == "float64" else torch.float32)
+    dev = torch.device(dev)
+    if dev.type == "cuda" and dev.index is None and torch.cuda.is_available():
+        dev = torch.device("cuda", torch.cuda.current_device())
+    return dev, (torch.float64 if cfg.dtype == "float64" else torch.float32)
 
 
 def check_supported(cfg: SolverConfig3D) -> None:

@@ -10,6 +10,11 @@ of 10 steps). ``value`` is Mcell-steps/s = 524288 cells x steps / s / 1e6.
 Also reported: ``roofline`` of the dominant kernel (the fused stage kernel; algorithmic bytes
 per launch / its average duration timed with HIP events on the solver's stream) and
 ``cpu_baseline`` (the CPU oracle on the host's cores, a bounded sample of the same case).
+
+``workloads.3d_car`` carries the same figures for the 3D car case at 256x128x128 (4.19M
+cells, FP32): a 4 x 1.6 x 1.2 rounded box on four rolling wheels over a floor moving at U,
+Re 5e4, WALE, periodic in y, upwind3 + SSP-RK3 at CFL 0.5, exact DCT/DHT projection;
+seeded 1 % Gaussian velocity noise on the inflow state; ``FlowSolver3D.advance``.
 """
 from __future__ import annotations
 
@@ -34,6 +39,12 @@ SEED = 202
 # of the stage (3 x 4 B)
 STAGE_BYTES_PER_CELL = 40
 STEP_BYTES_PER_CELL = 300     # whole fused step of this case (byte model of the design notes)
+NX3, NY3, NZ3 = 256, 128, 128
+SEED3 = 303
+# 3D stage kernel at stages 2-3 (FP32): read u^, v^, w^ of the previous stage, psi, and u0,
+# v0, w0 of the step start (7 x 4 B), write u^, v^, w^ (3 x 4 B)
+STAGE3_BYTES_PER_CELL = 40
+STEP3_BYTES_PER_CELL = 310
 
 
 def log(msg: str) -> None:
@@ -108,6 +119,78 @@ def gpu_run(steps: int, warmup: int):
     return solver, wall, ms.value
 
 
+def gpu_run_3d(steps: int, warmup: int):
+    import torch
+
+    from aerolume_amd.ns3d import solver as S
+    from tests.cases import c3_cfg
+    cfg = c3_cfg("float32", device="cuda")
+    solver = S.FlowSolver3D(cfg)
+    ops = solver.ops
+    u, v, w = noisy_velocity3d(ops, SEED3)
+    dev = ops.device
+    u, v, w = S.apply_boundary_values(torch.as_tensor(u, device=dev, dtype=ops.dtype),
+                                      torch.as_tensor(v, device=dev, dtype=ops.dtype),
+                                      torch.as_tensor(w, device=dev, dtype=ops.dtype), ops)
+    u, v, w, _ = S.project3d(u.contiguous(), v.contiguous(), w.contiguous(), ops)
+    solver.set_velocity(u, v, w)
+    log(f"bench3d: workspace {solver.workspace_bytes / 2**20:.1f} MiB, window {solver.geometry.window}")
+    solver.advance(warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solver.advance(steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    st = solver.state
+    if not all(bool(torch.isfinite(t).all()) for t in (st.u, st.v, st.w, st.p)):
+        raise RuntimeError("non-finite 3D state after the timed steps")
+    stage_ms = solver._plan.time_stage(solver._fields, 1, 100)
+    torch.cuda.synchronize()
+    return solver, wall, stage_ms
+
+
+def noisy_velocity3d(ops, seed):
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    u = 1.0 + 0.01 * rng.standard_normal((ops.nz, ops.ny, ops.nx + 1))
+    v = 0.01 * rng.standard_normal((ops.nz, ops.ny + 1, ops.nx))
+    w = 0.01 * rng.standard_normal((ops.nz + 1, ops.ny, ops.nx))
+    return u, v, w
+
+
+def cpu_baseline_3d(seconds: float = 20.0):
+    """The CPU oracle (torch, FP32, all host threads, flush-to-zero) on the 3D car case: one
+    warm-up step, then steps while they fit in about ``seconds``, best per-step time."""
+    import torch
+
+    from aerolume_amd.ns3d import setup as su
+    from oracle import ns3d as O
+    from tests.cases import c3_cfg
+    cfg = c3_cfg("float32", device="cpu")
+    ops = su.make_operators3d(cfg, device="cpu")
+    geom = su.build_geometry3d(cfg.bodies, ops)
+    u, v, w = (torch.as_tensor(a, dtype=ops.dtype) for a in noisy_velocity3d(ops, SEED3))
+    u, v, w = O.apply_boundary_values(u, v, w, ops)
+    u, v, w, _ = O.project3d(u, v, w, ops)
+    z = torch.zeros((), dtype=ops.dtype)
+    st = O.State3D(u, v, w, torch.zeros(ops.nz, ops.ny, ops.nx, dtype=ops.dtype), z.clone(), z.clone())
+    threads = torch.get_num_threads()
+    torch.set_flush_denormal(True)
+    st, _ = O.step3d_full(st, geom, ops)
+    n, t_total, best = 0, 0.0, float("inf")
+    while t_total < seconds and n < 20:
+        t0 = time.perf_counter()
+        st, _ = O.step3d_full(st, geom, ops)
+        dt = time.perf_counter() - t0
+        best = min(best, dt)
+        t_total += dt
+        n += 1
+    cells = NX3 * NY3 * NZ3
+    return {"value": round(cells / best / 1e6, 3), "unit": "Mcell-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{n} steps of the 256x128x128 car case (FP32) after 1 warm-up step, best step of "
+                      f"{t_total:.1f} s, torch CPU oracle, {threads} threads, flush-to-zero"}
+
+
 def cpu_baseline(seconds: float = 15.0):
     """The CPU oracle (torch, FP32, all host threads, flush-to-zero) on the same case:
     warm-up step, then as many steps as fit in about ``seconds``, best per-step time."""
@@ -147,6 +230,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--steps3d", type=int, default=500)
+    ap.add_argument("--warmup3d", type=int, default=10)
+    ap.add_argument("--no-3d", action="store_true", help="skip the 3D car workload")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -177,6 +263,31 @@ def main() -> int:
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
         line["gpu_over_cpu"] = round(value / line["cpu_baseline"]["value"], 1)
+    if not args.no_3d:
+        del solver
+        s3, wall3, stage3_ms = gpu_run_3d(args.steps3d, args.warmup3d)
+        cells3 = NX3 * NY3 * NZ3
+        v3 = cells3 * args.steps3d / wall3 / 1e6
+        sb3 = STAGE3_BYTES_PER_CELL * cells3
+        ach3 = sb3 / (stage3_ms * 1e-3) / 1e9
+        step3 = STEP3_BYTES_PER_CELL * cells3 / (wall3 / args.steps3d) / 1e9
+        w3 = {"value": round(v3, 2), "unit": "Mcell-steps/s", "steps": args.steps3d, "warmup": args.warmup3d,
+              "ms_per_step": round(wall3 / args.steps3d * 1e3, 5), "dtype": "f32", "data": "synthetic",
+              "config": {"workload": "3D 256x128x128 car (4 x 1.6 x 1.2 rounded box on 4 rolling wheels), "
+                                     "moving floor, periodic y, Re 5e4, WALE, upwind3 + SSP-RK3 CFL 0.5, "
+                                     "DCT/DHT projection", "nx": NX3, "ny": NY3, "nz": NZ3, "cells": cells3,
+                         "seed": SEED3, "graph_steps": 10},
+              "roofline": {"bound": "hbm", "kernel": "k_stage 3D (stage 2 of 3)", "achieved": round(ach3, 1),
+                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach3 / HBM_PEAK_GBS, 4),
+                           "traffic": None, "avg_launch_us": round(stage3_ms * 1e3, 3),
+                           "algorithmic_bytes_per_launch": sb3},
+              "step_hbm": {"algorithmic_bytes_per_step": STEP3_BYTES_PER_CELL * cells3,
+                           "achieved_gbs": round(step3, 1), "frac": round(step3 / HBM_PEAK_GBS, 4)}}
+        del s3
+        if not args.no_cpu:
+            w3["cpu_baseline"] = cpu_baseline_3d()
+            w3["gpu_over_cpu"] = round(v3 / w3["cpu_baseline"]["value"], 1)
+        line["workloads"] = {"3d_car": w3}
     print(json.dumps(line), flush=True)
     return 0
 

@@ -38,7 +38,9 @@
 namespace aero {
 namespace ns3d {
 
-constexpr int kThreads = 256;
+constexpr int kThreads = 256;        // Poisson passes
+constexpr int kTileThreads = 1024;   // stage / final tiles (16 waves)
+constexpr int kTileWaves = kTileThreads / 64;
 constexpr int kFinThreads = 1024;
 constexpr int kMaxBodies = 8;
 constexpr int kMaxQ = 9 * kMaxBodies;      // force sums per body: u (Fx, My, Mz), v (Fy, Mx, Mz), w (Fz, Mx, My)
@@ -46,16 +48,19 @@ constexpr double kSsp[3][2] = {{0.0, 1.0}, {0.75, 0.25}, {1.0 / 3.0, 2.0 / 3.0}}
 constexpr double kForceW[3] = {1.0 / 6.0, 2.0 / 3.0, 1.0};
 constexpr double kViscousDt = 1.0 / 6.0;   // ns3d/core.py:51
 
-// cells per stage tile (x, y, z); 256 threads per tile
+// cells per stage tile (x, y, z), 1024 threads per tile: 80 KB (float) of LDS per tile, so two
+// tiles (32 waves) fit a CU's 160 KB
 template <typename T>
 struct Tile;
 template <>
 struct Tile<float> {
-  static constexpr int X = 32, Y = 8, Z = 4;
+  static constexpr int X = 32, Y = 8, Z = 8;
+  static constexpr int WAVES_PER_SIMD = 8;    // two tiles per CU: registers capped to fit
 };
 template <>
 struct Tile<double> {
-  static constexpr int X = 16, Y = 8, Z = 4;
+  static constexpr int X = 16, Y = 8, Z = 8;
+  static constexpr int WAVES_PER_SIMD = 4;    // one tile per CU (LDS-bound)
 };
 
 template <typename T>
@@ -114,7 +119,7 @@ struct Lds3 {
   T v[VZ][VY][VX];
   T w[WZ][WY][WX];
   T nu[NZ][NY][NX];
-  double red[4 * 16];
+  double red[kTileWaves * 16];
 };
 
 // local (tile-relative) accessors: faces / cells from -2 (halo) on
@@ -156,91 +161,140 @@ __device__ __forceinline__ int ghost_axis(int code, int g, int n, int &op) {
   }
 }
 
-// Stage-input velocity of component comp at an in-range position: u^ - grad psi with the
-// outlet shift (the previous stage's projected field, core.py:281-297, 582-612).
-template <typename T>
-__device__ __forceinline__ T load_up(const Dev3<T> &d, const Call3<T> &c, int comp, int k, int j, int i, T corr) {
-  const int nx = d.nx, ny = d.ny, nz = d.nz;
-  if (comp == 0) {
-    T x = c.up[((size_t)k * ny + j) * (nx + 1) + i];
-    if (c.psi) {
-      const T *ps = c.psi + ((size_t)k * ny + j) * nx;
-      if (d.per[0]) {
-        const int ii = i == nx ? 0 : i, im = ii == 0 ? nx - 1 : ii - 1;
-        x = x - (ps[ii] - ps[im]) * d.inv_h;
-      } else if (i > 0 && i < nx) {
-        x = x - (ps[i] - ps[i - 1]) * d.inv_h;
-      }
+// Block sum of N values per thread over all waves of the block (fixed order); thread 0 gets
+// the result. lds holds (waves) x N doubles.
+template <int N>
+__device__ __forceinline__ void block_sum_w(double (&v)[N], double *lds) {
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int q = 0; q < N; ++q) v[q] = wave_sum(v[q]);
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < N; ++q) lds[w * N + q] = v[q];
+  __syncthreads();
+  if (tid == 0)
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+      double r = lds[q];
+      for (int i = 1; i < nw; ++i) r += lds[i * N + q];
+      v[q] = r;
     }
-    if (c.corr_stage >= 0 && d.outlet && i == nx) x = x + corr;
-    return x;
-  }
-  if (comp == 1) {
-    T x = c.vp[((size_t)k * (ny + 1) + j) * nx + i];
-    if (c.psi) {
-      const T *ps = c.psi + (size_t)k * ny * nx + i;
-      if (d.per[1]) {
-        const int jj = j == ny ? 0 : j, jm = jj == 0 ? ny - 1 : jj - 1;
-        x = x - (ps[(size_t)jj * nx] - ps[(size_t)jm * nx]) * d.inv_h;
-      } else if (j > 0 && j < ny) {
-        x = x - (ps[(size_t)j * nx] - ps[(size_t)(j - 1) * nx]) * d.inv_h;
-      }
-    }
-    return x;
-  }
-  T x = c.wp[((size_t)k * ny + j) * nx + i];
-  if (c.psi) {
-    const size_t pl = (size_t)ny * nx;
-    const T *ps = c.psi + (size_t)j * nx + i;
-    if (d.per[2]) {
-      const int kk = k == nz ? 0 : k, km = kk == 0 ? nz - 1 : kk - 1;
-      x = x - (ps[kk * pl] - ps[km * pl]) * d.inv_h;
-    } else if (k > 0 && k < nz) {
-      x = x - (ps[k * pl] - ps[(k - 1) * pl]) * d.inv_h;
-    }
-  }
-  return x;
 }
 
-// Ghost-padded value (pad3: the x rule, then y on the x-padded array, then z).
-template <typename T>
-__device__ T padded(const Dev3<T> &d, const Call3<T> &c, int comp, int gk, int gj, int gi, T corr) {
-  const int ex = d.nx + (comp == 0), ey = d.ny + (comp == 1), ez = d.nz + (comp == 2);
-  gi = gi > ex + 1 ? ex + 1 : gi;   // beyond the two ghost layers: never used, kept in bounds
-  gj = gj > ey + 1 ? ey + 1 : gj;
-  gk = gk > ez + 1 ? ez + 1 : gk;
-  const int *rule = d.rule + comp * 6;
-  const T *rv = d.rval2 + comp * 6;
-  int ox = 0, oy = 0, oz = 0, sx = 0, sy = 0, sz = 0;
-  if (gi < 0 || gi >= ex) { sx = gi < 0 ? 0 : 1; gi = ghost_axis(rule[sx], gi, ex, ox); ox += 1; }
-  if (gj < 0 || gj >= ey) { sy = gj < 0 ? 0 : 1; gj = ghost_axis(rule[2 + sy], gj, ey, oy); oy += 1; }
-  if (gk < 0 || gk >= ez) { sz = gk < 0 ? 0 : 1; gk = ghost_axis(rule[4 + sz], gk, ez, oz); oz += 1; }
-  T x = load_up(d, c, comp, gk, gj, gi, corr);
-  if (ox == 2) x = -x;
-  else if (ox == 3) x = rv[sx] - x;
-  if (oy == 2) x = -x;
-  else if (oy == 3) x = rv[2 + sy] - x;
-  if (oz == 2) x = -x;
-  else if (oz == 3) x = rv[4 + sz] - x;
-  return x;
+// Source index of position g along an axis of array extent ext (n cells): periodic axes wrap
+// (a face axis carries the duplicate face n, so it wraps by n), others clamp; positions the
+// clamp moves are ghost positions, filled afterwards by ghost_fill.
+__device__ __forceinline__ int src_axis(int g, int ext, int n, int per) {
+  if (per) g = g < 0 ? g + n : (g >= ext ? g - n : g);
+  return g < 0 ? 0 : (g >= ext ? ext - 1 : g);
+}
+
+// Pass 1 of the tile load for component C: the stage input at every LDS position (clamped
+// or wrapped source), u^ - grad psi with the outlet shift when c.psi is set (the previous
+// stage's projected field, core.py:281-297, 582-612).
+template <typename T, int C, int EX, int EY, int EZ>
+__device__ __forceinline__ void load_comp(const Dev3<T> &d, const Call3<T> &c, T *lds, int k0, int j0, int i0, T corr) {
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  const int ex = nx + (C == 0), ey = ny + (C == 1), ez = nz + (C == 2);
+  const T *src = C == 0 ? c.up : (C == 1 ? c.vp : c.wp);
+  const T *psi = c.psi;
+  const int pn = C == 0 ? nx : (C == 1 ? ny : nz);         // cells along the normal axis
+  const int pper = d.per[C];
+  const int pstride = C == 0 ? 1 : (C == 1 ? nx : nx * ny);
+  const bool shift = c.corr_stage >= 0 && C == 0 && d.outlet;
+  const T inv_h = d.inv_h;
+  for (int e = threadIdx.x; e < EX * EY * EZ; e += kTileThreads) {
+    const int x = e % EX, t = e / EX, y = t % EY, z = t / EY;
+    const int i = src_axis(i0 - 2 + x, ex, nx, d.per[0]);
+    const int j = src_axis(j0 - 2 + y, ey, ny, d.per[1]);
+    const int k = src_axis(k0 - 2 + z, ez, nz, d.per[2]);
+    T val = src[(unsigned)((k * ey + j) * ex + i)];
+    if (psi) {
+      const int f = C == 0 ? i : (C == 1 ? j : k);
+      if (pper || (f > 0 && f < pn)) {
+        const int fc = f == pn ? 0 : f;
+        const int fm = fc == 0 ? pn - 1 : fc - 1;
+        const int ci = C == 0 ? 0 : i, cj = C == 1 ? 0 : j, ck = C == 2 ? 0 : k;
+        const T *pb = psi + (unsigned)((ck * ny + cj) * nx + ci);
+        val = val - (pb[(unsigned)(fc * pstride)] - pb[(unsigned)(fm * pstride)]) * inv_h;
+      }
+    }
+    if (shift && i == nx) val = val + corr;
+    lds[e] = val;
+  }
+}
+
+// Pass 2 along one axis: ghost positions of a non-periodic side from their mirror source in
+// LDS (ns3d/core.py:223-272), for every position of the other two axes. Runs after the
+// lower axes' pass, so corner values compose as pad3 does (x rule, then y, then z).
+template <typename T, int AX, int EX, int EY, int EZ>
+__device__ __forceinline__ void ghost_fill(const Dev3<T> &d, T *lds, int comp, int a0, int ext) {
+  const int E[3] = {EX, EY, EZ};
+  const int ea = E[AX];
+  const int other = (EX * EY * EZ) / ea;
+  for (int side = 0; side < 2; ++side) {
+    // ghost positions g of this side inside the tile: lo g = -2, -1 (tile at the start), hi g = ext, ext + 1
+    const int g0 = side == 0 ? -2 : ext;
+    const int p0 = g0 - (a0 - 2);
+    if (side == 0 && a0 != 0) continue;
+    if (side == 1 && (p0 >= ea || p0 + 1 < 0)) continue;
+    const int code = d.rule[comp * 6 + AX * 2 + side];
+    const T rv = d.rval2[comp * 6 + AX * 2 + side];
+    for (int e = threadIdx.x; e < 2 * other; e += kTileThreads) {
+      const int q = e >> 1, gg = g0 + (e & 1);
+      const int p = gg - (a0 - 2);
+      if (p < 0 || p >= ea) continue;
+      int op;
+      const int sg = ghost_axis(code, gg, ext, op);
+      const int ps = sg - (a0 - 2);
+      int idx, sidx;
+      if (AX == 0) {
+        idx = q * EX + p;
+        sidx = q * EX + ps;
+      } else if (AX == 1) {
+        const int xx = q % EX, zz = q / EX;
+        idx = (zz * EY + p) * EX + xx;
+        sidx = (zz * EY + ps) * EX + xx;
+      } else {
+        idx = p * EX * EY + q;
+        sidx = ps * EX * EY + q;
+      }
+      const T m = lds[sidx];
+      lds[idx] = op == 0 ? m : (op == 1 ? -m : rv - m);
+    }
+  }
 }
 
 template <typename T>
 __device__ void load_tile(const Dev3<T> &d, const Call3<T> &c, Lds3<T> &L, int k0, int j0, int i0, T corr) {
   using S = Lds3<T>;
-  for (int e = threadIdx.x; e < S::UZ * S::UY * S::UX; e += kThreads) {
-    const int x = e % S::UX, t = e / S::UX, y = t % S::UY, z = t / S::UY;
-    (&L.u[0][0][0])[e] = padded(d, c, 0, k0 - 2 + z, j0 - 2 + y, i0 - 2 + x, corr);
-  }
-  for (int e = threadIdx.x; e < S::VZ * S::VY * S::VX; e += kThreads) {
-    const int x = e % S::VX, t = e / S::VX, y = t % S::VY, z = t / S::VY;
-    (&L.v[0][0][0])[e] = padded(d, c, 1, k0 - 2 + z, j0 - 2 + y, i0 - 2 + x, corr);
-  }
-  for (int e = threadIdx.x; e < S::WZ * S::WY * S::WX; e += kThreads) {
-    const int x = e % S::WX, t = e / S::WX, y = t % S::WY, z = t / S::WY;
-    (&L.w[0][0][0])[e] = padded(d, c, 2, k0 - 2 + z, j0 - 2 + y, i0 - 2 + x, corr);
-  }
+  load_comp<T, 0, S::UX, S::UY, S::UZ>(d, c, &L.u[0][0][0], k0, j0, i0, corr);
+  load_comp<T, 1, S::VX, S::VY, S::VZ>(d, c, &L.v[0][0][0], k0, j0, i0, corr);
+  load_comp<T, 2, S::WX, S::WY, S::WZ>(d, c, &L.w[0][0][0], k0, j0, i0, corr);
   __syncthreads();
+  // boundary tiles: ghost layers of non-periodic sides, axis by axis
+  const bool bx = !d.per[0] && (i0 == 0 || i0 + S::TX + 3 > d.nx);
+  const bool by = !d.per[1] && (j0 == 0 || j0 + S::TY + 3 > d.ny);
+  const bool bz = !d.per[2] && (k0 == 0 || k0 + S::TZ + 3 > d.nz);
+  if (bx) {
+    ghost_fill<T, 0, S::UX, S::UY, S::UZ>(d, &L.u[0][0][0], 0, i0, d.nx + 1);
+    ghost_fill<T, 0, S::VX, S::VY, S::VZ>(d, &L.v[0][0][0], 1, i0, d.nx);
+    ghost_fill<T, 0, S::WX, S::WY, S::WZ>(d, &L.w[0][0][0], 2, i0, d.nx);
+    __syncthreads();
+  }
+  if (by) {
+    ghost_fill<T, 1, S::UX, S::UY, S::UZ>(d, &L.u[0][0][0], 0, j0, d.ny);
+    ghost_fill<T, 1, S::VX, S::VY, S::VZ>(d, &L.v[0][0][0], 1, j0, d.ny + 1);
+    ghost_fill<T, 1, S::WX, S::WY, S::WZ>(d, &L.w[0][0][0], 2, j0, d.ny);
+    __syncthreads();
+  }
+  if (bz) {
+    ghost_fill<T, 2, S::UX, S::UY, S::UZ>(d, &L.u[0][0][0], 0, k0, d.nz);
+    ghost_fill<T, 2, S::VX, S::VY, S::VZ>(d, &L.v[0][0][0], 1, k0, d.nz);
+    ghost_fill<T, 2, S::WX, S::WY, S::WZ>(d, &L.w[0][0][0], 2, k0, d.nz + 1);
+    __syncthreads();
+  }
 }
 
 // ------------------------------------------------------------------ eddy viscosity
@@ -324,7 +378,7 @@ __device__ T nu_tile(const Dev3<T> &d, const Call3<T> &c, Lds3<T> &L, int k0, in
   using S = Lds3<T>;
   T mx = 0;
   const int ie = min(i0 + S::TX, d.nx), je = min(j0 + S::TY, d.ny), ke = min(k0 + S::TZ, d.nz);
-  for (int e = threadIdx.x; e < S::NZ * S::NY * S::NX; e += kThreads) {
+  for (int e = threadIdx.x; e < S::NZ * S::NY * S::NX; e += kTileThreads) {
     const int x = e % S::NX, t = e / S::NX, y = t % S::NY, z = t / S::NY;
     const int i = x - 1, j = y - 1, k = z - 1;
     T g[3][3];
@@ -445,9 +499,10 @@ __device__ __forceinline__ int face_end(int c0, int t, int n) { return c0 + t >=
 
 // ------------------------------------------------------------------ stage kernel
 template <typename T>
-__global__ void __launch_bounds__(kThreads) k_stage(const Dev3<T> *__restrict__ dp, Call3<T> c) {
+__global__ void __launch_bounds__(kTileThreads, Tile<T>::WAVES_PER_SIMD) k_stage(const Dev3<T> *__restrict__ dp, Call3<T> c) {
   using S = Lds3<T>;
-  __shared__ S L;
+  extern __shared__ unsigned char smem[];
+  S &L = *reinterpret_cast<S *>(smem);
   const Dev3<T> &d = *dp;
   int bx, by, bz, tile;
   tile_of(d, bx, by, bz, tile);
@@ -468,7 +523,7 @@ __global__ void __launch_bounds__(kThreads) k_stage(const Dev3<T> *__restrict__ 
   // u faces
   {
     const int fe = face_end(i0, S::TX, nx), nfx = fe - i0, nfy = je - j0, nfz = ke - k0;
-    for (int e = threadIdx.x; e < nfx * nfy * nfz; e += kThreads) {
+    for (int e = threadIdx.x; e < nfx * nfy * nfz; e += kTileThreads) {
       const int li = e % nfx, t = e / nfx, lj = t % nfy, lk = t / nfy;
       const int gi = i0 + li, gj = j0 + lj, gk = k0 + lk;
       const size_t idx = ((size_t)gk * ny + gj) * (nx + 1) + gi;
@@ -497,7 +552,7 @@ __global__ void __launch_bounds__(kThreads) k_stage(const Dev3<T> *__restrict__ 
   // v faces
   {
     const int fe = face_end(j0, S::TY, ny), nfx = ie - i0, nfy = fe - j0, nfz = ke - k0;
-    for (int e = threadIdx.x; e < nfx * nfy * nfz; e += kThreads) {
+    for (int e = threadIdx.x; e < nfx * nfy * nfz; e += kTileThreads) {
       const int li = e % nfx, t = e / nfx, lj = t % nfy, lk = t / nfy;
       const int gi = i0 + li, gj = j0 + lj, gk = k0 + lk;
       const size_t idx = ((size_t)gk * (ny + 1) + gj) * nx + gi;
@@ -518,7 +573,7 @@ __global__ void __launch_bounds__(kThreads) k_stage(const Dev3<T> *__restrict__ 
   // w faces
   {
     const int fe = face_end(k0, S::TZ, nz), nfx = ie - i0, nfy = je - j0, nfz = fe - k0;
-    for (int e = threadIdx.x; e < nfx * nfy * nfz; e += kThreads) {
+    for (int e = threadIdx.x; e < nfx * nfy * nfz; e += kTileThreads) {
       const int li = e % nfx, t = e / nfx, lj = t % nfy, lk = t / nfy;
       const int gi = i0 + li, gj = j0 + lj, gk = k0 + lk;
       const size_t idx = ((size_t)gk * ny + gj) * nx + gi;
@@ -539,7 +594,7 @@ __global__ void __launch_bounds__(kThreads) k_stage(const Dev3<T> *__restrict__ 
   // outlet plane sums of this tile (the outlet correction needs them all)
   if (d.outlet && i0 + S::TX >= nx) {
     double v[2] = {isum, osum};
-    block_sum4<2>(v, L.red);
+    block_sum_w<2>(v, L.red);
     if (threadIdx.x == 0) {
       d.pout[2 * (bz * d.nby + by)] = v[0];
       d.pout[2 * (bz * d.nby + by) + 1] = v[1];
@@ -552,9 +607,10 @@ __global__ void __launch_bounds__(kThreads) k_stage(const Dev3<T> *__restrict__ 
 // mode 1: the last stage's projection applied (out = u^ - grad psi), written to the state,
 //         plus the dt inputs of that state and the force sums of window tiles.
 template <typename T>
-__global__ void __launch_bounds__(kThreads) k_final(const Dev3<T> *__restrict__ dp, Call3<T> c, int mode) {
+__global__ void __launch_bounds__(kTileThreads, Tile<T>::WAVES_PER_SIMD) k_final(const Dev3<T> *__restrict__ dp, Call3<T> c, int mode) {
   using S = Lds3<T>;
-  __shared__ S L;
+  extern __shared__ unsigned char smem[];
+  S &L = *reinterpret_cast<S *>(smem);
   const Dev3<T> &d = *dp;
   int bx, by, bz, tile;
   tile_of(d, bx, by, bz, tile);
@@ -568,7 +624,7 @@ __global__ void __launch_bounds__(kThreads) k_final(const Dev3<T> *__restrict__ 
   T mu = 0, mv = 0, mw = 0;
   {
     const int fe = face_end(i0, S::TX, nx), nfx = fe - i0, nfy = je - j0, nfz = ke - k0;
-    for (int e = threadIdx.x; e < nfx * nfy * nfz; e += kThreads) {
+    for (int e = threadIdx.x; e < nfx * nfy * nfz; e += kTileThreads) {
       const int li = e % nfx, t = e / nfx, lj = t % nfy, lk = t / nfy;
       const T x = LU(lk, lj, li);
       if (mode) c.uo[((size_t)(k0 + lk) * ny + (j0 + lj)) * (nx + 1) + i0 + li] = x;
@@ -577,7 +633,7 @@ __global__ void __launch_bounds__(kThreads) k_final(const Dev3<T> *__restrict__ 
   }
   {
     const int fe = face_end(j0, S::TY, ny), nfx = ie - i0, nfy = fe - j0, nfz = ke - k0;
-    for (int e = threadIdx.x; e < nfx * nfy * nfz; e += kThreads) {
+    for (int e = threadIdx.x; e < nfx * nfy * nfz; e += kTileThreads) {
       const int li = e % nfx, t = e / nfx, lj = t % nfy, lk = t / nfy;
       const T x = LV(lk, lj, li);
       if (mode) c.vo[((size_t)(k0 + lk) * (ny + 1) + (j0 + lj)) * nx + i0 + li] = x;
@@ -586,7 +642,7 @@ __global__ void __launch_bounds__(kThreads) k_final(const Dev3<T> *__restrict__ 
   }
   {
     const int fe = face_end(k0, S::TZ, nz), nfx = ie - i0, nfy = je - j0, nfz = fe - k0;
-    for (int e = threadIdx.x; e < nfx * nfy * nfz; e += kThreads) {
+    for (int e = threadIdx.x; e < nfx * nfy * nfz; e += kTileThreads) {
       const int li = e % nfx, t = e / nfx, lj = t % nfy, lk = t / nfy;
       const T x = LW(lk, lj, li);
       if (mode) c.wo[((size_t)(k0 + lk) * ny + (j0 + lj)) * nx + i0 + li] = x;
@@ -609,7 +665,7 @@ __global__ void __launch_bounds__(kThreads) k_final(const Dev3<T> *__restrict__ 
   __syncthreads();
   if (threadIdx.x < 4) {
     T m = red[threadIdx.x];
-    for (int q = 1; q < 4; ++q) m = red[q * 4 + threadIdx.x] > m ? red[q * 4 + threadIdx.x] : m;
+    for (int q = 1; q < kTileWaves; ++q) m = red[q * 4 + threadIdx.x] > m ? red[q * 4 + threadIdx.x] : m;
     d.pmax[(size_t)tile * 4 + threadIdx.x] = m;
   }
   // force sums of this tile's window faces: sum_faces k[b, m] * acc (core.py:653-660)
@@ -631,7 +687,7 @@ __global__ void __launch_bounds__(kThreads) k_final(const Dev3<T> *__restrict__ 
         const size_t wn = (size_t)ex * ey * ez;
         const T *kk = (comp == 0 ? c.k_u : comp == 1 ? c.k_v : c.k_w) + ((size_t)b * 3 + m) * wn;
         const T *ac = comp == 0 ? d.acc_u : comp == 1 ? d.acc_v : d.acc_w;
-        for (int e = threadIdx.x; e < n0 * n1 * n2; e += kThreads) {
+        for (int e = threadIdx.x; e < n0 * n1 * n2; e += kTileThreads) {
           const int x = e % n0, t = e / n0, y = t % n1, z = t / n1;
           const size_t wi = ((size_t)(f0[2] + z - d.wk0) * ey + (f0[1] + y - d.wj0)) * ex + (f0[0] + x - d.wi0);
           acc += (double)kk[wi] * (double)ac[wi];
@@ -642,9 +698,11 @@ __global__ void __launch_bounds__(kThreads) k_final(const Dev3<T> *__restrict__ 
       if ((q & 15) == 15 || q == d.nq - 1) {
         __syncthreads();
         const int base = q & ~15;
-        if (threadIdx.x <= q - base)
-          d.pforce[(size_t)wt * d.nq + base + threadIdx.x] =
-              ((L.red[threadIdx.x] + L.red[16 + threadIdx.x]) + L.red[32 + threadIdx.x]) + L.red[48 + threadIdx.x];
+        if (threadIdx.x <= q - base) {
+          double r = L.red[threadIdx.x];
+          for (int w2 = 1; w2 < kTileWaves; ++w2) r += L.red[w2 * 16 + threadIdx.x];
+          d.pforce[(size_t)wt * d.nq + base + threadIdx.x] = r;
+        }
         __syncthreads();
       }
     }
@@ -1256,6 +1314,19 @@ int upload3(aero3d_plan *p) {
   d.ns3 = p->ns3;
   d.ns5 = p->ns5;
   AERO_HIP_CHECK(hipMemcpy(p->dev, &d, sizeof(d), hipMemcpyHostToDevice));
+  // the tile kernels take more than the default 64 KB of dynamic LDS
+  int dev_id = 0, max_lds = 0;
+  AERO_HIP_CHECK(hipGetDevice(&dev_id));
+  AERO_HIP_CHECK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev_id));
+  if ((size_t)max_lds < sizeof(Lds3<T>)) {
+    set_error("the device offers " + std::to_string(max_lds) + " B of LDS per block; the 3D tiles need " +
+              std::to_string(sizeof(Lds3<T>)));
+    return AERO_EUNSUPPORTED;
+  }
+  AERO_HIP_CHECK(hipFuncSetAttribute((const void *)k_stage<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)sizeof(Lds3<T>)));
+  AERO_HIP_CHECK(hipFuncSetAttribute((const void *)k_final<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)sizeof(Lds3<T>)));
   return AERO_OK;
 }
 
@@ -1324,9 +1395,10 @@ int step_impl(aero3d_plan *p, hipStream_t st, const aero3d_fields *in, aero3d_fi
   const aero3d_desc &s = p->desc;
   const Dev3<T> *dev = (const Dev3<T> *)p->dev;
   Call3<T> c = make_call<T>(in, g, pr);
-  const dim3 grid(p->nblk), blk(kThreads);
+  const dim3 grid(p->nblk), blk(kTileThreads);
+  const size_t lds = sizeof(Lds3<T>);
   if (!s.fixed_dt && !chained) {
-    hipLaunchKernelGGL(k_final<T>, grid, blk, 0, st, dev, c, 0);
+    hipLaunchKernelGGL(k_final<T>, grid, blk, lds, st, dev, c, 0);
     hipLaunchKernelGGL(k_dt<T>, dim3(1), dim3(kFinThreads), 0, st, dev, c);
   }
   T *ub[2] = {(T *)p->uA, (T *)p->uB}, *vb[2] = {(T *)p->vA, (T *)p->vB}, *wb[2] = {(T *)p->wA, (T *)p->wB};
@@ -1340,7 +1412,7 @@ int step_impl(aero3d_plan *p, hipStream_t st, const aero3d_fields *in, aero3d_fi
     c.uo = ub[stage & 1];
     c.vo = vb[stage & 1];
     c.wo = wb[stage & 1];
-    hipLaunchKernelGGL(k_stage<T>, grid, blk, 0, st, dev, c);
+    hipLaunchKernelGGL(k_stage<T>, grid, blk, lds, st, dev, c);
     const int mode = (!s.periodic_x && !s.closed) ? 2 : 1;
     int rc = poisson_passes<T>(p, st, dev, c, nullptr, mode, stage, (T *)p->psi, (T *)out->p);
     if (rc) return rc;
@@ -1354,7 +1426,7 @@ int step_impl(aero3d_plan *p, hipStream_t st, const aero3d_fields *in, aero3d_fi
   c.uo = (T *)out->u;
   c.vo = (T *)out->v;
   c.wo = (T *)out->w;
-  hipLaunchKernelGGL(k_final<T>, grid, blk, 0, st, dev, c, 1);
+  hipLaunchKernelGGL(k_final<T>, grid, blk, lds, st, dev, c, 1);
   Fin3<T> f;
   std::memset(&f, 0, sizeof(f));
   f.t_in = (const T *)in->t;
@@ -1415,9 +1487,11 @@ int time_stage_impl(aero3d_plan *p, hipStream_t st, const aero3d_fields *fl, con
   hipEvent_t e0, e1;
   AERO_HIP_CHECK(hipEventCreate(&e0));
   AERO_HIP_CHECK(hipEventCreate(&e1));
-  hipLaunchKernelGGL(k_stage<T>, dim3(p->nblk), dim3(kThreads), 0, st, dev, c);
+  const size_t lds = sizeof(Lds3<T>);
+  hipLaunchKernelGGL(k_stage<T>, dim3(p->nblk), dim3(kTileThreads), lds, st, dev, c);
   AERO_HIP_CHECK(hipEventRecord(e0, st));
-  for (int it = 0; it < iters; ++it) hipLaunchKernelGGL(k_stage<T>, dim3(p->nblk), dim3(kThreads), 0, st, dev, c);
+  for (int it = 0; it < iters; ++it)
+    hipLaunchKernelGGL(k_stage<T>, dim3(p->nblk), dim3(kTileThreads), lds, st, dev, c);
   AERO_HIP_CHECK(hipEventRecord(e1, st));
   AERO_HIP_CHECK(hipEventSynchronize(e1));
   float ms = 0.f;

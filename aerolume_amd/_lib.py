@@ -94,6 +94,7 @@ def load() -> ctypes.CDLL:
         getattr(lib, name).restype = ctypes.c_int
     from . import _lib3d
     _lib3d.bind(lib)
+    _bind_ensemble(lib)
     _LIB = lib
     return lib
 
@@ -114,3 +115,14 @@ def ptr(t) -> int | None:
 def stream_handle(device) -> int:
     import torch
     return torch.cuda.current_stream(device).cuda_stream
+
+
+def _bind_ensemble(lib) -> None:
+    P = ctypes.c_void_p
+    lib.aero2d_ensemble_create.argtypes = [ctypes.c_int32, ctypes.POINTER(P), ctypes.POINTER(Fields2D),
+                                           ctypes.POINTER(Geom2D), ctypes.POINTER(Params2D),
+                                           ctypes.POINTER(Outputs2D), ctypes.POINTER(P)]
+    lib.aero2d_ensemble_step.argtypes = [P, P, ctypes.c_int32]
+    lib.aero2d_ensemble_destroy.argtypes = [P]
+    for name in ("aero2d_ensemble_create", "aero2d_ensemble_step", "aero2d_ensemble_destroy"):
+        getattr(lib, name).restype = ctypes.c_int

@@ -550,9 +550,9 @@ __device__ T compute_dt(const Dev<T> &d, const DtIn<T> &in, T umax, T vmax, T th
 // ================================================================== kernels
 // The explicit stage on one 64x8 tile (core.py:855-935 per face / cell).
 template <typename T>
-__global__ void __launch_bounds__(256) k_stage(Dev<T> d, Stage<T> s) {
+__device__ __forceinline__ void stage_body(const Dev<T> &d, const Stage<T> &s, int bx, int by) {
   __shared__ Lds<T> L;
-  const int tj0 = blockIdx.y * TY, ti0 = blockIdx.x * TX;
+  const int tj0 = by * TY, ti0 = bx * TX;
   const int nx = d.nx, ny = d.ny;
   const T corr = s.psi_p ? d.red->corr : (T)0;
   load_uv(d, L, tj0, ti0, s.up, s.vp, s.psi_p, corr);
@@ -747,6 +747,10 @@ __global__ void __launch_bounds__(256) k_stage(Dev<T> d, Stage<T> s) {
     }
   }
 }
+template <typename T>
+__global__ void __launch_bounds__(256) k_stage(Dev<T> d, Stage<T> s) {
+  stage_body<T>(d, s, blockIdx.x, blockIdx.y);
+}
 
 // Per-block maxima of a velocity tile (+ theta, nu_t) and sum theta: the dt inputs.
 // Each tile also covers the ghost ring of nu_t cells at the domain edge (core.py:503).
@@ -794,7 +798,7 @@ __device__ void tile_maxima(const Dev<T> &d, Lds<T> &L, int tj0, int ti0, const 
   }
   __syncthreads();
   if (tid == 0) {
-    double *p = d.pmax + 5 * (blockIdx.y * gridDim.x + blockIdx.x);
+    double *p = d.pmax + 5 * ((tj0 / TY) * d.nbx + ti0 / TX);
     for (int q = 0; q < 4; ++q)
       p[q] = fmax(fmax(L.red[q], L.red[5 + q]), fmax(L.red[10 + q], L.red[15 + q]));
     p[4] = ((L.red[4] + L.red[9]) + L.red[14]) + L.red[19];
@@ -804,9 +808,9 @@ __device__ void tile_maxima(const Dev<T> &d, Lds<T> &L, int tj0, int ti0, const 
 
 // dt inputs of the step-start state (non-chained steps).
 template <typename T>
-__global__ void __launch_bounds__(256) k_pre(Dev<T> d, const T *u0, const T *v0, const T *th0) {
+__device__ __forceinline__ void pre_body(const Dev<T> &d, int bx, int by, const T *u0, const T *v0, const T *th0) {
   __shared__ Lds<T> L;
-  const int tj0 = blockIdx.y * TY, ti0 = blockIdx.x * TX;
+  const int tj0 = by * TY, ti0 = bx * TX;
   load_uv(d, L, tj0, ti0, u0, v0, (const T *)nullptr, (T)0);
   __syncthreads();
   if (d.les) {
@@ -814,6 +818,10 @@ __global__ void __launch_bounds__(256) k_pre(Dev<T> d, const T *u0, const T *v0,
     __syncthreads();
   }
   tile_maxima(d, L, tj0, ti0, th0);
+}
+template <typename T>
+__global__ void __launch_bounds__(256) k_pre(Dev<T> d, const T *u0, const T *v0, const T *th0) {
+  pre_body<T>(d, blockIdx.x, blockIdx.y, u0, v0, th0);
 }
 
 // Reduce the per-block dt inputs in a fixed order (one block; every thread gets out[]).
@@ -845,7 +853,7 @@ __device__ void reduce_pmax(const Dev<T> &d, int nblocks, double *lds, double ou
 }
 
 template <typename T>
-__global__ void __launch_bounds__(1024) k_dt(Dev<T> d) {
+__device__ __forceinline__ void dt_body(const Dev<T> &d) {
   __shared__ double lds[16 * 5];
   double r[5];
   DtIn<T> din;
@@ -855,6 +863,10 @@ __global__ void __launch_bounds__(1024) k_dt(Dev<T> d) {
     d.red->dt = compute_dt(d, din, (T)r[0], (T)r[1], (T)r[2], (T)r[3]);
     d.red->th_old = r[4];
   }
+}
+template <typename T>
+__global__ void __launch_bounds__(1024) k_dt(Dev<T> d) {
+  dt_body<T>(d);
 }
 
 // natural index of reordered position n (inverse of makhoul_pos)
@@ -897,8 +909,7 @@ __device__ __forceinline__ void load_twiddles(cplx<T> *dst, const cplx<T> *__res
 
 // Pass A: divergence of rows (2b, 2b+1) -> forward transform along x -> F rows.
 template <typename T>
-__global__ void __launch_bounds__(kFftThreads) k_poisA(Dev<T> d, const T *__restrict__ u, const T *__restrict__ v,
-                                                       const T *__restrict__ f_in, int mode) {
+__device__ __forceinline__ void poisA_body(const Dev<T> &d, int bx, const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ f_in, int mode) {
   // mode 0: f_in is the right-hand side itself; 1: divergence of (u, v); 2: + outlet correction
   extern __shared__ unsigned char smem[];
   const int nx = d.nx;
@@ -906,13 +917,13 @@ __global__ void __launch_bounds__(kFftThreads) k_poisA(Dev<T> d, const T *__rest
   cplx<T> *B = A + nx;
   cplx<T> *TW = B + nx;
   double *red = reinterpret_cast<double *>(TW + nx);
-  const int ja = 2 * blockIdx.x, jb = ja + 1;
+  const int ja = 2 * bx, jb = ja + 1;
   const bool hasb = jb < d.ny;
   load_twiddles(TW, d.tw_x, nx);
   T corr = 0;
   if (mode == 2) {
     corr = outlet_corr(d, red);
-    if (blockIdx.x == 0 && threadIdx.x == 0) d.red->corr = corr;
+    if (bx == 0 && threadIdx.x == 0) d.red->corr = corr;
   }
   // all loads of this thread first (row elements and DCT twiddles), then the FFT
   cplx<T> csr[kMaxPer];
@@ -954,6 +965,11 @@ __global__ void __launch_bounds__(kFftThreads) k_poisA(Dev<T> d, const T *__rest
     if (hasb) d.F[jb * nx + k] = xb;
   }
 }
+template <typename T>
+__global__ void __launch_bounds__(kFftThreads) k_poisA(Dev<T> d, const T *__restrict__ u, const T *__restrict__ v,
+                                                       const T *__restrict__ f_in, int mode) {
+  poisA_body<T>(d, blockIdx.x, u, v, f_in, mode);
+}
 
 // Inverse real transform packing (see common.h): conj(Z) for the DCT-III, plain pair for DHT.
 template <typename T>
@@ -972,7 +988,7 @@ __device__ __forceinline__ void pack_inverse(const T *xa, const T *xb, cplx<T> *
 
 // Pass B: column pairs, nslot per block: forward along y, x 1/lambda (FP64), inverse along y.
 template <typename T>
-__global__ void __launch_bounds__(kFftThreads) k_poisB(Dev<T> d, int nslot) {
+__device__ __forceinline__ void poisB_body(const Dev<T> &d, int bx, int nslot) {
   extern __shared__ unsigned char smem[];
   const int ny = d.ny, nx = d.nx;
   cplx<T> *A = reinterpret_cast<cplx<T> *>(smem);
@@ -980,7 +996,7 @@ __global__ void __launch_bounds__(kFftThreads) k_poisB(Dev<T> d, int nslot) {
   cplx<T> *TW = B + nslot * ny;
   T *X = reinterpret_cast<T *>(TW + ny);   // 2 * nslot * ny reals
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const int c0 = 2 * nslot * blockIdx.x;
+  const int c0 = 2 * nslot * bx;
   load_twiddles(TW, d.tw_y, ny);
   cplx<T> csr[kMaxPer];
   double lyr[kMaxPer], lxa[kMaxPer], lxb[kMaxPer];
@@ -1057,18 +1073,22 @@ __global__ void __launch_bounds__(kFftThreads) k_poisB(Dev<T> d, int nslot) {
     if (cb < nx) d.F[q * nx + cb] = vb;
   }
 }
+template <typename T>
+__global__ void __launch_bounds__(kFftThreads) k_poisB(Dev<T> d, int nslot) {
+  poisB_body<T>(d, blockIdx.x, nslot);
+}
 
 // Pass C: inverse along x of rows (2b, 2b+1) -> psi; accumulate the step-averaged pressure
 // p (core.py:935-942): stage 0 p = w psi, stage 1 p += w psi, stage 2 p = (p + psi) / dt.
 template <typename T>
-__global__ void __launch_bounds__(kFftThreads) k_poisC(Dev<T> d, T *__restrict__ psi, T *__restrict__ p, int stage) {
+__device__ __forceinline__ void poisC_body(const Dev<T> &d, int bx, T *__restrict__ psi, T *__restrict__ p, int stage) {
   extern __shared__ unsigned char smem[];
   const int nx = d.nx;
   cplx<T> *A = reinterpret_cast<cplx<T> *>(smem);
   cplx<T> *B = A + nx;
   cplx<T> *TW = B + nx;
   T *X = reinterpret_cast<T *>(TW + nx);
-  const int ja = 2 * blockIdx.x, jb = ja + 1;
+  const int ja = 2 * bx, jb = ja + 1;
   const bool hasb = jb < d.ny;
   load_twiddles(TW, d.tw_x, nx);
   cplx<T> csr[kMaxPer];
@@ -1137,6 +1157,10 @@ __global__ void __launch_bounds__(kFftThreads) k_poisC(Dev<T> d, T *__restrict__
     }
   }
 }
+template <typename T>
+__global__ void __launch_bounds__(kFftThreads) k_poisC(Dev<T> d, T *__restrict__ psi, T *__restrict__ p, int stage) {
+  poisC_body<T>(d, blockIdx.x, psi, p, stage);
+}
 
 // u -= grad psi (standalone projection, core.py:626-660).
 template <typename T>
@@ -1167,10 +1191,9 @@ __global__ void __launch_bounds__(256) k_correct(Dev<T> d, T *__restrict__ u, T 
 // End of step on one tile: write u = u^ - grad psi, smoke transport with it (physics.py:
 // 805-830 / core.py:709-771), next-step dt inputs and the window force / heat partials.
 template <typename T>
-__global__ void __launch_bounds__(256) k_final(Dev<T> d, const T *uh, const T *vh, T *u_out, T *v_out,
-                                              const T *th_new, const T *cin, T *cout, const T *t_in) {
+__device__ __forceinline__ void final_body(const Dev<T> &d, int bx, int by, const T *uh, const T *vh, T *u_out, T *v_out, const T *th_new, const T *cin, T *cout, const T *t_in) {
   __shared__ Lds<T> L;
-  const int tj0 = blockIdx.y * TY, ti0 = blockIdx.x * TX;
+  const int tj0 = by * TY, ti0 = bx * TX;
   const int nx = d.nx, ny = d.ny;
   const T corr = d.red->corr;
   load_uv(d, L, tj0, ti0, uh, vh, (const T *)d.psi, corr);
@@ -1235,7 +1258,7 @@ __global__ void __launch_bounds__(256) k_final(Dev<T> d, const T *uh, const T *v
     }
     block_sum4<kMaxWin>(acc, L.red);
     if (threadIdx.x == 0 && threadIdx.y == 0) {
-      double *p = d.pforce + kMaxWin * (blockIdx.y * gridDim.x + blockIdx.x);
+      double *p = d.pforce + kMaxWin * (by * d.nbx + bx);
       for (int q = 0; q < kMaxWin; ++q) p[q] = acc[q];
     }
   }
@@ -1283,6 +1306,11 @@ __global__ void __launch_bounds__(256) k_final(Dev<T> d, const T *uh, const T *v
     }
   }
 }
+template <typename T>
+__global__ void __launch_bounds__(256) k_final(Dev<T> d, const T *uh, const T *vh, T *u_out, T *v_out,
+                                              const T *th_new, const T *cin, T *cout, const T *t_in) {
+  final_body<T>(d, blockIdx.x, blockIdx.y, uh, vh, u_out, v_out, th_new, cin, cout, t_in);
+}
 
 template <typename T>
 __global__ void k_copy(T *__restrict__ dst, const T *__restrict__ src, size_t n) {
@@ -1292,9 +1320,7 @@ __global__ void k_copy(T *__restrict__ dst, const T *__restrict__ src, size_t n)
 
 // Per-step outputs (core.py:942-1004, physics.py:760-795) and the next step's dt.
 template <typename T>
-__global__ void __launch_bounds__(1024) k_finalize(Dev<T> d, const T *__restrict__ t_in, T *__restrict__ t_out,
-                                                   T *__restrict__ dt_out, const T *__restrict__ u_new,
-                                                   const T *__restrict__ th_new, aero2d_outputs o) {
+__device__ __forceinline__ void finalize_body(const Dev<T> &d, const T *__restrict__ t_in, T *__restrict__ t_out, T *__restrict__ dt_out, const T *__restrict__ u_new, const T *__restrict__ th_new, const aero2d_outputs &o) {
   __shared__ double lds[16 * kMaxWin];
   __shared__ double res[kMaxWin];
   const int tid = threadIdx.x, nthr = blockDim.x;
@@ -1445,6 +1471,93 @@ __global__ void __launch_bounds__(1024) k_finalize(Dev<T> d, const T *__restrict
   // the next step (chained) starts from this state
   d.red->dt = compute_dt(d, din, (T)mx[0], (T)mx[1], (T)mx[2], (T)mx[3]);
   d.red->th_old = mx[4];
+}
+template <typename T>
+__global__ void __launch_bounds__(1024) k_finalize(Dev<T> d, const T *__restrict__ t_in, T *__restrict__ t_out,
+                                                   T *__restrict__ dt_out, const T *__restrict__ u_new,
+                                                   const T *__restrict__ th_new, aero2d_outputs o) {
+  finalize_body<T>(d, t_in, t_out, dt_out, u_new, th_new, o);
+}
+
+
+// ================================================================== batched ensemble
+// Many independent runs of uneven size advance together: every launch covers the blocks of
+// all runs through a work table of (run, block x, block y) items, and each run reads its own
+// static description (Dev) and launch pointers (BRun) from device memory. The kernel bodies
+// are those of the single-run launches, so each run's arithmetic is the same.
+template <typename T>
+struct BRun {
+  Stage<T> st[3];
+  const T *u0, *v0, *th0;        // step start (in place: also the step's output)
+  T *p_out;
+  const T *uC, *vC;              // last stage's u^
+  T *u_out, *v_out;
+  const T *th_new, *t_in;
+  T *t_out, *dt_out;
+  aero2d_outputs o;
+  int nslotB, mode;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) kb_pre(const Dev<T> *__restrict__ devs, const BRun<T> *__restrict__ runs,
+                                             const int4 *__restrict__ items) {
+  const int4 it = items[blockIdx.x];
+  const BRun<T> &r = runs[it.x];
+  pre_body<T>(devs[it.x], it.y, it.z, r.u0, r.v0, r.th0);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024) kb_dt(const Dev<T> *__restrict__ devs) {
+  dt_body<T>(devs[blockIdx.x]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) kb_stage(const Dev<T> *__restrict__ devs, const BRun<T> *__restrict__ runs,
+                                               const int4 *__restrict__ items, int stage) {
+  const int4 it = items[blockIdx.x];
+  stage_body<T>(devs[it.x], runs[it.x].st[stage], it.y, it.z);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kFftThreads) kb_poisA(const Dev<T> *__restrict__ devs,
+                                                        const BRun<T> *__restrict__ runs,
+                                                        const int4 *__restrict__ items, int stage) {
+  const int4 it = items[blockIdx.x];
+  const BRun<T> &r = runs[it.x];
+  poisA_body<T>(devs[it.x], it.y, r.st[stage].uh, r.st[stage].vh, (const T *)nullptr, r.mode);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kFftThreads) kb_poisB(const Dev<T> *__restrict__ devs,
+                                                        const BRun<T> *__restrict__ runs,
+                                                        const int4 *__restrict__ items) {
+  const int4 it = items[blockIdx.x];
+  poisB_body<T>(devs[it.x], it.y, runs[it.x].nslotB);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kFftThreads) kb_poisC(const Dev<T> *__restrict__ devs,
+                                                        const BRun<T> *__restrict__ runs,
+                                                        const int4 *__restrict__ items, int stage) {
+  const int4 it = items[blockIdx.x];
+  const Dev<T> &d = devs[it.x];
+  poisC_body<T>(d, it.y, d.psi, runs[it.x].p_out, stage);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) kb_final(const Dev<T> *__restrict__ devs, const BRun<T> *__restrict__ runs,
+                                               const int4 *__restrict__ items) {
+  const int4 it = items[blockIdx.x];
+  const BRun<T> &r = runs[it.x];
+  final_body<T>(devs[it.x], it.y, it.z, r.uC, r.vC, r.u_out, r.v_out, r.th_new, (const T *)nullptr, (T *)nullptr,
+                r.t_in);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024) kb_finalize(const Dev<T> *__restrict__ devs,
+                                                    const BRun<T> *__restrict__ runs) {
+  const BRun<T> &r = runs[blockIdx.x];
+  finalize_body<T>(devs[blockIdx.x], r.t_in, r.t_out, r.dt_out, (const T *)r.u_out, r.th_new, r.o);
 }
 
 }  // namespace ns2d
@@ -1886,6 +1999,154 @@ int aero2d_poisson_solve(aero2d_plan *p, void *stream, const void *f, void *psi)
   if (!p || !f || !psi) { set_error("null argument"); return AERO_EINVAL; }
   hipStream_t st = (hipStream_t)stream;
   return p->dbl ? poisson_impl<double>(p, st, f, psi) : poisson_impl<float>(p, st, f, psi);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- ensemble (host)
+struct aero2d_ensemble {
+  int n = 0, dbl = 0;
+  std::vector<aero2d_plan *> plans;
+  std::vector<void *> allocs;
+  void *devs = nullptr, *runs = nullptr, *tiles = nullptr, *rows = nullptr, *cols = nullptr;
+  int ntiles = 0, nrows = 0, ncols = 0;
+  size_t ldsA = 0, ldsB = 0;
+};
+
+namespace {
+
+int ens_alloc(aero2d_ensemble *e, void **ptr, const void *src, size_t n) {
+  AERO_HIP_CHECK(hipMalloc(ptr, n));
+  e->allocs.push_back(*ptr);
+  AERO_HIP_CHECK(hipMemcpy(*ptr, src, n, hipMemcpyHostToDevice));
+  return AERO_OK;
+}
+
+template <typename T>
+int ensemble_build(aero2d_ensemble *e, const aero2d_fields *fields, const aero2d_geom *geoms,
+                   const aero2d_params *params, const aero2d_outputs *outputs) {
+  std::vector<Dev<T>> devs(e->n);
+  std::vector<BRun<T>> runs(e->n);
+  std::vector<int4> tiles, rows, cols;
+  for (int r = 0; r < e->n; ++r) {
+    aero2d_plan *p = e->plans[r];
+    const aero2d_desc &s = p->desc;
+    devs[r] = make_dev<T>(p, &geoms[r], &params[r]);
+    BRun<T> &b = runs[r];
+    std::memset(&b, 0, sizeof(b));
+    const aero2d_fields &f = fields[r];
+    const T *u0 = (const T *)f.u, *v0 = (const T *)f.v, *th0 = (const T *)f.theta;
+    T *ubuf[3] = {(T *)p->uA, (T *)p->uB, (T *)p->uC};
+    T *vbuf[3] = {(T *)p->vA, (T *)p->vB, (T *)p->vC};
+    T *tbuf[3] = {(T *)p->thA, (T *)p->thB, (T *)f.theta};
+    const T *up = u0, *vp = v0, *thp = th0, *psi_p = nullptr;
+    for (int k = 0; k < 3; ++k) {
+      Stage<T> &sg = b.st[k];
+      sg.s = k;
+      sg.a = kSsp[k][0];
+      sg.b = kSsp[k][1];
+      sg.w = kForceW[k];
+      sg.up = up; sg.vp = vp; sg.psi_p = psi_p; sg.thp = thp;
+      sg.u0 = u0; sg.v0 = v0; sg.th0 = th0;
+      sg.uh = ubuf[k]; sg.vh = vbuf[k]; sg.thh = s.thermal ? tbuf[k] : nullptr;
+      up = sg.uh; vp = sg.vh; thp = sg.thh; psi_p = (const T *)p->psi;
+    }
+    b.u0 = u0; b.v0 = v0; b.th0 = th0;
+    b.p_out = (T *)f.p;
+    b.uC = (const T *)p->uC; b.vC = (const T *)p->vC;
+    b.u_out = (T *)f.u; b.v_out = (T *)f.v;
+    b.th_new = (const T *)f.theta;
+    b.t_in = (const T *)f.t; b.t_out = (T *)f.t; b.dt_out = (T *)f.dt;
+    if (outputs) b.o = outputs[r];
+    b.nslotB = p->nslotB;
+    b.mode = (s.periodic || s.closed) ? 1 : 2;
+    for (int by = 0; by < p->nby; ++by)
+      for (int bx = 0; bx < p->nbx; ++bx) tiles.push_back(make_int4(r, bx, by, 0));
+    for (int bx = 0; bx < (s.ny + 1) / 2; ++bx) rows.push_back(make_int4(r, bx, 0, 0));
+    const int cpb = 2 * p->nslotB;
+    for (int bx = 0; bx < (s.nx + cpb - 1) / cpb; ++bx) cols.push_back(make_int4(r, bx, 0, 0));
+    e->ldsA = std::max(e->ldsA, lds_A<T>(p));
+    e->ldsB = std::max(e->ldsB, lds_B<T>(p));
+  }
+  e->ntiles = (int)tiles.size();
+  e->nrows = (int)rows.size();
+  e->ncols = (int)cols.size();
+  int rc;
+  if ((rc = ens_alloc(e, &e->devs, devs.data(), devs.size() * sizeof(Dev<T>)))) return rc;
+  if ((rc = ens_alloc(e, &e->runs, runs.data(), runs.size() * sizeof(BRun<T>)))) return rc;
+  if ((rc = ens_alloc(e, &e->tiles, tiles.data(), tiles.size() * sizeof(int4)))) return rc;
+  if ((rc = ens_alloc(e, &e->rows, rows.data(), rows.size() * sizeof(int4)))) return rc;
+  if ((rc = ens_alloc(e, &e->cols, cols.data(), cols.size() * sizeof(int4)))) return rc;
+  return AERO_OK;
+}
+
+template <typename T>
+int ensemble_step(aero2d_ensemble *e, hipStream_t st, bool chained) {
+  const Dev<T> *devs = (const Dev<T> *)e->devs;
+  const BRun<T> *runs = (const BRun<T> *)e->runs;
+  const int4 *tiles = (const int4 *)e->tiles, *rows = (const int4 *)e->rows, *cols = (const int4 *)e->cols;
+  const dim3 blk(64, 4);
+  if (!chained) {
+    hipLaunchKernelGGL(kb_pre<T>, dim3(e->ntiles), blk, 0, st, devs, runs, tiles);
+    hipLaunchKernelGGL(kb_dt<T>, dim3(e->n), dim3(1024), 0, st, devs);
+  }
+  for (int k = 0; k < 3; ++k) {
+    hipLaunchKernelGGL(kb_stage<T>, dim3(e->ntiles), blk, 0, st, devs, runs, tiles, k);
+    hipLaunchKernelGGL(kb_poisA<T>, dim3(e->nrows), dim3(kFftThreads), e->ldsA, st, devs, runs, rows, k);
+    hipLaunchKernelGGL(kb_poisB<T>, dim3(e->ncols), dim3(kFftThreads), e->ldsB, st, devs, runs, cols);
+    hipLaunchKernelGGL(kb_poisC<T>, dim3(e->nrows), dim3(kFftThreads), e->ldsA, st, devs, runs, rows, k);
+  }
+  hipLaunchKernelGGL(kb_final<T>, dim3(e->ntiles), blk, 0, st, devs, runs, tiles);
+  hipLaunchKernelGGL(kb_finalize<T>, dim3(e->n), dim3(1024), 0, st, devs, runs);
+  AERO_HIP_CHECK(hipGetLastError());
+  return AERO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int aero2d_ensemble_create(int32_t n, aero2d_plan *const *plans, const aero2d_fields *fields,
+                           const aero2d_geom *geoms, const aero2d_params *params, const aero2d_outputs *outputs,
+                           aero2d_ensemble **out) {
+  if (!out || n <= 0 || !plans || !fields || !geoms || !params) { set_error("bad ensemble arguments"); return AERO_EINVAL; }
+  *out = nullptr;
+  const int dbl = plans[0] ? plans[0]->dbl : 0;
+  for (int r = 0; r < n; ++r) {
+    aero2d_plan *p = plans[r];
+    if (!p) { set_error("null plan in ensemble"); return AERO_EINVAL; }
+    if (p->dbl != dbl) { set_error("ensemble plans must share one dtype"); return AERO_EINVAL; }
+    if (p->desc.dye_mode != 0) { set_error("ensemble runs carry no smoke"); return AERO_EUNSUPPORTED; }
+    aero2d_fields f = fields[r];
+    int rc = check_step_args(p, &fields[r], &f, &geoms[r], &params[r],
+                             outputs ? const_cast<aero2d_outputs *>(&outputs[r]) : nullptr);
+    if (rc) return rc;
+  }
+  aero2d_ensemble *e = new aero2d_ensemble();
+  e->n = n;
+  e->dbl = dbl;
+  e->plans.assign(plans, plans + n);
+  int rc = dbl ? ensemble_build<double>(e, fields, geoms, params, outputs)
+               : ensemble_build<float>(e, fields, geoms, params, outputs);
+  if (rc) {
+    aero2d_ensemble_destroy(e);
+    return rc;
+  }
+  *out = e;
+  return AERO_OK;
+}
+
+int aero2d_ensemble_step(aero2d_ensemble *e, void *stream, int32_t chained) {
+  if (!e) { set_error("null ensemble"); return AERO_EINVAL; }
+  hipStream_t st = (hipStream_t)stream;
+  return e->dbl ? ensemble_step<double>(e, st, chained != 0) : ensemble_step<float>(e, st, chained != 0);
+}
+
+int aero2d_ensemble_destroy(aero2d_ensemble *e) {
+  if (!e) return AERO_OK;
+  for (void *a : e->allocs) (void)hipFree(a);
+  delete e;
+  return AERO_OK;
 }
 
 }  // extern "C"

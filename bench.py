@@ -11,6 +11,10 @@ Also reported: ``roofline`` of the dominant kernel (the fused stage kernel; algo
 per launch / its average duration timed with HIP events on the solver's stream) and
 ``cpu_baseline`` (the CPU oracle on the host's cores, a bounded sample of the same case).
 
+``workloads.ensemble``: config 4, a 64-run angle / Reynolds sweep on uneven grids as one
+batched ensemble (solves/s for 1000-step solves) against the same solves run one after
+another on the same GPU.
+
 ``workloads.3d_car`` carries the same figures for the 3D car case at 256x128x128 (4.19M
 cells, FP32): a 4 x 1.6 x 1.2 rounded box on four rolling wheels over a floor moving at U,
 Re 5e4, WALE, periodic in y, upwind3 + SSP-RK3 at CFL 0.5, exact DCT/DHT projection;
@@ -149,6 +153,47 @@ def gpu_run_3d(steps: int, warmup: int):
     return solver, wall, stage_ms
 
 
+def gpu_run_ensemble(steps: int, warmup: int, n: int = 64, seed: int = 404):
+    """64-run sweep (NACA 0012, uneven grids 128..512 x 64..256, alpha U[-5, 15] deg,
+    Re 10^U[3, 5], FP32, no LES): the batched ensemble against the same runs solved one after
+    another with FlowSolver (HIP graphs on both sides)."""
+    import torch
+
+    from aerolume_amd.ns2d.ensemble import Ensemble2D, sweep_configs
+    from aerolume_amd.ns2d.solver import FlowSolver
+    cfgs = sweep_configs(n=n, seed=seed)
+    ens = Ensemble2D(cfgs)
+    cells = ens.cells
+    ens.advance(warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.advance(steps)
+    torch.cuda.synchronize()
+    wall_e = time.perf_counter() - t0
+    ok = all(bool(torch.isfinite(s.state.u).all()) for s in ens.solvers)
+    del ens
+    torch.cuda.empty_cache()
+    wall_s = 0.0
+    for c in cfgs:
+        fs = FlowSolver(c)
+        fs.advance(warmup)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fs.advance(steps)
+        torch.cuda.synchronize()
+        wall_s += time.perf_counter() - t0
+        del fs
+    if not ok:
+        raise RuntimeError("non-finite ensemble state")
+    return {"value": round(n / wall_e, 3), "unit": "solves/s", "sequential_value": round(n / wall_s, 3),
+            "speedup": round(wall_s / wall_e, 2), "runs": n, "steps_per_solve": steps, "warmup": warmup,
+            "cells_total": cells, "mcell_steps_per_s": round(cells * steps / wall_e / 1e6, 1),
+            "sequential_mcell_steps_per_s": round(cells * steps / wall_s / 1e6, 1),
+            "ms_per_ensemble_step": round(wall_e / steps * 1e3, 4), "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "64-run NACA 0012 sweep, nx in 5-smooth [128, 512], ny = nx/2, h = 8/nx, "
+                                   "alpha U[-5, 15] deg, Re 10^U[3, 5], upwind3, no LES, FP32", "seed": seed}}
+
+
 def noisy_velocity3d(ops, seed):
     import numpy as np
     rng = np.random.default_rng(seed)
@@ -233,6 +278,8 @@ def main() -> int:
     ap.add_argument("--steps3d", type=int, default=500)
     ap.add_argument("--warmup3d", type=int, default=10)
     ap.add_argument("--no-3d", action="store_true", help="skip the 3D car workload")
+    ap.add_argument("--ensemble-steps", type=int, default=1000)
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble workload")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -287,7 +334,9 @@ def main() -> int:
         if not args.no_cpu:
             w3["cpu_baseline"] = cpu_baseline_3d()
             w3["gpu_over_cpu"] = round(v3 / w3["cpu_baseline"]["value"], 1)
-        line["workloads"] = {"3d_car": w3}
+        line.setdefault("workloads", {})["3d_car"] = w3
+    if not args.no_ensemble:
+        line.setdefault("workloads", {})["ensemble"] = gpu_run_ensemble(args.ensemble_steps, 10)
     print(json.dumps(line), flush=True)
     return 0
 

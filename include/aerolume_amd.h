@@ -18,6 +18,7 @@
  *                          called as ns2d/solver.py:384-400 _step_static does
  *   aero2d_project       - ns2d/core.py:626-642 project
  *   aero2d_poisson_solve - ns2d/poisson.py:128-134 PoissonSolver.solve
+ *   aero2d_ensemble_*    - no counterpart (batched sweep; the reference loops over solves)
  *   aero3d_plan_create   - ns3d/core.py:184-203 make_operators3d (static operators,
  *                          ghost-rule table core.py:147-181, PoissonSolver3D
  *                          ns3d/poisson.py:107-126)
@@ -236,6 +237,22 @@ int aero3d_poisson_solve(aero3d_plan *plan, void *stream, const void *f, void *p
  * returns the average launch time in ms. Synchronises the stream. Call after one step. */
 int aero3d_time_stage(aero3d_plan *plan, void *stream, const aero3d_fields *fields, const aero3d_geom *geom,
                       const aero3d_params *params, int stage, int iters, double *avg_ms);
+
+/* ------------------------------------------------------------------ 2D ensemble */
+/* Many independent 2D runs of uneven size (angle / Reynolds sweeps, polars) advancing together:
+ * each launch of a step covers the blocks of every run through a device work table, so an
+ * ensemble step costs the same ~15 launches whatever the number of runs. No reference
+ * counterpart: the reference runs such sweeps one solve after another (api.py:4877-4930).
+ * Each run is bound at creation to a plan and its in-place fields (fields[r] is both the
+ * step's input and output), geometry, parameters and outputs; runs carry no smoke. */
+typedef struct aero2d_ensemble aero2d_ensemble;
+int aero2d_ensemble_create(int32_t n, aero2d_plan *const *plans, const aero2d_fields *fields,
+                           const aero2d_geom *geoms, const aero2d_params *params,
+                           const aero2d_outputs *outputs, aero2d_ensemble **ensemble);
+/* One step of every run; chained as for aero2d_step_chained (each run's dt inputs from its
+ * previous ensemble step). */
+int aero2d_ensemble_step(aero2d_ensemble *ensemble, void *stream, int32_t chained);
+int aero2d_ensemble_destroy(aero2d_ensemble *ensemble);
 
 const char *aero_last_error(void);
 int aero_version(void);

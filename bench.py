@@ -49,6 +49,11 @@ SEED3 = 303
 # v0, w0 of the step start (7 x 4 B), write u^, v^, w^ (3 x 4 B)
 STAGE3_BYTES_PER_CELL = 40
 STEP3_BYTES_PER_CELL = 310
+# HBM bytes per stage-2 launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by the
+# same counters on a 512 MiB device copy (fetch x2.000, write x1.000):
+# profiles/stage_kernels_hbm_counters.md
+TRAFFIC_STAGE_2D = 29.01e6
+TRAFFIC_STAGE_3D = 183.42e6
 
 
 def log(msg: str) -> None:
@@ -268,6 +273,30 @@ def cpu_baseline(seconds: float = 15.0):
                       f"of {t_total:.1f} s, torch CPU oracle, {threads} threads, flush-to-zero"}
 
 
+def run_only(args) -> int:
+    """One workload alone, for profiler runs: prints a short JSON summary."""
+    import torch
+    out = {"only": args.only}
+    if args.only == "2d":
+        _, wall, stage_ms = gpu_run(args.steps, args.warmup)
+        out.update(ms_per_step=wall / args.steps * 1e3, stage_us=stage_ms * 1e3)
+    elif args.only == "3d":
+        _, wall, stage_ms = gpu_run_3d(args.steps3d, args.warmup3d)
+        out.update(ms_per_step=wall / args.steps3d * 1e3, stage_us=stage_ms * 1e3)
+    elif args.only == "ensemble":
+        out.update(gpu_run_ensemble(args.ensemble_steps, 10))
+    else:
+        n = 512 * 2**20 // 4
+        x = torch.rand(n, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(10):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        out.update(copy_bytes_read=n * 4, copy_bytes_written=n * 4, launches=10)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -279,6 +308,9 @@ def main() -> int:
     ap.add_argument("--warmup3d", type=int, default=10)
     ap.add_argument("--no-3d", action="store_true", help="skip the 3D car workload")
     ap.add_argument("--ensemble-steps", type=int, default=1000)
+    ap.add_argument("--only", choices=("2d", "3d", "ensemble", "copy"), default=None,
+                    help="run one workload and print its figures (profiling runs); 'copy' times a plain "
+                         "512 MiB device copy (the HBM counter calibration)")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble workload")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -286,6 +318,8 @@ def main() -> int:
         print("bench.py measures one GPU: this build has no multi-GPU path (every GPU box has one "
               "MI355X); run with --gpus 1", file=sys.stderr)
         return 2
+    if args.only is not None:
+        return run_only(args)
     solver, wall, stage_ms = gpu_run(args.steps, args.warmup)
     cells = NX * NY
     value = cells * args.steps / wall / 1e6
@@ -302,7 +336,7 @@ def main() -> int:
                    "nx": NX, "ny": NY, "cells": cells, "seed": SEED, "graph_steps": 10},
         "roofline": {"bound": "hbm", "kernel": "k_stage (stage 2 of 3)", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "avg_launch_us": round(stage_ms * 1e3, 3),
+                     "traffic": TRAFFIC_STAGE_2D, "avg_launch_us": round(stage_ms * 1e3, 3),
                      "algorithmic_bytes_per_launch": stage_bytes},
         "step_hbm": {"algorithmic_bytes_per_step": STEP_BYTES_PER_CELL * cells, "achieved_gbs": round(step_gbs, 1),
                      "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
@@ -326,7 +360,7 @@ def main() -> int:
                          "seed": SEED3, "graph_steps": 10},
               "roofline": {"bound": "hbm", "kernel": "k_stage 3D (stage 2 of 3)", "achieved": round(ach3, 1),
                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach3 / HBM_PEAK_GBS, 4),
-                           "traffic": None, "avg_launch_us": round(stage3_ms * 1e3, 3),
+                           "traffic": TRAFFIC_STAGE_3D, "avg_launch_us": round(stage3_ms * 1e3, 3),
                            "algorithmic_bytes_per_launch": sb3},
               "step_hbm": {"algorithmic_bytes_per_step": STEP3_BYTES_PER_CELL * cells3,
                            "achieved_gbs": round(step3, 1), "frac": round(step3 / HBM_PEAK_GBS, 4)}}

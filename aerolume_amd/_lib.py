@@ -95,6 +95,8 @@ def load() -> ctypes.CDLL:
     from . import _lib3d
     _lib3d.bind(lib)
     _bind_ensemble(lib)
+    from . import _libcnn
+    _libcnn.bind(lib)
     _LIB = lib
     return lib
 

@@ -1,0 +1,454 @@
+// Small 3D convolutional network on a periodic grid (the learned eddy-viscosity closure):
+// layers of 3x3x3 convolutions with BF16 weights and FP32 accumulation, forward, backward
+// (data and weight gradients) and Adam, as hand-written HIP kernels for gfx950.
+//
+//   forward    y[co][p] = act(b[co] + sum_ci sum_k w[co][ci][k] x[ci][p + d_k])
+//   backward   dx[ci][p] = sum_co sum_k w[co][ci][k] g[co][p - d_k]  (the forward kernel on
+//              flipped, transposed weights), dw[co][ci][k] = sum_p g[co][p] x[ci][p + d_k],
+//              db[co] = sum_p g[co][p], with g = dy * act'(pre-activation)
+//
+// Layout: channel-major fields (C, nz, ny, nx), x fastest; positions wrap periodically.
+// Parameters: per layer w[co][ci][27] then b[co], float32 masters; the convolutions read a
+// bfloat16 copy of w (round to nearest even), so forward and backward use the same rounded
+// weights. No atomics: weight gradients go through per-block partials reduced in order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/aerolume_amd.h"
+#include "common.h"
+
+namespace aero {
+namespace cnn {
+
+constexpr int kMaxLayers = 8;
+constexpr int TX = 8, TY = 8, TZ = 4;       // output positions per block (one per thread)
+constexpr int kThreads = TX * TY * TZ;      // 256
+constexpr int CI_CH = 8;                    // input channels staged per pass
+constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2;
+constexpr int kWgBricks = 16;              // 8x8x4 bricks per weight-gradient block
+
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__host__ __device__ __forceinline__ uint16_t f32_to_bf16(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));  // inf / nan
+  const uint32_t lsb = (u >> 16) & 1u;
+  u += 0x7fffu + lsb;                        // round to nearest even
+  return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+__device__ __forceinline__ float act_f(int a, float x) {
+  if (a == 1) return x > 0.f ? x : 0.f;
+  if (a == 2) return x > 20.f ? x : log1pf(expf(x));      // softplus
+  return x;
+}
+__device__ __forceinline__ float act_df(int a, float x) {
+  if (a == 1) return x > 0.f ? 1.f : 0.f;
+  if (a == 2) return x > 20.f ? 1.f : 1.f / (1.f + expf(-x));
+  return 1.f;
+}
+
+// Forward convolution of one layer (COUT output channels per thread held in registers).
+// w16: bf16 weights [co][ci][27]; b: bias (null: none); pre: pre-activation output (null: not
+// kept); y: activated output. One thread per output position of an 8x8x4 block tile.
+template <int COUT>
+__global__ void __launch_bounds__(kThreads) k_conv_fwd(const float *__restrict__ x, const uint16_t *__restrict__ w16,
+                                                      const float *__restrict__ b, float *__restrict__ pre,
+                                                      float *__restrict__ y, int cin, int nz, int ny, int nx,
+                                                      int act) {
+  __shared__ float sx[CI_CH][HZ][HY][HX];
+  __shared__ float sw[CI_CH][27][COUT];
+  const int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+  const int tx = threadIdx.x % TX, ty = (threadIdx.x / TX) % TY, tz = threadIdx.x / (TX * TY);
+  const int x0 = bx * TX, y0 = by * TY, z0 = bz * TZ;
+  const size_t nxyz = (size_t)nx * ny * nz;
+  float acc[COUT];
+#pragma unroll
+  for (int co = 0; co < COUT; ++co) acc[co] = 0.f;
+  for (int c0 = 0; c0 < cin; c0 += CI_CH) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < CI_CH * HZ * HY * HX; e += kThreads) {
+      const int hx = e % HX, t1 = e / HX, hy = t1 % HY, t2 = t1 / HY, hz = t2 % HZ, ci = t2 / HZ;
+      float v = 0.f;
+      if (c0 + ci < cin) {
+        const int gx = wrapi(x0 - 1 + hx, nx), gy = wrapi(y0 - 1 + hy, ny), gz = wrapi(z0 - 1 + hz, nz);
+        v = x[(size_t)(c0 + ci) * nxyz + ((size_t)gz * ny + gy) * nx + gx];
+      }
+      sx[ci][hz][hy][hx] = v;
+    }
+    for (int e = threadIdx.x; e < CI_CH * 27 * COUT; e += kThreads) {
+      const int co = e % COUT, t1 = e / COUT, k = t1 % 27, ci = t1 / 27;
+      sw[ci][k][co] = c0 + ci < cin ? bf16_to_f32(w16[((size_t)co * cin + c0 + ci) * 27 + k]) : 0.f;
+    }
+    __syncthreads();
+    const int cn = min(CI_CH, cin - c0);
+    for (int ci = 0; ci < cn; ++ci) {
+#pragma unroll
+      for (int k = 0; k < 27; ++k) {
+        const int dz = k / 9, dy = (k / 3) % 3, dx = k % 3;
+        const float xv = sx[ci][tz + dz][ty + dy][tx + dx];
+#pragma unroll
+        for (int co = 0; co < COUT; ++co) acc[co] += sw[ci][k][co] * xv;
+      }
+    }
+  }
+  const int gx = x0 + tx, gy = y0 + ty, gz = z0 + tz;
+  if (gx >= nx || gy >= ny || gz >= nz) return;
+  const size_t p = ((size_t)gz * ny + gy) * nx + gx;
+#pragma unroll
+  for (int co = 0; co < COUT; ++co) {
+    const float v = acc[co] + (b ? b[co] : 0.f);
+    if (pre) pre[(size_t)co * nxyz + p] = v;
+    y[(size_t)co * nxyz + p] = act_f(act, v);
+  }
+}
+
+// g = dy * act'(pre) in place into g (elementwise).
+__global__ void k_act_grad(const float *__restrict__ dy, const float *__restrict__ pre, float *__restrict__ g, size_t n,
+                           int act) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+    g[e] = dy[e] * act_df(act, pre[e]);
+}
+
+// Weights for the data gradient: wt[ci][co][k] = w[co][ci][26 - k] (flipped, transposed), bf16.
+__global__ void k_flip(const uint16_t *__restrict__ w16, uint16_t *__restrict__ wt, int cout, int cin) {
+  const int n = cout * cin * 27;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    const int k = e % 27, t = e / 27, co = t % cout, ci = t / cout;
+    wt[e] = w16[((size_t)co * cin + ci) * 27 + (26 - k)];
+  }
+}
+
+// Weight-gradient partials: block (input-channel chunk, run of bricks) stages, per 8x8x4
+// brick, the output gradient g of every co and the input brick with its one-cell halo; thread
+// (ci, k) accumulates dw[co][ci][k] for all co over the bricks. Blocks of chunk 0 also sum
+// db[co]. Partials: [position chunk][co][cin][27], then [position chunk][co] for db.
+template <int COUT>
+__global__ void __launch_bounds__(256) k_conv_wgrad(const float *__restrict__ x, const float *__restrict__ g,
+                                                   float *__restrict__ part, int cin, int nz, int ny, int nx,
+                                                   int bricks_per_block) {
+  __shared__ float sg[COUT][kThreads];
+  __shared__ float sx[CI_CH][HZ][HY][HX];
+  const int c0 = blockIdx.x * CI_CH;
+  const int nbx = (nx + TX - 1) / TX, nby = (ny + TY - 1) / TY, nbz = (nz + TZ - 1) / TZ;
+  const int nbr = nbx * nby * nbz;
+  const int br0 = blockIdx.y * bricks_per_block, br1 = min(nbr, br0 + bricks_per_block);
+  const size_t nxyz = (size_t)nx * ny * nz;
+  const int t = threadIdx.x;
+  const int ci = t / 27, k = t % 27;
+  const bool work = t < CI_CH * 27 && c0 + ci < cin;
+  const int dz = k / 9, dy = (k / 3) % 3, dx = k % 3;
+  float acc[COUT];
+#pragma unroll
+  for (int co = 0; co < COUT; ++co) acc[co] = 0.f;
+  float bacc = 0.f;
+  for (int br = br0; br < br1; ++br) {
+    const int bx = br % nbx, by = (br / nbx) % nby, bz = br / (nbx * nby);
+    const int x0 = bx * TX, y0 = by * TY, z0 = bz * TZ;
+    __syncthreads();
+    for (int e = t; e < COUT * kThreads; e += 256) {
+      const int co = e / kThreads, q = e % kThreads;
+      const int gx = x0 + q % TX, gy = y0 + (q / TX) % TY, gz = z0 + q / (TX * TY);
+      sg[co][q] = (gx < nx && gy < ny && gz < nz) ? g[(size_t)co * nxyz + ((size_t)gz * ny + gy) * nx + gx] : 0.f;
+    }
+    for (int e = t; e < CI_CH * HZ * HY * HX; e += 256) {
+      const int hx = e % HX, t1 = e / HX, hy = t1 % HY, t2 = t1 / HY, hz = t2 % HZ, cc = t2 / HZ;
+      float v = 0.f;
+      if (c0 + cc < cin) {
+        const int gx = wrapi(x0 - 1 + hx, nx), gy = wrapi(y0 - 1 + hy, ny), gz = wrapi(z0 - 1 + hz, nz);
+        v = x[(size_t)(c0 + cc) * nxyz + ((size_t)gz * ny + gy) * nx + gx];
+      }
+      sx[cc][hz][hy][hx] = v;
+    }
+    __syncthreads();
+    if (work) {
+      for (int q = 0; q < kThreads; ++q) {
+        const int qx = q % TX, qy = (q / TX) % TY, qz = q / (TX * TY);
+        const float xv = sx[ci][qz + dz][qy + dy][qx + dx];
+#pragma unroll
+        for (int co = 0; co < COUT; ++co) acc[co] += sg[co][q] * xv;
+      }
+    }
+    if (blockIdx.x == 0 && t < COUT)
+      for (int q = 0; q < kThreads; ++q) bacc += sg[t][q];
+  }
+  const int npc = gridDim.y;
+  if (work) {
+#pragma unroll
+    for (int co = 0; co < COUT; ++co) part[(((size_t)blockIdx.y * COUT + co) * cin + c0 + ci) * 27 + k] = acc[co];
+  }
+  if (blockIdx.x == 0 && t < COUT) part[(size_t)npc * COUT * cin * 27 + (size_t)blockIdx.y * COUT + t] = bacc;
+}
+
+// Sum the partials over position chunks (fixed order) into grad (w then b of the layer).
+__global__ void k_wgrad_reduce(const float *__restrict__ part, float *__restrict__ grad, int nw, int nb, int npc) {
+  const int n = nw + nb;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    float s = 0.f;
+    if (e < nw) {
+      for (int c = 0; c < npc; ++c) s += part[(size_t)c * nw + e];
+    } else {
+      const int co = e - nw;
+      for (int c = 0; c < npc; ++c) s += part[(size_t)npc * nw + (size_t)c * nb + co];
+    }
+    grad[e] = s;
+  }
+}
+
+// Adam on the float32 masters, then the bf16 copy of every weight (biases stay float32).
+__global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m, float *__restrict__ v,
+                       int n, float lr, float b1, float b2, float eps, float c1, float c2) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    const float ge = g[e];
+    const float me = b1 * m[e] + (1.f - b1) * ge;
+    const float ve = b2 * v[e] + (1.f - b2) * ge * ge;
+    m[e] = me;
+    v[e] = ve;
+    p[e] -= lr * (me / c1) / (sqrtf(ve / c2) + eps);
+  }
+}
+
+__global__ void k_to_bf16(const float *__restrict__ src, uint16_t *__restrict__ dst, int n) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) dst[e] = f32_to_bf16(src[e]);
+}
+
+}  // namespace cnn
+}  // namespace aero
+
+using namespace aero;
+using namespace aero::cnn;
+
+struct aero_cnn {
+  aero_cnn_desc desc;
+  size_t nxyz = 0;
+  int nparam = 0;
+  int woff[kMaxLayers], boff[kMaxLayers];
+  std::vector<void *> allocs;
+  float *params = nullptr, *grads = nullptr, *adam_m = nullptr, *adam_v = nullptr, *part = nullptr;
+  uint16_t *w16 = nullptr, *wt16 = nullptr;
+  float *act[kMaxLayers + 1];   // act[0] = the input copy, act[l + 1] = layer l output
+  float *pre[kMaxLayers];
+  float *g0 = nullptr, *g1 = nullptr;
+  size_t bytes = 0;
+};
+
+namespace {
+
+int cnn_alloc(aero_cnn *c, void **ptr, size_t n) {
+  *ptr = nullptr;
+  if (!n) return AERO_OK;
+  AERO_HIP_CHECK(hipMalloc(ptr, n));
+  c->allocs.push_back(*ptr);
+  c->bytes += n;
+  AERO_HIP_CHECK(hipMemset(*ptr, 0, n));
+  return AERO_OK;
+}
+
+int max_ch(const aero_cnn_desc &d) {
+  int m = 0;
+  for (int l = 0; l <= d.n_layers; ++l) m = std::max(m, d.channels[l]);
+  return m;
+}
+
+void launch_fwd(hipStream_t st, const float *x, const uint16_t *w, const float *b, float *pre, float *y, int cin,
+                int cout, int nz, int ny, int nx, int act) {
+  const dim3 grid((nx + TX - 1) / TX, (ny + TY - 1) / TY, (nz + TZ - 1) / TZ);
+  if (cout == 1)
+    hipLaunchKernelGGL(k_conv_fwd<1>, grid, dim3(kThreads), 0, st, x, w, b, pre, y, cin, nz, ny, nx, act);
+  else if (cout == 8)
+    hipLaunchKernelGGL(k_conv_fwd<8>, grid, dim3(kThreads), 0, st, x, w, b, pre, y, cin, nz, ny, nx, act);
+  else if (cout == 9)
+    hipLaunchKernelGGL(k_conv_fwd<9>, grid, dim3(kThreads), 0, st, x, w, b, pre, y, cin, nz, ny, nx, act);
+  else if (cout == 16)
+    hipLaunchKernelGGL(k_conv_fwd<16>, grid, dim3(kThreads), 0, st, x, w, b, pre, y, cin, nz, ny, nx, act);
+  else
+    hipLaunchKernelGGL(k_conv_fwd<32>, grid, dim3(kThreads), 0, st, x, w, b, pre, y, cin, nz, ny, nx, act);
+}
+
+bool cout_ok(int c) { return c == 1 || c == 8 || c == 9 || c == 16 || c == 32; }
+
+}  // namespace
+
+extern "C" {
+
+int aero_cnn_create(const aero_cnn_desc *d, aero_cnn **out) {
+  if (!out || !d) { set_error("null argument"); return AERO_EINVAL; }
+  *out = nullptr;
+  if (d->n_layers < 1 || d->n_layers > kMaxLayers) { set_error("1..8 layers"); return AERO_EINVAL; }
+  if (d->nx < 3 || d->ny < 3 || d->nz < 3) { set_error("grid of at least 3 cells per axis"); return AERO_EINVAL; }
+  for (int l = 0; l < d->n_layers; ++l) {
+    if (d->channels[l] < 1 || d->channels[l] > 32) { set_error("1..32 input channels per layer"); return AERO_EINVAL; }
+    if (!cout_ok(d->channels[l + 1])) {
+      set_error("output channels per layer must be 1, 8, 9, 16 or 32");
+      return AERO_EUNSUPPORTED;
+    }
+    if (d->act[l] < 0 || d->act[l] > 2) { set_error("activation 0 (none), 1 (relu) or 2 (softplus)"); return AERO_EINVAL; }
+  }
+  aero_cnn *c = new aero_cnn();
+  c->desc = *d;
+  c->nxyz = (size_t)d->nx * d->ny * d->nz;
+  int off = 0, nw_max = 0;
+  for (int l = 0; l < d->n_layers; ++l) {
+    c->woff[l] = off;
+    off += d->channels[l + 1] * d->channels[l] * 27;
+    c->boff[l] = off;
+    off += d->channels[l + 1];
+    nw_max = std::max(nw_max, d->channels[l + 1] * d->channels[l] * 27);
+  }
+  c->nparam = off;
+  const int mc = max_ch(*d);
+  const size_t nbr = (size_t)((d->nx + TX - 1) / TX) * ((d->ny + TY - 1) / TY) * ((d->nz + TZ - 1) / TZ);
+  const size_t npc = (nbr + kWgBricks - 1) / kWgBricks;
+  int rc = AERO_OK;
+  auto A = [&](void **p, size_t n) { if (!rc) rc = cnn_alloc(c, p, n); };
+  A((void **)&c->params, off * 4);
+  A((void **)&c->grads, off * 4);
+  A((void **)&c->adam_m, off * 4);
+  A((void **)&c->adam_v, off * 4);
+  A((void **)&c->w16, off * 2);
+  A((void **)&c->wt16, (size_t)nw_max * 2);
+  A((void **)&c->part, (npc * (size_t)(nw_max + 32)) * 4);
+  for (int l = 0; l <= d->n_layers; ++l) A((void **)&c->act[l], (size_t)d->channels[l] * c->nxyz * 4);
+  for (int l = 0; l < d->n_layers; ++l) A((void **)&c->pre[l], (size_t)d->channels[l + 1] * c->nxyz * 4);
+  A((void **)&c->g0, (size_t)mc * c->nxyz * 4);
+  A((void **)&c->g1, (size_t)mc * c->nxyz * 4);
+  if (rc) {
+    aero_cnn_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return AERO_OK;
+}
+
+int aero_cnn_destroy(aero_cnn *c) {
+  if (!c) return AERO_OK;
+  for (void *a : c->allocs) (void)hipFree(a);
+  delete c;
+  return AERO_OK;
+}
+
+int aero_cnn_param_count(const aero_cnn *c) { return c ? c->nparam : 0; }
+size_t aero_cnn_workspace_bytes(const aero_cnn *c) { return c ? c->bytes : 0; }
+
+int aero_cnn_set_params(aero_cnn *c, void *stream, const float *params) {
+  if (!c || !params) { set_error("null argument"); return AERO_EINVAL; }
+  hipStream_t st = (hipStream_t)stream;
+  AERO_HIP_CHECK(hipMemcpyAsync(c->params, params, (size_t)c->nparam * 4, hipMemcpyDeviceToDevice, st));
+  AERO_HIP_CHECK(hipMemsetAsync(c->adam_m, 0, (size_t)c->nparam * 4, st));
+  AERO_HIP_CHECK(hipMemsetAsync(c->adam_v, 0, (size_t)c->nparam * 4, st));
+  hipLaunchKernelGGL(k_to_bf16, dim3(256), dim3(256), 0, st, c->params, c->w16, c->nparam);
+  AERO_HIP_CHECK(hipGetLastError());
+  return AERO_OK;
+}
+
+int aero_cnn_get_params(aero_cnn *c, void *stream, float *params) {
+  if (!c || !params) { set_error("null argument"); return AERO_EINVAL; }
+  AERO_HIP_CHECK(hipMemcpyAsync(params, c->params, (size_t)c->nparam * 4, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  return AERO_OK;
+}
+
+int aero_cnn_get_grads(aero_cnn *c, void *stream, float *grads) {
+  if (!c || !grads) { set_error("null argument"); return AERO_EINVAL; }
+  AERO_HIP_CHECK(hipMemcpyAsync(grads, c->grads, (size_t)c->nparam * 4, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  return AERO_OK;
+}
+
+int aero_cnn_forward(aero_cnn *c, void *stream, const float *x, float *y) {
+  if (!c || !x || !y) { set_error("null argument"); return AERO_EINVAL; }
+  hipStream_t st = (hipStream_t)stream;
+  const aero_cnn_desc &d = c->desc;
+  AERO_HIP_CHECK(hipMemcpyAsync(c->act[0], x, (size_t)d.channels[0] * c->nxyz * 4, hipMemcpyDeviceToDevice, st));
+  for (int l = 0; l < d.n_layers; ++l) {
+    float *dst = l == d.n_layers - 1 ? y : c->act[l + 1];
+    launch_fwd(st, c->act[l], c->w16 + c->woff[l], c->params + c->boff[l], c->pre[l], dst, d.channels[l],
+               d.channels[l + 1], d.nz, d.ny, d.nx, d.act[l]);
+    if (l == d.n_layers - 1)
+      AERO_HIP_CHECK(hipMemcpyAsync(c->act[l + 1], y, (size_t)d.channels[l + 1] * c->nxyz * 4,
+                                    hipMemcpyDeviceToDevice, st));
+  }
+  AERO_HIP_CHECK(hipGetLastError());
+  return AERO_OK;
+}
+
+int aero_cnn_infer(aero_cnn *c, void *stream, const float *x, float *y) {
+  if (!c || !x || !y) { set_error("null argument"); return AERO_EINVAL; }
+  hipStream_t st = (hipStream_t)stream;
+  const aero_cnn_desc &d = c->desc;
+  const float *src = x;
+  for (int l = 0; l < d.n_layers; ++l) {
+    float *dst = l == d.n_layers - 1 ? y : (l % 2 == 0 ? c->g0 : c->g1);
+    launch_fwd(st, src, c->w16 + c->woff[l], c->params + c->boff[l], nullptr, dst, d.channels[l], d.channels[l + 1],
+               d.nz, d.ny, d.nx, d.act[l]);
+    src = dst;
+  }
+  AERO_HIP_CHECK(hipGetLastError());
+  return AERO_OK;
+}
+
+int aero_cnn_backward(aero_cnn *c, void *stream, const float *dy, float *dx) {
+  if (!c || !dy) { set_error("null argument"); return AERO_EINVAL; }
+  hipStream_t st = (hipStream_t)stream;
+  const aero_cnn_desc &d = c->desc;
+  const size_t n = c->nxyz;
+  const int nbr = ((d.nx + TX - 1) / TX) * ((d.ny + TY - 1) / TY) * ((d.nz + TZ - 1) / TZ);
+  const int npc = (nbr + kWgBricks - 1) / kWgBricks;
+  const float *up = dy;       // gradient w.r.t. the layer's activated output
+  for (int l = d.n_layers - 1; l >= 0; --l) {
+    const int ci = d.channels[l], co = d.channels[l + 1];
+    float *g = (l % 2 == 0) ? c->g0 : c->g1;            // gradient w.r.t. the pre-activation
+    hipLaunchKernelGGL(k_act_grad, dim3(1024), dim3(256), 0, st, up, c->pre[l], g, (size_t)co * n, d.act[l]);
+    const dim3 wg((ci + CI_CH - 1) / CI_CH, npc);
+    if (co == 1)
+      hipLaunchKernelGGL(k_conv_wgrad<1>, wg, dim3(256), 0, st, c->act[l], g, c->part, ci, d.nz, d.ny, d.nx,
+                         kWgBricks);
+    else if (co == 8)
+      hipLaunchKernelGGL(k_conv_wgrad<8>, wg, dim3(256), 0, st, c->act[l], g, c->part, ci, d.nz, d.ny, d.nx,
+                         kWgBricks);
+    else if (co == 9)
+      hipLaunchKernelGGL(k_conv_wgrad<9>, wg, dim3(256), 0, st, c->act[l], g, c->part, ci, d.nz, d.ny, d.nx,
+                         kWgBricks);
+    else if (co == 16)
+      hipLaunchKernelGGL(k_conv_wgrad<16>, wg, dim3(256), 0, st, c->act[l], g, c->part, ci, d.nz, d.ny, d.nx,
+                         kWgBricks);
+    else
+      hipLaunchKernelGGL(k_conv_wgrad<32>, wg, dim3(256), 0, st, c->act[l], g, c->part, ci, d.nz, d.ny, d.nx,
+                         kWgBricks);
+    const int nw = co * ci * 27;
+    hipLaunchKernelGGL(k_wgrad_reduce, dim3((nw + co + 255) / 256), dim3(256), 0, st, c->part, c->grads + c->woff[l],
+                       nw, co, npc);
+    if (l > 0 || dx) {
+      // data gradient: the forward convolution with flipped, transposed weights, no bias
+      hipLaunchKernelGGL(k_flip, dim3(64), dim3(256), 0, st, c->w16 + c->woff[l], c->wt16, co, ci);
+      float *dst = l > 0 ? ((l % 2 == 0) ? c->g1 : c->g0) : dx;
+      if (!cout_ok(ci)) {
+        set_error("input channels of a layer must be 1, 8, 9, 16 or 32 for its data gradient");
+        return AERO_EUNSUPPORTED;
+      }
+      launch_fwd(st, g, c->wt16, nullptr, nullptr, dst, co, ci, d.nz, d.ny, d.nx, 0);
+      up = dst;
+    }
+  }
+  AERO_HIP_CHECK(hipGetLastError());
+  return AERO_OK;
+}
+
+int aero_cnn_adam_step(aero_cnn *c, void *stream, float lr, float beta1, float beta2, float eps, int t) {
+  if (!c || t < 1) { set_error("bad Adam arguments"); return AERO_EINVAL; }
+  hipStream_t st = (hipStream_t)stream;
+  const float c1 = 1.f - std::pow(beta1, (float)t), c2 = 1.f - std::pow(beta2, (float)t);
+  hipLaunchKernelGGL(k_adam, dim3(256), dim3(256), 0, st, c->params, c->grads, c->adam_m, c->adam_v, c->nparam, lr,
+                     beta1, beta2, eps, c1, c2);
+  hipLaunchKernelGGL(k_to_bf16, dim3(256), dim3(256), 0, st, c->params, c->w16, c->nparam);
+  AERO_HIP_CHECK(hipGetLastError());
+  return AERO_OK;
+}
+
+}  // extern "C"

@@ -254,6 +254,36 @@ int aero2d_ensemble_create(int32_t n, aero2d_plan *const *plans, const aero2d_fi
 int aero2d_ensemble_step(aero2d_ensemble *ensemble, void *stream, int32_t chained);
 int aero2d_ensemble_destroy(aero2d_ensemble *ensemble);
 
+/* ------------------------------------------------------------------ learned closure (3D CNN) */
+/* A small convolutional network on a periodic (nz, ny, nx) grid: n_layers 3x3x3 convolutions,
+ * channels[0] inputs -> channels[n_layers] outputs, BF16 weights with FP32 accumulation; its
+ * forward, backward and Adam kernels train it on the GPU. No reference counterpart (the
+ * learned eddy-viscosity closure is new). Fields are channel-major (C, nz, ny, nx) float32
+ * device arrays; parameters are float32, per layer w[co][ci][3][3][3] then b[co]. */
+typedef struct aero_cnn aero_cnn;
+typedef struct {
+  int32_t nz, ny, nx;
+  int32_t n_layers;        /* 1..8 */
+  int32_t channels[9];     /* inputs 1..32; outputs of each layer 1, 8, 9, 16 or 32 */
+  int32_t act[8];          /* per layer: 0 identity, 1 relu, 2 softplus */
+} aero_cnn_desc;
+int aero_cnn_create(const aero_cnn_desc *desc, aero_cnn **cnn);
+int aero_cnn_destroy(aero_cnn *cnn);
+int aero_cnn_param_count(const aero_cnn *cnn);
+size_t aero_cnn_workspace_bytes(const aero_cnn *cnn);
+/* params: device float32 array of aero_cnn_param_count values; also resets the Adam moments */
+int aero_cnn_set_params(aero_cnn *cnn, void *stream, const float *params);
+int aero_cnn_get_params(aero_cnn *cnn, void *stream, float *params);
+/* training forward: keeps every layer's activations for aero_cnn_backward */
+int aero_cnn_forward(aero_cnn *cnn, void *stream, const float *x, float *y);
+/* inference forward (keeps nothing; may run while a training forward's activations are kept) */
+int aero_cnn_infer(aero_cnn *cnn, void *stream, const float *x, float *y);
+/* parameter gradients of sum(dy * y) of the last aero_cnn_forward into the plan; dx optional */
+int aero_cnn_backward(aero_cnn *cnn, void *stream, const float *dy, float *dx);
+int aero_cnn_get_grads(aero_cnn *cnn, void *stream, float *grads);
+/* one Adam step (t >= 1) on the float32 parameters, then their BF16 copies */
+int aero_cnn_adam_step(aero_cnn *cnn, void *stream, float lr, float beta1, float beta2, float eps, int t);
+
 const char *aero_last_error(void);
 int aero_version(void);
 

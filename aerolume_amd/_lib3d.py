@@ -17,7 +17,8 @@ class Desc3D(ctypes.Structure):
                 ("win_k0", i32), ("win_k1", i32), ("win_j0", i32), ("win_j1", i32), ("win_i0", i32),
                 ("win_i1", i32), ("nbody", i32), ("symmetry", i32), ("ghost_kind", i32 * 18),
                 ("ghost_value", f64 * 18), ("h", f64), ("y0", f64), ("cfl", f64), ("les_c", f64),
-                ("ref_area", f64), ("length_ref", f64), ("span_ref", f64), ("moment_ref", f64 * 3)]
+                ("ref_area", f64), ("length_ref", f64), ("span_ref", f64), ("moment_ref", f64 * 3),
+                ("nut_model", i32), ("reserved", i32), ("nut_trust", f64)]
 
 
 class Fields3D(ctypes.Structure):
@@ -29,7 +30,7 @@ class Geom3D(ctypes.Structure):
 
 
 class Params3D(ctypes.Structure):
-    _fields_ = [(k, vp) for k in ("nu", "eta", "u_inlet", "dt_fixed")]
+    _fields_ = [(k, vp) for k in ("nu", "eta", "u_inlet", "dt_fixed", "nut_field")]
 
 
 class Outputs3D(ctypes.Structure):
@@ -47,9 +48,10 @@ def bind(lib) -> None:
     lib.aero3d_step_chained.argtypes = lib.aero3d_step.argtypes
     lib.aero3d_project.argtypes = [P, P, P, P, P, P]
     lib.aero3d_poisson_solve.argtypes = [P, P, P, P]
+    lib.aero3d_cell_gradient.argtypes = [P, P, P, P, P, P, ctypes.c_double]
     lib.aero3d_time_stage.argtypes = [P, P, ctypes.POINTER(Fields3D), ctypes.POINTER(Geom3D),
                                       ctypes.POINTER(Params3D), ctypes.c_int, ctypes.c_int,
                                       ctypes.POINTER(ctypes.c_double)]
     for name in ("aero3d_plan_create", "aero3d_plan_destroy", "aero3d_step", "aero3d_step_chained",
-                 "aero3d_project", "aero3d_poisson_solve", "aero3d_time_stage"):
+                 "aero3d_project", "aero3d_poisson_solve", "aero3d_time_stage", "aero3d_cell_gradient"):
         getattr(lib, name).restype = ctypes.c_int

@@ -80,6 +80,8 @@ struct Dev3 {
   int win, wk0, wk1, wj0, wj1, wi0, wi1, wz, wy, wx;
   int wbx0, wby0, wbz0, nwbx, nwby, nwbz, nwt;
   int nbody, symmetry, nq;
+  int nut_model;
+  T nut_trust;
   int rule[18];
   T rval2[18];
   T h, inv_h, inv_h2, kadv, les_k, cfl_h, visc_h2, corr_scale;
@@ -104,6 +106,7 @@ struct Call3 {
   T *uo, *vo, *wo;           // outputs
   const T *chi_u, *chi_v, *chi_w, *chi_c, *a_u, *a_v, *a_w, *k_u, *k_v, *k_w;
   const T *nu, *eta, *u_inlet, *dt_fixed;
+  const T *nut_field;        // learned-closure eddy viscosity (cells), when d.nut_model
   int stage;
   int corr_stage;            // red->corr slot added at the outlet face of up (-1: none)
 };
@@ -181,6 +184,8 @@ __device__ __forceinline__ void block_sum_w(double (&v)[N], double *lds) {
       v[q] = r;
     }
 }
+
+__device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
 // Source index of position g along an axis of array extent ext (n cells): periodic axes wrap
 // (a face axis carries the duplicate face n, so it wraps by n), others clamp; positions the
@@ -385,6 +390,16 @@ __device__ T nu_tile(const Dev3<T> &d, const Call3<T> &c, Lds3<T> &L, int k0, in
     cell_grad(L, k, j, i, d.inv_h, g);
     T nu = eddy3(d.turb, d.les_k, g);
     const int gi = i0 + i, gj = j0 + j, gk = k0 + k;
+    if (d.nut_model) {
+      // learned closure where its value is trusted (cells of the domain; periodic ghosts wrap)
+      const bool in_i = (gi >= 0 && gi < d.nx) || d.per[0], in_j = (gj >= 0 && gj < d.ny) || d.per[1],
+                 in_k = (gk >= 0 && gk < d.nz) || d.per[2];
+      if (in_i && in_j && in_k && gi <= d.nx && gj <= d.ny && gk <= d.nz) {
+        const int wi = wrapi(gi, d.nx), wj = wrapi(gj, d.ny), wk = wrapi(gk, d.nz);
+        const T f = c.nut_field[((size_t)wk * d.ny + wj) * d.nx + wi];
+        if (f >= (T)0 && f <= d.nut_trust) nu = f;
+      }
+    }
     if (d.win && gk >= d.wk0 && gk < d.wk1 && gj >= d.wj0 && gj < d.wj1 && gi >= d.wi0 && gi < d.wi1)
       nu = nu * ((T)1 - c.chi_c[((size_t)(gk - d.wk0) * d.wy + (gj - d.wj0)) * d.wx + (gi - d.wi0)]);
     L.nu[z][y][x] = nu;
@@ -706,6 +721,32 @@ __global__ void __launch_bounds__(kTileThreads, Tile<T>::WAVES_PER_SIMD) k_final
         __syncthreads();
       }
     }
+  }
+}
+
+// Cell-centred velocity gradient of a state, times scale: out[3 i + j] = du_i/dx_j.
+template <typename T>
+__global__ void __launch_bounds__(kTileThreads, Tile<T>::WAVES_PER_SIMD) k_grad(const Dev3<T> *__restrict__ dp,
+                                                                                Call3<T> c, T *__restrict__ out,
+                                                                                T scale) {
+  using S = Lds3<T>;
+  extern __shared__ unsigned char smem[];
+  S &L = *reinterpret_cast<S *>(smem);
+  const Dev3<T> &d = *dp;
+  int bx, by, bz, tile;
+  tile_of(d, bx, by, bz, tile);
+  const int i0 = bx * S::TX, j0 = by * S::TY, k0 = bz * S::TZ;
+  load_tile(d, c, L, k0, j0, i0, (T)0);
+  const int ie = min(i0 + S::TX, d.nx), je = min(j0 + S::TY, d.ny), ke = min(k0 + S::TZ, d.nz);
+  const int nfx = ie - i0, nfy = je - j0, nfz = ke - k0;
+  const size_t nc = (size_t)d.nx * d.ny * d.nz;
+  for (int e = threadIdx.x; e < nfx * nfy * nfz; e += kTileThreads) {
+    const int li = e % nfx, t = e / nfx, lj = t % nfy, lk = t / nfy;
+    T g[3][3];
+    cell_grad(L, lk, lj, li, d.inv_h, g);
+    const size_t p = ((size_t)(k0 + lk) * d.ny + (j0 + lj)) * d.nx + (i0 + li);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) out[q * nc + p] = g[q / 3][q % 3] * scale;
   }
 }
 
@@ -1273,6 +1314,8 @@ int upload3(aero3d_plan *p) {
   }
   d.nbody = d.win ? s.nbody : 0;
   d.symmetry = s.symmetry;
+  d.nut_model = s.nut_model;
+  d.nut_trust = (T)s.nut_trust;
   for (int q = 0; q < 18; ++q) {
     d.rule[q] = s.ghost_kind[q];
     d.rval2[q] = (T)(2.0 * s.ghost_value[q]);
@@ -1327,6 +1370,8 @@ int upload3(aero3d_plan *p) {
                                      (int)sizeof(Lds3<T>)));
   AERO_HIP_CHECK(hipFuncSetAttribute((const void *)k_final<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)sizeof(Lds3<T>)));
+  AERO_HIP_CHECK(hipFuncSetAttribute((const void *)k_grad<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)sizeof(Lds3<T>)));
   return AERO_OK;
 }
 
@@ -1366,6 +1411,7 @@ Call3<T> make_call(const aero3d_fields *in, const aero3d_geom *g, const aero3d_p
     c.eta = (const T *)pr->eta;
     c.u_inlet = (const T *)pr->u_inlet;
     c.dt_fixed = (const T *)pr->dt_fixed;
+    c.nut_field = (const T *)pr->nut_field;
   }
   c.corr_stage = -1;
   return c;
@@ -1474,6 +1520,20 @@ int poisson_impl(aero3d_plan *p, hipStream_t st, const void *f, void *psi) {
 }
 
 template <typename T>
+int grad_impl(aero3d_plan *p, hipStream_t st, const void *u, const void *v, const void *w, void *out, double scale) {
+  const Dev3<T> *dev = (const Dev3<T> *)p->dev;
+  Call3<T> c;
+  std::memset(&c, 0, sizeof(c));
+  c.up = (const T *)u;
+  c.vp = (const T *)v;
+  c.wp = (const T *)w;
+  c.corr_stage = -1;
+  hipLaunchKernelGGL(k_grad<T>, dim3(p->nblk), dim3(kTileThreads), sizeof(Lds3<T>), st, dev, c, (T *)out, (T)scale);
+  AERO_HIP_CHECK(hipGetLastError());
+  return AERO_OK;
+}
+
+template <typename T>
 int time_stage_impl(aero3d_plan *p, hipStream_t st, const aero3d_fields *fl, const aero3d_geom *g,
                     const aero3d_params *pr, int stage, int iters, double *avg_ms) {
   const Dev3<T> *dev = (const Dev3<T> *)p->dev;
@@ -1514,6 +1574,10 @@ int check3_desc(const aero3d_desc *d) {
     return AERO_EINVAL;
   }
   if (d->closed && d->periodic_x) { set_error("a closed domain cannot be periodic in x"); return AERO_EINVAL; }
+  if (d->nut_model && d->turbulence == 0) {
+    set_error("a learned closure needs a turbulence model to fall back to");
+    return AERO_EINVAL;
+  }
   for (int q = 0; q < 18; ++q)
     if (d->ghost_kind[q] < 0 || d->ghost_kind[q] > 6) { set_error("unknown ghost rule"); return AERO_EINVAL; }
   if (d->nbody < 0 || d->nbody > kMaxBodies) { set_error("at most 8 bodies"); return AERO_EUNSUPPORTED; }
@@ -1538,6 +1602,10 @@ int check3_step(const aero3d_plan *p, const aero3d_fields *in, const aero3d_fiel
   if (!p || !in || !out || !pr) { set_error("null plan or argument"); return AERO_EINVAL; }
   if (!in->u || !in->v || !in->w || !in->t || !out->u || !out->v || !out->w || !out->p || !out->t || !out->dt) {
     set_error("missing field pointer");
+    return AERO_EINVAL;
+  }
+  if (p->desc.nut_model && !pr->nut_field) {
+    set_error("a learned-closure plan needs params.nut_field");
     return AERO_EINVAL;
   }
   if (!pr->nu || !pr->eta || !pr->u_inlet || (p->desc.fixed_dt && !pr->dt_fixed)) {
@@ -1658,6 +1726,13 @@ int aero3d_poisson_solve(aero3d_plan *p, void *stream, const void *f, void *psi)
   if (!p || !f || !psi) { set_error("null plan or field"); return AERO_EINVAL; }
   hipStream_t st = (hipStream_t)stream;
   return p->dbl ? poisson_impl<double>(p, st, f, psi) : poisson_impl<float>(p, st, f, psi);
+}
+
+int aero3d_cell_gradient(aero3d_plan *p, void *stream, const void *u, const void *v, const void *w, void *out,
+                         double scale) {
+  if (!p || !u || !v || !w || !out) { set_error("null plan or field"); return AERO_EINVAL; }
+  hipStream_t st = (hipStream_t)stream;
+  return p->dbl ? grad_impl<double>(p, st, u, v, w, out, scale) : grad_impl<float>(p, st, u, v, w, out, scale);
 }
 
 int aero3d_time_stage(aero3d_plan *p, void *stream, const aero3d_fields *fields, const aero3d_geom *g,

@@ -188,6 +188,10 @@ typedef struct {
   double cfl, les_c;
   double ref_area, length_ref, span_ref;
   double moment_ref[3];
+  int32_t nut_model;        /* 1: eddy viscosity from params.nut_field where it lies in
+                               [0, nut_trust], the turbulence model elsewhere (learned closure) */
+  int32_t reserved;
+  double nut_trust;
 } aero3d_desc;
 
 /* Caller-owned device buffers of the plan's dtype: u (nz, ny, nx+1), v (nz, ny+1, nx),
@@ -204,9 +208,11 @@ typedef struct {
   const void *chi_u, *chi_v, *chi_w, *chi_c, *a_u, *a_v, *a_w, *k_u, *k_v, *k_w;
 } aero3d_geom;
 
-/* Live device scalars / arrays: nu, eta, dt_fixed 0-d; u_inlet (nz, ny). */
+/* Live device scalars / arrays: nu, eta, dt_fixed 0-d; u_inlet (nz, ny); nut_field (nz, ny, nx)
+ * cell-centred eddy viscosity of a learned closure (read when desc.nut_model is set). */
 typedef struct {
   const void *nu, *eta, *u_inlet, *dt_fixed;
+  const void *nut_field;
 } aero3d_params;
 
 /* Per-step outputs (device; any may be NULL): coeffs (nbody, 6) = [CD, CS, CL, C_roll, Cm, Cn];
@@ -231,6 +237,10 @@ int aero3d_step_chained(aero3d_plan *plan, void *stream, const aero3d_fields *in
 int aero3d_project(aero3d_plan *plan, void *stream, void *u, void *v, void *w, void *psi);
 /* psi = L^-1 f with zero mean (f, psi (nz, ny, nx); may alias). */
 int aero3d_poisson_solve(aero3d_plan *plan, void *stream, const void *f, void *psi);
+/* Cell-centred velocity gradient g[i][j] = du_i/dx_j (9, nz, ny, nx), times `scale`, with the
+ * plan's ghost rules (ns3d/core.py:400-434): the learned closure's input features. */
+int aero3d_cell_gradient(aero3d_plan *plan, void *stream, const void *u, const void *v, const void *w, void *out,
+                         double scale);
 /* Timing helper (no reference counterpart; bench.py prices the dominant kernel with it):
  * launches the stage kernel of RK stage `stage` `iters` times on `stream`, reading `fields`
  * as stage input and step start and writing only plan scratch, between two HIP events;

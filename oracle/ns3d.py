@@ -332,13 +332,32 @@ def eddy_viscosity3d(g, ops):
     raise ValueError(ops.turbulence)
 
 
-def sgs_divergence3d(U, V, W, geom, ops):
-    """div(2 nu_t S) at the faces, max nu_t (core.py:491-534)."""
+def learned_nut(nut, field, trust, ops):
+    """The learned closure's eddy viscosity where it lies in [0, trust], the model's elsewhere
+    (no reference counterpart). ``nut`` covers cells -1..n per axis; ``field`` (nz, ny, nx)
+    wraps onto the ghost ring of periodic axes; non-periodic ghost cells keep the model."""
+    nz, ny, nx = ops.nz, ops.ny, ops.nx
+    idx = []
+    for n, per in zip((nz, ny, nx), reversed(ops.periodic_axes)):
+        idx.append(torch.arange(-1, n + 1) % n if per else torch.arange(0, n))
+    sub = field[idx[0][:, None, None], idx[1][None, :, None], idx[2][None, None, :]]
+    sl = tuple(slice(None) if per else slice(1, -1) for per in reversed(ops.periodic_axes))
+    ok = (sub >= 0) & (sub <= trust)
+    out = nut.clone()
+    out[sl] = torch.where(ok, sub, nut[sl])
+    return out
+
+
+def sgs_divergence3d(U, V, W, geom, ops, nut_field=None, nut_trust=None):
+    """div(2 nu_t S) at the faces, max nu_t (core.py:491-534); with ``nut_field`` the learned
+    closure's values replace the model's where trusted (before the solid mask)."""
     nz, ny, nx = ops.nz, ops.ny, ops.nx
     inv = 1.0 / ops.h
     gr = velocity_gradient(U, V, W, ops.h)
     g = gr["g"]
     nut = eddy_viscosity3d(traceless(g), ops)
+    if nut_field is not None:
+        nut = learned_nut(nut, nut_field, nut_trust, ops)
     if not geom.empty:
         k0, k1, j0, j1, i0, i1 = geom.window
         win = nut[k0 + 1:k1 + 1, j0 + 1:j1 + 1, i0 + 1:i1 + 1] * (1.0 - geom.chi_c)
@@ -379,7 +398,7 @@ def boundary_tendencies(u, lu, lv, lw, ops):
     return lu, lv, lw
 
 
-def momentum_rhs3d(u, v, w, geom, ops):
+def momentum_rhs3d(u, v, w, geom, ops, nut_field=None, nut_trust=None):
     U, V, W = pad_velocity(u, v, w, ops)
     au, av, aw = advection3(U, V, W, ops)
     nu_h2 = ops.nu * (1.0 / ops.h ** 2)
@@ -388,7 +407,7 @@ def momentum_rhs3d(u, v, w, geom, ops):
     lw = torch.addcmul(aw, laplacian_padded(W), nu_h2)
     nut_max = None
     if ops.turbulence != "none":
-        fu, fv, fw, nut_max = sgs_divergence3d(U, V, W, geom, ops)
+        fu, fv, fw, nut_max = sgs_divergence3d(U, V, W, geom, ops, nut_field, nut_trust)
         lu, lv, lw = lu + fu, lv + fv, lw + fw
     lu, lv, lw = boundary_tendencies(u, lu, lv, lw, ops)
     return lu, lv, lw, nut_max
@@ -473,8 +492,9 @@ def force_coefficients(acc, geom, ops, dt):
                         mz / (q_a * ops.span_ref)], dim=1)
 
 
-def step3d_full(state: State3D, geom, ops, dt=None):
-    """One SSP-RK3 step without placed physics (core.py:765-883)."""
+def step3d_full(state: State3D, geom, ops, dt=None, nut_field=None, nut_trust=None):
+    """One SSP-RK3 step without placed physics (core.py:765-883); ``nut_field`` / ``nut_trust``:
+    the learned closure's eddy viscosity, held over the step (no reference counterpart)."""
     u0, v0, w0 = state.u, state.v, state.w
     up, vp, wp = u0, v0, w0
     acc = None
@@ -482,7 +502,7 @@ def step3d_full(state: State3D, geom, ops, dt=None):
     px = ops.periodic_axes[0]
     any_periodic = any(ops.periodic_axes)
     for s, ((a, b), wgt) in enumerate(zip(SSP_RK3, FORCE_WEIGHTS)):
-        lu, lv, lw, nut_max = momentum_rhs3d(up, vp, wp, geom, ops)
+        lu, lv, lw, nut_max = momentum_rhs3d(up, vp, wp, geom, ops, nut_field, nut_trust)
         if s == 0:
             if dt is None:
                 dt = ops.extras["dt_fixed"] if ops.dt_fixed is not None else cfl_dt3d(u0, v0, w0, ops, nut_max)

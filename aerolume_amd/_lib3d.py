@@ -18,11 +18,13 @@ class Desc3D(ctypes.Structure):
                 ("win_i1", i32), ("nbody", i32), ("symmetry", i32), ("ghost_kind", i32 * 18),
                 ("ghost_value", f64 * 18), ("h", f64), ("y0", f64), ("cfl", f64), ("les_c", f64),
                 ("ref_area", f64), ("length_ref", f64), ("span_ref", f64), ("moment_ref", f64 * 3),
-                ("nut_model", i32), ("reserved", i32), ("nut_trust", f64)]
+                ("nut_model", i32), ("reserved", i32), ("nut_trust", f64), ("dye", i32), ("dye_k0", i32),
+                ("dye_k1", i32), ("dye_j0", i32), ("dye_j1", i32), ("dye_i0", i32), ("dye_i1", i32),
+                ("reserved2", i32)]
 
 
 class Fields3D(ctypes.Structure):
-    _fields_ = [(k, vp) for k in ("u", "v", "w", "p", "t", "dt")]
+    _fields_ = [(k, vp) for k in ("u", "v", "w", "p", "t", "dt", "dye")]
 
 
 class Geom3D(ctypes.Structure):
@@ -30,7 +32,7 @@ class Geom3D(ctypes.Structure):
 
 
 class Params3D(ctypes.Structure):
-    _fields_ = [(k, vp) for k in ("nu", "eta", "u_inlet", "dt_fixed", "nut_field")]
+    _fields_ = [(k, vp) for k in ("nu", "eta", "u_inlet", "dt_fixed", "nut_field", "dye_weight", "dye_pulse")]
 
 
 class Outputs3D(ctypes.Structure):

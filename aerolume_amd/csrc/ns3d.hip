@@ -82,6 +82,7 @@ struct Dev3 {
   int nbody, symmetry, nq;
   int nut_model;
   T nut_trust;
+  int dye, dk0, dk1, dj0, dj1, di0, di1, dwx, dwy;
   int rule[18];
   T rval2[18];
   T h, inv_h, inv_h2, kadv, les_k, cfl_h, visc_h2, corr_scale;
@@ -107,6 +108,7 @@ struct Call3 {
   const T *chi_u, *chi_v, *chi_w, *chi_c, *a_u, *a_v, *a_w, *k_u, *k_v, *k_w;
   const T *nu, *eta, *u_inlet, *dt_fixed;
   const T *nut_field;        // learned-closure eddy viscosity (cells), when d.nut_model
+  const T *dye_weight, *dye_pulse;
   int stage;
   int corr_stage;            // red->corr slot added at the outlet face of up (-1: none)
 };
@@ -868,6 +870,89 @@ __global__ void __launch_bounds__(kFinThreads) k_finalize(const Dev3<T> *__restr
   if (f.next_dt && !d.fixed_dt) d.red->dt = cfl_dt(d, c, mx[0], mx[1], mx[2], mx[3]);
 }
 
+
+// ------------------------------------------------------------------ smoke
+// Smoke value at a cell one axis away from the domain (ns3d/dye.py:232-263): x clean inflow
+// / zero-gradient outlet, y and z mirrored, periodic axes wrapped.
+template <typename T>
+__device__ __forceinline__ T dye_at(const Dev3<T> &d, const T *__restrict__ c, int k, int j, int i) {
+  if (i < 0 || i >= d.nx) {
+    if (d.per[0]) i = wrapi(i, d.nx);
+    else if (i < 0) return (T)0;
+    else i = d.nx - 1;
+  }
+  if (j < 0 || j >= d.ny) j = d.per[1] ? wrapi(j, d.ny) : (j < 0 ? -1 - j : 2 * d.ny - 1 - j);
+  if (k < 0 || k >= d.nz) k = d.per[2] ? wrapi(k, d.nz) : (k < 0 ? -1 - k : 2 * d.nz - 1 - k);
+  return c[((size_t)k * d.ny + j) * d.nx + i];
+}
+
+// upwind MUSCL flux with the van Leer slope through the face between b and c (dye.py:267-278)
+template <typename T>
+__device__ __forceinline__ T dye_flux(T a, T b, T c, T dd, T vel) {
+  T p, q, r;
+  if (vel > (T)0) {
+    p = b - a;
+    q = c - b;
+    const T pq = p * q;
+    r = b + (T)0.5 * (pq > (T)0 ? (T)2 * pq / (p + q) : (T)0);
+  } else {
+    p = c - b;
+    q = dd - c;
+    const T pq = p * q;
+    r = c - (T)0.5 * (pq > (T)0 ? (T)2 * pq / (p + q) : (T)0);
+  }
+  return vel * r;
+}
+
+// One smoke step with the end-of-step velocity: forward Euler, clipped at 1, floored; zero in
+// solid cells (chi_c > 1/2); relaxed to 1 in the nozzles (dye.py:281-312).
+template <typename T>
+__global__ void __launch_bounds__(256) k_dye(const Dev3<T> *__restrict__ dp, Call3<T> c, const T *__restrict__ cin,
+                                            T *__restrict__ cout, const T *__restrict__ t_in, T floor_v) {
+  const Dev3<T> &d = *dp;
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  const size_t n = (size_t)nx * ny * nz;
+  const T dt = step_dt(d, c);
+  const T kk = dt * d.inv_h;
+  const T t = *t_in + dt;
+  T on = 0;
+  if (d.dye && d.dk1 > d.dk0) {
+    const T tm = fmod(t, c.dye_pulse[1]);
+    on = tm < c.dye_pulse[0] ? (T)1 : (T)0;
+  }
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e % nx), j = (int)((e / nx) % ny), k = (int)(e / ((size_t)nx * ny));
+    const T c0 = cin[e];
+    const T *uu = c.up + ((size_t)k * ny + j) * (nx + 1) + i;
+    const T *vv = c.vp + ((size_t)k * (ny + 1) + j) * nx + i;
+    const T *ww = c.wp + ((size_t)k * ny + j) * nx + i;
+    const T fx0 = dye_flux(dye_at(d, cin, k, j, i - 2), dye_at(d, cin, k, j, i - 1), c0, dye_at(d, cin, k, j, i + 1), uu[0]);
+    const T fx1 = dye_flux(dye_at(d, cin, k, j, i - 1), c0, dye_at(d, cin, k, j, i + 1), dye_at(d, cin, k, j, i + 2), uu[1]);
+    const T fy0 = dye_flux(dye_at(d, cin, k, j - 2, i), dye_at(d, cin, k, j - 1, i), c0, dye_at(d, cin, k, j + 1, i), vv[0]);
+    const T fy1 = dye_flux(dye_at(d, cin, k, j - 1, i), c0, dye_at(d, cin, k, j + 1, i), dye_at(d, cin, k, j + 2, i), vv[nx]);
+    const T fz0 = dye_flux(dye_at(d, cin, k - 2, j, i), dye_at(d, cin, k - 1, j, i), c0, dye_at(d, cin, k + 1, j, i), ww[0]);
+    const T fz1 = dye_flux(dye_at(d, cin, k - 1, j, i), c0, dye_at(d, cin, k + 1, j, i), dye_at(d, cin, k + 2, j, i),
+                           ww[(size_t)ny * nx]);
+    T r = c0 - kk * (((fx1 - fx0) + (fy1 - fy0)) + (fz1 - fz0));
+    r = r > (T)1 ? (T)1 : r;
+    r = r < floor_v ? (T)0 : r;
+    if (d.win && k >= d.wk0 && k < d.wk1 && j >= d.wj0 && j < d.wj1 && i >= d.wi0 && i < d.wi1 &&
+        c.chi_c[((size_t)(k - d.wk0) * d.wy + (j - d.wj0)) * d.wx + (i - d.wi0)] > (T)0.5)
+      r = (T)0;
+    if (k >= d.dk0 && k < d.dk1 && j >= d.dj0 && j < d.dj1 && i >= d.di0 && i < d.di1) {
+      const T wgt = c.dye_weight[((size_t)(k - d.dk0) * d.dwy + (j - d.dj0)) * d.dwx + (i - d.di0)];
+      r = r + wgt * on * ((T)1 - r);
+    }
+    cout[e] = r;
+  }
+}
+
+template <typename T>
+__global__ void k_copy3(T *__restrict__ dst, const T *__restrict__ src, size_t n) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+    dst[e] = src[e];
+}
+
 // ------------------------------------------------------------------ Poisson passes
 __device__ __forceinline__ int makhoul_src(int n, int len) { return n < (len + 1) / 2 ? 2 * n : 2 * (len - 1 - n) + 1; }
 
@@ -1223,7 +1308,7 @@ struct aero3d_plan {
   size_t bytes = 0;
   std::vector<void *> allocs;
   void *dev = nullptr;
-  void *uA, *vA, *wA, *uB, *vB, *wB, *F, *psi, *acc_u, *acc_v, *acc_w, *pmax, *pout, *pforce, *red;
+  void *uA, *vA, *wA, *uB, *vB, *wB, *F, *psi, *acc_u, *acc_v, *acc_w, *pmax, *pout, *pforce, *red, *dye_tmp;
   void *tw[3], *cs[3];
   double *lam[3];
   FftPlan fp[3];
@@ -1316,6 +1401,12 @@ int upload3(aero3d_plan *p) {
   d.symmetry = s.symmetry;
   d.nut_model = s.nut_model;
   d.nut_trust = (T)s.nut_trust;
+  d.dye = s.dye;
+  if (s.dye && s.dye_k1 > s.dye_k0 && s.dye_j1 > s.dye_j0 && s.dye_i1 > s.dye_i0) {
+    d.dk0 = s.dye_k0; d.dk1 = s.dye_k1; d.dj0 = s.dye_j0; d.dj1 = s.dye_j1; d.di0 = s.dye_i0; d.di1 = s.dye_i1;
+    d.dwx = s.dye_i1 - s.dye_i0;
+    d.dwy = s.dye_j1 - s.dye_j0;
+  }
   for (int q = 0; q < 18; ++q) {
     d.rule[q] = s.ghost_kind[q];
     d.rval2[q] = (T)(2.0 * s.ghost_value[q]);
@@ -1412,6 +1503,8 @@ Call3<T> make_call(const aero3d_fields *in, const aero3d_geom *g, const aero3d_p
     c.u_inlet = (const T *)pr->u_inlet;
     c.dt_fixed = (const T *)pr->dt_fixed;
     c.nut_field = (const T *)pr->nut_field;
+    c.dye_weight = (const T *)pr->dye_weight;
+    c.dye_pulse = (const T *)pr->dye_pulse;
   }
   c.corr_stage = -1;
   return c;
@@ -1473,6 +1566,19 @@ int step_impl(aero3d_plan *p, hipStream_t st, const aero3d_fields *in, aero3d_fi
   c.vo = (T *)out->v;
   c.wo = (T *)out->w;
   hipLaunchKernelGGL(k_final<T>, grid, blk, lds, st, dev, c, 1);
+  if (s.dye && in->dye && out->dye) {
+    // smoke with the end-of-step velocity, before the finalize advances t
+    Call3<T> cd = c;
+    cd.up = (const T *)out->u;
+    cd.vp = (const T *)out->v;
+    cd.wp = (const T *)out->w;
+    const size_t n = (size_t)s.nx * s.ny * s.nz;
+    const int nb = (int)std::min<size_t>(8192, (n + 255) / 256);
+    T *dst = in->dye == out->dye ? (T *)p->dye_tmp : (T *)out->dye;
+    hipLaunchKernelGGL(k_dye<T>, dim3(nb), dim3(256), 0, st, dev, cd, (const T *)in->dye, dst, (const T *)in->t,
+                       (T)1e-8);
+    if (dst != (T *)out->dye) hipLaunchKernelGGL(k_copy3<T>, dim3(nb), dim3(256), 0, st, (T *)out->dye, (const T *)dst, n);
+  }
   Fin3<T> f;
   std::memset(&f, 0, sizeof(f));
   f.t_in = (const T *)in->t;
@@ -1604,6 +1710,10 @@ int check3_step(const aero3d_plan *p, const aero3d_fields *in, const aero3d_fiel
     set_error("missing field pointer");
     return AERO_EINVAL;
   }
+  if (p->desc.dye && p->desc.dye_k1 > p->desc.dye_k0 && (!pr->dye_weight || !pr->dye_pulse)) {
+    set_error("smoke nozzles need params.dye_weight and dye_pulse");
+    return AERO_EINVAL;
+  }
   if (p->desc.nut_model && !pr->nut_field) {
     set_error("a learned-closure plan needs params.nut_field");
     return AERO_EINVAL;
@@ -1673,6 +1783,7 @@ int aero3d_plan_create(const aero3d_desc *desc, aero3d_plan **out) {
       {&p->acc_u, wz * wy * (wx + 1) * es}, {&p->acc_v, wz * (wy + 1) * wx * es}, {&p->acc_w, (wz + 1) * wy * wx * es},
       {&p->pmax, (size_t)p->nblk * 4 * es}, {&p->pout, (size_t)p->nby * p->nbz * 2 * 8},
       {&p->pforce, (size_t)std::max(1, p->nwt) * std::max(1, p->nq) * 8}, {&p->red, 256},
+      {&p->dye_tmp, desc->dye ? nc * es : 0},
       {&p->tw[0], nx * 2 * es}, {&p->tw[1], ny * 2 * es}, {&p->tw[2], nz * 2 * es},
       {&p->cs[0], nx * 2 * es}, {&p->cs[1], ny * 2 * es}, {&p->cs[2], nz * 2 * es},
       {(void **)&p->lam[0], nx * 8}, {(void **)&p->lam[1], ny * 8}, {(void **)&p->lam[2], nz * 8}};

@@ -1,4 +1,4 @@
 """3D solver: ``SolverConfig3D`` / ``FlowSolver3D`` API on the MI355X step."""
-from .config import AnalyticBody, ImmersedMesh, SolverConfig3D
+from .config import AnalyticBody, DyeRake, ImmersedMesh, SolverConfig3D
 
-__all__ = ["AnalyticBody", "ImmersedMesh", "SolverConfig3D"]
+__all__ = ["AnalyticBody", "DyeRake", "ImmersedMesh", "SolverConfig3D"]

@@ -16,7 +16,7 @@ from typing import NamedTuple
 import numpy as np
 import torch
 
-from .config import AnalyticBody, ImmersedMesh, SolverConfig3D
+from .config import AnalyticBody, DyeRake, ImmersedMesh, SolverConfig3D
 
 SSP_RK3 = ((0.0, 1.0), (0.75, 0.25), (1.0 / 3.0, 2.0 / 3.0))
 FORCE_WEIGHTS = (1.0 / 6.0, 2.0 / 3.0, 1.0)
@@ -26,6 +26,9 @@ STIFFNESS_MIN = 60.0
 MASK_SHIFT = 0.32
 VISCOUS_DT = 1.0 / 6.0
 COEFF_NAMES = ("cd", "cs", "cl", "c_roll", "cm", "cn")
+DYE_FLOOR = 1e-8                 # smoke below this is set to exactly zero
+NOZZLE_RADIUS_CELLS = 1.25
+NOZZLE_DEPTH_CELLS = 1.0
 
 # ghost-rule kinds of the C ABI (one per side of every axis of every component)
 RULE_CODES = {"edge": 0, "even": 1, "odd": 2, "affine": 3, "face_odd": 4, "periodic_cell": 5,
@@ -105,8 +108,8 @@ def check_supported(cfg: SolverConfig3D) -> None:
         missing.append("the diagnostics history")
     if cfg.adaptive_dt != "cfl":
         missing.append("adaptive_dt='pid'")
-    if cfg.dye:
-        missing.append("smoke (set dye=False)")
+    if cfg.dye and cfg.emitters is not None:
+        missing.append("Emitter3D smoke sources (the rake of dye_rake is carried)")
     for b in cfg.bodies:
         if b.thermal != "adiabatic":
             missing.append("heated bodies")
@@ -464,3 +467,121 @@ def build_geometry3d(bodies, ops: Ops3D, *, window=None) -> Geometry3D:
     k_w = torch.stack([ww, ww * (yw - yr), -ww * (xw - xr)], dim=1)
     g = Geometry3D(tuple(int(c) for c in window), chi_u, chi_v, chi_w, chi_c, a_u, a_v, a_w, k_u, k_v, k_w, nb)
     return g.to(ops.device, ops.dtype)
+
+
+# ---------------------------------------------------------------------------
+# smoke sources (reference ns3d/dye.py:84-250)
+# ---------------------------------------------------------------------------
+
+def default_dye_rake(bounds, extent, h: float, *, kind: str = "rake", symmetry: bool = False, ground: bool = False,
+                     n: int | None = None) -> DyeRake:
+    """Nozzles placed from the bodies' bounding box as the reference places them
+    (``ns3d/dye.py:84-150``): a release plane 0.4 of the body's smaller size upstream of the
+    nose (at least four cells from the inlet); "rake": one row across the span at 40 % of the
+    body height, 20 % wider than the body on each side, ~6 nozzle radii apart; "grid": the
+    frontal box with 10 % margins; "plane": a band over it. With a symmetry plane only the
+    solved side is kept; on a road the lowest nozzle stays two cells above it."""
+    bx0, bx1, by0, by1, bz0, bz1 = (float(b) for b in bounds)
+    ex0, _, ey0, ey1, ez0, ez1 = (float(e) for e in extent)
+    wy, hz = max(by1 - by0, h), max(bz1 - bz0, h)
+    size = max(wy, hz)
+    radius = NOZZLE_RADIUS_CELLS * h
+    x = bx0 - max(0.4 * min(size, max(bx1 - bx0, h)), 4.0 * h)
+    x = min(max(x, ex0 + 4.0 * h), bx0 - 2.0 * h)
+    ya, yb = by0 - 0.2 * wy, by1 + 0.2 * wy
+    ya = max(ya, 2.0 * ey0 - (ey1 - radius - h)) if symmetry else max(ya, ey0 + radius + h)
+    yb = min(yb, ey1 - radius - h)
+    za, zb = bz0 + 0.1 * hz, bz1 + 0.1 * hz
+    if ground:
+        za = max(za, ez0 + 2.0 * h + radius)
+    za, zb = max(za, ez0 + radius + h), min(zb, ez1 - radius - h)
+
+    def solved(ys):
+        return ys[ys >= ey0 - 1e-9] if symmetry else ys
+
+    if kind == "plane":
+        return DyeRake("plane", [(x, max(ya, ey0) if symmetry else ya, za), (x, yb, zb)], radius=radius)
+    gap = max(6.0 * radius, size / 7.0)
+    if kind == "rake":
+        k = int(n) if n else max(3, min(17, round((yb - ya) / gap) + 1))
+        ys = solved(np.linspace(ya, yb, k))
+        z = bz0 + 0.40 * hz
+        z = min(max(z, ez0 + radius + 2 * h), ez1 - radius - h)
+        return DyeRake("rake", [(x, y, z) for y in ys], radius=radius)
+    ky = max(3, min(11, round((yb - ya) / gap) + 1))
+    kz = max(2, min(9, round((zb - za) / gap) + 1))
+    if n:
+        ky = max(2, round(math.sqrt(n * (yb - ya) / max(zb - za, 1e-9))))
+        kz = max(2, round(n / ky))
+    ys = solved(np.linspace(ya, yb, ky))
+    zs = np.linspace(za, zb, kz)
+    return DyeRake("grid", [(x, y, z) for z in zs for y in ys], radius=radius)
+
+
+def _ramp(d, edge: float, h: float):
+    """1 inside edge - h/2, 0 beyond edge + h/2, linear between (anti-aliased edge)."""
+    return np.clip((edge + 0.5 * h - d) / h, 0.0, 1.0)
+
+
+def dye_source_weights(rake: DyeRake | None, ops: Ops3D):
+    """(window, weights (numpy)) of a rake's nozzles on the grid cells (reference
+    ``ns3d/dye.py:171-226``): radial ramp of the nozzle radius times an axial ramp of the
+    nozzle depth, the maximum over nozzles; a "plane" band is a product of ramps."""
+    empty = (0, 0, 0, 0, 0, 0), np.zeros((0, 0, 0))
+    if rake is None:
+        return empty
+    pos = np.asarray(rake.positions, dtype=np.float64).reshape(-1, 3)
+    if not len(pos):
+        return empty
+    h = ops.h
+    r = rake.radius if rake.radius > 0 else NOZZLE_RADIUS_CELLS * h
+    depth = NOZZLE_DEPTH_CELLS * h
+    if rake.kind == "plane":
+        lo, hi = pos.min(axis=0) - np.array([depth, 0.0, 0.0]), pos.max(axis=0) + np.array([depth, 0.0, 0.0])
+    else:
+        lo, hi = pos.min(axis=0) - np.array([depth, r, r]), pos.max(axis=0) + np.array([depth, r, r])
+    lo, hi = lo - h, hi + h
+    org, n = (ops.x0, ops.y0, ops.z0), (ops.nx, ops.ny, ops.nz)
+    c0 = [max(math.floor((lo[a] - org[a]) / h), 0) for a in range(3)]
+    c1 = [min(math.ceil((hi[a] - org[a]) / h), n[a]) for a in range(3)]
+    if any(c1[a] <= c0[a] for a in range(3)):
+        return empty
+    xc, yc, zc = (org[a] + (np.arange(c0[a], c1[a]) + 0.5) * h for a in range(3))
+    Z, Y, X = np.meshgrid(zc, yc, xc, indexing="ij")
+    if rake.kind == "plane":
+        (_, ya, za), (_, yb, zb) = pos[0], pos[1]
+        ya, yb = min(ya, yb), max(ya, yb)
+        za, zb = min(za, zb), max(za, zb)
+        w = (_ramp(np.abs(X - 0.5 * (pos[0, 0] + pos[1, 0])), depth, h) * _ramp(np.maximum(ya - Y, Y - yb), 0.0, h)
+             * _ramp(np.maximum(za - Z, Z - zb), 0.0, h))
+    else:
+        w = np.zeros_like(X)
+        for px, py, pz in pos:
+            rr = np.sqrt((Y - py) ** 2 + (Z - pz) ** 2)
+            w = np.maximum(w, _ramp(rr, r, h) * _ramp(np.abs(X - px), depth, h))
+    if not w.any():
+        return empty
+    return (c0[2], c1[2], c0[1], c1[1], c0[0], c1[0]), w
+
+
+def dye_pulse(rake: DyeRake | None) -> list[float]:
+    """[t_on, period]: a continuous rake gets t_on > period (always on)."""
+    if rake is not None and rake.pulsed:
+        return [rake.pulse_on, rake.pulse_on + rake.pulse_off]
+    return [2.0, 1.0]
+
+
+def auto_dye_rake(cfg: SolverConfig3D, ops: Ops3D, kind: str = "rake") -> DyeRake:
+    """The automatic rake of a configuration (reference ``FlowSolver3D.default_dye_rake``):
+    from the bodies' bounding box, or across the middle of the inlet third without bodies."""
+    if cfg.bodies:
+        v = np.vstack([b.vertices_numpy() for b in cfg.bodies])
+        lo, hi = v.min(axis=0), v.max(axis=0)
+        bounds = (lo[0], hi[0], lo[1], hi[1], lo[2], hi[2])
+    else:
+        ex = cfg.extent
+        cx, cy, cz = ex[0] + 0.3 * (ex[1] - ex[0]), 0.5 * (ex[2] + ex[3]), 0.5 * (ex[4] + ex[5])
+        q = 0.15 * min(ex[3] - ex[2], ex[5] - ex[4])
+        bounds = (cx, cx + q, cy - q, cy + q, cz - q, cz + q)
+    return default_dye_rake(bounds, cfg.extent, ops.h, kind=kind, symmetry=ops.symmetry,
+                            ground=cfg.bc_zmin in ("noslip", "moving"))

@@ -24,7 +24,8 @@
  *                          ns3d/poisson.py:107-126)
  *   aero3d_step          - ns3d/core.py:753-762 step3d_full (one SSP-RK3 step with
  *                          projection, penalisation and the six coefficients), called
- *                          as ns3d/solver.py:583-595 _step_fn dispatches the fused path
+ *                          as ns3d/solver.py:583-595 _step_fn dispatches the fused path,
+ *                          then the smoke of ns3d/solver.py:646-680 (ns3d/dye.py:281-312)
  *   aero3d_project       - ns3d/core.py:604-612 project3d
  *   aero3d_poisson_solve - ns3d/poisson.py:128-157 PoissonSolver3D.solve
  */
@@ -192,12 +193,16 @@ typedef struct {
                                [0, nut_trust], the turbulence model elsewhere (learned closure) */
   int32_t reserved;
   double nut_trust;
+  int32_t dye;              /* 1: passive smoke in fields.dye, advected after every step */
+  int32_t dye_k0, dye_k1, dye_j0, dye_j1, dye_i0, dye_i1;   /* nozzle cell window (empty: none) */
+  int32_t reserved2;
 } aero3d_desc;
 
 /* Caller-owned device buffers of the plan's dtype: u (nz, ny, nx+1), v (nz, ny+1, nx),
  * w (nz+1, ny, nx), p (nz, ny, nx), t and dt 0-d. in == out is allowed. */
 typedef struct {
   void *u, *v, *w, *p, *t, *dt;
+  void *dye;                /* (nz, ny, nx) smoke concentration in [0, 1] when desc.dye */
 } aero3d_fields;
 
 /* Window arrays (ns3d/geometry.py:275-303), wz = k1-k0, wy = j1-j0, wx = i1-i0:
@@ -213,6 +218,8 @@ typedef struct {
 typedef struct {
   const void *nu, *eta, *u_inlet, *dt_fixed;
   const void *nut_field;
+  const void *dye_weight;   /* nozzle weights on the nozzle window (wz, wy, wx) */
+  const void *dye_pulse;    /* [t_on, period]: nozzles emit while t mod period < t_on */
 } aero3d_params;
 
 /* Per-step outputs (device; any may be NULL): coeffs (nbody, 6) = [CD, CS, CL, C_roll, Cm, Cn];

@@ -548,3 +548,79 @@ def noisy_start3d(ops, seed, amplitude=0.01):
     u, v, w = apply_boundary_values(u, v, w, ops)
     u, v, w, _ = project3d(u, v, w, ops)
     return u, v, w
+
+
+# ---------------------------------------------------------------------------
+# smoke (ns3d/dye.py:232-305)
+# ---------------------------------------------------------------------------
+
+DYE_FLOOR = 1e-8                                          # ns3d/dye.py:63
+
+
+def pad_dye(c, ops):
+    """Two ghost layers per face (dye.py:232-263): x clean inflow / zero-gradient outlet,
+    y and z mirrored, periodic axes wrapped; edges and corners unused (left at zero)."""
+    nz, ny, nx = c.shape
+    P = c.new_zeros(nz + 4, ny + 4, nx + 4)
+    P[2:-2, 2:-2, 2:-2] = c
+    px, py, pz = ops.periodic_axes
+    m = slice(2, -2)
+    if px:
+        P[m, m, 0:2] = c[..., nx - 2:]
+        P[m, m, nx + 2:] = c[..., :2]
+    else:
+        P[m, m, 0:2] = 0.0
+        P[m, m, nx + 2:] = c[..., nx - 1:].expand(nz, ny, 2)
+    if py:
+        P[m, 0:2, m] = c[:, ny - 2:]
+        P[m, ny + 2:, m] = c[:, :2]
+    else:
+        P[m, 0:2, m] = c[:, :2].flip(1)
+        P[m, ny + 2:, m] = c[:, ny - 2:].flip(1)
+    if pz:
+        P[0:2, m, m] = c[nz - 2:]
+        P[nz + 2:, m, m] = c[:2]
+    else:
+        P[0:2, m, m] = c[:2].flip(0)
+        P[nz + 2:, m, m] = c[nz - 2:].flip(0)
+    return P
+
+
+def _van_leer(p, q):
+    pq = p * q
+    pos = pq > 0
+    return torch.where(pos, 2.0 * pq / torch.where(pos, p + q, torch.ones_like(p)), torch.zeros_like(p))
+
+
+def dye_face_flux(a, b, c, d, vel):
+    """Upwind MUSCL flux through the face between b and c (dye.py:274-278)."""
+    up = b + 0.5 * _van_leer(b - a, c - b)
+    dn = c - 0.5 * _van_leer(c - b, d - c)
+    return vel * torch.where(vel > 0, up, dn)
+
+
+def advect_dye3d(c, u, v, w, dt, ops):
+    """Forward Euler with MUSCL / van Leer fluxes, clipped at 1, floored (dye.py:281-297)."""
+    nz, ny, nx = c.shape
+    P = pad_dye(c, ops)
+    m = slice(2, -2)
+    fx = dye_face_flux(P[m, m, 0:nx + 1], P[m, m, 1:nx + 2], P[m, m, 2:nx + 3], P[m, m, 3:nx + 4], u)
+    fy = dye_face_flux(P[m, 0:ny + 1, m], P[m, 1:ny + 2, m], P[m, 2:ny + 3, m], P[m, 3:ny + 4, m], v)
+    fz = dye_face_flux(P[0:nz + 1, m, m], P[1:nz + 2, m, m], P[2:nz + 3, m, m], P[3:nz + 4, m, m], w)
+    k = dt * (1.0 / ops.h)
+    r = c - k * ((fx[..., 1:] - fx[..., :-1]) + (fy[:, 1:] - fy[:, :-1]) + (fz[1:] - fz[:-1]))
+    r = r.clamp(max=1.0)
+    return torch.where(r < DYE_FLOOR, torch.zeros_like(r), r)
+
+
+def apply_solids_and_sources(c, t, geom, window, weight, pulse):
+    """In place: zero where the solid fraction exceeds 1/2, then c += s (1 - c) in the nozzles
+    with s = weight x [t mod period < t_on] (dye.py:300-312)."""
+    if geom is not None and not geom.empty:
+        k0, k1, j0, j1, i0, i1 = geom.window
+        c[k0:k1, j0:j1, i0:i1].masked_fill_(geom.chi_c > 0.5, 0.0)
+    k0, k1, j0, j1, i0, i1 = window
+    if k1 > k0 and j1 > j0 and i1 > i0:
+        win = c[k0:k1, j0:j1, i0:i1]
+        on = (torch.remainder(t, pulse[1]) < pulse[0]).to(c.dtype)
+        win.add_(weight * on * (1.0 - win))

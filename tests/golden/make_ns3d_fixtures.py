@@ -15,6 +15,9 @@ Each file holds inputs and expected outputs only (numpy arrays, no code):
 * ``ns3d_wale_moving_py_f32.npz``: the car miniature in float32, 20 steps.
 * ``ns3d_masks.npz``: the reference's ``build_geometry3d`` on two overlapping triangulated
   boxes, one spinning, on the 24 x 16 x 16 grid (float64).
+* ``ns3d_dye_<case>.npz``: the reference's smoke on the laminar tunnel case and the periodic-y
+  car case: the automatic rake (positions, nozzle window and weights) and six steps of MUSCL
+  advection with the step-5 velocity, solids and (pulsed) nozzle sources (float64).
 
 Usage: ``python tests/golden/make_ns3d_fixtures.py`` (under a minute on 8 cores).
 """
@@ -126,6 +129,36 @@ def masks():
     return out
 
 
+def dye_case(name, pulse):
+    from aerolume.solvers.ns3d import dye as rdye
+    ops = rcore.make_operators3d(ref_config(name, "float64"))
+    g, rg = ref_geometry(name, "float64")
+    fx = np.load(OUT / f"ns3d_{name}_f64.npz")
+    cfg = cfg3d(name, "float64")
+    pts = np.vstack([b.vertices_numpy() for b in cfg.bodies])
+    lo, hi = pts.min(axis=0), pts.max(axis=0)
+    bounds = (lo[0], hi[0], lo[1], hi[1], lo[2], hi[2])
+    extent = (ops.x0, ops.x0 + ops.nx * ops.h, ops.y0, ops.y0 + ops.ny * ops.h, ops.z0, ops.z0 + ops.nz * ops.h)
+    rake = rdye.default_dye_rake(bounds, extent, ops.h, kind="rake", symmetry=ops.symmetry,
+                                 ground=ops.bc_zmin in ("noslip", "moving"))
+    if pulse:
+        rake.pulse_on, rake.pulse_off = 0.05, 0.03
+    src = rdye.build_dye_sources(rake, ops)
+    u, v, w = (torch.as_tensor(fx[f"{f}_5"]) for f in "uvw")
+    dt = torch.tensor(float(fx["dt"][4]), dtype=torch.float64)
+    t = torch.tensor(float(fx["t"][4]), dtype=torch.float64)
+    c = torch.zeros(ops.nz, ops.ny, ops.nx, dtype=torch.float64)
+    out = {"bounds": np.array(bounds), "positions": rake.positions, "radius": np.float64(rake.radius),
+           "pulse": np.array([rake.pulse_on, rake.pulse_off]), "window": np.asarray(src.window, np.int64),
+           "weight": src.weight.numpy(), "dt": dt.numpy(), "t0": t.numpy()}
+    for n in range(1, 7):
+        c = rdye.advect_dye3d(c, u, v, w, dt, ops)
+        t = t + dt
+        rdye.apply_solids_and_sources(c, t, rg, src)
+        out[f"c_{n}"] = c.numpy().copy()
+    return out
+
+
 def main():
     torch.set_num_threads(8)
     for name in CASES3D:
@@ -133,6 +166,8 @@ def main():
         print("wrote", name, flush=True)
     np.savez_compressed(OUT / "ns3d_wale_moving_py_f32.npz", **run_case("wale_moving_py", "float32", 20, (20,)))
     np.savez_compressed(OUT / "ns3d_masks.npz", **masks())
+    np.savez_compressed(OUT / "ns3d_dye_laminar.npz", **dye_case("laminar", pulse=False))
+    np.savez_compressed(OUT / "ns3d_dye_wale_moving_py.npz", **dye_case("wale_moving_py", pulse=True))
     print("done")
 
 

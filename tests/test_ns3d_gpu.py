@@ -220,3 +220,39 @@ def test_full_size_car_one_step_matches_oracle():
     assert _err(st.p, ref.p) < 5e-4
     assert _err(c, rc) < 1e-3
     assert float(S.divergence3d(st.u, st.v, st.w, ops.h).abs().max()) * ops.h < 1e-5
+
+
+@pytest.mark.parametrize("name", ["laminar", "wale_moving_py"])
+def test_flow_solver_smoke_matches_oracle(name):
+    """FlowSolver3D with smoke (automatic rake; pulsed on the car case), FP64, against the
+    oracle's flow steps followed by its smoke step (MUSCL advection with the end-of-step
+    velocity, solids, sources at the new time), 6 steps."""
+    _need_gpu()
+    import dataclasses
+    from oracle import ns3d as O
+    from aerolume_amd.ns3d import setup as su
+    from aerolume_amd.ns3d.config import DyeRake
+    S = _S()
+    cfg = cases.cfg3d(name, "float64", device="cuda")
+    cops = _cpu_ops(cases.cfg3d(name, "float64"))
+    rake = su.auto_dye_rake(cfg, cops)
+    if name == "wale_moving_py":
+        rake = DyeRake(rake.kind, rake.positions, rake.radius, pulse_on=0.05, pulse_off=0.03)
+    cfg = dataclasses.replace(cfg, dye=True, dye_rake=rake, use_cuda_graphs=False)
+    fs = S.FlowSolver3D(cfg)
+    u, v, w = O.noisy_start3d(cops, 17)
+    fs.set_velocity(u, v, w)
+    cg = su.build_geometry3d(cfg.bodies, cops)
+    win, wt = su.dye_source_weights(rake, cops)
+    pulse = torch.tensor(su.dye_pulse(rake), dtype=torch.float64)
+    z = torch.zeros((), dtype=torch.float64)
+    st = O.State3D(u, v, w, torch.zeros(cops.nz, cops.ny, cops.nx, dtype=torch.float64), z, z)
+    c = torch.zeros(cops.nz, cops.ny, cops.nx, dtype=torch.float64)
+    for n in range(6):
+        st, _ = O.step3d_full(st, cg, cops)
+        c = O.advect_dye3d(c, st.u, st.v, st.w, st.dt, cops)
+        O.apply_solids_and_sources(c, st.t, cg, win, torch.as_tensor(wt), pulse)
+    fs.advance(6)
+    assert _err(fs.state.u, st.u) < 1e-10
+    assert float(c.max()) > 0.5
+    assert _err(fs.field("dye"), c) < 1e-10

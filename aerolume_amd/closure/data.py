@@ -21,10 +21,11 @@ from ..ns3d import solver as S3
 from ..ns3d.config import SolverConfig3D
 
 
-def box_config(n: int, re: float = 1000.0, turbulence: str = "none", dtype: str = "float32", cfl: float = 0.5):
+def box_config(n: int, re: float = 1000.0, turbulence: str = "none", dtype: str = "float32", cfl: float = 0.5,
+               advection: str = "upwind3"):
     """The triply periodic [0, 2 pi)^3 box at n^3 cells, nu = 1 / re (U = u_rms = 1)."""
     return SolverConfig3D(nx=n, ny=n, nz=n, h=2.0 * math.pi / n, x0=0.0, y0=0.0, z0=0.0, re=re, periodic=True,
-                          turbulence=turbulence, advection="upwind3", cfl=cfl, dtype=dtype, device="cuda", dye=False,
+                          turbulence=turbulence, advection=advection, cfl=cfl, dtype=dtype, device="cuda", dye=False,
                           perturb=False)
 
 

@@ -70,30 +70,37 @@ def train(xs, ys, m: int, iters: int, lr: float, seed: int):
     net = CNN3D((m, m, m), seed=seed)
     xn = [x / g_scale for x in xs]
     yn = [y / nu_scale for y in ys]
-    losses = []
+
+    def data_loss():
+        return float(torch.stack([((net.infer(x)[0] - y) ** 2).mean() for x, y in zip(xn, yn)]).mean())
+
+    losses = [data_loss()]
     for it in range(iters):
         j = it % len(xn)
         out = net.forward(xn[j])
         diff = out[0] - yn[j]
-        losses.append(float((diff * diff).mean()) if it % 25 == 0 or it == iters - 1 else None)
         net.backward((2.0 / diff.numel()) * diff[None].contiguous())
         net.adam_step(lr=lr if it < iters // 2 else lr * 0.3)
+    losses.append(data_loss())
     # a priori: correlation of prediction and target over all snapshots
     pred = torch.cat([net.infer(x)[0].reshape(-1) for x in xn])
     targ = torch.cat([y.reshape(-1) for y in yn])
     corr = float(torch.corrcoef(torch.stack([pred, targ]))[0, 1])
-    return net, g_scale, nu_scale, trust, [l for l in losses if l is not None], corr
+    return net, g_scale, nu_scale, trust, losses, corr
 
 
-def run_coarse(m: int, start, t_end: float, re: float, closure=None, turbulence: str = "wale"):
+def run_coarse(m: int, start, t_end: float, re: float, closure=None, turbulence: str = "wale",
+               advection: str = "upwind3"):
     t0_sim, uc, vc, wc = start
-    fs = S3.FlowSolver3D(D.box_config(m, re=re, turbulence=turbulence), closure=closure)
+    fs = S3.FlowSolver3D(D.box_config(m, re=re, turbulence=turbulence, advection=advection), closure=closure)
     fs.set_velocity(uc, vc, wc)
     fs._state.t.fill_(t0_sim)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     while fs.t < t_end - 1e-12:
         fs.advance(10)
+        if not bool(torch.isfinite(fs.state.u[0, 0, :4]).all()):
+            break
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     st = fs.state
@@ -108,7 +115,8 @@ def spectrum_error(e, e_ref, kmax: int) -> float:
 
 
 def experiment(n: int = 256, r: int = 4, re: float = 1000.0, t_start: float = 0.5, t_end: float = 1.5,
-               n_snap: int = 6, iters: int = 300, lr: float = 2e-3, train_seed: int = 505, test_seed: int = 606):
+               n_snap: int = 6, iters: int = 300, lr: float = 2e-3, train_seed: int = 505, test_seed: int = 606,
+               coarse_advection: str = "upwind3"):
     m = n // r
     t0 = time.perf_counter()
     snaps, _, _ = run_fine(n, r, train_seed, t_start, t_end, n_snap, re)
@@ -124,18 +132,23 @@ def experiment(n: int = 256, r: int = 4, re: float = 1000.0, t_start: float = 0.
     _, ue, ve, we, _ = test[-1]
     k, e_ref = D.kinetic_energy_spectrum(ue, ve, we)
     closure = LearnedClosure(net, g_scale, nu_scale, trust)
-    runs = {"learned": run_coarse(m, start, t_end, re, closure=closure),
-            "wale": run_coarse(m, start, t_end, re, turbulence="wale"),
-            "none": run_coarse(m, start, t_end, re, turbulence="none")}
+    adv = coarse_advection
+    runs = {"learned": run_coarse(m, start, t_end, re, closure=closure, advection=adv),
+            "wale": run_coarse(m, start, t_end, re, turbulence="wale", advection=adv),
+            "none": run_coarse(m, start, t_end, re, turbulence="none", advection=adv)}
     kc = m // 2
     report = {"fine_grid": n, "coarse_grid": m, "filter": r, "re": re, "t_start": t_start, "t_end": t_end,
+              "coarse_advection": adv,
               "train_snapshots": n_snap, "train_iters": iters, "train_seconds": round(t_train, 2),
               "data_seconds": round(t_data, 2), "a_priori_correlation": round(corr, 4),
-              "loss_first_last": [round(losses[0], 5), round(losses[-1], 5)],
+              "loss_first_last": [round(losses[0], 5), round(losses[-1], 5)],   # mean over all snapshots
               "g_scale": g_scale, "nu_scale": nu_scale, "trust": trust,
               "fine_wall_s": round(fine_wall, 3), "fine_steps": fine_steps, "k_c": kc,
               "E_filtered_fine": [float(x) for x in e_ref[:kc + 1]]}
     for name, rr in runs.items():
+        if not rr["finite"]:
+            report[name] = {"finite": False, "wall_s": round(rr["wall"], 3), "steps": rr["steps"]}
+            continue
         report[name] = {"wall_s": round(rr["wall"], 3), "steps": rr["steps"], "finite": rr["finite"],
                         "max_rel_spectrum_error_k_le_kc": round(spectrum_error(rr["E"], e_ref, kc), 4),
                         "max_rel_spectrum_error_k_le_kc_half": round(spectrum_error(rr["E"], e_ref, kc // 2), 4),
@@ -151,8 +164,10 @@ def main() -> int:
     ap.add_argument("--iters", type=int, default=300)
     ap.add_argument("--snapshots", type=int, default=6)
     ap.add_argument("--t-end", type=float, default=1.5)
+    ap.add_argument("--coarse-advection", default="upwind3", choices=("upwind3", "central"))
     args = ap.parse_args()
-    rep = experiment(n=args.fine, r=args.ratio, iters=args.iters, n_snap=args.snapshots, t_end=args.t_end)
+    rep = experiment(n=args.fine, r=args.ratio, iters=args.iters, n_snap=args.snapshots, t_end=args.t_end,
+                     coarse_advection=args.coarse_advection)
     print(json.dumps(rep), flush=True)
     return 0
 

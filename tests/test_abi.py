@@ -81,6 +81,7 @@ def test_solver_refuses_cpu_device():
 def test_3d_unsupported_features_fail_loudly():
     from aerolume_amd.ns3d import setup as su
     from aerolume_amd.ns3d.config import SolverConfig3D
-    cfg = SolverConfig3D(nx=8, ny=8, nz=8, h=0.1, x0=0, y0=0, z0=0, re=100.0)    # smoke on by default
-    with pytest.raises(NotImplementedError, match="smoke"):
+    su.check_supported(SolverConfig3D(nx=8, ny=8, nz=8, h=0.1, x0=0, y0=0, z0=0, re=100.0))   # smoke is carried
+    cfg = SolverConfig3D(nx=8, ny=8, nz=8, h=0.1, x0=0, y0=0, z0=0, re=100.0, particle_capacity=16)
+    with pytest.raises(NotImplementedError, match="particle"):
         su.check_supported(cfg)

@@ -172,6 +172,7 @@ def make_operators3d(cfg: SolverConfig3D, device=None) -> Ops3D:
                 closed=cfg.closed)
     if cfg.dt is not None:
         ops.extras["dt_fixed"] = torch.tensor(float(cfg.dt), device=dev, dtype=dtype)
+    ops.extras["re"] = float(cfg.re)
     return ops
 
 
@@ -179,6 +180,7 @@ def set_reynolds(ops: Ops3D, re: float) -> None:
     """Change Re in place (nu, eta and the launch value), reference ``ns3d/core.py:140-144``."""
     ops.nu.fill_(1.0 / re)
     ops.nu_value = 1.0 / re
+    ops.extras["re"] = float(re)
     ops.eta.fill_(ops.eta_fixed if ops.eta_fixed is not None else auto_eta(ops.h, re))
 
 

@@ -1,7 +1,8 @@
 """CPU oracle of the 3D Navier-Stokes step (TEST INFRASTRUCTURE, never the product).
 
 A restatement, in plain torch on the CPU (float32 / float64), of the reference's 3D step
-``step3d_full`` (``solvers/ns3d/core.py:753-883``) and its helpers, without placed physics:
+``step3d_full`` / ``step3d_phys`` (``solvers/ns3d/core.py:753-883``) and its helpers, the
+smoke (``ns3d/dye.py``) and heat transfer (``ns3d/physics.py:494-830``):
 ghost rules per component and axis, upwind3 / central advection, 7-point diffusion,
 Smagorinsky / WALE / Vreman eddy viscosity on the traceless gradient, window penalisation,
 the exact Poisson solve with orthonormal DCT-II / Hartley bases (``solvers/ns3d/poisson.py``),
@@ -348,9 +349,10 @@ def learned_nut(nut, field, trust, ops):
     return out
 
 
-def sgs_divergence3d(U, V, W, geom, ops, nut_field=None, nut_trust=None):
+def sgs_divergence3d(U, V, W, geom, ops, nut_field=None, nut_trust=None, out=None):
     """div(2 nu_t S) at the faces, max nu_t (core.py:491-534); with ``nut_field`` the learned
-    closure's values replace the model's where trusted (before the solid mask)."""
+    closure's values replace the model's where trusted (before the solid mask). ``out`` (a dict)
+    receives ``nut`` (nz+2, ny+2, nx+2), the heat equation's turbulent diffusivity input."""
     nz, ny, nx = ops.nz, ops.ny, ops.nx
     inv = 1.0 / ops.h
     gr = velocity_gradient(U, V, W, ops.h)
@@ -379,6 +381,8 @@ def sgs_divergence3d(U, V, W, geom, ops, nut_field=None, nut_trust=None):
     fu = (t11[..., 1:] - t11[..., :-1] + t12[:, 1:] - t12[:, :-1] + t13[1:] - t13[:-1]) * inv
     fv = (t12[..., 1:] - t12[..., :-1] + t22[:, 1:] - t22[:, :-1] + t23[1:] - t23[:-1]) * inv
     fw = (t13[..., 1:] - t13[..., :-1] + t23[:, 1:] - t23[:, :-1] + t33[1:] - t33[:-1]) * inv
+    if out is not None:
+        out["nut"] = nut
     return fu, fv, fw, nut.amax()
 
 
@@ -398,7 +402,7 @@ def boundary_tendencies(u, lu, lv, lw, ops):
     return lu, lv, lw
 
 
-def momentum_rhs3d(u, v, w, geom, ops, nut_field=None, nut_trust=None):
+def momentum_rhs3d(u, v, w, geom, ops, nut_field=None, nut_trust=None, out=None):
     U, V, W = pad_velocity(u, v, w, ops)
     au, av, aw = advection3(U, V, W, ops)
     nu_h2 = ops.nu * (1.0 / ops.h ** 2)
@@ -407,7 +411,7 @@ def momentum_rhs3d(u, v, w, geom, ops, nut_field=None, nut_trust=None):
     lw = torch.addcmul(aw, laplacian_padded(W), nu_h2)
     nut_max = None
     if ops.turbulence != "none":
-        fu, fv, fw, nut_max = sgs_divergence3d(U, V, W, geom, ops, nut_field, nut_trust)
+        fu, fv, fw, nut_max = sgs_divergence3d(U, V, W, geom, ops, nut_field, nut_trust, out=out)
         lu, lv, lw = lu + fu, lv + fv, lw + fw
     lu, lv, lw = boundary_tendencies(u, lu, lv, lw, ops)
     return lu, lv, lw, nut_max
@@ -624,3 +628,317 @@ def apply_solids_and_sources(c, t, geom, window, weight, pulse):
         win = c[k0:k1, j0:j1, i0:i1]
         on = (torch.remainder(t, pulse[1]) < pulse[0]).to(c.dtype)
         win.add_(weight * on * (1.0 - win))
+
+
+# ---------------------------------------------------------------------------
+# heat transfer (ns3d/physics.py:494-790, core.py:765-883 with a Physics3D holding only heat)
+# ---------------------------------------------------------------------------
+
+THETA_COURANT_MAX = 0.5                                   # physics.py:80
+WALL_SIDES = ("left", "right", "front", "back", "bottom", "top")
+
+
+class Thermal:
+    """The heat equation's operators for the oracle (ThermalOps, physics.py:494-560), from the
+    static data of ``aerolume_amd.ns3d.thermal.build_thermal`` (float64 host tensors) cast to
+    the run's dtype. Heaters and heat-flux skins enter as their summed density ``q`` (one
+    addition per stage where the reference adds each part in turn: equal up to rounding)."""
+
+    def __init__(self, data, ops, re):
+        dt = ops.dtype
+        h = ops.h
+        self.pe = float(re) * data.prandtl
+        self.kappa = 1.0 / self.pe
+        self.ri = float(data.ri)
+        self.prt = float(data.prt)
+        eta = data.eta if data.eta is not None else min(0.05 * h, h * h * self.pe / 60.0)
+        self.eta = torch.tensor(eta, dtype=dt)
+        self.side_kinds = tuple(data.side_kinds)
+        self.side_vals = torch.tensor(data.side_vals, dtype=dt)
+        self.flux_w = {}
+        for i, kind in enumerate(self.side_kinds):
+            if kind == "heat_flux":
+                dim = (2, 2, 1, 1, 0, 0)[i]
+                shape = [1, 1, 1]
+                shape[dim] = 2
+                wts = torch.tensor([3.0, 1.0] if i % 2 == 0 else [1.0, 3.0], dtype=dt).reshape(shape)
+                self.flux_w[i] = wts * h
+        cast = (lambda t: None if t is None else t.to(dt))  # noqa: E731
+        self.window = tuple(data.window)
+        self.nbody = data.nbody
+        self.t_index = torch.as_tensor(data.t_index, dtype=torch.long)
+        self.chi_t, self.a_t, self.w_t = cast(data.chi_t), cast(data.a_t), cast(data.w_t)
+        self.adia = None if data.adia is None else tuple(cast(a) for a in data.adia)
+        self.vol_w = cast(data.vol_w)
+        self.q_window = tuple(data.q_window)
+        self.q = cast(data.q)
+        self.flux_rate = torch.as_tensor(data.flux_rate, dtype=torch.float64)
+        self.wall_idx, self.wall_seg = data.wall_idx, data.wall_seg
+        self.wall_w = cast(data.wall_w)
+        self.values = data.values
+        self.symmetry = bool(ops.symmetry)
+        self.heat_max = torch.tensor(self.kappa * data.q_max, dtype=dt) if data.q_max > 0.0 else None
+        self.heat_acc = torch.zeros(data.nbody, dtype=dt)
+        self.heat_rate = torch.zeros(data.nbody, dtype=dt)
+
+
+def _theta_ghosts(th, c, dim, side):
+    """Two ghost layers of one side (physics.py:589-606)."""
+    n = c.shape[dim]
+    lo = side % 2 == 0
+    kind = th.side_kinds[side]
+    if kind == "periodic":
+        return c.narrow(dim, n - 2, 2) if lo else c.narrow(dim, 0, 2)
+    if kind == "outflow":
+        e = c.narrow(dim, 0 if lo else n - 1, 1)
+        return torch.cat([e, e], dim=dim)
+    mirror = c.narrow(dim, 0, 2).flip(dim) if lo else c.narrow(dim, n - 2, 2).flip(dim)
+    if kind == "adiabatic":
+        return mirror
+    val = th.side_vals[side]
+    if kind == "inflow":
+        return torch.zeros_like(mirror) + val
+    if kind == "temperature":
+        return 2.0 * val - mirror
+    return mirror + val * th.flux_w[side]
+
+
+def pad_theta(th, c):
+    """theta with two ghost layers on the six faces (physics.py:608-620)."""
+    nz, ny, nx = c.shape
+    P = c.new_zeros(nz + 4, ny + 4, nx + 4)
+    m = slice(2, -2)
+    P[m, m, m] = c
+    P[m, m, 0:2] = _theta_ghosts(th, c, 2, 0)
+    P[m, m, nx + 2:] = _theta_ghosts(th, c, 2, 1)
+    P[m, 0:2, m] = _theta_ghosts(th, c, 1, 2)
+    P[m, ny + 2:, m] = _theta_ghosts(th, c, 1, 3)
+    P[0:2, m, m] = _theta_ghosts(th, c, 0, 4)
+    P[nz + 2:, m, m] = _theta_ghosts(th, c, 0, 5)
+    return P
+
+
+def face_kappa(th, nut, ops):
+    """kappa / h on the x-, y-, z-faces, + nu_t / Pr_t averaged from the two cells (molecular
+    on non-periodic boundary faces) (physics.py:623-645)."""
+    kh = th.kappa / ops.h
+    if nut is None:
+        return kh, kh, kh
+    c = 0.5 / (th.prt * ops.h)
+    m = slice(1, -1)
+    kx = kh + c * (nut[m, m, :-1] + nut[m, m, 1:])
+    ky = kh + c * (nut[m, :-1, m] + nut[m, 1:, m])
+    kz = kh + c * (nut[:-1, m, m] + nut[1:, m, m])
+    px, py, pz = ops.periodic_axes
+    if not px:
+        kx[..., 0] = kh
+        kx[..., -1] = kh
+    if not py:
+        ky[:, 0] = kh
+        ky[:, -1] = kh
+    if not pz:
+        kz[0] = kh
+        kz[-1] = kh
+    return kx, ky, kz
+
+
+def _win_slices(win, g):
+    k0, k1, j0, j1, i0, i1 = win
+    return (slice(k0, k1 + (1 if g == "w" else 0)), slice(j0, j1 + (1 if g == "v" else 0)),
+            slice(i0, i1 + (1 if g == "u" else 0)))
+
+
+def theta_fluxes(th, c, u, v, w, ops, nut=None):
+    """MUSCL advective minus diffusive theta flux on every face; adiabatic bodies remove the
+    conduction through their faces (physics.py:647-676)."""
+    nz, ny, nx = c.shape
+    P = pad_theta(th, c)
+    m = slice(2, -2)
+    kx, ky, kz = face_kappa(th, nut, ops)
+    sx = (P[m, m, 0:nx + 1], P[m, m, 1:nx + 2], P[m, m, 2:nx + 3], P[m, m, 3:nx + 4])
+    sy = (P[m, 0:ny + 1, m], P[m, 1:ny + 2, m], P[m, 2:ny + 3, m], P[m, 3:ny + 4, m])
+    sz = (P[0:nz + 1, m, m], P[1:nz + 2, m, m], P[2:nz + 3, m, m], P[3:nz + 4, m, m])
+    out = [dye_face_flux(*st, vel) - kf * (st[2] - st[1]) for st, vel, kf in ((sx, u, kx), (sy, v, ky), (sz, w, kz))]
+    if th.adia is not None:
+        k0, k1, j0, j1, i0, i1 = th.window
+        K, J, I = slice(k0 + 2, k1 + 2), slice(j0 + 2, j1 + 2), slice(i0 + 2, i1 + 2)
+        dx = P[K, J, i0 + 2:i1 + 3] - P[K, J, i0 + 1:i1 + 2]
+        dy = P[K, j0 + 2:j1 + 3, I] - P[K, j0 + 1:j1 + 2, I]
+        dz = P[k0 + 2:k1 + 3, J, I] - P[k0 + 1:k1 + 2, J, I]
+        for f, kf, d, chi, g in zip(out, (kx, ky, kz), (dx, dy, dz), th.adia, ("u", "v", "w")):
+            sl = _win_slices(th.window, g)
+            kw = kf if isinstance(kf, float) else kf[sl]
+            f[sl] += kw * chi * d
+    return out
+
+
+def theta_stage(th, c_prev, c0, up, vp, wp, a, bdt, wgt, ops, nut=None):
+    """One RK stage of theta: lerp(c_prev, c0, a) - b dt div F, sources, implicit
+    penalisation of held-temperature bodies with their heat increments (physics.py:678-708)."""
+    fx, fy, fz = theta_fluxes(th, c_prev, up, vp, wp, ops, nut)
+    inv_h = 1.0 / ops.h
+    D = ((fx[..., 1:] - fx[..., :-1]) + (fy[:, 1:] - fy[:, :-1]) + (fz[1:] - fz[:-1])) * inv_h
+    base = c_prev if a == 0.0 else torch.lerp(c_prev, c0, a)
+    ch = base - bdt * D
+    if th.q is not None:
+        bk = bdt * th.kappa
+        ch[_win_slices(th.q_window, "c")] += bk * th.q
+    if th.chi_t is not None:
+        sl = _win_slices(th.window, "c")
+        win = ch[sl]
+        k = bdt / th.eta
+        new = (win + k * th.a_t) / (1.0 + k * th.chi_t)
+        inc = (new - win) * wgt
+        ch[sl] = new
+        th.heat_acc.index_add_(0, th.t_index, torch.einsum("bzyx,zyx->b", th.w_t, inc))
+    return ch
+
+
+def add_buoyancy(th, wh, theta, bdt, wgt, geom, ops):
+    """b dt Ri theta (theta averaged to the w-faces) added after the penalisation everywhere;
+    returns wh and the solid's share for the force accumulator (physics.py:744-791)."""
+    if th.ri == 0.0:
+        return wh, None
+    tf = 0.5 * (theta[1:] + theta[:-1])
+    wh = torch.cat([wh[:1], wh[1:-1] + tf * (bdt * th.ri), wh[-1:]], dim=0)
+    if ops.periodic_axes[2]:
+        edge = (theta[:1] + theta[-1:]) * (0.5 * th.ri) * bdt
+        wh = torch.cat([wh[:1] + edge, wh[1:-1], wh[-1:] + edge], dim=0)
+    if geom.empty:
+        return wh, None
+    k0, k1, j0, j1, i0, i1 = geom.window
+    lo, hi = max(k0 - 1, 0), min(k1, ops.nz - 1)
+    tc = theta[lo:hi + 1, j0:j1, i0:i1]
+    pz = ops.periodic_axes[2]
+    if k0 == 0:
+        tc = torch.cat([theta[-1:, j0:j1, i0:i1] if pz else tc[:1], tc], dim=0)
+    if k1 == ops.nz:
+        tc = torch.cat([tc, theta[:1, j0:j1, i0:i1] if pz else tc[-1:]], dim=0)
+    tw = 0.5 * (tc[1:] + tc[:-1])
+    share = (tw * th.vol_w) * (bdt * (th.ri * wgt))
+    return wh, share
+
+
+def thermal_dt(th, u0, v0, w0, ops, nut_max, theta):
+    """The step's dt with heat transfer (Physics3D.dt_rule without fans / jets, physics.py:1230-1262):
+    the CFL / viscous rule, the combined advection-diffusion bound of theta, the free-fall
+    limit and the heaters' rate bounds."""
+    dt = cfl_dt3d(u0, v0, w0, ops, nut_max)
+    umax = (u0.abs().amax() + v0.abs().amax() + w0.abs().amax()).clamp_min(1e-6)
+    kap = th.kappa if nut_max is None else th.kappa + nut_max / th.prt
+    diff_rate = kap / (VISCOUS_DT * ops.h * ops.h)
+    dt = torch.minimum(dt, 1.0 / (umax * (1.0 / (THETA_COURANT_MAX * ops.h)) + diff_rate))
+    if th.ri != 0.0:
+        acc = (abs(th.ri) * theta.abs().amax()).clamp_min(1e-12)
+        dt = torch.minimum(dt, ops.cfl * torch.sqrt(ops.h / acc))
+    if th.heat_max is not None:
+        s_max = th.heat_max.clamp_min(1e-12)
+        dt = torch.minimum(dt, 1.0 / s_max)
+        if th.ri != 0.0:
+            dt = torch.minimum(dt, (6.0 * ops.cfl * ops.h / (abs(th.ri) * s_max)) ** (1.0 / 3.0))
+    return dt
+
+
+def end_step(th, dt, ops):
+    """Heat-transfer rate per body in conduction units (physics.py:713-721)."""
+    scale = (2.0 if th.symmetry else 1.0) * ops.h ** 3 * th.pe
+    q = th.heat_acc * (scale / dt)
+    q = q + th.flux_rate.to(q.dtype) * (2.0 if th.symmetry else 1.0)
+    th.heat_rate.copy_(q)
+
+
+def wall_theta(th, theta):
+    """Wall temperature per body: the held value, or the skin mean of a heat-flux body
+    (physics.py:420-441)."""
+    out = torch.zeros(th.nbody, dtype=theta.dtype)
+    if len(th.t_index):
+        out[th.t_index] = torch.as_tensor(th.values[th.t_index.numpy()], dtype=theta.dtype)
+    if th.wall_idx is not None:
+        out.index_add_(0, th.wall_seg, theta.reshape(-1)[th.wall_idx] * th.wall_w)
+    return out
+
+
+def wall_heat_rates(th, theta, u, ops):
+    """Heat into the fluid through each side now (conduction units; physics.py:793-827)."""
+    P = pad_theta(th, theta.double())
+    h, kap = ops.h, th.kappa
+    m = slice(2, -2)
+    out = {}
+    for i, side in enumerate(WALL_SIDES):
+        dim = (2, 2, 1, 1, 0, 0)[i]
+        lo = i % 2 == 0
+        n = theta.shape[dim]
+        inner = P.narrow(dim, 2 if lo else n + 1, 1)
+        ghost = P.narrow(dim, 1 if lo else n + 2, 1)
+        sl = [m, m, m]
+        sl[dim] = slice(None)
+        inner, ghost = inner[tuple(sl)], ghost[tuple(sl)]
+        kind = th.side_kinds[i]
+        if kind == "periodic":
+            out[side] = 0.0
+            continue
+        diff = kap * (ghost - inner) / h
+        q = float(diff.sum()) * h * h
+        if kind in ("inflow", "outflow"):
+            un = (u[..., 0] if lo else u[..., -1]).double()
+            face_val = ghost[..., 0] if kind == "inflow" else inner[..., 0]
+            adv = (un * face_val).sum() * h * h
+            q = (q + float(adv)) if lo else (-float(adv))
+        out[side] = q * th.pe * (2.0 if (th.symmetry and side != "front") else 1.0)
+    return out
+
+
+def step3d_phys(state: State3D, geom, ops, th, theta, dt=None):
+    """One SSP-RK3 step with heat transfer (core.py:765-883 with ``phys`` holding a ThermalOps):
+    returns (State3D, coefficients, new theta); ``th.heat_rate`` holds the bodies' heat rates."""
+    u0, v0, w0 = state.u, state.v, state.w
+    up, vp, wp = u0, v0, w0
+    acc = None
+    psi_w = None
+    px = ops.periodic_axes[0]
+    any_periodic = any(ops.periodic_axes)
+    t0 = theta
+    tp = theta
+    want_nut = ops.turbulence != "none"
+    for s, ((a, b), wgt) in enumerate(zip(SSP_RK3, FORCE_WEIGHTS)):
+        extra = {} if want_nut else None
+        lu, lv, lw, nut_max = momentum_rhs3d(up, vp, wp, geom, ops, out=extra)
+        if s == 0:
+            if dt is None:
+                dt = (ops.extras["dt_fixed"] if ops.dt_fixed is not None
+                      else thermal_dt(th, u0, v0, w0, ops, nut_max, t0))
+            if not torch.is_tensor(dt):
+                dt = torch.tensor(float(dt), dtype=ops.dtype)
+            th.heat_acc.zero_()
+        bdt = dt * b
+        if a == 0.0:
+            uh, vh, wh = torch.addcmul(up, lu, bdt), torch.addcmul(vp, lv, bdt), torch.addcmul(wp, lw, bdt)
+        else:
+            uh = torch.addcmul(torch.lerp(up, u0, a), lu, bdt)
+            vh = torch.addcmul(torch.lerp(vp, v0, a), lv, bdt)
+            wh = torch.addcmul(torch.lerp(wp, w0, a), lw, bdt)
+        if not px:
+            uh = inlet_outlet(uh, ops)
+        if not geom.empty:
+            uh, vh, wh, du, dv, dw = penalise3d(uh, vh, wh, geom, ops, bdt)
+            acc = (du * wgt, dv * wgt, dw * wgt) if acc is None else (
+                acc[0] + du * wgt, acc[1] + dv * wgt, acc[2] + dw * wgt)
+            if any_periodic:
+                uh, vh, wh = sync_periodic(uh, vh, wh, ops)
+        wh, share = add_buoyancy(th, wh, tp, bdt, wgt, geom, ops)
+        if share is not None:
+            acc = (acc[0], acc[1], acc[2] + share)
+        tp_new = theta_stage(th, tp, t0, up, vp, wp, a, bdt, wgt, ops, nut=extra.get("nut") if extra else None)
+        div = divergence3d(uh, vh, wh, ops.h)
+        psi = poisson(ops).solve(div)
+        gx, gy, gz = pressure_gradient(psi, ops)
+        up, vp, wp = uh - gx, vh - gy, wh - gz
+        tp = tp_new
+        psi_w = psi * wgt if psi_w is None else torch.add(psi_w, psi, alpha=wgt)
+    p = psi_w * (1.0 / dt)
+    if geom.empty:
+        coeffs = torch.zeros(geom.nbody, 6, dtype=ops.dtype)
+    else:
+        coeffs = force_coefficients(acc, geom, ops, dt)
+    end_step(th, dt, ops)
+    return State3D(up, vp, wp, p, state.t + dt, dt * torch.ones_like(state.t)), coeffs, tp

@@ -129,3 +129,49 @@ def box_mesh(center, size):
     for a, b, c, d in quads:
         f += [(a, b, c), (a, c, d)]
     return v, np.array(f, dtype=np.int64)
+
+
+# 3D heat-transfer cases (float64, CPU-sized): grid / physics kwargs, bodies as (shape, size,
+# centre, thermal mode, value) - triangulated spheres (icosphere level 2) and boxes (2 x 2 quads
+# per face) whose vertices the fixtures store - thermal kwargs, heaters, steps
+THERMAL3D = {
+    "box": dict(
+        grid=dict(nx=12, ny=12, nz=12, h=1.0 / 12, x0=0.0, y0=0.0, z0=0.0, re=80.0, domain="closed"),
+        bodies=[("sphere", 0.3, (0.35, 0.5, 0.35), "temperature", 1.0),
+                ("sphere", 0.28, (0.68, 0.5, 0.62), "heat_flux", 2.0),
+                ("box", (0.2, 0.2, 0.2), (0.5, 0.22, 0.75), "adiabatic", 0.0)],
+        thermal=dict(prandtl=0.71, richardson=1.0, walls={"bottom": ("heat_flux", 0.7), "left": ("temperature", 0.5)}),
+        heaters=[dict(kind="sphere", center=(0.4, 0.62, 0.7), size=(0.12,), power=3.0),
+                 dict(kind="box", center=(0.7, 0.3, 0.3), size=(0.2, 0.1, 0.3), power=-1.0)],
+        steps=8),
+    "tunnel_les": dict(
+        grid=dict(nx=24, ny=16, nz=16, h=1.0 / 8, x0=-1.0, y0=-1.0, z0=0.0, re=2000.0, bc_y="symmetry_ymin",
+                  turbulence="smagorinsky"),
+        bodies=[("sphere", 0.6, (0.0, -1.0, 0.9), "temperature", 1.0)],
+        thermal=dict(prandtl=0.71, richardson=0.5, theta_inlet=0.2,
+                     walls={"top": ("temperature", 0.0), "bottom": ("heat_flux", 0.3), "front": ("temperature", 3.0)}),
+        heaters=[],
+        steps=5),
+    "periodic_wale": dict(
+        grid=dict(nx=16, ny=12, nz=12, h=1.0 / 8, x0=-1.0, y0=-0.75, z0=-0.75, re=500.0, periodic=True,
+                  turbulence="wale"),
+        bodies=[("box", (0.5, 0.4, 0.3), (0.0, 0.0, 0.0), "temperature", -0.5)],
+        thermal=dict(prandtl=0.9, richardson=2.0, prandtl_turbulent=0.7),
+        heaters=[dict(kind="box", center=(0.6, 0.1, -0.2), size=(0.3, 0.3, 0.3), power=1.5,
+                      axes=((0.8, -0.6, 0.0), (0.6, 0.8, 0.0), (0.0, 0.0, 1.0)))],
+        steps=5),
+}
+THERMAL_SEED = 707
+
+
+def thermal_cfg(name, fixture, dtype="float64", device="cpu", **over):
+    """This package's config of a heat-transfer case, bodies from the meshes a fixture stores."""
+    from aerolume_amd.ns3d.config import HeatSource3D, ImmersedMesh, SolverConfig3D, ThermalConfig3D
+    spec = THERMAL3D[name]
+    bodies = [ImmersedMesh(fixture[f"mesh{i}_v"], fixture[f"mesh{i}_f"], name=f"b{i}", thermal=m, thermal_value=val)
+              for i, (_, _, _, m, val) in enumerate(spec["bodies"])]
+    kw = dict(spec["grid"], bodies=bodies, device=device, dtype=dtype, dye=False, perturb=False, ref_area=0.3,
+              length_ref=0.8, span_ref=0.6, moment_ref=(0.2, 0.0, 0.5), thermal=ThermalConfig3D(**spec["thermal"]),
+              heat_sources=[HeatSource3D(**h) for h in spec["heaters"]])
+    kw.update(over)
+    return SolverConfig3D(**kw)

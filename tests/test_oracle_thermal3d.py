@@ -1,0 +1,302 @@
+"""3D heat transfer on the CPU: this package's thermal builder and the oracle's heat-equation
+step against the reference's own outputs (``tests/golden/ns3d_thermal_*.npz``), and the
+reference's physical checks (tests/test_ns3d_physics.py:172-437) restated on the oracle."""
+from __future__ import annotations
+
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+import oracle.ns3d as O
+from aerolume_amd.ns3d import setup as su
+from aerolume_amd.ns3d import thermal as T
+from aerolume_amd.ns3d.config import AnalyticBody, HeatSource3D, SolverConfig3D, ThermalConfig3D
+from tests import cases
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+NAMES = list(cases.THERMAL3D)
+
+
+def _fixture(name):
+    path = GOLDEN / f"ns3d_thermal_{name}.npz"
+    if not path.exists():
+        pytest.skip(f"missing fixture {path.name}")
+    return dict(np.load(path))
+
+
+def _setup(name, fx, dtype="float64"):
+    cfg = cases.thermal_cfg(name, fx, dtype)
+    ops = su.make_operators3d(cfg, device="cpu")
+    g = su.build_geometry3d(cfg.bodies, ops)
+    data = T.build_thermal(cfg, ops, cfg.bodies, g.window)
+    return cfg, ops, g, data
+
+
+def _t(a, dtype=torch.float64):
+    return torch.as_tensor(np.asarray(a), dtype=dtype)
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_thermal_masks_match_reference(name):
+    fx = _fixture(name)
+    cfg, ops, g, d = _setup(name, fx)
+    assert tuple(g.window) == tuple(int(x) for x in fx["window"])
+    assert list(d.side_kinds) == [str(k) for k in fx["side_kinds"]]
+    assert np.allclose(d.side_vals, fx["side_vals"], rtol=0, atol=0)
+    tol = dict(rtol=0, atol=1e-13)
+    for k in ("chi_t", "a_t", "w_t", "vol_w", "skin"):
+        mine = getattr(d, k)
+        if "t_" + k not in fx:
+            assert mine is None, k
+            continue
+        assert np.allclose(mine.numpy(), fx["t_" + k], **tol), k
+    if "t_adia_u" in fx:
+        for c, a in zip("uvw", d.adia):
+            assert np.allclose(a.numpy(), fx["t_adia_" + c], **tol)
+    else:
+        assert d.adia is None
+    if "t_wall_idx" in fx:
+        assert np.array_equal(d.wall_idx.numpy(), fx["t_wall_idx"])
+        assert np.array_equal(d.wall_seg.numpy(), fx["t_wall_seg"])
+        assert np.allclose(d.wall_w.numpy(), fx["t_wall_w"], rtol=1e-12, atol=0)
+    assert np.allclose(d.flux_rate, fx["flux_rate"], rtol=1e-12, atol=1e-15)
+    assert np.allclose(d.area, fx["area"], rtol=1e-14)
+    assert d.source_total == pytest.approx(float(fx["source_total"]), rel=1e-12, abs=1e-15)
+    assert d.q_max == pytest.approx(float(fx["q_max"]), rel=1e-12, abs=0)
+    # the summed source density is the reference's heaters plus the skins
+    if d.q is not None:
+        k0, k1, j0, j1, i0, i1 = d.q_window
+        ref = torch.zeros(ops.nz, ops.ny, ops.nx, dtype=torch.float64)
+        n = 0
+        while f"src{n}_q" in fx:
+            a = fx[f"src{n}_window"]
+            ref[a[0]:a[1], a[2]:a[3], a[4]:a[5]] += _t(fx[f"src{n}_q"])
+            n += 1
+        if "t_skin" in fx:
+            a = fx["window"]
+            ref[a[0]:a[1], a[2]:a[3], a[4]:a[5]] += _t(fx["t_skin"])
+        mine = torch.zeros_like(ref)
+        mine[k0:k1, j0:j1, i0:i1] = d.q
+        assert torch.allclose(mine, ref, rtol=1e-14, atol=1e-12)
+    th = O.Thermal(d, ops, cfg.re)
+    assert float(th.eta) == pytest.approx(float(fx["eta_t"]), rel=1e-15)
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_oracle_thermal_steps_match_reference(name):
+    fx = _fixture(name)
+    cfg, ops, g, d = _setup(name, fx)
+    th = O.Thermal(d, ops, cfg.re)
+    z = torch.zeros((), dtype=torch.float64)
+    st = O.State3D(_t(fx["u0"]), _t(fx["v0"]), _t(fx["w0"]), torch.zeros(ops.nz, ops.ny, ops.nx, dtype=torch.float64),
+                   z.clone(), z.clone())
+    theta = _t(fx["theta0"])
+    steps = int(fx["steps"])
+    for n in range(1, steps + 1):
+        st, c, theta = O.step3d_phys(st, g, ops, th, theta)
+        assert float(st.dt) == pytest.approx(float(fx["dt"][n - 1]), rel=1e-12)
+        assert np.allclose(c.numpy(), fx["coeffs"][n - 1], rtol=1e-9, atol=1e-11)
+        assert np.allclose(th.heat_rate.numpy(), fx["heat_rate"][n - 1], rtol=1e-9, atol=1e-11)
+        if n in (1, steps):
+            for f in ("u", "v", "w", "p"):
+                ref = fx[f"{f}_{n}"]
+                err = np.abs(getattr(st, f).numpy() - ref).max() / max(np.abs(ref).max(), 1e-30)
+                assert err < 1e-11, (f, n, err)
+            err = np.abs(theta.numpy() - fx[f"theta_{n}"]).max()
+            assert err < 1e-12, ("theta", n, err)
+    walls = O.wall_heat_rates(th, theta, st.u, ops)
+    assert np.allclose([walls[s] for s in O.WALL_SIDES], fx["wall_heat"], rtol=1e-9, atol=1e-11)
+    assert np.allclose(O.wall_theta(th, theta).numpy(), fx["wall_theta"], rtol=1e-12, atol=1e-14)
+
+
+def test_planted_buoyancy_mistake_is_caught():
+    """A buoyancy of the wrong sign (a plausible mistake: gravity along +z) fails the
+    reference comparison of the tunnel case."""
+    fx = _fixture("tunnel_les")
+    cfg, ops, g, d = _setup("tunnel_les", fx)
+    th = O.Thermal(d, ops, cfg.re)
+    th.ri = -th.ri                                          # planted: buoyancy with the wrong sign
+    z = torch.zeros((), dtype=torch.float64)
+    st = O.State3D(_t(fx["u0"]), _t(fx["v0"]), _t(fx["w0"]), torch.zeros(ops.nz, ops.ny, ops.nx, dtype=torch.float64),
+                   z.clone(), z.clone())
+    st, _, _ = O.step3d_phys(st, g, ops, th, _t(fx["theta0"]))
+    assert np.abs(st.w.numpy() - fx["w_1"]).max() > 1e-6
+
+
+# ---------------------------------------------------------------------------
+# physical checks (reference tests/test_ns3d_physics.py), on the oracle in float64
+# ---------------------------------------------------------------------------
+
+def _box_cfg(n=12, **kw):
+    base = dict(nx=n, ny=n, nz=n, h=1.0 / n, x0=0.0, y0=0.0, z0=0.0, re=20.0, domain="closed", device="cpu",
+                dtype="float64", dye=False, perturb=False)
+    base.update(kw)
+    return SolverConfig3D(**base)
+
+
+def _run(cfg, theta0, steps, u=None):
+    ops = su.make_operators3d(cfg, device="cpu")
+    g = su.build_geometry3d(cfg.bodies, ops)
+    d = T.build_thermal(cfg, ops, cfg.bodies, g.window)
+    th = O.Thermal(d, ops, cfg.re)
+    z = torch.zeros((), dtype=torch.float64)
+    nz, ny, nx = ops.nz, ops.ny, ops.nx
+    if u is None:
+        u = (torch.zeros(nz, ny, nx + 1, dtype=torch.float64), torch.zeros(nz, ny + 1, nx, dtype=torch.float64),
+             torch.zeros(nz + 1, ny, nx, dtype=torch.float64))
+    st = O.State3D(*u, torch.zeros(nz, ny, nx, dtype=torch.float64), z.clone(), z.clone())
+    theta = torch.as_tensor(np.broadcast_to(theta0, (nz, ny, nx)).copy(), dtype=torch.float64)
+    for _ in range(steps):
+        st, c, theta = O.step3d_phys(st, g, ops, th, theta)
+    return ops, th, st, theta
+
+
+def test_conduction_between_walls_is_exact():
+    """The linear profile is an exact steady state and Nu = 1 on the walls (reference
+    test_conduction_between_walls_is_exact)."""
+    n = 12
+    x = (np.arange(n) + 0.5) / n
+    cfg = _box_cfg(n, ny=4, nz=4, re=10.0, thermal=ThermalConfig3D(
+        prandtl=1.0, walls={"left": ("temperature", 0.5), "right": ("temperature", -0.5)}))
+    ops, th, st, theta = _run(cfg, 0.5 - x, 20)
+    assert float((theta - torch.as_tensor(0.5 - x)).abs().max()) < 1e-13
+    w = O.wall_heat_rates(th, theta, st.u, ops)
+    area = (4 / n) ** 2
+    assert w["left"] == pytest.approx(area, rel=1e-12)
+    assert w["right"] == pytest.approx(-area, rel=1e-12)
+    assert abs(w["top"]) < 1e-15 and abs(w["front"]) < 1e-15
+    assert float(st.u.abs().max()) < 1e-14
+
+
+def test_heat_source_raises_the_stored_heat_at_its_power():
+    cfg = _box_cfg(thermal=ThermalConfig3D(prandtl=0.71),
+                   heat_sources=[HeatSource3D(kind="sphere", center=(0.4, 0.5, 0.55), size=(0.2,), power=3.0),
+                                 HeatSource3D(kind="box", center=(0.7, 0.3, 0.3), size=(0.2, 0.1, 0.3), power=-1.0)])
+    ops, th, st, theta = _run(cfg, 0.0, 20)
+    d = T.build_thermal(cfg, ops, cfg.bodies, (0, 0, 0, 0, 0, 0))
+    assert d.source_total == pytest.approx(2.0, rel=1e-12)
+    stored = float(theta.sum()) * ops.h ** 3 * th.pe
+    assert stored / float(st.t) == pytest.approx(2.0, rel=1e-10)
+
+
+def _sphere_body(d, centre, **kw):
+    r = 0.5 * d
+    c = torch.tensor(centre, dtype=torch.float64)
+
+    def sdf(x, y, z):
+        return torch.sqrt((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2) - r
+    lo = tuple(v - r for v in centre)
+    hi = tuple(v + r for v in centre)
+    return AnalyticBody(sdf=sdf, bounds=(lo, hi), area=math.pi * d * d, **kw)
+
+
+def test_heat_flux_wall_and_body_release_exactly_their_flux():
+    g_wall, g_body = 0.7, 2.0
+    body = _sphere_body(0.4, (0.5, 0.5, 0.5), name="heater", thermal="heat_flux", thermal_value=g_body)
+    cfg = _box_cfg(bodies=[body], thermal=ThermalConfig3D(prandtl=0.71, walls={"bottom": ("heat_flux", g_wall)}))
+    ops, th, st, theta = _run(cfg, 0.0, 20)
+    stored = float(theta.sum()) * ops.h ** 3 * th.pe
+    area = math.pi * 0.4 ** 2
+    assert stored / float(st.t) == pytest.approx(g_wall * 1.0 + g_body * area, rel=1e-10)
+    assert float(th.heat_rate[0]) == pytest.approx(g_body * area, rel=1e-12)
+    assert O.wall_heat_rates(th, theta, st.u, ops)["bottom"] == pytest.approx(g_wall, rel=1e-9)
+    theta.fill_(0.37)
+    assert float(O.wall_theta(th, theta)[0]) == pytest.approx(0.37, rel=1e-12)
+
+
+def test_temperature_body_heat_rate_is_the_stored_heat_rate():
+    body = _sphere_body(0.4, (0.5, 0.5, 0.5), name="hot", thermal="temperature", thermal_value=1.0)
+    cfg = _box_cfg(bodies=[body], thermal=ThermalConfig3D(prandtl=0.71))
+    ops = su.make_operators3d(cfg, device="cpu")
+    g = su.build_geometry3d(cfg.bodies, ops)
+    th = O.Thermal(T.build_thermal(cfg, ops, cfg.bodies, g.window), ops, cfg.re)
+    z = torch.zeros((), dtype=torch.float64)
+    n = ops.nx
+    zero = (torch.zeros(n, n, n + 1, dtype=torch.float64), torch.zeros(n, n + 1, n, dtype=torch.float64),
+            torch.zeros(n + 1, n, n, dtype=torch.float64))
+    st = O.State3D(*zero, torch.zeros(n, n, n, dtype=torch.float64), z.clone(), z.clone())
+    theta = torch.zeros(n, n, n, dtype=torch.float64)
+    for _ in range(10):
+        before = float(theta.sum())
+        st, _, theta = O.step3d_phys(st, g, ops, th, theta)
+        rate = (float(theta.sum()) - before) * ops.h ** 3 * th.pe / float(st.dt)
+        assert float(th.heat_rate[0]) == pytest.approx(rate, rel=1e-9)
+    assert float(th.heat_rate[0]) > 0
+
+
+def test_stratified_fluid_stays_at_rest():
+    """Ri theta e_z of a linear profile is a discrete gradient: the projection removes it."""
+    n = 10
+    cfg = _box_cfg(n, bc_zmin="noslip", bc_zmax="noslip", thermal=ThermalConfig3D(
+        prandtl=0.71, richardson=5.0, walls={"bottom": ("temperature", -0.5), "top": ("temperature", 0.5)}))
+    z = (np.arange(n) + 0.5) / n - 0.5
+    ops, th, st, theta = _run(cfg, z[:, None, None], 30)
+    assert float(st.w.abs().max()) < 1e-12 and float(st.u.abs().max()) < 1e-12
+    assert float((theta[:, 0, 0] - torch.as_tensor(z)).abs().max()) < 1e-12
+
+
+def test_theta_step_obeys_the_combined_bound_and_stays_bounded():
+    h, pr = 1 / 8, 0.71
+    cfg = SolverConfig3D(nx=32, ny=8, nz=8, h=h, x0=0, y0=0, z0=0, re=(3.0 / h) / pr, bc_y="periodic", device="cpu",
+                         dtype="float64", perturb=False, dye=False, thermal=ThermalConfig3D(prandtl=pr))
+    ops = su.make_operators3d(cfg, device="cpu")
+    u = torch.ones(8, 8, 33, dtype=torch.float64)
+    v = torch.zeros(8, 9, 32, dtype=torch.float64)
+    w = torch.zeros(9, 8, 32, dtype=torch.float64)
+    k, j, i = np.meshgrid(np.arange(8), np.arange(8), np.arange(32), indexing="ij")
+    ops, th, st, theta = _run(cfg, 0.5 + 0.5 * (-1.0) ** (i + j + k), 60, u=(u, v, w))
+    assert float(theta.min()) >= -1e-12 and float(theta.max()) <= 1.0 + 1e-12
+    c, d6 = float(st.dt) * 1.0 / h, 6.0 * th.kappa * float(st.dt) / h ** 2
+    assert c / 0.5 + d6 == pytest.approx(1.0, rel=1e-9)
+
+
+def test_first_step_from_rest_respects_the_free_fall_scale():
+    hot = _sphere_body(0.3, (0.5, 0.5, 0.4), name="hot", thermal="temperature", thermal_value=3.0)
+    cfg = _box_cfg(16, re=1e5, bodies=[hot], thermal=ThermalConfig3D(prandtl=0.71, richardson=2.0))
+    ops, th, st, theta = _run(cfg, 3.0, 1)
+    assert float(st.dt) <= 0.5 * math.sqrt((1 / 16) / (2.0 * 3.0)) * (1 + 1e-12)
+
+
+def test_heater_rate_bounds_the_first_step():
+    cfg = _box_cfg(16, re=1e5, thermal=ThermalConfig3D(prandtl=0.71, richardson=1.0),
+                   heat_sources=[HeatSource3D(kind="box", center=(0.5, 0.5, 0.5), size=(0.2, 0.2, 0.2), power=300.0)])
+    ops, th, st, theta = _run(cfg, 0.0, 1)
+    s_max = float(th.heat_max)
+    assert float(st.dt) <= min(1.0 / s_max, (6 * 0.5 * (1 / 16) / s_max) ** (1 / 3)) * (1 + 1e-12)
+
+
+def test_turbulent_prandtl_adds_diffusivity():
+    cfg = _box_cfg(12, thermal=ThermalConfig3D(prandtl=0.71), turbulence="smagorinsky")
+    ops = su.make_operators3d(cfg, device="cpu")
+    th = O.Thermal(T.build_thermal(cfg, ops, [], (0, 0, 0, 0, 0, 0)), ops, cfg.re)
+    nut = torch.full((14, 14, 14), 0.01, dtype=torch.float64)
+    kx, _, _ = O.face_kappa(th, nut, ops)
+    assert float(kx[..., 5].mean()) == pytest.approx(th.kappa / ops.h + 0.01 / (th.prt * ops.h), rel=1e-12)
+    assert float(kx[..., 0].mean()) == pytest.approx(th.kappa / ops.h, rel=1e-12)
+
+
+def test_tunnel_inlet_temperature_and_symmetry_plane_are_handled():
+    cfg = SolverConfig3D(nx=16, ny=4, nz=8, h=0.25, x0=0, y0=0, z0=-1, re=20.0, device="cpu", dtype="float64",
+                         bc_y="symmetry_ymin", dye=False, perturb=False,
+                         thermal=ThermalConfig3D(prandtl=1.0, theta_inlet=1.0, walls={"front": ("temperature", 3.0)}))
+    kinds, _, notes = T.thermal_sides(cfg)
+    assert tuple(kinds) == ("inflow", "outflow", "adiabatic", "adiabatic", "adiabatic", "adiabatic")
+    assert any("front" in n for n in notes)
+    ops = su.make_operators3d(cfg, device="cpu")
+    u = torch.ones(8, 4, 17, dtype=torch.float64)
+    v = torch.zeros(8, 5, 16, dtype=torch.float64)
+    w = torch.zeros(9, 4, 16, dtype=torch.float64)
+    ops, th, st, theta = _run(cfg, 0.0, 400, u=(u, v, w))
+    assert float(theta.min()) == pytest.approx(1.0, abs=1e-9)
+    assert float(theta.max()) == pytest.approx(1.0, abs=1e-9)
+
+
+def test_planning_helpers_match_the_step_rule():
+    assert T.planning_speed_sum(0.0, 2.1) == 2.1
+    assert T.planning_speed_sum(627.0, 2.1) == pytest.approx(T.PLUME_SPEED_FACTOR * math.sqrt(627.0))
+    h, pe, S = 1 / 12, 80 * 0.71, 3.0
+    ref = 1.0 / (S * (1.0 / (0.5 * h)) + (1.0 / pe) / (h * h / 6.0))
+    assert T.thermal_dt_estimate(h, S, pe) == pytest.approx(ref, rel=1e-12)

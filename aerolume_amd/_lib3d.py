@@ -20,23 +20,34 @@ class Desc3D(ctypes.Structure):
                 ("ref_area", f64), ("length_ref", f64), ("span_ref", f64), ("moment_ref", f64 * 3),
                 ("nut_model", i32), ("reserved", i32), ("nut_trust", f64), ("dye", i32), ("dye_k0", i32),
                 ("dye_k1", i32), ("dye_j0", i32), ("dye_j1", i32), ("dye_i0", i32), ("dye_i1", i32),
-                ("reserved2", i32)]
+                ("reserved2", i32), ("thermal", i32), ("theta_side", i32 * 6), ("theta_q_k0", i32),
+                ("theta_q_k1", i32), ("theta_q_j0", i32), ("theta_q_j1", i32), ("theta_q_i0", i32),
+                ("theta_q_i1", i32), ("theta_nt", i32), ("theta_tbody", i32 * 8), ("theta_adia", i32),
+                ("theta_nskin", i32), ("richardson", f64), ("prandtl_turbulent", f64)]
 
 
 class Fields3D(ctypes.Structure):
-    _fields_ = [(k, vp) for k in ("u", "v", "w", "p", "t", "dt", "dye")]
+    _fields_ = [(k, vp) for k in ("u", "v", "w", "p", "t", "dt", "dye", "theta")]
+
+
+#: momentum window arrays (the Geometry3D tensors of the same names)
+GEOM_MOMENTUM = ("chi_u", "chi_v", "chi_w", "chi_c", "a_u", "a_v", "a_w", "k_u", "k_v", "k_w")
+GEOM_THERMAL = ("chi_t", "a_t", "w_t", "adia_u", "adia_v", "adia_w", "vol_w", "theta_q", "skin_idx", "skin_w",
+                "skin_off", "body_theta", "flux_rate")
 
 
 class Geom3D(ctypes.Structure):
-    _fields_ = [(k, vp) for k in ("chi_u", "chi_v", "chi_w", "chi_c", "a_u", "a_v", "a_w", "k_u", "k_v", "k_w")]
+    _fields_ = [(k, vp) for k in GEOM_MOMENTUM + GEOM_THERMAL]
 
 
 class Params3D(ctypes.Structure):
-    _fields_ = [(k, vp) for k in ("nu", "eta", "u_inlet", "dt_fixed", "nut_field", "dye_weight", "dye_pulse")]
+    _fields_ = [(k, vp) for k in ("nu", "eta", "u_inlet", "dt_fixed", "nut_field", "dye_weight", "dye_pulse",
+                                  "theta_scalars", "theta_side_values")]
 
 
 class Outputs3D(ctypes.Structure):
-    _fields_ = [("coeffs", vp), ("hist", vp), ("hist_count", vp), ("hist_cap", ctypes.c_int64)]
+    _fields_ = [("coeffs", vp), ("hist", vp), ("hist_count", vp), ("hist_cap", ctypes.c_int64), ("heat_rate", vp),
+                ("theta_wall", vp), ("phist", vp), ("phist_walls", i32), ("reserved", i32)]
 
 
 def bind(lib) -> None:

@@ -17,9 +17,15 @@
 //       p3      DCT / DHT along z, x 1/lambda (FP64-built), inverse along z
 //       p4      inverse along y
 //       p5      inverse along x -> psi; step-averaged pressure (core.py:874-875)
+//   (theta)     with heat transfer, after each stage kernel: the heat-equation stage with the
+//               same stage velocity (MUSCL / van Leer advection, flux diffusion with the
+//               turbulent part, adiabatic bodies, sources, held-temperature penalisation);
+//               the stage kernel adds the buoyancy Ri theta to w after its penalisation
+//               (ns3d/physics.py:647-791)
 //   final       u = u^ - grad psi written to the state; per-block partials of the next
 //               step's dt inputs and of the force sums
-//   finalize    one block: six coefficients per body, t, dt, next dt, history ring
+//   finalize    one block: six coefficients per body, t, dt, next dt, history ring, heat
+//               rates and wall temperatures of the bodies
 //
 // Arithmetic in the plan's dtype T (float or double); the Poisson eigenvalue inverse is
 // formed in double; the outlet and force sums accumulate in double. No atomics: every
@@ -47,6 +53,8 @@ constexpr int kMaxQ = 9 * kMaxBodies;      // force sums per body: u (Fx, My, Mz
 constexpr double kSsp[3][2] = {{0.0, 1.0}, {0.75, 0.25}, {1.0 / 3.0, 2.0 / 3.0}};
 constexpr double kForceW[3] = {1.0 / 6.0, 2.0 / 3.0, 1.0};
 constexpr double kViscousDt = 1.0 / 6.0;   // ns3d/core.py:51
+constexpr int kPm = 5;                     // per-tile maxima: |u|, |v|, |w|, nu_t, |theta|
+constexpr int kMaxT = 8;                   // held-temperature bodies
 
 // cells per stage tile (x, y, z), 1024 threads per tile: 80 KB (float) of LDS per tile, so two
 // tiles (32 waves) fit a CU's 160 KB
@@ -96,6 +104,11 @@ struct Dev3 {
   FftPlan fp[3];
   int dct[3];
   int ns1, ns2, ns3, ns5;
+  // heat transfer (ns3d/physics.py:494-830)
+  int theta, buoy, tside[6], tqk0, tqk1, tqj0, tqj1, tqi0, tqi1, tqwx, tqwy, tnt, tbody[kMaxT], tadia, tnskin;
+  T ri, half_ri, ri_w[3], abs_ri, prt, kc, cadv, cfl, c6, h3;
+  T *th_a, *th_b, *nut_c;
+  double *pheat;             // [3][window tiles][kMaxT] heat increments of held-temperature bodies
 };
 
 // Per-launch pointers (by value).
@@ -109,6 +122,12 @@ struct Call3 {
   const T *nu, *eta, *u_inlet, *dt_fixed;
   const T *nut_field;        // learned-closure eddy viscosity (cells), when d.nut_model
   const T *dye_weight, *dye_pulse;
+  const T *th0, *thp;        // theta at the step start and of the previous stage
+  T *tho;                    // theta of this stage
+  const T *chi_t, *a_t, *w_t, *adia_u, *adia_v, *adia_w, *vol_w, *tq;
+  const int *skin_idx, *skin_off;
+  const T *skin_w, *body_theta, *flux_rate;
+  const T *th_sc, *th_side;  // [kappa, kappa / h, eta_t, kappa max|q|, heat scale, kappa / (h^2/6)]; side values
   int stage;
   int corr_stage;            // red->corr slot added at the outlet face of up (-1: none)
 };
@@ -381,7 +400,7 @@ __device__ __forceinline__ T eddy3(int turb, T les_k, T g[3][3]) {
 // the window, core.py:491-503); returns this thread's max over the cells the tile owns
 // (its own cells, plus the ghost ring at domain boundaries).
 template <typename T>
-__device__ T nu_tile(const Dev3<T> &d, const Call3<T> &c, Lds3<T> &L, int k0, int j0, int i0) {
+__device__ T nu_tile(const Dev3<T> &d, const Call3<T> &c, Lds3<T> &L, int k0, int j0, int i0, bool store = false) {
   using S = Lds3<T>;
   T mx = 0;
   const int ie = min(i0 + S::TX, d.nx), je = min(j0 + S::TY, d.ny), ke = min(k0 + S::TZ, d.nz);
@@ -405,6 +424,9 @@ __device__ T nu_tile(const Dev3<T> &d, const Call3<T> &c, Lds3<T> &L, int k0, in
     if (d.win && gk >= d.wk0 && gk < d.wk1 && gj >= d.wj0 && gj < d.wj1 && gi >= d.wi0 && gi < d.wi1)
       nu = nu * ((T)1 - c.chi_c[((size_t)(gk - d.wk0) * d.wy + (gj - d.wj0)) * d.wx + (gi - d.wi0)]);
     L.nu[z][y][x] = nu;
+    // the heat equation's turbulent diffusivity reads the stage's nu_t of the cells
+    if (store && gi >= i0 && gi < ie && gj >= j0 && gj < je && gk >= k0 && gk < ke)
+      d.nut_c[((size_t)gk * d.ny + gj) * d.nx + gi] = nu;
     const bool oi = (gi >= i0 && gi < ie) || gi == -1 || gi == d.nx;
     const bool oj = (gj >= j0 && gj < je) || gj == -1 || gj == d.ny;
     const bool ok = (gk >= k0 && gk < ke) || gk == -1 || gk == d.nz;
@@ -528,7 +550,7 @@ __global__ void __launch_bounds__(kTileThreads, Tile<T>::WAVES_PER_SIMD) k_stage
   const int s = c.stage;
   const T corr = c.corr_stage >= 0 ? d.red->corr[c.corr_stage] : (T)0;
   load_tile(d, c, L, k0, j0, i0, corr);
-  if (d.turb) (void)nu_tile(d, c, L, k0, j0, i0);
+  if (d.turb) (void)nu_tile(d, c, L, k0, j0, i0, d.theta != 0);
   const T dt = step_dt(d, c);
   const T a = (T)kSsp[s][0];
   const T bdt = dt * (T)kSsp[s][1];
@@ -598,12 +620,24 @@ __global__ void __launch_bounds__(kTileThreads, Tile<T>::WAVES_PER_SIMD) k_stage
       const T lw = zb ? (T)0 : rhs_w(d, L, lk, lj, li, nu_h2);
       const T wpv = LW(lk, lj, li);
       T wh = (s == 0 ? wpv : lerp_t(wpv, c.w0[idx], a)) + lw * bdt;
-      if (d.win && gk >= d.wk0 && gk <= d.wk1 && gj >= d.wj0 && gj < d.wj1 && gi >= d.wi0 && gi < d.wi1) {
-        const size_t wi = ((size_t)(gk - d.wk0) * d.wy + (gj - d.wj0)) * d.wx + (gi - d.wi0);
+      const bool inw = d.win && gk >= d.wk0 && gk <= d.wk1 && gj >= d.wj0 && gj < d.wj1 && gi >= d.wi0 && gi < d.wi1;
+      const size_t wi = inw ? ((size_t)(gk - d.wk0) * d.wy + (gj - d.wj0)) * d.wx + (gi - d.wi0) : 0;
+      if (inw) {
         const T wn = (wh + kpen * c.a_w[wi]) / ((T)1 + kpen * c.chi_w[wi]);
         const T dw = wn - wh;
         d.acc_w[wi] = s == 0 ? dw * ws : d.acc_w[wi] + dw * ws;
         wh = wn;
+      }
+      if (d.buoy) {
+        // b dt Ri theta on the w-faces after the penalisation, everywhere; the solid's share
+        // (Ri theta_f vol_w) leaves the body force (physics.py:744-791)
+        const size_t pl = (size_t)ny * nx, col = (size_t)gj * nx + gi;
+        const int zn = nz - 1;
+        const T tk = c.thp[(size_t)(gk < nz ? gk : (d.per[2] ? 0 : zn)) * pl + col];
+        const T tm = c.thp[(size_t)(gk > 0 ? gk - 1 : (d.per[2] ? zn : 0)) * pl + col];
+        if (gk > 0 && gk < nz) wh = wh + ((T)0.5 * (tk + tm)) * (bdt * d.ri);
+        else if (d.per[2]) wh = wh + ((tk + tm) * d.half_ri) * bdt;
+        if (inw) d.acc_w[wi] = d.acc_w[wi] + (((T)0.5 * (tk + tm)) * c.vol_w[wi]) * (bdt * d.ri_w[s]);
       }
       c.wo[idx] = wh;
     }
@@ -638,7 +672,16 @@ __global__ void __launch_bounds__(kTileThreads, Tile<T>::WAVES_PER_SIMD) k_final
   T mnu = 0;
   if (d.turb) mnu = nu_tile(d, c, L, k0, j0, i0);
   const int ie = min(i0 + S::TX, nx), je = min(j0 + S::TY, ny), ke = min(k0 + S::TZ, nz);
-  T mu = 0, mv = 0, mw = 0;
+  T mu = 0, mv = 0, mw = 0, mt = 0;
+  if (mode == 0 && d.theta) {
+    // max |theta| of the state (the free-fall limit of the dt rule)
+    const int nfx = ie - i0, nfy = je - j0, nfz = ke - k0;
+    for (int e = threadIdx.x; e < nfx * nfy * nfz; e += kTileThreads) {
+      const int li = e % nfx, t = e / nfx, lj = t % nfy, lk = t / nfy;
+      const T x = fabs(c.th0[((size_t)(k0 + lk) * ny + (j0 + lj)) * nx + i0 + li]);
+      mt = x > mt ? x : mt;
+    }
+  }
   {
     const int fe = face_end(i0, S::TX, nx), nfx = fe - i0, nfy = je - j0, nfz = ke - k0;
     for (int e = threadIdx.x; e < nfx * nfy * nfz; e += kTileThreads) {
@@ -670,20 +713,24 @@ __global__ void __launch_bounds__(kTileThreads, Tile<T>::WAVES_PER_SIMD) k_final
   mv = wave_max(mv);
   mw = wave_max(mw);
   mnu = wave_max(mnu);
+  mt = wave_max(mt);
   T *red = reinterpret_cast<T *>(L.red);
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   __syncthreads();
   if (lane == 0) {
-    red[wv * 4 + 0] = mu;
-    red[wv * 4 + 1] = mv;
-    red[wv * 4 + 2] = mw;
-    red[wv * 4 + 3] = mnu;
+    red[wv * kPm + 0] = mu;
+    red[wv * kPm + 1] = mv;
+    red[wv * kPm + 2] = mw;
+    red[wv * kPm + 3] = mnu;
+    red[wv * kPm + 4] = mt;
   }
   __syncthreads();
-  if (threadIdx.x < 4) {
+  // slot 4 (|theta|) of the end-of-step state comes from the last heat-equation stage
+  const int nslot = (mode == 0 && d.theta) ? kPm : 4;
+  if (threadIdx.x < nslot) {
     T m = red[threadIdx.x];
-    for (int q = 1; q < kTileWaves; ++q) m = red[q * 4 + threadIdx.x] > m ? red[q * 4 + threadIdx.x] : m;
-    d.pmax[(size_t)tile * 4 + threadIdx.x] = m;
+    for (int q = 1; q < kTileWaves; ++q) m = red[q * kPm + threadIdx.x] > m ? red[q * kPm + threadIdx.x] : m;
+    d.pmax[(size_t)tile * kPm + threadIdx.x] = m;
   }
   // force sums of this tile's window faces: sum_faces k[b, m] * acc (core.py:653-660)
   if (mode && d.win && bx >= d.wbx0 && bx < d.wbx0 + d.nwbx && by >= d.wby0 && by < d.wby0 + d.nwby &&
@@ -753,34 +800,59 @@ __global__ void __launch_bounds__(kTileThreads, Tile<T>::WAVES_PER_SIMD) k_grad(
 }
 
 // dt rule: min(cfl h / (max|u| + max|v| + max|w|), h^2 / 6 / (nu + nu_t,max)) (core.py:637-646)
+// With heat transfer also (physics.py:1230-1262, 723-742): the combined advection-diffusion
+// bound of theta 1 / (umax / (h/2) + kappa_eff / (h^2/6)), the free-fall limit
+// cfl sqrt(h / (|Ri| max|theta|)) and the heaters' bounds 1 / S, (6 cfl h / (|Ri| S))^(1/3).
 template <typename T>
-__device__ T cfl_dt(const Dev3<T> &d, const Call3<T> &c, T mu, T mv, T mw, T mnu) {
-  T umax = (mu + mv) + mw;
+__device__ T cfl_dt(const Dev3<T> &d, const Call3<T> &c, const T m[kPm]) {
+  T umax = (m[0] + m[1]) + m[2];
   umax = umax > (T)1e-6 ? umax : (T)1e-6;
   const T dta = d.cfl_h / umax;
-  const T nu = d.turb ? *c.nu + mnu : *c.nu;
+  const T nu = d.turb ? *c.nu + m[3] : *c.nu;
   const T dtb = d.visc_h2 / nu;
-  return dta < dtb ? dta : dtb;
+  T dt = dta < dtb ? dta : dtb;
+  if (d.theta) {
+    const T *sc = c.th_sc;
+    const T diff = d.turb ? (sc[0] + m[3] / d.prt) / d.visc_h2 : sc[5];
+    const T lim = (T)1 / (umax * d.cadv + diff);
+    dt = lim < dt ? lim : dt;
+    if (d.buoy) {
+      T acc = d.abs_ri * m[4];
+      acc = acc > (T)1e-12 ? acc : (T)1e-12;
+      const T ff = d.cfl * sqrt(d.h / acc);
+      dt = ff < dt ? ff : dt;
+    }
+    if (sc[3] > (T)0) {
+      const T smax = sc[3] > (T)1e-12 ? sc[3] : (T)1e-12;
+      const T a = (T)1 / smax;
+      dt = a < dt ? a : dt;
+      if (d.buoy) {
+        const T b = pow(d.c6 / (d.abs_ri * smax), (T)(1.0 / 3.0));
+        dt = b < dt ? b : dt;
+      }
+    }
+  }
+  return dt;
 }
 
 // Max of the per-tile partials (fixed order), by all threads of a 1024-thread block.
 template <typename T>
-__device__ void reduce_pmax(const Dev3<T> &d, T out[4], T *lds) {
-  T m[4] = {0, 0, 0, 0};
+__device__ void reduce_pmax(const Dev3<T> &d, T out[kPm], T *lds) {
+  const int nq = d.theta ? kPm : 4;
+  T m[kPm] = {0, 0, 0, 0, 0};
   for (int b = threadIdx.x; b < d.nblk; b += blockDim.x)
+    for (int q = 0; q < nq; ++q) m[q] = d.pmax[(size_t)b * kPm + q] > m[q] ? d.pmax[(size_t)b * kPm + q] : m[q];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) m[q] = d.pmax[(size_t)b * 4 + q] > m[q] ? d.pmax[(size_t)b * 4 + q] : m[q];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) m[q] = wave_max(m[q]);
+  for (int q = 0; q < kPm; ++q) m[q] = wave_max(m[q]);
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
   if (lane == 0)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) lds[wv * 4 + q] = m[q];
+    for (int q = 0; q < kPm; ++q) lds[wv * kPm + q] = m[q];
   __syncthreads();
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
+  for (int q = 0; q < kPm; ++q) {
     T r = lds[q];
-    for (int w = 1; w < nw; ++w) r = lds[w * 4 + q] > r ? lds[w * 4 + q] : r;
+    for (int w = 1; w < nw; ++w) r = lds[w * kPm + q] > r ? lds[w * kPm + q] : r;
     out[q] = r;
   }
   __syncthreads();
@@ -788,11 +860,11 @@ __device__ void reduce_pmax(const Dev3<T> &d, T out[4], T *lds) {
 
 template <typename T>
 __global__ void __launch_bounds__(kFinThreads) k_dt(const Dev3<T> *__restrict__ dp, Call3<T> c) {
-  __shared__ T lds[64];
+  __shared__ T lds[kPm * (kFinThreads / 64)];
   const Dev3<T> &d = *dp;
-  T m[4];
+  T m[kPm];
   reduce_pmax(d, m, lds);
-  if (threadIdx.x == 0) d.red->dt = cfl_dt(d, c, m[0], m[1], m[2], m[3]);
+  if (threadIdx.x == 0) d.red->dt = cfl_dt(d, c, m);
 }
 
 // Outputs of the finalize kernel.
@@ -803,15 +875,57 @@ struct Fin3 {
   long long *hist_count;
   long long hist_cap;
   int next_dt;
+  const T *theta;                   // end-of-step temperature (heat transfer)
+  T *heat_rate, *theta_wall, *phist;
+  int phist_walls;
 };
 
 template <typename T>
 __global__ void __launch_bounds__(kFinThreads) k_finalize(const Dev3<T> *__restrict__ dp, Call3<T> c, Fin3<T> f) {
-  __shared__ T lds[64];
+  __shared__ T lds[kPm * (kFinThreads / 64)];
   __shared__ double fs[kMaxQ];
   __shared__ double part[kFinThreads];
+  __shared__ double hsum[kMaxBodies], wsum[kMaxBodies];
   const Dev3<T> &d = *dp;
   const T dt = step_dt(d, c);
+  const bool heat = d.theta && d.nbody > 0 && (f.heat_rate || f.theta_wall || f.phist);
+  if (heat) {
+    // heat increments of the held-temperature bodies over the stages and window tiles
+    const int nt = d.tnt;
+    if (threadIdx.x < kMaxBodies) hsum[threadIdx.x] = wsum[threadIdx.x] = 0.0;
+    __syncthreads();
+    if (nt > 0) {
+      const int P = kFinThreads / nt, q = threadIdx.x % nt, sl = threadIdx.x / nt;
+      double a = 0.0;
+      if (sl < P)
+        for (int t = sl; t < 3 * d.nwt; t += P) a += d.pheat[(size_t)t * kMaxT + q];
+      part[threadIdx.x] = a;
+      __syncthreads();
+      if (threadIdx.x < nt) {
+        double r = 0.0;
+        for (int s2 = 0; s2 < P; ++s2) r += part[s2 * nt + threadIdx.x];
+        hsum[d.tbody[threadIdx.x]] = r;
+      }
+      __syncthreads();
+    }
+    // skin-weighted wall temperature of the heat-flux bodies (physics.py:420-441)
+    if (d.tnskin > 0)
+      for (int b = 0; b < d.nbody; ++b) {
+        const int o0 = c.skin_off[b], o1 = c.skin_off[b + 1];
+        double a = 0.0;
+        for (int e = o0 + (int)threadIdx.x; e < o1; e += kFinThreads) a += (double)(f.theta[c.skin_idx[e]] * c.skin_w[e]);
+        a = wave_sum(a);
+        const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        if (lane == 0) part[wv] = a;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+          double r = 0.0;
+          for (int w2 = 0; w2 < kFinThreads / 64; ++w2) r += part[w2];
+          wsum[b] = r;
+        }
+        __syncthreads();
+      }
+  }
   // force sums over window tiles: thread (q, slice) sums tiles slice, slice + P, ... in order
   const int nq = d.nq;
   if (nq > 0) {
@@ -829,7 +943,7 @@ __global__ void __launch_bounds__(kFinThreads) k_finalize(const Dev3<T> *__restr
     }
     __syncthreads();
   }
-  T mx[4] = {0, 0, 0, 0};
+  T mx[kPm] = {0, 0, 0, 0, 0};
   if (f.next_dt && !d.fixed_dt) reduce_pmax(d, mx, lds);
   if (threadIdx.x != 0) return;
   const T scale = d.neg_h3 / dt;
@@ -860,6 +974,27 @@ __global__ void __launch_bounds__(kFinThreads) k_finalize(const Dev3<T> *__restr
   const T tnew = *f.t_in + dt;
   if (f.t_out) *f.t_out = tnew;
   if (f.dt_out) *f.dt_out = dt;
+  if (heat) {
+    // heat rate in conduction units: increments x (sym h^3 Pe / dt) + released flux (physics.py:713-721)
+    const T scale = c.th_sc[4] / dt;
+    const T sym = d.symmetry ? (T)2 : (T)1;
+    T *prow = nullptr;
+    if (f.phist && f.hist_count && f.hist_cap > 0) {
+      const int cols = 1 + d.nbody * (f.phist_walls ? 2 : 1);
+      prow = f.phist + (size_t)(*f.hist_count % f.hist_cap) * cols;
+      prow[0] = tnew;
+    }
+    for (int b = 0; b < d.nbody; ++b) {
+      const T q = (T)hsum[b] * scale + c.flux_rate[b] * sym;
+      const T tw = c.body_theta[b] + (T)wsum[b];
+      if (f.heat_rate) f.heat_rate[b] = q;
+      if (f.theta_wall) f.theta_wall[b] = tw;
+      if (prow) {
+        prow[1 + b] = q;
+        if (f.phist_walls) prow[1 + d.nbody + b] = tw;
+      }
+    }
+  }
   if (f.hist && f.hist_count && f.hist_cap > 0) {
     long long n = *f.hist_count;
     T *row = f.hist + (size_t)(n % f.hist_cap) * 7;
@@ -867,7 +1002,7 @@ __global__ void __launch_bounds__(kFinThreads) k_finalize(const Dev3<T> *__restr
     for (int m = 0; m < 6; ++m) row[1 + m] = tot[m];
     *f.hist_count = n + 1;
   }
-  if (f.next_dt && !d.fixed_dt) d.red->dt = cfl_dt(d, c, mx[0], mx[1], mx[2], mx[3]);
+  if (f.next_dt && !d.fixed_dt) d.red->dt = cfl_dt(d, c, mx);
 }
 
 
@@ -944,6 +1079,196 @@ __global__ void __launch_bounds__(256) k_dye(const Dev3<T> *__restrict__ dp, Cal
       r = r + wgt * on * ((T)1 - r);
     }
     cout[e] = r;
+  }
+}
+
+// ------------------------------------------------------------------ heat transfer
+// theta tile of a stage: the tile's cells with two ghost layers (cross-shaped use: edge and
+// corner positions are never read).
+template <typename T>
+struct LdsT {
+  static constexpr int TX = Tile<T>::X, TY = Tile<T>::Y, TZ = Tile<T>::Z;
+  static constexpr int X = TX + 4, Y = TY + 4, Z = TZ + 4;
+  T c[Z][Y][X];
+  double red[kTileWaves * kMaxT];
+};
+#define LT(k, j, i) L.c[(k) + 2][(j) + 2][(i) + 2]
+
+// theta at position g (outside 0..n-1) of a non-periodic side (physics.py:589-606): outflow
+// zero gradient, adiabatic mirror, inflow value, temperature 2 theta_w - mirror, heat flux
+// mirror + G h (first layer) / 3 G h (second).
+template <typename T>
+__device__ __forceinline__ T theta_ghost(const Dev3<T> &d, const Call3<T> &c, const T *th, int side, int g, int n,
+                                         size_t base, size_t stride) {
+  const bool lo = g < 0;
+  const int m = lo ? -1 - g : 2 * n - 1 - g;
+  switch (d.tside[side]) {
+    case AERO3D_THETA_OUTFLOW: return th[base + (size_t)(lo ? 0 : n - 1) * stride];
+    case AERO3D_THETA_INFLOW: return c.th_side[side];
+    case AERO3D_THETA_TEMPERATURE: return (T)2 * c.th_side[side] - th[base + (size_t)m * stride];
+    case AERO3D_THETA_HEAT_FLUX: {
+      const T wf = (g == -1 || g == n) ? d.h : d.h3;
+      return th[base + (size_t)m * stride] + c.th_side[side] * wf;
+    }
+    default: return th[base + (size_t)m * stride];   // adiabatic
+  }
+}
+
+// Stage input of velocity component C at face (k, j, i): the previous stage's u^ with its
+// projection and outlet shift applied exactly as the stage kernel's tile load does.
+template <typename T, int C>
+__device__ __forceinline__ T face_vel(const Dev3<T> &d, const Call3<T> &c, int k, int j, int i, T corr) {
+  const int nx = d.nx, ny = d.ny;
+  const int ex = nx + (C == 0), ey = ny + (C == 1);
+  const T *src = C == 0 ? c.up : (C == 1 ? c.vp : c.wp);
+  T val = src[((size_t)k * ey + j) * ex + i];
+  if (c.psi) {
+    const int pn = C == 0 ? nx : (C == 1 ? ny : d.nz);
+    const int f = C == 0 ? i : (C == 1 ? j : k);
+    if (d.per[C] || (f > 0 && f < pn)) {
+      const int fc = f == pn ? 0 : f;
+      const int fm = fc == 0 ? pn - 1 : fc - 1;
+      const size_t ps = C == 0 ? 1 : (C == 1 ? (size_t)nx : (size_t)nx * ny);
+      const T *pb = c.psi + ((size_t)(C == 2 ? 0 : k) * ny + (C == 1 ? 0 : j)) * nx + (C == 0 ? 0 : i);
+      val = val - (pb[(size_t)fc * ps] - pb[(size_t)fm * ps]) * d.inv_h;
+    }
+  }
+  if (C == 0 && c.corr_stage >= 0 && d.outlet && i == nx) val = val + corr;
+  return val;
+}
+
+// Total theta flux through the low face (along axis AX) of local cell (lk, lj, li) = global
+// (gk, gj, gi): MUSCL advection minus kappa_f (c - b), conduction removed through adiabatic
+// bodies (physics.py:623-676).
+template <typename T, int AX>
+__device__ __forceinline__ T theta_face(const Dev3<T> &d, const Call3<T> &c, const LdsT<T> &L, int lk, int lj, int li,
+                                        int gk, int gj, int gi, T kh, T corr) {
+  T a, b, cc, dd;
+  if (AX == 0) {
+    a = LT(lk, lj, li - 2); b = LT(lk, lj, li - 1); cc = LT(lk, lj, li); dd = LT(lk, lj, li + 1);
+  } else if (AX == 1) {
+    a = LT(lk, lj - 2, li); b = LT(lk, lj - 1, li); cc = LT(lk, lj, li); dd = LT(lk, lj + 1, li);
+  } else {
+    a = LT(lk - 2, lj, li); b = LT(lk - 1, lj, li); cc = LT(lk, lj, li); dd = LT(lk + 1, lj, li);
+  }
+  const T vel = face_vel<T, AX>(d, c, gk, gj, gi, corr);
+  const T diff = cc - b;
+  const int g = AX == 0 ? gi : (AX == 1 ? gj : gk);
+  const int n = AX == 0 ? d.nx : (AX == 1 ? d.ny : d.nz);
+  T kf = kh;
+  if (d.turb && (d.per[AX] || (g > 0 && g < n))) {
+    const int gc = g == n ? 0 : g, gm = gc == 0 ? n - 1 : gc - 1;
+    const size_t col = AX == 0 ? ((size_t)gk * d.ny + gj) * d.nx : (AX == 1 ? (size_t)gk * d.ny * d.nx + gi
+                                                                            : (size_t)gj * d.nx + gi);
+    const size_t st = AX == 0 ? 1 : (AX == 1 ? (size_t)d.nx : (size_t)d.nx * d.ny);
+    kf = kh + d.kc * (d.nut_c[col + (size_t)gm * st] + d.nut_c[col + (size_t)gc * st]);
+  }
+  T f = dye_flux(a, b, cc, dd, vel) - kf * diff;
+  if (d.tadia) {
+    const int ek = AX == 2, ej = AX == 1, ei = AX == 0;
+    if (gk >= d.wk0 && gk < d.wk1 + ek && gj >= d.wj0 && gj < d.wj1 + ej && gi >= d.wi0 && gi < d.wi1 + ei) {
+      const T *chi = AX == 0 ? c.adia_u : (AX == 1 ? c.adia_v : c.adia_w);
+      const size_t wi = ((size_t)(gk - d.wk0) * (d.wy + ej) + (gj - d.wj0)) * (d.wx + ei) + (gi - d.wi0);
+      f = f + (kf * chi[wi]) * diff;
+    }
+  }
+  return f;
+}
+
+// One heat-equation stage (physics.py:678-708) on a tile: theta^ = lerp(theta_prev, theta0, a)
+// - b dt div F + b dt kappa q, then the held-temperature bodies' implicit penalisation with
+// their heat increments (per-tile partials); the last stage also leaves max |theta|.
+template <typename T>
+__global__ void __launch_bounds__(kTileThreads) k_theta(const Dev3<T> *__restrict__ dp, Call3<T> c) {
+  using S = LdsT<T>;
+  extern __shared__ unsigned char smem[];
+  S &L = *reinterpret_cast<S *>(smem);
+  const Dev3<T> &d = *dp;
+  int bx, by, bz, tile;
+  tile_of(d, bx, by, bz, tile);
+  const int i0 = bx * S::TX, j0 = by * S::TY, k0 = bz * S::TZ;
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  const int ie = min(i0 + S::TX, nx), je = min(j0 + S::TY, ny), ke = min(k0 + S::TZ, nz);
+  const T *th = c.thp;
+  for (int e = threadIdx.x; e < S::X * S::Y * S::Z; e += kTileThreads) {
+    const int x = e % S::X, t = e / S::X, y = t % S::Y, z = t / S::Y;
+    int gi = i0 - 2 + x, gj = j0 - 2 + y, gk = k0 - 2 + z;
+    if (gi >= ie + 2 || gj >= je + 2 || gk >= ke + 2) continue;     // beyond the tile's stencils
+    if (d.per[0]) gi = wrapi(gi, nx);
+    if (d.per[1]) gj = wrapi(gj, ny);
+    if (d.per[2]) gk = wrapi(gk, nz);
+    const int ox = gi < 0 || gi >= nx, oy = gj < 0 || gj >= ny, oz = gk < 0 || gk >= nz;
+    if (ox + oy + oz > 1) continue;                                   // edges / corners: never read
+    T val;
+    if (ox) val = theta_ghost(d, c, th, gi < 0 ? 0 : 1, gi, nx, ((size_t)gk * ny + gj) * nx, 1);
+    else if (oy) val = theta_ghost(d, c, th, gj < 0 ? 2 : 3, gj, ny, (size_t)gk * ny * nx + gi, (size_t)nx);
+    else if (oz) val = theta_ghost(d, c, th, gk < 0 ? 4 : 5, gk, nz, (size_t)gj * nx + gi, (size_t)nx * ny);
+    else val = th[((size_t)gk * ny + gj) * nx + gi];
+    L.c[z][y][x] = val;
+  }
+  __syncthreads();
+  const int s = c.stage;
+  const T dt = step_dt(d, c);
+  const T a = (T)kSsp[s][0];
+  const T bdt = dt * (T)kSsp[s][1];
+  const T wgt = (T)kForceW[s];
+  const T kh = c.th_sc[1];
+  const T bk = bdt * c.th_sc[0];
+  const T kpen = d.tnt ? bdt / c.th_sc[2] : (T)0;
+  const T corr = c.corr_stage >= 0 ? d.red->corr[c.corr_stage] : (T)0;
+  const int nfx = ie - i0, nfy = je - j0, nfz = ke - k0;
+  double hb[kMaxT];
+#pragma unroll
+  for (int b = 0; b < kMaxT; ++b) hb[b] = 0.0;
+  T mt = 0;
+  for (int e = threadIdx.x; e < nfx * nfy * nfz; e += kTileThreads) {
+    const int li = e % nfx, t = e / nfx, lj = t % nfy, lk = t / nfy;
+    const int gi = i0 + li, gj = j0 + lj, gk = k0 + lk;
+    const T fx0 = theta_face<T, 0>(d, c, L, lk, lj, li, gk, gj, gi, kh, corr);
+    const T fx1 = theta_face<T, 0>(d, c, L, lk, lj, li + 1, gk, gj, gi + 1, kh, corr);
+    const T fy0 = theta_face<T, 1>(d, c, L, lk, lj, li, gk, gj, gi, kh, corr);
+    const T fy1 = theta_face<T, 1>(d, c, L, lk, lj + 1, li, gk, gj + 1, gi, kh, corr);
+    const T fz0 = theta_face<T, 2>(d, c, L, lk, lj, li, gk, gj, gi, kh, corr);
+    const T fz1 = theta_face<T, 2>(d, c, L, lk + 1, lj, li, gk + 1, gj, gi, kh, corr);
+    const T D = (((fx1 - fx0) + (fy1 - fy0)) + (fz1 - fz0)) * d.inv_h;
+    const size_t idx = ((size_t)gk * ny + gj) * nx + gi;
+    const T tp = LT(lk, lj, li);
+    T ch = (s == 0 ? tp : lerp_t(tp, c.th0[idx], a)) - bdt * D;
+    if (gk >= d.tqk0 && gk < d.tqk1 && gj >= d.tqj0 && gj < d.tqj1 && gi >= d.tqi0 && gi < d.tqi1)
+      ch = ch + bk * c.tq[((size_t)(gk - d.tqk0) * d.tqwy + (gj - d.tqj0)) * d.tqwx + (gi - d.tqi0)];
+    if (d.tnt && gk >= d.wk0 && gk < d.wk1 && gj >= d.wj0 && gj < d.wj1 && gi >= d.wi0 && gi < d.wi1) {
+      const size_t wn = (size_t)d.wz * d.wy * d.wx;
+      const size_t wi = ((size_t)(gk - d.wk0) * d.wy + (gj - d.wj0)) * d.wx + (gi - d.wi0);
+      const T nw = (ch + kpen * c.a_t[wi]) / ((T)1 + kpen * c.chi_t[wi]);
+      const T inc = (nw - ch) * wgt;
+#pragma unroll
+      for (int b = 0; b < kMaxT; ++b)
+        if (b < d.tnt) hb[b] += (double)(c.w_t[(size_t)b * wn + wi] * inc);
+      ch = nw;
+    }
+    c.tho[idx] = ch;
+    mt = fabs(ch) > mt ? fabs(ch) : mt;
+  }
+  if (s == 2) {
+    mt = wave_max(mt);
+    T *red = reinterpret_cast<T *>(L.red);
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __syncthreads();
+    if (lane == 0) red[wv] = mt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      T m = red[0];
+      for (int q = 1; q < kTileWaves; ++q) m = red[q] > m ? red[q] : m;
+      d.pmax[(size_t)tile * kPm + 4] = m;
+    }
+  }
+  if (d.tnt && bx >= d.wbx0 && bx < d.wbx0 + d.nwbx && by >= d.wby0 && by < d.wby0 + d.nwby && bz >= d.wbz0 &&
+      bz < d.wbz0 + d.nwbz) {
+    const int wt = ((bz - d.wbz0) * d.nwby + (by - d.wby0)) * d.nwbx + (bx - d.wbx0);
+    __syncthreads();
+    block_sum_w<kMaxT>(hb, L.red);
+    if (threadIdx.x == 0)
+      for (int b = 0; b < kMaxT; ++b) d.pheat[((size_t)s * d.nwt + wt) * kMaxT + b] = b < d.tnt ? hb[b] : 0.0;
   }
 }
 
@@ -1309,12 +1634,14 @@ struct aero3d_plan {
   std::vector<void *> allocs;
   void *dev = nullptr;
   void *uA, *vA, *wA, *uB, *vB, *wB, *F, *psi, *acc_u, *acc_v, *acc_w, *pmax, *pout, *pforce, *red, *dye_tmp;
+  void *th_a, *th_b, *nut_c, *pheat;
   void *tw[3], *cs[3];
   double *lam[3];
   FftPlan fp[3];
   int nbx, nby, nbz, nblk, nwt, nq;
   int ns1, ns2, ns3, ns5;
   size_t lds1, lds2, lds3, lds5;
+  int nwt_alloc;
 };
 
 namespace {
@@ -1447,6 +1774,36 @@ int upload3(aero3d_plan *p) {
   d.ns2 = p->ns2;
   d.ns3 = p->ns3;
   d.ns5 = p->ns5;
+  if (s.thermal) {
+    d.theta = 1;
+    d.buoy = s.richardson != 0.0;
+    for (int q = 0; q < 6; ++q) d.tside[q] = s.theta_side[q];
+    if (s.theta_q_k1 > s.theta_q_k0 && s.theta_q_j1 > s.theta_q_j0 && s.theta_q_i1 > s.theta_q_i0) {
+      d.tqk0 = s.theta_q_k0; d.tqk1 = s.theta_q_k1; d.tqj0 = s.theta_q_j0; d.tqj1 = s.theta_q_j1;
+      d.tqi0 = s.theta_q_i0; d.tqi1 = s.theta_q_i1;
+      d.tqwx = s.theta_q_i1 - s.theta_q_i0;
+      d.tqwy = s.theta_q_j1 - s.theta_q_j0;
+    }
+    d.tnt = d.win ? s.theta_nt : 0;
+    for (int q = 0; q < kMaxT; ++q) d.tbody[q] = s.theta_tbody[q];
+    d.tadia = d.win ? s.theta_adia : 0;
+    d.tnskin = d.win ? s.theta_nskin : 0;
+    const double ri = s.richardson, ws[3] = {kForceW[0], kForceW[1], kForceW[2]};
+    d.ri = (T)ri;
+    d.half_ri = (T)(0.5 * ri);
+    for (int q = 0; q < 3; ++q) d.ri_w[q] = (T)(ri * ws[q]);
+    d.abs_ri = (T)std::fabs(ri);
+    d.prt = (T)s.prandtl_turbulent;
+    d.kc = (T)(0.5 / (s.prandtl_turbulent * h));
+    d.cadv = (T)(1.0 / (0.5 * h));
+    d.cfl = (T)s.cfl;
+    d.c6 = (T)(6.0 * s.cfl * h);
+    d.h3 = (T)3 * (T)h;
+    d.th_a = (T *)p->th_a;
+    d.th_b = (T *)p->th_b;
+    d.nut_c = (T *)p->nut_c;
+    d.pheat = (double *)p->pheat;
+  }
   AERO_HIP_CHECK(hipMemcpy(p->dev, &d, sizeof(d), hipMemcpyHostToDevice));
   // the tile kernels take more than the default 64 KB of dynamic LDS
   int dev_id = 0, max_lds = 0;
@@ -1463,6 +1820,9 @@ int upload3(aero3d_plan *p) {
                                      (int)sizeof(Lds3<T>)));
   AERO_HIP_CHECK(hipFuncSetAttribute((const void *)k_grad<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)sizeof(Lds3<T>)));
+  if (sizeof(LdsT<T>) > 64 * 1024)
+    AERO_HIP_CHECK(hipFuncSetAttribute((const void *)k_theta<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)sizeof(LdsT<T>)));
   return AERO_OK;
 }
 
@@ -1496,7 +1856,22 @@ Call3<T> make_call(const aero3d_fields *in, const aero3d_geom *g, const aero3d_p
     c.k_u = (const T *)g->k_u;
     c.k_v = (const T *)g->k_v;
     c.k_w = (const T *)g->k_w;
+    c.chi_t = (const T *)g->chi_t;
+    c.a_t = (const T *)g->a_t;
+    c.w_t = (const T *)g->w_t;
+    c.adia_u = (const T *)g->adia_u;
+    c.adia_v = (const T *)g->adia_v;
+    c.adia_w = (const T *)g->adia_w;
+    c.vol_w = (const T *)g->vol_w;
+    c.tq = (const T *)g->theta_q;
+    c.skin_idx = (const int *)g->skin_idx;
+    c.skin_off = (const int *)g->skin_off;
+    c.skin_w = (const T *)g->skin_w;
+    c.body_theta = (const T *)g->body_theta;
+    c.flux_rate = (const T *)g->flux_rate;
   }
+  c.th0 = (const T *)in->theta;
+  c.thp = c.th0;
   if (pr) {
     c.nu = (const T *)pr->nu;
     c.eta = (const T *)pr->eta;
@@ -1505,6 +1880,8 @@ Call3<T> make_call(const aero3d_fields *in, const aero3d_geom *g, const aero3d_p
     c.nut_field = (const T *)pr->nut_field;
     c.dye_weight = (const T *)pr->dye_weight;
     c.dye_pulse = (const T *)pr->dye_pulse;
+    c.th_sc = (const T *)pr->theta_scalars;
+    c.th_side = (const T *)pr->theta_side_values;
   }
   c.corr_stage = -1;
   return c;
@@ -1551,7 +1928,12 @@ int step_impl(aero3d_plan *p, hipStream_t st, const aero3d_fields *in, aero3d_fi
     c.uo = ub[stage & 1];
     c.vo = vb[stage & 1];
     c.wo = wb[stage & 1];
+    if (s.thermal) {
+      c.thp = stage == 0 ? (const T *)in->theta : (stage == 1 ? (const T *)p->th_a : (const T *)p->th_b);
+      c.tho = stage == 0 ? (T *)p->th_a : (stage == 1 ? (T *)p->th_b : (T *)out->theta);
+    }
     hipLaunchKernelGGL(k_stage<T>, grid, blk, lds, st, dev, c);
+    if (s.thermal) hipLaunchKernelGGL(k_theta<T>, grid, blk, sizeof(LdsT<T>), st, dev, c);
     const int mode = (!s.periodic_x && !s.closed) ? 2 : 1;
     int rc = poisson_passes<T>(p, st, dev, c, nullptr, mode, stage, (T *)p->psi, (T *)out->p);
     if (rc) return rc;
@@ -1591,6 +1973,15 @@ int step_impl(aero3d_plan *p, hipStream_t st, const aero3d_fields *in, aero3d_fi
     f.hist_cap = o->hist_cap;
   }
   f.next_dt = 1;
+  if (s.thermal) {
+    f.theta = (const T *)out->theta;
+    if (o) {
+      f.heat_rate = (T *)o->heat_rate;
+      f.theta_wall = (T *)o->theta_wall;
+      f.phist = (T *)o->phist;
+      f.phist_walls = o->phist_walls;
+    }
+  }
   hipLaunchKernelGGL(k_finalize<T>, dim3(1), dim3(kFinThreads), 0, st, dev, c, f);
   AERO_HIP_CHECK(hipGetLastError());
   return AERO_OK;
@@ -1650,6 +2041,10 @@ int time_stage_impl(aero3d_plan *p, hipStream_t st, const aero3d_fields *fl, con
   c.uo = (T *)p->uB;
   c.vo = (T *)p->vB;
   c.wo = (T *)p->wB;
+  if (p->desc.thermal && !fl->theta) {
+    set_error("a heat-transfer plan needs fields.theta");
+    return AERO_EINVAL;
+  }
   hipEvent_t e0, e1;
   AERO_HIP_CHECK(hipEventCreate(&e0));
   AERO_HIP_CHECK(hipEventCreate(&e1));
@@ -1700,6 +2095,32 @@ int check3_desc(const aero3d_desc *d) {
       return AERO_EUNSUPPORTED;
     }
   }
+  if (d->thermal) {
+    const int per[3] = {d->periodic_x, d->periodic_y, d->periodic_z};
+    for (int q = 0; q < 6; ++q) {
+      const int k = d->theta_side[q];
+      if (k < AERO3D_THETA_PERIODIC || k > AERO3D_THETA_HEAT_FLUX || ((k == AERO3D_THETA_PERIODIC) != (per[q / 2] != 0))) {
+        set_error("theta side kinds must be periodic exactly on the periodic axes");
+        return AERO_EINVAL;
+      }
+    }
+    if (d->theta_nt < 0 || d->theta_nt > kMaxT || d->theta_nt > d->nbody || d->theta_nskin < 0) {
+      set_error("bad heat-transfer body counts");
+      return AERO_EINVAL;
+    }
+    for (int q = 0; q < d->theta_nt; ++q)
+      if (d->theta_tbody[q] < 0 || d->theta_tbody[q] >= d->nbody) { set_error("bad held-temperature body index"); return AERO_EINVAL; }
+    const bool qw = d->theta_q_k1 > d->theta_q_k0 && d->theta_q_j1 > d->theta_q_j0 && d->theta_q_i1 > d->theta_q_i0;
+    if (qw && (d->theta_q_k0 < 0 || d->theta_q_j0 < 0 || d->theta_q_i0 < 0 || d->theta_q_k1 > d->nz ||
+               d->theta_q_j1 > d->ny || d->theta_q_i1 > d->nx)) {
+      set_error("heat source window outside the grid");
+      return AERO_EINVAL;
+    }
+    if (!(d->prandtl_turbulent > 0) || !std::isfinite(d->richardson)) {
+      set_error("the turbulent Prandtl number must be positive and Ri finite");
+      return AERO_EINVAL;
+    }
+  }
   return AERO_OK;
 }
 
@@ -1728,6 +2149,21 @@ int check3_step(const aero3d_plan *p, const aero3d_fields *in, const aero3d_fiel
               !g->k_v || !g->k_w)) {
     set_error("missing geometry pointer");
     return AERO_EINVAL;
+  }
+  if (s.thermal) {
+    if (!in->theta || !out->theta || !pr->theta_scalars || !pr->theta_side_values) {
+      set_error("a heat-transfer plan needs fields.theta and params.theta_scalars / theta_side_values");
+      return AERO_EINVAL;
+    }
+    const bool qw = s.theta_q_k1 > s.theta_q_k0 && s.theta_q_j1 > s.theta_q_j0 && s.theta_q_i1 > s.theta_q_i0;
+    if (qw && (!g || !g->theta_q)) { set_error("missing geom.theta_q"); return AERO_EINVAL; }
+    if (win && (!g->body_theta || !g->flux_rate || (s.theta_nt && (!g->chi_t || !g->a_t || !g->w_t)) ||
+                (s.theta_adia && (!g->adia_u || !g->adia_v || !g->adia_w)) || (s.richardson != 0.0 && !g->vol_w) ||
+                (s.theta_nskin && (!g->skin_idx || !g->skin_w || !g->skin_off)))) {
+      set_error("missing heat-transfer geometry pointer");
+      return AERO_EINVAL;
+    }
+    if (in->theta == out->theta && in->dye == out->theta) { set_error("theta aliases dye"); return AERO_EINVAL; }
   }
   return AERO_OK;
 }
@@ -1781,9 +2217,12 @@ int aero3d_plan_create(const aero3d_desc *desc, aero3d_plan **out) {
       {&p->uA, nu_ * es}, {&p->vA, nv_ * es}, {&p->wA, nw_ * es}, {&p->uB, nu_ * es}, {&p->vB, nv_ * es},
       {&p->wB, nw_ * es}, {&p->F, nc * es}, {&p->psi, nc * es},
       {&p->acc_u, wz * wy * (wx + 1) * es}, {&p->acc_v, wz * (wy + 1) * wx * es}, {&p->acc_w, (wz + 1) * wy * wx * es},
-      {&p->pmax, (size_t)p->nblk * 4 * es}, {&p->pout, (size_t)p->nby * p->nbz * 2 * 8},
+      {&p->pmax, (size_t)p->nblk * kPm * es}, {&p->pout, (size_t)p->nby * p->nbz * 2 * 8},
       {&p->pforce, (size_t)std::max(1, p->nwt) * std::max(1, p->nq) * 8}, {&p->red, 256},
       {&p->dye_tmp, desc->dye ? nc * es : 0},
+      {&p->th_a, desc->thermal ? nc * es : 0}, {&p->th_b, desc->thermal ? nc * es : 0},
+      {&p->nut_c, desc->thermal && desc->turbulence ? nc * es : 0},
+      {&p->pheat, desc->thermal ? (size_t)3 * std::max(1, p->nwt) * kMaxT * 8 : 0},
       {&p->tw[0], nx * 2 * es}, {&p->tw[1], ny * 2 * es}, {&p->tw[2], nz * 2 * es},
       {&p->cs[0], nx * 2 * es}, {&p->cs[1], ny * 2 * es}, {&p->cs[2], nz * 2 * es},
       {(void **)&p->lam[0], nx * 8}, {(void **)&p->lam[1], ny * 8}, {(void **)&p->lam[2], nz * 8}};

@@ -96,8 +96,8 @@ def resolve_device_dtype(cfg: SolverConfig3D, device=None) -> tuple[torch.device
 def check_supported(cfg: SolverConfig3D) -> None:
     """Raise ``NotImplementedError`` for configuration parts the MI355X 3D step does not carry."""
     missing = []
-    if cfg.thermal is not None or cfg.heat_sources:
-        missing.append("heat transfer (thermal / heat_sources)")
+    if cfg.heat_sources and cfg.thermal is None:
+        raise ValueError("heat sources need heat transfer (cfg.thermal)")
     if cfg.fans or cfg.ports:
         missing.append("fans / ports")
     if cfg.emitters:
@@ -110,10 +110,6 @@ def check_supported(cfg: SolverConfig3D) -> None:
         missing.append("adaptive_dt='pid'")
     if cfg.dye and cfg.emitters is not None:
         missing.append("Emitter3D smoke sources (the rake of dye_rake is carried)")
-    for b in cfg.bodies:
-        if b.thermal != "adiabatic":
-            missing.append("heated bodies")
-            break
     if missing:
         raise NotImplementedError("the MI355X 3D step does not carry: " + ", ".join(missing))
 

@@ -196,13 +196,30 @@ typedef struct {
   int32_t dye;              /* 1: passive smoke in fields.dye, advected after every step */
   int32_t dye_k0, dye_k1, dye_j0, dye_j1, dye_i0, dye_i1;   /* nozzle cell window (empty: none) */
   int32_t reserved2;
+  /* heat transfer (ns3d/physics.py:494-830): theta in fields.theta advanced in the RK stages */
+  int32_t thermal;
+  int32_t theta_side[6];    /* AERO3D_THETA_* of the sides x0, x1, y0, y1, z0, z1 */
+  int32_t theta_q_k0, theta_q_k1, theta_q_j0, theta_q_j1, theta_q_i0, theta_q_i1;   /* source window */
+  int32_t theta_nt;         /* held-temperature bodies (0..8), body indices in theta_tbody */
+  int32_t theta_tbody[8];
+  int32_t theta_adia;       /* 1: adiabatic-body face fractions in geom.adia_* */
+  int32_t theta_nskin;      /* entries of the heat-flux skin list (geom.skin_*) */
+  double richardson;        /* buoyancy Ri theta along +z (0: passive temperature) */
+  double prandtl_turbulent;
 } aero3d_desc;
+
+/* heat-equation side conditions (ns3d/physics.py:589-606) */
+enum {
+  AERO3D_THETA_PERIODIC = 0, AERO3D_THETA_OUTFLOW = 1, AERO3D_THETA_ADIABATIC = 2, AERO3D_THETA_INFLOW = 3,
+  AERO3D_THETA_TEMPERATURE = 4, AERO3D_THETA_HEAT_FLUX = 5
+};
 
 /* Caller-owned device buffers of the plan's dtype: u (nz, ny, nx+1), v (nz, ny+1, nx),
  * w (nz+1, ny, nx), p (nz, ny, nx), t and dt 0-d. in == out is allowed. */
 typedef struct {
   void *u, *v, *w, *p, *t, *dt;
   void *dye;                /* (nz, ny, nx) smoke concentration in [0, 1] when desc.dye */
+  void *theta;              /* (nz, ny, nx) temperature when desc.thermal */
 } aero3d_fields;
 
 /* Window arrays (ns3d/geometry.py:275-303), wz = k1-k0, wy = j1-j0, wx = i1-i0:
@@ -211,6 +228,15 @@ typedef struct {
  * k_v (nbody, 3, ...) = [Fy, Mx, Mz]; k_w (nbody, 3, ...) = [Fz, Mx, My]. */
 typedef struct {
   const void *chi_u, *chi_v, *chi_w, *chi_c, *a_u, *a_v, *a_w, *k_u, *k_v, *k_w;
+  /* heat transfer on the same window: chi_t, a_t (wz, wy, wx) held-temperature fraction and
+   * sum chi theta_b; w_t (theta_nt, wz, wy, wx) each such body's share; adia_u/v/w face
+   * fractions of adiabatic bodies (shapes of chi_u/v/w); vol_w (wz+1, wy, wx) unshifted volume
+   * fraction; theta_q source density on the desc.theta_q window; skin list: skin_idx (int32
+   * flat cell index), skin_w (weight), skin_off (int32, nbody + 1 offsets of each body's
+   * entries); body_theta (nbody) held values; flux_rate (nbody) heat a heat-flux body
+   * releases in the domain */
+  const void *chi_t, *a_t, *w_t, *adia_u, *adia_v, *adia_w, *vol_w, *theta_q;
+  const void *skin_idx, *skin_w, *skin_off, *body_theta, *flux_rate;
 } aero3d_geom;
 
 /* Live device scalars / arrays: nu, eta, dt_fixed 0-d; u_inlet (nz, ny); nut_field (nz, ny, nx)
@@ -220,6 +246,9 @@ typedef struct {
   const void *nut_field;
   const void *dye_weight;   /* nozzle weights on the nozzle window (wz, wy, wx) */
   const void *dye_pulse;    /* [t_on, period]: nozzles emit while t mod period < t_on */
+  /* heat transfer, live: theta_scalars = [kappa, kappa / h, eta_t, kappa max|q|, heat-rate
+   * scale (symmetry factor h^3 Pe), kappa / (h^2 / 6)]; theta_side_values (6) */
+  const void *theta_scalars, *theta_side_values;
 } aero3d_params;
 
 /* Per-step outputs (device; any may be NULL): coeffs (nbody, 6) = [CD, CS, CL, C_roll, Cm, Cn];
@@ -229,6 +258,11 @@ typedef struct {
   void *coeffs;
   void *hist, *hist_count;
   int64_t hist_cap;
+  /* heat transfer: heat_rate (nbody) conduction units, full body; theta_wall (nbody) held or
+   * skin-measured wall temperature; phist ring of hist_cap rows [t, heat_rate (nbody),
+   * theta_wall (nbody) when phist_walls] at the history row */
+  void *heat_rate, *theta_wall, *phist;
+  int32_t phist_walls, reserved;
 } aero3d_outputs;
 
 int aero3d_plan_create(const aero3d_desc *desc, aero3d_plan **plan);

@@ -153,7 +153,7 @@ THERMAL3D = {
         heaters=[],
         steps=5),
     "periodic_wale": dict(
-        grid=dict(nx=16, ny=12, nz=12, h=1.0 / 8, x0=-1.0, y0=-0.75, z0=-0.75, re=500.0, periodic=True,
+        grid=dict(nx=16, ny=16, nz=16, h=1.0 / 8, x0=-1.0, y0=-1.0, z0=-1.0, re=500.0, periodic=True,
                   turbulence="wale"),
         bodies=[("box", (0.5, 0.4, 0.3), (0.0, 0.0, 0.0), "temperature", -0.5)],
         thermal=dict(prandtl=0.9, richardson=2.0, prandtl_turbulent=0.7),

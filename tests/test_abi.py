@@ -85,3 +85,32 @@ def test_3d_unsupported_features_fail_loudly():
     cfg = SolverConfig3D(nx=8, ny=8, nz=8, h=0.1, x0=0, y0=0, z0=0, re=100.0, particle_capacity=16)
     with pytest.raises(NotImplementedError, match="particle"):
         su.check_supported(cfg)
+
+
+def test_ctypes_structures_match_the_header(tmp_path):
+    """Every ctypes mirror has the C structure's size and field offsets (gcc on the header)."""
+    import shutil
+    import subprocess
+    from aerolume_amd import _lib, _lib3d
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    pairs = {"aero2d_desc": _lib.Desc2D, "aero2d_fields": _lib.Fields2D, "aero2d_geom": _lib.Geom2D,
+             "aero2d_params": _lib.Params2D, "aero2d_outputs": _lib.Outputs2D, "aero3d_desc": _lib3d.Desc3D,
+             "aero3d_fields": _lib3d.Fields3D, "aero3d_geom": _lib3d.Geom3D, "aero3d_params": _lib3d.Params3D,
+             "aero3d_outputs": _lib3d.Outputs3D}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "aerolume_amd.h"', "int main(void) {"]
+    for cname, cls in pairs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(ln.split()[:2]): int(ln.split()[2]) for ln in out if ln}
+    for cname, cls in pairs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)

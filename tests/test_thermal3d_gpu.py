@@ -1,0 +1,182 @@
+"""3D heat transfer on the MI355X through the C ABI: the HIP step against the reference's own
+trajectories (float64, ``tests/golden/ns3d_thermal_*.npz``) and the oracle (float32), graph
+replays against eager steps, and the conservation properties of the reference's tests."""
+from __future__ import annotations
+
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+import oracle.ns3d as O
+from aerolume_amd.ns3d import setup as su
+from aerolume_amd.ns3d import solver as S3
+from aerolume_amd.ns3d import thermal as T
+from aerolume_amd.ns3d.config import HeatSource3D, SolverConfig3D, ThermalConfig3D
+from tests import cases
+
+pytestmark = pytest.mark.gpu
+GOLDEN = Path(__file__).resolve().parent / "golden"
+NAMES = list(cases.THERMAL3D)
+
+
+def _fixture(name):
+    path = GOLDEN / f"ns3d_thermal_{name}.npz"
+    if not path.exists():
+        pytest.skip(f"missing fixture {path.name}")
+    return dict(np.load(path))
+
+
+def _gpu_setup(name, fx, dtype="float64"):
+    cfg = cases.thermal_cfg(name, fx, dtype, device="cuda")
+    ops = S3.make_operators3d(cfg)
+    g = S3.build_geometry3d(cfg.bodies, ops)
+    th = T.ThermalOps3D(cfg, ops, g, cfg.bodies)
+    return cfg, ops, g, th
+
+
+def _state(fx, ops):
+    t = lambda a: torch.as_tensor(np.asarray(a), device=ops.device, dtype=ops.dtype).contiguous()  # noqa: E731
+    z = torch.zeros((), device=ops.device, dtype=ops.dtype)
+    return S3.State3D(t(fx["u0"]), t(fx["v0"]), t(fx["w0"]), torch.zeros(ops.nz, ops.ny, ops.nx, device=ops.device,
+                                                                           dtype=ops.dtype), z.clone(), z.clone()), t(fx["theta0"])
+
+
+def _rel(a, b):
+    a = a.detach().double().cpu().numpy() if torch.is_tensor(a) else np.asarray(a)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_thermal_steps_match_reference_float64(name):
+    print(f"[thermal gpu] reference case {name} float64", flush=True)
+    fx = _fixture(name)
+    cfg, ops, g, th = _gpu_setup(name, fx)
+    st, theta = _state(fx, ops)
+    steps = int(fx["steps"])
+    for n in range(1, steps + 1):
+        out = S3.step3d_phys(st, g, ops, thermal=th, theta=theta)
+        st, theta = out.state, out.theta
+        assert float(st.dt) == pytest.approx(float(fx["dt"][n - 1]), rel=1e-12)
+        assert np.allclose(out.coeffs.double().cpu().numpy(), fx["coeffs"][n - 1], rtol=1e-9, atol=1e-10)
+        assert np.allclose(th.heat_rate.double().cpu().numpy(), fx["heat_rate"][n - 1], rtol=1e-9, atol=1e-10)
+        if n in (1, steps):
+            for f in ("u", "v", "w", "p"):
+                assert _rel(getattr(st, f), fx[f"{f}_{n}"]) < 1e-10, (f, n)
+            err = float(np.abs(theta.double().cpu().numpy() - fx[f"theta_{n}"]).max())
+            assert err < 1e-11, ("theta", n, err)
+    walls = th.wall_heat_rates(theta, st.u)
+    assert np.allclose([walls[s] for s in O.WALL_SIDES], fx["wall_heat"], rtol=1e-9, atol=1e-10)
+    assert np.allclose(th.theta_wall.double().cpu().numpy(), fx["wall_theta"], rtol=1e-11, atol=1e-13)
+
+
+@pytest.mark.parametrize("name", ["box", "tunnel_les"])
+def test_thermal_float32_matches_oracle(name):
+    print(f"[thermal gpu] oracle case {name} float32", flush=True)
+    fx = _fixture(name)
+    cfg, ops, g, th = _gpu_setup(name, fx, "float32")
+    st, theta = _state(fx, ops)
+    ccfg = cases.thermal_cfg(name, fx, "float32")
+    cops = su.make_operators3d(ccfg, device="cpu")
+    cg = su.build_geometry3d(ccfg.bodies, cops)
+    oth = O.Thermal(T.build_thermal(ccfg, cops, ccfg.bodies, cg.window), cops, ccfg.re)
+    z = torch.zeros((), dtype=torch.float32)
+    ost = O.State3D(*(torch.as_tensor(fx[k], dtype=torch.float32) for k in ("u0", "v0", "w0")),
+                    torch.zeros(cops.nz, cops.ny, cops.nx), z.clone(), z.clone())
+    otheta = torch.as_tensor(fx["theta0"], dtype=torch.float32)
+    for n in range(5):
+        out = S3.step3d_phys(st, g, ops, thermal=th, theta=theta)
+        st, theta = out.state, out.theta
+        ost, oc, otheta = O.step3d_phys(ost, cg, cops, oth, otheta)
+    # float32 tolerances: five steps of the same arithmetic in a different summation order
+    assert float(st.dt) == pytest.approx(float(ost.dt), rel=1e-5)
+    for f in ("u", "v", "w"):
+        assert float((getattr(st, f).cpu() - getattr(ost, f)).abs().max()) < 2e-5, f
+    assert float((theta.cpu() - otheta).abs().max()) < 2e-5
+    assert np.allclose(th.heat_rate.cpu().numpy(), oth.heat_rate.numpy(), rtol=1e-3, atol=1e-4)
+
+
+def _box(n=12, **kw):
+    base = dict(nx=n, ny=n, nz=n, h=1.0 / n, x0=0.0, y0=0.0, z0=0.0, re=20.0, domain="closed", device="cuda",
+                dtype="float64", dye=False, perturb=False)
+    base.update(kw)
+    return SolverConfig3D(**base)
+
+
+def test_solver_graphs_replay_eager_steps_bit_for_bit():
+    print("[thermal gpu] graphs vs eager", flush=True)
+    fx = _fixture("box")
+    runs = []
+    for graphs in (True, False):
+        cfg = cases.thermal_cfg("box", fx, "float32", device="cuda", use_cuda_graphs=graphs)
+        s = S3.FlowSolver3D(cfg)
+        s.set_velocity(fx["u0"], fx["v0"], fx["w0"])
+        s.set_temperature(fx["theta0"])
+        s.advance(23)
+        runs.append(s)
+    a, b = runs
+    assert torch.equal(a.theta, b.theta)
+    assert torch.equal(a.state.w, b.state.w)
+    ha, hb = a.physics_history(), b.physics_history()
+    assert len(ha["t"]) == 23 and np.array_equal(ha["heat_rate"], hb["heat_rate"])
+    assert np.array_equal(ha["theta_wall_1"], hb["theta_wall_1"])
+    q = a.heat_rates()
+    assert q[0]["heat_rate"] == pytest.approx(ha["heat_rate_0"][-1], rel=1e-6)
+    assert q[2]["heat_rate"] == 0.0 and math.isnan(q[2]["nusselt"])
+
+
+def test_solver_conduction_between_walls_is_exact():
+    n = 12
+    x = (np.arange(n) + 0.5) / n
+    cfg = _box(n, ny=4, nz=4, re=10.0, thermal=ThermalConfig3D(
+        prandtl=1.0, walls={"left": ("temperature", 0.5), "right": ("temperature", -0.5)}))
+    s = S3.FlowSolver3D(cfg)
+    s.set_temperature(np.broadcast_to(0.5 - x, (4, 4, n)).copy())
+    s.advance(20)
+    th = s.field("temperature").double().cpu().numpy()
+    assert np.abs(th - (0.5 - x)[None, None, :]).max() < 1e-13
+    w = s.wall_heat_rates()
+    assert w["left"] == pytest.approx((4 / n) ** 2, rel=1e-12)
+    assert float(s.field("speed").max()) < 1e-14
+
+
+def test_solver_heat_source_raises_the_stored_heat_at_its_power():
+    cfg = _box(thermal=ThermalConfig3D(prandtl=0.71, richardson=0.5),
+               heat_sources=[HeatSource3D(kind="sphere", center=(0.4, 0.5, 0.55), size=(0.2,), power=3.0),
+                             HeatSource3D(kind="box", center=(0.7, 0.3, 0.3), size=(0.2, 0.1, 0.3), power=-1.0)])
+    s = S3.FlowSolver3D(cfg)
+    s.advance(25)
+    stored = float(s.theta.double().sum()) * s.ops.h ** 3 * s.thermal.pe
+    assert stored / s.t == pytest.approx(2.0, rel=1e-10)
+
+
+def test_solver_stratified_fluid_stays_at_rest():
+    n = 10
+    cfg = _box(n, bc_zmin="noslip", bc_zmax="noslip", thermal=ThermalConfig3D(
+        prandtl=0.71, richardson=5.0, walls={"bottom": ("temperature", -0.5), "top": ("temperature", 0.5)}))
+    s = S3.FlowSolver3D(cfg)
+    z = (np.arange(n) + 0.5) / n - 0.5
+    s.set_temperature(np.broadcast_to(z[:, None, None], (n, n, n)).copy())
+    s.advance(60)
+    assert float(s.field("speed").max()) < 1e-12
+    assert np.abs(s.field("temperature").cpu().numpy()[:, 0, 0] - z).max() < 1e-12
+
+
+def test_live_reynolds_and_wall_values_keep_graphs():
+    fx = _fixture("box")
+    cfg = cases.thermal_cfg("box", fx, "float64", device="cuda")
+    s = S3.FlowSolver3D(cfg)
+    s.advance(10)
+    pe0 = s.thermal.pe
+    s.set_inflow(re=160.0)
+    assert s.thermal.pe == pytest.approx(2 * pe0)
+    assert float(s.thermal.scalars[0]) == pytest.approx(1.0 / (160.0 * 0.71), rel=1e-15)
+    s.set_thermal(walls={"bottom": ("heat_flux", 1.4), "left": ("temperature", 0.5)})
+    assert float(s.thermal.side_vals[4]) == pytest.approx(1.4)
+    s.advance(10)
+    assert bool(torch.isfinite(s.theta).all())
+    s.set_thermal(richardson=0.0)
+    s.advance(5)
+    assert s.thermal.ri == 0.0 and bool(torch.isfinite(s.theta).all())

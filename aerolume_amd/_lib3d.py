@@ -23,7 +23,8 @@ class Desc3D(ctypes.Structure):
                 ("reserved2", i32), ("thermal", i32), ("theta_side", i32 * 6), ("theta_q_k0", i32),
                 ("theta_q_k1", i32), ("theta_q_j0", i32), ("theta_q_j1", i32), ("theta_q_i0", i32),
                 ("theta_q_i1", i32), ("theta_nt", i32), ("theta_tbody", i32 * 8), ("theta_adia", i32),
-                ("theta_nskin", i32), ("richardson", f64), ("prandtl_turbulent", f64)]
+                ("theta_nskin", i32), ("richardson", f64), ("prandtl_turbulent", f64),
+                ("diagnostics", i32), ("reserved3", i32)]
 
 
 class Fields3D(ctypes.Structure):
@@ -47,7 +48,12 @@ class Params3D(ctypes.Structure):
 
 class Outputs3D(ctypes.Structure):
     _fields_ = [("coeffs", vp), ("hist", vp), ("hist_count", vp), ("hist_cap", ctypes.c_int64), ("heat_rate", vp),
-                ("theta_wall", vp), ("phist", vp), ("phist_walls", i32), ("reserved", i32)]
+                ("theta_wall", vp), ("phist", vp), ("phist_walls", i32), ("reserved", i32), ("diag", vp),
+                ("diag_count", vp), ("diag_cap", ctypes.c_int64)]
+
+
+#: columns of a diagnostics row (AERO3D_DIAG_COLS)
+DIAG_COLS = 28
 
 
 def bind(lib) -> None:

@@ -33,6 +33,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <limits>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -109,6 +110,7 @@ struct Dev3 {
   T ri, half_ri, ri_w[3], abs_ri, prt, kc, cadv, cfl, c6, h3;
   T *th_a, *th_b, *nut_c;
   double *pheat;             // [3][window tiles][kMaxT] heat increments of held-temperature bodies
+  double *pdiag;             // diagnostics partials
 };
 
 // Per-launch pointers (by value).
@@ -1337,6 +1339,134 @@ __device__ __forceinline__ T div_at(const Dev3<T> &d, const T *u, const T *v, co
   return (((u1 - ur[i]) + (vr[nx] - vr[0])) + (wr[pl] - wr[0])) * d.inv_h;
 }
 
+// ------------------------------------------------------------------ diagnostics
+// One diagnostics row of a step (ns3d/solver.py:747-783): per-block partials of the
+// continuity residual of the new state, of the last stage's projection right-hand side, and
+// of the change of u, v, w, p, theta over the step; then one block forms the row.
+constexpr int kDiagBlocks = 256;
+constexpr int kDV = 25;    // 0 sum div^2, 1 max|div|, 2 sum rhs^2, 3-5 max|u,v,w| old, 6-8 new,
+                           // 9 + 3f: sum d^2, max|d|, sum a^2 of field f (u, v, w, p, theta), 24 sum d theta
+__device__ __forceinline__ bool diag_is_max(int q) {
+  return q == 1 || (q >= 3 && q <= 8) || (q >= 9 && q <= 23 && (q - 9) % 3 == 1);
+}
+
+template <typename T>
+struct Diag3 {
+  const T *uo, *vo, *wo, *po, *to;     // state before the step
+  const T *un, *vn, *wn, *pn, *tn;     // after
+  const T *uh, *vh, *wh;               // last stage's u^ (its projection's right-hand side)
+  const T *t_out, *dt_out;
+  T *row_buf;
+  long long *count;
+  long long cap;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_diag(const Dev3<T> *__restrict__ dp, Diag3<T> g) {
+  const Dev3<T> &d = *dp;
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  const size_t nc = (size_t)nx * ny * nz;
+  const size_t stride = (size_t)gridDim.x * blockDim.x, t0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double v[kDV];
+#pragma unroll
+  for (int q = 0; q < kDV; ++q) v[q] = 0.0;
+  const T corr = d.outlet ? d.red->corr[2] : (T)0;
+  for (size_t e = t0; e < nc; e += stride) {
+    const int i = (int)(e % nx), j = (int)((e / nx) % ny), k = (int)(e / ((size_t)nx * ny));
+    const double dv = (double)div_at(d, g.un, g.vn, g.wn, k, j, i, (T)0);
+    v[0] += dv * dv;
+    v[1] = fmax(v[1], fabs(dv));
+    const double dr = (double)div_at(d, g.uh, g.vh, g.wh, k, j, i, corr);
+    v[2] += dr * dr;
+    const double a = (double)g.pn[e], dd = (double)(g.pn[e] - g.po[e]);
+    v[18] += dd * dd;
+    v[19] = fmax(v[19], fabs(dd));
+    v[20] += a * a;
+    if (d.theta) {
+      const double at = (double)g.tn[e], dt_ = (double)(g.tn[e] - g.to[e]);
+      v[21] += dt_ * dt_;
+      v[22] = fmax(v[22], fabs(dt_));
+      v[23] += at * at;
+      v[24] += dt_;
+    }
+  }
+  const size_t nf[3] = {(size_t)nz * ny * (nx + 1), (size_t)nz * (ny + 1) * nx, (size_t)(nz + 1) * ny * nx};
+  const T *fo[3] = {g.uo, g.vo, g.wo}, *fn[3] = {g.un, g.vn, g.wn};
+#pragma unroll
+  for (int f = 0; f < 3; ++f)
+    for (size_t e = t0; e < nf[f]; e += stride) {
+      const T an = fn[f][e], ao = fo[f][e];
+      const double dd = (double)(an - ao);
+      v[9 + 3 * f] += dd * dd;
+      v[10 + 3 * f] = fmax(v[10 + 3 * f], fabs(dd));
+      v[11 + 3 * f] += (double)an * (double)an;
+      v[3 + f] = fmax(v[3 + f], (double)fabs(ao));
+      v[6 + f] = fmax(v[6 + f], (double)fabs(an));
+    }
+  __shared__ double red[kDV * 4];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int q = 0; q < kDV; ++q) v[q] = diag_is_max(q) ? wave_max(v[q]) : wave_sum(v[q]);
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < kDV; ++q) red[wv * kDV + q] = v[q];
+  __syncthreads();
+  if (threadIdx.x < kDV) {
+    const int q = threadIdx.x;
+    double r = red[q];
+    for (int w2 = 1; w2 < 4; ++w2) r = diag_is_max(q) ? fmax(r, red[w2 * kDV + q]) : r + red[w2 * kDV + q];
+    d.pdiag[(size_t)blockIdx.x * kDV + q] = r;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64) k_diag_fin(const Dev3<T> *__restrict__ dp, Call3<T> c, Diag3<T> g) {
+  const Dev3<T> &d = *dp;
+  __shared__ double tot[kDV];
+  if (threadIdx.x < kDV) {
+    const int q = threadIdx.x;
+    double r = 0.0;
+    for (int b = 0; b < kDiagBlocks; ++b) {
+      const double x = d.pdiag[(size_t)b * kDV + q];
+      r = diag_is_max(q) ? fmax(r, x) : r + x;
+    }
+    tot[q] = r;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  const double nc = (double)nx * ny * nz;
+  const double numel[5] = {(double)nz * ny * (nx + 1), (double)nz * (ny + 1) * nx, (double)(nz + 1) * ny * nx, nc, nc};
+  const double tiny = (double)std::numeric_limits<T>::min();
+  const double h = (double)d.h, dV = h * h * h;
+  const T dt = *g.dt_out;
+  const T umax_old = ((T)tot[3] + (T)tot[4]) + (T)tot[5];
+  const T umax_new = ((T)tot[6] + (T)tot[7]) + (T)tot[8];
+  T *row = g.row_buf + (size_t)(*g.count % g.cap) * AERO3D_DIAG_COLS;
+  const T nan = (T)NAN;
+  row[0] = *g.t_out;
+  row[1] = dt;
+  row[2] = dt * umax_old / d.h;
+  row[3] = umax_new;
+  row[4] = (T)sqrt(tot[0] / nc);
+  row[5] = (T)tot[1];
+  row[6] = (T)sqrt(tot[0] / fmax(tot[2], tiny));
+  for (int f = 0; f < 5; ++f) {
+    T *o = row + 7 + 4 * f;
+    if (f == 4 && !d.theta) {
+      o[0] = o[1] = o[2] = o[3] = nan;
+      continue;
+    }
+    const double s2 = tot[9 + 3 * f], a2 = tot[11 + 3 * f];
+    o[0] = (T)sqrt(s2 * dV);
+    o[1] = (T)sqrt(s2 / numel[f]);
+    o[2] = (T)tot[10 + 3 * f];
+    o[3] = (T)sqrt(s2 / fmax(a2, tiny));
+  }
+  row[27] = d.theta ? (T)tot[24] * c.th_sc[4] / dt : nan;
+  *g.count = *g.count + 1;
+}
+
 // P1: rows (pairs) along x, forward. mode 0: f is the right-hand side; 1: divergence of
 // (u, v, w); 2: divergence with the outlet correction of stage `stage` (computed here).
 template <typename T>
@@ -1635,6 +1765,7 @@ struct aero3d_plan {
   void *dev = nullptr;
   void *uA, *vA, *wA, *uB, *vB, *wB, *F, *psi, *acc_u, *acc_v, *acc_w, *pmax, *pout, *pforce, *red, *dye_tmp;
   void *th_a, *th_b, *nut_c, *pheat;
+  void *old_u, *old_v, *old_w, *old_p, *old_t, *pdiag;
   void *tw[3], *cs[3];
   double *lam[3];
   FftPlan fp[3];
@@ -1804,6 +1935,7 @@ int upload3(aero3d_plan *p) {
     d.nut_c = (T *)p->nut_c;
     d.pheat = (double *)p->pheat;
   }
+  d.pdiag = (double *)p->pdiag;
   AERO_HIP_CHECK(hipMemcpy(p->dev, &d, sizeof(d), hipMemcpyHostToDevice));
   // the tile kernels take more than the default 64 KB of dynamic LDS
   int dev_id = 0, max_lds = 0;
@@ -1913,6 +2045,19 @@ int step_impl(aero3d_plan *p, hipStream_t st, const aero3d_fields *in, aero3d_fi
   Call3<T> c = make_call<T>(in, g, pr);
   const dim3 grid(p->nblk), blk(kTileThreads);
   const size_t lds = sizeof(Lds3<T>);
+  const bool diag = o && o->diag;
+  const size_t ncell = (size_t)s.nx * s.ny * s.nz;
+  if (diag) {
+    // the state before the step (the step may run in place)
+    const size_t n3[5] = {(size_t)s.nz * s.ny * (s.nx + 1), (size_t)s.nz * (s.ny + 1) * s.nx,
+                          (size_t)(s.nz + 1) * s.ny * s.nx, ncell, ncell};
+    void *dst[5] = {p->old_u, p->old_v, p->old_w, p->old_p, p->old_t};
+    const void *src[5] = {in->u, in->v, in->w, in->p, in->theta};
+    for (int f = 0; f < (s.thermal ? 5 : 4); ++f) {
+      const int nb = (int)std::min<size_t>(4096, (n3[f] + 255) / 256);
+      hipLaunchKernelGGL(k_copy3<T>, dim3(nb), dim3(256), 0, st, (T *)dst[f], (const T *)src[f], n3[f]);
+    }
+  }
   if (!s.fixed_dt && !chained) {
     hipLaunchKernelGGL(k_final<T>, grid, blk, lds, st, dev, c, 0);
     hipLaunchKernelGGL(k_dt<T>, dim3(1), dim3(kFinThreads), 0, st, dev, c);
@@ -1983,6 +2128,20 @@ int step_impl(aero3d_plan *p, hipStream_t st, const aero3d_fields *in, aero3d_fi
     }
   }
   hipLaunchKernelGGL(k_finalize<T>, dim3(1), dim3(kFinThreads), 0, st, dev, c, f);
+  if (diag) {
+    Diag3<T> dg;
+    dg.uo = (const T *)p->old_u; dg.vo = (const T *)p->old_v; dg.wo = (const T *)p->old_w;
+    dg.po = (const T *)p->old_p; dg.to = (const T *)p->old_t;
+    dg.un = (const T *)out->u; dg.vn = (const T *)out->v; dg.wn = (const T *)out->w;
+    dg.pn = (const T *)out->p; dg.tn = (const T *)out->theta;
+    dg.uh = ub[0]; dg.vh = vb[0]; dg.wh = wb[0];
+    dg.t_out = (const T *)out->t; dg.dt_out = (const T *)out->dt;
+    dg.row_buf = (T *)o->diag;
+    dg.count = (long long *)o->diag_count;
+    dg.cap = o->diag_cap;
+    hipLaunchKernelGGL(k_diag<T>, dim3(kDiagBlocks), dim3(256), 0, st, dev, dg);
+    hipLaunchKernelGGL(k_diag_fin<T>, dim3(1), dim3(64), 0, st, dev, c, dg);
+  }
   AERO_HIP_CHECK(hipGetLastError());
   return AERO_OK;
 }
@@ -2125,7 +2284,7 @@ int check3_desc(const aero3d_desc *d) {
 }
 
 int check3_step(const aero3d_plan *p, const aero3d_fields *in, const aero3d_fields *out, const aero3d_geom *g,
-                const aero3d_params *pr) {
+                const aero3d_params *pr, const aero3d_outputs *o) {
   if (!p || !in || !out || !pr) { set_error("null plan or argument"); return AERO_EINVAL; }
   if (!in->u || !in->v || !in->w || !in->t || !out->u || !out->v || !out->w || !out->p || !out->t || !out->dt) {
     set_error("missing field pointer");
@@ -2141,6 +2300,10 @@ int check3_step(const aero3d_plan *p, const aero3d_fields *in, const aero3d_fiel
   }
   if (!pr->nu || !pr->eta || !pr->u_inlet || (p->desc.fixed_dt && !pr->dt_fixed)) {
     set_error("missing parameter pointer");
+    return AERO_EINVAL;
+  }
+  if (o && o->diag && (!p->desc.diagnostics || !o->diag_count || o->diag_cap < 1 || !in->p)) {
+    set_error("diagnostics rows need a plan made with desc.diagnostics, a row counter and fields.p");
     return AERO_EINVAL;
   }
   const aero3d_desc &s = p->desc;
@@ -2223,6 +2386,10 @@ int aero3d_plan_create(const aero3d_desc *desc, aero3d_plan **out) {
       {&p->th_a, desc->thermal ? nc * es : 0}, {&p->th_b, desc->thermal ? nc * es : 0},
       {&p->nut_c, desc->thermal && desc->turbulence ? nc * es : 0},
       {&p->pheat, desc->thermal ? (size_t)3 * std::max(1, p->nwt) * kMaxT * 8 : 0},
+      {&p->old_u, desc->diagnostics ? nu_ * es : 0}, {&p->old_v, desc->diagnostics ? nv_ * es : 0},
+      {&p->old_w, desc->diagnostics ? nw_ * es : 0}, {&p->old_p, desc->diagnostics ? nc * es : 0},
+      {&p->old_t, desc->diagnostics && desc->thermal ? nc * es : 0},
+      {&p->pdiag, desc->diagnostics ? (size_t)kDiagBlocks * kDV * 8 : 0},
       {&p->tw[0], nx * 2 * es}, {&p->tw[1], ny * 2 * es}, {&p->tw[2], nz * 2 * es},
       {&p->cs[0], nx * 2 * es}, {&p->cs[1], ny * 2 * es}, {&p->cs[2], nz * 2 * es},
       {(void **)&p->lam[0], nx * 8}, {(void **)&p->lam[1], ny * 8}, {(void **)&p->lam[2], nz * 8}};
@@ -2252,7 +2419,7 @@ size_t aero3d_plan_workspace_bytes(const aero3d_plan *p) { return p ? p->bytes :
 
 int aero3d_step(aero3d_plan *p, void *stream, const aero3d_fields *in, aero3d_fields *out, const aero3d_geom *g,
                 const aero3d_params *pr, aero3d_outputs *o) {
-  int rc = check3_step(p, in, out, g, pr);
+  int rc = check3_step(p, in, out, g, pr, o);
   if (rc) return rc;
   hipStream_t st = (hipStream_t)stream;
   return p->dbl ? step_impl<double>(p, st, in, out, g, pr, o, false) : step_impl<float>(p, st, in, out, g, pr, o, false);
@@ -2260,7 +2427,7 @@ int aero3d_step(aero3d_plan *p, void *stream, const aero3d_fields *in, aero3d_fi
 
 int aero3d_step_chained(aero3d_plan *p, void *stream, const aero3d_fields *in, aero3d_fields *out,
                         const aero3d_geom *g, const aero3d_params *pr, aero3d_outputs *o) {
-  int rc = check3_step(p, in, out, g, pr);
+  int rc = check3_step(p, in, out, g, pr, o);
   if (rc) return rc;
   hipStream_t st = (hipStream_t)stream;
   return p->dbl ? step_impl<double>(p, st, in, out, g, pr, o, true) : step_impl<float>(p, st, in, out, g, pr, o, true);

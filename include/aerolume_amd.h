@@ -206,6 +206,8 @@ typedef struct {
   int32_t theta_nskin;      /* entries of the heat-flux skin list (geom.skin_*) */
   double richardson;        /* buoyancy Ri theta along +z (0: passive temperature) */
   double prandtl_turbulent;
+  int32_t diagnostics;      /* 1: the plan keeps scratch for outputs.diag rows */
+  int32_t reserved3;
 } aero3d_desc;
 
 /* heat-equation side conditions (ns3d/physics.py:589-606) */
@@ -263,7 +265,16 @@ typedef struct {
    * theta_wall (nbody) when phist_walls] at the history row */
   void *heat_rate, *theta_wall, *phist;
   int32_t phist_walls, reserved;
+  /* diagnostics (plans made with desc.diagnostics): when diag is set the step writes one row
+   * of AERO3D_DIAG_COLS values at (*diag_count mod diag_cap), then *diag_count += 1:
+   * t, dt, CFL, umax, div_rms, div_max, Poisson residual, then per field u, v, w, p, theta
+   * the change over the step (L2, RMS, max, relative L2), then the heat storage rate
+   * (NaN without heat transfer) (ns3d/solver.py:747-783) */
+  void *diag, *diag_count;
+  int64_t diag_cap;
 } aero3d_outputs;
+
+#define AERO3D_DIAG_COLS 28
 
 int aero3d_plan_create(const aero3d_desc *desc, aero3d_plan **plan);
 int aero3d_plan_destroy(aero3d_plan *plan);

@@ -888,9 +888,11 @@ def wall_heat_rates(th, theta, u, ops):
     return out
 
 
-def step3d_phys(state: State3D, geom, ops, th, theta, dt=None):
+def step3d_phys(state: State3D, geom, ops, th, theta, dt=None, out=None):
     """One SSP-RK3 step with heat transfer (core.py:765-883 with ``phys`` holding a ThermalOps):
-    returns (State3D, coefficients, new theta); ``th.heat_rate`` holds the bodies' heat rates."""
+    returns (State3D, coefficients, new theta); ``th.heat_rate`` holds the bodies' heat rates.
+    ``out`` (a dict) receives ``div_pre``, the squared norm of the last stage's projection
+    right-hand side (Physics3D.record_divergence, physics.py:1303-1306)."""
     u0, v0, w0 = state.u, state.v, state.w
     up, vp, wp = u0, v0, w0
     acc = None
@@ -930,6 +932,8 @@ def step3d_phys(state: State3D, geom, ops, th, theta, dt=None):
             acc = (acc[0], acc[1], acc[2] + share)
         tp_new = theta_stage(th, tp, t0, up, vp, wp, a, bdt, wgt, ops, nut=extra.get("nut") if extra else None)
         div = divergence3d(uh, vh, wh, ops.h)
+        if out is not None and s == 2:
+            out["div_pre"] = (div * div).sum()
         psi = poisson(ops).solve(div)
         gx, gy, gz = pressure_gradient(psi, ops)
         up, vp, wp = uh - gx, vh - gy, wh - gz
@@ -942,3 +946,41 @@ def step3d_phys(state: State3D, geom, ops, th, theta, dt=None):
         coeffs = force_coefficients(acc, geom, ops, dt)
     end_step(th, dt, ops)
     return State3D(up, vp, wp, p, state.t + dt, dt * torch.ones_like(state.t)), coeffs, tp
+
+
+# ---------------------------------------------------------------------------
+# diagnostics row (ns3d/solver.py:97-105, 747-783)
+# ---------------------------------------------------------------------------
+
+DIAG_FIELDS = ("u", "v", "w", "p", "theta")
+
+
+def diag_row(old: State3D, theta_old, new: State3D, theta_new, div_pre, ops, heat_scale=None):
+    """The diagnostics row of the step old -> new: t, dt, CFL, umax, continuity residual (RMS,
+    max), Poisson residual sqrt(|div new|^2 / |rhs|^2), per field the change (L2, RMS, max,
+    relative L2), heat storage rate (``heat_scale`` = sym h^3 Pe; NaN without heat)."""
+    h = ops.h
+    dV = h ** 3
+    div = divergence3d(new.u, new.v, new.w, h)
+    d2 = (div * div).sum()
+    n = div.numel()
+    umax_old = old.u.abs().amax() + old.v.abs().amax() + old.w.abs().amax()
+    umax = new.u.abs().amax() + new.v.abs().amax() + new.w.abs().amax()
+    tiny = torch.finfo(new.u.dtype).tiny
+    res = torch.sqrt(d2 / torch.clamp(div_pre, min=tiny))
+    vals = [new.t, new.dt, new.dt * umax_old / h, umax, torch.sqrt(d2 / n), div.abs().amax(), res]
+    pairs = [(new.u, old.u), (new.v, old.v), (new.w, old.w), (new.p, old.p)]
+    if theta_new is not None:
+        pairs.append((theta_new, theta_old))
+    for a, b in pairs:
+        d = a - b
+        s2 = (d * d).sum()
+        vals += [torch.sqrt(s2 * dV), torch.sqrt(s2 / d.numel()), d.abs().amax(),
+                 torch.sqrt(s2 / torch.clamp((a * a).sum(), min=tiny))]
+    nan = torch.full_like(d2, float("nan"))
+    if theta_new is None:
+        vals += [nan] * 4
+        vals.append(nan)
+    else:
+        vals.append((theta_new - theta_old).sum() * heat_scale / new.dt)
+    return torch.stack([v.reshape(()).to(new.u.dtype) for v in vals])

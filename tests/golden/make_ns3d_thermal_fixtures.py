@@ -12,6 +12,10 @@ kinds, the seeded start (u, v, w after the reference's projection, theta), u, v,
 theta after the first and the last step, dt, t, the six coefficients and the bodies' heat
 rates of every step, the wall heat rates and wall temperatures at the end.
 
+``ns3d_diag_box.npz``: the reference's ``FlowSolver3D`` with ``diagnostics=True`` on the "box"
+case from the same start (``load_state_dict``), advanced 7 then 13 steps: its diagnostics rows
+(one per 10-step block) and the state after 20 steps.
+
 Usage: ``python tests/golden/make_ns3d_thermal_fixtures.py`` (about a minute on 8 cores).
 """
 from __future__ import annotations
@@ -132,11 +136,44 @@ def run(name, spec):
     return out
 
 
+def diag_case():
+    from aerolume.solvers.ns3d.solver import DIAG_KEYS
+    from aerolume.solvers.ns3d.solver import FlowSolver3D as RSolver
+    spec = CASES["box"]
+    fx = dict(np.load(OUT / "ns3d_thermal_box.npz"))
+    bodies = [RMesh(fx[f"mesh{i}_v"], fx[f"mesh{i}_f"], name=f"b{i}", thermal=m, thermal_value=val)
+              for i, (_, _, _, m, val) in enumerate(spec["bodies"])]
+    cfg = RConfig(**spec["grid"], bodies=bodies, device="cpu", dtype="float64", dye=False, perturb=False,
+                  ref_area=0.3, length_ref=0.8, span_ref=0.6, moment_ref=(0.2, 0.0, 0.5),
+                  thermal=RThermal(**spec["thermal"]), heat_sources=[RHeat(**h) for h in spec["heaters"]],
+                  diagnostics=True, use_cuda_graphs=False, fused_kernels=False)
+    s = RSolver(cfg)
+    n = s.ops
+    z = torch.zeros((), dtype=torch.float64)
+    s.load_state_dict({"u": torch.as_tensor(fx["u0"]), "v": torch.as_tensor(fx["v0"]), "w": torch.as_tensor(fx["w0"]),
+                       "p": torch.zeros(n.nz, n.ny, n.nx, dtype=torch.float64), "t": z.clone(), "dt": z.clone(),
+                       "theta": torch.as_tensor(fx["theta0"]), "steps": 0})
+    with torch.no_grad():
+        s.advance(7)
+        s.advance(13)
+    d = s.diagnostics()
+    out = {k: np.asarray(d[k]) for k in DIAG_KEYS}
+    out["step"] = np.asarray(d["step"])
+    st = s.state
+    for f in ("u", "v", "w", "p"):
+        out[f"{f}_20"] = getattr(st, f).numpy().copy()
+    out["theta_20"] = s.theta.numpy().copy()
+    out["keys"] = np.array(DIAG_KEYS)
+    return out
+
+
 def main():
     torch.set_num_threads(8)
     for name, spec in CASES.items():
         np.savez_compressed(OUT / f"ns3d_thermal_{name}.npz", **run(name, spec))
         print("wrote", name, flush=True)
+    np.savez_compressed(OUT / "ns3d_diag_box.npz", **diag_case())
+    print("wrote diag_box", flush=True)
 
 
 if __name__ == "__main__":

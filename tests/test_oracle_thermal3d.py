@@ -300,3 +300,40 @@ def test_planning_helpers_match_the_step_rule():
     h, pe, S = 1 / 12, 80 * 0.71, 3.0
     ref = 1.0 / (S * (1.0 / (0.5 * h)) + (1.0 / pe) / (h * h / 6.0))
     assert T.thermal_dt_estimate(h, S, pe) == pytest.approx(ref, rel=1e-12)
+
+
+def _oracle_diag_run(fx_box, steps=20):
+    """The oracle over ``steps`` steps of the box case, diagnostics rows at every 10th step."""
+    cfg, ops, g, d = _setup("box", fx_box)
+    th = O.Thermal(d, ops, cfg.re)
+    z = torch.zeros((), dtype=torch.float64)
+    st = O.State3D(_t(fx_box["u0"]), _t(fx_box["v0"]), _t(fx_box["w0"]),
+                   torch.zeros(ops.nz, ops.ny, ops.nx, dtype=torch.float64), z.clone(), z.clone())
+    theta = _t(fx_box["theta0"])
+    rows = []
+    scale = ops.h ** 3 * th.pe
+    for n in range(1, steps + 1):
+        rec = {}
+        new, _, th_new = O.step3d_phys(st, g, ops, th, theta, out=rec)
+        if n % 10 == 0:
+            rows.append(O.diag_row(st, theta, new, th_new, rec["div_pre"], ops, scale).numpy())
+        st, theta = new, th_new
+    return np.stack(rows), st, theta
+
+
+def test_oracle_diagnostics_match_reference():
+    """The reference FlowSolver3D's diagnostics rows (steps 10 and 20 of a 7 + 13 step advance)
+    against the oracle's row formula on its own trajectory; the continuity and Poisson
+    residuals are round-off quantities and are compared by magnitude only."""
+    fx = _fixture("box")
+    ref = dict(np.load(GOLDEN / "ns3d_diag_box.npz"))
+    keys = [str(k) for k in ref["keys"]]
+    assert list(ref["step"]) == [10.0, 20.0]
+    rows, st, theta = _oracle_diag_run(fx)
+    for i, k in enumerate(keys):
+        mine, theirs = rows[:, i], ref[k]
+        if k in ("div_rms", "div_max", "poisson_residual"):
+            assert np.all(mine < 1e-12) and np.all(theirs < 1e-12), k
+        else:
+            assert np.allclose(mine, theirs, rtol=1e-9, atol=1e-13), (k, mine, theirs)
+    assert float(np.abs(theta.numpy() - ref["theta_20"]).max()) < 1e-12

@@ -180,3 +180,36 @@ def test_live_reynolds_and_wall_values_keep_graphs():
     s.set_thermal(richardson=0.0)
     s.advance(5)
     assert s.thermal.ri == 0.0 and bool(torch.isfinite(s.theta).all())
+
+
+def test_solver_diagnostics_match_reference_rows(tmp_path):
+    """Diagnostics rows of the HIP step (steps 10 and 20 of a 7 + 13 step advance) against the
+    reference FlowSolver3D's rows from the same start (float64); round-off residuals by
+    magnitude; graph and eager runs give identical rows."""
+    print("[thermal gpu] diagnostics rows", flush=True)
+    fx = _fixture("box")
+    ref = dict(np.load(GOLDEN / "ns3d_diag_box.npz"))
+    keys = [str(k) for k in ref["keys"]]
+    assert tuple(keys) == S3.DIAG_KEYS
+    rows = []
+    for graphs in (True, False):
+        cfg = cases.thermal_cfg("box", fx, "float64", device="cuda", diagnostics=True, use_cuda_graphs=graphs)
+        s = S3.FlowSolver3D(cfg)
+        s.set_velocity(fx["u0"], fx["v0"], fx["w0"])
+        s.set_temperature(fx["theta0"])
+        s.advance(7)
+        s.advance(13)
+        d = s.diagnostics()
+        assert list(d["step"]) == [10.0, 20.0]
+        rows.append(np.stack([d[k] for k in keys], axis=1))
+        for k in keys:
+            if k in ("div_rms", "div_max", "poisson_residual"):
+                assert np.all(d[k] < 1e-12) and np.all(ref[k] < 1e-12), k
+            else:
+                assert np.allclose(d[k], ref[k], rtol=1e-9, atol=1e-13), (k, d[k], ref[k])
+        assert float(np.abs(s.theta.double().cpu().numpy() - ref["theta_20"]).max()) < 1e-11
+    assert np.array_equal(rows[0], rows[1])
+    path = s.diagnostics_csv(tmp_path / "diag.csv")
+    text = open(path).read().splitlines()
+    assert text[0].startswith("step,t,dt,cfl") and len(text) == 3
+    assert s.latest_diagnostics()["step"] == 20.0

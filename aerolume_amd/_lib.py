@@ -35,7 +35,8 @@ class Desc2D(ctypes.Structure):
                 ("wall_modes", i32 * 4), ("n_temp", i32), ("has_heat", i32), ("insulated", i32),
                 ("dye_mode", i32), ("n_dye", i32), ("n_groups", i32), ("group_channel", i32 * MAX_GROUPS),
                 ("h", f64), ("ground_speed", f64), ("cfl", f64), ("force_ref", f64), ("les_c", f64),
-                ("gravity_x", f64), ("gravity_y", f64), ("prt", f64), ("diff_coef", f64)]
+                ("gravity_x", f64), ("gravity_y", f64), ("prt", f64), ("diff_coef", f64),
+                ("tile_rows", i32), ("reserved", i32)]
 
 
 class Fields2D(ctypes.Structure):
@@ -125,6 +126,8 @@ def _bind_ensemble(lib) -> None:
                                            ctypes.POINTER(Geom2D), ctypes.POINTER(Params2D),
                                            ctypes.POINTER(Outputs2D), ctypes.POINTER(P)]
     lib.aero2d_ensemble_step.argtypes = [P, P, ctypes.c_int32]
+    lib.aero2d_ensemble_tune.argtypes = [P, P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
     lib.aero2d_ensemble_destroy.argtypes = [P]
-    for name in ("aero2d_ensemble_create", "aero2d_ensemble_step", "aero2d_ensemble_destroy"):
+    for name in ("aero2d_ensemble_create", "aero2d_ensemble_step", "aero2d_ensemble_tune",
+                 "aero2d_ensemble_destroy"):
         getattr(lib, name).restype = ctypes.c_int

@@ -104,18 +104,26 @@ __device__ __forceinline__ void dft5(cplx<T> *v) {
   v[3] = {t2.x - u2.y, t2.y + u2.x};
 }
 
+// log2 of a power of two, -1 otherwise (block-uniform stage constants: the index split of a
+// butterfly then costs a shift and a mask instead of an integer division)
+__device__ __forceinline__ int log2_or_neg(int v) { return (v & (v - 1)) == 0 ? __builtin_ctz((unsigned)v) : -1; }
+
+// a / n for a >= 0: a shift when n is a power of two (lg = log2_or_neg(n) >= 0)
+__device__ __forceinline__ int fdiv(int a, int n, int lg) { return lg >= 0 ? (a >> lg) : a / n; }
+
 template <typename T, int R>
 __device__ __forceinline__ void stockham_stage(const cplx<T> *__restrict__ src, cplx<T> *__restrict__ dst,
                                                int n, int p, const cplx<T> *__restrict__ tw, int nslot,
                                                int slot_stride, int tid, int nthr) {
   const int nr = n / R;
   const int stride = n / (p * R);
+  const int lnr = log2_or_neg(nr), lp = log2_or_neg(p);
   for (int w = tid; w < nslot * nr; w += nthr) {
-    const int f = w / nr;
+    const int f = lnr >= 0 ? (w >> lnr) : w / nr;
     const int k = w - f * nr;
     const cplx<T> *s = src + f * slot_stride;
     cplx<T> *d = dst + f * slot_stride;
-    const int i = k % p;
+    const int i = lp >= 0 ? (k & (p - 1)) : k % p;
     cplx<T> v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) v[r] = s[k + r * nr];

@@ -27,6 +27,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/aerolume_amd.h"
@@ -41,9 +42,15 @@ namespace ns2d {
 
 constexpr int kFftThreads = 256;
 constexpr int kMaxPer = 16;                // row elements per FFT thread (n <= 4096)
-constexpr int TX = 64, TY = 8;              // cells per tile (block of 64 x 4 threads, 2 rows each)
-constexpr int HW = TX + 4, HH = TY + 4;     // halo-2 tiles of u, v, scalar
-constexpr int NW = TX + 2, NH = TY + 2;     // eddy-viscosity tile (halo 1)
+constexpr int TX = 64;                      // tile width: one wave across (blocks of 64 x 4 threads)
+constexpr int HW = TX + 4, NW = TX + 2;     // halo-2 rows of u, v, scalar; eddy-viscosity rows (halo 1)
+constexpr int kDefaultRows = 8;             // tile height unless a plan or an ensemble picks another
+// A tile of R rows (R = 4, 8, 16; each thread owns R / 4 rows): halo-2 tiles of R + 4 rows,
+// eddy-viscosity tile of R + 2 rows, tile elements per thread of the loaders.
+template <int R>
+struct TR {
+  static constexpr int HH = R + 4, NH = R + 2, NE = (HH * HW + 255) / 256, RPT = R / 4;
+};
 constexpr int kMaxWin = 16;                 // force/heat partial slots per block (4 per body + temps)
 constexpr double kSsp[3][2] = {{0.0, 1.0}, {0.75, 0.25}, {1.0 / 3.0, 2.0 / 3.0}};
 constexpr double kForceW[3] = {1.0 / 6.0, 2.0 / 3.0, 1.0};
@@ -60,6 +67,7 @@ struct Red {
 template <typename T>
 struct Dev {
   int nx, ny, nbx, nby;
+  int ty;              // tile rows of the tile kernels (pmax / pforce block index = by * nbx + bx)
   int periodic, closed, upwind, turb, les;
   int bcb, bct, bcl, bcr;
   int j0, j1, i0, i1, wx, wy, win;
@@ -83,6 +91,7 @@ struct Dev {
   Red<T> *red;
   const cplx<T> *tw_x, *tw_y, *cs_x, *cs_y;
   const double *lam_x, *lam_y;
+  const T *inv_lam;   // (nx, ny), y fastest: 1 / (lam_x + lam_y) formed in float64, 0 at the zero mode
   FftPlan fx, fy;
   int dct_x, dct_y;   // 1: Neumann (DCT-II), 0: periodic (DHT)
 };
@@ -97,12 +106,12 @@ struct Stage {
   T *uh, *vh, *thh;          // stage output (before projection)
 };
 
-template <typename T>
+template <typename T, int R>
 struct Lds {
-  T u[HH][HW];
-  T v[HH][HW];
-  T c[HH][HW];
-  T nu[NH][NW];
+  T u[R + 4][HW];
+  T v[R + 4][HW];
+  T c[R + 4][HW];
+  T nu[R + 2][NW];
   double red[4 * kMaxWin];
 };
 
@@ -156,7 +165,6 @@ __device__ __forceinline__ T v_in(const Dev<T> &d, const T *__restrict__ v, cons
 // (pure integer work), then issue all loads at once, then apply the transforms: the loads
 // of a thread are independent, so the block waits for one memory latency, not one per
 // element. The transforms reproduce the ghost rules of core.py:287-325 and scalar.py:42-60.
-constexpr int NE = (HH * HW + 255) / 256;   // tile elements per thread
 
 // flag bits: 1 gradient correction, 2 outlet correction, 4 negate (mirror), 8/16/24 wall
 // kind slip/noslip/moving (u rows outside the domain)
@@ -256,23 +264,23 @@ __device__ __forceinline__ T finish_v(const Dev<T> &d, T x, T q1, T q0, int flag
 }
 
 // U, V tiles of the stage-input velocity around the tile at (tj0, ti0).
-template <typename T>
-__device__ void load_uv(const Dev<T> &d, Lds<T> &L, int tj0, int ti0, const T *u, const T *v, const T *psi, T corr) {
+template <typename T, int R>
+__device__ void load_uv(const Dev<T> &d, Lds<T, R> &L, int tj0, int ti0, const T *u, const T *v, const T *psi, T corr) {
   const int tid = threadIdx.x + threadIdx.y * blockDim.x;
   const bool hp = psi != nullptr;
-  int ui[NE], u1[NE], u0[NE], uf[NE], vi[NE], v1[NE], v0[NE], vf[NE], rr[NE];
+  int ui[TR<R>::NE], u1[TR<R>::NE], u0[TR<R>::NE], uf[TR<R>::NE], vi[TR<R>::NE], v1[TR<R>::NE], v0[TR<R>::NE], vf[TR<R>::NE], rr[TR<R>::NE];
 #pragma unroll
-  for (int q = 0; q < NE; ++q) {
+  for (int q = 0; q < TR<R>::NE; ++q) {
     const int e = tid + 256 * q;
-    const int ee = e < HH * HW ? e : 0;
+    const int ee = e < TR<R>::HH * HW ? e : 0;
     const int r = ee / HW, c = ee - r * HW;
     rr[q] = r;
     map_u(d, tj0 - 2 + r, ti0 - 2 + c, hp, ui[q], u1[q], u0[q], uf[q]);
     map_v(d, tj0 - 2 + r, ti0 - 2 + c, hp, vi[q], v1[q], v0[q], vf[q]);
   }
-  T xu[NE], xv[NE], a1[NE], a0[NE], b1[NE], b0[NE];
+  T xu[TR<R>::NE], xv[TR<R>::NE], a1[TR<R>::NE], a0[TR<R>::NE], b1[TR<R>::NE], b0[TR<R>::NE];
 #pragma unroll
-  for (int q = 0; q < NE; ++q) {
+  for (int q = 0; q < TR<R>::NE; ++q) {
     xu[q] = u[ui[q]];
     xv[q] = v[vi[q]];
     if (hp) {
@@ -285,9 +293,9 @@ __device__ void load_uv(const Dev<T> &d, Lds<T> &L, int tj0, int ti0, const T *u
     }
   }
 #pragma unroll
-  for (int q = 0; q < NE; ++q) {
+  for (int q = 0; q < TR<R>::NE; ++q) {
     const int e = tid + 256 * q;
-    if (e < HH * HW) {
+    if (e < TR<R>::HH * HW) {
       const int r = rr[q], c = e - r * HW;
       L.u[r][c] = finish_u(d, xu[q], a1[q], a0[q], corr, uf[q], tj0 - 2 + r < 0);
       L.v[r][c] = finish_v(d, xv[q], b1[q], b0[q], vf[q]);
@@ -297,17 +305,17 @@ __device__ void load_uv(const Dev<T> &d, Lds<T> &L, int tj0, int ti0, const T *u
 // scalar tile with the TVD ghost rules of scalar.py:42-60: inflow (or replicate) on the left,
 // replicate elsewhere, wrap when periodic; use_inflow 2 is the smoke comb's rule (inflow also
 // in a closed box, core.py:718-723)
-template <typename T>
-__device__ void load_c(const Dev<T> &d, Lds<T> &L, int tj0, int ti0, const T *c, const T *inflow_col, T inflow0,
+template <typename T, int R>
+__device__ void load_c(const Dev<T> &d, Lds<T, R> &L, int tj0, int ti0, const T *c, const T *inflow_col, T inflow0,
                        int use_inflow) {
   const int tid = threadIdx.x + threadIdx.y * blockDim.x;
   const int nx = d.nx, ny = d.ny;
-  int ci[NE], rr[NE], jr[NE];
-  bool inf[NE];
+  int ci[TR<R>::NE], rr[TR<R>::NE], jr[TR<R>::NE];
+  bool inf[TR<R>::NE];
 #pragma unroll
-  for (int q = 0; q < NE; ++q) {
+  for (int q = 0; q < TR<R>::NE; ++q) {
     const int e = tid + 256 * q;
-    const int ee = e < HH * HW ? e : 0;
+    const int ee = e < TR<R>::HH * HW ? e : 0;
     const int r = ee / HW, cc = ee - r * HW;
     const int j = tj0 - 2 + r, i = ti0 - 2 + cc;
     rr[q] = r;
@@ -320,16 +328,16 @@ __device__ void load_c(const Dev<T> &d, Lds<T> &L, int tj0, int ti0, const T *c,
       inf[q] = i < 0 && (use_inflow == 2 || (use_inflow == 1 && !d.closed));
     }
   }
-  T x[NE], fl[NE];
+  T x[TR<R>::NE], fl[TR<R>::NE];
 #pragma unroll
-  for (int q = 0; q < NE; ++q) {
+  for (int q = 0; q < TR<R>::NE; ++q) {
     x[q] = c[ci[q]];
     fl[q] = inflow_col ? inflow_col[jr[q]] : inflow0;
   }
 #pragma unroll
-  for (int q = 0; q < NE; ++q) {
+  for (int q = 0; q < TR<R>::NE; ++q) {
     const int e = tid + 256 * q;
-    if (e < HH * HW) {
+    if (e < TR<R>::HH * HW) {
       const int r = rr[q], cc = e - r * HW;
       L.c[r][cc] = inf[q] ? fl[q] : x[q];
     }
@@ -364,11 +372,11 @@ __device__ __forceinline__ T eddy(const Dev<T> &d, T g11, T g12, T g21, T g22) {
 }
 
 // nu_t on the tile's cells and their ring (core.py:421-503), switched off in solids.
-template <typename T>
-__device__ void tile_nut(const Dev<T> &d, Lds<T> &L, int tj0, int ti0) {
+template <typename T, int R>
+__device__ void tile_nut(const Dev<T> &d, Lds<T, R> &L, int tj0, int ti0) {
   const int tid = threadIdx.x + threadIdx.y * blockDim.x;
   const T ih = d.inv_h;
-  for (int e = tid; e < NH * NW; e += 256) {
+  for (int e = tid; e < TR<R>::NH * NW; e += 256) {
     const int r = e / NW, c = e - r * NW;
     const int j = tj0 - 1 + r, cc = ti0 - 1 + c;   // cell
     const int ur = r + 1, uc = c + 1;               // u/v tile index of (j, cc)
@@ -415,15 +423,15 @@ __device__ __forceinline__ T van_leer(T a, T b) {
 }
 
 // MUSCL / van Leer flux at the x-face between tile cells (r, c-1) and (r, c) (scalar.py:63-83)
-template <typename T>
-__device__ __forceinline__ T tvd_x(const Lds<T> &L, int r, int c, T uf) {
+template <typename T, int R>
+__device__ __forceinline__ T tvd_x(const Lds<T, R> &L, int r, int c, T uf) {
   const T cm2 = L.c[r][c - 2], cm1 = L.c[r][c - 1], c0 = L.c[r][c], c1 = L.c[r][c + 1];
   const T left = cm1 + (T)0.5 * van_leer(cm1 - cm2, c0 - cm1);
   const T right = c0 - (T)0.5 * van_leer(c0 - cm1, c1 - c0);
   return uf * (uf > (T)0 ? left : right);
 }
-template <typename T>
-__device__ __forceinline__ T tvd_y(const Lds<T> &L, int r, int c, T vf) {
+template <typename T, int R>
+__device__ __forceinline__ T tvd_y(const Lds<T, R> &L, int r, int c, T vf) {
   const T cm2 = L.c[r - 2][c], cm1 = L.c[r - 1][c], c0 = L.c[r][c], c1 = L.c[r + 1][c];
   const T low = cm1 + (T)0.5 * van_leer(cm1 - cm2, c0 - cm1);
   const T high = c0 - (T)0.5 * van_leer(c0 - cm1, c1 - c0);
@@ -431,8 +439,8 @@ __device__ __forceinline__ T tvd_y(const Lds<T> &L, int r, int c, T vf) {
 }
 
 // temperature diffusivity at x-face f of row j / y-face f of column i (physics.py:713-724)
-template <typename T>
-__device__ __forceinline__ T kappa_x(const Dev<T> &d, const Lds<T> &L, int tj0, int ti0, T kap, int j, int f) {
+template <typename T, int R>
+__device__ __forceinline__ T kappa_x(const Dev<T> &d, const Lds<T, R> &L, int tj0, int ti0, T kap, int j, int f) {
   T k = kap;
   if (d.les) {
     const T ip = (T)(1.0 / d.prt);
@@ -453,8 +461,8 @@ __device__ __forceinline__ T kappa_x(const Dev<T> &d, const Lds<T> &L, int tj0, 
     k = k * d.ins_u[(j - d.j0) * (d.wx + 1) + (f - d.i0)];
   return k;
 }
-template <typename T>
-__device__ __forceinline__ T kappa_y(const Dev<T> &d, const Lds<T> &L, int tj0, int ti0, T kap, int f, int i) {
+template <typename T, int R>
+__device__ __forceinline__ T kappa_y(const Dev<T> &d, const Lds<T, R> &L, int tj0, int ti0, T kap, int f, int i) {
   T k = kap;
   if (d.les) {
     const T ip = (T)(1.0 / d.prt);
@@ -549,10 +557,10 @@ __device__ T compute_dt(const Dev<T> &d, const DtIn<T> &in, T umax, T vmax, T th
 
 // ================================================================== kernels
 // The explicit stage on one 64x8 tile (core.py:855-935 per face / cell).
-template <typename T>
+template <typename T, int R>
 __device__ __forceinline__ void stage_body(const Dev<T> &d, const Stage<T> &s, int bx, int by) {
-  __shared__ Lds<T> L;
-  const int tj0 = by * TY, ti0 = bx * TX;
+  __shared__ Lds<T, R> L;
+  const int tj0 = by * R, ti0 = bx * TX;
   const int nx = d.nx, ny = d.ny;
   const T corr = s.psi_p ? d.red->corr : (T)0;
   load_uv(d, L, tj0, ti0, s.up, s.vp, s.psi_p, corr);
@@ -627,9 +635,11 @@ __device__ __forceinline__ void stage_body(const Dev<T> &d, const Stage<T> &s, i
     return (T)0.5 * (C(f, i) + C(f - 1, i));
   };
 
+  // u faces, v faces and temperature cells in separate passes over the tile's two rows per
+  // thread: each pass keeps only its own pointers and scalars live (fewer registers spilled)
   const int i = ti0 + threadIdx.x;
 #pragma unroll 1
-  for (int k = 0; k < 2; ++k) {
+  for (int k = 0; k < TR<R>::RPT; ++k) {
     const int j = tj0 + threadIdx.y + 4 * k;
     if (j >= ny || i >= nx) continue;
     // ---------------- u face (j, i) [+ (j, nx) by the last column]
@@ -674,6 +684,11 @@ __device__ __forceinline__ void stage_body(const Dev<T> &d, const Stage<T> &s, i
         s.uh[on] = uo;
       }
     }
+  }
+#pragma unroll 1
+  for (int k = 0; k < TR<R>::RPT; ++k) {
+    const int j = tj0 + threadIdx.y + 4 * k;
+    if (j >= ny || i >= nx) continue;
     // ---------------- v face (j, i) [+ (ny, i) by the last row]
     {
       const bool bnd = !d.periodic && j == 0;
@@ -710,6 +725,11 @@ __device__ __forceinline__ void stage_body(const Dev<T> &d, const Stage<T> &s, i
         s.vh[on] = vo;
       }
     }
+  }
+#pragma unroll 1
+  for (int k = 0; k < TR<R>::RPT; ++k) {
+    const int j = tj0 + threadIdx.y + 4 * k;
+    if (j >= ny || i >= nx) continue;
     // ---------------- temperature cell (j, i) (physics.py:696-757)
     if (d.thermal) {
       const int r = j - tj0 + 2, c = i - ti0 + 2;
@@ -747,19 +767,19 @@ __device__ __forceinline__ void stage_body(const Dev<T> &d, const Stage<T> &s, i
     }
   }
 }
-template <typename T>
+template <typename T, int R>
 __global__ void __launch_bounds__(256) k_stage(Dev<T> d, Stage<T> s) {
-  stage_body<T>(d, s, blockIdx.x, blockIdx.y);
+  stage_body<T, R>(d, s, blockIdx.x, blockIdx.y);
 }
 
 // Per-block maxima of a velocity tile (+ theta, nu_t) and sum theta: the dt inputs.
 // Each tile also covers the ghost ring of nu_t cells at the domain edge (core.py:503).
-template <typename T>
-__device__ void tile_maxima(const Dev<T> &d, Lds<T> &L, int tj0, int ti0, const T *th) {
+template <typename T, int R>
+__device__ void tile_maxima(const Dev<T> &d, Lds<T, R> &L, int tj0, int ti0, const T *th) {
   const int nx = d.nx, ny = d.ny;
   double mu = 0, mv = 0, mt = 0, mn = 0, st = 0;
   const int i = ti0 + threadIdx.x;
-  for (int k = 0; k < 2; ++k) {
+  for (int k = 0; k < TR<R>::RPT; ++k) {
     const int j = tj0 + threadIdx.y + 4 * k;
     if (j >= ny || i >= nx) continue;
     const int r = j - tj0 + 2, c = i - ti0 + 2;
@@ -775,7 +795,7 @@ __device__ void tile_maxima(const Dev<T> &d, Lds<T> &L, int tj0, int ti0, const 
   }
   const int tid = threadIdx.x + threadIdx.y * blockDim.x;
   if (d.les) {
-    for (int e = tid; e < NH * NW; e += 256) {
+    for (int e = tid; e < TR<R>::NH * NW; e += 256) {
       const int r = e / NW, c = e - r * NW;
       const int j = tj0 - 1 + r, cc = ti0 - 1 + c;
       if (j < -1 || j > ny || cc < -1 || cc > nx) continue;
@@ -798,7 +818,7 @@ __device__ void tile_maxima(const Dev<T> &d, Lds<T> &L, int tj0, int ti0, const 
   }
   __syncthreads();
   if (tid == 0) {
-    double *p = d.pmax + 5 * ((tj0 / TY) * d.nbx + ti0 / TX);
+    double *p = d.pmax + 5 * ((tj0 / R) * d.nbx + ti0 / TX);
     for (int q = 0; q < 4; ++q)
       p[q] = fmax(fmax(L.red[q], L.red[5 + q]), fmax(L.red[10 + q], L.red[15 + q]));
     p[4] = ((L.red[4] + L.red[9]) + L.red[14]) + L.red[19];
@@ -807,10 +827,10 @@ __device__ void tile_maxima(const Dev<T> &d, Lds<T> &L, int tj0, int ti0, const 
 }
 
 // dt inputs of the step-start state (non-chained steps).
-template <typename T>
+template <typename T, int R>
 __device__ __forceinline__ void pre_body(const Dev<T> &d, int bx, int by, const T *u0, const T *v0, const T *th0) {
-  __shared__ Lds<T> L;
-  const int tj0 = by * TY, ti0 = bx * TX;
+  __shared__ Lds<T, R> L;
+  const int tj0 = by * R, ti0 = bx * TX;
   load_uv(d, L, tj0, ti0, u0, v0, (const T *)nullptr, (T)0);
   __syncthreads();
   if (d.les) {
@@ -819,9 +839,9 @@ __device__ __forceinline__ void pre_body(const Dev<T> &d, int bx, int by, const 
   }
   tile_maxima(d, L, tj0, ti0, th0);
 }
-template <typename T>
+template <typename T, int R>
 __global__ void __launch_bounds__(256) k_pre(Dev<T> d, const T *u0, const T *v0, const T *th0) {
-  pre_body<T>(d, blockIdx.x, blockIdx.y, u0, v0, th0);
+  pre_body<T, R>(d, blockIdx.x, blockIdx.y, u0, v0, th0);
 }
 
 // Reduce the per-block dt inputs in a fixed order (one block; every thread gets out[]).
@@ -999,21 +1019,20 @@ __device__ __forceinline__ void poisB_body(const Dev<T> &d, int bx, int nslot) {
   const int c0 = 2 * nslot * bx;
   load_twiddles(TW, d.tw_y, ny);
   cplx<T> csr[kMaxPer];
-  double lyr[kMaxPer], lxa[kMaxPer], lxb[kMaxPer];
+  T iar[kMaxPer], ibr[kMaxPer];
 #pragma unroll
   for (int r = 0; r < kMaxPer; ++r) {
     const int w = tid + nthr * r;
     if (w < nslot * ny) {
-      const int f = w / ny, n = w - f * ny;
+      const int f = fdiv(w, ny, log2_or_neg(ny)), n = w - f * ny;
       const int ca = c0 + 2 * f, cb = ca + 1;
       const int q = d.dct_y ? makhoul_src(n, ny) : n;
       const T xa = ca < nx ? d.F[q * nx + ca] : (T)0;
       const T xb = cb < nx ? d.F[q * nx + cb] : (T)0;
       A[f * ny + n] = {xa, xb};
       csr[r] = d.cs_y[n];
-      lyr[r] = d.lam_y[n];
-      lxa[r] = d.lam_x[ca < nx ? ca : 0];
-      lxb[r] = d.lam_x[cb < nx ? cb : 0];
+      iar[r] = ca < nx ? d.inv_lam[(size_t)ca * ny + n] : (T)0;
+      ibr[r] = cb < nx ? d.inv_lam[(size_t)cb * ny + n] : (T)0;
     }
   }
   __syncthreads();
@@ -1022,7 +1041,7 @@ __device__ __forceinline__ void poisB_body(const Dev<T> &d, int bx, int nslot) {
   for (int rr = 0; rr < kMaxPer; ++rr) {
     const int w = tid + nthr * rr;
     if (w >= nslot * ny) continue;
-    const int f = w / ny, k = w - f * ny;
+    const int f = fdiv(w, ny, log2_or_neg(ny)), k = w - f * ny;
     const int ca = c0 + 2 * f, cb = ca + 1;
     cplx<T> va, vb;
     split_pair(R + f * ny, ny, k, va, vb);
@@ -1035,13 +1054,8 @@ __device__ __forceinline__ void poisB_body(const Dev<T> &d, int bx, int nslot) {
       xa = va.x - va.y;
       xb = vb.x - vb.y;
     }
-    const double ly = lyr[rr];
-    const double la = lxa[rr] + ly;
-    const double lb = lxb[rr] + ly;
-    const double ia = (la != 0.0 && !(ca == 0 && k == 0)) ? 1.0 / la : 0.0;
-    const double ib = (lb != 0.0) ? 1.0 / lb : 0.0;
-    X[(2 * f) * ny + k] = xa * (T)ia;
-    X[(2 * f + 1) * ny + k] = xb * (T)ib;
+    X[(2 * f) * ny + k] = xa * iar[rr];
+    X[(2 * f + 1) * ny + k] = xb * ibr[rr];
   }
   __syncthreads();
   cplx<T> *Z = (R == A) ? B : A;
@@ -1049,14 +1063,14 @@ __device__ __forceinline__ void poisB_body(const Dev<T> &d, int bx, int nslot) {
   for (int rr = 0; rr < kMaxPer; ++rr) {
     const int w = tid + nthr * rr;
     if (w >= nslot * ny) continue;
-    const int f = w / ny, k = w - f * ny;
+    const int f = fdiv(w, ny, log2_or_neg(ny)), k = w - f * ny;
     pack_inverse(X + (2 * f) * ny, X + (2 * f + 1) * ny, Z + f * ny, ny, d.dct_y, csr[rr], k);
   }
   __syncthreads();
   cplx<T> *R2 = fft_lds(Z, (Z == A) ? B : A, d.fy, TW, nslot, ny);
   const T inv_n = (T)(1.0 / (double)ny);
   for (int w = tid; w < nslot * ny; w += nthr) {
-    const int f = w / ny, m = w - f * ny;
+    const int f = fdiv(w, ny, log2_or_neg(ny)), m = w - f * ny;
     const int ca = c0 + 2 * f, cb = ca + 1;
     T va, vb;
     if (d.dct_y) {
@@ -1075,7 +1089,11 @@ __device__ __forceinline__ void poisB_body(const Dev<T> &d, int bx, int nslot) {
 }
 template <typename T>
 __global__ void __launch_bounds__(kFftThreads) k_poisB(Dev<T> d, int nslot) {
-  poisB_body<T>(d, blockIdx.x, nslot);
+  // column groups dealt to the 8 XCDs in contiguous runs: the groups that share cache lines
+  // of a row (nslot pairs of columns each) read them through one L2
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int g = (nb % 8 == 0) ? (b & 7) * (nb >> 3) + (b >> 3) : b;
+  poisB_body<T>(d, g, nslot);
 }
 
 // Pass C: inverse along x of rows (2b, 2b+1) -> psi; accumulate the step-averaged pressure
@@ -1190,16 +1208,16 @@ __global__ void __launch_bounds__(256) k_correct(Dev<T> d, T *__restrict__ u, T 
 
 // End of step on one tile: write u = u^ - grad psi, smoke transport with it (physics.py:
 // 805-830 / core.py:709-771), next-step dt inputs and the window force / heat partials.
-template <typename T>
+template <typename T, int R>
 __device__ __forceinline__ void final_body(const Dev<T> &d, int bx, int by, const T *uh, const T *vh, T *u_out, T *v_out, const T *th_new, const T *cin, T *cout, const T *t_in) {
-  __shared__ Lds<T> L;
-  const int tj0 = by * TY, ti0 = bx * TX;
+  __shared__ Lds<T, R> L;
+  const int tj0 = by * R, ti0 = bx * TX;
   const int nx = d.nx, ny = d.ny;
   const T corr = d.red->corr;
   load_uv(d, L, tj0, ti0, uh, vh, (const T *)d.psi, corr);
   __syncthreads();
   const int i = ti0 + threadIdx.x;
-  for (int k = 0; k < 2; ++k) {
+  for (int k = 0; k < TR<R>::RPT; ++k) {
     const int j = tj0 + threadIdx.y + 4 * k;
     if (j >= ny || i >= nx) continue;
     const int r = j - tj0 + 2, c = i - ti0 + 2;
@@ -1215,7 +1233,7 @@ __device__ __forceinline__ void final_body(const Dev<T> &d, int bx, int by, cons
   tile_maxima(d, L, tj0, ti0, th_new);
   // force / heat partials of the window faces this tile owns
   const bool touch = d.win && ti0 <= d.i1 && (ti0 + TX > d.i0 || d.periodic) && tj0 <= d.j1 &&
-                     (tj0 + TY > d.j0 || d.periodic);
+                     (tj0 + R > d.j0 || d.periodic);
   if (touch) {
     double acc[kMaxWin];
     for (int q = 0; q < kMaxWin; ++q) acc[q] = 0;
@@ -1224,23 +1242,27 @@ __device__ __forceinline__ void final_body(const Dev<T> &d, int bx, int by, cons
       if (j >= d.j0 && j < d.j1 && fi >= d.i0 && fi <= d.i1) {
         const int wo = (j - d.j0) * (d.wx + 1) + (fi - d.i0);
         const double a = (double)d.acc_u[wo];
-        for (int b = 0; b < d.nbody; ++b) {
-          acc[4 * b] += (double)d.k_u[(size_t)b * 2 * nuw + wo] * a;
-          acc[4 * b + 1] += (double)d.k_u[(size_t)b * 2 * nuw + nuw + wo] * a;
-        }
+#pragma unroll
+        for (int b = 0; b < kMaxWin / 4; ++b)
+          if (b < d.nbody) {
+            acc[4 * b] += (double)d.k_u[(size_t)b * 2 * nuw + wo] * a;
+            acc[4 * b + 1] += (double)d.k_u[(size_t)b * 2 * nuw + nuw + wo] * a;
+          }
       }
     };
     auto add_v = [&](int fj, int i_) {
       if (fj >= d.j0 && fj <= d.j1 && i_ >= d.i0 && i_ < d.i1) {
         const int wo = (fj - d.j0) * d.wx + (i_ - d.i0);
         const double a = (double)d.acc_v[wo];
-        for (int b = 0; b < d.nbody; ++b) {
-          acc[4 * b + 2] += (double)d.k_v[(size_t)b * 2 * nvw + wo] * a;
-          acc[4 * b + 3] += (double)d.k_v[(size_t)b * 2 * nvw + nvw + wo] * a;
-        }
+#pragma unroll
+        for (int b = 0; b < kMaxWin / 4; ++b)
+          if (b < d.nbody) {
+            acc[4 * b + 2] += (double)d.k_v[(size_t)b * 2 * nvw + wo] * a;
+            acc[4 * b + 3] += (double)d.k_v[(size_t)b * 2 * nvw + nvw + wo] * a;
+          }
       }
     };
-    for (int k = 0; k < 2; ++k) {
+    for (int k = 0; k < TR<R>::RPT; ++k) {
       const int j = tj0 + threadIdx.y + 4 * k;
       if (j >= ny || i >= nx) continue;
       add_u(j, i);
@@ -1252,8 +1274,12 @@ __device__ __forceinline__ void final_body(const Dev<T> &d, int bx, int by, cons
       if (j == 0 && d.periodic) add_v(ny, i);
       if (d.thermal && d.n_temp > 0 && j >= d.j0 && j < d.j1 && i >= d.i0 && i < d.i1) {
         const int wo = (j - d.j0) * d.wx + (i - d.i0);
-        for (int b = 0; b < d.n_temp; ++b)
-          acc[4 * d.nbody + b] += (double)d.w_t[(size_t)b * d.wy * d.wx + wo] * (double)d.acc_t[wo];
+        const double at = (double)d.acc_t[wo];
+#pragma unroll
+        for (int q = 0; q < kMaxWin; ++q) {
+          const int b = q - 4 * d.nbody;   // heat slots follow the force slots
+          if (b >= 0 && b < d.n_temp) acc[q] += (double)d.w_t[(size_t)b * d.wy * d.wx + wo] * at;
+        }
       }
     }
     block_sum4<kMaxWin>(acc, L.red);
@@ -1275,7 +1301,7 @@ __device__ __forceinline__ void final_body(const Dev<T> &d, int bx, int by, cons
       else
         load_c(d, L, tj0, ti0, c, d.dye_inflow, (T)0, 2);
       __syncthreads();
-      for (int k = 0; k < 2; ++k) {
+      for (int k = 0; k < TR<R>::RPT; ++k) {
         const int j = tj0 + threadIdx.y + 4 * k;
         if (j >= ny || i >= nx) continue;
         const int r = j - tj0 + 2, cc = i - ti0 + 2;
@@ -1306,10 +1332,10 @@ __device__ __forceinline__ void final_body(const Dev<T> &d, int bx, int by, cons
     }
   }
 }
-template <typename T>
+template <typename T, int R>
 __global__ void __launch_bounds__(256) k_final(Dev<T> d, const T *uh, const T *vh, T *u_out, T *v_out,
                                               const T *th_new, const T *cin, T *cout, const T *t_in) {
-  final_body<T>(d, blockIdx.x, blockIdx.y, uh, vh, u_out, v_out, th_new, cin, cout, t_in);
+  final_body<T, R>(d, blockIdx.x, blockIdx.y, uh, vh, u_out, v_out, th_new, cin, cout, t_in);
 }
 
 template <typename T>
@@ -1321,20 +1347,23 @@ __global__ void k_copy(T *__restrict__ dst, const T *__restrict__ src, size_t n)
 // Per-step outputs (core.py:942-1004, physics.py:760-795) and the next step's dt.
 template <typename T>
 __device__ __forceinline__ void finalize_body(const Dev<T> &d, const T *__restrict__ t_in, T *__restrict__ t_out, T *__restrict__ dt_out, const T *__restrict__ u_new, const T *__restrict__ th_new, const aero2d_outputs &o) {
-  __shared__ double lds[16 * kMaxWin];
-  __shared__ double res[kMaxWin];
+  __shared__ double lds[16 * (kMaxWin + 11)];
+  __shared__ double res[kMaxWin + 11];
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int nx = d.nx, ny = d.ny, nxp = nx + 1;
   const int w = tid >> 6, nw = nthr >> 6;
   // every load this block needs is issued here, before any reduction waits on it
   T dt = 0, tin = 0, kap = 0, tin_th = 0, wv[4] = {0, 0, 0, 0};
   double th_old = 0;
+  long long hist_n = 0, aux_n = 0;
   DtIn<T> din;
   if (tid == 0) {
     dt = d.red->dt;
     tin = *t_in;
     th_old = d.red->th_old;
     din = load_dt_in(d);
+    if (o.hist) hist_n = *reinterpret_cast<const long long *>(o.hist_count);
+    if (d.thermal && o.aux) aux_n = *reinterpret_cast<const long long *>(o.aux_count);
     if (d.thermal) {
       kap = *d.kappa;
       tin_th = *d.theta_in;
@@ -1361,12 +1390,13 @@ __device__ __forceinline__ void finalize_body(const Dev<T> &d, const T *__restri
       ws[3] += (double)(w3 - c1);
     }
   }
-  // window partials of the blocks that own window faces
+  // window partials of the blocks that own window faces, and the per-block dt inputs: all
+  // loads issued before the first barrier
   double acc[kMaxWin];
   for (int q = 0; q < kMaxWin; ++q) acc[q] = 0;
   if (d.win) {
     int bx0 = d.i0 / TX, bx1 = (d.i1 < nx ? d.i1 : nx - 1) / TX;
-    int by0 = d.j0 / TY, by1 = (d.j1 < ny ? d.j1 : ny - 1) / TY;
+    int by0 = d.j0 / d.ty, by1 = (d.j1 < ny ? d.j1 : ny - 1) / d.ty;
     if (d.periodic) { bx0 = 0; by0 = 0; }
     const int nbx = bx1 - bx0 + 1, nb = nbx * (by1 - by0 + 1);
 #pragma unroll 2
@@ -1376,35 +1406,42 @@ __device__ __forceinline__ void finalize_body(const Dev<T> &d, const T *__restri
       for (int q = 0; q < kMaxWin; ++q) acc[q] += p[q];
     }
   }
+  double pm[4] = {0, 0, 0, 0}, psum = 0;
+  const int nblocks = d.nbx * d.nby;
+#pragma unroll 4
+  for (int b = tid; b < nblocks; b += nthr) {
+    const double *p = d.pmax + 5 * b;
+    for (int q = 0; q < 4; ++q) pm[q] = fmax(pm[q], p[q]);
+    psum += p[4];
+  }
+  // wave reductions, one barrier, then one thread per quantity adds the waves in order
+  constexpr int kW = kMaxWin + 5 + 6;   // window sums, dt maxima + sum theta, wall sums
   for (int q = 0; q < kMaxWin; ++q) acc[q] = wave_sum(acc[q]);
-  if ((tid & 63) == 0)
-    for (int q = 0; q < kMaxWin; ++q) lds[w * kMaxWin + q] = acc[q];
-  __syncthreads();
-  if (tid == 0) {
-    for (int q = 0; q < kMaxWin; ++q) {
-      double sacc = 0;
-      for (int ww = 0; ww < nw; ++ww) sacc += lds[ww * kMaxWin + q];
-      res[q] = sacc;
-    }
-  }
-  __syncthreads();
-  // next step's dt inputs
-  double mx[5];
-  reduce_pmax(d, d.nbx * d.nby, lds, mx);
-  // heat outputs: wall sums, in/outflow
-  if (d.thermal) {
+  for (int q = 0; q < 4; ++q) pm[q] = wave_max(pm[q]);
+  psum = wave_sum(psum);
+  if (d.thermal)
     for (int q = 0; q < 6; ++q) ws[q] = wave_sum(ws[q]);
-    __syncthreads();
-    if ((tid & 63) == 0)
-      for (int q = 0; q < 6; ++q) lds[w * 6 + q] = ws[q];
-    __syncthreads();
-    if (tid == 0)
-      for (int q = 0; q < 6; ++q) {
-        double sacc = 0;
-        for (int ww = 0; ww < nw; ++ww) sacc += lds[ww * 6 + q];
-        ws[q] = sacc;
-      }
+  if ((tid & 63) == 0) {
+    for (int q = 0; q < kMaxWin; ++q) lds[w * kW + q] = acc[q];
+    for (int q = 0; q < 4; ++q) lds[w * kW + kMaxWin + q] = pm[q];
+    lds[w * kW + kMaxWin + 4] = psum;
+    for (int q = 0; q < 6; ++q) lds[w * kW + kMaxWin + 5 + q] = ws[q];
   }
+  __syncthreads();
+  if (tid < kW) {
+    const bool is_max = tid >= kMaxWin && tid < kMaxWin + 4;
+    double r = is_max ? 0.0 : lds[tid];
+    if (is_max)
+      for (int ww = 0; ww < nw; ++ww) r = fmax(r, lds[ww * kW + tid]);
+    else
+      for (int ww = 1; ww < nw; ++ww) r += lds[ww * kW + tid];
+    res[tid] = r;
+  }
+  __syncthreads();
+  double mx[5];
+  for (int q = 0; q < 5; ++q) mx[q] = res[kMaxWin + q];
+  if (d.thermal)
+    for (int q = 0; q < 6; ++q) ws[q] = res[kMaxWin + 5 + q];
   if (tid != 0) return;
   const T tnew = tin + dt;
   *t_out = tnew;
@@ -1426,13 +1463,13 @@ __device__ __forceinline__ void finalize_body(const Dev<T> &d, const T *__restri
   }
   if (o.hist) {
     long long *cnt = reinterpret_cast<long long *>(o.hist_count);
-    const long long row = *cnt % o.hist_cap;
+    const long long row = hist_n % o.hist_cap;
     T *hrow = reinterpret_cast<T *>(o.hist) + row * 4;
     hrow[0] = tnew;
     hrow[1] = sumc[0];
     hrow[2] = sumc[1];
     hrow[3] = sumc[2];
-    *cnt = *cnt + 1;
+    *cnt = hist_n + 1;
   }
   if (d.thermal) {
     const int nt = d.n_temp;
@@ -1460,12 +1497,12 @@ __device__ __forceinline__ void finalize_body(const Dev<T> &d, const T *__restri
     }
     if (o.aux) {
       long long *cnt = reinterpret_cast<long long *>(o.aux_count);
-      const long long row = *cnt % o.aux_cap;
+      const long long row = aux_n % o.aux_cap;
       const int ncol = 1 + nt + 7;
       T *arow = reinterpret_cast<T *>(o.aux) + row * ncol;
       arow[0] = tnew;
       for (int q = 0; q < nt + 7; ++q) arow[1 + q] = vals[q];
-      *cnt = *cnt + 1;
+      *cnt = aux_n + 1;
     }
   }
   // the next step (chained) starts from this state
@@ -1498,12 +1535,12 @@ struct BRun {
   int nslotB, mode;
 };
 
-template <typename T>
+template <typename T, int R>
 __global__ void __launch_bounds__(256) kb_pre(const Dev<T> *__restrict__ devs, const BRun<T> *__restrict__ runs,
                                              const int4 *__restrict__ items) {
   const int4 it = items[blockIdx.x];
   const BRun<T> &r = runs[it.x];
-  pre_body<T>(devs[it.x], it.y, it.z, r.u0, r.v0, r.th0);
+  pre_body<T, R>(devs[it.x], it.y, it.z, r.u0, r.v0, r.th0);
 }
 
 template <typename T>
@@ -1511,11 +1548,11 @@ __global__ void __launch_bounds__(1024) kb_dt(const Dev<T> *__restrict__ devs) {
   dt_body<T>(devs[blockIdx.x]);
 }
 
-template <typename T>
+template <typename T, int R>
 __global__ void __launch_bounds__(256) kb_stage(const Dev<T> *__restrict__ devs, const BRun<T> *__restrict__ runs,
                                                const int4 *__restrict__ items, int stage) {
   const int4 it = items[blockIdx.x];
-  stage_body<T>(devs[it.x], runs[it.x].st[stage], it.y, it.z);
+  stage_body<T, R>(devs[it.x], runs[it.x].st[stage], it.y, it.z);
 }
 
 template <typename T>
@@ -1538,18 +1575,18 @@ __global__ void __launch_bounds__(kFftThreads) kb_poisB(const Dev<T> *__restrict
 template <typename T>
 __global__ void __launch_bounds__(kFftThreads) kb_poisC(const Dev<T> *__restrict__ devs,
                                                         const BRun<T> *__restrict__ runs,
-                                                        const int4 *__restrict__ items, int stage) {
+                                                        const int4 *__restrict__ items, int stage, int write_p) {
   const int4 it = items[blockIdx.x];
   const Dev<T> &d = devs[it.x];
-  poisC_body<T>(d, it.y, d.psi, runs[it.x].p_out, stage);
+  poisC_body<T>(d, it.y, d.psi, write_p ? runs[it.x].p_out : (T *)nullptr, stage);
 }
 
-template <typename T>
+template <typename T, int R>
 __global__ void __launch_bounds__(256) kb_final(const Dev<T> *__restrict__ devs, const BRun<T> *__restrict__ runs,
                                                const int4 *__restrict__ items) {
   const int4 it = items[blockIdx.x];
   const BRun<T> &r = runs[it.x];
-  final_body<T>(devs[it.x], it.y, it.z, r.uC, r.vC, r.u_out, r.v_out, r.th_new, (const T *)nullptr, (T *)nullptr,
+  final_body<T, R>(devs[it.x], it.y, it.z, r.uC, r.vC, r.u_out, r.v_out, r.th_new, (const T *)nullptr, (T *)nullptr,
                 r.t_in);
 }
 
@@ -1576,13 +1613,25 @@ struct aero2d_plan {
   void *uA, *vA, *uB, *vB, *uC, *vC, *thA, *thB, *F, *psi, *acc_u, *acc_v, *acc_t, *outcol, *dye_tmp;
   void *tw_x, *tw_y, *cs_x, *cs_y;
   double *lam_x, *lam_y, *pmax, *pforce;
+  void *inv_lam;
   void *red;
   FftPlan fx, fy;
   int nbx, nby;
+  int ty;       // rows per tile (4, 8 or 16)
   int nslotB;
 };
 
 namespace {
+
+// calls f(std::integral_constant<int, R>) for the tile rows R = ty (4, 8 or 16)
+template <typename F>
+void by_rows(int ty, F &&f) {
+  switch (ty) {
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 16: f(std::integral_constant<int, 16>{}); break;
+    default: f(std::integral_constant<int, 8>{}); break;
+  }
+}
 
 int alloc(aero2d_plan *p, void **ptr, size_t n) {
   if (n == 0) n = 16;
@@ -1608,6 +1657,7 @@ int upload_tables(aero2d_plan *p) {
     AERO_HIP_CHECK(hipMemcpy(dst, src, n, hipMemcpyHostToDevice));
     return AERO_OK;
   };
+  std::vector<double> lamx, lamy;
   for (int axis = 0; axis < 2; ++axis) {
     const int n = axis == 0 ? d.nx : d.ny;
     std::vector<cplx<T>> tw(n), cs(n);
@@ -1625,8 +1675,16 @@ int upload_tables(aero2d_plan *p) {
     if ((rc = put(axis == 0 ? p->tw_x : p->tw_y, tw.data(), n * sizeof(cplx<T>)))) return rc;
     if ((rc = put(axis == 0 ? p->cs_x : p->cs_y, cs.data(), n * sizeof(cplx<T>)))) return rc;
     if ((rc = put(axis == 0 ? (void *)p->lam_x : (void *)p->lam_y, lam.data(), n * sizeof(double)))) return rc;
+    (axis == 0 ? lamx : lamy) = lam;
   }
-  return AERO_OK;
+  // the spectral scaling 1 / (lam_x + lam_y), formed in float64 and rounded once (ns2d/poisson.py:115-125)
+  std::vector<T> inv((size_t)d.nx * d.ny);
+  for (int i = 0; i < d.nx; ++i)
+    for (int k = 0; k < d.ny; ++k) {
+      const double la = lamx[i] + lamy[k];
+      inv[(size_t)i * d.ny + k] = (T)((la != 0.0 && !(i == 0 && k == 0)) ? 1.0 / la : 0.0);
+    }
+  return put(p->inv_lam, inv.data(), inv.size() * sizeof(T));
 }
 
 template <typename T>
@@ -1638,6 +1696,7 @@ Dev<T> make_dev(const aero2d_plan *p, const aero2d_geom *g, const aero2d_params 
   d.ny = s.ny;
   d.nbx = p->nbx;
   d.nby = p->nby;
+  d.ty = p->ty;
   d.periodic = s.periodic;
   d.closed = s.closed;
   d.upwind = s.advection == 1;
@@ -1701,6 +1760,7 @@ Dev<T> make_dev(const aero2d_plan *p, const aero2d_geom *g, const aero2d_params 
   d.tw_x = (const cplx<T> *)p->tw_x; d.tw_y = (const cplx<T> *)p->tw_y;
   d.cs_x = (const cplx<T> *)p->cs_x; d.cs_y = (const cplx<T> *)p->cs_y;
   d.lam_x = p->lam_x; d.lam_y = p->lam_y;
+  d.inv_lam = (const T *)p->inv_lam;
   d.fx = p->fx; d.fy = p->fy;
   d.dct_x = s.periodic ? 0 : 1;
   d.dct_y = s.periodic ? 0 : 1;
@@ -1737,7 +1797,7 @@ int step_impl(aero2d_plan *p, hipStream_t st, const aero2d_fields *in, aero2d_fi
   const dim3 tiles(p->nbx, p->nby);
   const T *u0 = (const T *)in->u, *v0 = (const T *)in->v, *th0 = (const T *)in->theta;
   if (!chained) {
-    hipLaunchKernelGGL(k_pre<T>, tiles, blk, 0, st, d, u0, v0, th0);
+    by_rows(p->ty, [&](auto r) { hipLaunchKernelGGL((k_pre<T, decltype(r)::value>), tiles, blk, 0, st, d, u0, v0, th0); });
     hipLaunchKernelGGL(k_dt<T>, dim3(1), dim3(1024), 0, st, d);
   }
   T *ubuf[3] = {(T *)p->uA, (T *)p->uB, (T *)p->uC};
@@ -1754,15 +1814,18 @@ int step_impl(aero2d_plan *p, hipStream_t st, const aero2d_fields *in, aero2d_fi
     sg.up = up; sg.vp = vp; sg.psi_p = psi_p; sg.thp = thp;
     sg.u0 = u0; sg.v0 = v0; sg.th0 = th0;
     sg.uh = ubuf[k]; sg.vh = vbuf[k]; sg.thh = s.thermal ? tbuf[k] : nullptr;
-    hipLaunchKernelGGL(k_stage<T>, tiles, blk, 0, st, d, sg);
+    by_rows(p->ty, [&](auto r) { hipLaunchKernelGGL((k_stage<T, decltype(r)::value>), tiles, blk, 0, st, d, sg); });
     solve_passes<T>(p, st, d, sg.uh, sg.vh, nullptr, mode, d.psi, (T *)out->p, k);
     up = sg.uh; vp = sg.vh; thp = sg.thh; psi_p = d.psi;
   }
   const bool dye = s.dye_mode != 0 && in->dye && out->dye;
   const bool inplace = dye && in->dye == out->dye;
   T *dye_dst = dye ? (inplace ? (T *)p->dye_tmp : (T *)out->dye) : nullptr;
-  hipLaunchKernelGGL(k_final<T>, tiles, blk, 0, st, d, (const T *)p->uC, (const T *)p->vC, (T *)out->u, (T *)out->v,
-                     (const T *)out->theta, dye ? (const T *)in->dye : (const T *)nullptr, dye_dst, (const T *)in->t);
+  by_rows(p->ty, [&](auto r) {
+    hipLaunchKernelGGL((k_final<T, decltype(r)::value>), tiles, blk, 0, st, d, (const T *)p->uC, (const T *)p->vC,
+                       (T *)out->u, (T *)out->v, (const T *)out->theta, dye ? (const T *)in->dye : (const T *)nullptr,
+                       dye_dst, (const T *)in->t);
+  });
   if (inplace) {
     const size_t n = (size_t)s.n_dye * ny * nx;
     const int cb = (int)std::min<size_t>(4096, (n + 255) / 256);
@@ -1819,9 +1882,12 @@ int time_stage_impl(aero2d_plan *p, hipStream_t st, const aero2d_fields *f, cons
   hipEvent_t e0, e1;
   AERO_HIP_CHECK(hipEventCreate(&e0));
   AERO_HIP_CHECK(hipEventCreate(&e1));
-  hipLaunchKernelGGL(k_stage<T>, tiles, blk, 0, st, d, sg);   // warm
-  AERO_HIP_CHECK(hipEventRecord(e0, st));
-  for (int it = 0; it < iters; ++it) hipLaunchKernelGGL(k_stage<T>, tiles, blk, 0, st, d, sg);
+  by_rows(p->ty, [&](auto r) {
+    constexpr int R = decltype(r)::value;
+    hipLaunchKernelGGL((k_stage<T, R>), tiles, blk, 0, st, d, sg);   // warm
+    (void)hipEventRecord(e0, st);
+    for (int it = 0; it < iters; ++it) hipLaunchKernelGGL((k_stage<T, R>), tiles, blk, 0, st, d, sg);
+  });
   AERO_HIP_CHECK(hipEventRecord(e1, st));
   AERO_HIP_CHECK(hipEventSynchronize(e1));
   float ms = 0.f;
@@ -1846,6 +1912,10 @@ int check_desc(const aero2d_desc *d) {
   if (ldsx > 65536 || ldsy > 65536) { set_error("grid too large for the LDS-resident transforms"); return AERO_EUNSUPPORTED; }
   if (d->n_groups > AERO2D_MAX_GROUPS || d->n_groups < 0) { set_error("too many emitter groups"); return AERO_EUNSUPPORTED; }
   if (d->dye_mode < 0 || d->dye_mode > 2) { set_error("unknown dye mode"); return AERO_EINVAL; }
+  if (d->tile_rows != 0 && d->tile_rows != 4 && d->tile_rows != 8 && d->tile_rows != 16) {
+    set_error("tile_rows must be 0 (default), 4, 8 or 16");
+    return AERO_EINVAL;
+  }
   const bool win = d->win_j1 > d->win_j0 && d->win_i1 > d->win_i0;
   if (win && (d->win_j0 < 0 || d->win_i0 < 0 || d->win_j1 > d->ny || d->win_i1 > d->nx)) {
     set_error("body window outside the grid");
@@ -1924,8 +1994,11 @@ int aero2d_plan_create(const aero2d_desc *desc, aero2d_plan **out) {
   const bool win = desc->win_j1 > desc->win_j0 && desc->win_i1 > desc->win_i0;
   const size_t wy = win ? desc->win_j1 - desc->win_j0 : 0, wx = win ? desc->win_i1 - desc->win_i0 : 0;
   p->nbx = (int)((nx + TX - 1) / TX);
-  p->nby = (int)((ny + TY - 1) / TY);
-  const size_t blocks = (size_t)p->nbx * p->nby;
+  p->ty = desc->tile_rows ? desc->tile_rows : kDefaultRows;
+  p->nby = (int)((ny + p->ty - 1) / p->ty);
+  // partial maxima and forces sized for the smallest tile (4 rows), so that a batched
+  // launch may run this plan with any tile shape
+  const size_t blocks = (size_t)p->nbx * ((ny + 3) / 4);
   p->nslotB = 2;
   while (p->nslotB > 1 && ny * p->nslotB * (2 * 2 * es + 2 * es) + ny * 2 * es > 65536) p->nslotB--;
   struct { void **ptr; size_t n; } tab[] = {
@@ -1935,7 +2008,8 @@ int aero2d_plan_create(const aero2d_desc *desc, aero2d_plan **out) {
       {&p->acc_u, wy * (wx + 1) * es}, {&p->acc_v, (wy + 1) * wx * es}, {&p->acc_t, wy * wx * es},
       {&p->outcol, ny * es}, {&p->dye_tmp, desc->dye_mode ? (size_t)desc->n_dye * nc * es : 0},
       {&p->tw_x, nx * 2 * es}, {&p->tw_y, ny * 2 * es}, {&p->cs_x, nx * 2 * es}, {&p->cs_y, ny * 2 * es},
-      {(void **)&p->lam_x, nx * 8}, {(void **)&p->lam_y, ny * 8}, {(void **)&p->pmax, blocks * 5 * 8},
+      {(void **)&p->lam_x, nx * 8}, {(void **)&p->lam_y, ny * 8}, {&p->inv_lam, nc * es},
+      {(void **)&p->pmax, blocks * 5 * 8},
       {(void **)&p->pforce, blocks * kMaxWin * 8}, {&p->red, 256}};
   for (auto &t : tab) {
     if ((rc = alloc(p, t.ptr, t.n))) {
@@ -2006,7 +2080,13 @@ int aero2d_poisson_solve(aero2d_plan *p, void *stream, const void *f, void *psi)
 // ---------------------------------------------------------------- ensemble (host)
 struct aero2d_ensemble {
   int n = 0, dbl = 0;
+  int ty = kDefaultRows;   // rows per tile of the batched cell launches
+  int nslot = 2;           // column pairs per block of the batched pass-B launch (capped per run)
   std::vector<aero2d_plan *> plans;
+  std::vector<aero2d_fields> fields;
+  std::vector<aero2d_geom> geoms;
+  std::vector<aero2d_params> params;
+  std::vector<aero2d_outputs> outputs;   // empty: no outputs
   std::vector<void *> allocs;
   void *devs = nullptr, *runs = nullptr, *tiles = nullptr, *rows = nullptr, *cols = nullptr;
   int ntiles = 0, nrows = 0, ncols = 0;
@@ -2022,9 +2102,34 @@ int ens_alloc(aero2d_ensemble *e, void **ptr, const void *src, size_t n) {
   return AERO_OK;
 }
 
+void ens_release(aero2d_ensemble *e) {
+  for (void *a : e->allocs) (void)hipFree(a);
+  e->allocs.clear();
+  e->devs = e->runs = e->tiles = e->rows = e->cols = nullptr;
+  e->ntiles = e->nrows = e->ncols = 0;
+  e->ldsA = e->ldsB = 0;
+}
+
+// column pairs per pass-B block for run p: the largest of want, want / 2, ..., 1 whose
+// columns fit the per-thread element count and 64 KB of LDS
 template <typename T>
-int ensemble_build(aero2d_ensemble *e, const aero2d_fields *fields, const aero2d_geom *geoms,
-                   const aero2d_params *params, const aero2d_outputs *outputs) {
+int run_slots(const aero2d_plan *p, int want) {
+  const size_t ny = p->desc.ny;
+  int s = want;
+  while (s > 1 && ((size_t)s * ny > (size_t)kMaxPer * kFftThreads ||
+                   (size_t)s * ny * (2 * sizeof(cplx<T>) + 2 * sizeof(T)) + ny * sizeof(cplx<T>) > 65536))
+    s /= 2;
+  return s;
+}
+
+// (re)builds the device run tables for the tile rows e->ty and column slots e->nslot
+template <typename T>
+int ensemble_build(aero2d_ensemble *e) {
+  ens_release(e);
+  const aero2d_fields *fields = e->fields.data();
+  const aero2d_geom *geoms = e->geoms.data();
+  const aero2d_params *params = e->params.data();
+  const aero2d_outputs *outputs = e->outputs.empty() ? nullptr : e->outputs.data();
   std::vector<Dev<T>> devs(e->n);
   std::vector<BRun<T>> runs(e->n);
   std::vector<int4> tiles, rows, cols;
@@ -2032,6 +2137,9 @@ int ensemble_build(aero2d_ensemble *e, const aero2d_fields *fields, const aero2d
     aero2d_plan *p = e->plans[r];
     const aero2d_desc &s = p->desc;
     devs[r] = make_dev<T>(p, &geoms[r], &params[r]);
+    const int nby = (s.ny + e->ty - 1) / e->ty;
+    devs[r].ty = e->ty;
+    devs[r].nby = nby;
     BRun<T> &b = runs[r];
     std::memset(&b, 0, sizeof(b));
     const aero2d_fields &f = fields[r];
@@ -2058,15 +2166,15 @@ int ensemble_build(aero2d_ensemble *e, const aero2d_fields *fields, const aero2d
     b.th_new = (const T *)f.theta;
     b.t_in = (const T *)f.t; b.t_out = (T *)f.t; b.dt_out = (T *)f.dt;
     if (outputs) b.o = outputs[r];
-    b.nslotB = p->nslotB;
+    b.nslotB = run_slots<T>(p, e->nslot);
     b.mode = (s.periodic || s.closed) ? 1 : 2;
-    for (int by = 0; by < p->nby; ++by)
+    for (int by = 0; by < nby; ++by)
       for (int bx = 0; bx < p->nbx; ++bx) tiles.push_back(make_int4(r, bx, by, 0));
     for (int bx = 0; bx < (s.ny + 1) / 2; ++bx) rows.push_back(make_int4(r, bx, 0, 0));
-    const int cpb = 2 * p->nslotB;
+    const int cpb = 2 * b.nslotB;
     for (int bx = 0; bx < (s.nx + cpb - 1) / cpb; ++bx) cols.push_back(make_int4(r, bx, 0, 0));
     e->ldsA = std::max(e->ldsA, lds_A<T>(p));
-    e->ldsB = std::max(e->ldsB, lds_B<T>(p));
+    e->ldsB = std::max(e->ldsB, b.nslotB * s.ny * (2 * sizeof(cplx<T>) + 2 * sizeof(T)) + s.ny * sizeof(cplx<T>));
   }
   e->ntiles = (int)tiles.size();
   e->nrows = (int)rows.size();
@@ -2086,20 +2194,94 @@ int ensemble_step(aero2d_ensemble *e, hipStream_t st, bool chained) {
   const BRun<T> *runs = (const BRun<T> *)e->runs;
   const int4 *tiles = (const int4 *)e->tiles, *rows = (const int4 *)e->rows, *cols = (const int4 *)e->cols;
   const dim3 blk(64, 4);
-  if (!chained) {
-    hipLaunchKernelGGL(kb_pre<T>, dim3(e->ntiles), blk, 0, st, devs, runs, tiles);
-    hipLaunchKernelGGL(kb_dt<T>, dim3(e->n), dim3(1024), 0, st, devs);
-  }
-  for (int k = 0; k < 3; ++k) {
-    hipLaunchKernelGGL(kb_stage<T>, dim3(e->ntiles), blk, 0, st, devs, runs, tiles, k);
-    hipLaunchKernelGGL(kb_poisA<T>, dim3(e->nrows), dim3(kFftThreads), e->ldsA, st, devs, runs, rows, k);
-    hipLaunchKernelGGL(kb_poisB<T>, dim3(e->ncols), dim3(kFftThreads), e->ldsB, st, devs, runs, cols);
-    hipLaunchKernelGGL(kb_poisC<T>, dim3(e->nrows), dim3(kFftThreads), e->ldsA, st, devs, runs, rows, k);
-  }
-  hipLaunchKernelGGL(kb_final<T>, dim3(e->ntiles), blk, 0, st, devs, runs, tiles);
+  by_rows(e->ty, [&](auto rr) {
+    constexpr int R = decltype(rr)::value;
+    if (!chained) {
+      hipLaunchKernelGGL((kb_pre<T, R>), dim3(e->ntiles), blk, 0, st, devs, runs, tiles);
+      hipLaunchKernelGGL(kb_dt<T>, dim3(e->n), dim3(1024), 0, st, devs);
+    }
+    for (int k = 0; k < 3; ++k) {
+      hipLaunchKernelGGL((kb_stage<T, R>), dim3(e->ntiles), blk, 0, st, devs, runs, tiles, k);
+      hipLaunchKernelGGL(kb_poisA<T>, dim3(e->nrows), dim3(kFftThreads), e->ldsA, st, devs, runs, rows, k);
+      hipLaunchKernelGGL(kb_poisB<T>, dim3(e->ncols), dim3(kFftThreads), e->ldsB, st, devs, runs, cols);
+      hipLaunchKernelGGL(kb_poisC<T>, dim3(e->nrows), dim3(kFftThreads), e->ldsA, st, devs, runs, rows, k, 1);
+    }
+    hipLaunchKernelGGL((kb_final<T, R>), dim3(e->ntiles), blk, 0, st, devs, runs, tiles);
+  });
   hipLaunchKernelGGL(kb_finalize<T>, dim3(e->n), dim3(1024), 0, st, devs, runs);
   AERO_HIP_CHECK(hipGetLastError());
   return AERO_OK;
+}
+
+// Launches RK stages 0 and 1 of every run (stage kernel and the three Poisson passes, the
+// pressure accumulator left alone) `iters` times between two events; these write only plan
+// scratch (stage buffers, right-hand side, potential), so the runs' state is untouched.
+template <typename T>
+int ensemble_time(aero2d_ensemble *e, hipStream_t st, int iters, float *ms) {
+  const Dev<T> *devs = (const Dev<T> *)e->devs;
+  const BRun<T> *runs = (const BRun<T> *)e->runs;
+  const int4 *tiles = (const int4 *)e->tiles, *rows = (const int4 *)e->rows, *cols = (const int4 *)e->cols;
+  const dim3 blk(64, 4);
+  hipEvent_t e0, e1;
+  AERO_HIP_CHECK(hipEventCreate(&e0));
+  AERO_HIP_CHECK(hipEventCreate(&e1));
+  by_rows(e->ty, [&](auto rr) {
+    constexpr int R = decltype(rr)::value;
+    for (int it = 0; it <= iters; ++it) {
+      if (it == 1) (void)hipEventRecord(e0, st);   // the first round warms up
+      for (int k = 0; k < 2; ++k) {
+        hipLaunchKernelGGL((kb_stage<T, R>), dim3(e->ntiles), blk, 0, st, devs, runs, tiles, k);
+        hipLaunchKernelGGL(kb_poisA<T>, dim3(e->nrows), dim3(kFftThreads), e->ldsA, st, devs, runs, rows, k);
+        hipLaunchKernelGGL(kb_poisB<T>, dim3(e->ncols), dim3(kFftThreads), e->ldsB, st, devs, runs, cols);
+        hipLaunchKernelGGL(kb_poisC<T>, dim3(e->nrows), dim3(kFftThreads), e->ldsA, st, devs, runs, rows, k, 0);
+      }
+    }
+  });
+  (void)hipEventRecord(e1, st);
+  hipError_t err = hipEventSynchronize(e1);
+  if (err == hipSuccess) err = hipEventElapsedTime(ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (err != hipSuccess) {
+    set_error(std::string("ensemble timing: ") + hipGetErrorString(err));
+    return AERO_EHIP;
+  }
+  AERO_HIP_CHECK(hipGetLastError());
+  return AERO_OK;
+}
+
+// Times every (tile rows, column slots) candidate and keeps the fastest; choice[0..1]
+// receives the rows and slots chosen, choice[2] the candidates timed.
+template <typename T>
+int ensemble_tune(aero2d_ensemble *e, hipStream_t st, int iters, int32_t *choice) {
+  static const int kRows[] = {4, 8, 16};
+  static const int kSlots[] = {1, 2, 4};
+  float best = 0.f;
+  int best_ty = e->ty, best_slot = e->nslot, tried = 0;
+  for (int ty : kRows)
+    for (int sl : kSlots) {
+      e->ty = ty;
+      e->nslot = sl;
+      int rc = ensemble_build<T>(e);
+      float ms = 0.f;
+      if (!rc) rc = ensemble_time<T>(e, st, iters, &ms);
+      if (rc) return rc;
+      ++tried;
+      if (tried == 1 || ms < best) {
+        best = ms;
+        best_ty = ty;
+        best_slot = sl;
+      }
+    }
+  e->ty = best_ty;
+  e->nslot = best_slot;
+  const int rc = ensemble_build<T>(e);
+  if (choice) {
+    choice[0] = best_ty;
+    choice[1] = best_slot;
+    choice[2] = tried;
+  }
+  return rc;
 }
 
 }  // namespace
@@ -2126,8 +2308,11 @@ int aero2d_ensemble_create(int32_t n, aero2d_plan *const *plans, const aero2d_fi
   e->n = n;
   e->dbl = dbl;
   e->plans.assign(plans, plans + n);
-  int rc = dbl ? ensemble_build<double>(e, fields, geoms, params, outputs)
-               : ensemble_build<float>(e, fields, geoms, params, outputs);
+  e->fields.assign(fields, fields + n);
+  e->geoms.assign(geoms, geoms + n);
+  e->params.assign(params, params + n);
+  if (outputs) e->outputs.assign(outputs, outputs + n);
+  int rc = dbl ? ensemble_build<double>(e) : ensemble_build<float>(e);
   if (rc) {
     aero2d_ensemble_destroy(e);
     return rc;
@@ -2142,9 +2327,16 @@ int aero2d_ensemble_step(aero2d_ensemble *e, void *stream, int32_t chained) {
   return e->dbl ? ensemble_step<double>(e, st, chained != 0) : ensemble_step<float>(e, st, chained != 0);
 }
 
+int aero2d_ensemble_tune(aero2d_ensemble *e, void *stream, int32_t iters, int32_t *choice) {
+  if (!e) { set_error("null ensemble"); return AERO_EINVAL; }
+  if (iters < 1 || iters > 1000) { set_error("iters must be in [1, 1000]"); return AERO_EINVAL; }
+  hipStream_t st = (hipStream_t)stream;
+  return e->dbl ? ensemble_tune<double>(e, st, iters, choice) : ensemble_tune<float>(e, st, iters, choice);
+}
+
 int aero2d_ensemble_destroy(aero2d_ensemble *e) {
   if (!e) return AERO_OK;
-  for (void *a : e->allocs) (void)hipFree(a);
+  ens_release(e);
   delete e;
   return AERO_OK;
 }

@@ -102,6 +102,7 @@ struct Dev3 {
   const cplx<T> *tw[3];
   const cplx<T> *cs[3];
   const double *lam[3];
+  const T *inv_lam;          // (ny, nx, nz), z fastest: 1 / (lam_x + lam_y + lam_z) formed in float64
   FftPlan fp[3];
   int dct[3];
   int ns1, ns2, ns3, ns5;
@@ -401,14 +402,23 @@ __device__ __forceinline__ T eddy3(int turb, T les_k, T g[3][3]) {
 // nu_t on the tile's cells and their one-cell ring, zero inside solids (x (1 - chi_c) on
 // the window, core.py:491-503); returns this thread's max over the cells the tile owns
 // (its own cells, plus the ghost ring at domain boundaries).
+// own_only: only the cells whose maximum the tile reports (the dt input), L.nu is not filled.
 template <typename T>
-__device__ T nu_tile(const Dev3<T> &d, const Call3<T> &c, Lds3<T> &L, int k0, int j0, int i0, bool store = false) {
+__device__ T nu_tile(const Dev3<T> &d, const Call3<T> &c, Lds3<T> &L, int k0, int j0, int i0, bool store = false,
+                     bool own_only = false) {
   using S = Lds3<T>;
   T mx = 0;
   const int ie = min(i0 + S::TX, d.nx), je = min(j0 + S::TY, d.ny), ke = min(k0 + S::TZ, d.nz);
   for (int e = threadIdx.x; e < S::NZ * S::NY * S::NX; e += kTileThreads) {
     const int x = e % S::NX, t = e / S::NX, y = t % S::NY, z = t / S::NY;
     const int i = x - 1, j = y - 1, k = z - 1;
+    if (own_only) {
+      const int gi = i0 + i, gj = j0 + j, gk = k0 + k;
+      const bool oi = (gi >= i0 && gi < ie) || gi == -1 || gi == d.nx;
+      const bool oj = (gj >= j0 && gj < je) || gj == -1 || gj == d.ny;
+      const bool ok = (gk >= k0 && gk < ke) || gk == -1 || gk == d.nz;
+      if (!(oi && oj && ok && gi <= ie && gj <= je && gk <= ke)) continue;
+    }
     T g[3][3];
     cell_grad(L, k, j, i, d.inv_h, g);
     T nu = eddy3(d.turb, d.les_k, g);
@@ -425,7 +435,7 @@ __device__ T nu_tile(const Dev3<T> &d, const Call3<T> &c, Lds3<T> &L, int k0, in
     }
     if (d.win && gk >= d.wk0 && gk < d.wk1 && gj >= d.wj0 && gj < d.wj1 && gi >= d.wi0 && gi < d.wi1)
       nu = nu * ((T)1 - c.chi_c[((size_t)(gk - d.wk0) * d.wy + (gj - d.wj0)) * d.wx + (gi - d.wi0)]);
-    L.nu[z][y][x] = nu;
+    if (!own_only) L.nu[z][y][x] = nu;
     // the heat equation's turbulent diffusivity reads the stage's nu_t of the cells
     if (store && gi >= i0 && gi < ie && gj >= j0 && gj < je && gk >= k0 && gk < ke)
       d.nut_c[((size_t)gk * d.ny + gj) * d.nx + gi] = nu;
@@ -672,7 +682,7 @@ __global__ void __launch_bounds__(kTileThreads, Tile<T>::WAVES_PER_SIMD) k_final
   const T corr = c.corr_stage >= 0 ? d.red->corr[c.corr_stage] : (T)0;
   load_tile(d, c, L, k0, j0, i0, corr);
   T mnu = 0;
-  if (d.turb) mnu = nu_tile(d, c, L, k0, j0, i0);
+  if (d.turb) mnu = nu_tile(d, c, L, k0, j0, i0, false, true);
   const int ie = min(i0 + S::TX, nx), je = min(j0 + S::TY, ny), ke = min(k0 + S::TZ, nz);
   T mu = 0, mv = 0, mw = 0, mt = 0;
   if (mode == 0 && d.theta) {
@@ -734,43 +744,54 @@ __global__ void __launch_bounds__(kTileThreads, Tile<T>::WAVES_PER_SIMD) k_final
     for (int q = 1; q < kTileWaves; ++q) m = red[q * kPm + threadIdx.x] > m ? red[q * kPm + threadIdx.x] : m;
     d.pmax[(size_t)tile * kPm + threadIdx.x] = m;
   }
-  // force sums of this tile's window faces: sum_faces k[b, m] * acc (core.py:653-660)
+  // force sums of this tile's window faces: sum_faces k[b, m] * acc (core.py:653-660), one
+  // pass over the faces per (component, body) for its three weights; per-wave sums go to the
+  // (no longer needed) u tile, then waves are added in order
   if (mode && d.win && bx >= d.wbx0 && bx < d.wbx0 + d.nwbx && by >= d.wby0 && by < d.wby0 + d.nwby &&
       bz >= d.wbz0 && bz < d.wbz0 + d.nwbz) {
     const int wt = ((bz - d.wbz0) * d.nwby + (by - d.wby0)) * d.nwbx + (bx - d.wbx0);
     __syncthreads();
-    for (int q = 0; q < d.nq; ++q) {
-      const int b = q / 9, r = q % 9, comp = r / 3, m = r % 3;
-      double acc = 0.0;
+    static_assert(sizeof(L.u) >= sizeof(double) * kTileWaves * kMaxQ, "force partials exceed the u tile");
+    double *fr = reinterpret_cast<double *>(&L.u[0][0][0]);
+    for (int comp = 0; comp < 3; ++comp) {
       // window faces of component comp inside this tile
-      int f0[3] = {max(i0, d.wi0), max(j0, d.wj0), max(k0, d.wk0)};
-      int f1[3] = {min(comp == 0 ? face_end(i0, S::TX, nx) : ie, d.wi1 + (comp == 0)),
-                   min(comp == 1 ? face_end(j0, S::TY, ny) : je, d.wj1 + (comp == 1)),
-                   min(comp == 2 ? face_end(k0, S::TZ, nz) : ke, d.wk1 + (comp == 2))};
+      const int f0[3] = {max(i0, d.wi0), max(j0, d.wj0), max(k0, d.wk0)};
+      const int f1[3] = {min(comp == 0 ? face_end(i0, S::TX, nx) : ie, d.wi1 + (comp == 0)),
+                         min(comp == 1 ? face_end(j0, S::TY, ny) : je, d.wj1 + (comp == 1)),
+                         min(comp == 2 ? face_end(k0, S::TZ, nz) : ke, d.wk1 + (comp == 2))};
       const int n0 = f1[0] - f0[0], n1 = f1[1] - f0[1], n2 = f1[2] - f0[2];
-      if (n0 > 0 && n1 > 0 && n2 > 0) {
-        const int ex = d.wx + (comp == 0), ey = d.wy + (comp == 1), ez = d.wz + (comp == 2);
-        const size_t wn = (size_t)ex * ey * ez;
-        const T *kk = (comp == 0 ? c.k_u : comp == 1 ? c.k_v : c.k_w) + ((size_t)b * 3 + m) * wn;
-        const T *ac = comp == 0 ? d.acc_u : comp == 1 ? d.acc_v : d.acc_w;
-        for (int e = threadIdx.x; e < n0 * n1 * n2; e += kTileThreads) {
+      const int nf = (n0 > 0 && n1 > 0 && n2 > 0) ? n0 * n1 * n2 : 0;
+      const int ex = d.wx + (comp == 0), ey = d.wy + (comp == 1), ez = d.wz + (comp == 2);
+      const size_t wn = (size_t)ex * ey * ez;
+      const T *ac = comp == 0 ? d.acc_u : comp == 1 ? d.acc_v : d.acc_w;
+      const T *kb = comp == 0 ? c.k_u : comp == 1 ? c.k_v : c.k_w;
+      for (int b = 0; b < d.nbody; ++b) {
+        const T *k0w = kb + (size_t)b * 3 * wn;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        for (int e = threadIdx.x; e < nf; e += kTileThreads) {
           const int x = e % n0, t = e / n0, y = t % n1, z = t / n1;
           const size_t wi = ((size_t)(f0[2] + z - d.wk0) * ey + (f0[1] + y - d.wj0)) * ex + (f0[0] + x - d.wi0);
-          acc += (double)kk[wi] * (double)ac[wi];
+          const double av = (double)ac[wi];
+          a0 += (double)k0w[wi] * av;
+          a1 += (double)k0w[wn + wi] * av;
+          a2 += (double)k0w[2 * wn + wi] * av;
+        }
+        a0 = wave_sum(a0);
+        a1 = wave_sum(a1);
+        a2 = wave_sum(a2);
+        if (lane == 0) {
+          double *o = fr + wv * kMaxQ + b * 9 + comp * 3;
+          o[0] = a0;
+          o[1] = a1;
+          o[2] = a2;
         }
       }
-      acc = wave_sum(acc);
-      if (lane == 0) L.red[wv * 16 + (q & 15)] = acc;
-      if ((q & 15) == 15 || q == d.nq - 1) {
-        __syncthreads();
-        const int base = q & ~15;
-        if (threadIdx.x <= q - base) {
-          double r = L.red[threadIdx.x];
-          for (int w2 = 1; w2 < kTileWaves; ++w2) r += L.red[w2 * 16 + threadIdx.x];
-          d.pforce[(size_t)wt * d.nq + base + threadIdx.x] = r;
-        }
-        __syncthreads();
-      }
+    }
+    __syncthreads();
+    if (threadIdx.x < d.nq) {
+      double r = fr[threadIdx.x];
+      for (int w2 = 1; w2 < kTileWaves; ++w2) r += fr[w2 * kMaxQ + threadIdx.x];
+      d.pforce[(size_t)wt * d.nq + threadIdx.x] = r;
     }
   }
 }
@@ -1501,7 +1522,7 @@ __global__ void __launch_bounds__(kThreads) k_p1(const Dev3<T> *__restrict__ dp,
   }
   const int r0 = 2 * ns * blockIdx.x;
   for (int e = threadIdx.x; e < ns * nx; e += kThreads) {
-    const int sl = e / nx, n = e - sl * nx;
+    const int sl = fdiv(e, nx, log2_or_neg(nx)), n = e - sl * nx;
     const int ra = r0 + 2 * sl, rb = ra + 1;
     const int q = d.dct[0] ? makhoul_src(n, nx) : n;
     T xa = 0, xb = 0;
@@ -1517,7 +1538,7 @@ __global__ void __launch_bounds__(kThreads) k_p1(const Dev3<T> *__restrict__ dp,
   __syncthreads();
   const cplx<T> *R = fft_lds(A, B, d.fp[0], TW, ns, nx);
   for (int e = threadIdx.x; e < ns * nx; e += kThreads) {
-    const int sl = e / nx, k = e - sl * nx;
+    const int sl = fdiv(e, nx, log2_or_neg(nx)), k = e - sl * nx;
     const int ra = r0 + 2 * sl, rb = ra + 1;
     T xa, xb;
     fwd_split(R + sl * nx, nx, k, d.dct[0], d.dct[0] ? d.cs[0][k] : cplx<T>{0, 0}, xa, xb);
@@ -1542,7 +1563,7 @@ __global__ void __launch_bounds__(kThreads) k_py(const Dev3<T> *__restrict__ dp,
   const int dct = d.dct[1];
   if (!inverse) {
     for (int e = threadIdx.x; e < ncol * ny; e += kThreads) {
-      const int n = e / ncol, col = e - n * ncol;
+      const int n = fdiv(e, ncol, log2_or_neg(ncol)), col = e - n * ncol;
       const int q = dct ? makhoul_src(n, ny) : n;
       const T val = c0 + col < nx ? base[(size_t)q * nx + col] : (T)0;
       reinterpret_cast<T *>(&A[(col >> 1) * st + n])[col & 1] = val;
@@ -1550,7 +1571,7 @@ __global__ void __launch_bounds__(kThreads) k_py(const Dev3<T> *__restrict__ dp,
     __syncthreads();
     const cplx<T> *R = fft_lds(A, B, d.fp[1], TW, ns, st);
     for (int e = threadIdx.x; e < ns * ny; e += kThreads) {
-      const int sl = e / ny, kk = e - sl * ny;
+      const int sl = fdiv(e, ny, log2_or_neg(ny)), kk = e - sl * ny;
       T xa, xb;
       fwd_split(R + sl * st, ny, kk, dct, dct ? d.cs[1][kk] : cplx<T>{0, 0}, xa, xb);
       X[(2 * sl) * st + kk] = xa;
@@ -1558,25 +1579,25 @@ __global__ void __launch_bounds__(kThreads) k_py(const Dev3<T> *__restrict__ dp,
     }
     __syncthreads();
     for (int e = threadIdx.x; e < ncol * ny; e += kThreads) {
-      const int kk = e / ncol, col = e - kk * ncol;
+      const int kk = fdiv(e, ncol, log2_or_neg(ncol)), col = e - kk * ncol;
       if (c0 + col < nx) base[(size_t)kk * nx + col] = X[col * st + kk];
     }
     return;
   }
   for (int e = threadIdx.x; e < ncol * ny; e += kThreads) {
-    const int kk = e / ncol, col = e - kk * ncol;
+    const int kk = fdiv(e, ncol, log2_or_neg(ncol)), col = e - kk * ncol;
     X[col * st + kk] = c0 + col < nx ? base[(size_t)kk * nx + col] : (T)0;
   }
   __syncthreads();
   for (int e = threadIdx.x; e < ns * ny; e += kThreads) {
-    const int sl = e / ny, kk = e - sl * ny;
+    const int sl = fdiv(e, ny, log2_or_neg(ny)), kk = e - sl * ny;
     A[sl * st + kk] = pack_inv(X + (2 * sl) * st, X + (2 * sl + 1) * st, ny, dct, dct ? d.cs[1][kk] : cplx<T>{0, 0}, kk);
   }
   __syncthreads();
   const cplx<T> *R = fft_lds(A, B, d.fp[1], TW, ns, st);
   const T inv_n = (T)(1.0 / (double)ny);
   for (int e = threadIdx.x; e < ns * ny; e += kThreads) {
-    const int sl = e / ny, m = e - sl * ny;
+    const int sl = fdiv(e, ny, log2_or_neg(ny)), m = e - sl * ny;
     T va, vb;
     inv_unpack(R + sl * st, ny, m, dct, inv_n, va, vb);
     const int q = dct ? makhoul_src(m, ny) : m;
@@ -1585,7 +1606,7 @@ __global__ void __launch_bounds__(kThreads) k_py(const Dev3<T> *__restrict__ dp,
   }
   __syncthreads();
   for (int e = threadIdx.x; e < ncol * ny; e += kThreads) {
-    const int q = e / ncol, col = e - q * ncol;
+    const int q = fdiv(e, ncol, log2_or_neg(ncol)), col = e - q * ncol;
     if (c0 + col < nx) base[(size_t)q * nx + col] = X[col * st + q];
   }
 }
@@ -1606,38 +1627,35 @@ __global__ void __launch_bounds__(kThreads) k_pz(const Dev3<T> *__restrict__ dp)
   T *base = d.F + (size_t)j * nx + c0;
   const int dct = d.dct[2];
   for (int e = threadIdx.x; e < ncol * nz; e += kThreads) {
-    const int n = e / ncol, col = e - n * ncol;
+    const int n = fdiv(e, ncol, log2_or_neg(ncol)), col = e - n * ncol;
     const int q = dct ? makhoul_src(n, nz) : n;
     const T val = c0 + col < nx ? base[(size_t)q * pl + col] : (T)0;
     reinterpret_cast<T *>(&A[(col >> 1) * st + n])[col & 1] = val;
   }
   __syncthreads();
   const cplx<T> *R = fft_lds(A, B, d.fp[2], TW, ns, st);
-  const double ly = d.lam[1][j];
+  const T *inv = d.inv_lam + (size_t)j * nx * nz;
   for (int e = threadIdx.x; e < ns * nz; e += kThreads) {
-    const int sl = e / nz, kz = e - sl * nz;
+    const int sl = fdiv(e, nz, log2_or_neg(nz)), kz = e - sl * nz;
     T xa, xb;
     fwd_split(R + sl * st, nz, kz, dct, dct ? d.cs[2][kz] : cplx<T>{0, 0}, xa, xb);
     const int ca = c0 + 2 * sl, cb = ca + 1;
-    const double lz = d.lam[2][kz] + ly;
-    const double la = lz + d.lam[0][ca < nx ? ca : 0];
-    const double lb = lz + d.lam[0][cb < nx ? cb : 0];
-    const double ia = (la != 0.0 && !(ca == 0 && j == 0 && kz == 0)) ? 1.0 / la : 0.0;
-    const double ib = lb != 0.0 ? 1.0 / lb : 0.0;
-    X[(2 * sl) * st + kz] = xa * (T)ia;
-    X[(2 * sl + 1) * st + kz] = xb * (T)ib;
+    const T ia = ca < nx ? inv[(size_t)ca * nz + kz] : (T)0;
+    const T ib = cb < nx ? inv[(size_t)cb * nz + kz] : (T)0;
+    X[(2 * sl) * st + kz] = xa * ia;
+    X[(2 * sl + 1) * st + kz] = xb * ib;
   }
   __syncthreads();
   cplx<T> *Z = (R == A) ? B : A;
   for (int e = threadIdx.x; e < ns * nz; e += kThreads) {
-    const int sl = e / nz, kz = e - sl * nz;
+    const int sl = fdiv(e, nz, log2_or_neg(nz)), kz = e - sl * nz;
     Z[sl * st + kz] = pack_inv(X + (2 * sl) * st, X + (2 * sl + 1) * st, nz, dct, dct ? d.cs[2][kz] : cplx<T>{0, 0}, kz);
   }
   __syncthreads();
   const cplx<T> *R2 = fft_lds(Z, (Z == A) ? B : A, d.fp[2], TW, ns, st);
   const T inv_n = (T)(1.0 / (double)nz);
   for (int e = threadIdx.x; e < ns * nz; e += kThreads) {
-    const int sl = e / nz, m = e - sl * nz;
+    const int sl = fdiv(e, nz, log2_or_neg(nz)), m = e - sl * nz;
     T va, vb;
     inv_unpack(R2 + sl * st, nz, m, dct, inv_n, va, vb);
     const int q = dct ? makhoul_src(m, nz) : m;
@@ -1646,7 +1664,7 @@ __global__ void __launch_bounds__(kThreads) k_pz(const Dev3<T> *__restrict__ dp)
   }
   __syncthreads();
   for (int e = threadIdx.x; e < ncol * nz; e += kThreads) {
-    const int q = e / ncol, col = e - q * ncol;
+    const int q = fdiv(e, ncol, log2_or_neg(ncol)), col = e - q * ncol;
     if (c0 + col < nx) base[(size_t)q * pl + col] = X[col * st + q];
   }
 }
@@ -1667,12 +1685,12 @@ __global__ void __launch_bounds__(kThreads) k_p5(const Dev3<T> *__restrict__ dp,
   const int r0 = 2 * ns * blockIdx.x;
   const int dct = d.dct[0];
   for (int e = threadIdx.x; e < 2 * ns * nx; e += kThreads) {
-    const int rr = e / nx, k = e - rr * nx;
+    const int rr = fdiv(e, nx, log2_or_neg(nx)), k = e - rr * nx;
     X[rr * st + k] = r0 + rr < nrows ? d.F[(size_t)(r0 + rr) * nx + k] : (T)0;
   }
   __syncthreads();
   for (int e = threadIdx.x; e < ns * nx; e += kThreads) {
-    const int sl = e / nx, k = e - sl * nx;
+    const int sl = fdiv(e, nx, log2_or_neg(nx)), k = e - sl * nx;
     A[sl * st + k] = pack_inv(X + (2 * sl) * st, X + (2 * sl + 1) * st, nx, dct, dct ? d.cs[0][k] : cplx<T>{0, 0}, k);
   }
   __syncthreads();
@@ -1682,7 +1700,7 @@ __global__ void __launch_bounds__(kThreads) k_p5(const Dev3<T> *__restrict__ dp,
   if (p && stage == 2) rdt = (T)1 / step_dt(d, c);
   const T w = (T)kForceW[stage];
   for (int e = threadIdx.x; e < ns * nx; e += kThreads) {
-    const int sl = e / nx, m = e - sl * nx;
+    const int sl = fdiv(e, nx, log2_or_neg(nx)), m = e - sl * nx;
     T va, vb;
     inv_unpack(R + sl * st, nx, m, dct, inv_n, va, vb);
     const int q = dct ? makhoul_src(m, nx) : m;
@@ -1691,7 +1709,7 @@ __global__ void __launch_bounds__(kThreads) k_p5(const Dev3<T> *__restrict__ dp,
   }
   __syncthreads();
   for (int e = threadIdx.x; e < 2 * ns * nx; e += kThreads) {
-    const int rr = e / nx, q = e - rr * nx;
+    const int rr = fdiv(e, nx, log2_or_neg(nx)), q = e - rr * nx;
     if (r0 + rr >= nrows) continue;
     const size_t idx = (size_t)(r0 + rr) * nx + q;
     const T ps = X[rr * st + q];
@@ -1768,6 +1786,7 @@ struct aero3d_plan {
   void *old_u, *old_v, *old_w, *old_p, *old_t, *pdiag;
   void *tw[3], *cs[3];
   double *lam[3];
+  void *inv_lam;
   FftPlan fp[3];
   int nbx, nby, nbz, nblk, nwt, nq;
   int ns1, ns2, ns3, ns5;
@@ -1800,6 +1819,7 @@ int upload3(aero3d_plan *p) {
   const aero3d_desc &s = p->desc;
   const int n3[3] = {s.nx, s.ny, s.nz};
   const int per[3] = {s.periodic_x, s.periodic_y, s.periodic_z};
+  std::vector<double> lams[3];
   for (int a = 0; a < 3; ++a) {
     const int n = n3[a];
     std::vector<cplx<T>> tw(n), cs(n);
@@ -1816,6 +1836,20 @@ int upload3(aero3d_plan *p) {
     AERO_HIP_CHECK(hipMemcpy(p->tw[a], tw.data(), n * sizeof(cplx<T>), hipMemcpyHostToDevice));
     AERO_HIP_CHECK(hipMemcpy(p->cs[a], cs.data(), n * sizeof(cplx<T>), hipMemcpyHostToDevice));
     AERO_HIP_CHECK(hipMemcpy(p->lam[a], lam.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    lams[a] = lam;
+  }
+  {
+    // the spectral scaling 1 / (lam_x + lam_y + lam_z), formed in float64 in the order of
+    // ns3d/poisson.py:115-125 and rounded once; the zero mode is 0
+    const int nx = s.nx, ny = s.ny, nz = s.nz;
+    std::vector<T> inv((size_t)nx * ny * nz);
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i)
+        for (int k = 0; k < nz; ++k) {
+          const double la = (lams[2][k] + lams[1][j]) + lams[0][i];
+          inv[((size_t)j * nx + i) * nz + k] = (T)((la != 0.0 && !(i == 0 && j == 0 && k == 0)) ? 1.0 / la : 0.0);
+        }
+    AERO_HIP_CHECK(hipMemcpy(p->inv_lam, inv.data(), inv.size() * sizeof(T), hipMemcpyHostToDevice));
   }
   Dev3<T> d;
   std::memset(&d, 0, sizeof(d));
@@ -1898,6 +1932,7 @@ int upload3(aero3d_plan *p) {
     d.tw[a] = (const cplx<T> *)p->tw[a];
     d.cs[a] = (const cplx<T> *)p->cs[a];
     d.lam[a] = p->lam[a];
+    d.inv_lam = (const T *)p->inv_lam;
     d.fp[a] = p->fp[a];
     d.dct[a] = !per[a];
   }
@@ -1959,11 +1994,13 @@ int upload3(aero3d_plan *p) {
 }
 
 // largest slot count (<= cap) whose LDS use stays within 48 KB
+// Largest power-of-two slot count (<= cap) whose LDS use stays within 64 KB: two columns per
+// slot, so 16 slots of float (8 of double) read and write whole 128-byte lines.
 size_t pick_slots(int n, size_t es, int extra_reals, int cap, int &ns) {
   const size_t c = 2 * es;   // sizeof(cplx<T>)
   ns = cap;
   auto need = [&](int k) { return 2 * (size_t)k * (n + 1) * c + (size_t)n * c + (size_t)extra_reals * k * (n + 1) * es + 128; };
-  while (ns > 1 && need(ns) > 48 * 1024) --ns;
+  while (ns > 1 && need(ns) > 64 * 1024) ns >>= 1;
   return need(ns);
 }
 
@@ -2392,7 +2429,8 @@ int aero3d_plan_create(const aero3d_desc *desc, aero3d_plan **out) {
       {&p->pdiag, desc->diagnostics ? (size_t)kDiagBlocks * kDV * 8 : 0},
       {&p->tw[0], nx * 2 * es}, {&p->tw[1], ny * 2 * es}, {&p->tw[2], nz * 2 * es},
       {&p->cs[0], nx * 2 * es}, {&p->cs[1], ny * 2 * es}, {&p->cs[2], nz * 2 * es},
-      {(void **)&p->lam[0], nx * 8}, {(void **)&p->lam[1], ny * 8}, {(void **)&p->lam[2], nz * 8}};
+      {(void **)&p->lam[0], nx * 8}, {(void **)&p->lam[1], ny * 8}, {(void **)&p->lam[2], nz * 8},
+      {&p->inv_lam, nc * es}};
   for (auto &t : tab) {
     if ((rc = alloc3(p, t.ptr, t.n))) {
       aero3d_plan_destroy(p);

@@ -18,7 +18,8 @@
  *                          called as ns2d/solver.py:384-400 _step_static does
  *   aero2d_project       - ns2d/core.py:626-642 project
  *   aero2d_poisson_solve - ns2d/poisson.py:128-134 PoissonSolver.solve
- *   aero2d_ensemble_*    - no counterpart (batched sweep; the reference loops over solves)
+ *   aero2d_ensemble_*    - no counterpart (batched sweep; the reference loops over solves;
+ *                          aero2d_ensemble_tune picks its launch shapes)
  *   aero3d_plan_create   - ns3d/core.py:184-203 make_operators3d (static operators,
  *                          ghost-rule table core.py:147-181, PoissonSolver3D
  *                          ns3d/poisson.py:107-126)
@@ -90,6 +91,8 @@ typedef struct {
   double gravity_x, gravity_y;
   double prt;              /* turbulent Prandtl number */
   double diff_coef;        /* largest temperature diffusion coefficient / (kappa / h^2) */
+  int32_t tile_rows;       /* rows per 64-wide tile of the cell kernels: 0 (default, 8), 4, 8 or 16 */
+  int32_t reserved;
 } aero2d_desc;
 
 /* Evolving fields, caller-owned device buffers of the plan's dtype.
@@ -314,6 +317,15 @@ int aero2d_ensemble_create(int32_t n, aero2d_plan *const *plans, const aero2d_fi
 /* One step of every run; chained as for aero2d_step_chained (each run's dt inputs from its
  * previous ensemble step). */
 int aero2d_ensemble_step(aero2d_ensemble *ensemble, void *stream, int32_t chained);
+/* Autotuner of the batched launches (no reference counterpart): times every candidate tile
+ * shape (4, 8 or 16 rows per 64-wide cell tile) x pass-B column slots (1, 2 or 4 column pairs
+ * per block, capped per run by LDS) on RK stages 0 and 1 of every run, `iters` rounds each,
+ * and rebuilds the work tables with the fastest. The timed launches write only plan scratch,
+ * so the runs' state and outputs are untouched (the potential buffer aside); fields, u and v
+ * of later steps do not depend on the choice, force sums round differently with the tile
+ * rows. choice[0..2] (may be NULL) receives the rows, the slots and the candidates timed.
+ * Synchronises the stream; call outside graph capture. */
+int aero2d_ensemble_tune(aero2d_ensemble *ensemble, void *stream, int32_t iters, int32_t *choice);
 int aero2d_ensemble_destroy(aero2d_ensemble *ensemble);
 
 /* ------------------------------------------------------------------ learned closure (3D CNN) */

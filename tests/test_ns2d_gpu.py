@@ -245,3 +245,25 @@ def test_periodic_taylor_green_matches_oracle():
         cur, _ = S.step_full(cur, geom, ops)
     assert _err(cur.u, ref.u) < 1e-12
     assert _err(cur.p, ref.p) < 1e-10
+
+
+@pytest.mark.parametrize("rows", [4, 16])
+def test_tile_rows_change_no_field(rows):
+    """The cell kernels' tile height (4 or 16 rows against the default 8) changes no field bit:
+    heated airfoil with Smagorinsky LES and a smoke emitter on 160 x 75 cells (a ragged last
+    tile row), float64; the force sums, summed over other tiles, agree to round-off."""
+    _need_gpu()
+    import dataclasses
+    print(f"tile rows {rows}", flush=True)
+    S = _S()
+    cfg = dataclasses.replace(cases.c2_cfg(nx=160, ny=75, dtype="float64", device="cuda"), use_cuda_graphs=False)
+    a, b = S.FlowSolver(cfg), S.FlowSolver(cfg, tile_rows=rows)
+    a.advance(12)
+    b.advance(12)
+    torch.cuda.synchronize()
+    for f in ("u", "v", "p", "t", "dt", "theta", "dye"):
+        assert torch.equal(getattr(a.state, f), getattr(b.state, f)), f
+    ha, hb = a.history(), b.history()
+    for k in ha:
+        assert np.allclose(ha[k], hb[k], rtol=1e-12, atol=1e-14), k
+    assert float(a.state.u.abs().max()) > 0.5

@@ -38,6 +38,10 @@ __device__ __forceinline__ cplx<T> cadd(cplx<T> a, cplx<T> b) { return {a.x + b.
 template <typename T>
 __device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T> b) { return {a.x - b.x, a.y - b.y}; }
 
+// LDS a workgroup may allocate on gfx950 (160 KB per CU, all of it to one workgroup);
+// dynamic allocations above 64 KB need hipFuncAttributeMaxDynamicSharedMemorySize.
+constexpr size_t kLdsBlockMax = 163840;
+
 // ------------------------------------------------------------------ FFT plan
 // Radix sequence of an N-point transform and the twiddle table exp(-2 pi i m / N).
 constexpr int kMaxRadices = 24;

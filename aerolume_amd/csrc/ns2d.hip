@@ -1688,6 +1688,33 @@ int upload_tables(aero2d_plan *p) {
 }
 
 template <typename T>
+size_t lds_A(const aero2d_plan *p);
+template <typename T>
+size_t lds_B(const aero2d_plan *p);
+
+// Rows of more than ~1000 float64 values take more than 64 KB of LDS in the x passes: the
+// transform kernels are then allowed the device's whole per-block LDS (one value for every
+// plan, so that plans of different sizes never lower each other's limit).
+template <typename T>
+int lds_limits(const aero2d_plan *p) {
+  const size_t a = lds_A<T>(p), b = lds_B<T>(p);
+  if (a <= 65536 && b <= 65536) return AERO_OK;
+  int dev = 0, max_lds = 0;
+  AERO_HIP_CHECK(hipGetDevice(&dev));
+  AERO_HIP_CHECK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  if (a > (size_t)max_lds || b > (size_t)max_lds) {
+    set_error("the device offers " + std::to_string(max_lds) + " B of LDS per block; the transforms need " +
+              std::to_string(std::max(a, b)));
+    return AERO_EUNSUPPORTED;
+  }
+  const void *fns[] = {(const void *)k_poisA<T>, (const void *)k_poisB<T>, (const void *)k_poisC<T>,
+                       (const void *)kb_poisA<T>, (const void *)kb_poisB<T>, (const void *)kb_poisC<T>};
+  for (const void *f : fns)
+    AERO_HIP_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+  return AERO_OK;
+}
+
+template <typename T>
 Dev<T> make_dev(const aero2d_plan *p, const aero2d_geom *g, const aero2d_params *pr) {
   const aero2d_desc &s = p->desc;
   Dev<T> d;
@@ -1909,7 +1936,10 @@ int check_desc(const aero2d_desc *d) {
   const size_t es = d->dtype ? 8 : 4;
   const size_t ldsx = 3 * d->nx * 2 * es + 2 * d->nx * es + 512;
   const size_t ldsy = d->ny * (2 * 2 * es + 2 * es) + d->ny * 2 * es;
-  if (ldsx > 65536 || ldsy > 65536) { set_error("grid too large for the LDS-resident transforms"); return AERO_EUNSUPPORTED; }
+  if (ldsx > kLdsBlockMax || ldsy > kLdsBlockMax) {
+    set_error("grid too large for the LDS-resident transforms");
+    return AERO_EUNSUPPORTED;
+  }
   if (d->n_groups > AERO2D_MAX_GROUPS || d->n_groups < 0) { set_error("too many emitter groups"); return AERO_EUNSUPPORTED; }
   if (d->dye_mode < 0 || d->dye_mode > 2) { set_error("unknown dye mode"); return AERO_EINVAL; }
   if (d->tile_rows != 0 && d->tile_rows != 4 && d->tile_rows != 8 && d->tile_rows != 16) {
@@ -2018,6 +2048,7 @@ int aero2d_plan_create(const aero2d_desc *desc, aero2d_plan **out) {
     }
   }
   rc = p->dbl ? upload_tables<double>(p) : upload_tables<float>(p);
+  if (!rc) rc = p->dbl ? lds_limits<double>(p) : lds_limits<float>(p);
   if (rc) {
     aero2d_plan_destroy(p);
     return rc;

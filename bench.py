@@ -210,6 +210,68 @@ def gpu_run_ensemble(steps: int, warmup: int, n: int = 64, seed: int = 404):
                                    "alpha U[-5, 15] deg, Re 10^U[3, 5], upwind3, no LES, FP32", "seed": seed}}
 
 
+def _rel_err(a, b) -> dict:
+    d = (a.double() - b.double()).abs()
+    scale = float(b.double().abs().max())
+    return {"max_rel": float(d.max()) / max(scale, 1e-300),
+            "l2_rel": float(d.norm()) / max(float(b.double().norm()), 1e-300)}
+
+
+def _fp64_report(s32, s64, fields, steps) -> dict:
+    out = {"steps": steps, "t_fp64": float(s64.state.t), "t_rel": abs(float(s32.state.t) / float(s64.state.t) - 1.0)}
+    for f in fields:
+        err = _rel_err(getattr(s32.state, f), getattr(s64.state, f))
+        out[f] = {k: float(f"{v:.3e}") for k, v in err.items()}
+    out["t_rel"] = float(f"{out['t_rel']:.3e}")
+    return out
+
+
+def fp64_check_2d(steps: int = 200) -> dict:
+    """The 2D workload in FP32 against the same kernels in FP64 from the same start (the FP64
+    projected seed, rounded for the FP32 run), ``steps`` steps each on its own CFL dt: the FP32
+    result's relative error (max and L2 over the field) and its time drift."""
+    import torch
+
+    from aerolume_amd.ns2d import solver as S
+    s64 = S.FlowSolver(c2_config(dtype="float64"))
+    ops = s64.ops
+    u, v = noisy_velocity(ops, SEED)
+    u, v = S.apply_boundary_values(torch.as_tensor(u, device=ops.device, dtype=ops.dtype),
+                                   torch.as_tensor(v, device=ops.device, dtype=ops.dtype), ops)
+    u, v, _ = S.project(u, v, ops)
+    s64.set_velocity(u, v)
+    s32 = S.FlowSolver(c2_config(dtype="float32"))
+    s32.set_velocity(u.float(), v.float())
+    s64.advance(steps)
+    s32.advance(steps)
+    torch.cuda.synchronize()
+    out = _fp64_report(s32, s64, ("u", "v", "p", "theta"), steps)
+    c32, c64 = s32.coefficients(), s64.coefficients()
+    out["coefficients_max_abs"] = float(f"{float(abs(c32 - c64).max()):.3e}")
+    return out
+
+
+def fp64_check_3d(steps: int = 100) -> dict:
+    """The 3D car workload in FP32 against the same kernels in FP64 from the same start."""
+    import torch
+
+    from aerolume_amd.ns3d import solver as S
+    from tests.cases import c3_cfg
+    s64 = S.FlowSolver3D(c3_cfg("float64", device="cuda"))
+    ops = s64.ops
+    dev = ops.device
+    u, v, w = (torch.as_tensor(a, device=dev, dtype=ops.dtype) for a in noisy_velocity3d(ops, SEED3))
+    u, v, w = S.apply_boundary_values(u, v, w, ops)
+    u, v, w, _ = S.project3d(u.contiguous(), v.contiguous(), w.contiguous(), ops)
+    s64.set_velocity(u, v, w)
+    s32 = S.FlowSolver3D(c3_cfg("float32", device="cuda"))
+    s32.set_velocity(u.float(), v.float(), w.float())
+    s64.advance(steps)
+    s32.advance(steps)
+    torch.cuda.synchronize()
+    return _fp64_report(s32, s64, ("u", "v", "w", "p"), steps)
+
+
 def noisy_velocity3d(ops, seed):
     import numpy as np
     rng = np.random.default_rng(seed)
@@ -355,8 +417,9 @@ def main() -> int:
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
         line["gpu_over_cpu"] = round(value / line["cpu_baseline"]["value"], 1)
+    del solver
+    line["fp64_check"] = fp64_check_2d()
     if not args.no_3d:
-        del solver
         s3, wall3, stage3_ms = gpu_run_3d(args.steps3d, args.warmup3d)
         cells3 = NX3 * NY3 * NZ3
         v3 = cells3 * args.steps3d / wall3 / 1e6
@@ -376,6 +439,7 @@ def main() -> int:
               "step_hbm": {"algorithmic_bytes_per_step": STEP3_BYTES_PER_CELL * cells3,
                            "achieved_gbs": round(step3, 1), "frac": round(step3 / HBM_PEAK_GBS, 4)}}
         del s3
+        w3["fp64_check"] = fp64_check_3d()
         if not args.no_cpu:
             w3["cpu_baseline"] = cpu_baseline_3d()
             w3["gpu_over_cpu"] = round(v3 / w3["cpu_baseline"]["value"], 1)

@@ -48,9 +48,11 @@ def _err(a, b):
 
 @pytest.mark.parametrize("nx,ny,periodic,dtype", [(64, 32, False, "float64"), (30, 20, False, "float64"),
                                                   (48, 40, True, "float64"), (64, 32, False, "float32"),
-                                                  (30, 20, True, "float32")])
+                                                  (30, 20, True, "float32"), (1024, 36, False, "float64"),
+                                                  (1200, 24, True, "float64")])
 def test_poisson_solve_matches_oracle(nx, ny, periodic, dtype):
-    """Exact DCT/DHT solve on the GPU equals the oracle's and satisfies L psi = f - mean."""
+    """Exact DCT/DHT solve on the GPU equals the oracle's and satisfies L psi = f - mean; rows of
+    1024 and 1200 float64 values take more than 64 KB of LDS in the x passes."""
     _need_gpu()
     from aerolume_amd.ns2d.config import SolverConfig
     from oracle import ns2d as O

@@ -228,22 +228,65 @@ __device__ T block_max(T v, T *lds) {
   return r;
 }
 
-// Wavefront (64-lane) reductions by butterfly shuffles: every lane gets the result, and the
-// combination order is fixed, so results are deterministic.
+// Wavefront (64-lane) reductions through DPP lane moves (register-to-register, no LDS round
+// trips): pairs, quads, half rows and rows by quad permutes and mirrors, then the row
+// broadcasts 15 and 31 carry rows 0 and 2 into rows 1 and 3 and row 1 into rows 2 and 3;
+// lane 63 ends with the total, which is returned to every lane. The combination order is
+// fixed, so results are deterministic. Call from uniform control flow (all 64 lanes active).
+namespace dpp {
+constexpr int kPairs = 0xb1;      // quad_perm [1, 0, 3, 2]
+constexpr int kQuads = 0x4e;      // quad_perm [2, 3, 0, 1]
+constexpr int kRowMirror = 0x140;
+constexpr int kHalfMirror = 0x141;
+constexpr int kBcast15 = 0x142;
+constexpr int kBcast31 = 0x143;
+// v moved by CTRL; lanes outside ROWS (a 4-bit row mask) get `old`
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ float mov(float v, float old) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), CTRL, ROWS, 0xf, false));
+}
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ double mov(double v, double old) {
+  const long long b = __double_as_longlong(v), o = __double_as_longlong(old);
+  const int lo = __builtin_amdgcn_update_dpp((int)(unsigned)(o & 0xffffffffLL), (int)(unsigned)(b & 0xffffffffLL), CTRL,
+                                             ROWS, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(b >> 32), CTRL, ROWS, 0xf, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ float lane63(float v) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63)); }
+__device__ __forceinline__ double lane63(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffLL), 63);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+}  // namespace dpp
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  v = v + dpp::mov<dpp::kPairs>(v, (T)0);
+  v = v + dpp::mov<dpp::kQuads>(v, (T)0);
+  v = v + dpp::mov<dpp::kHalfMirror>(v, (T)0);
+  v = v + dpp::mov<dpp::kRowMirror>(v, (T)0);
+  v = v + dpp::mov<dpp::kBcast15, 0xa>(v, (T)0);
+  v = v + dpp::mov<dpp::kBcast31, 0xc>(v, (T)0);
+  return dpp::lane63(v);
 }
 template <typename T>
 __device__ __forceinline__ T wave_max(T v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    const T w = __shfl_xor(v, o, 64);
-    v = w > v ? w : v;
-  }
-  return v;
+  T w = dpp::mov<dpp::kPairs>(v, v);
+  v = w > v ? w : v;
+  w = dpp::mov<dpp::kQuads>(v, v);
+  v = w > v ? w : v;
+  w = dpp::mov<dpp::kHalfMirror>(v, v);
+  v = w > v ? w : v;
+  w = dpp::mov<dpp::kRowMirror>(v, v);
+  v = w > v ? w : v;
+  w = dpp::mov<dpp::kBcast15, 0xa>(v, v);
+  v = w > v ? w : v;
+  w = dpp::mov<dpp::kBcast31, 0xc>(v, v);
+  v = w > v ? w : v;
+  return dpp::lane63(v);
 }
 // Block sum of `n` values per thread over a 256-thread block (4 wavefronts): wave sums, then
 // the four wave partials added in order. `lds` holds 4 * n doubles. Thread 0 gets the result.

@@ -1017,12 +1017,13 @@ __device__ __forceinline__ void poisB_body(const Dev<T> &d, int bx, int nslot) {
   T *X = reinterpret_cast<T *>(TW + ny);   // 2 * nslot * ny reals
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int c0 = 2 * nslot * bx;
-  load_twiddles(TW, d.tw_y, ny);
   cplx<T> csr[kMaxPer];
   T iar[kMaxPer], ibr[kMaxPer];
+  // twiddles, column elements, DCT twiddles and spectral scalings in one pass of loads
 #pragma unroll
   for (int r = 0; r < kMaxPer; ++r) {
     const int w = tid + nthr * r;
+    if (w < ny) TW[w] = d.tw_y[w];
     if (w < nslot * ny) {
       const int f = fdiv(w, ny, log2_or_neg(ny)), n = w - f * ny;
       const int ca = c0 + 2 * f, cb = ca + 1;
@@ -1108,15 +1109,24 @@ __device__ __forceinline__ void poisC_body(const Dev<T> &d, int bx, T *__restric
   T *X = reinterpret_cast<T *>(TW + nx);
   const int ja = 2 * bx, jb = ja + 1;
   const bool hasb = jb < d.ny;
-  load_twiddles(TW, d.tw_x, nx);
   cplx<T> csr[kMaxPer];
+  // previous pressure accumulator values (stages 1, 2), at the positions this thread writes
+  T pa[kMaxPer], pb[kMaxPer];
+  const bool acc_p = p != nullptr && stage > 0;
+  // twiddles, row elements, DCT twiddles and the accumulator in one pass of loads
 #pragma unroll
   for (int r = 0; r < kMaxPer; ++r) {
     const int k = threadIdx.x + kFftThreads * r;
     if (k < nx) {
+      TW[k] = d.tw_x[k];
       X[k] = d.F[ja * nx + k];
       X[nx + k] = hasb ? d.F[jb * nx + k] : (T)0;
       csr[r] = d.dct_x ? d.cs_x[k] : cplx<T>{0, 0};
+      if (acc_p) {
+        const int q = d.dct_x ? makhoul_src(k, nx) : k;
+        pa[r] = p[ja * nx + q];
+        pb[r] = hasb ? p[jb * nx + q] : (T)0;
+      }
     }
   }
   __syncthreads();
@@ -1131,19 +1141,6 @@ __device__ __forceinline__ void poisC_body(const Dev<T> &d, int bx, T *__restric
   T rdt = 0;
   if (p != nullptr && stage == 2) rdt = (T)1 / d.red->dt;
   const T w = (T)kForceW[stage];
-  // previous pressure accumulator values, loaded ahead of the FFT result
-  T pa[kMaxPer], pb[kMaxPer];
-  if (p != nullptr && stage > 0) {
-#pragma unroll
-    for (int r = 0; r < kMaxPer; ++r) {
-      const int m = threadIdx.x + kFftThreads * r;
-      if (m < nx) {
-        const int q = d.dct_x ? makhoul_src(m, nx) : m;
-        pa[r] = p[ja * nx + q];
-        pb[r] = hasb ? p[jb * nx + q] : (T)0;
-      }
-    }
-  }
 #pragma unroll
   for (int rr = 0; rr < kMaxPer; ++rr) {
     const int m = threadIdx.x + kFftThreads * rr;
@@ -1936,7 +1933,7 @@ int check_desc(const aero2d_desc *d) {
   const size_t es = d->dtype ? 8 : 4;
   const size_t ldsx = 3 * d->nx * 2 * es + 2 * d->nx * es + 512;
   const size_t ldsy = d->ny * (2 * 2 * es + 2 * es) + d->ny * 2 * es;
-  if (ldsx > kLdsBlockMax || ldsy > kLdsBlockMax) {
+  if (d->nx > kMaxPer * kFftThreads || d->ny > kMaxPer * kFftThreads || ldsx > kLdsBlockMax || ldsy > kLdsBlockMax) {
     set_error("grid too large for the LDS-resident transforms");
     return AERO_EUNSUPPORTED;
   }
@@ -2030,7 +2027,9 @@ int aero2d_plan_create(const aero2d_desc *desc, aero2d_plan **out) {
   // launch may run this plan with any tile shape
   const size_t blocks = (size_t)p->nbx * ((ny + 3) / 4);
   p->nslotB = 2;
-  while (p->nslotB > 1 && ny * p->nslotB * (2 * 2 * es + 2 * es) + ny * 2 * es > 65536) p->nslotB--;
+  while (p->nslotB > 1 &&
+         (ny * p->nslotB * (2 * 2 * es + 2 * es) + ny * 2 * es > 65536 || ny * p->nslotB > (size_t)kMaxPer * kFftThreads))
+    p->nslotB--;
   struct { void **ptr; size_t n; } tab[] = {
       {&p->uA, nu_ * es}, {&p->vA, nv_ * es}, {&p->uB, nu_ * es}, {&p->vB, nv_ * es}, {&p->uC, nu_ * es},
       {&p->vC, nv_ * es}, {&p->thA, desc->thermal ? nc * es : 0}, {&p->thB, desc->thermal ? nc * es : 0},

@@ -110,6 +110,7 @@ struct Dev3 {
   int theta, buoy, tside[6], tqk0, tqk1, tqj0, tqj1, tqi0, tqi1, tqwx, tqwy, tnt, tbody[kMaxT], tadia, tnskin;
   T ri, half_ri, ri_w[3], abs_ri, prt, kc, cadv, cfl, c6, h3;
   T *th_a, *th_b, *nut_c;
+  T *nu_pad;                 // nu_t of the last state the final kernel saw, cells -1..n per axis
   double *pheat;             // [3][window tiles][kMaxT] heat increments of held-temperature bodies
   double *pdiag;             // diagnostics partials
 };
@@ -133,6 +134,7 @@ struct Call3 {
   const T *th_sc, *th_side;  // [kappa, kappa / h, eta_t, kappa max|q|, heat scale, kappa / (h^2/6)]; side values
   int stage;
   int corr_stage;            // red->corr slot added at the outlet face of up (-1: none)
+  int nu_in;                 // stage input is the state the final kernel last saw: nu_t from d.nu_pad
 };
 
 template <typename T>
@@ -442,10 +444,34 @@ __device__ T nu_tile(const Dev3<T> &d, const Call3<T> &c, Lds3<T> &L, int k0, in
     const bool oi = (gi >= i0 && gi < ie) || gi == -1 || gi == d.nx;
     const bool oj = (gj >= j0 && gj < je) || gj == -1 || gj == d.ny;
     const bool ok = (gk >= k0 && gk < ke) || gk == -1 || gk == d.nz;
-    if (oi && oj && ok && gi <= ie && gj <= je && gk <= ke) mx = nu > mx ? nu : mx;
+    if (oi && oj && ok && gi <= ie && gj <= je && gk <= ke) {
+      mx = nu > mx ? nu : mx;
+      // the next step's first stage starts from this state: it reads these values back
+      if (own_only && d.nu_pad)
+        d.nu_pad[((size_t)(gk + 1) * (d.ny + 2) + (gj + 1)) * (d.nx + 2) + (gi + 1)] = nu;
+    }
   }
   __syncthreads();
   return mx;
+}
+
+// L.nu of a tile from d.nu_pad (the values nu_tile computes for the state the final kernel
+// last processed, tile by tile, ghost ring included); positions past the ring are unused.
+template <typename T>
+__device__ void nu_from_pad(const Dev3<T> &d, Lds3<T> &L, int k0, int j0, int i0, bool store) {
+  using S = Lds3<T>;
+  const int ie = min(i0 + S::TX, d.nx), je = min(j0 + S::TY, d.ny), ke = min(k0 + S::TZ, d.nz);
+  for (int e = threadIdx.x; e < S::NZ * S::NY * S::NX; e += kTileThreads) {
+    const int x = e % S::NX, t = e / S::NX, y = t % S::NY, z = t / S::NY;
+    const int gi = i0 + x - 1, gj = j0 + y - 1, gk = k0 + z - 1;
+    T nu = 0;
+    if (gi <= d.nx && gj <= d.ny && gk <= d.nz)
+      nu = d.nu_pad[((size_t)(gk + 1) * (d.ny + 2) + (gj + 1)) * (d.nx + 2) + (gi + 1)];
+    L.nu[z][y][x] = nu;
+    if (store && gi >= i0 && gi < ie && gj >= j0 && gj < je && gk >= k0 && gk < ke)
+      d.nut_c[((size_t)gk * d.ny + gj) * d.nx + gi] = nu;
+  }
+  __syncthreads();
 }
 
 // ------------------------------------------------------------------ right-hand side
@@ -549,7 +575,9 @@ __device__ __forceinline__ T rhs_w(const Dev3<T> &d, const Lds3<T> &L, int k, in
 __device__ __forceinline__ int face_end(int c0, int t, int n) { return c0 + t >= n ? n + 1 : c0 + t; }
 
 // ------------------------------------------------------------------ stage kernel
-template <typename T>
+// NU_IN: nu_t read from d.nu_pad (the first stage of a step whose start the final kernel
+// processed) instead of computed; a separate instantiation keeps the common path's registers.
+template <typename T, bool NU_IN>
 __global__ void __launch_bounds__(kTileThreads, Tile<T>::WAVES_PER_SIMD) k_stage(const Dev3<T> *__restrict__ dp, Call3<T> c) {
   using S = Lds3<T>;
   extern __shared__ unsigned char smem[];
@@ -562,7 +590,10 @@ __global__ void __launch_bounds__(kTileThreads, Tile<T>::WAVES_PER_SIMD) k_stage
   const int s = c.stage;
   const T corr = c.corr_stage >= 0 ? d.red->corr[c.corr_stage] : (T)0;
   load_tile(d, c, L, k0, j0, i0, corr);
-  if (d.turb) (void)nu_tile(d, c, L, k0, j0, i0, d.theta != 0);
+  if (d.turb) {
+    if constexpr (NU_IN) nu_from_pad(d, L, k0, j0, i0, d.theta != 0);
+    else (void)nu_tile(d, c, L, k0, j0, i0, d.theta != 0);
+  }
   const T dt = step_dt(d, c);
   const T a = (T)kSsp[s][0];
   const T bdt = dt * (T)kSsp[s][1];
@@ -1783,6 +1814,7 @@ struct aero3d_plan {
   void *dev = nullptr;
   void *uA, *vA, *wA, *uB, *vB, *wB, *F, *psi, *acc_u, *acc_v, *acc_w, *pmax, *pout, *pforce, *red, *dye_tmp;
   void *th_a, *th_b, *nut_c, *pheat;
+  void *nu_pad;
   void *old_u, *old_v, *old_w, *old_p, *old_t, *pdiag;
   void *tw[3], *cs[3];
   double *lam[3];
@@ -1971,6 +2003,7 @@ int upload3(aero3d_plan *p) {
     d.pheat = (double *)p->pheat;
   }
   d.pdiag = (double *)p->pdiag;
+  d.nu_pad = (T *)p->nu_pad;
   AERO_HIP_CHECK(hipMemcpy(p->dev, &d, sizeof(d), hipMemcpyHostToDevice));
   // the tile kernels take more than the default 64 KB of dynamic LDS
   int dev_id = 0, max_lds = 0;
@@ -1981,7 +2014,9 @@ int upload3(aero3d_plan *p) {
               std::to_string(sizeof(Lds3<T>)));
     return AERO_EUNSUPPORTED;
   }
-  AERO_HIP_CHECK(hipFuncSetAttribute((const void *)k_stage<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+  AERO_HIP_CHECK(hipFuncSetAttribute((const void *)k_stage<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)sizeof(Lds3<T>)));
+  AERO_HIP_CHECK(hipFuncSetAttribute((const void *)k_stage<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)sizeof(Lds3<T>)));
   AERO_HIP_CHECK(hipFuncSetAttribute((const void *)k_final<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)sizeof(Lds3<T>)));
@@ -2107,6 +2142,9 @@ int step_impl(aero3d_plan *p, hipStream_t st, const aero3d_fields *in, aero3d_fi
     c.wp = stage == 0 ? (const T *)in->w : wb[(stage - 1) & 1];
     c.psi = stage == 0 ? nullptr : (const T *)p->psi;
     c.corr_stage = stage == 0 ? -1 : stage - 1;
+    // the step start's nu_t was saved by this step's dt pass or, chained, by the previous
+    // step's final kernel
+    c.nu_in = (stage == 0 && p->nu_pad && (chained || !s.fixed_dt)) ? 1 : 0;
     c.uo = ub[stage & 1];
     c.vo = vb[stage & 1];
     c.wo = wb[stage & 1];
@@ -2114,13 +2152,15 @@ int step_impl(aero3d_plan *p, hipStream_t st, const aero3d_fields *in, aero3d_fi
       c.thp = stage == 0 ? (const T *)in->theta : (stage == 1 ? (const T *)p->th_a : (const T *)p->th_b);
       c.tho = stage == 0 ? (T *)p->th_a : (stage == 1 ? (T *)p->th_b : (T *)out->theta);
     }
-    hipLaunchKernelGGL(k_stage<T>, grid, blk, lds, st, dev, c);
+    if (c.nu_in) hipLaunchKernelGGL((k_stage<T, true>), grid, blk, lds, st, dev, c);
+    else hipLaunchKernelGGL((k_stage<T, false>), grid, blk, lds, st, dev, c);
     if (s.thermal) hipLaunchKernelGGL(k_theta<T>, grid, blk, sizeof(LdsT<T>), st, dev, c);
     const int mode = (!s.periodic_x && !s.closed) ? 2 : 1;
     int rc = poisson_passes<T>(p, st, dev, c, nullptr, mode, stage, (T *)p->psi, (T *)out->p);
     if (rc) return rc;
   }
   c.stage = 3;
+  c.nu_in = 0;
   c.up = ub[0];
   c.vp = vb[0];
   c.wp = wb[0];
@@ -2245,10 +2285,10 @@ int time_stage_impl(aero3d_plan *p, hipStream_t st, const aero3d_fields *fl, con
   AERO_HIP_CHECK(hipEventCreate(&e0));
   AERO_HIP_CHECK(hipEventCreate(&e1));
   const size_t lds = sizeof(Lds3<T>);
-  hipLaunchKernelGGL(k_stage<T>, dim3(p->nblk), dim3(kTileThreads), lds, st, dev, c);
+  hipLaunchKernelGGL((k_stage<T, false>), dim3(p->nblk), dim3(kTileThreads), lds, st, dev, c);
   AERO_HIP_CHECK(hipEventRecord(e0, st));
   for (int it = 0; it < iters; ++it)
-    hipLaunchKernelGGL(k_stage<T>, dim3(p->nblk), dim3(kTileThreads), lds, st, dev, c);
+    hipLaunchKernelGGL((k_stage<T, false>), dim3(p->nblk), dim3(kTileThreads), lds, st, dev, c);
   AERO_HIP_CHECK(hipEventRecord(e1, st));
   AERO_HIP_CHECK(hipEventSynchronize(e1));
   float ms = 0.f;
@@ -2422,6 +2462,7 @@ int aero3d_plan_create(const aero3d_desc *desc, aero3d_plan **out) {
       {&p->dye_tmp, desc->dye ? nc * es : 0},
       {&p->th_a, desc->thermal ? nc * es : 0}, {&p->th_b, desc->thermal ? nc * es : 0},
       {&p->nut_c, desc->thermal && desc->turbulence ? nc * es : 0},
+      {&p->nu_pad, desc->turbulence && !desc->nut_model ? (size_t)(desc->nx + 2) * (desc->ny + 2) * (desc->nz + 2) * es : 0},
       {&p->pheat, desc->thermal ? (size_t)3 * std::max(1, p->nwt) * kMaxT * 8 : 0},
       {&p->old_u, desc->diagnostics ? nu_ * es : 0}, {&p->old_v, desc->diagnostics ? nv_ * es : 0},
       {&p->old_w, desc->diagnostics ? nw_ * es : 0}, {&p->old_p, desc->diagnostics ? nc * es : 0},

@@ -949,9 +949,11 @@ __device__ __forceinline__ void poisA_body(const Dev<T> &d, int bx, const T *__r
   cplx<T> csr[kMaxPer];
 #pragma unroll
   for (int r = 0; r < kMaxPer; ++r) {
-    const int n = threadIdx.x + kFftThreads * r;
-    if (n < nx) {
-      const int q = d.dct_x ? makhoul_src(n, nx) : n;
+    // natural column q (coalesced loads) into transform position m; the DCT twiddles are
+    // those of output frequency k = q, which this thread finishes below
+    const int q = threadIdx.x + kFftThreads * r;
+    if (q < nx) {
+      const int m = d.dct_x ? makhoul_pos(q, nx) : q;
       T xa, xb = 0;
       if (mode == 0) {
         xa = f_in[ja * nx + q];
@@ -960,8 +962,8 @@ __device__ __forceinline__ void poisA_body(const Dev<T> &d, int bx, const T *__r
         xa = div_at(d, u, v, ja, q, corr);
         if (hasb) xb = div_at(d, u, v, jb, q, corr);
       }
-      A[n] = {xa, xb};
-      if (d.dct_x) csr[r] = d.cs_x[n];
+      A[m] = {xa, xb};
+      if (d.dct_x) csr[r] = d.cs_x[q];
     }
   }
   __syncthreads();
@@ -1123,9 +1125,8 @@ __device__ __forceinline__ void poisC_body(const Dev<T> &d, int bx, T *__restric
       X[nx + k] = hasb ? d.F[jb * nx + k] : (T)0;
       csr[r] = d.dct_x ? d.cs_x[k] : cplx<T>{0, 0};
       if (acc_p) {
-        const int q = d.dct_x ? makhoul_src(k, nx) : k;
-        pa[r] = p[ja * nx + q];
-        pb[r] = hasb ? p[jb * nx + q] : (T)0;
+        pa[r] = p[ja * nx + k];
+        pb[r] = hasb ? p[jb * nx + k] : (T)0;
       }
     }
   }
@@ -1143,8 +1144,10 @@ __device__ __forceinline__ void poisC_body(const Dev<T> &d, int bx, T *__restric
   const T w = (T)kForceW[stage];
 #pragma unroll
   for (int rr = 0; rr < kMaxPer; ++rr) {
-    const int m = threadIdx.x + kFftThreads * rr;
-    if (m >= nx) continue;
+    // natural column q (coalesced stores) from transform position m
+    const int q = threadIdx.x + kFftThreads * rr;
+    if (q >= nx) continue;
+    const int m = d.dct_x ? makhoul_pos(q, nx) : q;
     T va, vb;
     if (d.dct_x) {
       va = R[m].x * inv_n;
@@ -1155,7 +1158,6 @@ __device__ __forceinline__ void poisC_body(const Dev<T> &d, int bx, T *__restric
       va = (ha.x - ha.y) * inv_n;
       vb = (hb.x - hb.y) * inv_n;
     }
-    const int q = d.dct_x ? makhoul_src(m, nx) : m;
     psi[ja * nx + q] = va;
     if (hasb) psi[jb * nx + q] = vb;
     if (p != nullptr) {
@@ -2026,7 +2028,7 @@ int aero2d_plan_create(const aero2d_desc *desc, aero2d_plan **out) {
   // partial maxima and forces sized for the smallest tile (4 rows), so that a batched
   // launch may run this plan with any tile shape
   const size_t blocks = (size_t)p->nbx * ((ny + 3) / 4);
-  p->nslotB = 2;
+  p->nslotB = 1;
   while (p->nslotB > 1 &&
          (ny * p->nslotB * (2 * 2 * es + 2 * es) + ny * 2 * es > 65536 || ny * p->nslotB > (size_t)kMaxPer * kFftThreads))
     p->nslotB--;

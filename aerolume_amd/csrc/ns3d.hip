@@ -1553,9 +1553,10 @@ __global__ void __launch_bounds__(kThreads) k_p1(const Dev3<T> *__restrict__ dp,
   }
   const int r0 = 2 * ns * blockIdx.x;
   for (int e = threadIdx.x; e < ns * nx; e += kThreads) {
-    const int sl = fdiv(e, nx, log2_or_neg(nx)), n = e - sl * nx;
+    // natural column q (coalesced loads) into transform position n
+    const int sl = fdiv(e, nx, log2_or_neg(nx)), q = e - sl * nx;
     const int ra = r0 + 2 * sl, rb = ra + 1;
-    const int q = d.dct[0] ? makhoul_src(n, nx) : n;
+    const int n = d.dct[0] ? makhoul_pos(q, nx) : q;
     T xa = 0, xb = 0;
     if (mode == 0) {
       if (ra < nrows) xa = f[(size_t)ra * nx + q];
@@ -1564,7 +1565,7 @@ __global__ void __launch_bounds__(kThreads) k_p1(const Dev3<T> *__restrict__ dp,
       if (ra < nrows) xa = div_at(d, u, v, w, ra / d.ny, ra % d.ny, q, corr);
       if (rb < nrows) xb = div_at(d, u, v, w, rb / d.ny, rb % d.ny, q, corr);
     }
-    A[e] = {xa, xb};
+    A[sl * nx + n] = {xa, xb};
   }
   __syncthreads();
   const cplx<T> *R = fft_lds(A, B, d.fp[0], TW, ns, nx);

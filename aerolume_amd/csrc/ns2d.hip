@@ -44,7 +44,7 @@ constexpr int kFftThreads = 256;
 constexpr int kMaxPer = 16;                // row elements per FFT thread (n <= 4096)
 constexpr int TX = 64;                      // tile width: one wave across (blocks of 64 x 4 threads)
 constexpr int HW = TX + 4, NW = TX + 2;     // halo-2 rows of u, v, scalar; eddy-viscosity rows (halo 1)
-constexpr int kDefaultRows = 8;             // tile height unless a plan or an ensemble picks another
+constexpr int kDefaultRows = 4;             // tile height unless a plan or an ensemble picks another
 // A tile of R rows (R = 4, 8, 16; each thread owns R / 4 rows): halo-2 tiles of R + 4 rows,
 // eddy-viscosity tile of R + 2 rows, tile elements per thread of the loaders.
 template <int R>

@@ -91,7 +91,7 @@ typedef struct {
   double gravity_x, gravity_y;
   double prt;              /* turbulent Prandtl number */
   double diff_coef;        /* largest temperature diffusion coefficient / (kappa / h^2) */
-  int32_t tile_rows;       /* rows per 64-wide tile of the cell kernels: 0 (default, 8), 4, 8 or 16 */
+  int32_t tile_rows;       /* rows per 64-wide tile of the cell kernels: 0 (default, 4), 4, 8 or 16 */
   int32_t reserved;
 } aero2d_desc;
 

@@ -249,9 +249,9 @@ def test_periodic_taylor_green_matches_oracle():
     assert _err(cur.p, ref.p) < 1e-10
 
 
-@pytest.mark.parametrize("rows", [4, 16])
+@pytest.mark.parametrize("rows", [8, 16])
 def test_tile_rows_change_no_field(rows):
-    """The cell kernels' tile height (4 or 16 rows against the default 8) changes no field bit:
+    """The cell kernels' tile height (8 or 16 rows against the default 4) changes no field bit:
     heated airfoil with Smagorinsky LES and a smoke emitter on 160 x 75 cells (a ragged last
     tile row), float64; the force sums, summed over other tiles, agree to round-off."""
     _need_gpu()

@@ -78,6 +78,9 @@ struct Dev {
   int fixed_dt;
   double hd, cfl, force_ref, prt, diff_coef, les_c, gx, gy;
   T h, inv_h, kadv, inv_h2, ground;
+  // double-built constants of the per-cell paths, rounded once (the values the kernels formed
+  // in double and cast): 1/Pr_t, 2/h, Cs^2 h^2 (Smagorinsky, WALE), Cs h^2 (Vreman)
+  T inv_prt, two_h, les_k2, les_kv;
   // geometry
   const T *chi_u, *chi_v, *chi_c, *a_u, *a_v, *k_u, *k_v;
   const T *chi_t, *a_t, *w_t, *vol_u, *vol_v, *ins_u, *ins_v;
@@ -349,7 +352,7 @@ template <typename T>
 __device__ __forceinline__ T eddy(const Dev<T> &d, T g11, T g12, T g21, T g22) {
   const T s12 = (T)0.5 * (g12 + g21);
   const T ss = (g11 * g11 + g22 * g22) + (T)2 * s12 * s12;
-  if (d.turb == 1) return (T)(d.les_c * d.les_c * d.hd * d.hd) * sqrt((T)2 * ss + (T)1e-30);
+  if (d.turb == 1) return d.les_k2 * sqrt((T)2 * ss + (T)1e-30);
   if (d.turb == 2) {
     const T q11 = g11 * g11 + g12 * g21;
     const T q22 = g21 * g12 + g22 * g22;
@@ -360,7 +363,7 @@ __device__ __forceinline__ T eddy(const Dev<T> &d, T g11, T g12, T g21, T g22) {
     const T ssp = ss > (T)0 ? ss : (T)0;
     const T num = pow(sdp, (T)1.5);
     const T den = (pow(ssp, (T)2.5) + pow(sdp, (T)1.25)) + (T)1e-30;
-    return (T)(d.les_c * d.les_c * d.hd * d.hd) * num / den;
+    return d.les_k2 * num / den;
   }
   const T b11 = g11 * g11 + g12 * g12;
   const T b22 = g21 * g21 + g22 * g22;
@@ -368,7 +371,7 @@ __device__ __forceinline__ T eddy(const Dev<T> &d, T g11, T g12, T g21, T g22) {
   T bb = b11 * b22 - b12 * b12;
   bb = bb > (T)0 ? bb : (T)0;
   const T aa = ((g11 * g11 + g12 * g12) + g21 * g21) + g22 * g22;
-  return (T)(d.les_c * d.hd * d.hd) * sqrt(bb / (aa + (T)1e-30) + (T)1e-30);
+  return d.les_kv * sqrt(bb / (aa + (T)1e-30) + (T)1e-30);
 }
 
 // nu_t on the tile's cells and their ring (core.py:421-503), switched off in solids.
@@ -443,7 +446,7 @@ template <typename T, int R>
 __device__ __forceinline__ T kappa_x(const Dev<T> &d, const Lds<T, R> &L, int tj0, int ti0, T kap, int j, int f) {
   T k = kap;
   if (d.les) {
-    const T ip = (T)(1.0 / d.prt);
+    const T ip = d.inv_prt;
     const int r = j - tj0 + 1;
     auto N = [&](int c) { return L.nu[r][c - ti0 + 1]; };
     T fa;
@@ -465,7 +468,7 @@ template <typename T, int R>
 __device__ __forceinline__ T kappa_y(const Dev<T> &d, const Lds<T, R> &L, int tj0, int ti0, T kap, int f, int i) {
   T k = kap;
   if (d.les) {
-    const T ip = (T)(1.0 / d.prt);
+    const T ip = d.inv_prt;
     const int c = i - ti0 + 1;
     auto N = [&](int j) { return L.nu[j - tj0 + 1][c]; };
     T fa;
@@ -487,7 +490,7 @@ __device__ __forceinline__ T kappa_y(const Dev<T> &d, const Lds<T, R> &L, int tj
 // boundary diffusive flux of scalar.py:126-139; sign +1 low side, -1 high side
 template <typename T>
 __device__ __forceinline__ T wall_dflux(const Dev<T> &d, int mode, T k, T inner, T value, double sign) {
-  if (mode == AERO_TH_TEMPERATURE || mode == AERO_TH_INLET) return ((T)(-sign * 2.0 / d.hd) * k) * (inner - value);
+  if (mode == AERO_TH_TEMPERATURE || mode == AERO_TH_INLET) return ((sign > 0 ? -d.two_h : d.two_h) * k) * (inner - value);
   if (mode == AERO_TH_HEAT_FLUX) return ((T)sign * k) * value;
   return (T)0;
 }
@@ -1755,6 +1758,10 @@ Dev<T> make_dev(const aero2d_plan *p, const aero2d_geom *g, const aero2d_params 
   d.cfl = s.cfl;
   d.force_ref = s.force_ref;
   d.prt = s.prt;
+  d.inv_prt = (T)(1.0 / s.prt);
+  d.two_h = (T)(2.0 / s.h);
+  d.les_k2 = (T)(s.les_c * s.les_c * s.h * s.h);
+  d.les_kv = (T)(s.les_c * s.h * s.h);
   d.diff_coef = s.diff_coef;
   d.les_c = s.les_c;
   d.gx = s.gravity_x;

@@ -51,6 +51,24 @@ struct FftPlan {
   int radix[kMaxRadices];
 };
 
+// Twiddle table of an n-point transform in the order its Stockham stages read it: the stage
+// of span p (the product of the earlier radices) and radix R holds w^(r i n / (p R)) for
+// r = 1 .. R-1, i = 0 .. p-1 at (p - 1) + (r - 1) p + i, w = exp(-2 pi i / n) (the stages'
+// blocks add up to n - 1 entries). Consecutive butterflies of a stage read consecutive
+// entries (no LDS bank conflicts); the values are those of the natural table tw[r i n / (p R)].
+template <typename T, typename Plan>
+inline void stage_twiddles(const Plan &pl, const cplx<T> *natural, cplx<T> *out) {
+  const int n = pl.n;
+  int p = 1;
+  for (int s = 0; s < pl.nrad; ++s) {
+    const int R = pl.radix[s], stride = n / (p * R);
+    for (int r = 1; r < R; ++r)
+      for (int i = 0; i < p; ++i) out[(p - 1) + (r - 1) * p + i] = natural[r * i * stride];
+    p *= R;
+  }
+  out[n - 1] = natural[0];
+}
+
 // Radix sequence of an n-point transform (4s first, then 2, 3, 5); false when n has
 // another prime factor.
 inline bool factor_radices(int n, FftPlan &pl) {
@@ -120,7 +138,6 @@ __device__ __forceinline__ void stockham_stage(const cplx<T> *__restrict__ src, 
                                                int n, int p, const cplx<T> *__restrict__ tw, int nslot,
                                                int slot_stride, int tid, int nthr) {
   const int nr = n / R;
-  const int stride = n / (p * R);
   const int lnr = log2_or_neg(nr), lp = log2_or_neg(p);
   for (int w = tid; w < nslot * nr; w += nthr) {
     const int f = lnr >= 0 ? (w >> lnr) : w / nr;
@@ -133,7 +150,7 @@ __device__ __forceinline__ void stockham_stage(const cplx<T> *__restrict__ src, 
     for (int r = 0; r < R; ++r) v[r] = s[k + r * nr];
     if (p > 1) {
 #pragma unroll
-      for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[r * i * stride]);
+      for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[(p - 1) + (r - 1) * p + i]);
     }
     if constexpr (R == 2) dft2(v);
     if constexpr (R == 3) dft3(v);

@@ -1674,7 +1674,9 @@ int upload_tables(aero2d_plan *p) {
                           : (2.0 * std::cos(M_PI * (double)m / (double)n) - 2.0) / hh;
     }
     int rc;
-    if ((rc = put(axis == 0 ? p->tw_x : p->tw_y, tw.data(), n * sizeof(cplx<T>)))) return rc;
+    std::vector<cplx<T>> tws(n);
+    stage_twiddles(axis == 0 ? p->fx : p->fy, tw.data(), tws.data());
+    if ((rc = put(axis == 0 ? p->tw_x : p->tw_y, tws.data(), n * sizeof(cplx<T>)))) return rc;
     if ((rc = put(axis == 0 ? p->cs_x : p->cs_y, cs.data(), n * sizeof(cplx<T>)))) return rc;
     if ((rc = put(axis == 0 ? (void *)p->lam_x : (void *)p->lam_y, lam.data(), n * sizeof(double)))) return rc;
     (axis == 0 ? lamx : lamy) = lam;

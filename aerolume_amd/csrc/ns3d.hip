@@ -1866,7 +1866,9 @@ int upload3(aero3d_plan *p) {
       lam[m] = per[a] ? (2.0 * std::cos(2.0 * M_PI * (double)m / (double)n) - 2.0) / hh
                       : (2.0 * std::cos(M_PI * (double)m / (double)n) - 2.0) / hh;
     }
-    AERO_HIP_CHECK(hipMemcpy(p->tw[a], tw.data(), n * sizeof(cplx<T>), hipMemcpyHostToDevice));
+    std::vector<cplx<T>> tws(n);
+    stage_twiddles(p->fp[a], tw.data(), tws.data());
+    AERO_HIP_CHECK(hipMemcpy(p->tw[a], tws.data(), n * sizeof(cplx<T>), hipMemcpyHostToDevice));
     AERO_HIP_CHECK(hipMemcpy(p->cs[a], cs.data(), n * sizeof(cplx<T>), hipMemcpyHostToDevice));
     AERO_HIP_CHECK(hipMemcpy(p->lam[a], lam.data(), n * sizeof(double), hipMemcpyHostToDevice));
     lams[a] = lam;

@@ -61,6 +61,7 @@ struct Red {
   T dt;          // this step's dt
   T corr;        // outlet correction of the current stage
   double th_old; // sum of theta at step start
+  T ctrl[2];     // PID step: the controller memory after this step's dt (the state's after the step)
 };
 
 // Everything a kernel needs, passed by value.
@@ -76,6 +77,10 @@ struct Dev {
   int dye_mode, n_dye, n_groups;
   int gch[AERO2D_MAX_GROUPS];
   int fixed_dt;
+  int pid;           // adaptive_dt = "pid"
+  const T *dt_in;    // the step-start state's dt and controller memory (PID step)
+  const T *ctrl_in;
+  T *ctrl_out;       // the end-of-step state's controller memory
   double hd, cfl, force_ref, prt, diff_coef, les_c, gx, gy;
   T h, inv_h, kadv, inv_h2, ground;
   // double-built constants of the per-cell paths, rounded once (the values the kernels formed
@@ -558,6 +563,26 @@ __device__ T compute_dt(const Dev<T> &d, const DtIn<T> &in, T umax, T vmax, T th
   return dt;
 }
 
+// PID-controlled step (ns2d/diagnostics.py:104-123): steers dt toward 0.9 x the limit in log
+// space with gains (kP, kI, kD) = (0.10, 0.25, 0.02), never above the limit and never growing
+// more than 3 % per step; the first step (dt_prev = 0) takes the target. ctrl_prev / ctrl_new
+// hold the last two errors.
+template <typename T>
+__device__ T pid_step(T limit, T dt_prev, const T *ctrl_prev, T *ctrl_new) {
+  const T target = (T)0.9 * limit;
+  const bool first = dt_prev <= (T)0;
+  const T safe_prev = first ? target : dt_prev;
+  const T e = log(target / safe_prev);
+  const T e1 = ctrl_prev[0], e2 = ctrl_prev[1];
+  const T step = ((T)0.25 * e + (T)0.10 * (e - e1)) + (T)0.02 * ((e - (T)2 * e1) + e2);
+  const T cap = (T)0.02955880224154443;   // log(1.03) in double, rounded to T
+  const T grow = exp(step < cap ? step : cap);
+  const T cand = safe_prev * grow;
+  ctrl_new[0] = first ? (T)0 : e;
+  ctrl_new[1] = e1;
+  return first ? target : (cand < limit ? cand : limit);
+}
+
 // ================================================================== kernels
 // The explicit stage on one 64x8 tile (core.py:855-935 per face / cell).
 template <typename T, int R>
@@ -883,7 +908,9 @@ __device__ __forceinline__ void dt_body(const Dev<T> &d) {
   if (threadIdx.x == 0) din = load_dt_in(d);
   reduce_pmax(d, d.nbx * d.nby, lds, r);
   if (threadIdx.x == 0) {
-    d.red->dt = compute_dt(d, din, (T)r[0], (T)r[1], (T)r[2], (T)r[3]);
+    T dt = compute_dt(d, din, (T)r[0], (T)r[1], (T)r[2], (T)r[3]);
+    if (d.pid && !d.fixed_dt) dt = pid_step(dt, *d.dt_in, d.ctrl_in, d.red->ctrl);
+    d.red->dt = dt;
     d.red->th_old = r[4];
   }
 }
@@ -1508,7 +1535,18 @@ __device__ __forceinline__ void finalize_body(const Dev<T> &d, const T *__restri
     }
   }
   // the next step (chained) starts from this state
-  d.red->dt = compute_dt(d, din, (T)mx[0], (T)mx[1], (T)mx[2], (T)mx[3]);
+  T dt_next = compute_dt(d, din, (T)mx[0], (T)mx[1], (T)mx[2], (T)mx[3]);
+  if (d.pid && !d.fixed_dt) {
+    // this step's controller memory goes to the state; the next step's is computed from it
+    const T c0 = d.red->ctrl[0], c1 = d.red->ctrl[1];
+    if (d.ctrl_out) {
+      d.ctrl_out[0] = c0;
+      d.ctrl_out[1] = c1;
+    }
+    const T cur[2] = {c0, c1};
+    dt_next = pid_step(dt_next, dt, cur, d.red->ctrl);
+  }
+  d.red->dt = dt_next;
   d.red->th_old = mx[4];
 }
 template <typename T>
@@ -1756,6 +1794,7 @@ Dev<T> make_dev(const aero2d_plan *p, const aero2d_geom *g, const aero2d_params 
   d.n_groups = s.n_groups;
   for (int q = 0; q < AERO2D_MAX_GROUPS; ++q) d.gch[q] = s.group_channel[q];
   d.fixed_dt = s.fixed_dt;
+  d.pid = s.pid_dt;
   d.hd = s.h;
   d.cfl = s.cfl;
   d.force_ref = s.force_ref;
@@ -1827,6 +1866,9 @@ int step_impl(aero2d_plan *p, hipStream_t st, const aero2d_fields *in, aero2d_fi
               const aero2d_params *pr, aero2d_outputs *o, bool chained) {
   const aero2d_desc &s = p->desc;
   Dev<T> d = make_dev<T>(p, g, pr);
+  d.dt_in = (const T *)in->dt;
+  d.ctrl_in = (const T *)in->ctrl;
+  d.ctrl_out = (T *)out->ctrl;
   const int nx = s.nx, ny = s.ny;
   const dim3 blk(64, 4);
   const dim3 tiles(p->nbx, p->nby);
@@ -1976,6 +2018,10 @@ int check_step_args(aero2d_plan *p, const aero2d_fields *in, aero2d_fields *out,
   const aero2d_desc &s = p->desc;
   if (!s.periodic && !s.closed && !pr->u_inlet) { set_error("tunnel needs u_inlet"); return AERO_EINVAL; }
   if (s.fixed_dt && !pr->dt_fixed) { set_error("fixed dt needs dt_fixed"); return AERO_EINVAL; }
+  if (s.pid_dt && !s.fixed_dt && (!in->ctrl || !out->ctrl || !in->dt)) {
+    set_error("the PID step needs the state's dt and controller memory (fields ctrl, dt)");
+    return AERO_EINVAL;
+  }
   const bool win = s.win_j1 > s.win_j0 && s.win_i1 > s.win_i0;
   if (win && (!g || !g->chi_u || !g->chi_v || !g->chi_c || !g->a_u || !g->a_v || !g->k_u || !g->k_v)) {
     set_error("body window set but geometry pointers missing");
@@ -2178,6 +2224,9 @@ int ensemble_build(aero2d_ensemble *e) {
     aero2d_plan *p = e->plans[r];
     const aero2d_desc &s = p->desc;
     devs[r] = make_dev<T>(p, &geoms[r], &params[r]);
+    devs[r].dt_in = (const T *)fields[r].dt;
+    devs[r].ctrl_in = (const T *)fields[r].ctrl;
+    devs[r].ctrl_out = (T *)fields[r].ctrl;
     const int nby = (s.ny + e->ty - 1) / e->ty;
     devs[r].ty = e->ty;
     devs[r].nby = nby;

@@ -92,7 +92,7 @@ typedef struct {
   double prt;              /* turbulent Prandtl number */
   double diff_coef;        /* largest temperature diffusion coefficient / (kappa / h^2) */
   int32_t tile_rows;       /* rows per 64-wide tile of the cell kernels: 0 (default, 4), 4, 8 or 16 */
-  int32_t reserved;
+  int32_t pid_dt;          /* adaptive_dt = "pid" (ns2d/diagnostics.py:104-123): needs fields ctrl, dt */
 } aero2d_desc;
 
 /* Evolving fields, caller-owned device buffers of the plan's dtype.
@@ -100,6 +100,7 @@ typedef struct {
  * t and dt 0-d. in == out (in place) is allowed for every field. */
 typedef struct {
   void *u, *v, *p, *theta, *dye, *t, *dt;
+  void *ctrl;   /* PID step's controller memory (2 values), read from in, written to out */
 } aero2d_fields;
 
 /* Window masks (ns2d/geometry.py:160-197, 120-157); wy = j1-j0, wx = i1-i0.

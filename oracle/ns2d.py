@@ -20,6 +20,9 @@ Pinning (what checks each function besides retyping it):
   central scheme, quadratic exactness of the upwind3 reconstruction.
 * LES models - closed forms in float64 numpy, zero in pure shear for WALE / Vreman.
 * thermal, smoke - the heated-airfoil fixture, conduction / maximum-principle invariants.
+* ``pid_dt`` - the reference's PID trajectory (``tests/golden/ns2d_pid_diag.npz``, written by
+  ``tests/golden/make_ns2d_diag_fixtures.py``) and the controller's closed forms (first step,
+  fixed point at the target, growth cap, clip at the limit).
 """
 from __future__ import annotations
 
@@ -390,6 +393,26 @@ def physics_dt_limit(dt, u, v, theta, nut_max, ph, ops):
     return dt
 
 
+PID_TARGET = 0.9
+PID_GAINS = (0.10, 0.25, 0.02)    # (kP, kI, kD)
+PID_MAX_GROWTH = 1.03
+
+
+def pid_dt(dt_limit, dt_prev, ctrl):
+    """PID-controlled step in log space (diagnostics.py:104-123): returns (dt, new ctrl)."""
+    kp, ki, kd = PID_GAINS
+    target = PID_TARGET * dt_limit
+    first = dt_prev <= 0
+    safe_prev = torch.where(first, target, dt_prev)
+    e = torch.log(target / safe_prev)
+    e1, e2 = ctrl[0], ctrl[1]
+    step = ki * e + kp * (e - e1) + kd * (e - 2.0 * e1 + e2)
+    grow = torch.exp(step.clamp(max=math.log(PID_MAX_GROWTH)))
+    dt = torch.where(first, target, torch.minimum(safe_prev * grow, dt_limit))
+    e = torch.where(first, torch.zeros_like(e), e)
+    return dt, torch.cat([torch.stack([e, e1]), ctrl[2:]])
+
+
 # ---------------------------------------------------------------------------
 # scalars (scalar.py:35-141, core.py:702-771, physics.py:623-830)
 # ---------------------------------------------------------------------------
@@ -670,6 +693,7 @@ def step_full(state: State, geom, ops, dt=None, *, out: dict | None = None, trac
     psi_w = None
     heat_acc = None
     want_nut = thermal and ops.turbulence != "none"
+    ctrl = state.ctrl
     for s, ((a, b), w) in enumerate(zip(SSP_RK3, FORCE_WEIGHTS)):
         lu, lv, nut_max, nut = momentum_rhs(up, vp, geom, ops)
         if s == 0 and dt is None:
@@ -679,6 +703,8 @@ def step_full(state: State, geom, ops, dt=None, *, out: dict | None = None, trac
                 dt = cfl_dt(u0, v0, ops, nut_max)
                 if ph is not None:
                     dt = physics_dt_limit(dt, u0, v0, th0, nut_max, ph, ops)
+                    if ph.pid:
+                        dt, ctrl = pid_dt(dt, state.dt, ctrl)
         if not torch.is_tensor(dt):
             dt = torch.tensor(float(dt), dtype=ops.dtype)
         lt = theta_rhs(thp, up, vp, nut if want_nut else None, geom, ph, ops) if thermal else None
@@ -732,7 +758,7 @@ def step_full(state: State, geom, ops, dt=None, *, out: dict | None = None, trac
         dye, dye_extra = c[0], (c[1:] if c.shape[0] > 1 else None)
     elif dye is not None:
         dye = zero_dye_in_solids(advect_dye(dye, up, vp, dt, ops), geom)
-    new = State(up, vp, p, dye, t_new, dt * torch.ones_like(state.t), thp, dye_extra, state.ctrl)
+    new = State(up, vp, p, dye, t_new, dt * torch.ones_like(state.t), thp, dye_extra, ctrl)
     if out is not None and thermal:
         h2 = ops.h * ops.h
         tg = geom.thermal

@@ -175,3 +175,19 @@ def thermal_cfg(name, fixture, dtype="float64", device="cpu", **over):
               heat_sources=[HeatSource3D(**h) for h in spec["heaters"]])
     kw.update(over)
     return SolverConfig3D(**kw)
+
+
+# ---------------------------------------------------------------------------
+# 2D PID step and diagnostics (fixture tests/golden/ns2d_pid_diag.npz)
+# ---------------------------------------------------------------------------
+
+def pid_diag_cfg(dtype="float64", device="cpu", **over):
+    """64x32 tunnel, heated disc (Smagorinsky, Pr 0.71, Ri 0.1), adaptive_dt="pid"."""
+    from aerolume_amd.ns2d.config import ImmersedBody, SolverConfig, ThermalConfig
+    base = dict(nx=64, ny=32, h=1.0 / 8.0, x0=0.0, y0=0.0, re=400.0,
+                bodies=[ImmersedBody(circle(64, 0.75, 2.0, 2.0), thermal="temperature", thermal_value=1.0)],
+                advection="upwind3", turbulence="smagorinsky", cfl=0.5, dtype=dtype, device=device,
+                dye=False, perturb=False, thermal=ThermalConfig(prandtl=0.71, richardson=0.1),
+                adaptive_dt="pid", use_cuda_graphs=False)
+    base.update(over)
+    return SolverConfig(**base)

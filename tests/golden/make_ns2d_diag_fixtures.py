@@ -1,0 +1,101 @@
+"""Write the 2D PID-step and diagnostics golden fixture from the reference solver (build
+container only).
+
+The reference is imported from ``/root/reference/src`` (read-only) and run on the CPU in
+float64. ``ns2d_pid_diag.npz`` holds inputs and expected outputs only (numpy arrays):
+
+* the start of a 64x32 tunnel with a heated disc (Smagorinsky, Pr 0.71, Ri 0.1,
+  ``adaptive_dt="pid"``): u, v after the reference's boundary values and projection, theta;
+* ``core.step_full`` from that start for 30 steps: dt and the controller memory ``ctrl``
+  after every step, u, v, p, theta after the last;
+* the reference ``FlowSolver`` with ``diagnostics=True`` on the same case and start, advanced
+  7 then 13 steps: its diagnostics rows (one per 10-step block, columns ``DIAG_COLUMNS``)
+  and the state after 20 steps.
+
+Usage: ``python tests/golden/make_ns2d_diag_fixtures.py`` (a few seconds).
+"""
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+
+REF = Path("/root/reference/src")
+OUT = Path(__file__).resolve().parent
+sys.path.insert(0, str(REF))
+
+import torch  # noqa: E402
+
+from aerolume.solvers.ns2d import core  # noqa: E402
+from aerolume.solvers.ns2d.config import ImmersedBody, SolverConfig, ThermalConfig  # noqa: E402
+from aerolume.solvers.ns2d.diagnostics import DIAG_COLUMNS  # noqa: E402
+from aerolume.solvers.ns2d.geometry import build_geometry  # noqa: E402
+from aerolume.solvers.ns2d.solver import FlowSolver  # noqa: E402
+
+SEED = 808
+
+
+def circle(n, d, cx, cy):
+    th = np.linspace(0.0, 2.0 * np.pi, n, endpoint=False)
+    return np.c_[cx + 0.5 * d * np.cos(th), cy + 0.5 * d * np.sin(th)]
+
+
+def config(**over):
+    base = dict(nx=64, ny=32, h=1.0 / 8.0, x0=0.0, y0=0.0, re=400.0,
+                bodies=[ImmersedBody(circle(64, 0.75, 2.0, 2.0), thermal="temperature", thermal_value=1.0)],
+                advection="upwind3", turbulence="smagorinsky", cfl=0.5, dtype="float64", device="cpu",
+                dye=False, perturb=False, thermal=ThermalConfig(prandtl=0.71, richardson=0.1),
+                adaptive_dt="pid", use_cuda_graphs=False)
+    base.update(over)
+    return SolverConfig(**base)
+
+
+def start(ops):
+    rng = np.random.default_rng(SEED)
+    u = torch.as_tensor(1.0 + 0.01 * rng.standard_normal((ops.ny, ops.nx + 1)), dtype=ops.dtype)
+    v = torch.as_tensor(0.01 * rng.standard_normal((ops.ny + 1, ops.nx)), dtype=ops.dtype)
+    u, v = core._apply_boundary_values(u, v, ops)
+    u, v, _ = core.project(u, v, ops)
+    theta = torch.as_tensor(0.2 * rng.random((ops.ny, ops.nx)), dtype=ops.dtype)
+    return u, v, theta
+
+
+def np_(t):
+    return t.detach().cpu().numpy()
+
+
+def main():
+    torch.set_num_threads(8)
+    cfg = config()
+    ops = core.make_operators(cfg)
+    geom = build_geometry(cfg.bodies, ops)
+    u, v, theta = start(ops)
+    st = core.init_state(cfg, ops)._replace(u=u, v=v, theta=theta)
+    arrs = {"u0": np_(u), "v0": np_(v), "theta0": np_(theta), "ctrl0": np_(st.ctrl)}
+    dts, ctrls = [], []
+    cur = st
+    with torch.no_grad():
+        for _ in range(30):
+            cur, _ = core.step_full(cur, geom, ops, None)
+            dts.append(float(cur.dt))
+            ctrls.append(np_(cur.ctrl).copy())
+    arrs.update(dt_hist=np.asarray(dts), ctrl_hist=np.stack(ctrls), u30=np_(cur.u), v30=np_(cur.v),
+                p30=np_(cur.p), theta30=np_(cur.theta))
+    s = FlowSolver(config(diagnostics=True))
+    s._state = s._state._replace(u=u.clone(), v=v.clone(), theta=theta.clone())
+    s.advance(7)
+    s.advance(13)
+    d = s.diagnostics()
+    for k in DIAG_COLUMNS:
+        arrs["diag_" + k] = np.asarray(d[k])
+    arrs["diag_columns"] = np.array(DIAG_COLUMNS)
+    st20 = s.state
+    arrs.update(u20=np_(st20.u), v20=np_(st20.v), p20=np_(st20.p), theta20=np_(st20.theta),
+                dt20=np.float64(float(st20.dt)), ctrl20=np_(st20.ctrl))
+    np.savez_compressed(OUT / "ns2d_pid_diag.npz", **arrs)
+    print("wrote ns2d_pid_diag.npz", dts[:3], d["step"], flush=True)
+
+
+if __name__ == "__main__":
+    main()

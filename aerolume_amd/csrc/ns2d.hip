@@ -1556,6 +1556,151 @@ __global__ void __launch_bounds__(1024) k_finalize(Dev<T> d, const T *__restrict
   finalize_body<T>(d, t_in, t_out, dt_out, u_new, th_new, o);
 }
 
+// ------------------------------------------------------------------ diagnostics
+// One diagnostics row of a step (ns2d/diagnostics.py:70-101): per-block partials of the
+// continuity residual of the new state, of the last stage's Poisson residual L psi - f and
+// right-hand side f, and of the change of u, v, p, theta over the step; then one block forms
+// the float64 row. Sums and maxima accumulate in float64 like the reference's norms.
+constexpr int kDiagBlocks2 = 256;
+constexpr int kDV2 = 18;   // 0 sum div^2, 1 max|div|, 2 sum (L psi - f)^2, 3 sum f^2, 4 max|u|, 5 max|v| (new),
+                           // 6 + 3f: sum d^2, max|d|, sum a^2 of field f (u, v, p, theta) with d = new - old
+__device__ __forceinline__ bool diag2_is_max(int q) { return q == 1 || q == 4 || q == 5 || (q >= 6 && (q - 6) % 3 == 1); }
+
+template <typename T>
+struct Diag2 {
+  const T *uo, *vo, *po, *to;   // state before the step
+  const T *un, *vn, *pn, *tn;   // after
+  const T *uh, *vh;             // last stage's u^ (its projection's right-hand side)
+  const T *t_out, *dt_out;
+  double *part;                 // kDiagBlocks2 x kDV2 partials
+  double *row_buf;
+  long long *count;
+  long long cap;
+};
+
+// The discrete Laplacian of the Poisson solve at cell (j, i): Neumann walls mirror the edge
+// cell, periodic boxes wrap (ns2d/poisson.py:136-144).
+template <typename T>
+__device__ __forceinline__ T lap_at(const Dev<T> &d, const T *__restrict__ psi, int j, int i) {
+  const int nx = d.nx, ny = d.ny;
+  int ie, iw, jn, js;
+  if (d.periodic) {
+    ie = i + 1 == nx ? 0 : i + 1;
+    iw = i == 0 ? nx - 1 : i - 1;
+    jn = j + 1 == ny ? 0 : j + 1;
+    js = j == 0 ? ny - 1 : j - 1;
+  } else {
+    ie = i + 1 < nx ? i + 1 : nx - 1;
+    iw = i > 0 ? i - 1 : 0;
+    jn = j + 1 < ny ? j + 1 : ny - 1;
+    js = j > 0 ? j - 1 : 0;
+  }
+  const T e = psi[j * nx + ie], w = psi[j * nx + iw], n = psi[jn * nx + i], s = psi[js * nx + i];
+  return ((((e + w) + n) + s) - (T)4 * psi[j * nx + i]) / (T)(d.hd * d.hd);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_diag2(Dev<T> d, Diag2<T> g) {
+  const int nx = d.nx, ny = d.ny;
+  const size_t nc = (size_t)nx * ny;
+  const size_t stride = (size_t)gridDim.x * blockDim.x, t0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double v[kDV2];
+#pragma unroll
+  for (int q = 0; q < kDV2; ++q) v[q] = 0.0;
+  const T corr = (d.periodic || d.closed) ? (T)0 : d.red->corr;
+  const bool th = d.thermal && g.tn && g.to;
+  for (size_t e = t0; e < nc; e += stride) {
+    const int i = (int)(e % nx), j = (int)(e / nx);
+    const double dv = (double)div_at(d, g.un, g.vn, j, i, (T)0);
+    v[0] += dv * dv;
+    v[1] = fmax(v[1], fabs(dv));
+    const T f = div_at(d, g.uh, g.vh, j, i, corr);
+    const double r = (double)(lap_at(d, d.psi, j, i) - f);
+    v[2] += r * r;
+    v[3] += (double)f * (double)f;
+    const T pn = g.pn[e];
+    const double dp = (double)(pn - g.po[e]);
+    v[12] += dp * dp;
+    v[13] = fmax(v[13], fabs(dp));
+    v[14] += (double)pn * (double)pn;
+    if (th) {
+      const T tn = g.tn[e];
+      const double dth = (double)(tn - g.to[e]);
+      v[15] += dth * dth;
+      v[16] = fmax(v[16], fabs(dth));
+      v[17] += (double)tn * (double)tn;
+    }
+  }
+  const size_t nf[2] = {(size_t)ny * (nx + 1), (size_t)(ny + 1) * nx};
+  const T *fo[2] = {g.uo, g.vo}, *fn[2] = {g.un, g.vn};
+#pragma unroll
+  for (int f = 0; f < 2; ++f)
+    for (size_t e = t0; e < nf[f]; e += stride) {
+      const T an = fn[f][e];
+      const double dd = (double)(an - fo[f][e]);
+      v[6 + 3 * f] += dd * dd;
+      v[7 + 3 * f] = fmax(v[7 + 3 * f], fabs(dd));
+      v[8 + 3 * f] += (double)an * (double)an;
+      v[4 + f] = fmax(v[4 + f], (double)fabs(an));
+    }
+  __shared__ double red[kDV2 * 4];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int q = 0; q < kDV2; ++q) v[q] = diag2_is_max(q) ? wave_max(v[q]) : wave_sum(v[q]);
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < kDV2; ++q) red[wv * kDV2 + q] = v[q];
+  __syncthreads();
+  if (threadIdx.x < kDV2) {
+    const int q = threadIdx.x;
+    double r = red[q];
+    for (int w2 = 1; w2 < 4; ++w2) r = diag2_is_max(q) ? fmax(r, red[w2 * kDV2 + q]) : r + red[w2 * kDV2 + q];
+    g.part[(size_t)blockIdx.x * kDV2 + q] = r;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64) k_diag2_fin(Dev<T> d, Diag2<T> g) {
+  __shared__ double tot[kDV2];
+  if (threadIdx.x < kDV2) {
+    const int q = threadIdx.x;
+    double r = 0.0;
+    for (int b = 0; b < kDiagBlocks2; ++b) {
+      const double x = g.part[(size_t)b * kDV2 + q];
+      r = diag2_is_max(q) ? fmax(r, x) : r + x;
+    }
+    tot[q] = r;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const int nx = d.nx, ny = d.ny;
+  const double nc = (double)nx * ny;
+  const double numel[4] = {(double)ny * (nx + 1), (double)(ny + 1) * nx, nc, nc};
+  const double h = d.hd;
+  const double dt = (double)*g.dt_out;
+  const long long n = *g.count;
+  double *row = g.row_buf + (size_t)(n % g.cap) * AERO2D_DIAG_COLS;
+  row[0] = (double)*g.t_out;
+  row[1] = dt;
+  row[2] = dt * (tot[4] + tot[5]) / h;
+  row[3] = sqrt(tot[0]) / sqrt(nc);
+  row[4] = tot[1];
+  row[5] = sqrt(tot[2]) / fmax(sqrt(tot[3]), 1e-30);
+  for (int f = 0; f < 4; ++f) {
+    double *o = row + 6 + 4 * f;
+    if (f == 3 && !(d.thermal && g.tn && g.to)) {
+      o[0] = o[1] = o[2] = o[3] = (double)NAN;
+      continue;
+    }
+    const double nrm = sqrt(tot[6 + 3 * f]), l2 = nrm * h;
+    o[0] = l2;
+    o[1] = nrm / sqrt(numel[f]);
+    o[2] = tot[7 + 3 * f];
+    o[3] = l2 / fmax(sqrt(tot[8 + 3 * f]) * h, 1e-300);
+  }
+  *g.count = n + 1;
+}
+
 
 // ================================================================== batched ensemble
 // Many independent runs of uneven size advance together: every launch covers the blocks of
@@ -1655,6 +1800,7 @@ struct aero2d_plan {
   double *lam_x, *lam_y, *pmax, *pforce;
   void *inv_lam;
   void *red;
+  void *old_u, *old_v, *old_p, *old_t, *pdiag;   // diagnostics: the state before the step, partials
   FftPlan fx, fy;
   int nbx, nby;
   int ty;       // rows per tile (4, 8 or 16)
@@ -1873,6 +2019,17 @@ int step_impl(aero2d_plan *p, hipStream_t st, const aero2d_fields *in, aero2d_fi
   const dim3 blk(64, 4);
   const dim3 tiles(p->nbx, p->nby);
   const T *u0 = (const T *)in->u, *v0 = (const T *)in->v, *th0 = (const T *)in->theta;
+  const bool diag = o && o->diag;
+  if (diag) {
+    // the state before the step (the step may run in place)
+    const size_t nf[4] = {(size_t)ny * (nx + 1), (size_t)(ny + 1) * nx, (size_t)ny * nx, (size_t)ny * nx};
+    void *dst[4] = {p->old_u, p->old_v, p->old_p, p->old_t};
+    const void *src[4] = {in->u, in->v, in->p, in->theta};
+    for (int f = 0; f < (s.thermal ? 4 : 3); ++f) {
+      const int nb = (int)std::min<size_t>(4096, (nf[f] + 255) / 256);
+      hipLaunchKernelGGL(k_copy<T>, dim3(nb), dim3(256), 0, st, (T *)dst[f], (const T *)src[f], nf[f]);
+    }
+  }
   if (!chained) {
     by_rows(p->ty, [&](auto r) { hipLaunchKernelGGL((k_pre<T, decltype(r)::value>), tiles, blk, 0, st, d, u0, v0, th0); });
     hipLaunchKernelGGL(k_dt<T>, dim3(1), dim3(1024), 0, st, d);
@@ -1913,6 +2070,21 @@ int step_impl(aero2d_plan *p, hipStream_t st, const aero2d_fields *in, aero2d_fi
   if (o) oo = *o;
   hipLaunchKernelGGL(k_finalize<T>, dim3(1), dim3(1024), 0, st, d, (const T *)in->t, (T *)out->t, (T *)out->dt,
                      (const T *)out->u, (const T *)out->theta, oo);
+  if (diag) {
+    Diag2<T> dg;
+    dg.uo = (const T *)p->old_u; dg.vo = (const T *)p->old_v; dg.po = (const T *)p->old_p;
+    dg.to = s.thermal ? (const T *)p->old_t : nullptr;
+    dg.un = (const T *)out->u; dg.vn = (const T *)out->v; dg.pn = (const T *)out->p;
+    dg.tn = s.thermal ? (const T *)out->theta : nullptr;
+    dg.uh = (const T *)p->uC; dg.vh = (const T *)p->vC;
+    dg.t_out = (const T *)out->t; dg.dt_out = (const T *)out->dt;
+    dg.part = (double *)p->pdiag;
+    dg.row_buf = (double *)o->diag;
+    dg.count = (long long *)o->diag_count;
+    dg.cap = o->diag_cap;
+    hipLaunchKernelGGL(k_diag2<T>, dim3(kDiagBlocks2), dim3(256), 0, st, d, dg);
+    hipLaunchKernelGGL(k_diag2_fin<T>, dim3(1), dim3(64), 0, st, d, dg);
+  }
   AERO_HIP_CHECK(hipGetLastError());
   return AERO_OK;
 }
@@ -2048,6 +2220,10 @@ int check_step_args(aero2d_plan *p, const aero2d_fields *in, aero2d_fields *out,
   }
   if (o && o->hist && (!o->hist_count || o->hist_cap <= 0)) { set_error("history ring needs count and capacity"); return AERO_EINVAL; }
   if (o && o->aux && (!o->aux_count || o->aux_cap <= 0)) { set_error("aux ring needs count and capacity"); return AERO_EINVAL; }
+  if (o && o->diag && (!s.diagnostics || !o->diag_count || o->diag_cap < 1 || !in->p)) {
+    set_error("diagnostics rows need a plan made with desc.diagnostics, a row counter and fields.p");
+    return AERO_EINVAL;
+  }
   return AERO_OK;
 }
 
@@ -2096,7 +2272,10 @@ int aero2d_plan_create(const aero2d_desc *desc, aero2d_plan **out) {
       {&p->tw_x, nx * 2 * es}, {&p->tw_y, ny * 2 * es}, {&p->cs_x, nx * 2 * es}, {&p->cs_y, ny * 2 * es},
       {(void **)&p->lam_x, nx * 8}, {(void **)&p->lam_y, ny * 8}, {&p->inv_lam, nc * es},
       {(void **)&p->pmax, blocks * 5 * 8},
-      {(void **)&p->pforce, blocks * kMaxWin * 8}, {&p->red, 256}};
+      {(void **)&p->pforce, blocks * kMaxWin * 8}, {&p->red, 256},
+      {&p->old_u, desc->diagnostics ? nu_ * es : 0}, {&p->old_v, desc->diagnostics ? nv_ * es : 0},
+      {&p->old_p, desc->diagnostics ? nc * es : 0}, {&p->old_t, desc->diagnostics && desc->thermal ? nc * es : 0},
+      {&p->pdiag, desc->diagnostics ? (size_t)kDiagBlocks2 * kDV2 * 8 : 0}};
   for (auto &t : tab) {
     if ((rc = alloc(p, t.ptr, t.n))) {
       aero2d_plan_destroy(p);
@@ -2389,6 +2568,7 @@ int aero2d_ensemble_create(int32_t n, aero2d_plan *const *plans, const aero2d_fi
     if (!p) { set_error("null plan in ensemble"); return AERO_EINVAL; }
     if (p->dbl != dbl) { set_error("ensemble plans must share one dtype"); return AERO_EINVAL; }
     if (p->desc.dye_mode != 0) { set_error("ensemble runs carry no smoke"); return AERO_EUNSUPPORTED; }
+    if (outputs && outputs[r].diag) { set_error("ensemble runs record no diagnostics rows"); return AERO_EUNSUPPORTED; }
     aero2d_fields f = fields[r];
     int rc = check_step_args(p, &fields[r], &f, &geoms[r], &params[r],
                              outputs ? const_cast<aero2d_outputs *>(&outputs[r]) : nullptr);

@@ -93,6 +93,8 @@ typedef struct {
   double diff_coef;        /* largest temperature diffusion coefficient / (kappa / h^2) */
   int32_t tile_rows;       /* rows per 64-wide tile of the cell kernels: 0 (default, 4), 4, 8 or 16 */
   int32_t pid_dt;          /* adaptive_dt = "pid" (ns2d/diagnostics.py:104-123): needs fields ctrl, dt */
+  int32_t diagnostics;     /* 1: the plan keeps scratch for outputs.diag rows */
+  int32_t reserved;
 } aero2d_desc;
 
 /* Evolving fields, caller-owned device buffers of the plan's dtype.
@@ -136,7 +138,16 @@ typedef struct {
   int64_t hist_cap;
   void *aux, *aux_count;
   int64_t aux_cap;
+  /* diagnostics (plans made with desc.diagnostics): when diag is set the step writes one
+   * float64 row of AERO2D_DIAG_COLS values at (*diag_count mod diag_cap), then
+   * *diag_count += 1: t, dt, CFL, div_rms, div_max, Poisson residual of the last stage, then
+   * per field u, v, p, theta the change over the step (L2, RMS, max, relative L2; NaN
+   * without temperature) (ns2d/diagnostics.py:70-101; the "step" column is the caller's) */
+  void *diag, *diag_count;
+  int64_t diag_cap;
 } aero2d_outputs;
+
+#define AERO2D_DIAG_COLS 22
 
 int aero2d_plan_create(const aero2d_desc *desc, aero2d_plan **plan);
 int aero2d_plan_destroy(aero2d_plan *plan);
@@ -310,7 +321,8 @@ int aero3d_time_stage(aero3d_plan *plan, void *stream, const aero3d_fields *fiel
  * ensemble step costs the same ~15 launches whatever the number of runs. No reference
  * counterpart: the reference runs such sweeps one solve after another (api.py:4877-4930).
  * Each run is bound at creation to a plan and its in-place fields (fields[r] is both the
- * step's input and output), geometry, parameters and outputs; runs carry no smoke. */
+ * step's input and output), geometry, parameters and outputs; runs carry no smoke and record
+ * no diagnostics rows (outputs[r].diag must be NULL). */
 typedef struct aero2d_ensemble aero2d_ensemble;
 int aero2d_ensemble_create(int32_t n, aero2d_plan *const *plans, const aero2d_fields *fields,
                            const aero2d_geom *geoms, const aero2d_params *params,

@@ -681,8 +681,42 @@ def flow_heat_rates(th, u, ph, ops):
 # the step (core.py:815-969)
 # ---------------------------------------------------------------------------
 
-def step_full(state: State, geom, ops, dt=None, *, out: dict | None = None, trace: dict | None = None):
-    """One SSP-RK3 step; returns (new State, coeffs (nbody, 3)) (core.py:815-969)."""
+def change_norms(new, old, h):
+    """[L2, RMS, max, relative L2] of the change of one field over a step, in float64
+    (diagnostics.py:55-68)."""
+    d = (new - old).to(torch.float64)
+    nrm = math.sqrt(float((d * d).sum()))
+    l2 = nrm * h
+    ref = math.sqrt(float((new.to(torch.float64) ** 2).sum())) * h
+    return [l2, nrm / math.sqrt(d.numel()), float(d.abs().max()), l2 / max(ref, 1e-300)]
+
+
+def diagnostics_row(old, new, psi, rhs, ops):
+    """One diagnostics row without "step" (22 float64 values) from the states before and
+    after a step and the last stage's potential and Poisson right-hand side
+    (diagnostics.py:70-101): t, dt, CFL = dt (max|u| + max|v|) / h, RMS and max of the new
+    state's divergence, ||L psi - f|| / ||f||, then the change norms of u, v, p, theta (NaN
+    without temperature)."""
+    h = ops.h
+    div = divergence(new.u, new.v, h).to(torch.float64)
+    dt = float(new.dt)
+    umax = float(new.u.abs().max()) + float(new.v.abs().max())
+    r = (poisson(ops).laplacian(psi) - rhs).to(torch.float64)
+    f2 = float((rhs.to(torch.float64) ** 2).sum())
+    row = [float(new.t), dt, dt * umax / h, math.sqrt(float((div * div).sum())) / math.sqrt(div.numel()),
+           float(div.abs().max()), math.sqrt(float((r * r).sum())) / max(math.sqrt(f2), 1e-30)]
+    row += change_norms(new.u, old.u, h) + change_norms(new.v, old.v, h) + change_norms(new.p, old.p, h)
+    if new.theta is not None and old.theta is not None:
+        row += change_norms(new.theta, old.theta, h)
+    else:
+        row += [math.nan] * 4
+    return torch.tensor(row, dtype=torch.float64)
+
+
+def step_full(state: State, geom, ops, dt=None, *, out: dict | None = None, trace: dict | None = None,
+              diag: bool = False):
+    """One SSP-RK3 step; returns (new State, coeffs (nbody, 3)) (core.py:815-969). With
+    ``diag`` and ``out``, ``out["diag"]`` is the step's diagnostics row (core.py:923-925, 1003-1004)."""
     ph = ops.phys
     u0, v0 = state.u, state.v
     up, vp = u0, v0
@@ -759,6 +793,8 @@ def step_full(state: State, geom, ops, dt=None, *, out: dict | None = None, trac
     elif dye is not None:
         dye = zero_dye_in_solids(advect_dye(dye, up, vp, dt, ops), geom)
     new = State(up, vp, p, dye, t_new, dt * torch.ones_like(state.t), thp, dye_extra, ctrl)
+    if out is not None and diag:
+        out["diag"] = diagnostics_row(state, new, psi, rhs, ops)
     if out is not None and thermal:
         h2 = ops.h * ops.h
         tg = geom.thermal

@@ -1,6 +1,7 @@
-"""The 2D PID-controlled step (``adaptive_dt="pid"``) on the MI355X through the C ABI: the
-reference's trajectory in float64 (``tests/golden/ns2d_pid_diag.npz``), the oracle in float32,
-and the FlowSolver's graphs and chained steps against eager plain steps."""
+"""The 2D PID-controlled step (``adaptive_dt="pid"``) and the diagnostics rows on the MI355X
+through the C ABI: the reference's trajectory and rows in float64
+(``tests/golden/ns2d_pid_diag.npz``), the oracle in float32, and the FlowSolver's graphs and
+chained steps against eager plain steps."""
 from __future__ import annotations
 
 import dataclasses
@@ -88,3 +89,103 @@ def test_pid_flowsolver_graphs_and_chained_equal_eager():
     assert np.allclose(a.state.ctrl.cpu().numpy(), fx["ctrl20"], rtol=1e-10, atol=1e-13)
     sd = a.state_dict()
     assert "ctrl" in sd and sd["ctrl"].numel() == 2
+
+
+ROUNDOFF_COLS = ("div_rms", "div_max", "p_residual")
+
+
+def _diag_solver(dtype, graphs, fx):
+    cfg = dataclasses.replace(cases.pid_diag_cfg(dtype, device="cuda"), use_cuda_graphs=graphs, diagnostics=True)
+    s = S.FlowSolver(cfg)
+    s.set_velocity(fx["u0"], fx["v0"])
+    s.set_temperature(fx["theta0"])
+    s.advance(7)
+    s.advance(13)
+    return s
+
+
+def test_diagnostics_rows_match_reference_float64(tmp_path):
+    """Rows at steps 10 and 20 of a 7 + 13 step advance, graphs on and off, against the
+    reference FlowSolver's rows; the continuity and Poisson residuals are round-off numbers
+    of the direct solve and are compared by magnitude only."""
+    print("[pid gpu] diagnostics rows float64 vs reference", flush=True)
+    fx = _fixture()
+    cols = [str(c) for c in fx["diag_columns"]]
+    assert tuple(cols) == S.DIAG_COLUMNS
+    runs = []
+    for graphs in (True, False):
+        s = _diag_solver("float64", graphs, fx)
+        d = s.diagnostics()
+        assert list(d["step"]) == [10.0, 20.0], graphs
+        for c in cols[1:]:
+            mine, theirs = d[c], fx[f"diag_{c}"]
+            if c in ROUNDOFF_COLS:
+                assert np.all(mine < 1e-13) and np.all(mine > 0), (c, mine)
+            else:
+                assert np.allclose(mine, theirs, rtol=1e-9, atol=1e-14), (c, mine, theirs)
+        assert len(d["steps_per_s"]) == 2 and list(d["perf_steps"]) == [7.0, 20.0]
+        runs.append(s)
+    a, b = (r.diagnostics() for r in runs)
+    for c in cols:
+        assert np.array_equal(a[c], b[c]), c       # graph and eager rows bit for bit
+    s = runs[0]
+    last = s.latest_diagnostics()
+    assert last["step"] == 20.0 and last["t"] == pytest.approx(float(fx["diag_t"][1]), rel=1e-12)
+    path = s.diagnostics_csv(tmp_path / "diag.csv")
+    lines = Path(path).read_text().splitlines()
+    assert lines[0].split(",") == cols and len(lines) == 3
+    assert float(lines[2].split(",")[0]) == 20.0
+    # the rows stay with the run: a reset clears them, the next advance records afresh
+    s.reset()
+    assert len(s.diagnostics()["step"]) == 0
+    s.advance(10)
+    assert list(s.diagnostics()["step"]) == [10.0]
+
+
+def test_diagnostics_rows_float32_vs_oracle_and_float64():
+    """float32 rows against the oracle's float32 rows, and their error against the float64
+    reference rows (velocity / temperature changes ~1e-6, pressure change ~1e-3 relative)."""
+    print("[pid gpu] diagnostics rows float32 vs oracle", flush=True)
+    from tests.test_oracle_ns2d_pid import oracle_diag_rows
+    fx = _fixture()
+    cols = [str(c) for c in fx["diag_columns"]][1:]
+    d = _diag_solver("float32", True, fx).diagnostics()
+    orc, _ = oracle_diag_rows(fx, dtype=torch.float32)
+    for i, c in enumerate(cols):
+        mine, o32, f64 = d[c], orc[:, i], fx[f"diag_{c}"]
+        if c in ROUNDOFF_COLS:
+            assert np.all(mine < 1e-5) and np.all(o32 < 1e-5), (c, mine, o32)
+            continue
+        tol = 5e-3 if c.startswith("dp") else 2e-5
+        assert np.allclose(mine, o32, rtol=tol, atol=0), (c, mine, o32)
+        assert np.allclose(mine, f64, rtol=tol, atol=0), (c, mine, f64)
+
+
+def test_functional_diag_row_and_no_theta():
+    """step_full(diag=True) returns the row of that one step (oracle, float64), with NaN
+    temperature columns on a run without heat transfer."""
+    print("[pid gpu] functional diag rows", flush=True)
+    for thermal in (True, False):
+        over = {} if thermal else dict(thermal=None, bodies=[], adaptive_dt="cfl")
+        rows = []
+        for dev in ("cuda", "cpu"):
+            cfg = cases.pid_diag_cfg("float64", device=dev, **over)
+            ops = su.make_operators(cfg, device=dev)
+            geom = su.build_geometry(cfg.bodies, ops, window=su.live_window(cfg.bodies, ops))
+            st = S.init_state(cfg, ops)
+            step = S.step_full if dev == "cuda" else O.step_full
+            with torch.no_grad():
+                st, _ = step(st, geom, ops)
+                out = {}
+                st, _ = step(st, geom, ops, out=out, diag=True)
+            rows.append(out["diag"].double().cpu().numpy())
+        g, c = rows
+        assert g.shape == (22,)
+        if not thermal:
+            assert np.all(np.isnan(g[18:])) and np.all(np.isnan(c[18:]))
+        n = 18 if not thermal else 22
+        for i in range(n):
+            if i in (3, 4, 5):
+                assert g[i] < 1e-12 and c[i] < 1e-12, (i, g[i], c[i])
+            else:
+                assert g[i] == pytest.approx(c[i], rel=1e-9, abs=1e-14), (thermal, i, g[i], c[i])

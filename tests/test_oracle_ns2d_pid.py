@@ -1,6 +1,6 @@
-"""The 2D PID-controlled step (``adaptive_dt="pid"``) of the CPU oracle against the
-reference's own trajectory (``tests/golden/ns2d_pid_diag.npz``, float64) and the controller's
-closed forms."""
+"""The 2D PID-controlled step (``adaptive_dt="pid"``) and the diagnostics rows of the CPU
+oracle against the reference's own trajectory and rows (``tests/golden/ns2d_pid_diag.npz``,
+float64), and the controller's closed forms."""
 from __future__ import annotations
 
 import math
@@ -85,3 +85,84 @@ def test_pid_planted_gain_is_caught():
     finally:
         O.PID_GAINS = saved
     assert abs(float(st.t) - float(np.cumsum(fx["dt_hist"])[9])) > 1e-9
+
+
+ROUNDOFF_COLS = ("div_rms", "div_max", "p_residual")
+
+
+def oracle_diag_rows(fx, steps=20, dtype=torch.float64):
+    """The oracle from the fixture's start, diagnostics rows on every 10th step."""
+    ops, geom, st = _start(fx, dtype)
+    rows = []
+    with torch.no_grad():
+        for n in range(1, steps + 1):
+            out = {}
+            st, _ = O.step_full(st, geom, ops, out=out, diag=n % 10 == 0)
+            if n % 10 == 0:
+                rows.append(out["diag"].numpy())
+    return np.stack(rows), st
+
+
+def test_oracle_diagnostics_match_reference():
+    """The reference FlowSolver's rows at steps 10 and 20 (a 7 + 13 step advance) against the
+    oracle's row on its own trajectory. The continuity and Poisson residuals are round-off
+    numbers of a direct solve and are compared by magnitude only."""
+    fx = _fixture()
+    cols = [str(c) for c in fx["diag_columns"]]
+    assert list(fx["diag_step"]) == [10.0, 20.0]
+    rows, _ = oracle_diag_rows(fx)
+    for i, c in enumerate(cols[1:]):
+        mine, theirs = rows[:, i], fx[f"diag_{c}"]
+        if c in ROUNDOFF_COLS:
+            assert np.all(mine < 1e-13) and np.all(theirs < 1e-13), (c, mine, theirs)
+            assert np.all(mine > 0) and np.all(theirs > 0), c
+        else:
+            assert np.allclose(mine, theirs, rtol=1e-9, atol=1e-14), (c, mine, theirs)
+
+
+def test_diagnostics_planted_norm_is_caught():
+    """A change norm without its cell-size factor (a plausible slip) misses the reference row."""
+    fx = _fixture()
+    saved = O.change_norms
+
+    def no_h(new, old, h):
+        return saved(new, old, 1.0)
+    try:
+        O.change_norms = no_h
+        rows, _ = oracle_diag_rows(fx, steps=10)
+    finally:
+        O.change_norms = saved
+    assert not np.isclose(rows[0, 6], fx["diag_du_l2"][0], rtol=1e-3)
+
+
+def test_diagnostics_residual_closed_forms():
+    """The Poisson residual column is ||L psi - f|| / ||f||: round-off for the solved potential
+    of a compatible (zero-mean) f, exactly 1/2 for half of it, and |mean f| sqrt(N) / ||f|| for
+    an f that breaks the Neumann compatibility condition (the solve drops its mean)."""
+    fx = _fixture()
+    ops, _, st = _start(fx)
+    g = torch.Generator().manual_seed(5)
+    f = torch.randn(ops.ny, ops.nx, generator=g, dtype=torch.float64)
+    f0 = f - f.mean()
+    psi = O.poisson(ops).solve(f0)
+    good = O.diagnostics_row(st, st, psi, f0, ops)
+    half = O.diagnostics_row(st, st, 0.5 * psi, f0, ops)
+    assert float(good[5]) < 1e-12 and float(half[5]) == pytest.approx(0.5, rel=1e-9)
+    fm = f0 + 0.3
+    off = O.diagnostics_row(st, st, O.poisson(ops).solve(fm), fm, ops)
+    assert float(off[5]) == pytest.approx(0.3 * f.numel() ** 0.5 / float(fm.norm()), rel=1e-9)
+    # an unchanged state: zero change norms
+    assert np.all(good[6:].numpy() == 0.0)
+
+
+def test_diagnostics_without_temperature_are_nan():
+    cfg = cases.pid_diag_cfg("float64", thermal=None, bodies=[], adaptive_dt="cfl")
+    ops = su.make_operators(cfg, device="cpu")
+    geom = su.build_geometry(cfg.bodies, ops, window=su.live_window(cfg.bodies, ops))
+    st = S.init_state(cfg, ops)
+    out = {}
+    with torch.no_grad():
+        new, _ = O.step_full(st, geom, ops, out=out, diag=True)
+    row = out["diag"].numpy()
+    assert row.shape == (22,) and np.all(np.isnan(row[18:])) and np.all(np.isfinite(row[:18]))
+    assert row[0] == pytest.approx(float(new.t)) and row[1] == pytest.approx(float(new.dt))

@@ -24,11 +24,11 @@ class Desc3D(ctypes.Structure):
                 ("theta_q_k1", i32), ("theta_q_j0", i32), ("theta_q_j1", i32), ("theta_q_i0", i32),
                 ("theta_q_i1", i32), ("theta_nt", i32), ("theta_tbody", i32 * 8), ("theta_adia", i32),
                 ("theta_nskin", i32), ("richardson", f64), ("prandtl_turbulent", f64),
-                ("diagnostics", i32), ("reserved3", i32)]
+                ("diagnostics", i32), ("pid_dt", i32), ("pid_target", f64)]
 
 
 class Fields3D(ctypes.Structure):
-    _fields_ = [(k, vp) for k in ("u", "v", "w", "p", "t", "dt", "dye", "theta")]
+    _fields_ = [(k, vp) for k in ("u", "v", "w", "p", "t", "dt", "dye", "theta", "ctrl")]
 
 
 #: momentum window arrays (the Geometry3D tensors of the same names)

@@ -76,6 +76,7 @@ template <typename T>
 struct Red3 {
   T dt;          // dt of the step being taken
   T corr[3];     // outlet correction of each stage's u^
+  T ctrl[3];     // PID step: the controller memory after this dt (the state's after the step)
 };
 
 // Static description of a plan, resident in device memory.
@@ -85,6 +86,8 @@ struct Dev3 {
   int per[3];
   int closed, outlet;
   int upwind, turb, fixed_dt;
+  int pid;                   // adaptive_dt = "pid": dt from the controller (pid_target x the rule's dt)
+  T pid_target;
   int nbx, nby, nbz, nblk, xcd_chunk;
   int win, wk0, wk1, wj0, wj1, wi0, wi1, wz, wy, wx;
   int wbx0, wby0, wbz0, nwbx, nwby, nwbz, nwt;
@@ -135,6 +138,8 @@ struct Call3 {
   int stage;
   int corr_stage;            // red->corr slot added at the outlet face of up (-1: none)
   int nu_in;                 // stage input is the state the final kernel last saw: nu_t from d.nu_pad
+  const T *ctrl_in;          // PID step: controller memory [dt_prev, e1, e2] of the step start
+  T *ctrl_out;               // and of the step's end
 };
 
 template <typename T>
@@ -912,13 +917,38 @@ __device__ void reduce_pmax(const Dev3<T> &d, T out[kPm], T *lds) {
   __syncthreads();
 }
 
+// PID step (physics.py:1144-1177): from the rule's dt `cap` and the memory mem = [dt_prev, e1,
+// e2], e = log(target cap / dt_prev) and dt = dt_prev exp(kI e + kP (e - e1) + kD (e - 2 e1 + e2))
+// with gains (0.3, 0.4, 0.05), the factor clamped to [0.5, 1.2] and dt never above cap; the first
+// step (dt_prev <= 0) takes target cap. Writes the memory after this dt to out (may alias mem).
+template <typename T>
+__device__ T pid3(const Dev3<T> &d, T cap, const T *mem, T *out) {
+  const T want = cap * d.pid_target;
+  const T prev_dt = mem[0], e1 = mem[1], e2 = mem[2];
+  const bool first = prev_dt <= (T)0;
+  const T prev = first ? want : prev_dt;
+  const T e = log(want / prev);
+  T fac = exp(((T)0.3 * e + (T)0.4 * (e - e1)) + (T)0.05 * ((e - (T)2 * e1) + e2));
+  fac = fac < (T)0.5 ? (T)0.5 : (fac > (T)1.2 ? (T)1.2 : fac);
+  const T cand = first ? want : prev * fac;
+  const T dt = (cand < cap || cand != cand) ? cand : cap;
+  out[0] = dt;
+  out[1] = first ? (T)0 : e;
+  out[2] = e1;
+  return dt;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kFinThreads) k_dt(const Dev3<T> *__restrict__ dp, Call3<T> c) {
   __shared__ T lds[kPm * (kFinThreads / 64)];
   const Dev3<T> &d = *dp;
   T m[kPm];
   reduce_pmax(d, m, lds);
-  if (threadIdx.x == 0) d.red->dt = cfl_dt(d, c, m);
+  if (threadIdx.x == 0) {
+    T dt = cfl_dt(d, c, m);
+    if (d.pid) dt = pid3(d, dt, c.ctrl_in, d.red->ctrl);
+    d.red->dt = dt;
+  }
 }
 
 // Outputs of the finalize kernel.
@@ -1056,7 +1086,18 @@ __global__ void __launch_bounds__(kFinThreads) k_finalize(const Dev3<T> *__restr
     for (int m = 0; m < 6; ++m) row[1 + m] = tot[m];
     *f.hist_count = n + 1;
   }
-  if (f.next_dt && !d.fixed_dt) d.red->dt = cfl_dt(d, c, mx);
+  if (d.pid && c.ctrl_out) {
+    // commit the memory of this step's dt; fixed steps leave it as it was
+    for (int q = 0; q < 3; ++q) c.ctrl_out[q] = d.fixed_dt ? c.ctrl_in[q] : d.red->ctrl[q];
+  }
+  if (f.next_dt && !d.fixed_dt) {
+    T dt = cfl_dt(d, c, mx);
+    if (d.pid) {
+      T mem[3] = {d.red->ctrl[0], d.red->ctrl[1], d.red->ctrl[2]};
+      dt = pid3(d, dt, mem, d.red->ctrl);
+    }
+    d.red->dt = dt;
+  }
 }
 
 
@@ -1897,6 +1938,8 @@ int upload3(aero3d_plan *p) {
   d.upwind = s.advection == 1;
   d.turb = s.turbulence;
   d.fixed_dt = s.fixed_dt;
+  d.pid = s.pid_dt;
+  d.pid_target = (T)s.pid_target;
   d.nbx = p->nbx;
   d.nby = p->nby;
   d.nbz = p->nbz;
@@ -2119,6 +2162,8 @@ int step_impl(aero3d_plan *p, hipStream_t st, const aero3d_fields *in, aero3d_fi
   const aero3d_desc &s = p->desc;
   const Dev3<T> *dev = (const Dev3<T> *)p->dev;
   Call3<T> c = make_call<T>(in, g, pr);
+  c.ctrl_in = (const T *)in->ctrl;
+  c.ctrl_out = (T *)out->ctrl;
   const dim3 grid(p->nblk), blk(kTileThreads);
   const size_t lds = sizeof(Lds3<T>);
   const bool diag = o && o->diag;
@@ -2361,6 +2406,10 @@ int check3_desc(const aero3d_desc *d) {
       return AERO_EINVAL;
     }
   }
+  if (d->pid_dt && !(d->pid_target >= 0.1 && d->pid_target <= 1.0)) {
+    set_error("pid_target must be in [0.1, 1]");
+    return AERO_EINVAL;
+  }
   return AERO_OK;
 }
 
@@ -2373,6 +2422,10 @@ int check3_step(const aero3d_plan *p, const aero3d_fields *in, const aero3d_fiel
   }
   if (p->desc.dye && p->desc.dye_k1 > p->desc.dye_k0 && (!pr->dye_weight || !pr->dye_pulse)) {
     set_error("smoke nozzles need params.dye_weight and dye_pulse");
+    return AERO_EINVAL;
+  }
+  if (p->desc.pid_dt && (!in->ctrl || !out->ctrl)) {
+    set_error("the PID step needs the controller memory (fields ctrl, 3 values)");
     return AERO_EINVAL;
   }
   if (p->desc.nut_model && !pr->nut_field) {

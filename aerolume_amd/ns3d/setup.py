@@ -104,8 +104,6 @@ def check_supported(cfg: SolverConfig3D) -> None:
         missing.append("emitters")
     if cfg.particle_capacity:
         missing.append("tracer particles")
-    if cfg.adaptive_dt != "cfl":
-        missing.append("adaptive_dt='pid'")
     if cfg.dye and cfg.emitters is not None:
         missing.append("Emitter3D smoke sources (the rake of dye_rake is carried)")
     if missing:

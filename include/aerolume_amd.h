@@ -222,7 +222,8 @@ typedef struct {
   double richardson;        /* buoyancy Ri theta along +z (0: passive temperature) */
   double prandtl_turbulent;
   int32_t diagnostics;      /* 1: the plan keeps scratch for outputs.diag rows */
-  int32_t reserved3;
+  int32_t pid_dt;           /* adaptive_dt = "pid" (ns3d/physics.py:1144-1177): needs fields ctrl */
+  double pid_target;        /* the controller's target as a fraction of the rule's dt, in [0.1, 1] */
 } aero3d_desc;
 
 /* heat-equation side conditions (ns3d/physics.py:589-606) */
@@ -237,6 +238,7 @@ typedef struct {
   void *u, *v, *w, *p, *t, *dt;
   void *dye;                /* (nz, ny, nx) smoke concentration in [0, 1] when desc.dye */
   void *theta;              /* (nz, ny, nx) temperature when desc.thermal */
+  void *ctrl;               /* PID step: controller memory [dt_prev, e1, e2], read from in, written to out */
 } aero3d_fields;
 
 /* Window arrays (ns3d/geometry.py:275-303), wz = k1-k0, wy = j1-j0, wx = i1-i0:

@@ -14,7 +14,9 @@ data objects (``ops`` with the fields of the reference's ``Ops3D``, ``geom`` wit
 
 Pinning: the reference's own trajectories (``tests/golden/make_ns3d_fixtures.py``), the
 Poisson residual and round trip (closed form), divergence of projected noise, Taylor-Green
-decay, zero eddy viscosity in pure shear for WALE / Vreman.
+decay, zero eddy viscosity in pure shear for WALE / Vreman. ``PidDt3`` - the reference's PID
+trajectory on the heated box (``tests/golden/ns3d_pid_box.npz``) and the controller's closed
+forms (first step, fixed point at the target, both growth clamps, clip at the limit).
 """
 from __future__ import annotations
 
@@ -496,9 +498,10 @@ def force_coefficients(acc, geom, ops, dt):
                         mz / (q_a * ops.span_ref)], dim=1)
 
 
-def step3d_full(state: State3D, geom, ops, dt=None, nut_field=None, nut_trust=None):
+def step3d_full(state: State3D, geom, ops, dt=None, nut_field=None, nut_trust=None, pid=None):
     """One SSP-RK3 step without placed physics (core.py:765-883); ``nut_field`` / ``nut_trust``:
-    the learned closure's eddy viscosity, held over the step (no reference counterpart)."""
+    the learned closure's eddy viscosity, held over the step (no reference counterpart);
+    ``pid``: a :class:`PidDt3` applied after the CFL rule (physics.py:1230-1262 without heat)."""
     u0, v0, w0 = state.u, state.v, state.w
     up, vp, wp = u0, v0, w0
     acc = None
@@ -509,7 +512,12 @@ def step3d_full(state: State3D, geom, ops, dt=None, nut_field=None, nut_trust=No
         lu, lv, lw, nut_max = momentum_rhs3d(up, vp, wp, geom, ops, nut_field, nut_trust)
         if s == 0:
             if dt is None:
-                dt = ops.extras["dt_fixed"] if ops.dt_fixed is not None else cfl_dt3d(u0, v0, w0, ops, nut_max)
+                if ops.dt_fixed is not None:
+                    dt = ops.extras["dt_fixed"]
+                else:
+                    dt = cfl_dt3d(u0, v0, w0, ops, nut_max)
+                    if pid is not None:
+                        dt = pid(dt)
             if not torch.is_tensor(dt):
                 dt = torch.tensor(float(dt), dtype=ops.dtype)
         bdt = dt * b
@@ -819,6 +827,34 @@ def add_buoyancy(th, wh, theta, bdt, wgt, geom, ops):
     return wh, share
 
 
+PID_GAINS_3D = (0.3, 0.4, 0.05)       # (kI, kP, kD), physics.py:79
+PID_LIMITS_3D = (0.5, 1.2)            # growth factor clamp per step, physics.py:81
+
+
+class PidDt3:
+    """The 3D PID step (physics.py:1144-1177): ``e = log(target dt_cap / dt_prev)``,
+    ``dt = dt_prev exp(kI e + kP (e - e1) + kD (e - 2 e1 + e2))`` with the factor clamped to
+    PID_LIMITS_3D, never above ``dt_cap``; the first call (dt_prev = 0) takes the target.
+    ``mem`` = [dt_prev, e1, e2] is updated in place."""
+
+    def __init__(self, target: float, dtype=torch.float64):
+        self.target = float(target)
+        self.mem = torch.zeros(3, dtype=dtype)
+
+    def __call__(self, dt_cap):
+        ki, kp, kd = PID_GAINS_3D
+        prev_dt, e1, e2 = self.mem[0], self.mem[1], self.mem[2]
+        want = dt_cap * self.target
+        first = prev_dt <= 0
+        prev = torch.where(first, want, prev_dt)
+        e = torch.log(want / prev)
+        fac = torch.exp(ki * e + kp * (e - e1) + kd * (e - 2.0 * e1 + e2))
+        fac = fac.clamp(PID_LIMITS_3D[0], PID_LIMITS_3D[1])
+        dt = torch.minimum(torch.where(first, want, prev * fac), dt_cap)
+        self.mem = torch.stack([dt, torch.where(first, torch.zeros_like(e), e), e1])
+        return dt
+
+
 def thermal_dt(th, u0, v0, w0, ops, nut_max, theta):
     """The step's dt with heat transfer (Physics3D.dt_rule without fans / jets, physics.py:1230-1262):
     the CFL / viscous rule, the combined advection-diffusion bound of theta, the free-fall
@@ -888,11 +924,12 @@ def wall_heat_rates(th, theta, u, ops):
     return out
 
 
-def step3d_phys(state: State3D, geom, ops, th, theta, dt=None, out=None):
+def step3d_phys(state: State3D, geom, ops, th, theta, dt=None, out=None, pid=None):
     """One SSP-RK3 step with heat transfer (core.py:765-883 with ``phys`` holding a ThermalOps):
     returns (State3D, coefficients, new theta); ``th.heat_rate`` holds the bodies' heat rates.
     ``out`` (a dict) receives ``div_pre``, the squared norm of the last stage's projection
-    right-hand side (Physics3D.record_divergence, physics.py:1303-1306)."""
+    right-hand side (Physics3D.record_divergence, physics.py:1303-1306). ``pid``: a
+    :class:`PidDt3` applied after the dt rule (physics.py:1260-1261)."""
     u0, v0, w0 = state.u, state.v, state.w
     up, vp, wp = u0, v0, w0
     acc = None
@@ -907,8 +944,12 @@ def step3d_phys(state: State3D, geom, ops, th, theta, dt=None, out=None):
         lu, lv, lw, nut_max = momentum_rhs3d(up, vp, wp, geom, ops, out=extra)
         if s == 0:
             if dt is None:
-                dt = (ops.extras["dt_fixed"] if ops.dt_fixed is not None
-                      else thermal_dt(th, u0, v0, w0, ops, nut_max, t0))
+                if ops.dt_fixed is not None:
+                    dt = ops.extras["dt_fixed"]
+                else:
+                    dt = thermal_dt(th, u0, v0, w0, ops, nut_max, t0)
+                    if pid is not None:
+                        dt = pid(dt)
             if not torch.is_tensor(dt):
                 dt = torch.tensor(float(dt), dtype=ops.dtype)
             th.heat_acc.zero_()

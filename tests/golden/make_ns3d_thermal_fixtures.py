@@ -16,7 +16,12 @@ rates of every step, the wall heat rates and wall temperatures at the end.
 case from the same start (``load_state_dict``), advanced 7 then 13 steps: its diagnostics rows
 (one per 10-step block) and the state after 20 steps.
 
-Usage: ``python tests/golden/make_ns3d_thermal_fixtures.py`` (about a minute on 8 cores).
+``ns3d_pid_box.npz``: ``core.step3d_phys`` on the "box" case from the same start with
+``adaptive_dt="pid"`` (``pid_target`` 0.8) for 30 steps: dt and the controller memory
+(dt_prev, e1, e2) after every step, u, v, w, p and theta after the last.
+
+Usage: ``python tests/golden/make_ns3d_thermal_fixtures.py [pid]`` (about a minute on 8 cores;
+``pid``: only the PID fixture).
 """
 from __future__ import annotations
 
@@ -167,8 +172,43 @@ def diag_case():
     return out
 
 
+def pid_case(steps=30, target=0.8):
+    spec = CASES["box"]
+    fx = dict(np.load(OUT / "ns3d_thermal_box.npz"))
+    bodies = [RMesh(fx[f"mesh{i}_v"], fx[f"mesh{i}_f"], name=f"b{i}", thermal=m, thermal_value=val)
+              for i, (_, _, _, m, val) in enumerate(spec["bodies"])]
+    cfg = RConfig(**spec["grid"], bodies=bodies, device="cpu", dtype="float64", dye=False, perturb=False,
+                  ref_area=0.3, length_ref=0.8, span_ref=0.6, moment_ref=(0.2, 0.0, 0.5),
+                  thermal=RThermal(**spec["thermal"]), heat_sources=[RHeat(**h) for h in spec["heaters"]],
+                  adaptive_dt="pid", pid_target=target)
+    ops = rcore.make_operators3d(cfg)
+    g = rgeo.build_geometry3d(bodies, ops, use_shared=False)
+    phys = Physics3D(cfg, ops, g, bodies)
+    z = torch.zeros((), dtype=torch.float64)
+    st = rcore.State3D(torch.as_tensor(fx["u0"]), torch.as_tensor(fx["v0"]), torch.as_tensor(fx["w0"]),
+                       torch.zeros(ops.nz, ops.ny, ops.nx, dtype=torch.float64), z.clone(), z.clone())
+    theta = torch.as_tensor(fx["theta0"])
+    dts, mem = [], []
+    with torch.no_grad():
+        for _ in range(steps):
+            r = rcore.step3d_phys(st, g, ops, phys=phys, theta=theta)
+            st, theta = r.state, r.theta
+            dts.append(float(st.dt))
+            mem.append([float(phys.pid.dt_prev), float(phys.pid.e1), float(phys.pid.e2)])
+    out = {"dt": np.array(dts), "ctrl": np.array(mem), "pid_target": np.float64(target), "steps": np.int64(steps)}
+    for f in ("u", "v", "w", "p"):
+        out[f"{f}_{steps}"] = getattr(st, f).numpy().copy()
+    out[f"theta_{steps}"] = theta.numpy().copy()
+    return out
+
+
 def main():
     torch.set_num_threads(8)
+    if sys.argv[1:] == ["pid"]:
+        np.savez_compressed(OUT / "ns3d_pid_box.npz", **pid_case())
+        print("wrote pid_box", flush=True)
+        return
+    np.savez_compressed(OUT / "ns3d_pid_box.npz", **pid_case())
     for name, spec in CASES.items():
         np.savez_compressed(OUT / f"ns3d_thermal_{name}.npz", **run(name, spec))
         print("wrote", name, flush=True)

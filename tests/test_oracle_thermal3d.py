@@ -337,3 +337,79 @@ def test_oracle_diagnostics_match_reference():
         else:
             assert np.allclose(mine, theirs, rtol=1e-9, atol=1e-13), (k, mine, theirs)
     assert float(np.abs(theta.numpy() - ref["theta_20"]).max()) < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# PID step (adaptive_dt="pid", physics.py:1144-1177)
+# ---------------------------------------------------------------------------
+
+def _pid_fixture():
+    path = GOLDEN / "ns3d_pid_box.npz"
+    if not path.exists():
+        pytest.skip("missing fixture ns3d_pid_box.npz")
+    return dict(np.load(path))
+
+
+def _oracle_pid_run(fx_box, target, steps):
+    cfg, ops, g, d = _setup("box", fx_box)
+    th = O.Thermal(d, ops, cfg.re)
+    pid = O.PidDt3(target)
+    z = torch.zeros((), dtype=torch.float64)
+    st = O.State3D(_t(fx_box["u0"]), _t(fx_box["v0"]), _t(fx_box["w0"]),
+                   torch.zeros(ops.nz, ops.ny, ops.nx, dtype=torch.float64), z.clone(), z.clone())
+    theta = _t(fx_box["theta0"])
+    dts, mem = [], []
+    with torch.no_grad():
+        for _ in range(steps):
+            st, _, theta = O.step3d_phys(st, g, ops, th, theta, pid=pid)
+            dts.append(float(st.dt))
+            mem.append(pid.mem.numpy().copy())
+    return np.array(dts), np.stack(mem), st, theta
+
+
+def test_pid3_closed_forms():
+    cap = torch.tensor(0.02, dtype=torch.float64)
+    p = O.PidDt3(0.85)
+    assert float(p(cap)) == pytest.approx(0.017, rel=1e-15)          # first step: the target
+    assert np.allclose(p.mem.numpy(), [0.017, 0.0, 0.0], rtol=1e-15, atol=0)
+    assert float(p(cap)) == pytest.approx(0.017, rel=1e-14)          # at the target: no change
+    p.mem = torch.tensor([1e-4, 0.0, 0.0], dtype=torch.float64)      # far below: factor clamp 1.2
+    assert float(p(cap)) == pytest.approx(1.2e-4, rel=1e-14)
+    assert float(p.mem[1]) == pytest.approx(math.log(0.017 / 1e-4), rel=1e-14)
+    p.mem = torch.tensor([0.5, 0.0, 0.0], dtype=torch.float64)       # far above: clamp 0.5, then the cap
+    assert float(p(cap)) == pytest.approx(0.02, rel=1e-15)
+    p.mem = torch.tensor([0.03, 0.0, 0.0], dtype=torch.float64)      # above the cap: 0.5 x 0.03 < cap
+    e = math.log(0.017 / 0.03)
+    assert float(p(cap)) == pytest.approx(0.03 * max(math.exp(0.75 * e), 0.5), rel=1e-14)
+    p.mem = torch.tensor([0.016, 0.01, -0.02], dtype=torch.float64)  # all three terms
+    e, e1, e2 = math.log(0.017 / 0.016), 0.01, -0.02
+    fac = math.exp(0.3 * e + 0.4 * (e - e1) + 0.05 * (e - 2 * e1 + e2))
+    assert float(p(cap)) == pytest.approx(0.016 * fac, rel=1e-14)
+    assert np.allclose(p.mem.numpy(), [0.016 * fac, e, e1], rtol=1e-13, atol=0)
+
+
+def test_oracle_pid3_matches_reference():
+    """dt and the controller memory of every step of the reference's 30-step PID run on the
+    heated box (pid_target 0.8), and the fields after it."""
+    fx, ref = _fixture("box"), _pid_fixture()
+    n = int(ref["steps"])
+    dts, mem, st, theta = _oracle_pid_run(fx, float(ref["pid_target"]), n)
+    assert np.allclose(dts, ref["dt"], rtol=1e-12, atol=0)
+    assert np.allclose(mem, ref["ctrl"], rtol=1e-10, atol=1e-13)
+    for f in ("u", "v", "w", "p"):
+        r = ref[f"{f}_{n}"]
+        assert float(np.abs(getattr(st, f).numpy() - r).max()) <= 1e-11 * max(float(np.abs(r).max()), 1.0), f
+    assert float(np.abs(theta.numpy() - ref[f"theta_{n}"]).max()) < 1e-11
+    # the controller steers: growth never above 1.2 per step and at least one step off the cap
+    assert np.all(dts[1:] <= 1.2 * dts[:-1] * (1 + 1e-12))
+
+
+def test_pid3_planted_gain_is_caught():
+    fx, ref = _fixture("box"), _pid_fixture()
+    saved = O.PID_GAINS_3D
+    try:
+        O.PID_GAINS_3D = (0.4, 0.3, 0.05)      # kI and kP swapped
+        dts, _, _, _ = _oracle_pid_run(fx, float(ref["pid_target"]), 5)
+    finally:
+        O.PID_GAINS_3D = saved
+    assert not np.allclose(dts, ref["dt"][:5], rtol=1e-6)

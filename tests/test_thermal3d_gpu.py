@@ -213,3 +213,86 @@ def test_solver_diagnostics_match_reference_rows(tmp_path):
     text = open(path).read().splitlines()
     assert text[0].startswith("step,t,dt,cfl") and len(text) == 3
     assert s.latest_diagnostics()["step"] == 20.0
+
+
+# ---------------------------------------------------------------------------
+# PID step (adaptive_dt="pid")
+# ---------------------------------------------------------------------------
+
+def _pid_fixture():
+    path = GOLDEN / "ns3d_pid_box.npz"
+    if not path.exists():
+        pytest.skip("missing fixture ns3d_pid_box.npz")
+    return dict(np.load(path))
+
+
+def test_pid_steps_match_reference_float64():
+    """dt and the controller memory of each of the reference's 30 PID steps on the heated box
+    (pid_target 0.8), and the fields after them, through the functional step."""
+    print("[thermal gpu] PID steps float64 vs reference", flush=True)
+    fx, ref = _fixture("box"), _pid_fixture()
+    cfg, ops, g, th = _gpu_setup("box", fx)
+    st, theta = _state(fx, ops)
+    pid = S3.PidDt3(ops, float(ref["pid_target"]))
+    n = int(ref["steps"])
+    for k in range(n):
+        out = S3.step3d_phys(st, g, ops, thermal=th, theta=theta, pid=pid)
+        st, theta = out.state, out.theta
+        assert float(st.dt) == pytest.approx(float(ref["dt"][k]), rel=1e-12), k
+        assert np.allclose(pid.mem.cpu().numpy(), ref["ctrl"][k], rtol=1e-10, atol=1e-13), k
+    for f in ("u", "v", "w", "p"):
+        assert _rel(getattr(st, f), ref[f"{f}_{n}"]) < 1e-10, f
+    assert float(np.abs(theta.double().cpu().numpy() - ref[f"theta_{n}"]).max()) < 1e-11
+
+
+def test_pid_solver_graphs_chained_and_float32():
+    """The FlowSolver3D PID run (graphs on and off, 7 + 23 steps, chained dt) equals the
+    reference's 30 steps; reset starts the controller over; float32 against the oracle."""
+    print("[thermal gpu] PID FlowSolver3D", flush=True)
+    fx, ref = _fixture("box"), _pid_fixture()
+    n = int(ref["steps"])
+    runs = []
+    for graphs in (True, False):
+        cfg = cases.thermal_cfg("box", fx, "float64", device="cuda", use_cuda_graphs=graphs, adaptive_dt="pid",
+                                pid_target=float(ref["pid_target"]))
+        s = S3.FlowSolver3D(cfg)
+        s.set_velocity(fx["u0"], fx["v0"], fx["w0"])
+        s.set_temperature(fx["theta0"])
+        s.advance(7)
+        s.advance(n - 7)
+        h = s.history()
+        assert np.allclose(np.diff(np.r_[0.0, h["t"]]), ref["dt"], rtol=1e-11, atol=0)
+        assert np.allclose(s._pid.mem.cpu().numpy(), ref["ctrl"][-1], rtol=1e-10, atol=1e-13)
+        assert _rel(s.state.u, ref[f"u_{n}"]) < 1e-10
+        runs.append(s)
+    a, b = runs
+    for f in ("u", "v", "w", "p", "t", "dt"):
+        assert torch.equal(getattr(a.state, f), getattr(b.state, f)), f
+    assert torch.equal(a._pid.mem, b._pid.mem)
+    sd = a.state_dict()
+    assert "pid" in sd
+    a.reset()
+    assert float(a._pid.mem.abs().max()) == 0.0
+    # float32 against the oracle over 8 steps
+    cfg, ops, g, th = _gpu_setup("box", fx, "float32")
+    st, theta = _state(fx, ops)
+    pid = S3.PidDt3(ops, float(ref["pid_target"]))
+    ocfg = cases.thermal_cfg("box", fx, "float32")
+    oops = su.make_operators3d(ocfg, device="cpu")
+    og = su.build_geometry3d(ocfg.bodies, oops)
+    oth = O.Thermal(T.build_thermal(ocfg, oops, ocfg.bodies, og.window), oops, ocfg.re)
+    opid = O.PidDt3(float(ref["pid_target"]), dtype=torch.float32)
+    z = torch.zeros((), dtype=torch.float32)
+    f32 = lambda a: torch.as_tensor(np.asarray(a), dtype=torch.float32)  # noqa: E731
+    ost = O.State3D(f32(fx["u0"]), f32(fx["v0"]), f32(fx["w0"]), torch.zeros(oops.nz, oops.ny, oops.nx), z.clone(),
+                    z.clone())
+    otheta = f32(fx["theta0"])
+    for _ in range(8):
+        out = S3.step3d_phys(st, g, ops, thermal=th, theta=theta, pid=pid)
+        st, theta = out.state, out.theta
+        ost, _, otheta = O.step3d_phys(ost, og, oops, oth, otheta, pid=opid)
+        assert float(st.dt) == pytest.approx(float(ost.dt), rel=2e-5)
+    assert float((st.u.cpu() - ost.u).abs().max()) < 2e-5
+    assert float((theta.cpu() - otheta).abs().max()) < 2e-5
+    # and its error against the float64 reference dt
+    assert float(st.dt) == pytest.approx(float(ref["dt"][7]), rel=1e-4)

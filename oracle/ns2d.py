@@ -23,6 +23,9 @@ Pinning (what checks each function besides retyping it):
 * ``pid_dt`` - the reference's PID trajectory (``tests/golden/ns2d_pid_diag.npz``, written by
   ``tests/golden/make_ns2d_diag_fixtures.py``) and the controller's closed forms (first step,
   fixed point at the target, growth cap, clip at the limit).
+* ``diagnostics_row`` - the reference FlowSolver's rows at steps 10 and 20 (same fixture), the
+  residual column's closed forms (exact potential, half of it, an incompatible right-hand
+  side), a planted change norm without its h factor.
 """
 from __future__ import annotations
 

@@ -10,7 +10,11 @@ float64. ``ns2d_pid_diag.npz`` holds inputs and expected outputs only (numpy arr
   after every step, u, v, p, theta after the last;
 * the reference ``FlowSolver`` with ``diagnostics=True`` on the same case and start, advanced
   7 then 13 steps: its diagnostics rows (one per 10-step block, columns ``DIAG_COLUMNS``)
-  and the state after 20 steps.
+  and the state after 20 steps;
+* the same solver's host reports after those 20 steps, with running means started after step
+  7: ``surface(48)`` and ``surface(48, mean=True)`` (``surf_`` / ``msurf_`` + key),
+  ``heat_transfer()`` / ``heat_balance()`` and their ``mean=True`` forms (``ht_`` / ``mht_``,
+  ``hb_`` / ``mhb_``), ``mean_fields()`` (``mean_`` + key) and ``averaging_time``.
 
 Usage: ``python tests/golden/make_ns2d_diag_fixtures.py`` (a few seconds).
 """
@@ -85,8 +89,29 @@ def main():
     s = FlowSolver(config(diagnostics=True))
     s._state = s._state._replace(u=u.clone(), v=v.clone(), theta=theta.clone())
     s.advance(7)
+    s.start_averaging()
     s.advance(13)
     d = s.diagnostics()
+    for pre, mean in (("surf_", False), ("msurf_", True)):
+        rec = s.surface(48, mean=mean)[0]
+        for k, val in rec.items():
+            if k != "name":
+                arrs[pre + k] = np.asarray(val, dtype=np.float64)
+    for pre, mean in (("ht_", False), ("mht_", True)):
+        ht = s.heat_transfer(mean=mean)
+        b = ht["bodies"][0]
+        for k in ("perimeter", "heat_rate", "theta_wall", "nusselt"):
+            arrs[pre + k] = np.float64(b[k])
+        arrs[pre + "walls"] = np.array([ht["walls"][w] for w in ("left", "right", "bottom", "top")])
+        arrs[pre + "nusselt_mean"] = np.float64(ht["nusselt_mean"])
+        arrs[pre + "pe"] = np.float64(ht["pe"])
+    for pre, mean in (("hb_", False), ("mhb_", True)):
+        hb = s.heat_balance(mean=mean)
+        arrs[pre + "keys"] = np.array(list(hb))
+        arrs[pre + "values"] = np.array([hb[k] for k in hb], dtype=np.float64)
+    for k, val in s.mean_fields().items():
+        arrs["mean_" + k] = np_(val)
+    arrs["averaging_time"] = np.float64(s.averaging_time)
     for k in DIAG_COLUMNS:
         arrs["diag_" + k] = np.asarray(d[k])
     arrs["diag_columns"] = np.array(DIAG_COLUMNS)

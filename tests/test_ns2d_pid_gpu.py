@@ -189,3 +189,81 @@ def test_functional_diag_row_and_no_theta():
                 assert g[i] < 1e-12 and c[i] < 1e-12, (i, g[i], c[i])
             else:
                 assert g[i] == pytest.approx(c[i], rel=1e-9, abs=1e-14), (thermal, i, g[i], c[i])
+
+
+def test_host_reports_match_reference():
+    """surface(), heat_transfer(), heat_balance() (instantaneous and of the running means),
+    mean_fields() and averaging_time after the reference's 7 + 13 step run with averaging
+    started after step 7, against the reference FlowSolver's own reports (float64)."""
+    print("[pid gpu] host reports vs reference", flush=True)
+    fx = _fixture()
+    cfg = dataclasses.replace(cases.pid_diag_cfg("float64", device="cuda"), use_cuda_graphs=True, diagnostics=True)
+    s = S.FlowSolver(cfg)
+    s.set_velocity(fx["u0"], fx["v0"])
+    s.set_temperature(fx["theta0"])
+    s.advance(7)
+    s.start_averaging()
+    s.advance(13)
+    assert s.averaging_time == pytest.approx(float(fx["averaging_time"]), rel=1e-12)
+    assert s.peclet == pytest.approx(400 * 0.71)
+    for pre, mean in (("surf_", False), ("msurf_", True)):
+        rec = s.surface(48, mean=mean)[0]
+        for k in ("s", "x", "y", "nx", "ny", "cp", "cf", "theta_wall", "nu_local"):
+            assert np.allclose(rec[k], fx[pre + k], rtol=1e-9, atol=1e-12), (pre, k)
+    for pre, mean in (("ht_", False), ("mht_", True)):
+        ht = s.heat_transfer(mean=mean)
+        b = ht["bodies"][0]
+        for k in ("perimeter", "heat_rate", "theta_wall", "nusselt"):
+            assert b[k] == pytest.approx(float(fx[pre + k]), rel=1e-9), (pre, k)
+        walls = [ht["walls"][w] for w in ("left", "right", "bottom", "top")]
+        assert np.allclose(walls, fx[pre + "walls"], rtol=1e-9, atol=1e-13), pre
+        assert ht["nusselt_mean"] == pytest.approx(float(fx[pre + "nusselt_mean"]), rel=1e-9)
+    for pre, mean in (("hb_", False), ("mhb_", True)):
+        hb = s.heat_balance(mean=mean)
+        assert list(hb) == [str(k) for k in fx[pre + "keys"]]
+        assert np.allclose([hb[k] for k in hb], fx[pre + "values"], rtol=1e-8, atol=1e-12), pre
+    mf = s.mean_fields()
+    for k in ("u", "v", "speed", "p", "cp", "vorticity", "u_rms", "v_rms", "p_rms", "cp_rms", "mask", "theta",
+              "theta_rms"):
+        ref, mine = fx["mean_" + k], mf[k].double().cpu().numpy()
+        if k.endswith("_rms"):
+            # sqrt(E[x^2] - E[x]^2): compare the variances, which carry the round-off linearly
+            err = float(np.abs(mine ** 2 - ref ** 2).max())
+            assert err <= 1e-13, (k, err)
+            continue
+        err = float(np.abs(mine - ref).max())
+        assert err <= 1e-9 * max(float(np.abs(ref).max()), 1.0), (k, err)
+    # trim_history keeps the rows of the last 0.2 convective times
+    t_last = s.history()["t"][-1]
+    s.trim_history(0.2)
+    h = s.history()
+    assert h["t"][0] >= t_last - 0.2 and h["t"][-1] == t_last
+    d = s.diagnostics()
+    assert list(d["step"]) == [20.0]
+
+
+def test_set_bodies_keeps_flow_and_equals_fresh_solver():
+    """Bodies replaced live: the next steps equal a solver built with the new bodies and started
+    from the same state (the window moves, the plan is rebuilt, graphs re-captured)."""
+    print("[pid gpu] set_bodies", flush=True)
+    fx = _fixture()
+    from aerolume_amd.ns2d.config import ImmersedBody
+    new = [ImmersedBody(cases.circle(64, 0.6, 3.0, 1.8), thermal="temperature", thermal_value=1.0, name="moved")]
+    cfg = dataclasses.replace(cases.pid_diag_cfg("float64", device="cuda"), use_cuda_graphs=True)
+    a = S.FlowSolver(cfg)
+    a.set_velocity(fx["u0"], fx["v0"])
+    a.set_temperature(fx["theta0"])
+    a.advance(5)
+    sd = a.state_dict()
+    a.set_bodies(new)
+    assert a.bodies[0].name == "moved" and a.geometry.window != S.FlowSolver(cfg).geometry.window
+    a.advance(15)
+    b = S.FlowSolver(dataclasses.replace(cfg, bodies=new))
+    b.load_state_dict(sd)
+    b.advance(15)
+    for f in ("u", "v", "p", "theta", "t", "dt", "ctrl"):
+        assert torch.equal(getattr(a.state, f), getattr(b.state, f)), f
+    assert np.array_equal(a.coefficients(), b.coefficients())
+    ha = a.history()
+    assert len(ha["t"]) == 20 and np.array_equal(ha["cd"][-15:], b.history()["cd"][-15:])
+    assert np.array_equal(a.physics_history()["heat:moved"][-15:], b.physics_history()["heat:moved"][-15:])

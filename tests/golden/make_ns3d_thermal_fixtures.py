@@ -13,15 +13,18 @@ theta after the first and the last step, dt, t, the six coefficients and the bod
 rates of every step, the wall heat rates and wall temperatures at the end.
 
 ``ns3d_diag_box.npz``: the reference's ``FlowSolver3D`` with ``diagnostics=True`` on the "box"
-case from the same start (``load_state_dict``), advanced 7 then 13 steps: its diagnostics rows
-(one per 10-step block) and the state after 20 steps.
+case from the same start (``load_state_dict``), advanced 7 then 13 steps with running means
+started after step 7: its diagnostics rows (one per 10-step block), the state after 20 steps,
+``mean_fields()`` (``mean_`` + key), ``averaging_time``, slices (``slice_<axis>_<name>`` and
+``mslice_`` of the means), ``surface()`` / ``surface(mean=True)`` of body 0 (``surf_`` /
+``msurf_`` + key) and ``heat_balance()`` (``hb_keys``, ``hb_values``).
 
 ``ns3d_pid_box.npz``: ``core.step3d_phys`` on the "box" case from the same start with
 ``adaptive_dt="pid"`` (``pid_target`` 0.8) for 30 steps: dt and the controller memory
 (dt_prev, e1, e2) after every step, u, v, w, p and theta after the last.
 
 Usage: ``python tests/golden/make_ns3d_thermal_fixtures.py [pid]`` (about a minute on 8 cores;
-``pid``: only the PID fixture).
+``pid`` / ``diag``: only that fixture).
 """
 from __future__ import annotations
 
@@ -160,9 +163,27 @@ def diag_case():
                        "theta": torch.as_tensor(fx["theta0"]), "steps": 0})
     with torch.no_grad():
         s.advance(7)
+        s.start_averaging()
         s.advance(13)
     d = s.diagnostics()
     out = {k: np.asarray(d[k]) for k in DIAG_KEYS}
+    with torch.no_grad():
+        for k, v in s.mean_fields().items():
+            out["mean_" + k] = v.numpy().copy()
+        out["averaging_time"] = np.float64(s.averaging_time)
+        for axis, pos in (("x", 0.37), ("y", 0.5), ("z", 0.21)):
+            for name in ("u", "p", "temperature", "q"):
+                out[f"slice_{axis}_{name}"] = s.slice(axis, pos, name).numpy().copy()
+            for name in ("w", "u_rms", "temperature_rms"):
+                out[f"mslice_{axis}_{name}"] = s.slice(axis, pos, name, mean=True).numpy().copy()
+        for pre, mean in (("surf_", False), ("msurf_", True)):
+            rec = s.surface(mean=mean)[0]
+            for k, v in rec.items():
+                if k != "name":
+                    out[pre + k] = np.asarray(v)
+        hb = s.heat_balance()
+        out["hb_keys"] = np.array(list(hb))
+        out["hb_values"] = np.array([hb[k] for k in hb], dtype=np.float64)
     out["step"] = np.asarray(d["step"])
     st = s.state
     for f in ("u", "v", "w", "p"):
@@ -207,6 +228,10 @@ def main():
     if sys.argv[1:] == ["pid"]:
         np.savez_compressed(OUT / "ns3d_pid_box.npz", **pid_case())
         print("wrote pid_box", flush=True)
+        return
+    if sys.argv[1:] == ["diag"]:
+        np.savez_compressed(OUT / "ns3d_diag_box.npz", **diag_case())
+        print("wrote diag_box", flush=True)
         return
     np.savez_compressed(OUT / "ns3d_pid_box.npz", **pid_case())
     for name, spec in CASES.items():

@@ -296,3 +296,80 @@ def test_pid_solver_graphs_chained_and_float32():
     assert float((theta.cpu() - otheta).abs().max()) < 2e-5
     # and its error against the float64 reference dt
     assert float(st.dt) == pytest.approx(float(ref["dt"][7]), rel=1e-4)
+
+
+def _rms_close(mine, ref, tol=1e-12):
+    """rms fields compare by their squares (the square root magnifies round-off near zero)."""
+    return float(np.abs(mine ** 2 - ref ** 2).max()) <= tol * max(float((ref ** 2).max()), 1.0)
+
+
+def test_solver_host_reports_match_reference():
+    """Running means (time-weighted, sampled every 10 steps), slices, surface probes and the
+    heat balance of the reference FlowSolver3D's 7 + 13 step run with averaging started after
+    step 7 (float64), graphs on."""
+    print("[thermal gpu] host reports", flush=True)
+    fx = _fixture("box")
+    ref = dict(np.load(GOLDEN / "ns3d_diag_box.npz"))
+    if "averaging_time" not in ref:
+        pytest.skip("fixture without host reports")
+    cfg = cases.thermal_cfg("box", fx, "float64", device="cuda", diagnostics=True, use_cuda_graphs=True)
+    s = S3.FlowSolver3D(cfg)
+    s.set_velocity(fx["u0"], fx["v0"], fx["w0"])
+    s.set_temperature(fx["theta0"])
+    s.advance(7)
+    s.start_averaging()
+    s.advance(13)
+    assert s.averaging_time == pytest.approx(float(ref["averaging_time"]), rel=1e-12)
+    mf = s.mean_fields()
+    assert set(mf) == {k[5:] for k in ref if k.startswith("mean_")}
+    for k, v in mf.items():
+        mine, r = v.double().cpu().numpy(), ref["mean_" + k]
+        if k.endswith("_rms"):
+            assert _rms_close(mine, r), k
+        else:
+            assert float(np.abs(mine - r).max()) <= 1e-9 * max(float(np.abs(r).max()), 1.0), k
+    for key in [k for k in ref if k.startswith("slice_") or k.startswith("mslice_")]:
+        pre, axis, name = key.split("_", 2)
+        pos = {"x": 0.37, "y": 0.5, "z": 0.21}[axis]
+        mine = s.slice(axis, pos, name, mean=pre == "mslice").double().cpu().numpy()
+        if name.endswith("_rms"):
+            assert _rms_close(mine, ref[key]), key
+        else:
+            assert float(np.abs(mine - ref[key]).max()) <= 1e-9 * max(float(np.abs(ref[key]).max()), 1.0), key
+    for pre, mean in (("surf_", False), ("msurf_", True)):
+        rec = s.surface(mean=mean)[0]
+        for k in ("vertices", "faces", "normals", "cp", "cf", "tau", "theta_wall", "heat_flux"):
+            assert np.allclose(rec[k], ref[pre + k], rtol=1e-8, atol=1e-10), (pre, k)
+    hb = s.heat_balance()
+    assert list(hb) == [str(k) for k in ref["hb_keys"]]
+    assert np.allclose([hb[k] for k in hb], ref["hb_values"], rtol=1e-8, atol=1e-10)
+    t_last = s.history()["t"][-1]
+    s.trim_history(0.1)
+    assert s.history()["t"][0] >= t_last - 0.1 and list(s.diagnostics()["step"]) == [20.0]
+
+
+def test_solver_set_bodies_equals_fresh_solver():
+    """Bodies moved live: the next steps equal a solver built with the moved bodies and started
+    from the same state (float64; masks, heat-equation data and plan rebuilt)."""
+    print("[thermal gpu] set_bodies", flush=True)
+    from aerolume_amd.ns3d.config import ImmersedMesh
+    fx = _fixture("box")
+    cfg = cases.thermal_cfg("box", fx, "float64", device="cuda", use_cuda_graphs=True)
+    moved = [ImmersedMesh(b.vertices_numpy() + np.array([0.06, 0.0, 0.03]), b.faces, name=b.name,
+                          thermal=b.thermal, thermal_value=b.thermal_value) for b in cfg.bodies]
+    a = S3.FlowSolver3D(cfg)
+    a.set_velocity(fx["u0"], fx["v0"], fx["w0"])
+    a.set_temperature(fx["theta0"])
+    a.advance(5)
+    sd = a.state_dict()
+    a.set_bodies(moved)
+    a.advance(15)
+    import dataclasses
+    b = S3.FlowSolver3D(dataclasses.replace(cfg, bodies=moved))
+    b.load_state_dict(sd)
+    b.advance(15)
+    for f in ("u", "v", "w", "p", "t", "dt"):
+        assert torch.equal(getattr(a.state, f), getattr(b.state, f)), f
+    assert torch.equal(a.theta, b.theta)
+    assert np.array_equal(a.coefficients(), b.coefficients())
+    assert len(a.history()["t"]) == 20

@@ -96,6 +96,8 @@ def load() -> ctypes.CDLL:
     for name in ("aero2d_plan_create", "aero2d_plan_destroy", "aero2d_step", "aero2d_project",
                  "aero2d_poisson_solve"):
         getattr(lib, name).restype = ctypes.c_int
+    lib.aero_polygon_sdf.argtypes = [P, ctypes.c_int32, P, ctypes.c_int32, P, P, ctypes.c_int64, P]
+    lib.aero_polygon_sdf.restype = ctypes.c_int
     from . import _lib3d
     _lib3d.bind(lib)
     _bind_ensemble(lib)

@@ -317,6 +317,15 @@ int aero3d_cell_gradient(aero3d_plan *plan, void *stream, const void *u, const v
 int aero3d_time_stage(aero3d_plan *plan, void *stream, const aero3d_fields *fields, const aero3d_geom *geom,
                       const aero3d_params *params, int stage, int iters, double *avg_ms);
 
+/* ------------------------------------------------------------------ geometry set-up */
+/* Exact signed distance (negative inside) of npts points (x, y arrays of the dtype: 0 float32,
+ * 1 float64) to the closed polygon poly (n float64 (x, y) pairs: the vertices as rounded to
+ * the dtype), into out (npts values of the dtype), on `stream`. Replaces the brute-force
+ * distance of the 2D mask builder (ns2d/geometry.py:229-345 via geometry/sdf.py:322-392):
+ * nearest segment in float64 (first minimum), even-odd crossing sign, distance in the dtype. */
+int aero_polygon_sdf(void *stream, int32_t dtype, const double *poly, int32_t n, const void *x, const void *y,
+                     int64_t npts, void *out);
+
 /* ------------------------------------------------------------------ 2D ensemble */
 /* Many independent 2D runs of uneven size (angle / Reynolds sweeps, polars) advancing together:
  * each launch of a step covers the blocks of every run through a device work table, so an

@@ -63,6 +63,16 @@ class Outputs2D(ctypes.Structure):
 #: columns of a 2D diagnostics row (AERO2D_DIAG_COLS; float64)
 DIAG_COLS = 22
 
+class ParticlesGrid2D(ctypes.Structure):
+    _fields_ = [("dtype", i32), ("nx", i32), ("ny", i32), ("reserved", i32), ("x0", f64), ("y0", f64), ("h", f64)]
+
+
+class Particles2D(ctypes.Structure):
+    _fields_ = [(k, vp) for k in ("pos", "birth", "alive", "head", "acc", "last", "noz_pos", "noz_owner", "rate",
+                                  "timing", "pulsed")] + [("n_nozzles", i32), ("n_emitters", i32), ("ring", i32),
+                                                          ("reserved", i32)]
+
+
 _LIB = None
 
 
@@ -98,6 +108,9 @@ def load() -> ctypes.CDLL:
         getattr(lib, name).restype = ctypes.c_int
     lib.aero_polygon_sdf.argtypes = [P, ctypes.c_int32, P, ctypes.c_int32, P, P, ctypes.c_int64, P]
     lib.aero_polygon_sdf.restype = ctypes.c_int
+    lib.aero2d_particles_step.argtypes = [P, ctypes.POINTER(ParticlesGrid2D), P, P, P, P, P,
+                                          ctypes.POINTER(Particles2D)]
+    lib.aero2d_particles_step.restype = ctypes.c_int
     from . import _lib3d
     _lib3d.bind(lib)
     _bind_ensemble(lib)

@@ -13,7 +13,7 @@ from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
-SOURCES = ("ns2d.hip", "ns3d.hip", "cnn3d.hip", "geometry.hip")
+SOURCES = ("ns2d.hip", "ns3d.hip", "cnn3d.hip", "geometry.hip", "particles.hip")
 LIB = HERE / "libaerolume_amd.so"
 ARCH = os.environ.get("AEROLUME_AMD_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

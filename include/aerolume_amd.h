@@ -326,6 +326,33 @@ int aero3d_time_stage(aero3d_plan *plan, void *stream, const aero3d_fields *fiel
 int aero_polygon_sdf(void *stream, int32_t dtype, const double *poly, int32_t n, const void *x, const void *y,
                      int64_t npts, void *out);
 
+/* ------------------------------------------------------------------ 2D tracer particles */
+/* The grid the velocities live on: u (ny, nx+1), v (ny+1, nx), origin (x0, y0), cell h. */
+typedef struct {
+  int32_t dtype, nx, ny, reserved;
+  double x0, y0, h;
+} aero2d_particles_grid;
+
+/* A fixed particle pool (ns2d/particles.py:59-93), device buffers of the dtype unless noted:
+ * pos (capacity, 2), birth, alive (capacity) with capacity = n_nozzles * ring, nozzle n owning
+ * slots [n ring, (n + 1) ring); head (n_nozzles, int64) its next slot; per emitter acc (release
+ * accumulator), last (last pulse index), rate (per nozzle and unit time), timing (4: start,
+ * stop, period, duty), pulsed (1: pulsed batches); noz_pos (n_nozzles, 2), noz_owner
+ * (n_nozzles, int64) the emitter of each nozzle. */
+typedef struct {
+  void *pos, *birth, *alive, *head, *acc, *last;
+  const void *noz_pos, *noz_owner, *rate, *timing, *pulsed;
+  int32_t n_nozzles, n_emitters, ring, reserved;
+} aero2d_particles;
+
+/* One particle step after a flow step (ns2d/particles.py:122-173): advect every slot with the
+ * end-of-step (u, v) over dt (midpoint rule, bilinear samples on the staggered faces), kill
+ * slots that leave the box or land where solid (ny, nx; may be NULL) exceeds 1/2, then run the
+ * emitters' release clocks at time t (the step's end) and release into the nozzles' rings.
+ * dt and t are device scalars; two launches on `stream`, nothing allocated (graph-safe). */
+int aero2d_particles_step(void *stream, const aero2d_particles_grid *grid, const void *u, const void *v,
+                          const void *dt, const void *t, const void *solid, aero2d_particles *particles);
+
 /* ------------------------------------------------------------------ 2D ensemble */
 /* Many independent 2D runs of uneven size (angle / Reynolds sweeps, polars) advancing together:
  * each launch of a step covers the blocks of every run through a device work table, so an

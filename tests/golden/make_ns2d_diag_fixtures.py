@@ -16,6 +16,11 @@ float64. ``ns2d_pid_diag.npz`` holds inputs and expected outputs only (numpy arr
   ``heat_transfer()`` / ``heat_balance()`` and their ``mean=True`` forms (``ht_`` / ``mht_``,
   ``hb_`` / ``mhb_``), ``mean_fields()`` (``mean_`` + key) and ``averaging_time``.
 
+``ns2d_particles.npz``: the same case and start with two particle emitters (a continuous
+5-nozzle rake, rate 400, and a pulsed 6-nozzle line, period 0.15) on a 512-particle pool, the
+reference ``FlowSolver`` advanced 25 steps: the pool's buffers (pos, birth, alive, head, acc,
+last), the snapshot and the number of polylines.
+
 Usage: ``python tests/golden/make_ns2d_diag_fixtures.py`` (a few seconds).
 """
 from __future__ import annotations
@@ -32,7 +37,7 @@ sys.path.insert(0, str(REF))
 import torch  # noqa: E402
 
 from aerolume.solvers.ns2d import core  # noqa: E402
-from aerolume.solvers.ns2d.config import ImmersedBody, SolverConfig, ThermalConfig  # noqa: E402
+from aerolume.solvers.ns2d.config import EmitterSpec, ImmersedBody, SolverConfig, ThermalConfig  # noqa: E402
 from aerolume.solvers.ns2d.diagnostics import DIAG_COLUMNS  # noqa: E402
 from aerolume.solvers.ns2d.geometry import build_geometry  # noqa: E402
 from aerolume.solvers.ns2d.solver import FlowSolver  # noqa: E402
@@ -122,5 +127,34 @@ def main():
     print("wrote ns2d_pid_diag.npz", dts[:3], d["step"], flush=True)
 
 
+PARTICLE_EMITTERS = dict(
+    rake=dict(position=(0.6, 1.2), kind="rake", end=(0.6, 2.8), count=5, species="particles", rate=400.0),
+    line=dict(position=(1.2, 1.0), kind="line", end=(1.2, 3.0), count=6, species="particles", rate=1.0,
+              pulse_period=0.15, pulse_duty=0.5, start=0.05))
+
+
+def particles_case():
+    ems = [EmitterSpec(**PARTICLE_EMITTERS["rake"]), EmitterSpec(**PARTICLE_EMITTERS["line"])]
+    cfg = config(emitters=ems, particle_capacity=512)
+    ops = core.make_operators(cfg)
+    u, v, theta = start(ops)
+    s = FlowSolver(cfg)
+    s._state = s._state._replace(u=u.clone(), v=v.clone(), theta=theta.clone())
+    s.advance(25)
+    pt = s._particles
+    arrs = {k: np_(getattr(pt, k)).copy() for k in ("pos", "birth", "alive", "head", "acc", "last")}
+    snap = s.particles()
+    for k, val in snap.items():
+        arrs["snap_" + k] = np.asarray(val)
+    arrs["n_lines"] = np.int64(len(s.particle_lines()))
+    arrs["t"] = np.float64(s.t)
+    np.savez_compressed(OUT / "ns2d_particles.npz", **arrs)
+    print("wrote ns2d_particles.npz", int(arrs["alive"].sum()), "alive", flush=True)
+
+
 if __name__ == "__main__":
+    if sys.argv[1:] == ["particles"]:
+        torch.set_num_threads(8)
+        particles_case()
+        sys.exit(0)
     main()

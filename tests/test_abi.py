@@ -97,7 +97,8 @@ def test_ctypes_structures_match_the_header(tmp_path):
     pairs = {"aero2d_desc": _lib.Desc2D, "aero2d_fields": _lib.Fields2D, "aero2d_geom": _lib.Geom2D,
              "aero2d_params": _lib.Params2D, "aero2d_outputs": _lib.Outputs2D, "aero3d_desc": _lib3d.Desc3D,
              "aero3d_fields": _lib3d.Fields3D, "aero3d_geom": _lib3d.Geom3D, "aero3d_params": _lib3d.Params3D,
-             "aero3d_outputs": _lib3d.Outputs3D}
+             "aero3d_outputs": _lib3d.Outputs3D, "aero2d_particles_grid": _lib.ParticlesGrid2D,
+             "aero2d_particles": _lib.Particles2D}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "aerolume_amd.h"', "int main(void) {"]
     for cname, cls in pairs.items():
         lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')

@@ -54,8 +54,8 @@ STEP3_BYTES_PER_CELL = 312
 # HBM bytes per stage-2 launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by the
 # same counters on a 512 MiB device copy (fetch x2.000, write x1.000):
 # profiles/stage_kernels_hbm_counters.md
-TRAFFIC_STAGE_2D = 28.99e6
-TRAFFIC_STAGE_3D = 182.39e6
+TRAFFIC_STAGE_2D = 28.93e6
+TRAFFIC_STAGE_3D = 184.16e6
 
 
 def log(msg: str) -> None:

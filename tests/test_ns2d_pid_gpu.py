@@ -267,3 +267,34 @@ def test_set_bodies_keeps_flow_and_equals_fresh_solver():
     ha = a.history()
     assert len(ha["t"]) == 20 and np.array_equal(ha["cd"][-15:], b.history()["cd"][-15:])
     assert np.array_equal(a.physics_history()["heat:moved"][-15:], b.physics_history()["heat:moved"][-15:])
+
+
+def test_live_physics_edits_equal_fresh_solver():
+    """set_emitters / set_thermal / set_sources on a running solver: the next steps equal a
+    solver built with the edited configuration from the same state (smoke kept, theta kept)."""
+    print("[pid gpu] live physics edits", flush=True)
+    from aerolume_amd.ns2d.config import EmitterSpec, HeatSourceSpec
+    fx = _fixture()
+    e1 = EmitterSpec(position=(0.8, 1.0), kind="line", end=(0.8, 3.0), count=6, rate=1.0)
+    e2 = EmitterSpec(position=(1.1, 1.4), kind="line", end=(1.1, 2.6), count=4, rate=0.8, pulse_period=0.2)
+    heater = HeatSourceSpec(position=(3.2, 2.0), size=(0.3, 0.2), power=2.0)
+    cfg = dataclasses.replace(cases.pid_diag_cfg("float64", device="cuda"), use_cuda_graphs=True, emitters=[e1])
+    a = S.FlowSolver(cfg)
+    a.set_velocity(fx["u0"], fx["v0"])
+    a.set_temperature(fx["theta0"])
+    a.advance(6)
+    sd = a.state_dict()
+    assert float(sd["dye"].abs().max()) > 0
+    a.set_emitters([e2])
+    a.set_thermal(richardson=0.3, theta_inlet=0.1)
+    a.set_sources(heat_sources=[heater])
+    a.advance(8)
+    th = dataclasses.replace(cfg.thermal, richardson=0.3, theta_inlet=0.1)
+    b = S.FlowSolver(dataclasses.replace(cfg, emitters=[e2], thermal=th, heat_sources=[heater]))
+    b.load_state_dict(sd)
+    b.advance(8)
+    for f in ("u", "v", "p", "theta", "dye", "t", "dt", "ctrl"):
+        assert torch.equal(getattr(a.state, f), getattr(b.state, f)), f
+    assert a.physics_history().keys() == b.physics_history().keys()
+    with pytest.raises(NotImplementedError):
+        a.set_sources(fans=[object()])

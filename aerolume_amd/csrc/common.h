@@ -305,12 +305,13 @@ __device__ __forceinline__ T wave_max(T v) {
   v = w > v ? w : v;
   return dpp::lane63(v);
 }
-// Block sum of `n` values per thread over a 256-thread block (4 wavefronts): wave sums, then
-// the four wave partials added in order. `lds` holds 4 * n doubles. Thread 0 gets the result.
+// Block sum of `n` values per thread (blocks of 64 to 1024 threads): wave sums, then the wave
+// partials added in order. `lds` holds (waves) * n doubles. Thread 0 gets the result.
 template <int N>
 __device__ __forceinline__ void block_sum4(double (&v)[N], double *lds) {
   const int tid = threadIdx.x + threadIdx.y * blockDim.x;
   const int w = tid >> 6, lane = tid & 63;
+  const int nw = (int)(blockDim.x * blockDim.y) >> 6;
 #pragma unroll
   for (int q = 0; q < N; ++q) v[q] = wave_sum(v[q]);
   __syncthreads();
@@ -321,7 +322,11 @@ __device__ __forceinline__ void block_sum4(double (&v)[N], double *lds) {
   __syncthreads();
   if (tid == 0) {
 #pragma unroll
-    for (int q = 0; q < N; ++q) v[q] = ((lds[q] + lds[N + q]) + lds[2 * N + q]) + lds[3 * N + q];
+    for (int q = 0; q < N; ++q) {
+      double r = lds[q];
+      for (int w2 = 1; w2 < nw; ++w2) r += lds[w2 * N + q];   // waves in order
+      v[q] = r;
+    }
   }
 }
 

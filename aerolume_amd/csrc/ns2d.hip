@@ -40,8 +40,8 @@ void set_error(const std::string &msg) { g_error = msg; }
 
 namespace ns2d {
 
-constexpr int kFftThreads = 256;
-constexpr int kMaxPer = 16;                // row elements per FFT thread (n <= 4096)
+constexpr int kFftThreads = 256;          // threads of the batched transform passes
+constexpr int kMaxLen = 4096;              // longest transform
 constexpr int TX = 64;                      // tile width: one wave across (blocks of 64 x 4 threads)
 constexpr int HW = TX + 4, NW = TX + 2;     // halo-2 rows of u, v, scalar; eddy-viscosity rows (halo 1)
 constexpr int kDefaultRows = 4;             // tile height unless a plan or an ensemble picks another
@@ -958,8 +958,9 @@ __device__ __forceinline__ void load_twiddles(cplx<T> *dst, const cplx<T> *__res
 }
 
 // Pass A: divergence of rows (2b, 2b+1) -> forward transform along x -> F rows.
-template <typename T>
+template <typename T, int NT>
 __device__ __forceinline__ void poisA_body(const Dev<T> &d, int bx, const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ f_in, int mode) {
+  constexpr int MP = kMaxLen / NT;   // row elements per thread
   // mode 0: f_in is the right-hand side itself; 1: divergence of (u, v); 2: + outlet correction
   extern __shared__ unsigned char smem[];
   const int nx = d.nx;
@@ -976,12 +977,12 @@ __device__ __forceinline__ void poisA_body(const Dev<T> &d, int bx, const T *__r
     if (bx == 0 && threadIdx.x == 0) d.red->corr = corr;
   }
   // all loads of this thread first (row elements and DCT twiddles), then the FFT
-  cplx<T> csr[kMaxPer];
+  cplx<T> csr[MP];
 #pragma unroll
-  for (int r = 0; r < kMaxPer; ++r) {
+  for (int r = 0; r < MP; ++r) {
     // natural column q (coalesced loads) into transform position m; the DCT twiddles are
     // those of output frequency k = q, which this thread finishes below
-    const int q = threadIdx.x + kFftThreads * r;
+    const int q = threadIdx.x + NT * r;
     if (q < nx) {
       const int m = d.dct_x ? makhoul_pos(q, nx) : q;
       T xa, xb = 0;
@@ -999,8 +1000,8 @@ __device__ __forceinline__ void poisA_body(const Dev<T> &d, int bx, const T *__r
   __syncthreads();
   cplx<T> *R = fft_lds(A, B, d.fx, TW, 1, nx);
 #pragma unroll
-  for (int rr = 0; rr < kMaxPer; ++rr) {
-    const int k = threadIdx.x + kFftThreads * rr;
+  for (int rr = 0; rr < MP; ++rr) {
+    const int k = threadIdx.x + NT * rr;
     if (k >= nx) continue;
     cplx<T> va, vb;
     split_pair(R, nx, k, va, vb);
@@ -1017,10 +1018,10 @@ __device__ __forceinline__ void poisA_body(const Dev<T> &d, int bx, const T *__r
     if (hasb) d.F[jb * nx + k] = xb;
   }
 }
-template <typename T>
-__global__ void __launch_bounds__(kFftThreads) k_poisA(Dev<T> d, const T *__restrict__ u, const T *__restrict__ v,
-                                                       const T *__restrict__ f_in, int mode) {
-  poisA_body<T>(d, blockIdx.x, u, v, f_in, mode);
+template <typename T, int NT>
+__global__ void __launch_bounds__(NT) k_poisA(Dev<T> d, const T *__restrict__ u, const T *__restrict__ v,
+                                              const T *__restrict__ f_in, int mode) {
+  poisA_body<T, NT>(d, blockIdx.x, u, v, f_in, mode);
 }
 
 // Inverse real transform packing (see common.h): conj(Z) for the DCT-III, plain pair for DHT.
@@ -1039,8 +1040,9 @@ __device__ __forceinline__ void pack_inverse(const T *xa, const T *xb, cplx<T> *
 }
 
 // Pass B: column pairs, nslot per block: forward along y, x 1/lambda (FP64), inverse along y.
-template <typename T>
+template <typename T, int NT>
 __device__ __forceinline__ void poisB_body(const Dev<T> &d, int bx, int nslot) {
+  constexpr int MP = kMaxLen / NT;   // column elements per thread
   extern __shared__ unsigned char smem[];
   const int ny = d.ny, nx = d.nx;
   cplx<T> *A = reinterpret_cast<cplx<T> *>(smem);
@@ -1049,11 +1051,11 @@ __device__ __forceinline__ void poisB_body(const Dev<T> &d, int bx, int nslot) {
   T *X = reinterpret_cast<T *>(TW + ny);   // 2 * nslot * ny reals
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int c0 = 2 * nslot * bx;
-  cplx<T> csr[kMaxPer];
-  T iar[kMaxPer], ibr[kMaxPer];
+  cplx<T> csr[MP];
+  T iar[MP], ibr[MP];
   // twiddles, column elements, DCT twiddles and spectral scalings in one pass of loads
 #pragma unroll
-  for (int r = 0; r < kMaxPer; ++r) {
+  for (int r = 0; r < MP; ++r) {
     const int w = tid + nthr * r;
     if (w < ny) TW[w] = d.tw_y[w];
     if (w < nslot * ny) {
@@ -1071,7 +1073,7 @@ __device__ __forceinline__ void poisB_body(const Dev<T> &d, int bx, int nslot) {
   __syncthreads();
   cplx<T> *R = fft_lds(A, B, d.fy, TW, nslot, ny);
 #pragma unroll
-  for (int rr = 0; rr < kMaxPer; ++rr) {
+  for (int rr = 0; rr < MP; ++rr) {
     const int w = tid + nthr * rr;
     if (w >= nslot * ny) continue;
     const int f = fdiv(w, ny, log2_or_neg(ny)), k = w - f * ny;
@@ -1093,7 +1095,7 @@ __device__ __forceinline__ void poisB_body(const Dev<T> &d, int bx, int nslot) {
   __syncthreads();
   cplx<T> *Z = (R == A) ? B : A;
 #pragma unroll
-  for (int rr = 0; rr < kMaxPer; ++rr) {
+  for (int rr = 0; rr < MP; ++rr) {
     const int w = tid + nthr * rr;
     if (w >= nslot * ny) continue;
     const int f = fdiv(w, ny, log2_or_neg(ny)), k = w - f * ny;
@@ -1120,19 +1122,20 @@ __device__ __forceinline__ void poisB_body(const Dev<T> &d, int bx, int nslot) {
     if (cb < nx) d.F[q * nx + cb] = vb;
   }
 }
-template <typename T>
-__global__ void __launch_bounds__(kFftThreads) k_poisB(Dev<T> d, int nslot) {
+template <typename T, int NT>
+__global__ void __launch_bounds__(NT) k_poisB(Dev<T> d, int nslot) {
   // column groups dealt to the 8 XCDs in contiguous runs: the groups that share cache lines
   // of a row (nslot pairs of columns each) read them through one L2
   const int nb = gridDim.x, b = blockIdx.x;
   const int g = (nb % 8 == 0) ? (b & 7) * (nb >> 3) + (b >> 3) : b;
-  poisB_body<T>(d, g, nslot);
+  poisB_body<T, NT>(d, g, nslot);
 }
 
 // Pass C: inverse along x of rows (2b, 2b+1) -> psi; accumulate the step-averaged pressure
 // p (core.py:935-942): stage 0 p = w psi, stage 1 p += w psi, stage 2 p = (p + psi) / dt.
-template <typename T>
+template <typename T, int NT>
 __device__ __forceinline__ void poisC_body(const Dev<T> &d, int bx, T *__restrict__ psi, T *__restrict__ p, int stage) {
+  constexpr int MP = kMaxLen / NT;   // row elements per thread
   extern __shared__ unsigned char smem[];
   const int nx = d.nx;
   cplx<T> *A = reinterpret_cast<cplx<T> *>(smem);
@@ -1141,14 +1144,14 @@ __device__ __forceinline__ void poisC_body(const Dev<T> &d, int bx, T *__restric
   T *X = reinterpret_cast<T *>(TW + nx);
   const int ja = 2 * bx, jb = ja + 1;
   const bool hasb = jb < d.ny;
-  cplx<T> csr[kMaxPer];
+  cplx<T> csr[MP];
   // previous pressure accumulator values (stages 1, 2), at the positions this thread writes
-  T pa[kMaxPer], pb[kMaxPer];
+  T pa[MP], pb[MP];
   const bool acc_p = p != nullptr && stage > 0;
   // twiddles, row elements, DCT twiddles and the accumulator in one pass of loads
 #pragma unroll
-  for (int r = 0; r < kMaxPer; ++r) {
-    const int k = threadIdx.x + kFftThreads * r;
+  for (int r = 0; r < MP; ++r) {
+    const int k = threadIdx.x + NT * r;
     if (k < nx) {
       TW[k] = d.tw_x[k];
       X[k] = d.F[ja * nx + k];
@@ -1162,8 +1165,8 @@ __device__ __forceinline__ void poisC_body(const Dev<T> &d, int bx, T *__restric
   }
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < kMaxPer; ++r) {
-    const int k = threadIdx.x + kFftThreads * r;
+  for (int r = 0; r < MP; ++r) {
+    const int k = threadIdx.x + NT * r;
     if (k < nx) pack_inverse(X, X + nx, A, nx, d.dct_x, csr[r], k);
   }
   __syncthreads();
@@ -1173,9 +1176,9 @@ __device__ __forceinline__ void poisC_body(const Dev<T> &d, int bx, T *__restric
   if (p != nullptr && stage == 2) rdt = (T)1 / d.red->dt;
   const T w = (T)kForceW[stage];
 #pragma unroll
-  for (int rr = 0; rr < kMaxPer; ++rr) {
+  for (int rr = 0; rr < MP; ++rr) {
     // natural column q (coalesced stores) from transform position m
-    const int q = threadIdx.x + kFftThreads * rr;
+    const int q = threadIdx.x + NT * rr;
     if (q >= nx) continue;
     const int m = d.dct_x ? makhoul_pos(q, nx) : q;
     T va, vb;
@@ -1204,9 +1207,9 @@ __device__ __forceinline__ void poisC_body(const Dev<T> &d, int bx, T *__restric
     }
   }
 }
-template <typename T>
-__global__ void __launch_bounds__(kFftThreads) k_poisC(Dev<T> d, T *__restrict__ psi, T *__restrict__ p, int stage) {
-  poisC_body<T>(d, blockIdx.x, psi, p, stage);
+template <typename T, int NT>
+__global__ void __launch_bounds__(NT) k_poisC(Dev<T> d, T *__restrict__ psi, T *__restrict__ p, int stage) {
+  poisC_body<T, NT>(d, blockIdx.x, psi, p, stage);
 }
 
 // u -= grad psi (standalone projection, core.py:626-660).
@@ -1746,7 +1749,7 @@ __global__ void __launch_bounds__(kFftThreads) kb_poisA(const Dev<T> *__restrict
                                                         const int4 *__restrict__ items, int stage) {
   const int4 it = items[blockIdx.x];
   const BRun<T> &r = runs[it.x];
-  poisA_body<T>(devs[it.x], it.y, r.st[stage].uh, r.st[stage].vh, (const T *)nullptr, r.mode);
+  poisA_body<T, kFftThreads>(devs[it.x], it.y, r.st[stage].uh, r.st[stage].vh, (const T *)nullptr, r.mode);
 }
 
 template <typename T>
@@ -1754,7 +1757,7 @@ __global__ void __launch_bounds__(kFftThreads) kb_poisB(const Dev<T> *__restrict
                                                         const BRun<T> *__restrict__ runs,
                                                         const int4 *__restrict__ items) {
   const int4 it = items[blockIdx.x];
-  poisB_body<T>(devs[it.x], it.y, runs[it.x].nslotB);
+  poisB_body<T, kFftThreads>(devs[it.x], it.y, runs[it.x].nslotB);
 }
 
 template <typename T>
@@ -1763,7 +1766,7 @@ __global__ void __launch_bounds__(kFftThreads) kb_poisC(const Dev<T> *__restrict
                                                         const int4 *__restrict__ items, int stage, int write_p) {
   const int4 it = items[blockIdx.x];
   const Dev<T> &d = devs[it.x];
-  poisC_body<T>(d, it.y, d.psi, write_p ? runs[it.x].p_out : (T *)nullptr, stage);
+  poisC_body<T, kFftThreads>(d, it.y, d.psi, write_p ? runs[it.x].p_out : (T *)nullptr, stage);
 }
 
 template <typename T, int R>
@@ -1805,6 +1808,7 @@ struct aero2d_plan {
   int nbx, nby;
   int ty;       // rows per tile (4, 8 or 16)
   int nslotB;
+  int ntA, ntB; // threads of the row passes (A, C) and of the column pass (B): 256, 512 or 1024
 };
 
 namespace {
@@ -1818,6 +1822,20 @@ void by_rows(int ty, F &&f) {
     default: f(std::integral_constant<int, 8>{}); break;
   }
 }
+
+// calls f(std::integral_constant<int, NT>) for the transform block size NT = nt (256, 512, 1024)
+template <typename F>
+void by_threads(int nt, F &&f) {
+  switch (nt) {
+    case 1024: f(std::integral_constant<int, 1024>{}); break;
+    case 512: f(std::integral_constant<int, 512>{}); break;
+    default: f(std::integral_constant<int, 256>{}); break;
+  }
+}
+
+// One transform element per thread where the length allows it: the row and column passes run
+// one block per row (column) pair, so a block of about n threads keeps every wave busy.
+int transform_threads(size_t n) { return n <= 256 ? 256 : (n <= 512 ? 512 : 1024); }
 
 int alloc(aero2d_plan *p, void **ptr, size_t n) {
   if (n == 0) n = 16;
@@ -1895,7 +1913,9 @@ int lds_limits(const aero2d_plan *p) {
               std::to_string(std::max(a, b)));
     return AERO_EUNSUPPORTED;
   }
-  const void *fns[] = {(const void *)k_poisA<T>, (const void *)k_poisB<T>, (const void *)k_poisC<T>,
+  const void *fns[] = {(const void *)k_poisA<T, 256>, (const void *)k_poisB<T, 256>, (const void *)k_poisC<T, 256>,
+                       (const void *)k_poisA<T, 512>, (const void *)k_poisB<T, 512>, (const void *)k_poisC<T, 512>,
+                       (const void *)k_poisA<T, 1024>, (const void *)k_poisB<T, 1024>, (const void *)k_poisC<T, 1024>,
                        (const void *)kb_poisA<T>, (const void *)kb_poisB<T>, (const void *)kb_poisC<T>};
   for (const void *f : fns)
     AERO_HIP_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
@@ -1999,11 +2019,20 @@ template <typename T>
 void solve_passes(aero2d_plan *p, hipStream_t st, Dev<T> &d, const T *u, const T *v, const T *f, int mode, T *psi,
                   T *pacc, int stage) {
   const int ny = p->desc.ny, nx = p->desc.nx;
-  hipLaunchKernelGGL(k_poisA<T>, dim3((ny + 1) / 2), dim3(kFftThreads), lds_A<T>(p), st, d, u, v, f, mode);
+  by_threads(p->ntA, [&](auto nt) {
+    constexpr int NT = decltype(nt)::value;
+    hipLaunchKernelGGL((k_poisA<T, NT>), dim3((ny + 1) / 2), dim3(NT), lds_A<T>(p), st, d, u, v, f, mode);
+  });
   const int cols_per_block = 2 * p->nslotB;
-  hipLaunchKernelGGL(k_poisB<T>, dim3((nx + cols_per_block - 1) / cols_per_block), dim3(kFftThreads), lds_B<T>(p),
-                     st, d, p->nslotB);
-  hipLaunchKernelGGL(k_poisC<T>, dim3((ny + 1) / 2), dim3(kFftThreads), lds_A<T>(p), st, d, psi, pacc, stage);
+  by_threads(p->ntB, [&](auto nt) {
+    constexpr int NT = decltype(nt)::value;
+    hipLaunchKernelGGL((k_poisB<T, NT>), dim3((nx + cols_per_block - 1) / cols_per_block), dim3(NT), lds_B<T>(p),
+                       st, d, p->nslotB);
+  });
+  by_threads(p->ntA, [&](auto nt) {
+    constexpr int NT = decltype(nt)::value;
+    hipLaunchKernelGGL((k_poisC<T, NT>), dim3((ny + 1) / 2), dim3(NT), lds_A<T>(p), st, d, psi, pacc, stage);
+  });
 }
 
 // one step; chained: the dt inputs come from the previous step of this plan
@@ -2158,7 +2187,7 @@ int check_desc(const aero2d_desc *d) {
   const size_t es = d->dtype ? 8 : 4;
   const size_t ldsx = 3 * d->nx * 2 * es + 2 * d->nx * es + 512;
   const size_t ldsy = d->ny * (2 * 2 * es + 2 * es) + d->ny * 2 * es;
-  if (d->nx > kMaxPer * kFftThreads || d->ny > kMaxPer * kFftThreads || ldsx > kLdsBlockMax || ldsy > kLdsBlockMax) {
+  if (d->nx > kMaxLen || d->ny > kMaxLen || ldsx > kLdsBlockMax || ldsy > kLdsBlockMax) {
     set_error("grid too large for the LDS-resident transforms");
     return AERO_EUNSUPPORTED;
   }
@@ -2260,8 +2289,10 @@ int aero2d_plan_create(const aero2d_desc *desc, aero2d_plan **out) {
   // launch may run this plan with any tile shape
   const size_t blocks = (size_t)p->nbx * ((ny + 3) / 4);
   p->nslotB = 1;
+  p->ntA = transform_threads(nx);
+  p->ntB = transform_threads(ny);
   while (p->nslotB > 1 &&
-         (ny * p->nslotB * (2 * 2 * es + 2 * es) + ny * 2 * es > 65536 || ny * p->nslotB > (size_t)kMaxPer * kFftThreads))
+         (ny * p->nslotB * (2 * 2 * es + 2 * es) + ny * 2 * es > 65536 || ny * p->nslotB > (size_t)kMaxLen))
     p->nslotB--;
   struct { void **ptr; size_t n; } tab[] = {
       {&p->uA, nu_ * es}, {&p->vA, nv_ * es}, {&p->uB, nu_ * es}, {&p->vB, nv_ * es}, {&p->uC, nu_ * es},
@@ -2382,7 +2413,7 @@ template <typename T>
 int run_slots(const aero2d_plan *p, int want) {
   const size_t ny = p->desc.ny;
   int s = want;
-  while (s > 1 && ((size_t)s * ny > (size_t)kMaxPer * kFftThreads ||
+  while (s > 1 && ((size_t)s * ny > (size_t)kMaxLen ||
                    (size_t)s * ny * (2 * sizeof(cplx<T>) + 2 * sizeof(T)) + ny * sizeof(cplx<T>) > 65536))
     s /= 2;
   return s;

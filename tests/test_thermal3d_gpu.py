@@ -373,3 +373,25 @@ def test_solver_set_bodies_equals_fresh_solver():
     assert torch.equal(a.theta, b.theta)
     assert np.array_equal(a.coefficients(), b.coefficients())
     assert len(a.history()["t"]) == 20
+
+
+def test_solver_set_sources_equals_fresh_solver():
+    """Heaters replaced live equal a solver configured with them from the same state (FP64)."""
+    print("[thermal gpu] set_sources", flush=True)
+    import dataclasses
+    fx = _fixture("box")
+    cfg = cases.thermal_cfg("box", fx, "float64", device="cuda", use_cuda_graphs=True)
+    heaters = [HeatSource3D(**dict(h, power=2.0 * h.get("power", 1.0))) for h in cases.THERMAL3D["box"]["heaters"]]
+    a = S3.FlowSolver3D(cfg)
+    a.set_velocity(fx["u0"], fx["v0"], fx["w0"])
+    a.set_temperature(fx["theta0"])
+    a.advance(4)
+    sd = a.state_dict()
+    a.set_sources(heat_sources=heaters)
+    a.advance(6)
+    b = S3.FlowSolver3D(dataclasses.replace(cfg, heat_sources=heaters))
+    b.load_state_dict(sd)
+    b.advance(6)
+    assert torch.equal(a.theta, b.theta) and torch.equal(a.state.w, b.state.w)
+    with pytest.raises(NotImplementedError):
+        a.set_sources(fans=[object()])

@@ -298,3 +298,36 @@ def test_live_physics_edits_equal_fresh_solver():
     assert a.physics_history().keys() == b.physics_history().keys()
     with pytest.raises(NotImplementedError):
         a.set_sources(fans=[object()])
+
+
+@pytest.mark.parametrize("kind", ["periodic", "closed"])
+def test_functional_diag_row_periodic_and_closed(kind):
+    """The row's Poisson residual and divergence use the solve's own boundary rules: periodic
+    wrap and closed walls (no outlet correction), against the oracle's row (float64)."""
+    print(f"[pid gpu] diag row {kind}", flush=True)
+    over = dict(thermal=None, bodies=[], adaptive_dt="cfl", turbulence="none", perturb=True)
+    over.update(periodic=True) if kind == "periodic" else over.update(domain="closed")
+    rows, states = [], []
+    u0 = v0 = None
+    for dev in ("cpu", "cuda"):
+        cfg = cases.pid_diag_cfg("float64", device=dev, **over)
+        ops = su.make_operators(cfg, device=dev)
+        geom = su.build_geometry(cfg.bodies, ops, window=su.live_window(cfg.bodies, ops))
+        if u0 is None:
+            u0, v0 = O.noisy_start(ops, 31, amplitude=0.5)
+        st = S.init_state(cfg, ops)._replace(u=u0.to(ops.device), v=v0.to(ops.device))
+        step = S.step_full if dev == "cuda" else O.step_full
+        with torch.no_grad():
+            st, _ = step(st, geom, ops)
+            out = {}
+            st, _ = step(st, geom, ops, out=out, diag=True)
+        rows.append(out["diag"].double().cpu().numpy())
+        states.append(st)
+    c, g = rows
+    assert float(states[0].u.abs().max()) > 0.1
+    for i in range(18):
+        if i in (3, 4, 5):
+            assert g[i] < 1e-12 and c[i] < 1e-12, (kind, i, g[i], c[i])
+        else:
+            assert g[i] == pytest.approx(c[i], rel=1e-9, abs=1e-14), (kind, i, g[i], c[i])
+    assert np.all(np.isnan(g[18:]))

@@ -1402,12 +1402,16 @@ __device__ __forceinline__ void finalize_body(const Dev<T> &d, const T *__restri
       for (int q = 0; q < 4; ++q) wv[q] = d.wall_values[q];
     }
   }
+  // the three load phases run in different waves at once (each costs a memory round trip):
+  // wall sums in the first quarter of the block, window partials in the second, the per-block
+  // dt inputs in the second half
+  const int r1 = nthr >> 2, r2 = nthr >> 1;
   double ws[6] = {0, 0, 0, 0, 0, 0};
-  if (d.thermal) {
+  if (d.thermal && tid < r1) {
     const T tin_all = *d.theta_in;
     T w0 = d.wall_values[0], w1 = d.wall_values[1], w2 = d.wall_values[2], w3 = d.wall_values[3];
 #pragma unroll 2
-    for (int jj = tid; jj < ny; jj += nthr) {
+    for (int jj = tid; jj < ny; jj += r1) {
       const T a0 = th_new[jj * nx], a1 = th_new[jj * nx + nx - 1];
       const T b0 = u_new[jj * nxp], b1 = u_new[jj * nxp + nx];
       ws[0] += (double)(w0 - a0);
@@ -1416,7 +1420,7 @@ __device__ __forceinline__ void finalize_body(const Dev<T> &d, const T *__restri
       ws[5] += (double)(b1 * a1);
     }
 #pragma unroll 2
-    for (int ii = tid; ii < nx; ii += nthr) {
+    for (int ii = tid; ii < nx; ii += r1) {
       const T c0 = th_new[ii], c1 = th_new[(ny - 1) * nx + ii];
       ws[2] += (double)(w2 - c0);
       ws[3] += (double)(w3 - c1);
@@ -1426,13 +1430,13 @@ __device__ __forceinline__ void finalize_body(const Dev<T> &d, const T *__restri
   // loads issued before the first barrier
   double acc[kMaxWin];
   for (int q = 0; q < kMaxWin; ++q) acc[q] = 0;
-  if (d.win) {
+  if (d.win && tid >= r1 && tid < r2) {
     int bx0 = d.i0 / TX, bx1 = (d.i1 < nx ? d.i1 : nx - 1) / TX;
     int by0 = d.j0 / d.ty, by1 = (d.j1 < ny ? d.j1 : ny - 1) / d.ty;
     if (d.periodic) { bx0 = 0; by0 = 0; }
     const int nbx = bx1 - bx0 + 1, nb = nbx * (by1 - by0 + 1);
 #pragma unroll 2
-    for (int e = tid; e < nb; e += nthr) {
+    for (int e = tid - r1; e < nb; e += r2 - r1) {
       const int bx = bx0 + e % nbx, by = by0 + e / nbx;
       const double *p = d.pforce + kMaxWin * (by * d.nbx + bx);
       for (int q = 0; q < kMaxWin; ++q) acc[q] += p[q];
@@ -1441,7 +1445,7 @@ __device__ __forceinline__ void finalize_body(const Dev<T> &d, const T *__restri
   double pm[4] = {0, 0, 0, 0}, psum = 0;
   const int nblocks = d.nbx * d.nby;
 #pragma unroll 4
-  for (int b = tid; b < nblocks; b += nthr) {
+  for (int b = tid - r2; tid >= r2 && b < nblocks; b += nthr - r2) {
     const double *p = d.pmax + 5 * b;
     for (int q = 0; q < 4; ++q) pm[q] = fmax(pm[q], p[q]);
     psum += p[4];

@@ -361,7 +361,7 @@ def run_only(args) -> int:
         _, wall, stage_ms = gpu_run_3d(args.steps3d, max(args.warmup3d, 20))
         out.update(ms_per_step=wall / args.steps3d * 1e3, stage_us=stage_ms * 1e3)
     elif args.only == "ensemble":
-        out.update(gpu_run_ensemble(args.ensemble_steps, 10))
+        out.update(gpu_run_ensemble(args.ensemble_steps, 20))
     else:
         n = 512 * 2**20 // 4
         x = torch.rand(n, device="cuda")
@@ -382,7 +382,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--steps3d", type=int, default=500)
-    ap.add_argument("--warmup3d", type=int, default=10)
+    ap.add_argument("--warmup3d", type=int, default=20)
     ap.add_argument("--no-3d", action="store_true", help="skip the 3D car workload")
     ap.add_argument("--ensemble-steps", type=int, default=1000)
     ap.add_argument("--only", choices=("2d", "3d", "ensemble", "copy"), default=None,
@@ -449,7 +449,7 @@ def main() -> int:
             w3["gpu_over_cpu"] = round(v3 / w3["cpu_baseline"]["value"], 1)
         line.setdefault("workloads", {})["3d_car"] = w3
     if not args.no_ensemble:
-        line.setdefault("workloads", {})["ensemble"] = gpu_run_ensemble(args.ensemble_steps, 10)
+        line.setdefault("workloads", {})["ensemble"] = gpu_run_ensemble(args.ensemble_steps, 20)
     print(json.dumps(line), flush=True)
     return 0
 

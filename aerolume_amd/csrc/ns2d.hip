@@ -1747,13 +1747,12 @@ __global__ void __launch_bounds__(256) kb_stage(const Dev<T> *__restrict__ devs,
   stage_body<T, R>(devs[it.x], runs[it.x].st[stage], it.y, it.z);
 }
 
-template <typename T>
-__global__ void __launch_bounds__(kFftThreads) kb_poisA(const Dev<T> *__restrict__ devs,
-                                                        const BRun<T> *__restrict__ runs,
-                                                        const int4 *__restrict__ items, int stage) {
+template <typename T, int NT>
+__global__ void __launch_bounds__(NT) kb_poisA(const Dev<T> *__restrict__ devs, const BRun<T> *__restrict__ runs,
+                                               const int4 *__restrict__ items, int stage) {
   const int4 it = items[blockIdx.x];
   const BRun<T> &r = runs[it.x];
-  poisA_body<T, kFftThreads>(devs[it.x], it.y, r.st[stage].uh, r.st[stage].vh, (const T *)nullptr, r.mode);
+  poisA_body<T, NT>(devs[it.x], it.y, r.st[stage].uh, r.st[stage].vh, (const T *)nullptr, r.mode);
 }
 
 template <typename T>
@@ -1764,13 +1763,12 @@ __global__ void __launch_bounds__(kFftThreads) kb_poisB(const Dev<T> *__restrict
   poisB_body<T, kFftThreads>(devs[it.x], it.y, runs[it.x].nslotB);
 }
 
-template <typename T>
-__global__ void __launch_bounds__(kFftThreads) kb_poisC(const Dev<T> *__restrict__ devs,
-                                                        const BRun<T> *__restrict__ runs,
-                                                        const int4 *__restrict__ items, int stage, int write_p) {
+template <typename T, int NT>
+__global__ void __launch_bounds__(NT) kb_poisC(const Dev<T> *__restrict__ devs, const BRun<T> *__restrict__ runs,
+                                               const int4 *__restrict__ items, int stage, int write_p) {
   const int4 it = items[blockIdx.x];
   const Dev<T> &d = devs[it.x];
-  poisC_body<T, kFftThreads>(d, it.y, d.psi, write_p ? runs[it.x].p_out : (T *)nullptr, stage);
+  poisC_body<T, NT>(d, it.y, d.psi, write_p ? runs[it.x].p_out : (T *)nullptr, stage);
 }
 
 template <typename T, int R>
@@ -1920,7 +1918,8 @@ int lds_limits(const aero2d_plan *p) {
   const void *fns[] = {(const void *)k_poisA<T, 256>, (const void *)k_poisB<T, 256>, (const void *)k_poisC<T, 256>,
                        (const void *)k_poisA<T, 512>, (const void *)k_poisB<T, 512>, (const void *)k_poisC<T, 512>,
                        (const void *)k_poisA<T, 1024>, (const void *)k_poisB<T, 1024>, (const void *)k_poisC<T, 1024>,
-                       (const void *)kb_poisA<T>, (const void *)kb_poisB<T>, (const void *)kb_poisC<T>};
+                       (const void *)kb_poisA<T, 256>, (const void *)kb_poisB<T>, (const void *)kb_poisC<T, 256>,
+                       (const void *)kb_poisA<T, 512>, (const void *)kb_poisC<T, 512>};
   for (const void *f : fns)
     AERO_HIP_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
   return AERO_OK;
@@ -2383,6 +2382,7 @@ struct aero2d_ensemble {
   int n = 0, dbl = 0;
   int ty = kDefaultRows;   // rows per tile of the batched cell launches
   int nslot = 2;           // column pairs per block of the batched pass-B launch (capped per run)
+  int ntrow = kFftThreads; // threads per block of the batched row passes (A, C): 256 or 512
   std::vector<aero2d_plan *> plans;
   std::vector<aero2d_fields> fields;
   std::vector<aero2d_geom> geoms;
@@ -2506,9 +2506,15 @@ int ensemble_step(aero2d_ensemble *e, hipStream_t st, bool chained) {
     }
     for (int k = 0; k < 3; ++k) {
       hipLaunchKernelGGL((kb_stage<T, R>), dim3(e->ntiles), blk, 0, st, devs, runs, tiles, k);
-      hipLaunchKernelGGL(kb_poisA<T>, dim3(e->nrows), dim3(kFftThreads), e->ldsA, st, devs, runs, rows, k);
+      by_threads(e->ntrow, [&](auto nt) {
+        constexpr int NT = decltype(nt)::value;
+        hipLaunchKernelGGL((kb_poisA<T, NT>), dim3(e->nrows), dim3(NT), e->ldsA, st, devs, runs, rows, k);
+      });
       hipLaunchKernelGGL(kb_poisB<T>, dim3(e->ncols), dim3(kFftThreads), e->ldsB, st, devs, runs, cols);
-      hipLaunchKernelGGL(kb_poisC<T>, dim3(e->nrows), dim3(kFftThreads), e->ldsA, st, devs, runs, rows, k, 1);
+      by_threads(e->ntrow, [&](auto nt) {
+        constexpr int NT = decltype(nt)::value;
+        hipLaunchKernelGGL((kb_poisC<T, NT>), dim3(e->nrows), dim3(NT), e->ldsA, st, devs, runs, rows, k, 1);
+      });
     }
     hipLaunchKernelGGL((kb_final<T, R>), dim3(e->ntiles), blk, 0, st, devs, runs, tiles);
   });
@@ -2535,9 +2541,15 @@ int ensemble_time(aero2d_ensemble *e, hipStream_t st, int iters, float *ms) {
       if (it == 1) (void)hipEventRecord(e0, st);   // the first round warms up
       for (int k = 0; k < 2; ++k) {
         hipLaunchKernelGGL((kb_stage<T, R>), dim3(e->ntiles), blk, 0, st, devs, runs, tiles, k);
-        hipLaunchKernelGGL(kb_poisA<T>, dim3(e->nrows), dim3(kFftThreads), e->ldsA, st, devs, runs, rows, k);
+        by_threads(e->ntrow, [&](auto nt) {
+          constexpr int NT = decltype(nt)::value;
+          hipLaunchKernelGGL((kb_poisA<T, NT>), dim3(e->nrows), dim3(NT), e->ldsA, st, devs, runs, rows, k);
+        });
         hipLaunchKernelGGL(kb_poisB<T>, dim3(e->ncols), dim3(kFftThreads), e->ldsB, st, devs, runs, cols);
-        hipLaunchKernelGGL(kb_poisC<T>, dim3(e->nrows), dim3(kFftThreads), e->ldsA, st, devs, runs, rows, k, 0);
+        by_threads(e->ntrow, [&](auto nt) {
+          constexpr int NT = decltype(nt)::value;
+          hipLaunchKernelGGL((kb_poisC<T, NT>), dim3(e->nrows), dim3(NT), e->ldsA, st, devs, runs, rows, k, 0);
+        });
       }
     }
   });
@@ -2560,30 +2572,37 @@ template <typename T>
 int ensemble_tune(aero2d_ensemble *e, hipStream_t st, int iters, int32_t *choice) {
   static const int kRows[] = {4, 8, 16};
   static const int kSlots[] = {1, 2, 4};
+  static const int kRowThreads[] = {256, 512};
   float best = 0.f;
-  int best_ty = e->ty, best_slot = e->nslot, tried = 0;
+  int best_ty = e->ty, best_slot = e->nslot, best_nt = e->ntrow, tried = 0;
   for (int ty : kRows)
     for (int sl : kSlots) {
       e->ty = ty;
       e->nslot = sl;
       int rc = ensemble_build<T>(e);
-      float ms = 0.f;
-      if (!rc) rc = ensemble_time<T>(e, st, iters, &ms);
-      if (rc) return rc;
-      ++tried;
-      if (tried == 1 || ms < best) {
-        best = ms;
-        best_ty = ty;
-        best_slot = sl;
+      for (int nt : kRowThreads) {
+        e->ntrow = nt;   // a launch shape only: the work tables do not depend on it
+        float ms = 0.f;
+        if (!rc) rc = ensemble_time<T>(e, st, iters, &ms);
+        if (rc) return rc;
+        ++tried;
+        if (tried == 1 || ms < best) {
+          best = ms;
+          best_ty = ty;
+          best_slot = sl;
+          best_nt = nt;
+        }
       }
     }
   e->ty = best_ty;
   e->nslot = best_slot;
+  e->ntrow = best_nt;
   const int rc = ensemble_build<T>(e);
   if (choice) {
     choice[0] = best_ty;
     choice[1] = best_slot;
     choice[2] = tried;
+    choice[3] = best_nt;
   }
   return rc;
 }

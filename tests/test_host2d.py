@@ -49,3 +49,37 @@ def test_resample_closed_square_and_circle():
     seg = np.hypot(*np.diff(np.vstack([pts, pts[:1]]), axis=0).T)
     assert np.allclose(seg, seg.mean(), rtol=1e-3) and np.allclose(np.hypot(*pts.T), 1.0, atol=1e-4)
     assert arc[0] == 0.0 and np.all(np.diff(arc) > 0)
+
+
+def test_particle_polylines_group_streaklines_and_timelines():
+    """``TracerParticles.lines``: one streakline per continuous nozzle (youngest first) and one
+    timeline per pulsed release (in nozzle order), from a hand-filled pool on the CPU."""
+    from aerolume_amd.ns2d import setup as su
+    from aerolume_amd.ns2d.config import EmitterSpec, SolverConfig
+    from aerolume_amd.ns2d.particles import TracerParticles
+    cfg = SolverConfig(nx=16, ny=8, h=0.25, x0=0.0, y0=0.0, re=100.0, dtype="float64", device="cpu", dye=False,
+                       perturb=False)
+    ops = su.make_operators(cfg, device="cpu")
+    ems = [EmitterSpec(position=(0.5, 0.5), kind="rake", end=(0.5, 1.5), count=2, species="particles", rate=10.0),
+           EmitterSpec(position=(1.0, 0.5), kind="line", end=(1.0, 1.5), count=3, species="particles",
+                       pulse_period=0.5)]
+    tp = TracerParticles(ems, ops, capacity=40)          # 5 nozzles x 8 slots
+    assert tp.ring == 8 and tp.capacity == 40 and tp.pulsed.tolist() == [0.0, 1.0]
+    t = 1.0
+    # nozzle 0: three particles of ages 0.3, 0.1, 0.2; nozzle 1: one (no line)
+    for slot, (x, age) in zip((0, 1, 2), ((0.8, 0.3), (0.6, 0.1), (0.7, 0.2))):
+        tp.pos[slot] = torch.tensor([x, 0.5], dtype=torch.float64)
+        tp.birth[slot], tp.alive[slot] = t - age, 1.0
+    tp.pos[8], tp.birth[8], tp.alive[8] = torch.tensor([0.9, 1.5], dtype=torch.float64), t - 0.2, 1.0
+    # pulsed nozzles 2..4: two releases (births 0.5 and 0.9), written in shuffled nozzle order
+    for n, b, x in ((4, 0.5, 1.4), (2, 0.5, 1.2), (3, 0.5, 1.3), (3, 0.9, 1.05), (2, 0.9, 1.02)):
+        slot = n * 8 + int(b * 10) % 8
+        tp.pos[slot] = torch.tensor([x, 0.5 + 0.5 * (n - 2)], dtype=torch.float64)
+        tp.birth[slot], tp.alive[slot] = b, 1.0
+    lines = tp.lines(t)
+    assert len(lines) == 3
+    assert np.allclose(lines[0][:, 0], [0.6, 0.7, 0.8])                  # youngest first
+    tl = sorted(lines[1:], key=len)
+    assert np.allclose(tl[0][:, 1], [0.5, 1.0]) and np.allclose(tl[1][:, 1], [0.5, 1.0, 1.5])
+    snap = tp.snapshot(t)
+    assert len(snap["x"]) == 9 and set(snap["emitter"].tolist()) == {0, 1}

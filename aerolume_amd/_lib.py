@@ -63,6 +63,10 @@ class Outputs2D(ctypes.Structure):
 #: columns of a 2D diagnostics row (AERO2D_DIAG_COLS; float64)
 DIAG_COLS = 22
 
+class Trace2D(ctypes.Structure):
+    _fields_ = [(k, vp * 3) for k in ("pen_u", "pen_v", "div", "psi", "proj_u", "proj_v", "theta")]
+
+
 class ParticlesGrid2D(ctypes.Structure):
     _fields_ = [("dtype", i32), ("nx", i32), ("ny", i32), ("reserved", i32), ("x0", f64), ("y0", f64), ("h", f64)]
 
@@ -106,6 +110,8 @@ def load() -> ctypes.CDLL:
     for name in ("aero2d_plan_create", "aero2d_plan_destroy", "aero2d_step", "aero2d_project",
                  "aero2d_poisson_solve"):
         getattr(lib, name).restype = ctypes.c_int
+    lib.aero2d_step_trace.argtypes = lib.aero2d_step.argtypes + [ctypes.POINTER(Trace2D)]
+    lib.aero2d_step_trace.restype = ctypes.c_int
     lib.aero_polygon_sdf.argtypes = [P, ctypes.c_int32, P, ctypes.c_int32, P, P, ctypes.c_int64, P]
     lib.aero_polygon_sdf.restype = ctypes.c_int
     lib.aero2d_particles_step.argtypes = [P, ctypes.POINTER(ParticlesGrid2D), P, P, P, P, P,

@@ -1370,6 +1370,30 @@ __global__ void __launch_bounds__(256) k_final(Dev<T> d, const T *uh, const T *v
   final_body<T, R>(d, blockIdx.x, blockIdx.y, uh, vh, u_out, v_out, th_new, cin, cout, t_in);
 }
 
+// Stage trace (core.py:897-934): from a stage's u^ and potential, the velocity as the
+// reference's trace holds it after penalisation (outlet face corrected), the projection's
+// right-hand side and the projected velocity (u^ - grad psi, boundary-normal faces untouched).
+template <typename T>
+__global__ void __launch_bounds__(256) k_trace_stage(Dev<T> d, const T *__restrict__ uh, const T *__restrict__ vh,
+                                                    const T *__restrict__ psi, T *pen_u, T *pen_v, T *div, T *pu,
+                                                    T *pv) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  const int nx = d.nx, ny = d.ny;
+  const bool tunnel = !(d.periodic || d.closed);
+  const T corr = tunnel ? d.red->corr : (T)0;
+  if (j < ny && i <= nx) {
+    const int o = j * (nx + 1) + i;
+    if (pen_u) pen_u[o] = (tunnel && i == nx) ? uh[o] + corr : uh[o];
+    if (pu) pu[o] = u_in(d, uh, psi, corr, j, i);
+  }
+  if (j <= ny && i < nx) {
+    const int o = j * nx + i;
+    if (pen_v) pen_v[o] = vh[o];
+    if (pv) pv[o] = v_in(d, vh, psi, j, i);
+  }
+  if (div && j < ny && i < nx) div[j * nx + i] = div_at(d, uh, vh, j, i, corr);
+}
+
 template <typename T>
 __global__ void k_copy(T *__restrict__ dst, const T *__restrict__ src, size_t n) {
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
@@ -2041,7 +2065,7 @@ void solve_passes(aero2d_plan *p, hipStream_t st, Dev<T> &d, const T *u, const T
 // one step; chained: the dt inputs come from the previous step of this plan
 template <typename T>
 int step_impl(aero2d_plan *p, hipStream_t st, const aero2d_fields *in, aero2d_fields *out, const aero2d_geom *g,
-              const aero2d_params *pr, aero2d_outputs *o, bool chained) {
+              const aero2d_params *pr, aero2d_outputs *o, bool chained, const aero2d_trace *tr = nullptr) {
   const aero2d_desc &s = p->desc;
   Dev<T> d = make_dev<T>(p, g, pr);
   d.dt_in = (const T *)in->dt;
@@ -2082,6 +2106,17 @@ int step_impl(aero2d_plan *p, hipStream_t st, const aero2d_fields *in, aero2d_fi
     sg.uh = ubuf[k]; sg.vh = vbuf[k]; sg.thh = s.thermal ? tbuf[k] : nullptr;
     by_rows(p->ty, [&](auto r) { hipLaunchKernelGGL((k_stage<T, decltype(r)::value>), tiles, blk, 0, st, d, sg); });
     solve_passes<T>(p, st, d, sg.uh, sg.vh, nullptr, mode, d.psi, (T *)out->p, k);
+    if (tr) {
+      const dim3 gf((nx + 1 + 63) / 64, (ny + 1 + 3) / 4);
+      hipLaunchKernelGGL(k_trace_stage<T>, gf, blk, 0, st, d, (const T *)sg.uh, (const T *)sg.vh, (const T *)d.psi,
+                         (T *)tr->pen_u[k], (T *)tr->pen_v[k], (T *)tr->div[k], (T *)tr->proj_u[k],
+                         (T *)tr->proj_v[k]);
+      const size_t nc = (size_t)nx * ny;
+      const int cb = (int)std::min<size_t>(4096, (nc + 255) / 256);
+      if (tr->psi[k]) hipLaunchKernelGGL(k_copy<T>, dim3(cb), dim3(256), 0, st, (T *)tr->psi[k], (const T *)d.psi, nc);
+      if (tr->theta[k] && sg.thh)
+        hipLaunchKernelGGL(k_copy<T>, dim3(cb), dim3(256), 0, st, (T *)tr->theta[k], (const T *)sg.thh, nc);
+    }
     up = sg.uh; vp = sg.vh; thp = sg.thh; psi_p = d.psi;
   }
   const bool dye = s.dye_mode != 0 && in->dye && out->dye;
@@ -2334,6 +2369,16 @@ int aero2d_plan_destroy(aero2d_plan *p) {
 }
 
 size_t aero2d_plan_workspace_bytes(const aero2d_plan *p) { return p ? p->bytes : 0; }
+
+int aero2d_step_trace(aero2d_plan *p, void *stream, const aero2d_fields *in, aero2d_fields *out,
+                      const aero2d_geom *g, const aero2d_params *pr, aero2d_outputs *o, const aero2d_trace *tr) {
+  int rc = check_step_args(p, in, out, g, pr, o);
+  if (rc) return rc;
+  if (!tr) { set_error("aero2d_step_trace: null trace"); return AERO_EINVAL; }
+  hipStream_t st = (hipStream_t)stream;
+  return p->dbl ? step_impl<double>(p, st, in, out, g, pr, o, false, tr)
+                : step_impl<float>(p, st, in, out, g, pr, o, false, tr);
+}
 
 int aero2d_step(aero2d_plan *p, void *stream, const aero2d_fields *in, aero2d_fields *out, const aero2d_geom *g,
                 const aero2d_params *pr, aero2d_outputs *o) {

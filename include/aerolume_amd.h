@@ -160,6 +160,19 @@ int aero2d_step(aero2d_plan *plan, void *stream, const aero2d_fields *in, aero2d
  * `in` is that output, unmodified (the stateful solver's hot loop); saves two launches. */
 int aero2d_step_chained(aero2d_plan *plan, void *stream, const aero2d_fields *in, aero2d_fields *out,
                         const aero2d_geom *geom, const aero2d_params *params, aero2d_outputs *outputs);
+/* Stage trace of one step (core.py:897-934 with a trace dict): per RK stage s (0..2) caller
+ * buffers of the plan's dtype, any NULL: pen_u (ny, nx+1) / pen_v (ny+1, nx) the stage velocity
+ * after penalisation and buoyancy (outlet face corrected), div (ny, nx) its projection's
+ * right-hand side, psi (ny, nx) the potential, proj_u / proj_v the projected velocity, theta
+ * (ny, nx) the stage temperature. The velocity before penalisation is not formed by the fused
+ * stage kernel and is not traced. */
+typedef struct {
+  void *pen_u[3], *pen_v[3], *div[3], *psi[3], *proj_u[3], *proj_v[3], *theta[3];
+} aero2d_trace;
+/* aero2d_step that also fills `trace` (a debugging step: extra copies after every stage). */
+int aero2d_step_trace(aero2d_plan *plan, void *stream, const aero2d_fields *in, aero2d_fields *out,
+                      const aero2d_geom *geom, const aero2d_params *params, aero2d_outputs *outputs,
+                      const aero2d_trace *trace);
 /* u, v in place: u - grad psi with lap psi = div u (no inlet/outlet change); psi (ny, nx) optional. */
 int aero2d_project(aero2d_plan *plan, void *stream, void *u, void *v, void *psi);
 /* psi = L^-1 f with zero mean (f, psi (ny, nx); may alias). */

@@ -98,7 +98,7 @@ def test_ctypes_structures_match_the_header(tmp_path):
              "aero2d_params": _lib.Params2D, "aero2d_outputs": _lib.Outputs2D, "aero3d_desc": _lib3d.Desc3D,
              "aero3d_fields": _lib3d.Fields3D, "aero3d_geom": _lib3d.Geom3D, "aero3d_params": _lib3d.Params3D,
              "aero3d_outputs": _lib3d.Outputs3D, "aero2d_particles_grid": _lib.ParticlesGrid2D,
-             "aero2d_particles": _lib.Particles2D}
+             "aero2d_particles": _lib.Particles2D, "aero2d_trace": _lib.Trace2D}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "aerolume_amd.h"', "int main(void) {"]
     for cname, cls in pairs.items():
         lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')

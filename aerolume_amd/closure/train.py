@@ -1,4 +1,4 @@
-"""Train the closure network on filtered fine-grid solves and judge it by the spectrum of
+"""Train the closure network on filtered fine-grid solves and evaluate it by the spectrum of
 coarse solves (``python -m aerolume_amd.closure.train``; prints one JSON report).
 
 1. Fine solves (no model) of decaying box turbulence from seeded spectral fields; snapshots are

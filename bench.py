@@ -62,6 +62,62 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
+#: graph blocks every leg warms up before its clock starts: each HIP graph a timed block replays
+#: is captured (and replayed once) before the first timed step
+WARM_BLOCKS = 2
+#: timed blocks per leg; the figure reported is the best block (the reference's protocol,
+#: validation/bench_ns2d.py:71-90, validation/bench_ns3d.py:73-101)
+TIMED_BLOCKS = 3
+
+
+def warm_steps(warmup: int, graph_steps: int = 10) -> int:
+    """Untimed steps before the clock: at least ``WARM_BLOCKS`` whole graph blocks, so that no
+    timed step pays for a graph capture, whatever ``--warmup`` says."""
+    n = max(int(warmup), WARM_BLOCKS * graph_steps)
+    return -(-n // graph_steps) * graph_steps
+
+
+def timed_blocks(advance, steps: int, blocks: int = TIMED_BLOCKS) -> list:
+    """Wall seconds of ``blocks`` runs of exactly ``steps`` steps of ``advance``, each bracketed
+    by a device synchronisation on both sides."""
+    import torch
+    walls = []
+    for _ in range(blocks):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        advance(steps)
+        torch.cuda.synchronize()
+        walls.append(time.perf_counter() - t0)
+    return walls
+
+
+def host_cpu() -> dict:
+    """Model name and physical core count of this host (from /proc/cpuinfo), and the CPU share
+    this process may use (its affinity mask)."""
+    model, cores = "unknown", set()
+    phys = core = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name":
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif not k and phys is not None:
+                    cores.add((phys, core))
+                    phys = core = None
+        if phys is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    return {"model": model, "physical_cores": len(cores) or os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0))}
+
+
 def c2_config(dtype="float32", device="cuda"):
     import math
 
@@ -109,12 +165,9 @@ def gpu_run(steps: int, warmup: int):
     u, v, _ = S.project(u, v, ops)
     solver.set_velocity(u, v)
     log(f"bench: workspace {solver.workspace_bytes / 2**20:.1f} MiB, window {solver.geometry.window}")
-    solver.advance(warmup)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    solver.advance(steps)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    solver.advance(warm_steps(warmup))
+    walls = timed_blocks(solver.advance, steps)
+    log(f"bench: 2D blocks of {steps} steps: " + ", ".join(f"{w * 1e3:.3f} ms" for w in walls))
     if not all(bool(torch.isfinite(t).all()) for t in (solver.state.u, solver.state.v, solver.state.theta)):
         raise RuntimeError("non-finite state after the timed steps")
     # dominant kernel: the stage kernel, timed alone with HIP events on the solver's stream
@@ -127,7 +180,7 @@ def gpu_run(steps: int, warmup: int):
                                ctypes.byref(pl.geom_ptrs), ctypes.byref(pl.params), 1, iters, ctypes.byref(ms))
     _lib.check(rc, "aero2d_time_stage")
     torch.cuda.synchronize()
-    return solver, wall, ms.value
+    return solver, walls, ms.value
 
 
 def gpu_run_3d(steps: int, warmup: int):
@@ -146,18 +199,15 @@ def gpu_run_3d(steps: int, warmup: int):
     u, v, w, _ = S.project3d(u.contiguous(), v.contiguous(), w.contiguous(), ops)
     solver.set_velocity(u, v, w)
     log(f"bench3d: workspace {solver.workspace_bytes / 2**20:.1f} MiB, window {solver.geometry.window}")
-    solver.advance(warmup)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    solver.advance(steps)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    solver.advance(warm_steps(warmup))
+    walls = timed_blocks(solver.advance, steps)
+    log(f"bench3d: blocks of {steps} steps: " + ", ".join(f"{w * 1e3:.3f} ms" for w in walls))
     st = solver.state
     if not all(bool(torch.isfinite(t).all()) for t in (st.u, st.v, st.w, st.p)):
         raise RuntimeError("non-finite 3D state after the timed steps")
     stage_ms = solver._plan.time_stage(solver._fields, 1, 100)
     torch.cuda.synchronize()
-    return solver, wall, stage_ms
+    return solver, walls, stage_ms
 
 
 def gpu_run_ensemble(steps: int, warmup: int, n: int = 64, seed: int = 404):
@@ -283,9 +333,47 @@ def noisy_velocity3d(ops, seed):
     return u, v, w
 
 
-def cpu_baseline_3d(seconds: float = 20.0):
-    """The CPU oracle (torch, FP32, all host threads, flush-to-zero) on the 3D car case: one
-    warm-up step, then steps while they fit in about ``seconds``, best per-step time."""
+def cpu_blocks(step, threads: int, block_steps: int, blocks: int = TIMED_BLOCKS) -> float:
+    """Best wall seconds per step over ``blocks`` blocks of ``block_steps`` calls of ``step``
+    on ``threads`` torch CPU threads with flush-to-zero (the reference's CPU stepping mode,
+    solvers/fpmode.py:45-63), after one warm-up call."""
+    import torch
+    old = torch.get_num_threads()
+    torch.set_num_threads(threads)
+    torch.set_flush_denormal(True)
+    try:
+        step()
+        best = float("inf")
+        for _ in range(blocks):
+            t0 = time.perf_counter()
+            for _ in range(block_steps):
+                step()
+            best = min(best, (time.perf_counter() - t0) / block_steps)
+    finally:
+        torch.set_flush_denormal(False)
+        torch.set_num_threads(old)
+    return best
+
+
+def cpu_report(step, cells: int, block_steps: int, what: str) -> dict:
+    """``cpu_baseline`` of one case: the CPU oracle timed at all the threads torch uses here
+    (``value``) and at 4 threads (the reference's published CPU protocol,
+    docs/theory_ns2d.md:426-428), best of three blocks each."""
+    import torch
+    threads = torch.get_num_threads()
+    host = host_cpu()
+    t_all = cpu_blocks(step, threads, block_steps)
+    t_4 = cpu_blocks(step, 4, block_steps)
+    return {"value": round(cells / t_all / 1e6, 3), "unit": "Mcell-steps/s", "cores": threads, "kind": "port",
+            "value_4_threads": round(cells / t_4 / 1e6, 3), "host": host,
+            "sample": f"{what}, FP32, torch CPU oracle with flush-to-zero: one warm-up step, then the best of "
+                      f"{TIMED_BLOCKS} blocks of {block_steps} steps, at {threads} threads (value) and at 4 "
+                      f"threads (value_4_threads); host {host['model']}, {host['physical_cores']} physical "
+                      f"cores, {host['affinity_cpus']} in this process's CPU share"}
+
+
+def cpu_baseline_3d(block_steps: int = 3):
+    """The CPU oracle on the 3D car case (see ``cpu_report``)."""
     import torch
 
     from aerolume_amd.ns3d import setup as su
@@ -298,27 +386,16 @@ def cpu_baseline_3d(seconds: float = 20.0):
     u, v, w = O.apply_boundary_values(u, v, w, ops)
     u, v, w, _ = O.project3d(u, v, w, ops)
     z = torch.zeros((), dtype=ops.dtype)
-    st = O.State3D(u, v, w, torch.zeros(ops.nz, ops.ny, ops.nx, dtype=ops.dtype), z.clone(), z.clone())
-    threads = torch.get_num_threads()
-    torch.set_flush_denormal(True)
-    st, _ = O.step3d_full(st, geom, ops)
-    n, t_total, best = 0, 0.0, float("inf")
-    while t_total < seconds and n < 20:
-        t0 = time.perf_counter()
-        st, _ = O.step3d_full(st, geom, ops)
-        dt = time.perf_counter() - t0
-        best = min(best, dt)
-        t_total += dt
-        n += 1
-    cells = NX3 * NY3 * NZ3
-    return {"value": round(cells / best / 1e6, 3), "unit": "Mcell-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{n} steps of the 256x128x128 car case (FP32) after 1 warm-up step, best step of "
-                      f"{t_total:.1f} s, torch CPU oracle, {threads} threads, flush-to-zero"}
+    box = [O.State3D(u, v, w, torch.zeros(ops.nz, ops.ny, ops.nx, dtype=ops.dtype), z.clone(), z.clone())]
+
+    def step():
+        box[0], _ = O.step3d_full(box[0], geom, ops)
+    return cpu_report(step, NX3 * NY3 * NZ3, block_steps, "the 256x128x128 car case")
 
 
-def cpu_baseline(seconds: float = 15.0):
-    """The CPU oracle (torch, FP32, all host threads, flush-to-zero) on the same case:
-    warm-up step, then as many steps as fit in about ``seconds``, best per-step time."""
+def cpu_baseline(block_steps: int = 60):
+    """The CPU oracle on the 2D heated-airfoil case (see ``cpu_report``); 60-step blocks as
+    validation/bench_ns2d.py:81."""
     import torch
 
     from aerolume_amd.ns2d import setup as su
@@ -330,22 +407,12 @@ def cpu_baseline(seconds: float = 15.0):
     u, v = O.apply_boundary_values(torch.as_tensor(u, dtype=ops.dtype), torch.as_tensor(v, dtype=ops.dtype), ops)
     u, v, _, _ = O.project(u, v, ops)
     z = torch.zeros((), dtype=ops.dtype)
-    st = O.State(u, v, torch.zeros(ops.ny, ops.nx, dtype=ops.dtype), torch.zeros(ops.ny, ops.nx, dtype=ops.dtype),
-                 z.clone(), z.clone(), torch.zeros(ops.ny, ops.nx, dtype=ops.dtype))
-    threads = torch.get_num_threads()
-    torch.set_flush_denormal(True)
-    st, _ = O.step_full(st, geom, ops)
-    n, t_total, best = 0, 0.0, float("inf")
-    while t_total < seconds and n < 200:
-        t0 = time.perf_counter()
-        st, _ = O.step_full(st, geom, ops)
-        dt = time.perf_counter() - t0
-        best = min(best, dt)
-        t_total += dt
-        n += 1
-    return {"value": round(NX * NY / best / 1e6, 3), "unit": "Mcell-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{n} steps of the 1024x512 heated-airfoil case (FP32) after 1 warm-up step, best step "
-                      f"of {t_total:.1f} s, torch CPU oracle, {threads} threads, flush-to-zero"}
+    box = [O.State(u, v, torch.zeros(ops.ny, ops.nx, dtype=ops.dtype), torch.zeros(ops.ny, ops.nx, dtype=ops.dtype),
+                   z.clone(), z.clone(), torch.zeros(ops.ny, ops.nx, dtype=ops.dtype))]
+
+    def step():
+        box[0], _ = O.step_full(box[0], geom, ops)
+    return cpu_report(step, NX * NY, block_steps, "the 1024x512 heated-airfoil case")
 
 
 def run_only(args) -> int:
@@ -355,11 +422,11 @@ def run_only(args) -> int:
     # at least two 10-step blocks of warm-up, so that every HIP graph the timed steps replay
     # is captured before the clock starts
     if args.only == "2d":
-        _, wall, stage_ms = gpu_run(args.steps, max(args.warmup, 20))
-        out.update(ms_per_step=wall / args.steps * 1e3, stage_us=stage_ms * 1e3)
+        _, walls, stage_ms = gpu_run(args.steps, args.warmup)
+        out.update(ms_per_step=min(walls) / args.steps * 1e3, stage_us=stage_ms * 1e3)
     elif args.only == "3d":
-        _, wall, stage_ms = gpu_run_3d(args.steps3d, max(args.warmup3d, 20))
-        out.update(ms_per_step=wall / args.steps3d * 1e3, stage_us=stage_ms * 1e3)
+        _, walls, stage_ms = gpu_run_3d(args.steps3d, args.warmup3d)
+        out.update(ms_per_step=min(walls) / args.steps3d * 1e3, stage_us=stage_ms * 1e3)
     elif args.only == "ensemble":
         out.update(gpu_run_ensemble(args.ensemble_steps, 20))
     else:
@@ -380,7 +447,6 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--steps3d", type=int, default=500)
     ap.add_argument("--warmup3d", type=int, default=20)
     ap.add_argument("--no-3d", action="store_true", help="skip the 3D car workload")
@@ -397,7 +463,8 @@ def main() -> int:
         return 2
     if args.only is not None:
         return run_only(args)
-    solver, wall, stage_ms = gpu_run(args.steps, args.warmup)
+    solver, walls, stage_ms = gpu_run(args.steps, args.warmup)
+    wall = min(walls)
     cells = NX * NY
     value = cells * args.steps / wall / 1e6
     ms_step = wall / args.steps * 1e3
@@ -406,7 +473,8 @@ def main() -> int:
     step_gbs = STEP_BYTES_PER_CELL * cells / (wall / args.steps) / 1e9
     line = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mcell-steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(ms_step, 5), "higher_is_better": True, "scaling": "weak",
+        "warmup": warm_steps(args.warmup), "ms_per_step": round(ms_step, 5),
+        "timed_blocks_ms": [round(w * 1e3, 4) for w in walls], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "2D 1024x512 heated NACA 0012 at 8 deg, Re 1e5, Smagorinsky, Boussinesq Pr 0.71 "
                                "Ri 0.1, line smoke emitter, upwind3 + SSP-RK3 CFL 0.5, DCT projection",
@@ -419,19 +487,21 @@ def main() -> int:
                      "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline()
         line["gpu_over_cpu"] = round(value / line["cpu_baseline"]["value"], 1)
     del solver
     line["fp64_check"] = fp64_check_2d()
     if not args.no_3d:
-        s3, wall3, stage3_ms = gpu_run_3d(args.steps3d, args.warmup3d)
+        s3, walls3, stage3_ms = gpu_run_3d(args.steps3d, args.warmup3d)
+        wall3 = min(walls3)
         cells3 = NX3 * NY3 * NZ3
         v3 = cells3 * args.steps3d / wall3 / 1e6
         sb3 = STAGE3_BYTES_PER_CELL * cells3
         ach3 = sb3 / (stage3_ms * 1e-3) / 1e9
         step3 = STEP3_BYTES_PER_CELL * cells3 / (wall3 / args.steps3d) / 1e9
-        w3 = {"value": round(v3, 2), "unit": "Mcell-steps/s", "steps": args.steps3d, "warmup": args.warmup3d,
-              "ms_per_step": round(wall3 / args.steps3d * 1e3, 5), "dtype": "f32", "data": "synthetic",
+        w3 = {"value": round(v3, 2), "unit": "Mcell-steps/s", "steps": args.steps3d,
+              "warmup": warm_steps(args.warmup3d), "ms_per_step": round(wall3 / args.steps3d * 1e3, 5),
+              "timed_blocks_ms": [round(w * 1e3, 4) for w in walls3], "dtype": "f32", "data": "synthetic",
               "config": {"workload": "3D 256x128x128 car (4 x 1.6 x 1.2 rounded box on 4 rolling wheels), "
                                      "moving floor, periodic y, Re 5e4, WALE, upwind3 + SSP-RK3 CFL 0.5, "
                                      "DCT/DHT projection", "nx": NX3, "ny": NY3, "nz": NZ3, "cells": cells3,

@@ -103,6 +103,9 @@ def load() -> ctypes.CDLL:
     lib.aero2d_step_chained.restype = ctypes.c_int
     lib.aero2d_project.argtypes = [P, P, P, P, P]
     lib.aero2d_poisson_solve.argtypes = [P, P, P, P]
+    lib.aero2d_poisson_solve_static.argtypes = [P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
+                                                ctypes.c_int32, P, P]
+    lib.aero2d_poisson_solve_static.restype = ctypes.c_int
     lib.aero2d_time_stage.argtypes = [P, P, ctypes.POINTER(Fields2D), ctypes.POINTER(Geom2D),
                                       ctypes.POINTER(Params2D), ctypes.c_int, ctypes.c_int,
                                       ctypes.POINTER(ctypes.c_double)]

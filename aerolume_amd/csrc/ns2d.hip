@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -2711,3 +2712,195 @@ int aero2d_ensemble_destroy(aero2d_ensemble *e) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- allocation-free Poisson solve
+// aero2d_poisson_solve_static: the solve of aero2d_poisson_solve without a plan, for callers
+// that may not allocate device memory (code under graph capture, serving loops). Its spectral
+// work array and transform tables live in `scratch`, a static device array each GPU holds
+// until the process ends; every call rebuilds the tables there (two small kernels) before the
+// three transform passes, which are the plan path's own kernels. Calls that overlap on one
+// GPU share that array and race.
+namespace aero {
+namespace ns2d {
+
+constexpr size_t kStaticCells = size_t(1) << 20;   // largest nx * ny of the static solve
+constexpr size_t kStaticAlign = 256;
+// float64 layout at the largest sizes: F and 1/lambda (cells each), twiddles and DCT
+// twiddles (complex, per axis), eigenvalues (float64, per axis)
+constexpr size_t kStaticBytes = 2 * kStaticCells * 8 + 2 * 2 * kMaxLen * 16 + 2 * kMaxLen * 8 + 8 * kStaticAlign;
+
+__device__ __align__(256) unsigned char scratch[kStaticBytes];
+
+// Entry o of the stage-ordered twiddle table of an n-point transform (the layout
+// stage_twiddles writes on the host for a plan): stage of span p and radix R holds
+// w^(r i n / (p R)) at (p - 1) + (r - 1) p + i; the last entry is w^0.
+template <typename T>
+__device__ cplx<T> stage_twiddle_at(const FftPlan &pl, int o) {
+  const int n = pl.n;
+  int p = 1;
+  for (int s = 0; s < pl.nrad; ++s) {
+    const int R = pl.radix[s];
+    if (o < (p - 1) + (R - 1) * p) {
+      const int q = o - (p - 1);
+      const int m = (q / p + 1) * (q % p) * (n / (p * R));
+      const double ang = -2.0 * M_PI * (double)m / (double)n;
+      return {(T)cos(ang), (T)sin(ang)};
+    }
+    p *= R;
+  }
+  return {(T)1, (T)0};
+}
+
+// Per-axis tables of an n-point axis (the values upload_tables forms on the host): stage
+// twiddles, quarter-wave DCT twiddles and the Laplacian eigenvalues in float64.
+template <typename T>
+__global__ void __launch_bounds__(256) k_static_axis(FftPlan pl, double hh, int periodic, cplx<T> *__restrict__ tw,
+                                                     cplx<T> *__restrict__ cs, double *__restrict__ lam) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = pl.n;
+  if (m >= n) return;
+  tw[m] = stage_twiddle_at<T>(pl, m);
+  const double th = M_PI * (double)m / (2.0 * (double)n);
+  cs[m] = {(T)cos(th), (T)sin(th)};
+  lam[m] = periodic ? (2.0 * cos(2.0 * M_PI * (double)m / (double)n) - 2.0) / hh
+                    : (2.0 * cos(M_PI * (double)m / (double)n) - 2.0) / hh;
+}
+
+// 1 / (lam_x + lam_y) formed in float64 and rounded once, the zero mode 0 (ns2d/poisson.py:115-125).
+template <typename T>
+__global__ void __launch_bounds__(256) k_static_invlam(int nx, int ny, const double *__restrict__ lx,
+                                                       const double *__restrict__ ly, T *__restrict__ inv) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)nx * ny) return;
+  const int i = (int)(e / ny), k = (int)(e % ny);
+  const double la = lx[i] + ly[k];
+  inv[e] = (T)((la != 0.0 && !(i == 0 && k == 0)) ? 1.0 / la : 0.0);
+}
+
+}  // namespace ns2d
+}  // namespace aero
+
+namespace {
+
+constexpr int kStaticMaxDevices = 64;
+std::mutex g_static_mu;
+unsigned char *g_static_ptr[kStaticMaxDevices] = {};   // device address of `scratch`, per device
+
+// Device address of `scratch` on the current device, looked up once per device (with that
+// device current) together with the transform kernels' LDS limit.
+int static_scratch(unsigned char **out) {
+  int dev = 0;
+  AERO_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kStaticMaxDevices) {
+    set_error("aero2d_poisson_solve_static: device index out of range");
+    return AERO_EUNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lock(g_static_mu);
+  if (!g_static_ptr[dev]) {
+    void *ptr = nullptr;
+    AERO_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(aero::ns2d::scratch)));
+    int max_lds = 0;
+    AERO_HIP_CHECK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    const void *fns[] = {(const void *)k_poisA<float, 256>,   (const void *)k_poisB<float, 256>,
+                         (const void *)k_poisC<float, 256>,   (const void *)k_poisA<float, 512>,
+                         (const void *)k_poisB<float, 512>,   (const void *)k_poisC<float, 512>,
+                         (const void *)k_poisA<float, 1024>,  (const void *)k_poisB<float, 1024>,
+                         (const void *)k_poisC<float, 1024>,  (const void *)k_poisA<double, 256>,
+                         (const void *)k_poisB<double, 256>,  (const void *)k_poisC<double, 256>,
+                         (const void *)k_poisA<double, 512>,  (const void *)k_poisB<double, 512>,
+                         (const void *)k_poisC<double, 512>,  (const void *)k_poisA<double, 1024>,
+                         (const void *)k_poisB<double, 1024>, (const void *)k_poisC<double, 1024>};
+    for (const void *f : fns)
+      AERO_HIP_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+    g_static_ptr[dev] = (unsigned char *)ptr;
+  }
+  *out = g_static_ptr[dev];
+  return AERO_OK;
+}
+
+size_t align_up(size_t n) { return (n + kStaticAlign - 1) / kStaticAlign * kStaticAlign; }
+
+template <typename T>
+int static_solve(unsigned char *base, hipStream_t st, int nx, int ny, double h, int periodic, const T *f, T *psi) {
+  const size_t cells = (size_t)nx * ny;
+  Dev<T> d;
+  std::memset(&d, 0, sizeof(d));
+  d.nx = nx;
+  d.ny = ny;
+  d.hd = h;
+  d.h = (T)h;
+  d.periodic = periodic;
+  d.dct_x = periodic ? 0 : 1;
+  d.dct_y = periodic ? 0 : 1;
+  if (!factor_radices(nx, d.fx) || !factor_radices(ny, d.fy)) {
+    set_error("aero2d_poisson_solve_static: nx and ny must be products of 2, 3 and 5");
+    return AERO_EUNSUPPORTED;
+  }
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    unsigned char *q = base + off;
+    off += align_up(bytes);
+    return q;
+  };
+  d.F = (T *)take(cells * sizeof(T));
+  T *inv = (T *)take(cells * sizeof(T));
+  cplx<T> *tw_x = (cplx<T> *)take(nx * sizeof(cplx<T>)), *tw_y = (cplx<T> *)take(ny * sizeof(cplx<T>));
+  cplx<T> *cs_x = (cplx<T> *)take(nx * sizeof(cplx<T>)), *cs_y = (cplx<T> *)take(ny * sizeof(cplx<T>));
+  double *lam_x = (double *)take(nx * sizeof(double)), *lam_y = (double *)take(ny * sizeof(double));
+  if (off > kStaticBytes) {
+    set_error("aero2d_poisson_solve_static: scratch layout exceeds the static array");
+    return AERO_EUNSUPPORTED;
+  }
+  d.inv_lam = inv;
+  d.tw_x = tw_x; d.tw_y = tw_y; d.cs_x = cs_x; d.cs_y = cs_y; d.lam_x = lam_x; d.lam_y = lam_y;
+  const double hh = h * h;
+  hipLaunchKernelGGL(k_static_axis<T>, dim3((nx + 255) / 256), dim3(256), 0, st, d.fx, hh, periodic, tw_x, cs_x, lam_x);
+  hipLaunchKernelGGL(k_static_axis<T>, dim3((ny + 255) / 256), dim3(256), 0, st, d.fy, hh, periodic, tw_y, cs_y, lam_y);
+  hipLaunchKernelGGL(k_static_invlam<T>, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, st, nx, ny,
+                     (const double *)lam_x, (const double *)lam_y, inv);
+  const int ntA = transform_threads(nx), ntB = transform_threads(ny);
+  const size_t ldsA = 3 * nx * sizeof(cplx<T>) + 2 * nx * sizeof(T) + 64 * 8;
+  const size_t ldsB = ny * (2 * sizeof(cplx<T>) + 2 * sizeof(T)) + ny * sizeof(cplx<T>);
+  by_threads(ntA, [&](auto nt) {
+    constexpr int NT = decltype(nt)::value;
+    hipLaunchKernelGGL((k_poisA<T, NT>), dim3((ny + 1) / 2), dim3(NT), ldsA, st, d, (const T *)nullptr,
+                       (const T *)nullptr, f, 0);
+  });
+  by_threads(ntB, [&](auto nt) {
+    constexpr int NT = decltype(nt)::value;
+    hipLaunchKernelGGL((k_poisB<T, NT>), dim3((nx + 1) / 2), dim3(NT), ldsB, st, d, 1);
+  });
+  by_threads(ntA, [&](auto nt) {
+    constexpr int NT = decltype(nt)::value;
+    hipLaunchKernelGGL((k_poisC<T, NT>), dim3((ny + 1) / 2), dim3(NT), ldsA, st, d, psi, (T *)nullptr, 0);
+  });
+  AERO_HIP_CHECK(hipGetLastError());
+  return AERO_OK;
+}
+
+}  // namespace
+
+extern "C" int aero2d_poisson_solve_static(void *stream, int32_t dtype, int32_t nx, int32_t ny, double h,
+                                           int32_t periodic, const void *f, void *psi) {
+  if (!f || !psi) { set_error("aero2d_poisson_solve_static: null argument"); return AERO_EINVAL; }
+  if (dtype != 0 && dtype != 1) { set_error("aero2d_poisson_solve_static: dtype must be 0 or 1"); return AERO_EINVAL; }
+  if (nx < 4 || ny < 4 || !(h > 0)) {
+    set_error("aero2d_poisson_solve_static: needs at least 4 x 4 cells and h > 0");
+    return AERO_EINVAL;
+  }
+  const size_t es = dtype ? 8 : 4;
+  const size_t ldsx = 3 * (size_t)nx * 2 * es + 2 * (size_t)nx * es + 512;
+  const size_t ldsy = (size_t)ny * (2 * 2 * es + 2 * es) + (size_t)ny * 2 * es;
+  if ((size_t)nx * ny > kStaticCells || nx > kMaxLen || ny > kMaxLen || ldsx > kLdsBlockMax || ldsy > kLdsBlockMax) {
+    set_error("aero2d_poisson_solve_static: grid larger than the static scratch allows (nx * ny <= " +
+              std::to_string(kStaticCells) + ", nx, ny <= " + std::to_string(kMaxLen) +
+              ", rows within one block's LDS)");
+    return AERO_EUNSUPPORTED;
+  }
+  unsigned char *base = nullptr;
+  int rc = static_scratch(&base);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  return dtype ? static_solve<double>(base, st, nx, ny, h, periodic != 0, (const double *)f, (double *)psi)
+               : static_solve<float>(base, st, nx, ny, h, periodic != 0, (const float *)f, (float *)psi);
+}

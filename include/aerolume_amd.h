@@ -177,6 +177,16 @@ int aero2d_step_trace(aero2d_plan *plan, void *stream, const aero2d_fields *in, 
 int aero2d_project(aero2d_plan *plan, void *stream, void *u, void *v, void *psi);
 /* psi = L^-1 f with zero mean (f, psi (ny, nx); may alias). */
 int aero2d_poisson_solve(aero2d_plan *plan, void *stream, const void *f, void *psi);
+/* No counterpart in the reference: aero2d_poisson_solve without a plan, for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops). The
+ * grid is described by the arguments (dtype 0 float32 / 1 float64, nx x ny cells of size h,
+ * periodic 0: Neumann DCT solve, 1: periodic); f and psi (ny, nx) are the caller's and may
+ * alias. Nothing is allocated: the spectral work array and the transform tables live in a
+ * static device array of 17.1 MB (16.3 MiB) that each GPU holds until the process ends, rebuilt by
+ * every call. Limits: nx * ny <= 1048576 cells, nx, ny <= 4096 and 2, 3, 5-smooth (larger
+ * grids return AERO_EUNSUPPORTED). Calls that overlap on one GPU share the array and race. */
+int aero2d_poisson_solve_static(void *stream, int32_t dtype, int32_t nx, int32_t ny, double h, int32_t periodic,
+                                const void *f, void *psi);
 
 /* Timing helper (no reference counterpart; used by bench.py to price the dominant kernel
  * against the HBM roofline): launches the fused stage kernel of RK stage `stage` (0..2)

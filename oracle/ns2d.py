@@ -809,6 +809,61 @@ def step_full(state: State, geom, ops, dt=None, *, out: dict | None = None, trac
     return new, coeffs
 
 
+def init_state(cfg, ops, perturb=None) -> State:
+    """Impulsive start (core.py:216-271): the inlet profile everywhere with v = 0 (zero flow
+    in a periodic or closed box); with ``perturb`` and bodies in a tunnel, a Lamb-Oseen
+    vortex of peak swirl 0.05 centred on the bodies' bounding box (core radius 0.75 of its
+    size), boundary values and projection. Smoke and temperature start as the reference's."""
+    import math
+
+    import numpy as np
+    perturb = cfg.perturb if perturb is None else perturb
+    dt_ = ops.dtype
+    tunnel = not (ops.periodic or ops.closed)
+    if tunnel:
+        u = ops.u_inlet.expand(ops.ny, ops.nx + 1).clone()
+    else:
+        u = torch.zeros(ops.ny, ops.nx + 1, dtype=dt_)
+    v = torch.zeros(ops.ny + 1, ops.nx, dtype=dt_)
+    if perturb and cfg.bodies and tunnel:
+        pts = np.vstack([b.points for b in cfg.bodies])
+        lo, hi = pts.min(axis=0), pts.max(axis=0)
+        cx, cy = 0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1])
+        size = max(hi[1] - lo[1], 0.25 * (hi[0] - lo[0]), 4 * ops.h)
+        rc = 0.75 * size
+        gam = 0.05 * 2.0 * math.pi * rc / 0.638
+        xc = ops.x0 + (torch.arange(ops.nx, dtype=dt_) + 0.5) * ops.h
+        yc = ops.y0 + (torch.arange(ops.ny, dtype=dt_) + 0.5) * ops.h
+        xu = ops.x0 + torch.arange(ops.nx + 1, dtype=dt_) * ops.h
+        yv = ops.y0 + torch.arange(ops.ny + 1, dtype=dt_) * ops.h
+
+        def swirl(x, y):
+            dx, dy = x - cx, y - cy
+            r2 = (dx * dx + dy * dy).clamp_min(1e-12)
+            f = gam / (2 * math.pi * r2) * (1 - torch.exp(-r2 / rc**2))
+            return -f * dy, f * dx
+        yy, xx = torch.meshgrid(yc, xu, indexing="ij")
+        u = u + swirl(xx, yy)[0]
+        yy, xx = torch.meshgrid(yv, xc, indexing="ij")
+        v = v + swirl(xx, yy)[1]
+        u, v = apply_boundary_values(u, v, ops)
+        u, v, _, _ = project(u, v, ops)
+    zero = torch.zeros((), dtype=dt_)
+    ph = ops.phys
+    n_dye = cfg.dye_channels
+    dye = torch.zeros(ops.ny, ops.nx, dtype=dt_) if n_dye else None
+    theta = dye_extra = ctrl = None
+    if ph is not None:
+        if ph.thermal:
+            th_init = ph.theta_init if not tunnel else float(cfg.thermal.theta_inlet)
+            theta = torch.full((ops.ny, ops.nx), th_init, dtype=dt_)
+        if n_dye > 1:
+            dye_extra = torch.zeros(n_dye - 1, ops.ny, ops.nx, dtype=dt_)
+        if getattr(ph, "ctrl_size", 0):
+            ctrl = torch.zeros(ph.ctrl_size, dtype=dt_)
+    return State(u, v, torch.zeros(ops.ny, ops.nx, dtype=dt_), dye, zero.clone(), zero.clone(), theta, dye_extra, ctrl)
+
+
 def noisy_start(ops, seed: int, amplitude: float = 0.01):
     """Inlet flow + seeded Gaussian noise on every face, boundary values, projection
     (the noisy initial state of the benchmark cases; core.py:328-334, 626-642)."""

@@ -56,6 +56,34 @@ def c2_cfg(nx=1024, ny=512, dtype="float32", device="cpu", turbulence="smagorins
                                               rate=1.0)])
 
 
+def default_cfg(dtype="float64", device="cpu"):
+    """The reference's default 2D path: every SolverConfig default (smoke with the inlet comb,
+    the Lamb-Oseen start, upwind3, no LES) on a 128x64 tunnel (h = 1/16, Re 200) around a disc
+    of diameter 0.5 at (2, 2)."""
+    from aerolume_amd.ns2d.config import ImmersedBody, SolverConfig
+    return SolverConfig(nx=128, ny=64, h=1.0 / 16.0, x0=0.0, y0=0.0, re=200.0,
+                        bodies=[ImmersedBody(circle(128, 0.5, 2.0, 2.0))], dtype=dtype, device=device)
+
+
+# 2D features stepped only by the variants fixture (64x32, h = 1/8, Re 400, float64, 5 steps)
+VARIANTS2D = {
+    "central_vreman_noslip": dict(advection="central", turbulence="vreman", bc_top="noslip", bc_bottom="noslip",
+                                  inlet="parabolic", bodies=True),
+    "wale_moving": dict(turbulence="wale", bc_bottom="moving", ground_speed=1.0, bodies=True),
+    "periodic_smag": dict(periodic=True, turbulence="smagorinsky", bodies=False),
+    "closed_lid": dict(domain="closed", bc_left="slip", bc_right="slip", bc_bottom="moving", ground_speed=1.0,
+                       bodies=True),
+}
+
+
+def variant_cfg(name, dtype="float64", device="cpu"):
+    from aerolume_amd.ns2d.config import ImmersedBody, SolverConfig
+    kw = dict(VARIANTS2D[name])
+    bodies = [ImmersedBody(circle(64, 0.75, 2.0, 2.0))] if kw.pop("bodies") else []
+    return SolverConfig(nx=64, ny=32, h=1.0 / 8.0, x0=0.0, y0=0.0, re=400.0, bodies=bodies, dtype=dtype,
+                        device=device, dye=False, perturb=False, **kw)
+
+
 # ---------------------------------------------------------------------------
 # 3D
 # ---------------------------------------------------------------------------

@@ -147,6 +147,12 @@ def test_c1_500_steps_match_reference(golden, dtype):
     div = float(S.divergence(cur.u, cur.v, ops.h).abs().max())
     ref = F["u500"], F["v500"]
     ref_div = float(np.abs((ref[0][:, 1:] - ref[0][:, :-1] + ref[1][1:] - ref[1][:-1]) / ops.h).max())
+    dv = S.divergence(cur.u, cur.v, ops.h).double()
+    rms = float(dv.pow(2).mean().sqrt())
+    rd = (ref[0][:, 1:] - ref[0][:, :-1] + ref[1][1:] - ref[1][:-1]) / ops.h
+    ref_rms = float(np.sqrt(np.mean(np.asarray(rd, dtype=np.float64) ** 2)))
+    print(f"c1 {dtype} step 500 max|div| gpu {div:.4e} reference {ref_div:.4e}; rms gpu {rms:.4e} "
+          f"reference {ref_rms:.4e}", flush=True)
     assert div <= max(1.5 * ref_div, 1e-12 if dtype == "float64" else 1e-5)
 
 

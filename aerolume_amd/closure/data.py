@@ -108,6 +108,22 @@ def sgs_target(tau, g, eps: float = 1e-12):
     return torch.clamp(num / den, min=0.0)
 
 
+def wale_nut(g):
+    """WALE eddy viscosity up to its constant factor (Cw h)^2 from a cell-centred gradient
+    (9, ...) = du_i/dx_j: sd^1.5 / (S:S^2.5 + sd^1.25), sd = |S^d|^2 of the traceless square
+    of the gradient (the solver's closure on the same gradient, made traceless as the solver
+    does, by du_3/dx_3 = -(du_1/dx_1 + du_2/dx_2); the factor drops out of a correlation)."""
+    G = g.view(3, 3, *g.shape[1:]).clone()
+    G[2, 2] = -(G[0, 0] + G[1, 1])
+    eye = torch.eye(3, dtype=g.dtype, device=g.device).view(3, 3, *([1] * (g.dim() - 1)))
+    s = 0.5 * (G + G.transpose(0, 1))
+    g2 = torch.einsum("ik...,kj...->ij...", G, G)
+    sd = 0.5 * (g2 + g2.transpose(0, 1)) - ((g2[0, 0] + g2[1, 1] + g2[2, 2]) / 3.0) * eye
+    ss = (s * s).sum(dim=(0, 1)).clamp_min(0.0)
+    sdd = (sd * sd).sum(dim=(0, 1)).clamp_min(0.0)
+    return sdd ** 1.5 / (ss ** 2.5 + sdd ** 1.25 + 1e-30)
+
+
 def kinetic_energy_spectrum(u, v, w) -> tuple[np.ndarray, np.ndarray]:
     """(k, E(k)) of the cell-centred velocity of a periodic staggered field, E summed over
     integer shells |k| in [k - 1/2, k + 1/2), normalised so that sum E = mean(|u|^2) / 2."""

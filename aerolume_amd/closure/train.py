@@ -86,7 +86,10 @@ def train(xs, ys, m: int, iters: int, lr: float, seed: int):
     pred = torch.cat([net.infer(x)[0].reshape(-1) for x in xn])
     targ = torch.cat([y.reshape(-1) for y in yn])
     corr = float(torch.corrcoef(torch.stack([pred, targ]))[0, 1])
-    return net, g_scale, nu_scale, trust, losses, corr
+    # the same statistic for WALE, the model the network stands in for
+    wale = torch.cat([D.wale_nut(x).reshape(-1) for x in xs])
+    corr_wale = float(torch.corrcoef(torch.stack([wale, targ]))[0, 1])
+    return net, g_scale, nu_scale, trust, losses, corr, corr_wale
 
 
 def run_coarse(m: int, start, t_end: float, re: float, closure=None, turbulence: str = "wale",
@@ -123,7 +126,7 @@ def experiment(n: int = 256, r: int = 4, re: float = 1000.0, t_start: float = 0.
     xs, ys = features_and_targets(snaps, m, re)
     t_data = time.perf_counter() - t0
     t0 = time.perf_counter()
-    net, g_scale, nu_scale, trust, losses, corr = train(xs, ys, m, iters, lr, seed=0)
+    net, g_scale, nu_scale, trust, losses, corr, corr_wale = train(xs, ys, m, iters, lr, seed=0)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     del snaps, xs, ys
@@ -141,6 +144,7 @@ def experiment(n: int = 256, r: int = 4, re: float = 1000.0, t_start: float = 0.
               "coarse_advection": adv,
               "train_snapshots": n_snap, "train_iters": iters, "train_seconds": round(t_train, 2),
               "data_seconds": round(t_data, 2), "a_priori_correlation": round(corr, 4),
+              "a_priori_correlation_wale": round(corr_wale, 4),
               "loss_first_last": [round(losses[0], 5), round(losses[-1], 5)],   # mean over all snapshots
               "g_scale": g_scale, "nu_scale": nu_scale, "trust": trust,
               "fine_wall_s": round(fine_wall, 3), "fine_steps": fine_steps, "k_c": kc,

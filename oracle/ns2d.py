@@ -20,6 +20,10 @@ Pinning (what checks each function besides retyping it):
   central scheme, quadratic exactness of the upwind3 reconstruction.
 * LES models - closed forms in float64 numpy, zero in pure shear for WALE / Vreman.
 * thermal, smoke - the heated-airfoil fixture, conduction / maximum-principle invariants.
+* ``init_state``, ``advect_dye`` - the reference FlowSolver's default path (Lamb-Oseen start,
+  inlet smoke comb; ``tests/golden/ns2d_default_*.npz`` written by
+  ``tests/golden/make_ns2d_default_fixtures.py``), a planted zero-inflow comb, and the comb's
+  invariants (conservation, no new extrema, uniform stays uniform).
 * ``pid_dt`` - the reference's PID trajectory (``tests/golden/ns2d_pid_diag.npz``, written by
   ``tests/golden/make_ns2d_diag_fixtures.py``) and the controller's closed forms (first step,
   fixed point at the target, growth cap, clip at the limit).

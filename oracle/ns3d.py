@@ -17,6 +17,9 @@ Poisson residual and round trip (closed form), divergence of projected noise, Ta
 decay, zero eddy viscosity in pure shear for WALE / Vreman. ``PidDt3`` - the reference's PID
 trajectory on the heated box (``tests/golden/ns3d_pid_box.npz``) and the controller's closed
 forms (first step, fixed point at the target, both growth clamps, clip at the limit).
+``learned_nut`` (no reference counterpart) - closed forms in a step: an untrusted field
+everywhere gives the WALE step bit for bit, a trusted constant c gives the laminar step at
+viscosity nu + c (``tests/test_closure_data.py``).
 """
 from __future__ import annotations
 

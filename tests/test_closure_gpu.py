@@ -77,7 +77,7 @@ def test_closure_features_match_oracle_gradient():
         assert float((out[q].cpu() - ref).abs().max()) < 1e-12 * max(1.0, float(ref.abs().max()))
 
 
-def test_trained_closure_gains_over_no_model():
+def test_trained_closure_gains_over_wale_fallback():
     _need_gpu()
     from aerolume_amd.closure.train import experiment
     print("closure experiment 96 -> 24", flush=True)
@@ -86,7 +86,12 @@ def test_trained_closure_gains_over_no_model():
     for name in ("learned", "wale", "none"):
         print(name, {k: v for k, v in rep[name].items() if k != "E"}, flush=True)
         assert rep[name]["finite"]
-    assert rep["a_priori_correlation"] > 0.2
+    # a priori bar: the trained network's prediction correlates with the subgrid-transfer
+    # target better than WALE's eddy viscosity does on the same filtered snapshots (WALE is the
+    # model the network replaces; a correlation below it would mean the network adds nothing)
+    print(f"a priori correlation: network {rep['a_priori_correlation']}, WALE {rep['a_priori_correlation_wale']}",
+          flush=True)
+    assert rep["a_priori_correlation"] > rep["a_priori_correlation_wale"]
     assert rep["loss_first_last"][1] < rep["loss_first_last"][0]
     # the gain over the method without the network (WALE, its fallback): the learned closure's
     # coarse spectrum is closer to the filtered fine spectrum for k <= k_c / 2 (deterministic:

@@ -159,10 +159,13 @@ def test_heated_airfoil_full_size_matches_oracle(dtype):
     ref_rms = float(O.divergence(ref.u, ref.v, ops_c.h).double().pow(2).mean().sqrt())
     print(f"c2 1024x512 {dtype} max|div| gpu {div:.3e} oracle {ref_div:.3e}; rms gpu {rms:.3e} oracle "
           f"{ref_rms:.3e}", flush=True)
-    # post-projection divergence: a rounding floor in float32 (half an ulp of u on each of a
-    # cell's four faces, over h), no higher than the oracle's by more than that floor's spread
-    floor = 2.0 * float(np.spacing(np.float32(fs.state.u.abs().max().item()))) / fs.ops.h
-    assert div <= (max(ref_div, 1e-12) * 1.5 if f64 else max(ref_div, floor))
+    # post-projection divergence: the maximum under the rounding floor (half an ulp of u on each
+    # of a cell's four faces, over h) or the oracle's maximum; the RMS within 2 % of the
+    # oracle's (measured: float32 5.112e-6 against 5.108e-6, float64 9.551e-15 against 9.496e-15)
+    eps = float(np.spacing(np.float64(1.0) if f64 else np.float32(1.0)))
+    floor = 2.0 * eps * float(fs.state.u.abs().max()) / fs.ops.h
+    assert div <= max(ref_div, floor)
+    assert rms <= 1.02 * ref_rms
 
 
 def test_sweep_64_runs_equal_single_runs_and_oracle():

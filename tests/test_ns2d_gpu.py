@@ -125,8 +125,11 @@ def test_traced_step_matches_reference(golden):
 def test_c1_500_steps_match_reference(golden, dtype):
     """256x128 disc case, 500 steps from the seeded noisy start, against the reference's
     trajectory. float64: 1e-9; float32: |du|,|dv| <= 5e-5 and |dp| <= 2e-4 (the reference's
-    own FP32-vs-FP64 spread at step 500 is of that order); divergence no higher than the
-    reference's float32 state."""
+    own FP32-vs-FP64 spread at step 500 is of that order). Post-projection divergence: its
+    maximum is rounding noise (half an ulp of u on each of a cell's four faces, over h), so it
+    must stay under that floor or the reference's own maximum, whichever is larger, and its RMS
+    within 2 % (float32) / 5 % (float64) of the reference state's RMS (measured on MI355X:
+    1.3757e-6 against 1.3719e-6 in float32, 2.612e-15 against 2.569e-15 in float64)."""
     _need_gpu()
     S = _S()
     tag = "f64" if dtype == "float64" else "f32"
@@ -153,7 +156,9 @@ def test_c1_500_steps_match_reference(golden, dtype):
     ref_rms = float(np.sqrt(np.mean(np.asarray(rd, dtype=np.float64) ** 2)))
     print(f"c1 {dtype} step 500 max|div| gpu {div:.4e} reference {ref_div:.4e}; rms gpu {rms:.4e} "
           f"reference {ref_rms:.4e}", flush=True)
-    assert div <= max(1.5 * ref_div, 1e-12 if dtype == "float64" else 1e-5)
+    floor = 2.0 * float(np.spacing(np.float32(1.0) if dtype == "float32" else 1.0)) * float(cur.u.abs().max()) / ops.h
+    assert div <= max(ref_div, floor)
+    assert rms <= (1.02 if dtype == "float32" else 1.05) * ref_rms
 
 
 @pytest.mark.parametrize("dtype", ["float64", "float32"])

@@ -228,36 +228,67 @@ __device__ __forceinline__ int src_axis(int g, int ext, int n, int per) {
 
 // Pass 1 of the tile load for component C: the stage input at every LDS position (clamped
 // or wrapped source), u^ - grad psi with the outlet shift when c.psi is set (the previous
-// stage's projected field, core.py:281-297, 582-612).
+// stage's projected field, core.py:281-297, 582-612). The tile is walked row by row: each
+// half-wave takes one (z, y) row of the tile, its lanes the row's x positions (32 at a time),
+// so the source column of each lane and the source row of each half-wave are worked out once
+// (shifts and masks, no per-element division) and reused across the row's elements.
 template <typename T, int C, int EX, int EY, int EZ>
 __device__ __forceinline__ void load_comp(const Dev3<T> &d, const Call3<T> &c, T *lds, int k0, int j0, int i0, T corr) {
   const int nx = d.nx, ny = d.ny, nz = d.nz;
   const int ex = nx + (C == 0), ey = ny + (C == 1), ez = nz + (C == 2);
   const T *src = C == 0 ? c.up : (C == 1 ? c.vp : c.wp);
   const T *psi = c.psi;
-  const int pn = C == 0 ? nx : (C == 1 ? ny : nz);         // cells along the normal axis
+  const int pn = C == 0 ? nx : (C == 1 ? ny : nz);          // cells along the normal axis
   const int pper = d.per[C];
-  const int pstride = C == 0 ? 1 : (C == 1 ? nx : nx * ny);
   const bool shift = c.corr_stage >= 0 && C == 0 && d.outlet;
   const T inv_h = d.inv_h;
-  for (int e = threadIdx.x; e < EX * EY * EZ; e += kTileThreads) {
-    const int x = e % EX, t = e / EX, y = t % EY, z = t / EY;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int xl = lane & 31;
+  constexpr int NXP = (EX + 31) / 32;                        // 32-wide passes over a row
+  int sx[NXP], fcx[NXP], fmx[NXP];
+  bool xin[NXP], pvx[NXP], shx[NXP];
+#pragma unroll
+  for (int q = 0; q < NXP; ++q) {
+    const int x = xl + 32 * q;
+    xin[q] = x < EX;
     const int i = src_axis(i0 - 2 + x, ex, nx, d.per[0]);
+    sx[q] = i;
+    const int fc = i == pn ? 0 : i;
+    fcx[q] = fc;
+    fmx[q] = fc == 0 ? pn - 1 : fc - 1;
+    pvx[q] = C == 0 && (pper || (i > 0 && i < pn));
+    shx[q] = shift && i == nx;
+  }
+  for (int row = 2 * wave + (lane >> 5); row < EY * EZ; row += 2 * kTileWaves) {
+    const int y = row % EY, z = row / EY;
     const int j = src_axis(j0 - 2 + y, ey, ny, d.per[1]);
     const int k = src_axis(k0 - 2 + z, ez, nz, d.per[2]);
-    T val = src[(unsigned)((k * ey + j) * ex + i)];
-    if (psi) {
-      const int f = C == 0 ? i : (C == 1 ? j : k);
-      if (pper || (f > 0 && f < pn)) {
-        const int fc = f == pn ? 0 : f;
-        const int fm = fc == 0 ? pn - 1 : fc - 1;
-        const int ci = C == 0 ? 0 : i, cj = C == 1 ? 0 : j, ck = C == 2 ? 0 : k;
-        const T *pb = psi + (unsigned)((ck * ny + cj) * nx + ci);
-        val = val - (pb[(unsigned)(fc * pstride)] - pb[(unsigned)(fm * pstride)]) * inv_h;
+    const T *srow = src + (size_t)(k * ey + j) * ex;
+    T *lrow = lds + (z * EY + y) * EX;
+    // the normal-axis face index of a v (w) row is its y (z) position: row-uniform
+    const int f = C == 1 ? j : k;
+    const bool pv_row = C != 0 && (pper || (f > 0 && f < pn));
+    const int fc = f == pn ? 0 : f;
+    const int fm = fc == 0 ? pn - 1 : fc - 1;
+#pragma unroll
+    for (int q = 0; q < NXP; ++q) {
+      if (!xin[q]) continue;
+      T val = srow[sx[q]];
+      if (psi) {
+        if (C == 0) {
+          if (pvx[q]) {
+            const T *pb = psi + (size_t)(k * ny + j) * nx;
+            val = val - (pb[fcx[q]] - pb[fmx[q]]) * inv_h;
+          }
+        } else if (pv_row) {
+          const size_t pstride = C == 1 ? (size_t)nx : (size_t)nx * ny;
+          const T *pb = psi + (C == 1 ? (size_t)k * ny * nx : (size_t)j * nx) + sx[q];
+          val = val - (pb[fc * pstride] - pb[fm * pstride]) * inv_h;
+        }
       }
+      if (shx[q]) val = val + corr;
+      lrow[xl + 32 * q] = val;
     }
-    if (shift && i == nx) val = val + corr;
-    lds[e] = val;
   }
 }
 
@@ -607,88 +638,114 @@ __global__ void __launch_bounds__(kTileThreads, Tile<T>::WAVES_PER_SIMD) k_stage
   const T kpen = d.win ? bdt / *c.eta : (T)0;
   const int ie = min(i0 + S::TX, nx), je = min(j0 + S::TY, ny), ke = min(k0 + S::TZ, nz);
   double osum = 0.0, isum = 0.0;
-  // u faces
-  {
-    const int fe = face_end(i0, S::TX, nx), nfx = fe - i0, nfy = je - j0, nfz = ke - k0;
-    for (int e = threadIdx.x; e < nfx * nfy * nfz; e += kTileThreads) {
-      const int li = e % nfx, t = e / nfx, lj = t % nfy, lk = t / nfy;
-      const int gi = i0 + li, gj = j0 + lj, gk = k0 + lk;
-      const size_t idx = ((size_t)gk * ny + gj) * (nx + 1) + gi;
-      T lu;
-      const bool xb = !d.per[0] && (gi == 0 || gi == nx);
-      if (xb) lu = (gi == nx && d.outlet) ? (LU(lk, lj, li - 1) - LU(lk, lj, li)) * d.inv_h : (T)0;
-      else lu = rhs_u(d, L, lk, lj, li, nu_h2);
-      const T upv = LU(lk, lj, li);
-      T uh = (s == 0 ? upv : lerp_t(upv, c.u0[idx], a)) + lu * bdt;
-      if (xb && gi == 0) uh = d.outlet ? c.u_inlet[(size_t)gk * ny + gj] : (T)0;
-      if (xb && gi == nx && d.closed) uh = (T)0;
-      if (d.win && gk >= d.wk0 && gk < d.wk1 && gj >= d.wj0 && gj < d.wj1 && gi >= d.wi0 && gi <= d.wi1) {
-        const size_t wi = ((size_t)(gk - d.wk0) * d.wy + (gj - d.wj0)) * (d.wx + 1) + (gi - d.wi0);
-        const T un = (uh + kpen * c.a_u[wi]) / ((T)1 + kpen * c.chi_u[wi]);
-        const T du = un - uh;
-        d.acc_u[wi] = s == 0 ? du * ws : d.acc_u[wi] + du * ws;
-        uh = un;
-      }
-      c.uo[idx] = uh;
-      if (gi == nx && d.outlet) {
-        osum += (double)uh;
-        isum += (double)c.u_inlet[(size_t)gk * ny + gj];
-      }
+  // Faces of the tile: every thread takes the cells (li, lj, lk) of the tile that its index
+  // gives by shifts and masks (no runtime division); the last tile along an axis also writes
+  // the domain's closing face along it (the faces a tile writes: [c0, end), face_end).
+  constexpr int TX = S::TX, TY = S::TY, TZ = S::TZ;
+  constexpr int NCELL = TX * TY * TZ, NR = (NCELL + kTileThreads - 1) / kTileThreads;
+  const int ncx = ie - i0, ncy = je - j0, ncz = ke - k0;
+  auto u_face = [&](int li, int lj, int lk) {
+    const int gi = i0 + li, gj = j0 + lj, gk = k0 + lk;
+    const size_t idx = ((size_t)gk * ny + gj) * (nx + 1) + gi;
+    T lu;
+    const bool xb = !d.per[0] && (gi == 0 || gi == nx);
+    if (xb) lu = (gi == nx && d.outlet) ? (LU(lk, lj, li - 1) - LU(lk, lj, li)) * d.inv_h : (T)0;
+    else lu = rhs_u(d, L, lk, lj, li, nu_h2);
+    const T upv = LU(lk, lj, li);
+    T uh = (s == 0 ? upv : lerp_t(upv, c.u0[idx], a)) + lu * bdt;
+    if (xb && gi == 0) uh = d.outlet ? c.u_inlet[(size_t)gk * ny + gj] : (T)0;
+    if (xb && gi == nx && d.closed) uh = (T)0;
+    if (d.win && gk >= d.wk0 && gk < d.wk1 && gj >= d.wj0 && gj < d.wj1 && gi >= d.wi0 && gi <= d.wi1) {
+      const size_t wi = ((size_t)(gk - d.wk0) * d.wy + (gj - d.wj0)) * (d.wx + 1) + (gi - d.wi0);
+      const T un = (uh + kpen * c.a_u[wi]) / ((T)1 + kpen * c.chi_u[wi]);
+      const T du = un - uh;
+      d.acc_u[wi] = s == 0 ? du * ws : d.acc_u[wi] + du * ws;
+      uh = un;
     }
+    c.uo[idx] = uh;
+    if (gi == nx && d.outlet) {
+      osum += (double)uh;
+      isum += (double)c.u_inlet[(size_t)gk * ny + gj];
+    }
+  };
+  auto v_face = [&](int li, int lj, int lk) {
+    const int gi = i0 + li, gj = j0 + lj, gk = k0 + lk;
+    const size_t idx = ((size_t)gk * (ny + 1) + gj) * nx + gi;
+    const bool yb = !d.per[1] && (gj == 0 || gj == ny);
+    const T lv = yb ? (T)0 : rhs_v(d, L, lk, lj, li, nu_h2);
+    const T vpv = LV(lk, lj, li);
+    T vh = (s == 0 ? vpv : lerp_t(vpv, c.v0[idx], a)) + lv * bdt;
+    if (d.win && gk >= d.wk0 && gk < d.wk1 && gj >= d.wj0 && gj <= d.wj1 && gi >= d.wi0 && gi < d.wi1) {
+      const size_t wi = ((size_t)(gk - d.wk0) * (d.wy + 1) + (gj - d.wj0)) * d.wx + (gi - d.wi0);
+      const T vn = (vh + kpen * c.a_v[wi]) / ((T)1 + kpen * c.chi_v[wi]);
+      const T dv = vn - vh;
+      d.acc_v[wi] = s == 0 ? dv * ws : d.acc_v[wi] + dv * ws;
+      vh = vn;
+    }
+    c.vo[idx] = vh;
+  };
+  auto w_face = [&](int li, int lj, int lk) {
+    const int gi = i0 + li, gj = j0 + lj, gk = k0 + lk;
+    const size_t idx = ((size_t)gk * ny + gj) * nx + gi;
+    const bool zb = !d.per[2] && (gk == 0 || gk == nz);
+    const T lw = zb ? (T)0 : rhs_w(d, L, lk, lj, li, nu_h2);
+    const T wpv = LW(lk, lj, li);
+    T wh = (s == 0 ? wpv : lerp_t(wpv, c.w0[idx], a)) + lw * bdt;
+    const bool inw = d.win && gk >= d.wk0 && gk <= d.wk1 && gj >= d.wj0 && gj < d.wj1 && gi >= d.wi0 && gi < d.wi1;
+    const size_t wi = inw ? ((size_t)(gk - d.wk0) * d.wy + (gj - d.wj0)) * d.wx + (gi - d.wi0) : 0;
+    if (inw) {
+      const T wn = (wh + kpen * c.a_w[wi]) / ((T)1 + kpen * c.chi_w[wi]);
+      const T dw = wn - wh;
+      d.acc_w[wi] = s == 0 ? dw * ws : d.acc_w[wi] + dw * ws;
+      wh = wn;
+    }
+    if (d.buoy) {
+      // b dt Ri theta on the w-faces after the penalisation, everywhere; the solid's share
+      // (Ri theta_f vol_w) leaves the body force (physics.py:744-791)
+      const size_t pl = (size_t)ny * nx, col = (size_t)gj * nx + gi;
+      const int zn = nz - 1;
+      const T tk = c.thp[(size_t)(gk < nz ? gk : (d.per[2] ? 0 : zn)) * pl + col];
+      const T tm = c.thp[(size_t)(gk > 0 ? gk - 1 : (d.per[2] ? zn : 0)) * pl + col];
+      if (gk > 0 && gk < nz) wh = wh + ((T)0.5 * (tk + tm)) * (bdt * d.ri);
+      else if (d.per[2]) wh = wh + ((tk + tm) * d.half_ri) * bdt;
+      if (inw) d.acc_w[wi] = d.acc_w[wi] + (((T)0.5 * (tk + tm)) * c.vol_w[wi]) * (bdt * d.ri_w[s]);
+    }
+    c.wo[idx] = wh;
+  };
+  // one call site per face kind: slots 0 .. NR - 1 are the tile's cells, slot NR the closing face
+  const int tid = threadIdx.x;
+#pragma unroll 1
+  for (int r = 0; r <= NR; ++r) {
+    const int e = tid + kTileThreads * r;
+    int li = e % TX, lj = (e / TX) % TY, lk = e / (TX * TY);
+    bool ok = e < NCELL;
+    if (r == NR) {
+      li = ncx, lj = tid % TY, lk = tid / TY;
+      ok = i0 + TX >= nx && tid < TY * TZ;
+    }
+    if (ok && li <= ncx && lj < ncy && lk < ncz && (li < ncx || r == NR)) u_face(li, lj, lk);
   }
-  // v faces
-  {
-    const int fe = face_end(j0, S::TY, ny), nfx = ie - i0, nfy = fe - j0, nfz = ke - k0;
-    for (int e = threadIdx.x; e < nfx * nfy * nfz; e += kTileThreads) {
-      const int li = e % nfx, t = e / nfx, lj = t % nfy, lk = t / nfy;
-      const int gi = i0 + li, gj = j0 + lj, gk = k0 + lk;
-      const size_t idx = ((size_t)gk * (ny + 1) + gj) * nx + gi;
-      const bool yb = !d.per[1] && (gj == 0 || gj == ny);
-      const T lv = yb ? (T)0 : rhs_v(d, L, lk, lj, li, nu_h2);
-      const T vpv = LV(lk, lj, li);
-      T vh = (s == 0 ? vpv : lerp_t(vpv, c.v0[idx], a)) + lv * bdt;
-      if (d.win && gk >= d.wk0 && gk < d.wk1 && gj >= d.wj0 && gj <= d.wj1 && gi >= d.wi0 && gi < d.wi1) {
-        const size_t wi = ((size_t)(gk - d.wk0) * (d.wy + 1) + (gj - d.wj0)) * d.wx + (gi - d.wi0);
-        const T vn = (vh + kpen * c.a_v[wi]) / ((T)1 + kpen * c.chi_v[wi]);
-        const T dv = vn - vh;
-        d.acc_v[wi] = s == 0 ? dv * ws : d.acc_v[wi] + dv * ws;
-        vh = vn;
-      }
-      c.vo[idx] = vh;
+#pragma unroll 1
+  for (int r = 0; r <= NR; ++r) {
+    const int e = tid + kTileThreads * r;
+    int li = e % TX, lj = (e / TX) % TY, lk = e / (TX * TY);
+    bool ok = e < NCELL;
+    if (r == NR) {
+      li = tid % TX, lj = ncy, lk = tid / TX;
+      ok = j0 + TY >= ny && tid < TX * TZ;
     }
+    if (ok && li < ncx && lj <= ncy && lk < ncz && (lj < ncy || r == NR)) v_face(li, lj, lk);
   }
-  // w faces
-  {
-    const int fe = face_end(k0, S::TZ, nz), nfx = ie - i0, nfy = je - j0, nfz = fe - k0;
-    for (int e = threadIdx.x; e < nfx * nfy * nfz; e += kTileThreads) {
-      const int li = e % nfx, t = e / nfx, lj = t % nfy, lk = t / nfy;
-      const int gi = i0 + li, gj = j0 + lj, gk = k0 + lk;
-      const size_t idx = ((size_t)gk * ny + gj) * nx + gi;
-      const bool zb = !d.per[2] && (gk == 0 || gk == nz);
-      const T lw = zb ? (T)0 : rhs_w(d, L, lk, lj, li, nu_h2);
-      const T wpv = LW(lk, lj, li);
-      T wh = (s == 0 ? wpv : lerp_t(wpv, c.w0[idx], a)) + lw * bdt;
-      const bool inw = d.win && gk >= d.wk0 && gk <= d.wk1 && gj >= d.wj0 && gj < d.wj1 && gi >= d.wi0 && gi < d.wi1;
-      const size_t wi = inw ? ((size_t)(gk - d.wk0) * d.wy + (gj - d.wj0)) * d.wx + (gi - d.wi0) : 0;
-      if (inw) {
-        const T wn = (wh + kpen * c.a_w[wi]) / ((T)1 + kpen * c.chi_w[wi]);
-        const T dw = wn - wh;
-        d.acc_w[wi] = s == 0 ? dw * ws : d.acc_w[wi] + dw * ws;
-        wh = wn;
-      }
-      if (d.buoy) {
-        // b dt Ri theta on the w-faces after the penalisation, everywhere; the solid's share
-        // (Ri theta_f vol_w) leaves the body force (physics.py:744-791)
-        const size_t pl = (size_t)ny * nx, col = (size_t)gj * nx + gi;
-        const int zn = nz - 1;
-        const T tk = c.thp[(size_t)(gk < nz ? gk : (d.per[2] ? 0 : zn)) * pl + col];
-        const T tm = c.thp[(size_t)(gk > 0 ? gk - 1 : (d.per[2] ? zn : 0)) * pl + col];
-        if (gk > 0 && gk < nz) wh = wh + ((T)0.5 * (tk + tm)) * (bdt * d.ri);
-        else if (d.per[2]) wh = wh + ((tk + tm) * d.half_ri) * bdt;
-        if (inw) d.acc_w[wi] = d.acc_w[wi] + (((T)0.5 * (tk + tm)) * c.vol_w[wi]) * (bdt * d.ri_w[s]);
-      }
-      c.wo[idx] = wh;
+#pragma unroll 1
+  for (int r = 0; r <= NR; ++r) {
+    const int e = tid + kTileThreads * r;
+    int li = e % TX, lj = (e / TX) % TY, lk = e / (TX * TY);
+    bool ok = e < NCELL;
+    if (r == NR) {
+      li = tid % TX, lj = tid / TX, lk = ncz;
+      ok = k0 + TZ >= nz && tid < TX * TY;
     }
+    if (ok && li < ncx && lj < ncy && lk <= ncz && (lk < ncz || r == NR)) w_face(li, lj, lk);
   }
   // outlet plane sums of this tile (the outlet correction needs them all)
   if (d.outlet && i0 + S::TX >= nx) {

@@ -170,53 +170,11 @@ __device__ __forceinline__ T v_in(const Dev<T> &d, const T *__restrict__ v, cons
 }
 
 // ------------------------------------------------------------------ tile loads
-// The tile loaders first map every tile position to an in-domain address and a transform
-// (pure integer work), then issue all loads at once, then apply the transforms: the loads
-// of a thread are independent, so the block waits for one memory latency, not one per
-// element. The transforms reproduce the ghost rules of core.py:287-325 and scalar.py:42-60.
+// The tile loaders map every tile position to an in-domain address and a transform (the
+// ghost rules of core.py:287-325 and scalar.py:42-60), load, then apply the transform.
 
 // flag bits: 1 gradient correction, 2 outlet correction, 4 negate (mirror), 8/16/24 wall
 // kind slip/noslip/moving (u rows outside the domain)
-template <typename T>
-__device__ __forceinline__ void map_u(const Dev<T> &d, int j, int i, bool has_psi, int &idx, int &p1, int &p0,
-                                      int &flag) {
-  const int nx = d.nx, ny = d.ny;
-  flag = 0;
-  int jj, ii;
-  if (d.periodic) {
-    jj = clampi(wrap(j, ny), 0, ny - 1);
-    ii = clampi(wrap(i, nx), 0, nx);
-    const int ie = ii == nx ? 0 : ii;
-    p1 = jj * nx + ie;
-    p0 = jj * nx + wrap(ie - 1, nx);
-    if (has_psi) flag |= 1;
-  } else {
-    if (i < 0) {
-      if (d.closed) { ii = -i; flag |= 4; } else ii = 0;
-    } else if (i > nx) {
-      if (d.closed) { ii = 2 * nx - i; flag |= 4; } else ii = nx;
-    } else {
-      ii = i;
-    }
-    ii = clampi(ii, 0, nx);
-    if (j < 0) {
-      jj = clampi(-1 - j, 0, ny - 1);
-      flag |= 8 * (d.bcb + 1);
-    } else if (j >= ny) {
-      jj = clampi(2 * ny - 1 - j, 0, ny - 1);
-      flag |= 8 * (d.bct + 1);
-    } else {
-      jj = j;
-    }
-    p1 = jj * nx + (ii < nx ? ii : nx - 1);
-    p0 = ii > 0 ? p1 - 1 : p1;
-    if (has_psi) {
-      if (ii > 0 && ii < nx) flag |= 1;
-      else if (ii == nx && !d.closed) flag |= 2;
-    }
-  }
-  idx = jj * (nx + 1) + ii;
-}
 template <typename T>
 __device__ __forceinline__ T finish_u(const Dev<T> &d, T x, T q1, T q0, T corr, int flag, int is_bottom) {
   if (flag & 1) x = x - (q1 - q0) * d.inv_h;
@@ -228,86 +186,145 @@ __device__ __forceinline__ T finish_u(const Dev<T> &d, T x, T q1, T q0, T corr, 
   return x;
 }
 template <typename T>
-__device__ __forceinline__ void map_v(const Dev<T> &d, int j, int i, bool has_psi, int &idx, int &p1, int &p0,
-                                      int &flag) {
-  const int nx = d.nx, ny = d.ny;
-  flag = 0;
-  int jj, ii;
-  if (d.periodic) {
-    jj = clampi(wrap(j, ny), 0, ny);
-    ii = clampi(wrap(i, nx), 0, nx - 1);
-    const int je = jj == ny ? 0 : jj;
-    p1 = je * nx + ii;
-    p0 = wrap(je - 1, ny) * nx + ii;
-    if (has_psi) flag |= 1;
-  } else {
-    bool neg = false;
-    if (j < 0) { jj = -j; neg = !neg; } else if (j > ny) { jj = 2 * ny - j; neg = !neg; } else jj = j;
-    if (i < 0) {
-      ii = -1 - i;
-      if (!(d.closed && d.bcl != AERO_BC_NOSLIP)) neg = !neg;
-    } else if (i >= nx) {
-      if (d.closed) {
-        ii = 2 * nx - 1 - i;
-        if (d.bcr == AERO_BC_NOSLIP) neg = !neg;
-      } else {
-        ii = nx - 1;
-      }
-    } else {
-      ii = i;
-    }
-    jj = clampi(jj, 0, ny);
-    ii = clampi(ii, 0, nx - 1);
-    p1 = (jj < ny ? jj : ny - 1) * nx + ii;
-    p0 = jj > 0 ? p1 - nx : p1;
-    if (neg) flag |= 4;
-    if (has_psi && jj > 0 && jj < ny) flag |= 1;
-  }
-  idx = jj * nx + ii;
-}
-template <typename T>
 __device__ __forceinline__ T finish_v(const Dev<T> &d, T x, T q1, T q0, int flag) {
   if (flag & 1) x = x - (q1 - q0) * d.inv_h;
   if (flag & 4) x = -x;
   return x;
 }
 
-// U, V tiles of the stage-input velocity around the tile at (tj0, ti0).
+// U, V tiles of the stage-input velocity around the tile at (tj0, ti0). The ghost rules split
+// into a column part (inlet / outlet columns, closed-box mirrors) and a row part (wall rows,
+// periodic wrap): each lane works out the column part of its tile columns once (lane c and
+// c + 64), each wave the row part of its rows, and every element then costs its loads and
+// finish_u / finish_v.
 template <typename T, int R>
 __device__ void load_uv(const Dev<T> &d, Lds<T, R> &L, int tj0, int ti0, const T *u, const T *v, const T *psi, T corr) {
-  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  const int nx = d.nx, ny = d.ny;
   const bool hp = psi != nullptr;
-  int ui[TR<R>::NE], u1[TR<R>::NE], u0[TR<R>::NE], uf[TR<R>::NE], vi[TR<R>::NE], v1[TR<R>::NE], v0[TR<R>::NE], vf[TR<R>::NE], rr[TR<R>::NE];
+  const int lane = threadIdx.x, wave = threadIdx.y;
+  constexpr int NC = (HW + 63) / 64;   // 64-wide passes over a tile row
+  // column part, per lane and pass: u - source column, psi columns, flags (1 gradient, 2 outlet
+  // shift, 4 negate); v - source column, negate flag
+  int ucol[NC], upc1[NC], upc0[NC], uflag[NC], vcol[NC], vneg[NC];
 #pragma unroll
-  for (int q = 0; q < TR<R>::NE; ++q) {
-    const int e = tid + 256 * q;
-    const int ee = e < TR<R>::HH * HW ? e : 0;
-    const int r = ee / HW, c = ee - r * HW;
-    rr[q] = r;
-    map_u(d, tj0 - 2 + r, ti0 - 2 + c, hp, ui[q], u1[q], u0[q], uf[q]);
-    map_v(d, tj0 - 2 + r, ti0 - 2 + c, hp, vi[q], v1[q], v0[q], vf[q]);
-  }
-  T xu[TR<R>::NE], xv[TR<R>::NE], a1[TR<R>::NE], a0[TR<R>::NE], b1[TR<R>::NE], b0[TR<R>::NE];
-#pragma unroll
-  for (int q = 0; q < TR<R>::NE; ++q) {
-    xu[q] = u[ui[q]];
-    xv[q] = v[vi[q]];
-    if (hp) {
-      a1[q] = psi[u1[q]];
-      a0[q] = psi[u0[q]];
-      b1[q] = psi[v1[q]];
-      b0[q] = psi[v0[q]];
+  for (int q = 0; q < NC; ++q) {
+    const int i = ti0 - 2 + lane + 64 * q;
+    int ii, f = 0;
+    if (d.periodic) {
+      ii = clampi(wrap(i, nx), 0, nx);
+      const int ie = ii == nx ? 0 : ii;
+      upc1[q] = ie;
+      upc0[q] = wrap(ie - 1, nx);
+      if (hp) f |= 1;
     } else {
-      a1[q] = a0[q] = b1[q] = b0[q] = (T)0;
+      if (i < 0) {
+        if (d.closed) { ii = -i; f |= 4; } else ii = 0;
+      } else if (i > nx) {
+        if (d.closed) { ii = 2 * nx - i; f |= 4; } else ii = nx;
+      } else {
+        ii = i;
+      }
+      ii = clampi(ii, 0, nx);
+      upc1[q] = ii < nx ? ii : nx - 1;
+      upc0[q] = ii > 0 ? upc1[q] - 1 : upc1[q];
+      if (hp) {
+        if (ii > 0 && ii < nx) f |= 1;
+        else if (ii == nx && !d.closed) f |= 2;
+      }
+    }
+    ucol[q] = ii;
+    uflag[q] = f;
+    int vi;
+    bool neg = false;
+    if (d.periodic) {
+      vi = clampi(wrap(i, nx), 0, nx - 1);
+    } else {
+      if (i < 0) {
+        vi = -1 - i;
+        if (!(d.closed && d.bcl != AERO_BC_NOSLIP)) neg = !neg;
+      } else if (i >= nx) {
+        if (d.closed) {
+          vi = 2 * nx - 1 - i;
+          if (d.bcr == AERO_BC_NOSLIP) neg = !neg;
+        } else {
+          vi = nx - 1;
+        }
+      } else {
+        vi = i;
+      }
+      vi = clampi(vi, 0, nx - 1);
+    }
+    vcol[q] = vi;
+    vneg[q] = neg ? 1 : 0;
+  }
+  // all loads of this thread first (independent, one memory round trip), then the transforms
+  constexpr int NRW = (TR<R>::HH + 3) / 4;   // tile rows per wave
+  T xu[NRW][NC], xv[NRW][NC], a1[NRW][NC], a0[NRW][NC], b1[NRW][NC], b0[NRW][NC];
+  int uwall[NRW];
+  bool vgrad[NRW], vnr[NRW];
+#pragma unroll
+  for (int rr = 0; rr < NRW; ++rr) {
+    const int r = wave + 4 * rr;
+    const int j = tj0 - 2 + r;
+    // row part of u: source row, wall kind of ghost rows (flag bits 8 / 16 / 24)
+    int uj;
+    uwall[rr] = 0;
+    if (d.periodic) {
+      uj = clampi(wrap(j, ny), 0, ny - 1);
+    } else if (j < 0) {
+      uj = clampi(-1 - j, 0, ny - 1);
+      uwall[rr] = 8 * (d.bcb + 1);
+    } else if (j >= ny) {
+      uj = clampi(2 * ny - 1 - j, 0, ny - 1);
+      uwall[rr] = 8 * (d.bct + 1);
+    } else {
+      uj = j;
+    }
+    // row part of v: source row, psi rows, gradient flag, negation of mirrored rows
+    int vj, vp1, vp0;
+    vgrad[rr] = false;
+    vnr[rr] = false;
+    if (d.periodic) {
+      vj = clampi(wrap(j, ny), 0, ny);
+      const int je = vj == ny ? 0 : vj;
+      vp1 = je;
+      vp0 = wrap(je - 1, ny);
+      vgrad[rr] = hp;
+    } else {
+      if (j < 0) { vj = -j; vnr[rr] = true; } else if (j > ny) { vj = 2 * ny - j; vnr[rr] = true; } else vj = j;
+      vj = clampi(vj, 0, ny);
+      vp1 = vj < ny ? vj : ny - 1;
+      vp0 = vj > 0 ? vp1 - 1 : vp1;
+      vgrad[rr] = hp && vj > 0 && vj < ny;
+    }
+    const T *urow = u + uj * (nx + 1);
+    const T *vrow = v + vj * nx;
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+      const bool live = r < TR<R>::HH && lane + 64 * q < HW;
+      xu[rr][q] = live ? urow[ucol[q]] : (T)0;
+      xv[rr][q] = live ? vrow[vcol[q]] : (T)0;
+      a1[rr][q] = a0[rr][q] = b1[rr][q] = b0[rr][q] = (T)0;
+      if (hp && live) {
+        a1[rr][q] = psi[uj * nx + upc1[q]];
+        a0[rr][q] = psi[uj * nx + upc0[q]];
+        b1[rr][q] = psi[vp1 * nx + vcol[q]];
+        b0[rr][q] = psi[vp0 * nx + vcol[q]];
+      }
     }
   }
 #pragma unroll
-  for (int q = 0; q < TR<R>::NE; ++q) {
-    const int e = tid + 256 * q;
-    if (e < TR<R>::HH * HW) {
-      const int r = rr[q], c = e - r * HW;
-      L.u[r][c] = finish_u(d, xu[q], a1[q], a0[q], corr, uf[q], tj0 - 2 + r < 0);
-      L.v[r][c] = finish_v(d, xv[q], b1[q], b0[q], vf[q]);
+  for (int rr = 0; rr < NRW; ++rr) {
+    const int r = wave + 4 * rr;
+    if (r >= TR<R>::HH) continue;
+    const int j = tj0 - 2 + r;
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+      const int c = lane + 64 * q;
+      if (c >= HW) continue;
+      L.u[r][c] = finish_u(d, xu[rr][q], a1[rr][q], a0[rr][q], corr, uflag[q] | uwall[rr], j < 0);
+      const int vf = (vgrad[rr] ? 1 : 0) | ((vneg[q] != 0) != vnr[rr] ? 4 : 0);
+      L.v[r][c] = finish_v(d, xv[rr][q], b1[rr][q], b0[rr][q], vf);
     }
   }
 }
@@ -317,38 +334,43 @@ __device__ void load_uv(const Dev<T> &d, Lds<T, R> &L, int tj0, int ti0, const T
 template <typename T, int R>
 __device__ void load_c(const Dev<T> &d, Lds<T, R> &L, int tj0, int ti0, const T *c, const T *inflow_col, T inflow0,
                        int use_inflow) {
-  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
   const int nx = d.nx, ny = d.ny;
-  int ci[TR<R>::NE], rr[TR<R>::NE], jr[TR<R>::NE];
-  bool inf[TR<R>::NE];
+  const int lane = threadIdx.x, wave = threadIdx.y;
+  constexpr int NC = (HW + 63) / 64;
+  int col[NC];
+  bool inf[NC];
 #pragma unroll
-  for (int q = 0; q < TR<R>::NE; ++q) {
-    const int e = tid + 256 * q;
-    const int ee = e < TR<R>::HH * HW ? e : 0;
-    const int r = ee / HW, cc = ee - r * HW;
-    const int j = tj0 - 2 + r, i = ti0 - 2 + cc;
-    rr[q] = r;
-    jr[q] = clampi(j, 0, ny - 1);
+  for (int q = 0; q < NC; ++q) {
+    const int i = ti0 - 2 + lane + 64 * q;
     if (d.periodic) {
-      ci[q] = clampi(wrap(j, ny), 0, ny - 1) * nx + clampi(wrap(i, nx), 0, nx - 1);
+      col[q] = clampi(wrap(i, nx), 0, nx - 1);
       inf[q] = false;
     } else {
-      ci[q] = jr[q] * nx + clampi(i, 0, nx - 1);
+      col[q] = clampi(i, 0, nx - 1);
       inf[q] = i < 0 && (use_inflow == 2 || (use_inflow == 1 && !d.closed));
     }
   }
-  T x[TR<R>::NE], fl[TR<R>::NE];
+  constexpr int NRW = (TR<R>::HH + 3) / 4;
+  T x[NRW][NC], fl[NRW];
 #pragma unroll
-  for (int q = 0; q < TR<R>::NE; ++q) {
-    x[q] = c[ci[q]];
-    fl[q] = inflow_col ? inflow_col[jr[q]] : inflow0;
+  for (int rr = 0; rr < NRW; ++rr) {
+    const int r = wave + 4 * rr;
+    const int j = tj0 - 2 + r;
+    const int jr = clampi(j, 0, ny - 1);
+    const int row = d.periodic ? clampi(wrap(j, ny), 0, ny - 1) : jr;
+    fl[rr] = inflow_col ? inflow_col[jr] : inflow0;
+    const T *crow = c + row * nx;
+#pragma unroll
+    for (int q = 0; q < NC; ++q) x[rr][q] = (r < TR<R>::HH && lane + 64 * q < HW) ? crow[col[q]] : (T)0;
   }
 #pragma unroll
-  for (int q = 0; q < TR<R>::NE; ++q) {
-    const int e = tid + 256 * q;
-    if (e < TR<R>::HH * HW) {
-      const int r = rr[q], cc = e - r * HW;
-      L.c[r][cc] = inf[q] ? fl[q] : x[q];
+  for (int rr = 0; rr < NRW; ++rr) {
+    const int r = wave + 4 * rr;
+    if (r >= TR<R>::HH) continue;
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+      const int cc = lane + 64 * q;
+      if (cc < HW) L.c[r][cc] = inf[q] ? fl[rr] : x[rr][q];
     }
   }
 }

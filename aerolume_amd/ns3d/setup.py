@@ -469,46 +469,44 @@ def build_geometry3d(bodies, ops: Ops3D, *, window=None) -> Geometry3D:
 
 def default_dye_rake(bounds, extent, h: float, *, kind: str = "rake", symmetry: bool = False, ground: bool = False,
                      n: int | None = None) -> DyeRake:
-    """Nozzles placed from the bodies' bounding box as the reference places them
-    (``ns3d/dye.py:84-150``): a release plane 0.4 of the body's smaller size upstream of the
-    nose (at least four cells from the inlet); "rake": one row across the span at 40 % of the
-    body height, 20 % wider than the body on each side, ~6 nozzle radii apart; "grid": the
-    frontal box with 10 % margins; "plane": a band over it. With a symmetry plane only the
-    solved side is kept; on a road the lowest nozzle stays two cells above it."""
-    bx0, bx1, by0, by1, bz0, bz1 = (float(b) for b in bounds)
-    ex0, _, ey0, ey1, ez0, ez1 = (float(e) for e in extent)
-    wy, hz = max(by1 - by0, h), max(bz1 - bz0, h)
-    size = max(wy, hz)
+    """Smoke nozzles placed from the bodies' bounding box (the reference's placement,
+    ``ns3d/dye.py:84-150``). Release plane: 0.4 of the body's smaller size (frontal or
+    streamwise) upstream of the nose, kept four cells off the inlet and two off the nose.
+    Cross-stream: the body's span widened by 20 % each side and its height shifted up by
+    10 %, both kept a nozzle radius and a cell inside the domain (on a road, two cells above
+    it). "rake": one row across the span at 40 % of the body height; "grid": span x height;
+    "plane": a band between two corners; nozzles about six radii apart. With a symmetry plane
+    only the solved side's nozzles are kept."""
+    (bx0, bx1, by0, by1, bz0, bz1), (ex0, _, ey0, ey1, ez0, ez1) = ([float(v) for v in q] for q in (bounds, extent))
+    span, height = max(by1 - by0, h), max(bz1 - bz0, h)
+    frontal = max(span, height)
     radius = NOZZLE_RADIUS_CELLS * h
-    x = bx0 - max(0.4 * min(size, max(bx1 - bx0, h)), 4.0 * h)
-    x = min(max(x, ex0 + 4.0 * h), bx0 - 2.0 * h)
-    ya, yb = by0 - 0.2 * wy, by1 + 0.2 * wy
-    ya = max(ya, 2.0 * ey0 - (ey1 - radius - h)) if symmetry else max(ya, ey0 + radius + h)
-    yb = min(yb, ey1 - radius - h)
-    za, zb = bz0 + 0.1 * hz, bz1 + 0.1 * hz
+    inward = lambda e: e - radius - h  # noqa: E731   (a nozzle's extent plus a cell inside a far side)
+    outward = lambda e: e + radius + h  # noqa: E731
+    upstream = max(0.4 * min(frontal, max(bx1 - bx0, h)), 4.0 * h)
+    x = min(max(bx0 - upstream, ex0 + 4.0 * h), bx0 - 2.0 * h)
+    # the mirrored body of a half-domain run spans both sides of the plane, like a full run
+    y_floor = 2.0 * ey0 - inward(ey1) if symmetry else outward(ey0)
+    y_lo, y_hi = max(by0 - 0.2 * span, y_floor), min(by1 + 0.2 * span, inward(ey1))
+    z_lo, z_hi = bz0 + 0.1 * height, bz1 + 0.1 * height
     if ground:
-        za = max(za, ez0 + 2.0 * h + radius)
-    za, zb = max(za, ez0 + radius + h), min(zb, ez1 - radius - h)
-
-    def solved(ys):
-        return ys[ys >= ey0 - 1e-9] if symmetry else ys
-
+        z_lo = max(z_lo, ez0 + 2.0 * h + radius)
+    z_lo, z_hi = max(z_lo, outward(ez0)), min(z_hi, inward(ez1))
     if kind == "plane":
-        return DyeRake("plane", [(x, max(ya, ey0) if symmetry else ya, za), (x, yb, zb)], radius=radius)
-    gap = max(6.0 * radius, size / 7.0)
+        return DyeRake("plane", [(x, max(y_lo, ey0) if symmetry else y_lo, z_lo), (x, y_hi, z_hi)], radius=radius)
+    gap = max(6.0 * radius, frontal / 7.0)
+    count = lambda extent_, lo, hi: max(lo, min(hi, round(extent_ / gap) + 1))  # noqa: E731
+    kept = lambda ys: ys[ys >= ey0 - 1e-9] if symmetry else ys  # noqa: E731
     if kind == "rake":
-        k = int(n) if n else max(3, min(17, round((yb - ya) / gap) + 1))
-        ys = solved(np.linspace(ya, yb, k))
-        z = bz0 + 0.40 * hz
-        z = min(max(z, ez0 + radius + 2 * h), ez1 - radius - h)
+        ys = kept(np.linspace(y_lo, y_hi, int(n) if n else count(y_hi - y_lo, 3, 17)))
+        z = min(max(bz0 + 0.40 * height, ez0 + radius + 2 * h), inward(ez1))
         return DyeRake("rake", [(x, y, z) for y in ys], radius=radius)
-    ky = max(3, min(11, round((yb - ya) / gap) + 1))
-    kz = max(2, min(9, round((zb - za) / gap) + 1))
     if n:
-        ky = max(2, round(math.sqrt(n * (yb - ya) / max(zb - za, 1e-9))))
+        ky = max(2, round(math.sqrt(n * (y_hi - y_lo) / max(z_hi - z_lo, 1e-9))))
         kz = max(2, round(n / ky))
-    ys = solved(np.linspace(ya, yb, ky))
-    zs = np.linspace(za, zb, kz)
+    else:
+        ky, kz = count(y_hi - y_lo, 3, 11), count(z_hi - z_lo, 2, 9)
+    ys, zs = kept(np.linspace(y_lo, y_hi, ky)), np.linspace(z_lo, z_hi, kz)
     return DyeRake("grid", [(x, y, z) for z in zs for y in ys], radius=radius)
 
 

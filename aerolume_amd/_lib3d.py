@@ -52,6 +52,15 @@ class Outputs3D(ctypes.Structure):
                 ("diag_count", vp), ("diag_cap", ctypes.c_int64)]
 
 
+#: per-stage arrays of a traced step (aero3d_trace), three stage pointers each
+TRACE_FIELDS = ("tend_u", "tend_v", "tend_w", "star_u", "star_v", "star_w", "pen_u", "pen_v", "pen_w", "div", "psi",
+                "proj_u", "proj_v", "proj_w", "theta")
+
+
+class Trace3D(ctypes.Structure):
+    _fields_ = [(k, vp * 3) for k in TRACE_FIELDS]
+
+
 #: columns of a diagnostics row (AERO3D_DIAG_COLS)
 DIAG_COLS = 28
 
@@ -65,12 +74,13 @@ def bind(lib) -> None:
     lib.aero3d_step.argtypes = [P, P, ctypes.POINTER(Fields3D), ctypes.POINTER(Fields3D), ctypes.POINTER(Geom3D),
                                 ctypes.POINTER(Params3D), ctypes.POINTER(Outputs3D)]
     lib.aero3d_step_chained.argtypes = lib.aero3d_step.argtypes
+    lib.aero3d_step_trace.argtypes = lib.aero3d_step.argtypes + [ctypes.POINTER(Trace3D)]
     lib.aero3d_project.argtypes = [P, P, P, P, P, P]
     lib.aero3d_poisson_solve.argtypes = [P, P, P, P]
     lib.aero3d_cell_gradient.argtypes = [P, P, P, P, P, P, ctypes.c_double]
     lib.aero3d_time_stage.argtypes = [P, P, ctypes.POINTER(Fields3D), ctypes.POINTER(Geom3D),
                                       ctypes.POINTER(Params3D), ctypes.c_int, ctypes.c_int,
                                       ctypes.POINTER(ctypes.c_double)]
-    for name in ("aero3d_plan_create", "aero3d_plan_destroy", "aero3d_step", "aero3d_step_chained",
+    for name in ("aero3d_plan_create", "aero3d_plan_destroy", "aero3d_step", "aero3d_step_chained", "aero3d_step_trace",
                  "aero3d_project", "aero3d_poisson_solve", "aero3d_time_stage", "aero3d_cell_gradient"):
         getattr(lib, name).restype = ctypes.c_int

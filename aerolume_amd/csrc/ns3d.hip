@@ -140,6 +140,7 @@ struct Call3 {
   int nu_in;                 // stage input is the state the final kernel last saw: nu_t from d.nu_pad
   const T *ctrl_in;          // PID step: controller memory [dt_prev, e1, e2] of the step start
   T *ctrl_out;               // and of the step's end
+  T *tr_l[3], *tr_s[3];      // traced stage: tendencies and values before penalisation (u, v, w)
 };
 
 template <typename T>
@@ -613,7 +614,9 @@ __device__ __forceinline__ int face_end(int c0, int t, int n) { return c0 + t >=
 // ------------------------------------------------------------------ stage kernel
 // NU_IN: nu_t read from d.nu_pad (the first stage of a step whose start the final kernel
 // processed) instead of computed; a separate instantiation keeps the common path's registers.
-template <typename T, bool NU_IN>
+// TRACE: also store each face's tendency and its value before the penalisation (c.tr_l, c.tr_s;
+// null pointers skipped), for the traced step only.
+template <typename T, bool NU_IN, bool TRACE = false>
 __global__ void __launch_bounds__(kTileThreads, Tile<T>::WAVES_PER_SIMD) k_stage(const Dev3<T> *__restrict__ dp, Call3<T> c) {
   using S = Lds3<T>;
   extern __shared__ unsigned char smem[];
@@ -655,6 +658,10 @@ __global__ void __launch_bounds__(kTileThreads, Tile<T>::WAVES_PER_SIMD) k_stage
     T uh = (s == 0 ? upv : lerp_t(upv, c.u0[idx], a)) + lu * bdt;
     if (xb && gi == 0) uh = d.outlet ? c.u_inlet[(size_t)gk * ny + gj] : (T)0;
     if (xb && gi == nx && d.closed) uh = (T)0;
+    if constexpr (TRACE) {
+      if (c.tr_l[0]) c.tr_l[0][idx] = lu;
+      if (c.tr_s[0]) c.tr_s[0][idx] = uh;
+    }
     if (d.win && gk >= d.wk0 && gk < d.wk1 && gj >= d.wj0 && gj < d.wj1 && gi >= d.wi0 && gi <= d.wi1) {
       const size_t wi = ((size_t)(gk - d.wk0) * d.wy + (gj - d.wj0)) * (d.wx + 1) + (gi - d.wi0);
       const T un = (uh + kpen * c.a_u[wi]) / ((T)1 + kpen * c.chi_u[wi]);
@@ -675,6 +682,10 @@ __global__ void __launch_bounds__(kTileThreads, Tile<T>::WAVES_PER_SIMD) k_stage
     const T lv = yb ? (T)0 : rhs_v(d, L, lk, lj, li, nu_h2);
     const T vpv = LV(lk, lj, li);
     T vh = (s == 0 ? vpv : lerp_t(vpv, c.v0[idx], a)) + lv * bdt;
+    if constexpr (TRACE) {
+      if (c.tr_l[1]) c.tr_l[1][idx] = lv;
+      if (c.tr_s[1]) c.tr_s[1][idx] = vh;
+    }
     if (d.win && gk >= d.wk0 && gk < d.wk1 && gj >= d.wj0 && gj <= d.wj1 && gi >= d.wi0 && gi < d.wi1) {
       const size_t wi = ((size_t)(gk - d.wk0) * (d.wy + 1) + (gj - d.wj0)) * d.wx + (gi - d.wi0);
       const T vn = (vh + kpen * c.a_v[wi]) / ((T)1 + kpen * c.chi_v[wi]);
@@ -691,6 +702,10 @@ __global__ void __launch_bounds__(kTileThreads, Tile<T>::WAVES_PER_SIMD) k_stage
     const T lw = zb ? (T)0 : rhs_w(d, L, lk, lj, li, nu_h2);
     const T wpv = LW(lk, lj, li);
     T wh = (s == 0 ? wpv : lerp_t(wpv, c.w0[idx], a)) + lw * bdt;
+    if constexpr (TRACE) {
+      if (c.tr_l[2]) c.tr_l[2][idx] = lw;
+      if (c.tr_s[2]) c.tr_s[2][idx] = wh;
+    }
     const bool inw = d.win && gk >= d.wk0 && gk <= d.wk1 && gj >= d.wj0 && gj < d.wj1 && gi >= d.wi0 && gi < d.wi1;
     const size_t wi = inw ? ((size_t)(gk - d.wk0) * d.wy + (gj - d.wj0)) * d.wx + (gi - d.wi0) : 0;
     if (inw) {
@@ -1897,6 +1912,69 @@ __global__ void k_correct(const Dev3<T> *__restrict__ dp, T *u, T *v, T *w, cons
   }
 }
 
+// Traced stage (core.py:793-862 trace records), after the stage's potential is solved: the
+// outlet correction of the stage added on the outlet face of the value before penalisation and
+// of the penalised value, the projection's right-hand side and the projected velocity as the
+// next stage loads it (u^ - grad psi on interior and periodic faces, the outlet shift on the
+// outlet face). Null outputs are skipped.
+template <typename T>
+__global__ void k_trace3(const Dev3<T> *__restrict__ dp, int stage, const T *__restrict__ uh, const T *__restrict__ vh,
+                         const T *__restrict__ wh, const T *__restrict__ psi, T *star_u, T *pen_u, T *pen_v,
+                         T *pen_w, T *div, T *pu, T *pv, T *pw) {
+  const Dev3<T> &d = *dp;
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  const T corr = d.outlet ? d.red->corr[stage] : (T)0;
+  const size_t nu = (size_t)nz * ny * (nx + 1), nv = (size_t)nz * (ny + 1) * nx, nw = (size_t)(nz + 1) * ny * nx;
+  const size_t pl = (size_t)ny * nx, nc = pl * nz;
+  const T inv_h = d.inv_h;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nu + nv + nw + nc;
+       e += (size_t)gridDim.x * blockDim.x) {
+    if (e < nu) {
+      const int i = (int)(e % (nx + 1));
+      const size_t r = e / (nx + 1);
+      const bool out_face = d.outlet && i == nx;
+      T val = uh[e];
+      if (out_face) {
+        val = val + corr;
+        if (star_u) star_u[e] = star_u[e] + corr;
+      }
+      if (pen_u) pen_u[e] = val;
+      if (d.per[0] || (i > 0 && i < nx)) {
+        const int fc = i == nx ? 0 : i, fm = fc == 0 ? nx - 1 : fc - 1;
+        val = val - (psi[r * nx + fc] - psi[r * nx + fm]) * inv_h;
+      }
+      if (pu) pu[e] = val;
+    } else if (e < nu + nv) {
+      const size_t f = e - nu;
+      const int i = (int)(f % nx), j = (int)((f / nx) % (ny + 1));
+      const size_t k = f / ((size_t)nx * (ny + 1));
+      T val = vh[f];
+      if (pen_v) pen_v[f] = val;
+      if (d.per[1] || (j > 0 && j < ny)) {
+        const int fc = j == ny ? 0 : j, fm = fc == 0 ? ny - 1 : fc - 1;
+        const T *pb = psi + k * pl + i;
+        val = val - (pb[(size_t)fc * nx] - pb[(size_t)fm * nx]) * inv_h;
+      }
+      if (pv) pv[f] = val;
+    } else if (e < nu + nv + nw) {
+      const size_t f = e - nu - nv;
+      const int k = (int)(f / pl);
+      const size_t r = f % pl;
+      T val = wh[f];
+      if (pen_w) pen_w[f] = val;
+      if (d.per[2] || (k > 0 && k < nz)) {
+        const int fc = k == nz ? 0 : k, fm = fc == 0 ? nz - 1 : fc - 1;
+        val = val - (psi[(size_t)fc * pl + r] - psi[(size_t)fm * pl + r]) * inv_h;
+      }
+      if (pw) pw[f] = val;
+    } else if (div) {
+      const size_t q = e - nu - nv - nw;
+      const int k = (int)(q / pl), j = (int)((q % pl) / nx), i = (int)(q % nx);
+      div[q] = div_at(d, uh, vh, wh, k, j, i, corr);
+    }
+  }
+}
+
 }  // namespace ns3d
 }  // namespace aero
 
@@ -2215,7 +2293,7 @@ int poisson_passes(aero3d_plan *p, hipStream_t st, const Dev3<T> *dev, Call3<T> 
 
 template <typename T>
 int step_impl(aero3d_plan *p, hipStream_t st, const aero3d_fields *in, aero3d_fields *out, const aero3d_geom *g,
-              const aero3d_params *pr, aero3d_outputs *o, bool chained) {
+              const aero3d_params *pr, aero3d_outputs *o, bool chained, const aero3d_trace *tr = nullptr) {
   const aero3d_desc &s = p->desc;
   const Dev3<T> *dev = (const Dev3<T> *)p->dev;
   Call3<T> c = make_call<T>(in, g, pr);
@@ -2258,12 +2336,38 @@ int step_impl(aero3d_plan *p, hipStream_t st, const aero3d_fields *in, aero3d_fi
       c.thp = stage == 0 ? (const T *)in->theta : (stage == 1 ? (const T *)p->th_a : (const T *)p->th_b);
       c.tho = stage == 0 ? (T *)p->th_a : (stage == 1 ? (T *)p->th_b : (T *)out->theta);
     }
-    if (c.nu_in) hipLaunchKernelGGL((k_stage<T, true>), grid, blk, lds, st, dev, c);
-    else hipLaunchKernelGGL((k_stage<T, false>), grid, blk, lds, st, dev, c);
+    if (tr) {
+      // the traced step forms the step start's nu_t in the stage kernel (the same values)
+      c.nu_in = 0;
+      for (int q = 0; q < 3; ++q) {
+        c.tr_l[q] = (T *)(q == 0 ? tr->tend_u : (q == 1 ? tr->tend_v : tr->tend_w))[stage];
+        c.tr_s[q] = (T *)(q == 0 ? tr->star_u : (q == 1 ? tr->star_v : tr->star_w))[stage];
+      }
+      hipLaunchKernelGGL((k_stage<T, false, true>), grid, blk, lds, st, dev, c);
+    } else if (c.nu_in) {
+      hipLaunchKernelGGL((k_stage<T, true>), grid, blk, lds, st, dev, c);
+    } else {
+      hipLaunchKernelGGL((k_stage<T, false>), grid, blk, lds, st, dev, c);
+    }
     if (s.thermal) hipLaunchKernelGGL(k_theta<T>, grid, blk, sizeof(LdsT<T>), st, dev, c);
     const int mode = (!s.periodic_x && !s.closed) ? 2 : 1;
     int rc = poisson_passes<T>(p, st, dev, c, nullptr, mode, stage, (T *)p->psi, (T *)out->p);
     if (rc) return rc;
+    if (tr) {
+      const size_t nf = (size_t)s.nz * s.ny * (s.nx + 1) + (size_t)s.nz * (s.ny + 1) * s.nx +
+                        (size_t)(s.nz + 1) * s.ny * s.nx + ncell;
+      const int nb = (int)std::min<size_t>(8192, (nf + 255) / 256);
+      hipLaunchKernelGGL(k_trace3<T>, dim3(nb), dim3(256), 0, st, dev, stage, (const T *)c.uo, (const T *)c.vo,
+                         (const T *)c.wo, (const T *)p->psi, (T *)tr->star_u[stage], (T *)tr->pen_u[stage],
+                         (T *)tr->pen_v[stage], (T *)tr->pen_w[stage], (T *)tr->div[stage], (T *)tr->proj_u[stage],
+                         (T *)tr->proj_v[stage], (T *)tr->proj_w[stage]);
+      const int cb = (int)std::min<size_t>(8192, (ncell + 255) / 256);
+      if (tr->psi[stage])
+        hipLaunchKernelGGL(k_copy3<T>, dim3(cb), dim3(256), 0, st, (T *)tr->psi[stage], (const T *)p->psi, ncell);
+      if (tr->theta[stage] && s.thermal)
+        hipLaunchKernelGGL(k_copy3<T>, dim3(cb), dim3(256), 0, st, (T *)tr->theta[stage], (const T *)c.tho, ncell);
+      for (int q = 0; q < 3; ++q) c.tr_l[q] = c.tr_s[q] = nullptr;
+    }
   }
   c.stage = 3;
   c.nu_in = 0;
@@ -2616,6 +2720,16 @@ int aero3d_step(aero3d_plan *p, void *stream, const aero3d_fields *in, aero3d_fi
   if (rc) return rc;
   hipStream_t st = (hipStream_t)stream;
   return p->dbl ? step_impl<double>(p, st, in, out, g, pr, o, false) : step_impl<float>(p, st, in, out, g, pr, o, false);
+}
+
+int aero3d_step_trace(aero3d_plan *p, void *stream, const aero3d_fields *in, aero3d_fields *out,
+                      const aero3d_geom *g, const aero3d_params *pr, aero3d_outputs *o, const aero3d_trace *tr) {
+  if (!tr) { set_error("aero3d_step_trace: null trace"); return AERO_EINVAL; }
+  int rc = check3_step(p, in, out, g, pr, o);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  return p->dbl ? step_impl<double>(p, st, in, out, g, pr, o, false, tr)
+                : step_impl<float>(p, st, in, out, g, pr, o, false, tr);
 }
 
 int aero3d_step_chained(aero3d_plan *p, void *stream, const aero3d_fields *in, aero3d_fields *out,

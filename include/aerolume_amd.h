@@ -325,6 +325,25 @@ int aero3d_step(aero3d_plan *plan, void *stream, const aero3d_fields *in, aero3d
  * plan computed from its output state; valid only when `in` is that output, unmodified. */
 int aero3d_step_chained(aero3d_plan *plan, void *stream, const aero3d_fields *in, aero3d_fields *out,
                         const aero3d_geom *geom, const aero3d_params *params, aero3d_outputs *outputs);
+/* Stage fields of one traced step (reference FlowSolver3D.trace / trace_step,
+ * ns3d/solver.py:832-919, recorded as core.py:822-862 records them), per RK stage, staggered
+ * like the state; null pointers are skipped. tend: the explicit tendency (boundary rules
+ * applied); star: the stage value before penalisation (inlet held, outlet balanced);
+ * pen: after the penalisation and the buoyancy impulse; div: the projection's right-hand side
+ * (nz, ny, nx); psi: the potential; proj: the projected velocity; theta: the stage's temperature. */
+typedef struct {
+  void *tend_u[3], *tend_v[3], *tend_w[3];
+  void *star_u[3], *star_v[3], *star_w[3];
+  void *pen_u[3], *pen_v[3], *pen_w[3];
+  void *div[3], *psi[3];
+  void *proj_u[3], *proj_v[3], *proj_w[3];
+  void *theta[3];
+} aero3d_trace;
+/* aero3d_step that also fills `trace` (a debugging step: unchained, extra stores and copies
+ * per stage; its state, outputs and history row are those of aero3d_step). */
+int aero3d_step_trace(aero3d_plan *plan, void *stream, const aero3d_fields *in, aero3d_fields *out,
+                      const aero3d_geom *geom, const aero3d_params *params, aero3d_outputs *outputs,
+                      const aero3d_trace *trace);
 /* u, v, w in place: u - grad psi with lap psi = div u; psi (nz, ny, nx) optional. */
 int aero3d_project(aero3d_plan *plan, void *stream, void *u, void *v, void *w, void *psi);
 /* psi = L^-1 f with zero mean (f, psi (nz, ny, nx); may alias). */

@@ -501,10 +501,30 @@ def force_coefficients(acc, geom, ops, dt):
                         mz / (q_a * ops.span_ref)], dim=1)
 
 
-def step3d_full(state: State3D, geom, ops, dt=None, nut_field=None, nut_trust=None, pid=None):
+def _trace_start(trace, s, dt, tendency, star):
+    """A stage record (core.py:822-826): the explicit tendency and the value before penalisation."""
+    if trace is None:
+        return None
+    return {"stage": s, "dt": dt.clone(), "tendency": tuple(x.clone() for x in tendency),
+            "u_star": tuple(x.clone() for x in star)}
+
+
+def _trace_end(trace, rec, penalised, rhs, psi, projected, theta=None):
+    """The rest of a stage record (core.py:853-862), appended to ``trace``."""
+    if rec is None:
+        return
+    rec.update(u_penalised=tuple(x.clone() for x in penalised), rhs=rhs.clone(), psi=psi.clone(),
+               u_projected=tuple(x.clone() for x in projected))
+    if theta is not None:
+        rec["theta"] = theta.clone()
+    trace.append(rec)
+
+
+def step3d_full(state: State3D, geom, ops, dt=None, nut_field=None, nut_trust=None, pid=None, trace=None):
     """One SSP-RK3 step without placed physics (core.py:765-883); ``nut_field`` / ``nut_trust``:
     the learned closure's eddy viscosity, held over the step (no reference counterpart);
-    ``pid``: a :class:`PidDt3` applied after the CFL rule (physics.py:1230-1262 without heat)."""
+    ``pid``: a :class:`PidDt3` applied after the CFL rule (physics.py:1230-1262 without heat).
+    ``trace`` (a list) receives one record per stage as core.py:822-862 forms them."""
     u0, v0, w0 = state.u, state.v, state.w
     up, vp, wp = u0, v0, w0
     acc = None
@@ -532,6 +552,7 @@ def step3d_full(state: State3D, geom, ops, dt=None, nut_field=None, nut_trust=No
             wh = torch.addcmul(torch.lerp(wp, w0, a), lw, bdt)
         if not px:
             uh = inlet_outlet(uh, ops)
+        rec = _trace_start(trace, s, dt, (lu, lv, lw), (uh, vh, wh))
         if not geom.empty:
             uh, vh, wh, du, dv, dw = penalise3d(uh, vh, wh, geom, ops, bdt)
             acc = (du * wgt, dv * wgt, dw * wgt) if acc is None else (
@@ -539,6 +560,7 @@ def step3d_full(state: State3D, geom, ops, dt=None, nut_field=None, nut_trust=No
             if any_periodic:
                 uh, vh, wh = sync_periodic(uh, vh, wh, ops)
         up, vp, wp, psi = project3d(uh, vh, wh, ops)
+        _trace_end(trace, rec, (uh, vh, wh), divergence3d(uh, vh, wh, ops.h), psi, (up, vp, wp))
         psi_w = psi * wgt if psi_w is None else torch.add(psi_w, psi, alpha=wgt)
     p = psi_w * (1.0 / dt)
     if geom.empty:
@@ -927,12 +949,13 @@ def wall_heat_rates(th, theta, u, ops):
     return out
 
 
-def step3d_phys(state: State3D, geom, ops, th, theta, dt=None, out=None, pid=None):
+def step3d_phys(state: State3D, geom, ops, th, theta, dt=None, out=None, pid=None, trace=None):
     """One SSP-RK3 step with heat transfer (core.py:765-883 with ``phys`` holding a ThermalOps):
     returns (State3D, coefficients, new theta); ``th.heat_rate`` holds the bodies' heat rates.
     ``out`` (a dict) receives ``div_pre``, the squared norm of the last stage's projection
     right-hand side (Physics3D.record_divergence, physics.py:1303-1306). ``pid``: a
-    :class:`PidDt3` applied after the dt rule (physics.py:1260-1261)."""
+    :class:`PidDt3` applied after the dt rule (physics.py:1260-1261). ``trace`` (a list)
+    receives one record per stage as core.py:822-862 forms them, with the stage's theta."""
     u0, v0, w0 = state.u, state.v, state.w
     up, vp, wp = u0, v0, w0
     acc = None
@@ -965,6 +988,7 @@ def step3d_phys(state: State3D, geom, ops, th, theta, dt=None, out=None, pid=Non
             wh = torch.addcmul(torch.lerp(wp, w0, a), lw, bdt)
         if not px:
             uh = inlet_outlet(uh, ops)
+        rec = _trace_start(trace, s, dt, (lu, lv, lw), (uh, vh, wh))
         if not geom.empty:
             uh, vh, wh, du, dv, dw = penalise3d(uh, vh, wh, geom, ops, bdt)
             acc = (du * wgt, dv * wgt, dw * wgt) if acc is None else (
@@ -981,6 +1005,7 @@ def step3d_phys(state: State3D, geom, ops, th, theta, dt=None, out=None, pid=Non
         psi = poisson(ops).solve(div)
         gx, gy, gz = pressure_gradient(psi, ops)
         up, vp, wp = uh - gx, vh - gy, wh - gz
+        _trace_end(trace, rec, (uh, vh, wh), div, psi, (up, vp, wp), tp_new)
         tp = tp_new
         psi_w = psi * wgt if psi_w is None else torch.add(psi_w, psi, alpha=wgt)
     p = psi_w * (1.0 / dt)

@@ -36,7 +36,7 @@ class Desc2D(ctypes.Structure):
                 ("dye_mode", i32), ("n_dye", i32), ("n_groups", i32), ("group_channel", i32 * MAX_GROUPS),
                 ("h", f64), ("ground_speed", f64), ("cfl", f64), ("force_ref", f64), ("les_c", f64),
                 ("gravity_x", f64), ("gravity_y", f64), ("prt", f64), ("diff_coef", f64),
-                ("tile_rows", i32), ("pid_dt", i32), ("diagnostics", i32), ("reserved", i32)]
+                ("tile_rows", i32), ("pid_dt", i32), ("diagnostics", i32), ("wall_model", i32)]
 
 
 class Fields2D(ctypes.Structure):

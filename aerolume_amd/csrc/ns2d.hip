@@ -78,6 +78,7 @@ struct Dev {
   int dye_mode, n_dye, n_groups;
   int gch[AERO2D_MAX_GROUPS];
   int fixed_dt;
+  int wallm;         // Werner-Wengle wall shear on
   int pid;           // adaptive_dt = "pid"
   const T *dt_in;    // the step-start state's dt and controller memory (PID step)
   const T *ctrl_in;
@@ -87,6 +88,7 @@ struct Dev {
   // double-built constants of the per-cell paths, rounded once (the values the kernels formed
   // in double and cast): 1/Pr_t, 2/h, Cs^2 h^2 (Smagorinsky, WALE), Cs h^2 (Vreman)
   T inv_prt, two_h, les_k2, les_kv;
+  T wall_y;          // distance of the first face row from a wall, h / 2
   // geometry
   const T *chi_u, *chi_v, *chi_c, *a_u, *a_v, *k_u, *k_v;
   const T *chi_t, *a_t, *w_t, *vol_u, *vol_v, *ins_u, *ins_v;
@@ -608,6 +610,23 @@ __device__ T pid_step(T limit, T dt_prev, const T *ctrl_prev, T *ctrl_new) {
 
 // ================================================================== kernels
 // The explicit stage on one 64x8 tile (core.py:855-935 per face / cell).
+// Werner-Wengle wall model (core.py:509-568): the power law u+ = A (y+)^B inverted for the
+// wall stress, tau = (|us| (nu / y)^B / A)^(2 / (1 + B)); what it adds to the resolved no-slip
+// stress nu |us| / y (zero inside the viscous sublayer) acts on the face row next to the wall as
+// a force per unit volume -sign(us) excess / h. us: the face velocity relative to the wall.
+constexpr double kWwA = 8.3, kWwB = 1.0 / 7.0;
+template <typename T>
+__device__ __forceinline__ T wall_force(const Dev<T> &d, T us) {
+  const T mag = us < (T)0 ? -us : us;
+  const T nu_y = *d.nu / d.wall_y;
+  const T lam = mag * nu_y;
+  const T ww = pow((mag * pow(nu_y, (T)kWwB)) * (T)(1.0 / kWwA), (T)(2.0 / (1.0 + kWwB)));
+  T excess = ww - lam;
+  excess = excess > (T)0 ? excess : (T)0;
+  const T sgn = us > (T)0 ? (T)1 : (us < (T)0 ? (T)-1 : (T)0);
+  return -(sgn * excess) * d.inv_h;
+}
+
 template <typename T, int R>
 __device__ __forceinline__ void stage_body(const Dev<T> &d, const Stage<T> &s, int bx, int by) {
   __shared__ Lds<T, R> L;
@@ -708,6 +727,11 @@ __device__ __forceinline__ void stage_body(const Dev<T> &d, const Stage<T> &s, i
           const T txx_l = (T)2 * N(j, i - 1) * ((U(j, i) - U(j, i - 1)) * ih);
           lu = lu + (((txx_r - txx_l) + nutk(j + 1, i) * shear(j + 1, i)) - nutk(j, i) * shear(j, i)) * ih;
         }
+        if (d.wallm) {
+          if (j == 0 && (d.bcb == AERO_BC_NOSLIP || d.bcb == AERO_BC_MOVING))
+            lu = lu + wall_force(d, d.bcb == AERO_BC_MOVING ? U(j, i) - d.ground : U(j, i));
+          if (j == ny - 1 && d.bct == AERO_BC_NOSLIP) lu = lu + wall_force(d, U(j, i));
+        }
       }
       const int o = j * (nx + 1) + i;
       const T upv = U(j, i);
@@ -754,6 +778,10 @@ __device__ __forceinline__ void stage_body(const Dev<T> &d, const Stage<T> &s, i
           const T tyy_t = (T)2 * N(j, i) * ((V(j + 1, i) - V(j, i)) * ih);
           const T tyy_b = (T)2 * N(j - 1, i) * ((V(j, i) - V(j - 1, i)) * ih);
           lv = lv + (((nutk(j, i + 1) * shear(j, i + 1) - nutk(j, i) * shear(j, i)) + tyy_t) - tyy_b) * ih;
+        }
+        if (d.wallm && d.closed && (d.bcl == AERO_BC_NOSLIP || d.bcr == AERO_BC_NOSLIP)) {
+          if (i == 0 && d.bcl == AERO_BC_NOSLIP) lv = lv + wall_force(d, V(j, i));
+          if (i == nx - 1 && d.bcr == AERO_BC_NOSLIP) lv = lv + wall_force(d, V(j, i));
         }
       }
       const int o = j * nx + i;
@@ -2010,6 +2038,8 @@ Dev<T> make_dev(const aero2d_plan *p, const aero2d_geom *g, const aero2d_params 
   d.n_groups = s.n_groups;
   for (int q = 0; q < AERO2D_MAX_GROUPS; ++q) d.gch[q] = s.group_channel[q];
   d.fixed_dt = s.fixed_dt;
+  d.wallm = s.wall_model;
+  d.wall_y = (T)(0.5 * s.h);
   d.pid = s.pid_dt;
   d.hd = s.h;
   d.cfl = s.cfl;

@@ -94,7 +94,9 @@ typedef struct {
   int32_t tile_rows;       /* rows per 64-wide tile of the cell kernels: 0 (default, 4), 4, 8 or 16 */
   int32_t pid_dt;          /* adaptive_dt = "pid" (ns2d/diagnostics.py:104-123): needs fields ctrl, dt */
   int32_t diagnostics;     /* 1: the plan keeps scratch for outputs.diag rows */
-  int32_t reserved;
+  int32_t wall_model;      /* 1: Werner-Wengle shear on the faces next to no-slip domain walls
+                              (ns2d/core.py:509-568): u faces of no-slip / moving bottom and
+                              no-slip top walls, v faces of no-slip side walls when closed */
 } aero2d_desc;
 
 /* Evolving fields, caller-owned device buffers of the plan's dtype.

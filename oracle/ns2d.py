@@ -306,6 +306,42 @@ def sgs_stress_divergence(U, V, geom, ops):
     return fu, fv, nut.amax(), nut
 
 
+WW_A = 8.3          # Werner-Wengle power law u+ = A (y+)^B (core.py:509-513)
+WW_B = 1.0 / 7.0
+
+
+def wall_shear_excess(us, y, nu):
+    """The Werner-Wengle wall stress above the resolved no-slip one nu |us| / y, signed like
+    ``us`` (core.py:516-533): tau = (|us| (nu / y)^B / A)^(2 / (1 + B)), clamped at zero below
+    the crossover y+ = A^(1/(1-B))."""
+    a = us.abs()
+    tau_lam = a * (nu / y)
+    tau_ww = (a * (nu / y) ** WW_B * (1.0 / WW_A)) ** (2.0 / (1.0 + WW_B))
+    return torch.sign(us) * (tau_ww - tau_lam).clamp_min(0.0)
+
+
+def wall_model_force(u, v, ops):
+    """Force per unit volume (-excess / h) on the u faces next to no-slip (moving: relative to
+    the wall speed) bottom / top walls, and with a closed domain on the v faces next to
+    no-slip side walls (core.py:536-568); v part None when no side wall is modelled."""
+    y = 0.5 * ops.h
+    inv_h = 1.0 / ops.h
+    fu = torch.zeros_like(u)
+    if ops.bc_bottom in ("noslip", "moving"):
+        us = u[:1] - ops.ground_speed if ops.bc_bottom == "moving" else u[:1]
+        fu[:1] = -wall_shear_excess(us, y, ops.nu) * inv_h
+    if ops.bc_top == "noslip":
+        fu[-1:] = -wall_shear_excess(u[-1:], y, ops.nu) * inv_h
+    if not ops.closed or (ops.bc_left != "noslip" and ops.bc_right != "noslip"):
+        return fu, None
+    fv = torch.zeros_like(v)
+    if ops.bc_left == "noslip":
+        fv[:, :1] = -wall_shear_excess(v[:, :1], y, ops.nu) * inv_h
+    if ops.bc_right == "noslip":
+        fv[:, -1:] = -wall_shear_excess(v[:, -1:], y, ops.nu) * inv_h
+    return fu, fv
+
+
 def momentum_rhs(u, v, geom, ops):
     """Tendencies with boundary rows set, max nu_t, nu_t field (core.py:571-614)."""
     U, V = pad_u(u, ops), pad_v(v, ops)
@@ -318,6 +354,11 @@ def momentum_rhs(u, v, geom, ops):
         fu, fv, nut_max, nut = sgs_stress_divergence(U, V, geom, ops)
         lu = lu + fu
         lv = lv + fv
+    if ops.wall_model:
+        wu, wv = wall_model_force(u, v, ops)
+        lu = lu + wu
+        if wv is not None:
+            lv = lv + wv
     if ops.periodic:
         return lu, lv, nut_max, nut
     if ops.closed:

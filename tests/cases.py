@@ -73,6 +73,10 @@ VARIANTS2D = {
     "periodic_smag": dict(periodic=True, turbulence="smagorinsky", bodies=False),
     "closed_lid": dict(domain="closed", bc_left="slip", bc_right="slip", bc_bottom="moving", ground_speed=1.0,
                        bodies=True),
+    "ww_tunnel_moving": dict(re=1e5, turbulence="smagorinsky", bc_bottom="moving", ground_speed=1.0,
+                             bc_top="noslip", wall_model="werner_wengle", bodies=True),
+    "ww_closed_room": dict(re=1e6, domain="closed", bc_left="noslip", bc_right="noslip", bc_bottom="moving",
+                           ground_speed=1.0, bc_top="noslip", wall_model="werner_wengle", bodies=True),
 }
 
 
@@ -80,8 +84,8 @@ def variant_cfg(name, dtype="float64", device="cpu"):
     from aerolume_amd.ns2d.config import ImmersedBody, SolverConfig
     kw = dict(VARIANTS2D[name])
     bodies = [ImmersedBody(circle(64, 0.75, 2.0, 2.0))] if kw.pop("bodies") else []
-    return SolverConfig(nx=64, ny=32, h=1.0 / 8.0, x0=0.0, y0=0.0, re=400.0, bodies=bodies, dtype=dtype,
-                        device=device, dye=False, perturb=False, **kw)
+    return SolverConfig(nx=64, ny=32, h=1.0 / 8.0, x0=0.0, y0=0.0, re=kw.pop("re", 400.0), bodies=bodies,
+                        dtype=dtype, device=device, dye=False, perturb=False, **kw)
 
 
 # ---------------------------------------------------------------------------

@@ -10,11 +10,14 @@ other fixtures leave out (run in the build container only; it imports the refere
 * ``ns2d_variants.npz`` - five float64 steps of ``core.step_full`` from a seeded noisy start
   for the 2D features no other fixture steps: central advection with Vreman and no-slip
   walls behind a parabolic inlet; WALE over a moving bottom wall; a doubly periodic box with
-  Smagorinsky; a closed lid-driven box with slip side walls.
+  Smagorinsky; a closed lid-driven box with slip side walls; the Werner-Wengle wall model
+  (``wall_model="werner_wengle"``) in a Re 1e5 tunnel over a moving floor under a no-slip
+  roof, and in a closed Re 1e6 room with a moving floor and no-slip walls all round.
 
 Each file holds inputs and expected outputs only.
 
-Usage: ``python tests/golden/make_ns2d_default_fixtures.py`` (about half a minute).
+Usage: ``python tests/golden/make_ns2d_default_fixtures.py [variants]`` (about half a minute;
+``variants``: only ``ns2d_variants.npz``).
 """
 from __future__ import annotations
 
@@ -87,6 +90,10 @@ VARIANTS = {
     "periodic_smag": dict(periodic=True, turbulence="smagorinsky", bodies=False),
     "closed_lid": dict(domain="closed", bc_left="slip", bc_right="slip", bc_bottom="moving", ground_speed=1.0,
                        bodies=True),
+    "ww_tunnel_moving": dict(re=1e5, turbulence="smagorinsky", bc_bottom="moving", ground_speed=1.0,
+                             bc_top="noslip", wall_model="werner_wengle", bodies=True),
+    "ww_closed_room": dict(re=1e6, domain="closed", bc_left="noslip", bc_right="noslip", bc_bottom="moving",
+                           ground_speed=1.0, bc_top="noslip", wall_model="werner_wengle", bodies=True),
 }
 
 
@@ -95,8 +102,8 @@ def variants_fixture():
     for name, kw in VARIANTS.items():
         kw = dict(kw)
         bodies = [ImmersedBody(circle(64, 0.75, 2.0, 2.0))] if kw.pop("bodies") else []
-        cfg = SolverConfig(nx=64, ny=32, h=1.0 / 8.0, x0=0.0, y0=0.0, re=400.0, bodies=bodies, dtype="float64",
-                           device="cpu", dye=False, perturb=False, **kw)
+        cfg = SolverConfig(nx=64, ny=32, h=1.0 / 8.0, x0=0.0, y0=0.0, re=kw.pop("re", 400.0), bodies=bodies,
+                           dtype="float64", device="cpu", dye=False, perturb=False, **kw)
         ops = core.make_operators(cfg)
         geom = build_geometry(cfg.bodies, ops)
         u, v = noisy_start(ops, 17, closed=kw.get("domain") == "closed", periodic=kw.get("periodic", False))
@@ -113,7 +120,8 @@ def variants_fixture():
 
 if __name__ == "__main__":
     torch.set_num_threads(8)
-    default_fixture("float64")
-    default_fixture("float32")
+    if sys.argv[1:] != ["variants"]:
+        default_fixture("float64")
+        default_fixture("float32")
     variants_fixture()
     print("written", flush=True)

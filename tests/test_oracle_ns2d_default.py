@@ -124,3 +124,47 @@ def test_oracle_variants_match_reference(golden, name):
     assert _err(cur.t.numpy(), F[f"{name}:t5"]) < 1e-14
     if cs[0].size:
         assert _err(np.stack(cs), F[f"{name}:coeffs"]) < 1e-10
+
+
+def test_wall_shear_excess_solves_the_power_law():
+    """The oracle's Werner-Wengle stress against the power law solved numerically: for a face
+    velocity us at distance y, find u_tau with us / u_tau = A (u_tau y / nu)^B (a root finder,
+    not the closed form), then the excess is max(u_tau^2 - nu |us| / y, 0) with the sign of us;
+    zero in the viscous sublayer (us y / nu below A^(2/(1-B)))."""
+    from scipy.optimize import brentq
+    nu, y = 1e-4, 0.03
+    for us in (-2.0, -0.3, -1e-3, 0.0, 2e-4, 5e-3, 0.07, 1.3, 9.0):
+        got = float(O.wall_shear_excess(torch.tensor(us, dtype=torch.float64), y, torch.tensor(nu,
+                                                                                                dtype=torch.float64)))
+        a = abs(us)
+        if a == 0.0:
+            assert got == 0.0
+            continue
+        ut = brentq(lambda t: a / t - O.WW_A * (t * y / nu) ** O.WW_B, 1e-12, 1e3, xtol=1e-300, rtol=1e-15)
+        want = np.sign(us) * max(ut * ut - nu * a / y, 0.0)
+        assert abs(got - want) <= 1e-12 * max(abs(want), nu * a / y), us
+        sublayer = a * y / nu < O.WW_A ** (2.0 / (1.0 - O.WW_B))
+        assert (got == 0.0) == sublayer, us
+
+
+@pytest.mark.parametrize("name", ["ww_tunnel_moving", "ww_closed_room"])
+def test_wall_model_acts_in_its_fixture_cases(golden, name):
+    """The wall-model variants differ from the same runs without the model (so the fixture
+    comparison pins the model, not just the rest of the step), and the difference starts on
+    the modelled face rows: the tendencies of the reference's state after five steps change on
+    the wall rows only (and on both walls)."""
+    import dataclasses
+    F = golden("ns2d_variants.npz")
+    cfg = cases.variant_cfg(name)
+    ops, geom = _setup(cfg)
+    off = dataclasses.replace(ops, wall_model=False)
+    u5, v5 = torch.as_tensor(F[f"{name}:u5"]), torch.as_tensor(F[f"{name}:v5"])
+    lu1, lv1, _, _ = O.momentum_rhs(u5, v5, geom, ops)
+    lu0, lv0, _, _ = O.momentum_rhs(u5, v5, geom, off)
+    rows = (lu1 - lu0).abs().amax(dim=1)
+    assert float(rows[0]) > 0 and float(rows[-1]) > 0 and float(rows[1:-1].abs().max()) == 0.0
+    cols = (lv1 - lv0).abs().amax(dim=0)
+    if ops.closed:
+        assert float(cols[0]) > 0 and float(cols[-1]) > 0 and float(cols[1:-1].max()) == 0.0
+    else:
+        assert float(cols.max()) == 0.0

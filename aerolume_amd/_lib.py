@@ -36,7 +36,8 @@ class Desc2D(ctypes.Structure):
                 ("dye_mode", i32), ("n_dye", i32), ("n_groups", i32), ("group_channel", i32 * MAX_GROUPS),
                 ("h", f64), ("ground_speed", f64), ("cfl", f64), ("force_ref", f64), ("les_c", f64),
                 ("gravity_x", f64), ("gravity_y", f64), ("prt", f64), ("diff_coef", f64),
-                ("tile_rows", i32), ("pid_dt", i32), ("diagnostics", i32), ("wall_model", i32)]
+                ("tile_rows", i32), ("pid_dt", i32), ("diagnostics", i32), ("wall_model", i32),
+                ("n_flux", i32), ("reserved", i32)]
 
 
 class Fields2D(ctypes.Structure):
@@ -45,7 +46,7 @@ class Fields2D(ctypes.Structure):
 
 class Geom2D(ctypes.Structure):
     _fields_ = [(k, vp) for k in ("chi_u", "chi_v", "chi_c", "a_u", "a_v", "k_u", "k_v", "chi_t", "a_t", "w_t",
-                                  "vol_u", "vol_v", "ins_u", "ins_v")]
+                                  "vol_u", "vol_v", "ins_u", "ins_v", "q_skin", "w_f")]
 
 
 class Params2D(ctypes.Structure):
@@ -57,7 +58,8 @@ class Params2D(ctypes.Structure):
 class Outputs2D(ctypes.Structure):
     _fields_ = [("coeffs", vp), ("heat", vp), ("wall_heat", vp), ("heat_storage", vp), ("flow_heat", vp),
                 ("hist", vp), ("hist_count", vp), ("hist_cap", ctypes.c_int64), ("aux", vp), ("aux_count", vp),
-                ("aux_cap", ctypes.c_int64), ("diag", vp), ("diag_count", vp), ("diag_cap", ctypes.c_int64)]
+                ("aux_cap", ctypes.c_int64), ("diag", vp), ("diag_count", vp), ("diag_cap", ctypes.c_int64),
+                ("theta_wall", vp)]
 
 
 #: columns of a 2D diagnostics row (AERO2D_DIAG_COLS; float64)

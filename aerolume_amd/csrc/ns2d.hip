@@ -73,7 +73,7 @@ struct Dev {
   int periodic, closed, upwind, turb, les;
   int bcb, bct, bcl, bcr;
   int j0, j1, i0, i1, wx, wy, win;
-  int nbody, thermal, buoyant, n_temp, has_heat, insulated;
+  int nbody, thermal, buoyant, n_temp, has_heat, insulated, n_flux;
   int wm[4];
   int dye_mode, n_dye, n_groups;
   int gch[AERO2D_MAX_GROUPS];
@@ -92,6 +92,7 @@ struct Dev {
   // geometry
   const T *chi_u, *chi_v, *chi_c, *a_u, *a_v, *k_u, *k_v;
   const T *chi_t, *a_t, *w_t, *vol_u, *vol_v, *ins_u, *ins_v;
+  const T *q_skin, *w_f;   // heat-flux bodies: skin source per unit kappa, skin weights
   // params
   const T *nu, *eta, *u_inlet, *dt_fixed, *kappa, *ri, *eta_t, *theta_in, *wall_values, *heat, *heat_max;
   const T *dye_inflow, *g_weight, *g_target, *g_timing;
@@ -831,6 +832,9 @@ __device__ __forceinline__ void stage_body(const Dev<T> &d, const Stage<T> &s, i
       lt = lt + (((dfx(i) - dfx(i + 1)) + dfy(j)) - dfy(j + 1)) * ih;
       const int o = j * nx + i;
       if (d.has_heat) lt = lt + d.heat[o];
+      // heat-flux bodies release kappa G P over their skins (physics.py:726-727)
+      if (d.n_flux > 0 && d.win && j >= d.j0 && j < d.j1 && i >= d.i0 && i < d.i1)
+        lt = lt + kap * d.q_skin[(j - d.j0) * d.wx + (i - d.i0)];
       const T tp = C(j, i);
       T thh = (s.a == 0.0 ? tp : lerp_t(tp, s.th0[o], a)) + lt * bdt;
       if (d.n_temp > 0 && d.win && j >= d.j0 && j < d.j1 && i >= d.i0 && i < d.i1) {
@@ -1364,6 +1368,16 @@ __device__ __forceinline__ void final_body(const Dev<T> &d, int bx, int by, cons
           if (b >= 0 && b < d.n_temp) acc[q] += (double)d.w_t[(size_t)b * d.wy * d.wx + wo] * at;
         }
       }
+      if (d.n_flux > 0 && j >= d.j0 && j < d.j1 && i >= d.i0 && i < d.i1) {
+        // skin-weighted temperature of each heat-flux body (core.py:988-990), slots after the heat
+        const int wo = (j - d.j0) * d.wx + (i - d.i0);
+        const double tv = (double)th_new[j * nx + i];
+#pragma unroll
+        for (int q = 0; q < kMaxWin; ++q) {
+          const int b = q - 4 * d.nbody - d.n_temp;
+          if (b >= 0 && b < d.n_flux) acc[q] += (double)d.w_f[(size_t)b * d.wy * d.wx + wo] * tv;
+        }
+      }
     }
     block_sum4<kMaxWin>(acc, L.red);
     if (threadIdx.x == 0 && threadIdx.y == 0) {
@@ -1583,36 +1597,40 @@ __device__ __forceinline__ void finalize_body(const Dev<T> &d, const T *__restri
     *cnt = hist_n + 1;
   }
   if (d.thermal) {
-    const int nt = d.n_temp;
+    // row layout: heat of the temperature bodies, skin temperature of the heat-flux bodies,
+    // wall heat (4), storage, advective in / out
+    const int nt = d.n_temp, nf = d.win ? d.n_flux : 0, w0 = nt + nf;
     const T h2dt = (T)(d.hd * d.hd) / dt;
     T vals[kMaxWin + 8];
     for (int b = 0; b < nt; ++b) vals[b] = (T)res[4 * d.nbody + b] * h2dt;
+    for (int b = 0; b < nf; ++b) vals[nt + b] = (T)res[4 * d.nbody + nt + b];
     const int len[4] = {ny, ny, nx, nx};
     for (int q = 0; q < 4; ++q) {
       const int m = d.wm[q];
       T val = 0;
       if (m == AERO_TH_TEMPERATURE || m == AERO_TH_INLET) val = ((T)2 * kap) * (T)ws[q];
       else if (m == AERO_TH_HEAT_FLUX) val = kap * wv[q] * (T)(len[q] * d.hd);
-      vals[nt + q] = val;
+      vals[w0 + q] = val;
     }
-    vals[nt + 4] = (T)(mx[4] - th_old) * h2dt;
+    vals[w0 + 4] = (T)(mx[4] - th_old) * h2dt;
     const bool tunnel = !d.periodic && !d.closed;
-    vals[nt + 5] = tunnel ? (T)ws[4] * (T)d.hd : (T)0;
-    vals[nt + 6] = tunnel ? (T)ws[5] * (T)d.hd : (T)0;
+    vals[w0 + 5] = tunnel ? (T)ws[4] * (T)d.hd : (T)0;
+    vals[w0 + 6] = tunnel ? (T)ws[5] * (T)d.hd : (T)0;
     if (o.heat) for (int b = 0; b < nt; ++b) reinterpret_cast<T *>(o.heat)[b] = vals[b];
-    if (o.wall_heat) for (int q = 0; q < 4; ++q) reinterpret_cast<T *>(o.wall_heat)[q] = vals[nt + q];
-    if (o.heat_storage) reinterpret_cast<T *>(o.heat_storage)[0] = vals[nt + 4];
+    if (o.theta_wall) for (int b = 0; b < nf; ++b) reinterpret_cast<T *>(o.theta_wall)[b] = vals[nt + b];
+    if (o.wall_heat) for (int q = 0; q < 4; ++q) reinterpret_cast<T *>(o.wall_heat)[q] = vals[w0 + q];
+    if (o.heat_storage) reinterpret_cast<T *>(o.heat_storage)[0] = vals[w0 + 4];
     if (o.flow_heat) {
-      reinterpret_cast<T *>(o.flow_heat)[0] = vals[nt + 5];
-      reinterpret_cast<T *>(o.flow_heat)[1] = vals[nt + 6];
+      reinterpret_cast<T *>(o.flow_heat)[0] = vals[w0 + 5];
+      reinterpret_cast<T *>(o.flow_heat)[1] = vals[w0 + 6];
     }
     if (o.aux) {
       long long *cnt = reinterpret_cast<long long *>(o.aux_count);
       const long long row = aux_n % o.aux_cap;
-      const int ncol = 1 + nt + 7;
+      const int ncol = 1 + w0 + 7;
       T *arow = reinterpret_cast<T *>(o.aux) + row * ncol;
       arow[0] = tnew;
-      for (int q = 0; q < nt + 7; ++q) arow[1 + q] = vals[q];
+      for (int q = 0; q < w0 + 7; ++q) arow[1 + q] = vals[q];
       *cnt = aux_n + 1;
     }
   }
@@ -2032,6 +2050,7 @@ Dev<T> make_dev(const aero2d_plan *p, const aero2d_geom *g, const aero2d_params 
   d.n_temp = s.n_temp;
   d.has_heat = s.has_heat;
   d.insulated = s.insulated;
+  d.n_flux = s.thermal ? s.n_flux : 0;
   for (int q = 0; q < 4; ++q) d.wm[q] = s.wall_modes[q];
   d.dye_mode = s.dye_mode;
   d.n_dye = s.n_dye;
@@ -2064,6 +2083,7 @@ Dev<T> make_dev(const aero2d_plan *p, const aero2d_geom *g, const aero2d_params 
     d.chi_t = (const T *)g->chi_t; d.a_t = (const T *)g->a_t; d.w_t = (const T *)g->w_t;
     d.vol_u = (const T *)g->vol_u; d.vol_v = (const T *)g->vol_v;
     d.ins_u = (const T *)g->ins_u; d.ins_v = (const T *)g->ins_v;
+    d.q_skin = (const T *)g->q_skin; d.w_f = (const T *)g->w_f;
   }
   if (pr) {
     d.nu = (const T *)pr->nu; d.eta = (const T *)pr->eta; d.u_inlet = (const T *)pr->u_inlet;
@@ -2293,9 +2313,14 @@ int check_desc(const aero2d_desc *d) {
     set_error("body window outside the grid");
     return AERO_EINVAL;
   }
-  if (4 * d->nbody + (d->thermal ? d->n_temp : 0) > kMaxWin) {
-    set_error("too many bodies for the per-block force partials (4 per body + 1 per temperature body <= 16)");
+  if (4 * d->nbody + (d->thermal ? d->n_temp + d->n_flux : 0) > kMaxWin) {
+    set_error("too many bodies for the per-block window partials (4 per body + 1 per temperature or heat-flux "
+              "body <= 16)");
     return AERO_EUNSUPPORTED;
+  }
+  if (d->n_flux < 0 || (d->n_flux > 0 && !d->thermal) || d->n_flux + d->n_temp > d->nbody) {
+    set_error("bad heat-flux body count");
+    return AERO_EINVAL;
   }
   return AERO_OK;
 }
@@ -2330,6 +2355,10 @@ int check_step_args(aero2d_plan *p, const aero2d_fields *in, aero2d_fields *out,
   }
   if (s.thermal && s.insulated && win && (!g->ins_u || !g->ins_v)) {
     set_error("insulated window needs ins_u, ins_v");
+    return AERO_EINVAL;
+  }
+  if (s.thermal && s.n_flux > 0 && win && (!g->q_skin || !g->w_f)) {
+    set_error("heat-flux bodies need q_skin and w_f");
     return AERO_EINVAL;
   }
   if (s.has_heat && (!pr->heat || !pr->heat_max)) { set_error("heaters need heat and heat_max"); return AERO_EINVAL; }

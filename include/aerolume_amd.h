@@ -97,6 +97,9 @@ typedef struct {
   int32_t wall_model;      /* 1: Werner-Wengle shear on the faces next to no-slip domain walls
                               (ns2d/core.py:509-568): u faces of no-slip / moving bottom and
                               no-slip top walls, v faces of no-slip side walls when closed */
+  int32_t n_flux;          /* heat-flux bodies: skin sources kappa q_skin on the window and their
+                              mean skin temperatures (ns2d/physics.py:726-727, core.py:988-990) */
+  int32_t reserved;
 } aero2d_desc;
 
 /* Evolving fields, caller-owned device buffers of the plan's dtype.
@@ -111,10 +114,12 @@ typedef struct {
  * chi_u, a_u (wy, wx+1); chi_v, a_v (wy+1, wx); chi_c (wy, wx);
  * k_u (nbody, 2, wy, wx+1); k_v (nbody, 2, wy+1, wx);
  * chi_t, a_t (wy, wx); w_t (n_temp, wy, wx); vol_u (wy, wx+1); vol_v (wy+1, wx);
- * ins_u (wy, wx+1); ins_v (wy+1, wx). */
+ * ins_u (wy, wx+1); ins_v (wy+1, wx); q_skin (wy, wx): the heat-flux bodies' skin source per
+ * unit kappa (ns2d/geometry.py:403-411); w_f (n_flux, wy, wx): their normalised skin weights. */
 typedef struct {
   const void *chi_u, *chi_v, *chi_c, *a_u, *a_v, *k_u, *k_v;
   const void *chi_t, *a_t, *w_t, *vol_u, *vol_v, *ins_u, *ins_v;
+  const void *q_skin, *w_f;
 } aero2d_geom;
 
 /* Live values as device scalars / arrays (edited in place between steps, so a
@@ -129,11 +134,11 @@ typedef struct {
 } aero2d_params;
 
 /* Per-step outputs (device; any may be NULL): coeffs (nbody, 3) = [Cd, Cl, Cm];
- * heat (n_temp); wall_heat (4); heat_storage (1); flow_heat (2).
+ * heat (n_temp); wall_heat (4); heat_storage (1); flow_heat (2); theta_wall (n_flux).
  * hist: ring of hist_cap rows [t, Cd, Cl, Cm] (summed over bodies) written at
  * row (*hist_count mod hist_cap), then *hist_count += 1 (int64 device counter).
- * aux: ring of aux_cap rows [t, heat.., wall_heat(4), heat_storage, flow_heat(2)]
- * with counter aux_count. */
+ * aux: ring of aux_cap rows [t, heat.., theta_wall.., wall_heat(4), heat_storage,
+ * flow_heat(2)] with counter aux_count. */
 typedef struct {
   void *coeffs, *heat, *wall_heat, *heat_storage, *flow_heat;
   void *hist, *hist_count;
@@ -147,6 +152,7 @@ typedef struct {
    * without temperature) (ns2d/diagnostics.py:70-101; the "step" column is the caller's) */
   void *diag, *diag_count;
   int64_t diag_cap;
+  void *theta_wall;
 } aero2d_outputs;
 
 #define AERO2D_DIAG_COLS 22

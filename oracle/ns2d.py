@@ -848,6 +848,10 @@ def step_full(state: State, geom, ops, dt=None, *, out: dict | None = None, trac
         tg = geom.thermal
         if tg is not None and tg.temp_ids:
             out["heat"] = heat_acc * (h2 / dt)
+        if tg is not None and tg.flux_ids and not geom.empty:
+            # mean skin temperature of each heat-flux body (core.py:988-990)
+            j0, j1, i0, i1 = geom.window
+            out["theta_wall"] = torch.einsum("bij,ij->b", tg.w_f, new.theta[j0:j1, i0:i1])
         out["wall_heat"] = wall_heat_rates(new.theta, ph, ops)
         out["heat_storage"] = ((new.theta.sum() - state.theta.sum()) * (h2 / dt)).reshape(1)
         out["flow_heat"] = flow_heat_rates(new.theta, new.u, ph, ops)

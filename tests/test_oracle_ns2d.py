@@ -304,3 +304,55 @@ def test_oracle_wrong_stage_order_is_caught(golden):
     finally:
         O.add_buoyancy = orig
     assert np.abs(cur.v.numpy() - F["v1"]).max() > 1e-8
+
+
+def test_oracle_heat_flux_bodies_match_reference(golden):
+    """Heat-flux bodies (a disc releasing G = 2 through its skin beside a disc held at theta = 1),
+    float64, 20 steps: this package's thermal window arrays equal the reference's, and the
+    oracle's u, v, p, theta and heat outputs (the held disc's heat, the flux disc's mean skin
+    temperature, walls, storage, advective flow) follow the reference's to round-off."""
+    F = golden("ns2d_flux_bodies.npz")
+    cfg = cases.flux_cfg("float64")
+    ops = _ops(cfg)
+    geom = _geom(cfg, ops, live=True)
+    assert tuple(geom.window) == tuple(int(x) for x in F["window"])
+    tg = geom.thermal
+    assert tuple(tg.flux_ids) == tuple(F["flux_ids"]) and tuple(tg.temp_ids) == tuple(F["temp_ids"])
+    for k in ("q_skin", "w_f", "perimeter", "ins_u", "ins_v", "chi_t", "flux_g"):
+        assert np.abs(getattr(tg, k).numpy() - F["g_" + k]).max() <= 1e-12 * max(1.0, np.abs(F["g_" + k]).max()), k
+    cur = _state(F["u0"], F["v0"], ops, theta=F["theta0"])
+    for n in range(1, 21):
+        out = {}
+        cur, c = O.step_full(cur, geom, ops, out=out)
+        assert np.abs(c.numpy() - F["hist_coeffs"][n - 1]).max() < 1e-11
+        for k in ("heat", "theta_wall", "wall_heat", "heat_storage", "flow_heat"):
+            want = F["hist_" + k][n - 1]
+            assert np.abs(out[k].numpy() - want).max() <= 1e-11 * max(1.0, np.abs(want).max()), (n, k)
+    for k in ("u", "v", "theta"):
+        assert np.abs(getattr(cur, k).numpy() - F[f"{k}20"]).max() < 1e-12, k
+    assert np.abs(cur.p.numpy() - F["p20"]).max() < 1e-10
+
+
+def test_heat_flux_skin_releases_the_prescribed_rate(golden):
+    """What the mathematics fixes for the skin source: kappa * q_skin summed over the window
+    cells times h^2 is the body's heat rate G P / Pe (the skin weights sum to one), the skin
+    lies in the diffuse band of the flux body only, and a planted mistake (the source without
+    kappa) breaks the reference comparison."""
+    F = golden("ns2d_flux_bodies.npz")
+    cfg = cases.flux_cfg("float64")
+    ops = _ops(cfg)
+    geom = _geom(cfg, ops, live=True)
+    tg = geom.thermal
+    kappa = float(ops.phys.kappa)
+    rate = kappa * float(tg.q_skin.sum()) * ops.h ** 2
+    b = tg.flux_ids[0]
+    want = cfg.bodies[b].thermal_value * float(tg.perimeter[b]) / (cfg.re * cfg.thermal.prandtl)
+    assert rate == pytest.approx(want, rel=1e-12)
+    assert float(tg.w_f.sum()) == pytest.approx(1.0, rel=1e-14)
+    assert float(tg.w_f.min()) >= 0.0
+    # planted mistake: the skin source without kappa moves theta off the reference's at step 1
+    good = O.step_full(_state(F["u0"], F["v0"], ops, theta=F["theta0"]), geom, ops)[0]
+    assert np.abs(good.theta.numpy() - F["theta1"]).max() < 1e-12
+    tg.q_skin = tg.q_skin / kappa
+    bad = O.step_full(_state(F["u0"], F["v0"], ops, theta=F["theta0"]), geom, ops)[0]
+    assert np.abs(bad.theta.numpy() - F["theta1"]).max() > 1e-3

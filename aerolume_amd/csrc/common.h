@@ -189,6 +189,76 @@ __device__ cplx<T> *fft_lds(cplx<T> *a, cplx<T> *b, const FftPlan &pl, const cpl
   return src;
 }
 
+// Power-of-two lengths with the stage structure fixed at compile time: radix-4 stages of span
+// 1, 4, 16, ... and, for odd log2 n, a last radix-2 stage (the radix order factor_radices
+// gives, so the twiddle table and the results are those of fft_lds). Every index split is a
+// shift or a mask and every loop bound a constant, which leaves each stage its loads, the
+// butterfly and its stores.
+template <int V>
+struct Log2 { static constexpr int value = 1 + Log2<V / 2>::value; };
+template <>
+struct Log2<1> { static constexpr int value = 0; };
+
+template <typename T, int N, int R, int P>
+__device__ __forceinline__ void pow2_stage(const cplx<T> *__restrict__ src, cplx<T> *__restrict__ dst,
+                                           const cplx<T> *__restrict__ tw, int nslot, int slot_stride, int tid,
+                                           int nthr) {
+  constexpr int NR = N / R, LNR = Log2<NR>::value, LR = Log2<R>::value;
+  for (int w = tid; w < (nslot << LNR); w += nthr) {
+    const int f = w >> LNR, k = w & (NR - 1);
+    const cplx<T> *s = src + f * slot_stride;
+    cplx<T> *d = dst + f * slot_stride;
+    const int i = k & (P - 1);
+    cplx<T> v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = s[k + r * NR];
+    if constexpr (P > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[(P - 1) + (r - 1) * P + i]);
+    }
+    if constexpr (R == 2) dft2(v);
+    if constexpr (R == 4) dft4(v);
+    const int j = ((k - i) << LR) + i;
+#pragma unroll
+    for (int r = 0; r < R; ++r) d[j + r * P] = v[r];
+  }
+}
+
+template <typename T, int N, int S>
+__device__ __forceinline__ cplx<T> *pow2_stages(cplx<T> *src, cplx<T> *dst, const cplx<T> *__restrict__ tw, int nslot,
+                                                int slot_stride, int tid, int nthr) {
+  constexpr int LG = Log2<N>::value;
+  if constexpr (2 * S + 2 <= LG) {
+    pow2_stage<T, N, 4, (1 << (2 * S))>(src, dst, tw, nslot, slot_stride, tid, nthr);
+    __syncthreads();
+    return pow2_stages<T, N, S + 1>(dst, src, tw, nslot, slot_stride, tid, nthr);
+  } else if constexpr (2 * S + 1 == LG) {
+    pow2_stage<T, N, 2, (1 << (2 * S))>(src, dst, tw, nslot, slot_stride, tid, nthr);
+    __syncthreads();
+    return dst;
+  } else {
+    return src;
+  }
+}
+
+// Forward transform of nslot sequences, as fft_lds: the power-of-two lengths 64 .. 2048 take
+// their compile-time stage sequence, other lengths the runtime plan.
+template <typename T>
+__device__ cplx<T> *fft_run(cplx<T> *a, cplx<T> *b, const FftPlan &pl, const cplx<T> *__restrict__ tw, int nslot,
+                            int slot_stride) {
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  const int nthr = blockDim.x * blockDim.y;
+  switch (pl.n) {
+    case 64: return pow2_stages<T, 64, 0>(a, b, tw, nslot, slot_stride, tid, nthr);
+    case 128: return pow2_stages<T, 128, 0>(a, b, tw, nslot, slot_stride, tid, nthr);
+    case 256: return pow2_stages<T, 256, 0>(a, b, tw, nslot, slot_stride, tid, nthr);
+    case 512: return pow2_stages<T, 512, 0>(a, b, tw, nslot, slot_stride, tid, nthr);
+    case 1024: return pow2_stages<T, 1024, 0>(a, b, tw, nslot, slot_stride, tid, nthr);
+    case 2048: return pow2_stages<T, 2048, 0>(a, b, tw, nslot, slot_stride, tid, nthr);
+    default: return fft_lds(a, b, pl, tw, nslot, slot_stride);
+  }
+}
+
 // ------------------------------------------------------------------ real transforms of pairs
 // Two real sequences xa, xb of length n are packed into one complex sequence
 // z = xa + i xb; one complex FFT gives both spectra (Hermitian split).

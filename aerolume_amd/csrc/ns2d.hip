@@ -201,7 +201,7 @@ __device__ __forceinline__ T finish_v(const Dev<T> &d, T x, T q1, T q0, int flag
 // c + 64), each wave the row part of its rows, and every element then costs its loads and
 // finish_u / finish_v.
 template <typename T, int R>
-__device__ void load_uv(const Dev<T> &d, Lds<T, R> &L, int tj0, int ti0, const T *u, const T *v, const T *psi, T corr) {
+__device__ __forceinline__ void load_uv(const Dev<T> &d, Lds<T, R> &L, int tj0, int ti0, const T *u, const T *v, const T *psi, T corr) {
   const int nx = d.nx, ny = d.ny;
   const bool hp = psi != nullptr;
   const int lane = threadIdx.x, wave = threadIdx.y;
@@ -335,7 +335,7 @@ __device__ void load_uv(const Dev<T> &d, Lds<T, R> &L, int tj0, int ti0, const T
 // replicate elsewhere, wrap when periodic; use_inflow 2 is the smoke comb's rule (inflow also
 // in a closed box, core.py:718-723)
 template <typename T, int R>
-__device__ void load_c(const Dev<T> &d, Lds<T, R> &L, int tj0, int ti0, const T *c, const T *inflow_col, T inflow0,
+__device__ __forceinline__ void load_c(const Dev<T> &d, Lds<T, R> &L, int tj0, int ti0, const T *c, const T *inflow_col, T inflow0,
                        int use_inflow) {
   const int nx = d.nx, ny = d.ny;
   const int lane = threadIdx.x, wave = threadIdx.y;
@@ -407,7 +407,7 @@ __device__ __forceinline__ T eddy(const Dev<T> &d, T g11, T g12, T g21, T g22) {
 
 // nu_t on the tile's cells and their ring (core.py:421-503), switched off in solids.
 template <typename T, int R>
-__device__ void tile_nut(const Dev<T> &d, Lds<T, R> &L, int tj0, int ti0) {
+__device__ __forceinline__ void tile_nut(const Dev<T> &d, Lds<T, R> &L, int tj0, int ti0) {
   const int tid = threadIdx.x + threadIdx.y * blockDim.x;
   const T ih = d.inv_h;
   for (int e = tid; e < TR<R>::NH * NW; e += 256) {
@@ -858,7 +858,7 @@ __global__ void __launch_bounds__(256) k_stage(Dev<T> d, Stage<T> s) {
 // Per-block maxima of a velocity tile (+ theta, nu_t) and sum theta: the dt inputs.
 // Each tile also covers the ghost ring of nu_t cells at the domain edge (core.py:503).
 template <typename T, int R>
-__device__ void tile_maxima(const Dev<T> &d, Lds<T, R> &L, int tj0, int ti0, const T *th) {
+__device__ __forceinline__ void tile_maxima(const Dev<T> &d, Lds<T, R> &L, int tj0, int ti0, const T *th) {
   const int nx = d.nx, ny = d.ny;
   double mu = 0, mv = 0, mt = 0, mn = 0, st = 0;
   const int i = ti0 + threadIdx.x;
@@ -1026,34 +1026,67 @@ __device__ __forceinline__ void poisA_body(const Dev<T> &d, int bx, const T *__r
   const int ja = 2 * bx, jb = ja + 1;
   const bool hasb = jb < d.ny;
   load_twiddles(TW, d.tw_x, nx);
+  // every load of this thread is issued before the outlet correction's reduction waits:
+  // the row elements (the face values of the divergence, or the right-hand side) and the DCT
+  // twiddles of output frequency k = q, which this thread finishes below
+  // (the first PF elements of each thread; longer rows on smaller blocks load the rest after)
+  constexpr int PF = MP < 4 ? MP : 4;
+  cplx<T> csr[MP];
+  T ur[PF][2], ul[PF][2], vt[PF][2], vb[PF][2];
+#pragma unroll
+  for (int r = 0; r < MP; ++r) {
+    const int q = threadIdx.x + NT * r;
+    if (q < nx && d.dct_x) csr[r] = d.cs_x[q];
+  }
+#pragma unroll
+  for (int r = 0; r < PF; ++r) {
+    const int q = threadIdx.x + NT * r;
+    if (q < nx) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int j = ja + h;
+        ur[r][h] = ul[r][h] = vt[r][h] = vb[r][h] = (T)0;
+        if (h == 1 && !hasb) continue;
+        if (mode == 0) {
+          ur[r][h] = f_in[j * nx + q];
+        } else {
+          ur[r][h] = u[j * (nx + 1) + q + 1];
+          ul[r][h] = u[j * (nx + 1) + q];
+          vt[r][h] = v[(j + 1) * nx + q];
+          vb[r][h] = v[j * nx + q];
+        }
+      }
+    }
+  }
   T corr = 0;
   if (mode == 2) {
     corr = outlet_corr(d, red);
     if (bx == 0 && threadIdx.x == 0) d.red->corr = corr;
   }
-  // all loads of this thread first (row elements and DCT twiddles), then the FFT
-  cplx<T> csr[MP];
 #pragma unroll
   for (int r = 0; r < MP; ++r) {
-    // natural column q (coalesced loads) into transform position m; the DCT twiddles are
-    // those of output frequency k = q, which this thread finishes below
+    // natural column q into transform position m; the divergence as div_at forms it
     const int q = threadIdx.x + NT * r;
     if (q < nx) {
       const int m = d.dct_x ? makhoul_pos(q, nx) : q;
-      T xa, xb = 0;
-      if (mode == 0) {
-        xa = f_in[ja * nx + q];
-        if (hasb) xb = f_in[jb * nx + q];
-      } else {
-        xa = div_at(d, u, v, ja, q, corr);
-        if (hasb) xb = div_at(d, u, v, jb, q, corr);
+      T x[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (r >= PF) {
+          x[h] = (h == 1 && !hasb) ? (T)0 : (mode == 0 ? f_in[(ja + h) * nx + q] : div_at(d, u, v, ja + h, q, corr));
+        } else if (mode == 0) {
+          x[h] = ur[r < PF ? r : 0][h];
+        } else {
+          const int rp = r < PF ? r : 0;
+          const T right = q + 1 == nx ? ur[rp][h] + corr : ur[rp][h];
+          x[h] = (((right - ul[rp][h]) + vt[rp][h]) - vb[rp][h]) * d.inv_h;
+        }
       }
-      A[m] = {xa, xb};
-      if (d.dct_x) csr[r] = d.cs_x[q];
+      A[m] = {x[0], hasb ? x[1] : (T)0};
     }
   }
   __syncthreads();
-  cplx<T> *R = fft_lds(A, B, d.fx, TW, 1, nx);
+  cplx<T> *R = fft_run(A, B, d.fx, TW, 1, nx);
 #pragma unroll
   for (int rr = 0; rr < MP; ++rr) {
     const int k = threadIdx.x + NT * rr;
@@ -1126,7 +1159,7 @@ __device__ __forceinline__ void poisB_body(const Dev<T> &d, int bx, int nslot) {
     }
   }
   __syncthreads();
-  cplx<T> *R = fft_lds(A, B, d.fy, TW, nslot, ny);
+  cplx<T> *R = fft_run(A, B, d.fy, TW, nslot, ny);
 #pragma unroll
   for (int rr = 0; rr < MP; ++rr) {
     const int w = tid + nthr * rr;
@@ -1157,7 +1190,7 @@ __device__ __forceinline__ void poisB_body(const Dev<T> &d, int bx, int nslot) {
     pack_inverse(X + (2 * f) * ny, X + (2 * f + 1) * ny, Z + f * ny, ny, d.dct_y, csr[rr], k);
   }
   __syncthreads();
-  cplx<T> *R2 = fft_lds(Z, (Z == A) ? B : A, d.fy, TW, nslot, ny);
+  cplx<T> *R2 = fft_run(Z, (Z == A) ? B : A, d.fy, TW, nslot, ny);
   const T inv_n = (T)(1.0 / (double)ny);
   for (int w = tid; w < nslot * ny; w += nthr) {
     const int f = fdiv(w, ny, log2_or_neg(ny)), m = w - f * ny;
@@ -1225,7 +1258,7 @@ __device__ __forceinline__ void poisC_body(const Dev<T> &d, int bx, T *__restric
     if (k < nx) pack_inverse(X, X + nx, A, nx, d.dct_x, csr[r], k);
   }
   __syncthreads();
-  cplx<T> *R = fft_lds(A, B, d.fx, TW, 1, nx);
+  cplx<T> *R = fft_run(A, B, d.fx, TW, 1, nx);
   const T inv_n = (T)(1.0 / (double)nx);
   T rdt = 0;
   if (p != nullptr && stage == 2) rdt = (T)1 / d.red->dt;

@@ -140,7 +140,8 @@ struct Call3 {
   int nu_in;                 // stage input is the state the final kernel last saw: nu_t from d.nu_pad
   const T *ctrl_in;          // PID step: controller memory [dt_prev, e1, e2] of the step start
   T *ctrl_out;               // and of the step's end
-  T *tr_l[3], *tr_s[3];      // traced stage: tendencies and values before penalisation (u, v, w)
+  T *tr_lu, *tr_lv, *tr_lw;  // traced stage: tendencies (u, v, w faces)
+  T *tr_su, *tr_sv, *tr_sw;  // and values before penalisation
 };
 
 template <typename T>
@@ -293,6 +294,40 @@ __device__ __forceinline__ void load_comp(const Dev3<T> &d, const Call3<T> &c, T
   }
 }
 
+// The same values element by element (the index split per element): fewer registers than the
+// row walk, which the final kernel needs at its register budget.
+template <typename T, int C, int EX, int EY, int EZ>
+__device__ __forceinline__ void load_comp_cells(const Dev3<T> &d, const Call3<T> &c, T *lds, int k0, int j0, int i0, T corr) {
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  const int ex = nx + (C == 0), ey = ny + (C == 1), ez = nz + (C == 2);
+  const T *src = C == 0 ? c.up : (C == 1 ? c.vp : c.wp);
+  const T *psi = c.psi;
+  const int pn = C == 0 ? nx : (C == 1 ? ny : nz);         // cells along the normal axis
+  const int pper = d.per[C];
+  const int pstride = C == 0 ? 1 : (C == 1 ? nx : nx * ny);
+  const bool shift = c.corr_stage >= 0 && C == 0 && d.outlet;
+  const T inv_h = d.inv_h;
+  for (int e = threadIdx.x; e < EX * EY * EZ; e += kTileThreads) {
+    const int x = e % EX, t = e / EX, y = t % EY, z = t / EY;
+    const int i = src_axis(i0 - 2 + x, ex, nx, d.per[0]);
+    const int j = src_axis(j0 - 2 + y, ey, ny, d.per[1]);
+    const int k = src_axis(k0 - 2 + z, ez, nz, d.per[2]);
+    T val = src[(unsigned)((k * ey + j) * ex + i)];
+    if (psi) {
+      const int f = C == 0 ? i : (C == 1 ? j : k);
+      if (pper || (f > 0 && f < pn)) {
+        const int fc = f == pn ? 0 : f;
+        const int fm = fc == 0 ? pn - 1 : fc - 1;
+        const int ci = C == 0 ? 0 : i, cj = C == 1 ? 0 : j, ck = C == 2 ? 0 : k;
+        const T *pb = psi + (unsigned)((ck * ny + cj) * nx + ci);
+        val = val - (pb[(unsigned)(fc * pstride)] - pb[(unsigned)(fm * pstride)]) * inv_h;
+      }
+    }
+    if (shift && i == nx) val = val + corr;
+    lds[e] = val;
+  }
+}
+
 // Pass 2 along one axis: ghost positions of a non-periodic side from their mirror source in
 // LDS (ns3d/core.py:223-272), for every position of the other two axes. Runs after the
 // lower axes' pass, so corner values compose as pad3 does (x rule, then y, then z).
@@ -334,12 +369,18 @@ __device__ __forceinline__ void ghost_fill(const Dev3<T> &d, T *lds, int comp, i
   }
 }
 
-template <typename T>
-__device__ void load_tile(const Dev3<T> &d, const Call3<T> &c, Lds3<T> &L, int k0, int j0, int i0, T corr) {
+template <typename T, bool ROWS = true>
+__device__ __forceinline__ void load_tile(const Dev3<T> &d, const Call3<T> &c, Lds3<T> &L, int k0, int j0, int i0, T corr) {
   using S = Lds3<T>;
-  load_comp<T, 0, S::UX, S::UY, S::UZ>(d, c, &L.u[0][0][0], k0, j0, i0, corr);
-  load_comp<T, 1, S::VX, S::VY, S::VZ>(d, c, &L.v[0][0][0], k0, j0, i0, corr);
-  load_comp<T, 2, S::WX, S::WY, S::WZ>(d, c, &L.w[0][0][0], k0, j0, i0, corr);
+  if constexpr (ROWS) {
+    load_comp<T, 0, S::UX, S::UY, S::UZ>(d, c, &L.u[0][0][0], k0, j0, i0, corr);
+    load_comp<T, 1, S::VX, S::VY, S::VZ>(d, c, &L.v[0][0][0], k0, j0, i0, corr);
+    load_comp<T, 2, S::WX, S::WY, S::WZ>(d, c, &L.w[0][0][0], k0, j0, i0, corr);
+  } else {
+    load_comp_cells<T, 0, S::UX, S::UY, S::UZ>(d, c, &L.u[0][0][0], k0, j0, i0, corr);
+    load_comp_cells<T, 1, S::VX, S::VY, S::VZ>(d, c, &L.v[0][0][0], k0, j0, i0, corr);
+    load_comp_cells<T, 2, S::WX, S::WY, S::WZ>(d, c, &L.w[0][0][0], k0, j0, i0, corr);
+  }
   __syncthreads();
   // boundary tiles: ghost layers of non-periodic sides, axis by axis
   const bool bx = !d.per[0] && (i0 == 0 || i0 + S::TX + 3 > d.nx);
@@ -443,7 +484,7 @@ __device__ __forceinline__ T eddy3(int turb, T les_k, T g[3][3]) {
 // (its own cells, plus the ghost ring at domain boundaries).
 // own_only: only the cells whose maximum the tile reports (the dt input), L.nu is not filled.
 template <typename T>
-__device__ T nu_tile(const Dev3<T> &d, const Call3<T> &c, Lds3<T> &L, int k0, int j0, int i0, bool store = false,
+__device__ __forceinline__ T nu_tile(const Dev3<T> &d, const Call3<T> &c, Lds3<T> &L, int k0, int j0, int i0, bool store = false,
                      bool own_only = false) {
   using S = Lds3<T>;
   T mx = 0;
@@ -495,7 +536,7 @@ __device__ T nu_tile(const Dev3<T> &d, const Call3<T> &c, Lds3<T> &L, int k0, in
 // L.nu of a tile from d.nu_pad (the values nu_tile computes for the state the final kernel
 // last processed, tile by tile, ghost ring included); positions past the ring are unused.
 template <typename T>
-__device__ void nu_from_pad(const Dev3<T> &d, Lds3<T> &L, int k0, int j0, int i0, bool store) {
+__device__ __forceinline__ void nu_from_pad(const Dev3<T> &d, Lds3<T> &L, int k0, int j0, int i0, bool store) {
   using S = Lds3<T>;
   const int ie = min(i0 + S::TX, d.nx), je = min(j0 + S::TY, d.ny), ke = min(k0 + S::TZ, d.nz);
   for (int e = threadIdx.x; e < S::NZ * S::NY * S::NX; e += kTileThreads) {
@@ -659,8 +700,8 @@ __global__ void __launch_bounds__(kTileThreads, Tile<T>::WAVES_PER_SIMD) k_stage
     if (xb && gi == 0) uh = d.outlet ? c.u_inlet[(size_t)gk * ny + gj] : (T)0;
     if (xb && gi == nx && d.closed) uh = (T)0;
     if constexpr (TRACE) {
-      if (c.tr_l[0]) c.tr_l[0][idx] = lu;
-      if (c.tr_s[0]) c.tr_s[0][idx] = uh;
+      if (c.tr_lu) c.tr_lu[idx] = lu;
+      if (c.tr_su) c.tr_su[idx] = uh;
     }
     if (d.win && gk >= d.wk0 && gk < d.wk1 && gj >= d.wj0 && gj < d.wj1 && gi >= d.wi0 && gi <= d.wi1) {
       const size_t wi = ((size_t)(gk - d.wk0) * d.wy + (gj - d.wj0)) * (d.wx + 1) + (gi - d.wi0);
@@ -683,8 +724,8 @@ __global__ void __launch_bounds__(kTileThreads, Tile<T>::WAVES_PER_SIMD) k_stage
     const T vpv = LV(lk, lj, li);
     T vh = (s == 0 ? vpv : lerp_t(vpv, c.v0[idx], a)) + lv * bdt;
     if constexpr (TRACE) {
-      if (c.tr_l[1]) c.tr_l[1][idx] = lv;
-      if (c.tr_s[1]) c.tr_s[1][idx] = vh;
+      if (c.tr_lv) c.tr_lv[idx] = lv;
+      if (c.tr_sv) c.tr_sv[idx] = vh;
     }
     if (d.win && gk >= d.wk0 && gk < d.wk1 && gj >= d.wj0 && gj <= d.wj1 && gi >= d.wi0 && gi < d.wi1) {
       const size_t wi = ((size_t)(gk - d.wk0) * (d.wy + 1) + (gj - d.wj0)) * d.wx + (gi - d.wi0);
@@ -703,8 +744,8 @@ __global__ void __launch_bounds__(kTileThreads, Tile<T>::WAVES_PER_SIMD) k_stage
     const T wpv = LW(lk, lj, li);
     T wh = (s == 0 ? wpv : lerp_t(wpv, c.w0[idx], a)) + lw * bdt;
     if constexpr (TRACE) {
-      if (c.tr_l[2]) c.tr_l[2][idx] = lw;
-      if (c.tr_s[2]) c.tr_s[2][idx] = wh;
+      if (c.tr_lw) c.tr_lw[idx] = lw;
+      if (c.tr_sw) c.tr_sw[idx] = wh;
     }
     const bool inw = d.win && gk >= d.wk0 && gk <= d.wk1 && gj >= d.wj0 && gj < d.wj1 && gi >= d.wi0 && gi < d.wi1;
     const size_t wi = inw ? ((size_t)(gk - d.wk0) * d.wy + (gj - d.wj0)) * d.wx + (gi - d.wi0) : 0;
@@ -788,7 +829,7 @@ __global__ void __launch_bounds__(kTileThreads, Tile<T>::WAVES_PER_SIMD) k_final
   const int i0 = bx * S::TX, j0 = by * S::TY, k0 = bz * S::TZ;
   const int nx = d.nx, ny = d.ny, nz = d.nz;
   const T corr = c.corr_stage >= 0 ? d.red->corr[c.corr_stage] : (T)0;
-  load_tile(d, c, L, k0, j0, i0, corr);
+  load_tile<T, false>(d, c, L, k0, j0, i0, corr);
   T mnu = 0;
   if (d.turb) mnu = nu_tile(d, c, L, k0, j0, i0, false, true);
   const int ie = min(i0 + S::TX, nx), je = min(j0 + S::TY, ny), ke = min(k0 + S::TZ, nz);
@@ -2339,10 +2380,12 @@ int step_impl(aero3d_plan *p, hipStream_t st, const aero3d_fields *in, aero3d_fi
     if (tr) {
       // the traced step forms the step start's nu_t in the stage kernel (the same values)
       c.nu_in = 0;
-      for (int q = 0; q < 3; ++q) {
-        c.tr_l[q] = (T *)(q == 0 ? tr->tend_u : (q == 1 ? tr->tend_v : tr->tend_w))[stage];
-        c.tr_s[q] = (T *)(q == 0 ? tr->star_u : (q == 1 ? tr->star_v : tr->star_w))[stage];
-      }
+      c.tr_lu = (T *)tr->tend_u[stage];
+      c.tr_lv = (T *)tr->tend_v[stage];
+      c.tr_lw = (T *)tr->tend_w[stage];
+      c.tr_su = (T *)tr->star_u[stage];
+      c.tr_sv = (T *)tr->star_v[stage];
+      c.tr_sw = (T *)tr->star_w[stage];
       hipLaunchKernelGGL((k_stage<T, false, true>), grid, blk, lds, st, dev, c);
     } else if (c.nu_in) {
       hipLaunchKernelGGL((k_stage<T, true>), grid, blk, lds, st, dev, c);
@@ -2366,7 +2409,7 @@ int step_impl(aero3d_plan *p, hipStream_t st, const aero3d_fields *in, aero3d_fi
         hipLaunchKernelGGL(k_copy3<T>, dim3(cb), dim3(256), 0, st, (T *)tr->psi[stage], (const T *)p->psi, ncell);
       if (tr->theta[stage] && s.thermal)
         hipLaunchKernelGGL(k_copy3<T>, dim3(cb), dim3(256), 0, st, (T *)tr->theta[stage], (const T *)c.tho, ncell);
-      for (int q = 0; q < 3; ++q) c.tr_l[q] = c.tr_s[q] = nullptr;
+      c.tr_lu = c.tr_lv = c.tr_lw = c.tr_su = c.tr_sv = c.tr_sw = nullptr;
     }
   }
   c.stage = 3;

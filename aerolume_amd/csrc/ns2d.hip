@@ -616,16 +616,23 @@ __device__ T pid_step(T limit, T dt_prev, const T *ctrl_prev, T *ctrl_new) {
 // stress nu |us| / y (zero inside the viscous sublayer) acts on the face row next to the wall as
 // a force per unit volume -sign(us) excess / h. us: the face velocity relative to the wall.
 constexpr double kWwA = 8.3, kWwB = 1.0 / 7.0;
+// x^e for x >= 0: the library power in float64; in float32 the hardware base-2 logarithm and
+// exponential (a few ulp), which keeps the library routine's code out of the stage kernel.
 template <typename T>
-__device__ __forceinline__ T wall_force(const Dev<T> &d, T us) {
+__device__ __forceinline__ T wall_pow(T x, T e) { return pow(x, e); }
+template <>
+__device__ __forceinline__ float wall_pow<float>(float x, float e) {
+  return __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(x));
+}
+template <typename T>
+__device__ __forceinline__ T wall_force(T us, T nu_y, T inv_h) {
   const T mag = us < (T)0 ? -us : us;
-  const T nu_y = *d.nu / d.wall_y;
   const T lam = mag * nu_y;
-  const T ww = pow((mag * pow(nu_y, (T)kWwB)) * (T)(1.0 / kWwA), (T)(2.0 / (1.0 + kWwB)));
+  const T ww = wall_pow((mag * wall_pow(nu_y, (T)kWwB)) * (T)(1.0 / kWwA), (T)(2.0 / (1.0 + kWwB)));
   T excess = ww - lam;
   excess = excess > (T)0 ? excess : (T)0;
   const T sgn = us > (T)0 ? (T)1 : (us < (T)0 ? (T)-1 : (T)0);
-  return -(sgn * excess) * d.inv_h;
+  return -(sgn * excess) * inv_h;
 }
 
 template <typename T, int R>
@@ -730,8 +737,8 @@ __device__ __forceinline__ void stage_body(const Dev<T> &d, const Stage<T> &s, i
         }
         if (d.wallm) {
           if (j == 0 && (d.bcb == AERO_BC_NOSLIP || d.bcb == AERO_BC_MOVING))
-            lu = lu + wall_force(d, d.bcb == AERO_BC_MOVING ? U(j, i) - d.ground : U(j, i));
-          if (j == ny - 1 && d.bct == AERO_BC_NOSLIP) lu = lu + wall_force(d, U(j, i));
+            lu = lu + wall_force(d.bcb == AERO_BC_MOVING ? U(j, i) - d.ground : U(j, i), *d.nu / d.wall_y, ih);
+          if (j == ny - 1 && d.bct == AERO_BC_NOSLIP) lu = lu + wall_force(U(j, i), *d.nu / d.wall_y, ih);
         }
       }
       const int o = j * (nx + 1) + i;
@@ -781,8 +788,8 @@ __device__ __forceinline__ void stage_body(const Dev<T> &d, const Stage<T> &s, i
           lv = lv + (((nutk(j, i + 1) * shear(j, i + 1) - nutk(j, i) * shear(j, i)) + tyy_t) - tyy_b) * ih;
         }
         if (d.wallm && d.closed && (d.bcl == AERO_BC_NOSLIP || d.bcr == AERO_BC_NOSLIP)) {
-          if (i == 0 && d.bcl == AERO_BC_NOSLIP) lv = lv + wall_force(d, V(j, i));
-          if (i == nx - 1 && d.bcr == AERO_BC_NOSLIP) lv = lv + wall_force(d, V(j, i));
+          if (i == 0 && d.bcl == AERO_BC_NOSLIP) lv = lv + wall_force(V(j, i), *d.nu / d.wall_y, ih);
+          if (i == nx - 1 && d.bcr == AERO_BC_NOSLIP) lv = lv + wall_force(V(j, i), *d.nu / d.wall_y, ih);
         }
       }
       const int o = j * nx + i;

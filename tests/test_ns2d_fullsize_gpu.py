@@ -211,3 +211,34 @@ def test_sweep_64_runs_equal_single_runs_and_oracle():
         eu, ep = max(_err(st.u, cur.u), _err(st.v, cur.v)), _err(st.p, cur.p)
         print(f"sweep run {i} ({c.nx}x{c.ny}) vs oracle: uv {eu:.3e} p {ep:.3e}", flush=True)
         assert eu < 2e-6 and ep < 5e-5
+
+
+@pytest.mark.parametrize("name", ["ww_tunnel_moving", "ww_closed_room"])
+def test_wall_model_float32_matches_oracle(golden, name):
+    """The wall-model variants in float32 (whose power law runs on the hardware base-2
+    logarithm and exponential) against the float32 oracle (library powers) from the same
+    start, five steps: u, v within 2e-6 of the velocity scale."""
+    _need_gpu()
+    import oracle.ns2d as O
+    from aerolume_amd.ns2d import setup as su
+    from aerolume_amd.ns2d import solver as S
+    F = golden("ns2d_variants.npz")
+    cfg = cases.variant_cfg(name, dtype="float32", device="cuda")
+    ops = S.make_operators(cfg, device=DEV)
+    geom = su.build_geometry(cfg.bodies, ops)
+    cfg_c = cases.variant_cfg(name, dtype="float32")
+    ops_c = su.make_operators(cfg_c, device="cpu")
+    geom_c = su.build_geometry(cfg_c.bodies, ops_c)
+    t = lambda a, d: torch.as_tensor(a, device=d, dtype=torch.float32)  # noqa: E731
+    z = lambda d: torch.zeros((), device=d, dtype=torch.float32)  # noqa: E731
+    cur = S.State(t(F[f"{name}:u0"], DEV), t(F[f"{name}:v0"], DEV), torch.zeros(ops.ny, ops.nx, device=DEV), None,
+                  z(DEV), z(DEV))
+    ref = O.State(t(F[f"{name}:u0"], "cpu"), t(F[f"{name}:v0"], "cpu"), torch.zeros(ops.ny, ops.nx), None, z("cpu"),
+                  z("cpu"))
+    for _ in range(5):
+        cur, _ = S.step_full(cur, geom, ops)
+        ref, _ = O.step_full(ref, geom_c, ops_c)
+    scale = max(float(ref.u.abs().max()), 1.0)
+    eu = max(_err(cur.u, ref.u.numpy()), _err(cur.v, ref.v.numpy()))
+    print(f"wall model float32 {name}: {eu:.3e}", flush=True)
+    assert eu < 2e-6 * scale

@@ -1581,7 +1581,11 @@ __device__ __forceinline__ void finalize_body(const Dev<T> &d, const T *__restri
   }
   // wave reductions, one barrier, then one thread per quantity adds the waves in order
   constexpr int kW = kMaxWin + 5 + 6;   // window sums, dt maxima + sum theta, wall sums
-  for (int q = 0; q < kMaxWin; ++q) acc[q] = wave_sum(acc[q]);
+  // only the slots in use (4 per body, 1 per temperature or heat-flux body) are reduced
+  const int nq = 4 * d.nbody + (d.thermal ? d.n_temp + (d.win ? d.n_flux : 0) : 0);
+#pragma unroll
+  for (int q = 0; q < kMaxWin; ++q)
+    if (q < nq) acc[q] = wave_sum(acc[q]);
   for (int q = 0; q < 4; ++q) pm[q] = wave_max(pm[q]);
   psum = wave_sum(psum);
   if (d.thermal)
@@ -1638,40 +1642,34 @@ __device__ __forceinline__ void finalize_body(const Dev<T> &d, const T *__restri
   }
   if (d.thermal) {
     // row layout: heat of the temperature bodies, skin temperature of the heat-flux bodies,
-    // wall heat (4), storage, advective in / out
+    // wall heat (4), storage, advective in / out. Each value goes straight to its outputs (a
+    // local array indexed by the body counts would live in scratch memory).
     const int nt = d.n_temp, nf = d.win ? d.n_flux : 0, w0 = nt + nf;
     const T h2dt = (T)(d.hd * d.hd) / dt;
-    T vals[kMaxWin + 8];
-    for (int b = 0; b < nt; ++b) vals[b] = (T)res[4 * d.nbody + b] * h2dt;
-    for (int b = 0; b < nf; ++b) vals[nt + b] = (T)res[4 * d.nbody + nt + b];
+    T *arow = nullptr;
+    if (o.aux) arow = reinterpret_cast<T *>(o.aux) + (aux_n % o.aux_cap) * (1 + w0 + 7) + 1;
+    auto put = [&](int col, T val, void *dst, int at) {
+      if (dst) reinterpret_cast<T *>(dst)[at] = val;
+      if (arow) arow[col] = val;
+    };
+    for (int b = 0; b < nt; ++b) put(b, (T)res[4 * d.nbody + b] * h2dt, o.heat, b);
+    for (int b = 0; b < nf; ++b) put(nt + b, (T)res[4 * d.nbody + nt + b], o.theta_wall, b);
     const int len[4] = {ny, ny, nx, nx};
+#pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int m = d.wm[q];
       T val = 0;
       if (m == AERO_TH_TEMPERATURE || m == AERO_TH_INLET) val = ((T)2 * kap) * (T)ws[q];
       else if (m == AERO_TH_HEAT_FLUX) val = kap * wv[q] * (T)(len[q] * d.hd);
-      vals[w0 + q] = val;
+      put(w0 + q, val, o.wall_heat, q);
     }
-    vals[w0 + 4] = (T)(mx[4] - th_old) * h2dt;
+    put(w0 + 4, (T)(mx[4] - th_old) * h2dt, o.heat_storage, 0);
     const bool tunnel = !d.periodic && !d.closed;
-    vals[w0 + 5] = tunnel ? (T)ws[4] * (T)d.hd : (T)0;
-    vals[w0 + 6] = tunnel ? (T)ws[5] * (T)d.hd : (T)0;
-    if (o.heat) for (int b = 0; b < nt; ++b) reinterpret_cast<T *>(o.heat)[b] = vals[b];
-    if (o.theta_wall) for (int b = 0; b < nf; ++b) reinterpret_cast<T *>(o.theta_wall)[b] = vals[nt + b];
-    if (o.wall_heat) for (int q = 0; q < 4; ++q) reinterpret_cast<T *>(o.wall_heat)[q] = vals[w0 + q];
-    if (o.heat_storage) reinterpret_cast<T *>(o.heat_storage)[0] = vals[w0 + 4];
-    if (o.flow_heat) {
-      reinterpret_cast<T *>(o.flow_heat)[0] = vals[w0 + 5];
-      reinterpret_cast<T *>(o.flow_heat)[1] = vals[w0 + 6];
-    }
-    if (o.aux) {
-      long long *cnt = reinterpret_cast<long long *>(o.aux_count);
-      const long long row = aux_n % o.aux_cap;
-      const int ncol = 1 + w0 + 7;
-      T *arow = reinterpret_cast<T *>(o.aux) + row * ncol;
-      arow[0] = tnew;
-      for (int q = 0; q < w0 + 7; ++q) arow[1 + q] = vals[q];
-      *cnt = aux_n + 1;
+    put(w0 + 5, tunnel ? (T)ws[4] * (T)d.hd : (T)0, o.flow_heat, 0);
+    put(w0 + 6, tunnel ? (T)ws[5] * (T)d.hd : (T)0, o.flow_heat, 1);
+    if (arow) {
+      arow[-1] = tnew;
+      *reinterpret_cast<long long *>(o.aux_count) = aux_n + 1;
     }
   }
   // the next step (chained) starts from this state

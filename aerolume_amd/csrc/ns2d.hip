@@ -457,19 +457,22 @@ __device__ __forceinline__ T van_leer(T a, T b) {
 }
 
 // MUSCL / van Leer flux at the x-face between tile cells (r, c-1) and (r, c) (scalar.py:63-83)
+// Only the upwind side's reconstruction is formed: the slopes are picked first, so each face
+// costs one limiter (one division) and gives the value the two-sided form would select.
+template <typename T>
+__device__ __forceinline__ T tvd_face(T cm2, T cm1, T c0, T c1, T vel) {
+  const bool up = vel > (T)0;
+  const T base = up ? cm1 : c0;
+  const T lim = van_leer(up ? cm1 - cm2 : c0 - cm1, up ? c0 - cm1 : c1 - c0);
+  return vel * (up ? base + (T)0.5 * lim : base - (T)0.5 * lim);
+}
 template <typename T, int R>
 __device__ __forceinline__ T tvd_x(const Lds<T, R> &L, int r, int c, T uf) {
-  const T cm2 = L.c[r][c - 2], cm1 = L.c[r][c - 1], c0 = L.c[r][c], c1 = L.c[r][c + 1];
-  const T left = cm1 + (T)0.5 * van_leer(cm1 - cm2, c0 - cm1);
-  const T right = c0 - (T)0.5 * van_leer(c0 - cm1, c1 - c0);
-  return uf * (uf > (T)0 ? left : right);
+  return tvd_face(L.c[r][c - 2], L.c[r][c - 1], L.c[r][c], L.c[r][c + 1], uf);
 }
 template <typename T, int R>
 __device__ __forceinline__ T tvd_y(const Lds<T, R> &L, int r, int c, T vf) {
-  const T cm2 = L.c[r - 2][c], cm1 = L.c[r - 1][c], c0 = L.c[r][c], c1 = L.c[r + 1][c];
-  const T low = cm1 + (T)0.5 * van_leer(cm1 - cm2, c0 - cm1);
-  const T high = c0 - (T)0.5 * van_leer(c0 - cm1, c1 - c0);
-  return vf * (vf > (T)0 ? low : high);
+  return tvd_face(L.c[r - 2][c], L.c[r - 1][c], L.c[r][c], L.c[r + 1][c], vf);
 }
 
 // temperature diffusivity at x-face f of row j / y-face f of column i (physics.py:713-724)

@@ -325,3 +325,12 @@ def test_traced_step_matches_reference_trace():
     assert fs.steps == 1 and len(fs.history()["t"]) == 1
     assert float((tr2["stage3/projected_u"] - tr["stage3/projected_u"]).abs().max()) < 1e-13
     assert float((fs.state.u - new.u).abs().max()) < 1e-13
+    # the committed step continues like a plain run: rows, time and state after two more steps
+    fs.advance(2)
+    plain = S.FlowSolver(dataclasses.replace(cfg, use_cuda_graphs=False))
+    plain.set_velocity(F["u0"], F["v0"])
+    plain.advance(3)
+    h, hp = fs.history(), plain.history()
+    assert len(h["t"]) == 3 and np.all(np.diff(h["t"]) > 0)
+    assert np.abs(h["t"] - hp["t"]).max() < 1e-13 and np.abs(h["cd"] - hp["cd"]).max() < 1e-11
+    assert float((fs.state.u - plain.state.u).abs().max()) < 1e-13

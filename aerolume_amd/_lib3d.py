@@ -61,6 +61,15 @@ class Trace3D(ctypes.Structure):
     _fields_ = [(k, vp * 3) for k in TRACE_FIELDS]
 
 
+class Shape3D(ctypes.Structure):
+    """aero3d_shape: an analytic body (kind 0 rounded box, 1 cylinder along y) and its seven
+    parameters."""
+    _fields_ = [("kind", ctypes.c_int32), ("reserved", ctypes.c_int32), ("p", ctypes.c_double * 7)]
+
+
+#: shape kinds (AERO3D_SHAPE_*)
+SHAPE_KINDS = {"rounded_box": 0, "cylinder_y": 1}
+
 #: columns of a diagnostics row (AERO3D_DIAG_COLS)
 DIAG_COLS = 28
 
@@ -81,6 +90,10 @@ def bind(lib) -> None:
     lib.aero3d_time_stage.argtypes = [P, P, ctypes.POINTER(Fields3D), ctypes.POINTER(Geom3D),
                                       ctypes.POINTER(Params3D), ctypes.c_int, ctypes.c_int,
                                       ctypes.POINTER(ctypes.c_double)]
+    lib.aero3d_analytic_fractions.argtypes = [P, ctypes.POINTER(Shape3D), ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
+                                              ctypes.c_double, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                              ctypes.c_double, ctypes.c_double, P]
+    lib.aero3d_analytic_fractions.restype = ctypes.c_int
     for name in ("aero3d_plan_create", "aero3d_plan_destroy", "aero3d_step", "aero3d_step_chained", "aero3d_step_trace",
                  "aero3d_project", "aero3d_poisson_solve", "aero3d_time_stage", "aero3d_cell_gradient"):
         getattr(lib, name).restype = ctypes.c_int

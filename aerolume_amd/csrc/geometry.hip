@@ -12,6 +12,7 @@
 // One thread per point; the polygon is staged through LDS in chunks of kChunk vertices.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <string>
@@ -93,6 +94,55 @@ __global__ void __launch_bounds__(kThreads) k_polygon_sdf(const double *__restri
   out[k] = inside ? -d : d;
 }
 
+// Solid fractions of analytic 3D bodies on one staggered grid of the mask window
+// (ns3d/geometry.py:385-495 builds the masks from the signed distance the same way): per sample
+// point (x0 + i h, y0 + j h, z0 + k h) and body, smooth_step(sdf + shift, width) =
+// 1/2 - 1/2 sin(pi/2 clamp((sdf + shift) / width, -1, 1)), all in float64 with the host
+// formulas' operation order (contraction off): a rounded box (centre, half extents less the
+// edge radius, radius) or a cylinder along y (centre, radius, half width).
+constexpr int kMaxShapes = 16;
+struct Shapes3 {
+  int n;
+  int kind[kMaxShapes];
+  double p[kMaxShapes][7];
+};
+
+__device__ __forceinline__ double shape_sdf(int kind, const double *p, double x, double y, double z) {
+  if (kind == AERO3D_SHAPE_ROUNDED_BOX) {
+    const double qx = fabs(x - p[0]) - p[3], qy = fabs(y - p[1]) - p[4], qz = fabs(z - p[2]) - p[5];
+    const double ox = qx > 0.0 ? qx : 0.0, oy = qy > 0.0 ? qy : 0.0, oz = qz > 0.0 ? qz : 0.0;
+    const double out = sqrt(ox * ox + oy * oy + oz * oz);
+    double m = qx > qy ? qx : qy;
+    m = m > qz ? m : qz;
+    const double inside = m < 0.0 ? m : 0.0;
+    return out + inside - p[6];
+  }
+  const double dx = x - p[0], dz = z - p[2];
+  const double dr = sqrt(dx * dx + dz * dz) - p[3];
+  const double da = fabs(y - p[1]) - p[4];
+  const double orr = dr > 0.0 ? dr : 0.0, oa = da > 0.0 ? da : 0.0;
+  const double out = sqrt(orr * orr + oa * oa);
+  const double m = dr > da ? dr : da;
+  return out + (m < 0.0 ? m : 0.0);
+}
+
+__global__ void __launch_bounds__(kThreads) k_analytic_fraction(Shapes3 sh, double x0, double y0, double z0, double h,
+                                                                int nx, int ny, int nz, double shift, double width,
+                                                                double *__restrict__ out) {
+  const long long per = (long long)nx * ny * nz, total = per * sh.n;
+  const double half_pi = 0.5 * M_PI;
+  for (long long e = (long long)blockIdx.x * kThreads + threadIdx.x; e < total;
+       e += (long long)gridDim.x * kThreads) {
+    const int b = (int)(e / per);
+    const long long r = e - (long long)b * per;
+    const int k = (int)(r / ((long long)nx * ny)), j = (int)((r / nx) % ny), i = (int)(r % nx);
+    const double x = x0 + (double)i * h, y = y0 + (double)j * h, z = z0 + (double)k * h;
+    double s = (shape_sdf(sh.kind[b], sh.p[b], x, y, z) + shift) / width;
+    s = s < -1.0 ? -1.0 : (s > 1.0 ? 1.0 : s);
+    out[e] = 0.5 - 0.5 * sin(half_pi * s);
+  }
+}
+
 }  // namespace geo
 }  // namespace aero
 
@@ -112,6 +162,34 @@ extern "C" int aero_polygon_sdf(void *stream, int32_t dtype, const double *poly,
   else
     hipLaunchKernelGGL(geo::k_polygon_sdf<float>, dim3((unsigned)blocks), dim3(geo::kThreads), 0, st, poly, n,
                        (const float *)x, (const float *)y, (long long)npts, (float *)out);
+  AERO_HIP_CHECK(hipGetLastError());
+  return AERO_OK;
+}
+
+extern "C" int aero3d_analytic_fractions(void *stream, const aero3d_shape *shapes, int32_t nshapes,
+                                         const double *origin, double h, int32_t nx, int32_t ny, int32_t nz,
+                                         double shift, double width, void *out) {
+  using namespace aero;
+  if (!shapes || !origin || !out || nshapes < 1 || nshapes > geo::kMaxShapes || nx < 1 || ny < 1 || nz < 1 ||
+      !(h > 0) || !(width > 0)) {
+    set_error("aero3d_analytic_fractions: 1 .. 16 shapes, a positive spacing and width, a non-empty grid");
+    return AERO_EINVAL;
+  }
+  geo::Shapes3 sh;
+  sh.n = nshapes;
+  for (int b = 0; b < nshapes; ++b) {
+    const int k = shapes[b].kind;
+    if (k != AERO3D_SHAPE_ROUNDED_BOX && k != AERO3D_SHAPE_CYLINDER_Y) {
+      set_error("aero3d_analytic_fractions: unknown shape kind");
+      return AERO_EINVAL;
+    }
+    sh.kind[b] = k;
+    for (int q = 0; q < 7; ++q) sh.p[b][q] = shapes[b].p[q];
+  }
+  const long long total = (long long)nx * ny * nz * nshapes;
+  const long long blocks = std::min<long long>((total + geo::kThreads - 1) / geo::kThreads, 65536);
+  hipLaunchKernelGGL(geo::k_analytic_fraction, dim3((unsigned)blocks), dim3(geo::kThreads), 0, (hipStream_t)stream, sh,
+                     origin[0], origin[1], origin[2], h, (int)nx, (int)ny, (int)nz, shift, width, (double *)out);
   AERO_HIP_CHECK(hipGetLastError());
   return AERO_OK;
 }

@@ -196,6 +196,7 @@ def rounded_box_sdf(center, size, radius):
         out = torch.sqrt(qx.clamp_min(0) ** 2 + qy.clamp_min(0) ** 2 + qz.clamp_min(0) ** 2)
         inside = torch.maximum(torch.maximum(qx, qy), qz).clamp_max(0.0)
         return out + inside - radius
+    sdf.shape = ("rounded_box", (float(cx), float(cy), float(cz), float(hx), float(hy), float(hz), float(radius)))
     return sdf
 
 
@@ -208,6 +209,7 @@ def cylinder_y_sdf(center, radius, width):
         da = (y - cy).abs() - 0.5 * width
         out = torch.sqrt(dr.clamp_min(0) ** 2 + da.clamp_min(0) ** 2)
         return out + torch.maximum(dr, da).clamp_max(0.0)
+    sdf.shape = ("cylinder_y", (float(cx), float(cy), float(cz), float(radius), float(0.5 * width), 0.0, 0.0))
     return sdf
 
 
@@ -380,9 +382,48 @@ def empty_geometry(nb: int, device=None, dtype=torch.float64) -> Geometry3D:
     return Geometry3D((0, 0, 0, 0, 0, 0), z, z, z, z, z, z, z, k, k, k, nb)
 
 
+def _analytic_shapes(bodies):
+    """``(kind, params)`` of every body when all are analytic shapes the HIP fraction kernel
+    knows (the rounded box and the y cylinder above), else None."""
+    out = []
+    for b in bodies:
+        shape = getattr(b.sdf, "shape", None) if isinstance(b, AnalyticBody) else None
+        if shape is None:
+            return None
+        out.append(shape)
+    return out
+
+
+def _hip_fractions(shapes, start, n, h, shift, width, device) -> torch.Tensor:
+    """Solid fractions (nbody, nz, ny, nx) float64 of analytic shapes on the grid whose first
+    sample is ``start`` = (z, y, x) and counts ``n`` = (nz, ny, nx), by the HIP kernel
+    ``aero3d_analytic_fractions`` (the host builder's float64 formulas, at most 16 bodies a
+    launch)."""
+    import ctypes
+
+    from .. import _lib
+    from .._lib3d import SHAPE_KINDS, Shape3D
+    lib = _lib.load()
+    nz, ny, nx = (int(v) for v in n)
+    out = torch.empty(len(shapes), nz, ny, nx, dtype=torch.float64, device=device)
+    origin = (ctypes.c_double * 3)(float(start[2]), float(start[1]), float(start[0]))
+    stream = ctypes.c_void_p(_lib.stream_handle(device))
+    for b0 in range(0, len(shapes), 16):
+        part = shapes[b0:b0 + 16]
+        arr = (Shape3D * len(part))()
+        for s, (kind, params) in zip(arr, part):
+            s.kind = SHAPE_KINDS[kind]
+            for q, val in enumerate(params):
+                s.p[q] = float(val)
+        _lib.check(lib.aero3d_analytic_fractions(stream, arr, len(part), origin, float(h), nx, ny, nz, float(shift),
+                                                 float(width), out[b0].data_ptr()), "aero3d_analytic_fractions")
+    return out
+
+
 def build_geometry3d(bodies, ops: Ops3D, *, window=None) -> Geometry3D:
-    """Masks, solid velocities and force weights on the window, built in float64 on the host
-    and returned in the solver's dtype on its device (semantics of the reference's
+    """Masks, solid velocities and force weights on the window, built in float64 (on a GPU
+    device the fractions of analytic shapes come from the HIP kernel and the rest runs there;
+    otherwise on the host) and returned in the solver's dtype on its device (semantics of the reference's
     ``ns3d/geometry.py:385-508``: per-body fraction ``smooth_step(d + 0.32 h, mask_width h)``
     on each staggered grid, zero on domain-boundary faces, union clamped at 1; rigid rotation
     pre-multiplied by each body's fraction; each body's share of a face where bodies overlap)."""
@@ -399,24 +440,29 @@ def build_geometry3d(bodies, ops: Ops3D, *, window=None) -> Geometry3D:
     w = ops.mask_width * h
     shift = ops.mask_shift * h
     X0, Y0, Z0 = ops.x0 + i0 * h, ops.y0 + j0 * h, ops.z0 + k0 * h
+    shapes = _analytic_shapes(bodies) if ops.device.type == "cuda" else None
+    dev = ops.device if shapes is not None else torch.device("cpu")
 
-    def axis(n, start):
-        return start + torch.arange(n, dtype=f64) * h
-
-    # sample positions of each staggered grid: (z, y, x) 1D coordinate vectors
-    grids = {
-        "u": (axis(wz, Z0 + 0.5 * h), axis(wy, Y0 + 0.5 * h), axis(wx + 1, X0)),
-        "v": (axis(wz, Z0 + 0.5 * h), axis(wy + 1, Y0), axis(wx, X0 + 0.5 * h)),
-        "w": (axis(wz + 1, Z0), axis(wy, Y0 + 0.5 * h), axis(wx, X0 + 0.5 * h)),
-        "c": (axis(wz, Z0 + 0.5 * h), axis(wy, Y0 + 0.5 * h), axis(wx, X0 + 0.5 * h)),
+    # first sample (z, y, x) and point counts (z, y, x) of each staggered grid
+    starts = {
+        "u": ((Z0 + 0.5 * h, Y0 + 0.5 * h, X0), (wz, wy, wx + 1)),
+        "v": ((Z0 + 0.5 * h, Y0, X0 + 0.5 * h), (wz, wy + 1, wx)),
+        "w": ((Z0, Y0 + 0.5 * h, X0 + 0.5 * h), (wz + 1, wy, wx)),
+        "c": ((Z0 + 0.5 * h, Y0 + 0.5 * h, X0 + 0.5 * h), (wz, wy, wx)),
     }
-    chis = {k: [] for k in grids}
-    for b in bodies:
-        sdf = body_sdf(b)
-        for key, (zs, ys, xs) in grids.items():
-            zz, yy, xx = torch.meshgrid(zs, ys, xs, indexing="ij")
-            chis[key].append(smooth_step(sdf(xx, yy, zz) + shift, w))
-    cu, cv, cw, cc = (torch.stack(chis[k]) for k in ("u", "v", "w", "c"))
+    grids = {k: tuple(s0 + torch.arange(n, dtype=f64, device=dev) * h for s0, n in zip(*v))
+             for k, v in starts.items()}
+    if shapes is not None:
+        chis = {k: _hip_fractions(shapes, start, n, h, shift, w, dev) for k, (start, n) in starts.items()}
+    else:
+        chis = {k: [] for k in grids}
+        for b in bodies:
+            sdf = body_sdf(b)
+            for key, (zs, ys, xs) in grids.items():
+                zz, yy, xx = torch.meshgrid(zs, ys, xs, indexing="ij")
+                chis[key].append(smooth_step(sdf(xx, yy, zz) + shift, w))
+        chis = {k: torch.stack(v) for k, v in chis.items()}
+    cu, cv, cw, cc = (chis[k] for k in ("u", "v", "w", "c"))
     px, py, pz = ops.periodic_axes
     if not px:
         if i0 == 0:

@@ -376,6 +376,23 @@ int aero3d_time_stage(aero3d_plan *plan, void *stream, const aero3d_fields *fiel
 int aero_polygon_sdf(void *stream, int32_t dtype, const double *poly, int32_t n, const void *x, const void *y,
                      int64_t npts, void *out);
 
+/* An analytic 3D body (ns3d/geometry.py:385-495 samples such signed distances): kind 0 a
+ * rounded box, p = centre (3), half extents less the edge radius (3), edge radius; kind 1 a
+ * cylinder with its axis along y, p = centre (3), radius, half width (p[5], p[6] unused). */
+#define AERO3D_SHAPE_ROUNDED_BOX 0
+#define AERO3D_SHAPE_CYLINDER_Y 1
+typedef struct {
+  int32_t kind, reserved;
+  double p[7];
+} aero3d_shape;
+/* Solid fractions of nshapes (1 .. 16) analytic bodies on one staggered grid, on `stream`:
+ * out (nshapes, nz, ny, nx) float64 with out[b, k, j, i] = smooth_step(sdf_b(origin + (i, j, k)
+ * h) + shift, width), smooth_step(d, w) = 1/2 - 1/2 sin(pi/2 clamp(d / w, -1, 1)), evaluated in
+ * float64 in the host builder's operation order. Replaces the per-body meshgrid evaluation of
+ * the 3D mask builder (ns3d/geometry.py:385-495) for analytic bodies. */
+int aero3d_analytic_fractions(void *stream, const aero3d_shape *shapes, int32_t nshapes, const double *origin,
+                              double h, int32_t nx, int32_t ny, int32_t nz, double shift, double width, void *out);
+
 /* ------------------------------------------------------------------ 2D tracer particles */
 /* The grid the velocities live on: u (ny, nx+1), v (ny+1, nx), origin (x0, y0), cell h. */
 typedef struct {

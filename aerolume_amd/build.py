@@ -16,8 +16,11 @@ CSRC = HERE / "csrc"
 SOURCES = ("ns2d.hip", "ns3d.hip", "cnn3d.hip", "geometry.hip", "particles.hip")
 LIB = HERE / "libaerolume_amd.so"
 ARCH = os.environ.get("AEROLUME_AMD_ARCH", "gfx950")
-FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-         "-Wno-unused-command-line-argument"]
+# -ffp-contract=on: a * b + c becomes one fused multiply-add only inside one source expression, so
+# a kernel body gives the same bits whatever code surrounds it (the plan views of the tile
+# kernels and the batched ensemble's generic kernels share bodies and must agree bit for bit)
+FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=on", "-Xclang", "-target-feature", "-Xclang",
+         "-packed-fp32-ops", "-Wno-unused-command-line-argument"]
 
 
 def hipcc() -> str:

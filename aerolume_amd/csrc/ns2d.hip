@@ -108,6 +108,33 @@ struct Dev {
   int dct_x, dct_y;   // 1: Neumann (DCT-II), 0: periodic (DHT)
 };
 
+
+// Compile-time views of a plan for the tile kernels. DevK<T, LES, TH, PLAIN> is a Dev<T> whose
+// static members hide some of the plan's run-time flags: LES fixes the eddy-viscosity model
+// (0 none, 1 Smagorinsky, 2 WALE, 3 Vreman), TH heat transfer (0 off, 1 on), and PLAIN = 1 a
+// wind tunnel (inflow / outflow columns) with upwind3 advection and none of the rarer features
+// (wall model, heat-flux skins, heaters); LES = -1, TH = -1 or PLAIN = 0 leave
+// the flag to the plan. The tile kernels are written once against the members of a Dev, so
+// each view compiles to a kernel with only the code, plan fields and registers of its case.
+template <typename T, int LES>
+struct DevL : Dev<T> {
+  static constexpr int turb = LES, les = LES != 0;
+};
+template <typename T>
+struct DevL<T, -1> : Dev<T> {};
+template <typename T, int LES, int TH>
+struct DevLT : DevL<T, LES> {
+  static constexpr int thermal = TH;
+};
+template <typename T, int LES>
+struct DevLT<T, LES, -1> : DevL<T, LES> {};
+template <typename T, int LES, int TH, int PLAIN>
+struct DevK : DevLT<T, LES, TH> {
+  static constexpr int periodic = 0, closed = 0, upwind = 1, wallm = 0, n_flux = 0, has_heat = 0;
+};
+template <typename T, int LES, int TH>
+struct DevK<T, LES, TH, 0> : DevLT<T, LES, TH> {};
+
 template <typename T>
 struct Stage {
   int s;
@@ -200,8 +227,8 @@ __device__ __forceinline__ T finish_v(const Dev<T> &d, T x, T q1, T q0, int flag
 // periodic wrap): each lane works out the column part of its tile columns once (lane c and
 // c + 64), each wave the row part of its rows, and every element then costs its loads and
 // finish_u / finish_v.
-template <typename T, int R>
-__device__ __forceinline__ void load_uv(const Dev<T> &d, Lds<T, R> &L, int tj0, int ti0, const T *u, const T *v, const T *psi, T corr) {
+template <typename T, int R, typename D>
+__device__ __forceinline__ void load_uv(const D &d, Lds<T, R> &L, int tj0, int ti0, const T *u, const T *v, const T *psi, T corr) {
   const int nx = d.nx, ny = d.ny;
   const bool hp = psi != nullptr;
   const int lane = threadIdx.x, wave = threadIdx.y;
@@ -334,8 +361,8 @@ __device__ __forceinline__ void load_uv(const Dev<T> &d, Lds<T, R> &L, int tj0, 
 // scalar tile with the TVD ghost rules of scalar.py:42-60: inflow (or replicate) on the left,
 // replicate elsewhere, wrap when periodic; use_inflow 2 is the smoke comb's rule (inflow also
 // in a closed box, core.py:718-723)
-template <typename T, int R>
-__device__ __forceinline__ void load_c(const Dev<T> &d, Lds<T, R> &L, int tj0, int ti0, const T *c, const T *inflow_col, T inflow0,
+template <typename T, int R, typename D>
+__device__ __forceinline__ void load_c(const D &d, Lds<T, R> &L, int tj0, int ti0, const T *c, const T *inflow_col, T inflow0,
                        int use_inflow) {
   const int nx = d.nx, ny = d.ny;
   const int lane = threadIdx.x, wave = threadIdx.y;
@@ -379,8 +406,8 @@ __device__ __forceinline__ void load_c(const Dev<T> &d, Lds<T, R> &L, int tj0, i
 }
 
 // Eddy viscosity of core.py:444-478 from the gradient tensor.
-template <typename T>
-__device__ __forceinline__ T eddy(const Dev<T> &d, T g11, T g12, T g21, T g22) {
+template <typename T, typename D>
+__device__ __forceinline__ T eddy(const D &d, T g11, T g12, T g21, T g22) {
   const T s12 = (T)0.5 * (g12 + g21);
   const T ss = (g11 * g11 + g22 * g22) + (T)2 * s12 * s12;
   if (d.turb == 1) return d.les_k2 * sqrt((T)2 * ss + (T)1e-30);
@@ -406,8 +433,8 @@ __device__ __forceinline__ T eddy(const Dev<T> &d, T g11, T g12, T g21, T g22) {
 }
 
 // nu_t on the tile's cells and their ring (core.py:421-503), switched off in solids.
-template <typename T, int R>
-__device__ __forceinline__ void tile_nut(const Dev<T> &d, Lds<T, R> &L, int tj0, int ti0) {
+template <bool WIN = true, typename T, int R, typename D>
+__device__ __forceinline__ void tile_nut(const D &d, Lds<T, R> &L, int tj0, int ti0) {
   const int tid = threadIdx.x + threadIdx.y * blockDim.x;
   const T ih = d.inv_h;
   for (int e = tid; e < TR<R>::NH * NW; e += 256) {
@@ -427,7 +454,7 @@ __device__ __forceinline__ void tile_nut(const Dev<T> &d, Lds<T, R> &L, int tj0,
     const T g12 = (((du00 + du10) + du01) + du11) * (T)0.25;
     const T g21 = (((dv00 + dv10) + dv01) + dv11) * (T)0.25;
     T val = eddy(d, g11, g12, g21, g22);
-    if (d.win && j >= d.j0 && j < d.j1 && cc >= d.i0 && cc < d.i1)
+    if (WIN && d.win && j >= d.j0 && j < d.j1 && cc >= d.i0 && cc < d.i1)
       val = val * ((T)1 - d.chi_c[(j - d.j0) * d.wx + (cc - d.i0)]);
     L.nu[r][c] = val;
   }
@@ -476,8 +503,8 @@ __device__ __forceinline__ T tvd_y(const Lds<T, R> &L, int r, int c, T vf) {
 }
 
 // temperature diffusivity at x-face f of row j / y-face f of column i (physics.py:713-724)
-template <typename T, int R>
-__device__ __forceinline__ T kappa_x(const Dev<T> &d, const Lds<T, R> &L, int tj0, int ti0, T kap, int j, int f) {
+template <bool WIN = true, typename T, int R, typename D>
+__device__ __forceinline__ T kappa_x(const D &d, const Lds<T, R> &L, int tj0, int ti0, T kap, int j, int f) {
   T k = kap;
   if (d.les) {
     const T ip = d.inv_prt;
@@ -494,12 +521,12 @@ __device__ __forceinline__ T kappa_x(const Dev<T> &d, const Lds<T, R> &L, int tj
       fa = (T)0.5 * (N(f) * ip + N(f - 1) * ip);
     k = kap + fa;
   }
-  if (d.insulated && d.win && j >= d.j0 && j < d.j1 && f >= d.i0 && f <= d.i1)
+  if (WIN && d.insulated && d.win && j >= d.j0 && j < d.j1 && f >= d.i0 && f <= d.i1)
     k = k * d.ins_u[(j - d.j0) * (d.wx + 1) + (f - d.i0)];
   return k;
 }
-template <typename T, int R>
-__device__ __forceinline__ T kappa_y(const Dev<T> &d, const Lds<T, R> &L, int tj0, int ti0, T kap, int f, int i) {
+template <bool WIN = true, typename T, int R, typename D>
+__device__ __forceinline__ T kappa_y(const D &d, const Lds<T, R> &L, int tj0, int ti0, T kap, int f, int i) {
   T k = kap;
   if (d.les) {
     const T ip = d.inv_prt;
@@ -516,7 +543,7 @@ __device__ __forceinline__ T kappa_y(const Dev<T> &d, const Lds<T, R> &L, int tj
       fa = (T)0.5 * (N(f) * ip + N(f - 1) * ip);
     k = kap + fa;
   }
-  if (d.insulated && d.win && f >= d.j0 && f <= d.j1 && i >= d.i0 && i < d.i1)
+  if (WIN && d.insulated && d.win && f >= d.j0 && f <= d.j1 && i >= d.i0 && i < d.i1)
     k = k * d.ins_v[(f - d.j0) * d.wx + (i - d.i0)];
   return k;
 }
@@ -638,8 +665,8 @@ __device__ __forceinline__ T wall_force(T us, T nu_y, T inv_h) {
   return -(sgn * excess) * inv_h;
 }
 
-template <typename T, int R>
-__device__ __forceinline__ void stage_body(const Dev<T> &d, const Stage<T> &s, int bx, int by) {
+template <typename T, int R, bool WIN = true, typename D>
+__device__ __forceinline__ void stage_body(const D &d, const Stage<T> &s, int bx, int by) {
   __shared__ Lds<T, R> L;
   const int tj0 = by * R, ti0 = bx * TX;
   const int nx = d.nx, ny = d.ny;
@@ -648,7 +675,7 @@ __device__ __forceinline__ void stage_body(const Dev<T> &d, const Stage<T> &s, i
   if (d.thermal) load_c(d, L, tj0, ti0, s.thp, (const T *)nullptr, *d.theta_in, 1);
   __syncthreads();
   if (d.les) {
-    tile_nut(d, L, tj0, ti0);
+    tile_nut<WIN>(d, L, tj0, ti0);
     __syncthreads();
   }
   const T dt = d.red->dt;
@@ -705,8 +732,8 @@ __device__ __forceinline__ void stage_body(const Dev<T> &d, const Stage<T> &s, i
     d.acc_v[wo] = acc;
     return vn;
   };
-  auto in_win_u = [&](int j, int i) { return d.win && j >= d.j0 && j < d.j1 && i >= d.i0 && i <= d.i1; };
-  auto in_win_v = [&](int j, int i) { return d.win && j >= d.j0 && j <= d.j1 && i >= d.i0 && i < d.i1; };
+  auto in_win_u = [&](int j, int i) { return WIN && d.win && j >= d.j0 && j < d.j1 && i >= d.i0 && i <= d.i1; };
+  auto in_win_v = [&](int j, int i) { return WIN && d.win && j >= d.j0 && j <= d.j1 && i >= d.i0 && i < d.i1; };
   auto theta_fx = [&](int j, int f) -> T {   // face average of theta (zero on boundary faces)
     if (f == 0 || f == nx) return d.periodic ? (T)0.5 * (C(j, 0) + C(j, -1)) : (T)0;
     return (T)0.5 * (C(j, f) + C(j, f - 1));
@@ -829,25 +856,25 @@ __device__ __forceinline__ void stage_body(const Dev<T> &d, const Stage<T> &s, i
       auto dfx = [&](int f) -> T {
         // periodic boundary faces: the wrapped difference (scalar.py:133-134) read from the tile
         if ((f > 0 && f < nx) || d.periodic)
-          return -(kappa_x(d, L, tj0, ti0, kap, j, f) * (C(j, f) - C(j, f - 1))) * ih;
-        if (f == 0) return wall_dflux(d, d.wm[0], kappa_x(d, L, tj0, ti0, kap, j, 0), C(j, 0), d.wall_values[0], 1.0);
-        return wall_dflux(d, d.wm[1], kappa_x(d, L, tj0, ti0, kap, j, nx), C(j, nx - 1), d.wall_values[1], -1.0);
+          return -(kappa_x<WIN>(d, L, tj0, ti0, kap, j, f) * (C(j, f) - C(j, f - 1))) * ih;
+        if (f == 0) return wall_dflux(d, d.wm[0], kappa_x<WIN>(d, L, tj0, ti0, kap, j, 0), C(j, 0), d.wall_values[0], 1.0);
+        return wall_dflux(d, d.wm[1], kappa_x<WIN>(d, L, tj0, ti0, kap, j, nx), C(j, nx - 1), d.wall_values[1], -1.0);
       };
       auto dfy = [&](int f) -> T {
         if ((f > 0 && f < ny) || d.periodic)
-          return -(kappa_y(d, L, tj0, ti0, kap, f, i) * (C(f, i) - C(f - 1, i))) * ih;
-        if (f == 0) return wall_dflux(d, d.wm[2], kappa_y(d, L, tj0, ti0, kap, 0, i), C(0, i), d.wall_values[2], 1.0);
-        return wall_dflux(d, d.wm[3], kappa_y(d, L, tj0, ti0, kap, ny, i), C(ny - 1, i), d.wall_values[3], -1.0);
+          return -(kappa_y<WIN>(d, L, tj0, ti0, kap, f, i) * (C(f, i) - C(f - 1, i))) * ih;
+        if (f == 0) return wall_dflux(d, d.wm[2], kappa_y<WIN>(d, L, tj0, ti0, kap, 0, i), C(0, i), d.wall_values[2], 1.0);
+        return wall_dflux(d, d.wm[3], kappa_y<WIN>(d, L, tj0, ti0, kap, ny, i), C(ny - 1, i), d.wall_values[3], -1.0);
       };
       lt = lt + (((dfx(i) - dfx(i + 1)) + dfy(j)) - dfy(j + 1)) * ih;
       const int o = j * nx + i;
       if (d.has_heat) lt = lt + d.heat[o];
       // heat-flux bodies release kappa G P over their skins (physics.py:726-727)
-      if (d.n_flux > 0 && d.win && j >= d.j0 && j < d.j1 && i >= d.i0 && i < d.i1)
+      if (WIN && d.n_flux > 0 && d.win && j >= d.j0 && j < d.j1 && i >= d.i0 && i < d.i1)
         lt = lt + kap * d.q_skin[(j - d.j0) * d.wx + (i - d.i0)];
       const T tp = C(j, i);
       T thh = (s.a == 0.0 ? tp : lerp_t(tp, s.th0[o], a)) + lt * bdt;
-      if (d.n_temp > 0 && d.win && j >= d.j0 && j < d.j1 && i >= d.i0 && i < d.i1) {
+      if (WIN && d.n_temp > 0 && d.win && j >= d.j0 && j < d.j1 && i >= d.i0 && i < d.i1) {
         const int wo = (j - d.j0) * d.wx + (i - d.i0);
         const T kt = bdt / *d.eta_t;
         const T tn = (thh + kt * d.a_t[wo]) / ((T)1 + kt * d.chi_t[wo]);
@@ -860,15 +887,20 @@ __device__ __forceinline__ void stage_body(const Dev<T> &d, const Stage<T> &s, i
     }
   }
 }
-template <typename T, int R>
-__global__ void __launch_bounds__(256) k_stage(Dev<T> d, Stage<T> s) {
-  stage_body<T, R>(d, s, blockIdx.x, blockIdx.y);
+template <typename T, int R, typename D = Dev<T>>
+__global__ void __launch_bounds__(256) k_stage(D d, Stage<T> s) {
+  // tiles clear of the body window (2 cells around it) run without any window code
+  const int tj0 = blockIdx.y * R, ti0 = blockIdx.x * TX;
+  const bool touch = d.win && (d.periodic || (tj0 - 2 <= d.j1 && tj0 + R + 2 >= d.j0 && ti0 - 2 <= d.i1 &&
+                                              ti0 + TX + 2 >= d.i0));
+  if (touch) stage_body<T, R, true>(d, s, blockIdx.x, blockIdx.y);
+  else stage_body<T, R, false>(d, s, blockIdx.x, blockIdx.y);
 }
 
 // Per-block maxima of a velocity tile (+ theta, nu_t) and sum theta: the dt inputs.
 // Each tile also covers the ghost ring of nu_t cells at the domain edge (core.py:503).
-template <typename T, int R>
-__device__ __forceinline__ void tile_maxima(const Dev<T> &d, Lds<T, R> &L, int tj0, int ti0, const T *th) {
+template <typename T, int R, typename D>
+__device__ __forceinline__ void tile_maxima(const D &d, Lds<T, R> &L, int tj0, int ti0, const T *th) {
   const int nx = d.nx, ny = d.ny;
   double mu = 0, mv = 0, mt = 0, mn = 0, st = 0;
   const int i = ti0 + threadIdx.x;
@@ -920,8 +952,8 @@ __device__ __forceinline__ void tile_maxima(const Dev<T> &d, Lds<T, R> &L, int t
 }
 
 // dt inputs of the step-start state (non-chained steps).
-template <typename T, int R>
-__device__ __forceinline__ void pre_body(const Dev<T> &d, int bx, int by, const T *u0, const T *v0, const T *th0) {
+template <typename T, int R, typename D>
+__device__ __forceinline__ void pre_body(const D &d, int bx, int by, const T *u0, const T *v0, const T *th0) {
   __shared__ Lds<T, R> L;
   const int tj0 = by * R, ti0 = bx * TX;
   load_uv(d, L, tj0, ti0, u0, v0, (const T *)nullptr, (T)0);
@@ -932,8 +964,8 @@ __device__ __forceinline__ void pre_body(const Dev<T> &d, int bx, int by, const 
   }
   tile_maxima(d, L, tj0, ti0, th0);
 }
-template <typename T, int R>
-__global__ void __launch_bounds__(256) k_pre(Dev<T> d, const T *u0, const T *v0, const T *th0) {
+template <typename T, int R, typename D = Dev<T>>
+__global__ void __launch_bounds__(256) k_pre(D d, const T *u0, const T *v0, const T *th0) {
   pre_body<T, R>(d, blockIdx.x, blockIdx.y, u0, v0, th0);
 }
 
@@ -1338,8 +1370,8 @@ __global__ void __launch_bounds__(256) k_correct(Dev<T> d, T *__restrict__ u, T 
 
 // End of step on one tile: write u = u^ - grad psi, smoke transport with it (physics.py:
 // 805-830 / core.py:709-771), next-step dt inputs and the window force / heat partials.
-template <typename T, int R>
-__device__ __forceinline__ void final_body(const Dev<T> &d, int bx, int by, const T *uh, const T *vh, T *u_out, T *v_out, const T *th_new, const T *cin, T *cout, const T *t_in) {
+template <typename T, int R, typename D>
+__device__ __forceinline__ void final_body(const D &d, int bx, int by, const T *uh, const T *vh, T *u_out, T *v_out, const T *th_new, const T *cin, T *cout, const T *t_in) {
   __shared__ Lds<T, R> L;
   const int tj0 = by * R, ti0 = bx * TX;
   const int nx = d.nx, ny = d.ny;
@@ -1472,8 +1504,8 @@ __device__ __forceinline__ void final_body(const Dev<T> &d, int bx, int by, cons
     }
   }
 }
-template <typename T, int R>
-__global__ void __launch_bounds__(256) k_final(Dev<T> d, const T *uh, const T *vh, T *u_out, T *v_out,
+template <typename T, int R, typename D = Dev<T>>
+__global__ void __launch_bounds__(256) k_final(D d, const T *uh, const T *vh, T *u_out, T *v_out,
                                               const T *th_new, const T *cin, T *cout, const T *t_in) {
   final_body<T, R>(d, blockIdx.x, blockIdx.y, uh, vh, u_out, v_out, th_new, cin, cout, t_in);
 }
@@ -1959,6 +1991,50 @@ void by_rows(int ty, F &&f) {
   }
 }
 
+template <typename X>
+struct Tag {
+  using type = X;
+};
+
+// calls f(std::integral_constant<int, R>, Tag<D>) with the tile height R and the plan view D of
+// the tile kernels: at the default height a view that fixes the LES model, heat transfer and (for
+// plain wind-tunnel plans) the boundary and feature flags; at other heights the generic kernel
+template <typename T, typename F>
+void by_tile(const aero2d_plan *p, F &&f) {
+  const aero2d_desc &s = p->desc;
+  if (p->ty != kDefaultRows) {
+    by_rows(p->ty, [&](auto r) { f(r, Tag<Dev<T>>{}); });
+    return;
+  }
+  const std::integral_constant<int, kDefaultRows> r{};
+  const bool plain = !s.periodic && !s.closed && s.advection == 1 && !s.wall_model && !s.has_heat &&
+                     !(s.thermal && s.n_flux > 0);
+  auto pick = [&](auto les) {
+    constexpr int L = decltype(les)::value;
+    if (plain) {
+      if (s.thermal) f(r, Tag<DevK<T, L, 1, 1>>{});
+      else f(r, Tag<DevK<T, L, 0, 1>>{});
+    } else {
+      if (s.thermal) f(r, Tag<DevK<T, L, 1, 0>>{});
+      else f(r, Tag<DevK<T, L, 0, 0>>{});
+    }
+  };
+  switch (s.turbulence) {
+    case 1: pick(std::integral_constant<int, 1>{}); break;
+    case 2: pick(std::integral_constant<int, 2>{}); break;
+    case 3: pick(std::integral_constant<int, 3>{}); break;
+    default: pick(std::integral_constant<int, 0>{}); break;
+  }
+}
+
+// a plan view of d (the same values; the view's static members replace some flags)
+template <typename D, typename T>
+D view_of(const Dev<T> &d) {
+  D k;
+  static_cast<Dev<T> &>(k) = d;
+  return k;
+}
+
 // calls f(std::integral_constant<int, NT>) for the transform block size NT = nt (256, 512, 1024)
 template <typename F>
 void by_threads(int nt, F &&f) {
@@ -2201,7 +2277,10 @@ int step_impl(aero2d_plan *p, hipStream_t st, const aero2d_fields *in, aero2d_fi
     }
   }
   if (!chained) {
-    by_rows(p->ty, [&](auto r) { hipLaunchKernelGGL((k_pre<T, decltype(r)::value>), tiles, blk, 0, st, d, u0, v0, th0); });
+    by_tile<T>(p, [&](auto r, auto tag) {
+      using D = typename decltype(tag)::type;
+      hipLaunchKernelGGL((k_pre<T, decltype(r)::value, D>), tiles, blk, 0, st, view_of<D>(d), u0, v0, th0);
+    });
     hipLaunchKernelGGL(k_dt<T>, dim3(1), dim3(1024), 0, st, d);
   }
   T *ubuf[3] = {(T *)p->uA, (T *)p->uB, (T *)p->uC};
@@ -2218,7 +2297,10 @@ int step_impl(aero2d_plan *p, hipStream_t st, const aero2d_fields *in, aero2d_fi
     sg.up = up; sg.vp = vp; sg.psi_p = psi_p; sg.thp = thp;
     sg.u0 = u0; sg.v0 = v0; sg.th0 = th0;
     sg.uh = ubuf[k]; sg.vh = vbuf[k]; sg.thh = s.thermal ? tbuf[k] : nullptr;
-    by_rows(p->ty, [&](auto r) { hipLaunchKernelGGL((k_stage<T, decltype(r)::value>), tiles, blk, 0, st, d, sg); });
+    by_tile<T>(p, [&](auto r, auto tag) {
+      using D = typename decltype(tag)::type;
+      hipLaunchKernelGGL((k_stage<T, decltype(r)::value, D>), tiles, blk, 0, st, view_of<D>(d), sg);
+    });
     solve_passes<T>(p, st, d, sg.uh, sg.vh, nullptr, mode, d.psi, (T *)out->p, k);
     if (tr) {
       const dim3 gf((nx + 1 + 63) / 64, (ny + 1 + 3) / 4);
@@ -2236,8 +2318,9 @@ int step_impl(aero2d_plan *p, hipStream_t st, const aero2d_fields *in, aero2d_fi
   const bool dye = s.dye_mode != 0 && in->dye && out->dye;
   const bool inplace = dye && in->dye == out->dye;
   T *dye_dst = dye ? (inplace ? (T *)p->dye_tmp : (T *)out->dye) : nullptr;
-  by_rows(p->ty, [&](auto r) {
-    hipLaunchKernelGGL((k_final<T, decltype(r)::value>), tiles, blk, 0, st, d, (const T *)p->uC, (const T *)p->vC,
+  by_tile<T>(p, [&](auto r, auto tag) {
+    using D = typename decltype(tag)::type;
+    hipLaunchKernelGGL((k_final<T, decltype(r)::value, D>), tiles, blk, 0, st, view_of<D>(d), (const T *)p->uC, (const T *)p->vC,
                        (T *)out->u, (T *)out->v, (const T *)out->theta, dye ? (const T *)in->dye : (const T *)nullptr,
                        dye_dst, (const T *)in->t);
   });
@@ -2312,11 +2395,13 @@ int time_stage_impl(aero2d_plan *p, hipStream_t st, const aero2d_fields *f, cons
   hipEvent_t e0, e1;
   AERO_HIP_CHECK(hipEventCreate(&e0));
   AERO_HIP_CHECK(hipEventCreate(&e1));
-  by_rows(p->ty, [&](auto r) {
+  by_tile<T>(p, [&](auto r, auto tag) {
     constexpr int R = decltype(r)::value;
-    hipLaunchKernelGGL((k_stage<T, R>), tiles, blk, 0, st, d, sg);   // warm
+    using D = typename decltype(tag)::type;
+    const D dk = view_of<D>(d);
+    hipLaunchKernelGGL((k_stage<T, R, D>), tiles, blk, 0, st, dk, sg);   // warm
     (void)hipEventRecord(e0, st);
-    for (int it = 0; it < iters; ++it) hipLaunchKernelGGL((k_stage<T, R>), tiles, blk, 0, st, d, sg);
+    for (int it = 0; it < iters; ++it) hipLaunchKernelGGL((k_stage<T, R, D>), tiles, blk, 0, st, dk, sg);
   });
   AERO_HIP_CHECK(hipEventRecord(e1, st));
   AERO_HIP_CHECK(hipEventSynchronize(e1));

@@ -1370,7 +1370,7 @@ __global__ void __launch_bounds__(256) k_correct(Dev<T> d, T *__restrict__ u, T 
 
 // End of step on one tile: write u = u^ - grad psi, smoke transport with it (physics.py:
 // 805-830 / core.py:709-771), next-step dt inputs and the window force / heat partials.
-template <typename T, int R, typename D>
+template <typename T, int R, bool WIN = true, typename D>
 __device__ __forceinline__ void final_body(const D &d, int bx, int by, const T *uh, const T *vh, T *u_out, T *v_out, const T *th_new, const T *cin, T *cout, const T *t_in) {
   __shared__ Lds<T, R> L;
   const int tj0 = by * R, ti0 = bx * TX;
@@ -1389,12 +1389,12 @@ __device__ __forceinline__ void final_body(const D &d, int bx, int by, const T *
     if (j == ny - 1) v_out[ny * nx + i] = L.v[r + 1][c];
   }
   if (d.les) {
-    tile_nut(d, L, tj0, ti0);
+    tile_nut<WIN>(d, L, tj0, ti0);
     __syncthreads();
   }
   tile_maxima(d, L, tj0, ti0, th_new);
   // force / heat partials of the window faces this tile owns
-  const bool touch = d.win && ti0 <= d.i1 && (ti0 + TX > d.i0 || d.periodic) && tj0 <= d.j1 &&
+  const bool touch = WIN && d.win && ti0 <= d.i1 && (ti0 + TX > d.i0 || d.periodic) && tj0 <= d.j1 &&
                      (tj0 + R > d.j0 || d.periodic);
   if (touch) {
     double acc[kMaxWin];
@@ -1481,7 +1481,7 @@ __device__ __forceinline__ void final_body(const D &d, int bx, int by, const T *
         const T fyb = tvd_y(L, r, cc, L.v[r][cc]), fyt = tvd_y(L, r + 1, cc, L.v[r + 1][cc]);
         const int o = j * nx + i;
         T val;
-        const bool solid = d.win && j >= d.j0 && j < d.j1 && i >= d.i0 && i < d.i1 &&
+        const bool solid = WIN && d.win && j >= d.j0 && j < d.j1 && i >= d.i0 && i < d.i1 &&
                            !(d.chi_c[(j - d.j0) * d.wx + (i - d.i0)] <= (T)0.5);
         if (d.dye_mode == 2) {
           val = L.c[r][cc] + dt * ((((fxl - fxr) + fyb) - fyt) * ih);
@@ -1507,7 +1507,12 @@ __device__ __forceinline__ void final_body(const D &d, int bx, int by, const T *
 template <typename T, int R, typename D = Dev<T>>
 __global__ void __launch_bounds__(256) k_final(D d, const T *uh, const T *vh, T *u_out, T *v_out,
                                               const T *th_new, const T *cin, T *cout, const T *t_in) {
-  final_body<T, R>(d, blockIdx.x, blockIdx.y, uh, vh, u_out, v_out, th_new, cin, cout, t_in);
+  // tiles clear of the body window (2 cells around it) run without any window code
+  const int tj0 = blockIdx.y * R, ti0 = blockIdx.x * TX;
+  const bool touch = d.win && (d.periodic || (tj0 - 2 <= d.j1 && tj0 + R + 2 >= d.j0 && ti0 - 2 <= d.i1 &&
+                                              ti0 + TX + 2 >= d.i0));
+  if (touch) final_body<T, R, true>(d, blockIdx.x, blockIdx.y, uh, vh, u_out, v_out, th_new, cin, cout, t_in);
+  else final_body<T, R, false>(d, blockIdx.x, blockIdx.y, uh, vh, u_out, v_out, th_new, cin, cout, t_in);
 }
 
 // Stage trace (core.py:897-934): from a stage's u^ and potential, the velocity as the

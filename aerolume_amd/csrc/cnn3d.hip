@@ -111,6 +111,181 @@ __global__ void __launch_bounds__(kThreads) k_conv_fwd(const float *__restrict__
   }
 }
 
+
+// ------------------------------------------------------------------ MFMA forward convolution
+// The same convolution for 32 output channels as an implicit GEMM on the matrix cores
+// (v_mfma_f32_32x32x16_bf16): D[position][co] = sum_K A[position][K] B[K][co] with K = (tap,
+// input channel). The weights are bf16 already; each float32 input value is split exactly into
+// three bf16 parts (x = hi + mid + lo: truncations to 8 significant bits, then to 8 of the
+// remainder, which leaves at most 8), so the three products with a bf16 weight are exact in
+// float32 and the MFMA sums them in float32: the arithmetic of the scalar kernel (float32
+// accumulation of exact products), in another order.
+//
+// Block: a 16x8x4 output brick, 512 threads; wave w takes rows y 4 (w & 1) .. + 3 of z-plane
+// w / 2 as two 32-row M blocks of two 16-wide x rows each (so the 16 lanes of an LDS read group
+// read 256 contiguous bytes: no bank conflicts). Input channels go in chunks of 8: the chunk's
+// input brick with its one-cell halo (18x10x6 positions) is staged in LDS as the three bf16
+// parts, position-major with the 8 channels of a position in 16 bytes, and the chunk's weights
+// as [tap][co][8]. One K step of 16 is a pair of taps: lanes 0-31 read tap 2q, lanes 32-63 tap
+// 2q + 1 (tap 27 has zero weights), 16 bytes each from LDS, so a K step costs 7 LDS reads and
+// 6 MFMAs per lane.
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+constexpr int MX = 16, MY = 8, MZ = 4;                 // output brick of the MFMA kernel
+constexpr int kMThreads = 512;                         // one wave per 16x4 half plane
+constexpr int MHX = MX + 2, MHY = MY + 2, MHZ = MZ + 2, MHALO = MHX * MHY * MHZ;
+constexpr int kMTaps = 28;                             // 27 taps and a zero one (pairs of taps)
+
+struct MfmaLds {
+  struct {
+    uint4 x[3][MHALO];                                 // bf16 parts hi / mid / lo, 8 channels each
+    uint4 w[kMTaps][32];                               // bf16 weights [tap][co][8 channels]
+  } in;
+};
+
+__device__ __forceinline__ uint32_t hi16(float v) { return __float_as_uint(v) >> 16; }
+
+// exact three-way bf16 split of v: the upper 16 bits of hi, mid, lo
+__device__ __forceinline__ void split3(float v, uint32_t &h, uint32_t &m, uint32_t &l) {
+  const float fh = __uint_as_float(__float_as_uint(v) & 0xffff0000u);
+  const float r1 = v - fh;
+  const float fm = __uint_as_float(__float_as_uint(r1) & 0xffff0000u);
+  const float r2 = r1 - fm;
+  h = hi16(fh);
+  m = hi16(fm);
+  l = hi16(r2);
+}
+
+__global__ void __launch_bounds__(kMThreads) k_conv_fwd_mfma(const float *__restrict__ x,
+                                                           const uint16_t *__restrict__ w16,
+                                                           const float *__restrict__ b, float *__restrict__ pre,
+                                                           float *__restrict__ y, int cin, int nz, int ny, int nx,
+                                                           int act) {
+  extern __shared__ unsigned char smem[];
+  MfmaLds &L = *reinterpret_cast<MfmaLds *>(smem);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int x0 = blockIdx.x * MX, y0 = blockIdx.y * MY, z0 = blockIdx.z * MZ;
+  const size_t nxyz = (size_t)nx * ny * nz;
+  // this lane's rows: M block mb holds positions (x = m % 16, y = yb + 2 mb + m / 16) of plane wz
+  const int m = lane & 31, half = lane >> 5;
+  const int ax = m & 15, ay = m >> 4;
+  const int wz = wave >> 1, yb = 4 * (wave & 1);
+  f32x16_t acc[2];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[0][r] = acc[1][r] = 0.f;
+  // the staging of a chunk: every thread loads its halo positions (e = tid, tid + 512) and
+  // weight slots (tid, tid + 512) into registers; the next chunk's loads are issued before the
+  // current chunk's MFMAs, so their latency hides behind the matrix work
+  constexpr int kPos = (MHALO + kMThreads - 1) / kMThreads, kWs = (kMTaps * 32 + kMThreads - 1) / kMThreads;
+  float xr[kPos][8];
+  uint32_t wr[kWs][4];
+  auto fetch = [&](int c0) {
+#pragma unroll
+    for (int i = 0; i < kPos; ++i) {
+      const int e = tid + kMThreads * i;
+      const int ee = e < MHALO ? e : 0;
+      const int hx = ee % MHX, t1 = ee / MHX, hy = t1 % MHY, hz = t1 / MHY;
+      const int gx = wrapi(x0 - 1 + hx, nx), gy = wrapi(y0 - 1 + hy, ny), gz = wrapi(z0 - 1 + hz, nz);
+      const size_t p = ((size_t)gz * ny + gy) * nx + gx;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) xr[i][c] = c0 + c < cin ? x[(size_t)(c0 + c) * nxyz + p] : 0.f;
+    }
+#pragma unroll
+    for (int i = 0; i < kWs; ++i) {
+      const int e = tid + kMThreads * i;
+      const int co = e & 31, k = (e >> 5) < kMTaps ? (e >> 5) : 0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int ca = c0 + 2 * q, cb = ca + 1;
+        const uint32_t wa = (k < 27 && ca < cin) ? w16[((size_t)co * cin + ca) * 27 + k] : 0u;
+        const uint32_t wb = (k < 27 && cb < cin) ? w16[((size_t)co * cin + cb) * 27 + k] : 0u;
+        wr[i][q] = wa | (wb << 16);
+      }
+    }
+  };
+  fetch(0);
+  for (int c0 = 0; c0 < cin; c0 += 8) {
+    __syncthreads();
+    // the chunk's input brick + halo, split into three bf16 parts, and its weights
+#pragma unroll
+    for (int i = 0; i < kPos; ++i) {
+      const int e = tid + kMThreads * i;
+      if (e >= MHALO) continue;
+      uint32_t ph[4], pm[4], pl[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        uint32_t h0, m0, l0, h1, m1, l1;
+        split3(xr[i][2 * q], h0, m0, l0);
+        split3(xr[i][2 * q + 1], h1, m1, l1);
+        ph[q] = h0 | (h1 << 16);
+        pm[q] = m0 | (m1 << 16);
+        pl[q] = l0 | (l1 << 16);
+      }
+      L.in.x[0][e] = make_uint4(ph[0], ph[1], ph[2], ph[3]);
+      L.in.x[1][e] = make_uint4(pm[0], pm[1], pm[2], pm[3]);
+      L.in.x[2][e] = make_uint4(pl[0], pl[1], pl[2], pl[3]);
+    }
+#pragma unroll
+    for (int i = 0; i < kWs; ++i) {
+      const int e = tid + kMThreads * i;
+      if (e < kMTaps * 32) L.in.w[e >> 5][e & 31] = make_uint4(wr[i][0], wr[i][1], wr[i][2], wr[i][3]);
+    }
+    __syncthreads();
+    if (c0 + 8 < cin) fetch(c0 + 8);
+#pragma unroll 2
+    for (int tp = 0; tp < kMTaps / 2; ++tp) {
+      const int k = 2 * tp + half;                       // this lane's tap
+      const int kk = k < 27 ? k : 13;                    // the zero tap reads the centre
+      const int dz = kk / 9, dy = (kk / 3) % 3, dx = kk % 3;
+      const uint4 wv = L.in.w[k][m];
+      bf16x8_t bw;
+      __builtin_memcpy(&bw, &wv, 16);
+#pragma unroll
+      for (int mb = 0; mb < 2; ++mb) {
+        const int pos = ((wz + dz) * MHY + (yb + 2 * mb + ay + dy)) * MHX + (ax + dx);
+#pragma unroll
+        for (int part = 0; part < 3; ++part) {
+          const uint4 av = L.in.x[part][pos];
+          bf16x8_t a;
+          __builtin_memcpy(&a, &av, 16);
+          acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bw, acc[mb], 0, 0, 0);
+        }
+      }
+    }
+  }
+  // results: registers 4g .. 4g + 3 of a lane hold four x-neighbours (M rows 8g + 4 half + 0..3,
+  // x = row % 16, y = yb + 2 mb + row / 16) of channel m, stored as one 16-byte write per array
+  const float bias = b ? b[m] : 0.f;
+  const bool full = x0 + MX <= nx && (nx & 3) == 0;
+#pragma unroll
+  for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const int row = 8 * g + 4 * half;
+      const int gx = x0 + (row & 15), gy = y0 + yb + 2 * mb + (row >> 4), gz = z0 + wz;
+      if (gy >= ny || gz >= nz) continue;
+      const size_t p = (size_t)m * nxyz + ((size_t)gz * ny + gy) * nx + gx;
+      float v[4], a[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        v[q] = acc[mb][4 * g + q] + bias;
+        a[q] = act_f(act, v[q]);
+      }
+      if (full) {
+        if (pre) *reinterpret_cast<float4 *>(pre + p) = make_float4(v[0], v[1], v[2], v[3]);
+        *reinterpret_cast<float4 *>(y + p) = make_float4(a[0], a[1], a[2], a[3]);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (gx + q >= nx) continue;
+          if (pre) pre[p + q] = v[q];
+          y[p + q] = a[q];
+        }
+      }
+    }
+}
+
 // g = dy * act'(pre) in place into g (elementwise).
 __global__ void k_act_grad(const float *__restrict__ dy, const float *__restrict__ pre, float *__restrict__ g, size_t n,
                            int act) {
@@ -260,6 +435,13 @@ int max_ch(const aero_cnn_desc &d) {
 
 void launch_fwd(hipStream_t st, const float *x, const uint16_t *w, const float *b, float *pre, float *y, int cin,
                 int cout, int nz, int ny, int nx, int act) {
+  if (cout == 32) {
+    // 32 output channels: the matrix-core kernel
+    const dim3 mgrid((nx + MX - 1) / MX, (ny + MY - 1) / MY, (nz + MZ - 1) / MZ);
+    hipLaunchKernelGGL(k_conv_fwd_mfma, mgrid, dim3(kMThreads), sizeof(MfmaLds), st, x, w, b, pre, y, cin, nz, ny,
+                       nx, act);
+    return;
+  }
   const dim3 grid((nx + TX - 1) / TX, (ny + TY - 1) / TY, (nz + TZ - 1) / TZ);
   if (cout == 1)
     hipLaunchKernelGGL(k_conv_fwd<1>, grid, dim3(kThreads), 0, st, x, w, b, pre, y, cin, nz, ny, nx, act);
@@ -292,6 +474,9 @@ int aero_cnn_create(const aero_cnn_desc *d, aero_cnn **out) {
     }
     if (d->act[l] < 0 || d->act[l] > 2) { set_error("activation 0 (none), 1 (relu) or 2 (softplus)"); return AERO_EINVAL; }
   }
+  // the matrix-core convolution stages more than the default 64 KB of LDS
+  AERO_HIP_CHECK(hipFuncSetAttribute((const void *)k_conv_fwd_mfma, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)sizeof(MfmaLds)));
   aero_cnn *c = new aero_cnn();
   c->desc = *d;
   c->nxyz = (size_t)d->nx * d->ny * d->nz;

@@ -137,6 +137,16 @@ constexpr int kMThreads = 512;                         // one wave per 16x4 half
 constexpr int MHX = MX + 2, MHY = MY + 2, MHZ = MZ + 2, MHALO = MHX * MHY * MHZ;
 constexpr int kMTaps = 28;                             // 27 taps and a zero one (pairs of taps)
 
+// LDS position offset of tap k = (dz, dy, dx) in the staged halo brick (tap 27: the centre)
+#define TAP_OFF(dz, dy, dx) (((dz) * MHY + (dy)) * MHX + (dx))
+__constant__ int kTapOff[kMTaps] = {
+    TAP_OFF(0, 0, 0), TAP_OFF(0, 0, 1), TAP_OFF(0, 0, 2), TAP_OFF(0, 1, 0), TAP_OFF(0, 1, 1), TAP_OFF(0, 1, 2),
+    TAP_OFF(0, 2, 0), TAP_OFF(0, 2, 1), TAP_OFF(0, 2, 2), TAP_OFF(1, 0, 0), TAP_OFF(1, 0, 1), TAP_OFF(1, 0, 2),
+    TAP_OFF(1, 1, 0), TAP_OFF(1, 1, 1), TAP_OFF(1, 1, 2), TAP_OFF(1, 2, 0), TAP_OFF(1, 2, 1), TAP_OFF(1, 2, 2),
+    TAP_OFF(2, 0, 0), TAP_OFF(2, 0, 1), TAP_OFF(2, 0, 2), TAP_OFF(2, 1, 0), TAP_OFF(2, 1, 1), TAP_OFF(2, 1, 2),
+    TAP_OFF(2, 2, 0), TAP_OFF(2, 2, 1), TAP_OFF(2, 2, 2), TAP_OFF(1, 1, 1)};
+#undef TAP_OFF
+
 struct MfmaLds {
   struct {
     uint4 x[3][MHALO];                                 // bf16 parts hi / mid / lo, 8 channels each
@@ -174,38 +184,49 @@ __global__ void __launch_bounds__(kMThreads) k_conv_fwd_mfma(const float *__rest
   f32x16_t acc[2];
 #pragma unroll
   for (int r = 0; r < 16; ++r) acc[0][r] = acc[1][r] = 0.f;
-  // the staging of a chunk: every thread loads its halo positions (e = tid, tid + 512) and
-  // weight slots (tid, tid + 512) into registers; the next chunk's loads are issued before the
-  // current chunk's MFMAs, so their latency hides behind the matrix work
+  // the staging of a chunk: every thread loads its halo positions (e = tid, tid + 512, ...) and
+  // weight slots (tid, tid + 512) into registers, then splits and stores them; their global
+  // offsets are the same for every chunk and worked out once
   constexpr int kPos = (MHALO + kMThreads - 1) / kMThreads, kWs = (kMTaps * 32 + kMThreads - 1) / kMThreads;
+  size_t gpos[kPos];
+  int wsrc[kWs];
+#pragma unroll
+  for (int i = 0; i < kPos; ++i) {
+    const int e = tid + kMThreads * i;
+    const int ee = e < MHALO ? e : 0;
+    const int hx = ee % MHX, t1 = ee / MHX, hy = t1 % MHY, hz = t1 / MHY;
+    const int gx = wrapi(x0 - 1 + hx, nx), gy = wrapi(y0 - 1 + hy, ny), gz = wrapi(z0 - 1 + hz, nz);
+    gpos[i] = ((size_t)gz * ny + gy) * nx + gx;
+  }
+#pragma unroll
+  for (int i = 0; i < kWs; ++i) {
+    const int e = tid + kMThreads * i;
+    const int co = e & 31, k = e >> 5;
+    wsrc[i] = k < 27 ? co * cin * 27 + k : -1;           // -1: the zero tap (or past the table)
+  }
   float xr[kPos][8];
   uint32_t wr[kWs][4];
   auto fetch = [&](int c0) {
 #pragma unroll
-    for (int i = 0; i < kPos; ++i) {
-      const int e = tid + kMThreads * i;
-      const int ee = e < MHALO ? e : 0;
-      const int hx = ee % MHX, t1 = ee / MHX, hy = t1 % MHY, hz = t1 / MHY;
-      const int gx = wrapi(x0 - 1 + hx, nx), gy = wrapi(y0 - 1 + hy, ny), gz = wrapi(z0 - 1 + hz, nz);
-      const size_t p = ((size_t)gz * ny + gy) * nx + gx;
+    for (int i = 0; i < kPos; ++i)
 #pragma unroll
-      for (int c = 0; c < 8; ++c) xr[i][c] = c0 + c < cin ? x[(size_t)(c0 + c) * nxyz + p] : 0.f;
-    }
+      for (int c = 0; c < 8; ++c) xr[i][c] = c0 + c < cin ? x[(size_t)(c0 + c) * nxyz + gpos[i]] : 0.f;
 #pragma unroll
-    for (int i = 0; i < kWs; ++i) {
-      const int e = tid + kMThreads * i;
-      const int co = e & 31, k = (e >> 5) < kMTaps ? (e >> 5) : 0;
+    for (int i = 0; i < kWs; ++i)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int ca = c0 + 2 * q, cb = ca + 1;
-        const uint32_t wa = (k < 27 && ca < cin) ? w16[((size_t)co * cin + ca) * 27 + k] : 0u;
-        const uint32_t wb = (k < 27 && cb < cin) ? w16[((size_t)co * cin + cb) * 27 + k] : 0u;
+        const uint32_t wa = (wsrc[i] >= 0 && ca < cin) ? w16[wsrc[i] + ca * 27] : 0u;
+        const uint32_t wb = (wsrc[i] >= 0 && cb < cin) ? w16[wsrc[i] + cb * 27] : 0u;
         wr[i][q] = wa | (wb << 16);
       }
-    }
   };
-  fetch(0);
+  // LDS position of this lane's row in each M block; tap offsets are compile-time constants
+  int base[2];
+#pragma unroll
+  for (int mb = 0; mb < 2; ++mb) base[mb] = (wz * MHY + (yb + 2 * mb + ay)) * MHX + ax;
   for (int c0 = 0; c0 < cin; c0 += 8) {
+    fetch(c0);
     __syncthreads();
     // the chunk's input brick + halo, split into three bf16 parts, and its weights
 #pragma unroll
@@ -232,18 +253,16 @@ __global__ void __launch_bounds__(kMThreads) k_conv_fwd_mfma(const float *__rest
       if (e < kMTaps * 32) L.in.w[e >> 5][e & 31] = make_uint4(wr[i][0], wr[i][1], wr[i][2], wr[i][3]);
     }
     __syncthreads();
-    if (c0 + 8 < cin) fetch(c0 + 8);
 #pragma unroll 2
     for (int tp = 0; tp < kMTaps / 2; ++tp) {
-      const int k = 2 * tp + half;                       // this lane's tap
-      const int kk = k < 27 ? k : 13;                    // the zero tap reads the centre
-      const int dz = kk / 9, dy = (kk / 3) % 3, dx = kk % 3;
-      const uint4 wv = L.in.w[k][m];
+      // taps 2 tp (lanes 0-31) and 2 tp + 1 (lanes 32-63); the zero tap 27 reads the centre
+      const int off = half ? kTapOff[2 * tp + 1] : kTapOff[2 * tp];
+      const uint4 wv = L.in.w[2 * tp + half][m];
       bf16x8_t bw;
       __builtin_memcpy(&bw, &wv, 16);
 #pragma unroll
       for (int mb = 0; mb < 2; ++mb) {
-        const int pos = ((wz + dz) * MHY + (yb + 2 * mb + ay + dy)) * MHX + (ax + dx);
+        const int pos = base[mb] + off;
 #pragma unroll
         for (int part = 0; part < 3; ++part) {
           const uint4 av = L.in.x[part][pos];
@@ -284,6 +303,98 @@ __global__ void __launch_bounds__(kMThreads) k_conv_fwd_mfma(const float *__rest
         }
       }
     }
+}
+
+
+// ------------------------------------------------------------------ one output channel
+// The convolution to a single output channel (the network's last layer) in two passes:
+// z[k][p] = sum_ci w[ci][k] x[ci][p] for every position p and tap k, a plain GEMM (positions x
+// input channels) x (input channels x 27 taps) on the matrix cores with the same exact bf16
+// split of x; then y[p] = act(b + sum_k z[k][p + d_k]), a 27-point gather. Each wave takes M
+// blocks of 32 consecutive positions; a K step is 16 input channels.
+__global__ void __launch_bounds__(256) k_conv_taps_mfma(const float *__restrict__ x, const uint16_t *__restrict__ w16,
+                                                       float *__restrict__ z, int cin, size_t nxyz) {
+  const int lane = threadIdx.x & 63, half = lane >> 5, col = lane & 31;
+  const int nks = (cin + 15) / 16;
+  // B fragments (weights of tap `col`, input channels 16 s + 8 half + i), the same for every block
+  bf16x8_t bw[2];
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    uint32_t pw[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int ca = 16 * s + 8 * half + 2 * q, cb = ca + 1;
+      const uint32_t wa = (s < nks && col < 27 && ca < cin) ? w16[ca * 27 + col] : 0u;
+      const uint32_t wb = (s < nks && col < 27 && cb < cin) ? w16[cb * 27 + col] : 0u;
+      pw[q] = wa | (wb << 16);
+    }
+    __builtin_memcpy(&bw[s], pw, 16);
+  }
+  const size_t nblk = (nxyz + 31) / 32;
+  const size_t wstride = (size_t)gridDim.x * (blockDim.x >> 6);
+  for (size_t blk = (size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); blk < nblk; blk += wstride) {
+    const size_t p0 = blk * 32;
+    const size_t p = p0 + col < nxyz ? p0 + col : nxyz - 1;   // row of this lane (clamped: stores skip it)
+    f32x16_t acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      if (s >= nks) break;
+      uint32_t ph[4], pm[4], pl[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int ca = 16 * s + 8 * half + 2 * q, cb = ca + 1;
+        uint32_t h0, m0, l0, h1, m1, l1;
+        split3(ca < cin ? x[(size_t)ca * nxyz + p] : 0.f, h0, m0, l0);
+        split3(cb < cin ? x[(size_t)cb * nxyz + p] : 0.f, h1, m1, l1);
+        ph[q] = h0 | (h1 << 16);
+        pm[q] = m0 | (m1 << 16);
+        pl[q] = l0 | (l1 << 16);
+      }
+      bf16x8_t a;
+      __builtin_memcpy(&a, ph, 16);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bw[s], acc, 0, 0, 0);
+      __builtin_memcpy(&a, pm, 16);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bw[s], acc, 0, 0, 0);
+      __builtin_memcpy(&a, pl, 16);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bw[s], acc, 0, 0, 0);
+    }
+    // register 4g + q: position p0 + 8g + 4 half + q, tap col
+    if (col < 27) {
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const size_t pr = p0 + 8 * g + 4 * half;
+        float *dst = z + (size_t)col * nxyz + pr;
+        if (pr + 3 < nxyz && (nxyz & 3) == 0) {
+          *reinterpret_cast<float4 *>(dst) = make_float4(acc[4 * g], acc[4 * g + 1], acc[4 * g + 2], acc[4 * g + 3]);
+        } else {
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            if (pr + q < nxyz) dst[q] = acc[4 * g + q];
+        }
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_conv_taps_sum(const float *__restrict__ z, const float *__restrict__ b,
+                                                      float *__restrict__ pre, float *__restrict__ y, int nz, int ny,
+                                                      int nx, int act) {
+  const size_t nxyz = (size_t)nx * ny * nz;
+  for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < nxyz; p += (size_t)gridDim.x * blockDim.x) {
+    const int gx = (int)(p % nx), gy = (int)((p / nx) % ny), gz = (int)(p / ((size_t)nx * ny));
+    float v = 0.f;
+#pragma unroll
+    for (int k = 0; k < 27; ++k) {
+      const int dz = k / 9 - 1, dy = (k / 3) % 3 - 1, dx = k % 3 - 1;
+      const size_t q = ((size_t)wrapi(gz + dz, nz) * ny + wrapi(gy + dy, ny)) * nx + wrapi(gx + dx, nx);
+      v += z[(size_t)k * nxyz + q];
+    }
+    v = v + (b ? b[0] : 0.f);
+    if (pre) pre[p] = v;
+    y[p] = act_f(act, v);
+  }
 }
 
 // g = dy * act'(pre) in place into g (elementwise).
@@ -433,8 +544,17 @@ int max_ch(const aero_cnn_desc &d) {
   return m;
 }
 
+// zbuf: 27 x cells of scratch for the single-output-channel path (null: the scalar kernel)
 void launch_fwd(hipStream_t st, const float *x, const uint16_t *w, const float *b, float *pre, float *y, int cin,
-                int cout, int nz, int ny, int nx, int act) {
+                int cout, int nz, int ny, int nx, int act, float *zbuf = nullptr) {
+  if (cout == 1 && zbuf && cin <= 32) {
+    const size_t nxyz = (size_t)nx * ny * nz;
+    const int nb = (int)std::min<size_t>(2048, ((nxyz + 31) / 32 + 3) / 4);
+    hipLaunchKernelGGL(k_conv_taps_mfma, dim3(nb), dim3(256), 0, st, x, w, zbuf, cin, nxyz);
+    const int ns = (int)std::min<size_t>(4096, (nxyz + 255) / 256);
+    hipLaunchKernelGGL(k_conv_taps_sum, dim3(ns), dim3(256), 0, st, zbuf, b, pre, y, nz, ny, nx, act);
+    return;
+  }
   if (cout == 32) {
     // 32 output channels: the matrix-core kernel
     const dim3 mgrid((nx + MX - 1) / MX, (ny + MY - 1) / MY, (nz + MZ - 1) / MZ);
@@ -503,8 +623,11 @@ int aero_cnn_create(const aero_cnn_desc *d, aero_cnn **out) {
   A((void **)&c->part, (npc * (size_t)(nw_max + 32)) * 4);
   for (int l = 0; l <= d->n_layers; ++l) A((void **)&c->act[l], (size_t)d->channels[l] * c->nxyz * 4);
   for (int l = 0; l < d->n_layers; ++l) A((void **)&c->pre[l], (size_t)d->channels[l + 1] * c->nxyz * 4);
-  A((void **)&c->g0, (size_t)mc * c->nxyz * 4);
-  A((void **)&c->g1, (size_t)mc * c->nxyz * 4);
+  // gradient buffers; between backward passes they also hold the 27 tap sums of a
+  // single-output-channel layer
+  const size_t gch = (size_t)std::max(mc, 27);
+  A((void **)&c->g0, gch * c->nxyz * 4);
+  A((void **)&c->g1, gch * c->nxyz * 4);
   if (rc) {
     aero_cnn_destroy(c);
     return rc;
@@ -554,7 +677,7 @@ int aero_cnn_forward(aero_cnn *c, void *stream, const float *x, float *y) {
   for (int l = 0; l < d.n_layers; ++l) {
     float *dst = l == d.n_layers - 1 ? y : c->act[l + 1];
     launch_fwd(st, c->act[l], c->w16 + c->woff[l], c->params + c->boff[l], c->pre[l], dst, d.channels[l],
-               d.channels[l + 1], d.nz, d.ny, d.nx, d.act[l]);
+               d.channels[l + 1], d.nz, d.ny, d.nx, d.act[l], c->g0);
     if (l == d.n_layers - 1)
       AERO_HIP_CHECK(hipMemcpyAsync(c->act[l + 1], y, (size_t)d.channels[l + 1] * c->nxyz * 4,
                                     hipMemcpyDeviceToDevice, st));
@@ -570,8 +693,10 @@ int aero_cnn_infer(aero_cnn *c, void *stream, const float *x, float *y) {
   const float *src = x;
   for (int l = 0; l < d.n_layers; ++l) {
     float *dst = l == d.n_layers - 1 ? y : (l % 2 == 0 ? c->g0 : c->g1);
+    // the buffer src is not in holds the taps of a single-output-channel layer
+    float *zb = src == c->g0 ? c->g1 : c->g0;
     launch_fwd(st, src, c->w16 + c->woff[l], c->params + c->boff[l], nullptr, dst, d.channels[l], d.channels[l + 1],
-               d.nz, d.ny, d.nx, d.act[l]);
+               d.nz, d.ny, d.nx, d.act[l], zb);
     src = dst;
   }
   AERO_HIP_CHECK(hipGetLastError());

@@ -474,19 +474,184 @@ __global__ void __launch_bounds__(256) k_conv_wgrad(const float *__restrict__ x,
   if (blockIdx.x == 0 && t < COUT) part[(size_t)npc * COUT * cin * 27 + (size_t)blockIdx.y * COUT + t] = bacc;
 }
 
-// Sum the partials over position chunks (fixed order) into grad (w then b of the layer).
-__global__ void k_wgrad_reduce(const float *__restrict__ part, float *__restrict__ grad, int nw, int nb, int npc) {
-  const int n = nw + nb;
-  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
-    float s = 0.f;
-    if (e < nw) {
-      for (int c = 0; c < npc; ++c) s += part[(size_t)c * nw + e];
-    } else {
-      const int co = e - nw;
-      for (int c = 0; c < npc; ++c) s += part[(size_t)npc * nw + (size_t)c * nb + co];
+
+// ------------------------------------------------------------------ MFMA weight gradient
+// dw[co][ci][k] = sum_p g[co][p] x[ci][p + d_k] for 32 output channels as GEMMs over positions
+// on the matrix cores: D[co][(ci, k)] with K = positions. g and x are split into two bf16 parts
+// each (hi + lo, 16 significant bits) and the three products hi*hi + hi*lo + lo*hi are summed in
+// float32 (relative error of a product about 2^-16). db[co] = sum_p g[co][p] in float32.
+//
+// Block: 512 threads, bricks of 16x8x4 positions (one CU's worth; a block walks `bricks`
+// consecutive bricks). Per brick, g of all 32 channels is staged as [part][co][position]; per
+// chunk of 8 input channels, x with its halo is staged three times, shifted by dx = 0, 1, 2
+// along x, as [dx][part][ci][z][y][16], so that the 8 positions of a K slice start on a 16-byte
+// boundary for every tap. A K step is one 16-wide x row of the brick; wave w < 7 accumulates the
+// columns (ci, k) = (n % 8, 4 w + n / 8) of each chunk (the eighth wave only stages).
+constexpr int WX = 16, WY = 8, WZ = 4, WPOS = WX * WY * WZ;
+constexpr int WHY = WY + 2, WHZ = WZ + 2;
+constexpr int kWChunks = 4;                            // up to 32 input channels
+
+// Rows are padded by 16 bytes (g: per channel, x: per input channel of a copy) so that the 16
+// lanes of an LDS read group, which read different channels, hit different banks.
+constexpr int kWgRow = WPOS / 8 + 1, kWxCi = WHZ * WHY * (WX / 8) + 1;
+struct WgradLds {
+  uint4 g[2][32][kWgRow];                              // bf16 g parts, 8 positions per 16 bytes
+  uint4 x[3][2][8][kWxCi];                             // bf16 x parts per dx shift, [z][y][16] per ci
+};
+
+__device__ __forceinline__ void split2(float v, uint32_t &h, uint32_t &l) {
+  const float fh = __uint_as_float(__float_as_uint(v) & 0xffff0000u);
+  h = hi16(fh);
+  l = hi16(v - fh);
+}
+
+__global__ void __launch_bounds__(512) k_conv_wgrad_mfma(const float *__restrict__ x, const float *__restrict__ g,
+                                                        float *__restrict__ part, int cin, int nz, int ny, int nx,
+                                                        int bricks) {
+  extern __shared__ unsigned char smem[];
+  WgradLds &L = *reinterpret_cast<WgradLds *>(smem);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, half = lane >> 5, n = lane & 31;
+  const size_t nxyz = (size_t)nx * ny * nz;
+  const int nbx = (nx + WX - 1) / WX, nby = (ny + WY - 1) / WY, nbz = (nz + WZ - 1) / WZ;
+  const int nbr = nbx * nby * nbz;
+  const int nch = (cin + 7) / 8;
+  // this lane's column: input channel n % 8 of the chunk, tap 4 wave + n / 8 (27: none)
+  const int tap = 4 * wave + (n >> 3), cil = n & 7;
+  const int tk = tap < 27 ? tap : 13;
+  const int tdz = tk / 9, tdy = (tk / 3) % 3, tdx = tk % 3;
+  f32x16_t acc[kWChunks];
+#pragma unroll
+  for (int c = 0; c < kWChunks; ++c)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[c][r] = 0.f;
+  float dbs[32];                                        // this thread's db shares per channel
+#pragma unroll
+  for (int co = 0; co < 32; ++co) dbs[co] = 0.f;
+  const int br0 = blockIdx.x * bricks, br1 = min(nbr, br0 + bricks);
+  for (int br = br0; br < br1; ++br) {
+    const int bx = br % nbx, by = (br / nbx) % nby, bz = br / (nbx * nby);
+    const int x0 = bx * WX, y0 = by * WY, z0 = bz * WZ;
+    __syncthreads();
+    // g of the brick (zero outside the domain) in two parts, one position per thread slot
+    // (consecutive threads: consecutive x), and the thread's db shares: slot e = tid + 512 i
+    // holds channel e / 512 = i
+    {
+      uint16_t *gh = reinterpret_cast<uint16_t *>(&L.g[0][0][0]), *gl = reinterpret_cast<uint16_t *>(&L.g[1][0][0]);
+      const int px = tid % WX, py = (tid / WX) % WY, pz = tid / (WX * WY);
+      const int gx = x0 + px, gy = y0 + py, gz = z0 + pz;
+      const bool in = gx < nx && gy < ny && gz < nz;
+      const size_t pp = in ? ((size_t)gz * ny + gy) * nx + gx : 0;
+#pragma unroll 8
+      for (int co = 0; co < 32; ++co) {
+        const float v = in ? g[(size_t)co * nxyz + pp] : 0.f;
+        dbs[co] += v;
+        uint32_t h, l;
+        split2(v, h, l);
+        gh[co * kWgRow * 8 + tid] = (uint16_t)h;
+        gl[co * kWgRow * 8 + tid] = (uint16_t)l;
+      }
     }
-    grad[e] = s;
+    for (int c = 0; c < nch; ++c) {
+      const int c0 = 8 * c;
+      __syncthreads();
+      // x of the chunk with its halo (18 positions per row), two parts, each value loaded once
+      // (consecutive threads: consecutive x) and written to the copies shifted by dx = 0, 1, 2
+      {
+        constexpr int HXW = WX + 2;
+        uint16_t *xb = reinterpret_cast<uint16_t *>(&L.x[0][0][0][0]);
+        constexpr int kCi = kWxCi * 8, kPart = 8 * kCi, kShift = 2 * kPart;   // uint16 strides
+        for (int e = tid; e < 8 * WHZ * WHY * HXW; e += 512) {
+          const int hx = e % HXW, t1 = e / HXW, hy = t1 % WHY, t2 = t1 / WHY, hz = t2 % WHZ, ci = t2 / WHZ;
+          const int gx = wrapi(x0 - 1 + hx, nx), gy = wrapi(y0 - 1 + hy, ny), gz = wrapi(z0 - 1 + hz, nz);
+          const float v = c0 + ci < cin ? x[(size_t)(c0 + ci) * nxyz + ((size_t)gz * ny + gy) * nx + gx] : 0.f;
+          uint32_t h, l;
+          split2(v, h, l);
+          const int rowbase = ci * kCi + (hz * WHY + hy) * WX;
+#pragma unroll
+          for (int dx = 0; dx < 3; ++dx) {
+            const int j = hx - dx;                     // copy dx holds x0 + j + dx - 1 at j
+            if (j < 0 || j >= WX) continue;
+            xb[dx * kShift + rowbase + j] = (uint16_t)h;
+            xb[dx * kShift + kPart + rowbase + j] = (uint16_t)l;
+          }
+        }
+      }
+      __syncthreads();
+      if (wave < 7) {
+#pragma unroll 2
+        for (int ks = 0; ks < WY * WZ; ++ks) {
+          const int ry = ks % WY, rz = ks / WY;          // the K step's row of the brick
+          bf16x8_t ah, al, bh, bl;
+          const uint4 gh = L.g[0][n][(rz * WY + ry) * 2 + half], gl = L.g[1][n][(rz * WY + ry) * 2 + half];
+          const int xo = ((rz + tdz) * WHY + (ry + tdy)) * 2 + half;
+          const uint4 xh = L.x[tdx][0][cil][xo];
+          const uint4 xl = L.x[tdx][1][cil][xo];
+          __builtin_memcpy(&ah, &gh, 16);
+          __builtin_memcpy(&al, &gl, 16);
+          __builtin_memcpy(&bh, &xh, 16);
+          __builtin_memcpy(&bl, &xl, 16);
+#pragma unroll
+          for (int cc = 0; cc < kWChunks; ++cc)
+            if (cc == c) {
+              acc[cc] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc[cc], 0, 0, 0);
+              acc[cc] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc[cc], 0, 0, 0);
+              acc[cc] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc[cc], 0, 0, 0);
+            }
+        }
+      }
+    }
   }
+  // partials of this block, in the layout of the scalar kernel: [block][co][cin][27], then db
+  const int npc = gridDim.x;
+  if (wave < 7 && tap < 27) {
+#pragma unroll
+    for (int cc = 0; cc < kWChunks; ++cc) {
+      const int ci = 8 * cc + cil;
+      if (ci >= cin) continue;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int co = 8 * (r >> 2) + 4 * half + (r & 3);
+        part[(((size_t)blockIdx.x * 32 + co) * cin + ci) * 27 + tap] = acc[cc][r];
+      }
+    }
+  }
+  // db: per channel, the shares of the block's threads summed in a fixed order (waves, then
+  // the eight wave sums through LDS, which the staging no longer needs)
+  __syncthreads();
+  float *red = reinterpret_cast<float *>(&L.g[0][0][0]);
+#pragma unroll
+  for (int co = 0; co < 32; ++co) {
+    const float sw = wave_sum(dbs[co]);
+    if (lane == 0) red[co * 8 + wave] = sw;
+  }
+  __syncthreads();
+  if (tid < 32) {
+    float sdb = 0.f;
+    for (int w = 0; w < 8; ++w) sdb += red[tid * 8 + w];
+    part[(size_t)npc * 32 * cin * 27 + (size_t)blockIdx.x * 32 + tid] = sdb;
+  }
+}
+
+// Sum the partials over position chunks into grad (w then b of the layer): a block takes 64
+// values, each of its four thread rows a quarter of the chunks, and the quarter sums are added
+// in order (a fixed order: the result does not depend on timing).
+__global__ void __launch_bounds__(256) k_wgrad_reduce(const float *__restrict__ part, float *__restrict__ grad, int nw,
+                                                      int nb, int npc) {
+  __shared__ float qs[4][64];
+  const int n = nw + nb;
+  const int el = threadIdx.x & 63, qr = threadIdx.x >> 6;
+  const int e = blockIdx.x * 64 + el;
+  const int cq = (npc + 3) / 4, ca = qr * cq, cb = min(npc, ca + cq);
+  float s = 0.f;
+  if (e < nw) {
+    for (int c = ca; c < cb; ++c) s += part[(size_t)c * nw + e];
+  } else if (e < n) {
+    const int co = e - nw;
+    for (int c = ca; c < cb; ++c) s += part[(size_t)npc * nw + (size_t)c * nb + co];
+  }
+  qs[qr][el] = s;
+  __syncthreads();
+  if (qr == 0 && e < n) grad[e] = ((qs[0][el] + qs[1][el]) + qs[2][el]) + qs[3][el];
 }
 
 // Adam on the float32 masters, then the bf16 copy of every weight (biases stay float32).
@@ -577,6 +742,17 @@ void launch_fwd(hipStream_t st, const float *x, const uint16_t *w, const float *
 
 bool cout_ok(int c) { return c == 1 || c == 8 || c == 9 || c == 16 || c == 32; }
 
+// blocks of the matrix-core weight gradient (one per CU's worth of bricks, at most 256) and
+// the bricks each walks
+int wgrad_bricks_total(const aero_cnn_desc &d) {
+  return ((d.nx + WX - 1) / WX) * ((d.ny + WY - 1) / WY) * ((d.nz + WZ - 1) / WZ);
+}
+int wgrad_blocks(const aero_cnn_desc &d) {
+  const int nbr = wgrad_bricks_total(d);
+  const int per = (nbr + 255) / 256;
+  return (nbr + per - 1) / per;
+}
+
 }  // namespace
 
 extern "C" {
@@ -597,6 +773,8 @@ int aero_cnn_create(const aero_cnn_desc *d, aero_cnn **out) {
   // the matrix-core convolution stages more than the default 64 KB of LDS
   AERO_HIP_CHECK(hipFuncSetAttribute((const void *)k_conv_fwd_mfma, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)sizeof(MfmaLds)));
+  AERO_HIP_CHECK(hipFuncSetAttribute((const void *)k_conv_wgrad_mfma, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)sizeof(WgradLds)));
   aero_cnn *c = new aero_cnn();
   c->desc = *d;
   c->nxyz = (size_t)d->nx * d->ny * d->nz;
@@ -611,7 +789,8 @@ int aero_cnn_create(const aero_cnn_desc *d, aero_cnn **out) {
   c->nparam = off;
   const int mc = max_ch(*d);
   const size_t nbr = (size_t)((d->nx + TX - 1) / TX) * ((d->ny + TY - 1) / TY) * ((d->nz + TZ - 1) / TZ);
-  const size_t npc = (nbr + kWgBricks - 1) / kWgBricks;
+  // weight-gradient partials: the scalar kernel's position chunks or the matrix-core kernel's blocks
+  const size_t npc = std::max((nbr + kWgBricks - 1) / kWgBricks, (size_t)wgrad_blocks(*d));
   int rc = AERO_OK;
   auto A = [&](void **p, size_t n) { if (!rc) rc = cnn_alloc(c, p, n); };
   A((void **)&c->params, off * 4);
@@ -716,7 +895,14 @@ int aero_cnn_backward(aero_cnn *c, void *stream, const float *dy, float *dx) {
     float *g = (l % 2 == 0) ? c->g0 : c->g1;            // gradient w.r.t. the pre-activation
     hipLaunchKernelGGL(k_act_grad, dim3(1024), dim3(256), 0, st, up, c->pre[l], g, (size_t)co * n, d.act[l]);
     const dim3 wg((ci + CI_CH - 1) / CI_CH, npc);
-    if (co == 1)
+    int npart = npc;
+    if (co == 32) {
+      const int nbr_m = wgrad_bricks_total(d), nblk = wgrad_blocks(d);
+      const int per = (nbr_m + nblk - 1) / nblk;
+      hipLaunchKernelGGL(k_conv_wgrad_mfma, dim3(nblk), dim3(512), sizeof(WgradLds), st, c->act[l], g, c->part, ci,
+                         d.nz, d.ny, d.nx, per);
+      npart = nblk;
+    } else if (co == 1)
       hipLaunchKernelGGL(k_conv_wgrad<1>, wg, dim3(256), 0, st, c->act[l], g, c->part, ci, d.nz, d.ny, d.nx,
                          kWgBricks);
     else if (co == 8)
@@ -732,8 +918,8 @@ int aero_cnn_backward(aero_cnn *c, void *stream, const float *dy, float *dx) {
       hipLaunchKernelGGL(k_conv_wgrad<32>, wg, dim3(256), 0, st, c->act[l], g, c->part, ci, d.nz, d.ny, d.nx,
                          kWgBricks);
     const int nw = co * ci * 27;
-    hipLaunchKernelGGL(k_wgrad_reduce, dim3((nw + co + 255) / 256), dim3(256), 0, st, c->part, c->grads + c->woff[l],
-                       nw, co, npc);
+    hipLaunchKernelGGL(k_wgrad_reduce, dim3((nw + co + 63) / 64), dim3(256), 0, st, c->part, c->grads + c->woff[l],
+                       nw, co, npart);
     if (l > 0 || dx) {
       // data gradient: the forward convolution with flipped, transposed weights, no bias
       hipLaunchKernelGGL(k_flip, dim3(64), dim3(256), 0, st, c->w16 + c->woff[l], c->wt16, co, ci);

@@ -15,6 +15,11 @@ per launch / its average duration timed with HIP events on the solver's stream) 
 batched ensemble (solves/s for 1000-step solves) against the same solves run one after
 another on the same GPU.
 
+``workloads.closure_cnn``: the learned closure's network (9 -> 32 -> 32 -> 32 -> 32 -> 1 channels
+of 3x3x3 periodic convolutions, bf16 weights, float32 accumulation) on a 64^3 grid: inference,
+training forward and backward in ms (best of three blocks of 10 calls, HIP events), and the
+matrix throughput of the forward pass against the 2.5 PFLOP/s dense bf16 peak.
+
 ``workloads.3d_car`` carries the same figures for the 3D car case at 256x128x128 (4.19M
 cells, FP32): a 4 x 1.6 x 1.2 rounded box on four rolling wheels over a floor moving at U,
 Re 5e4, WALE, periodic in y, upwind3 + SSP-RK3 at CFL 0.5, exact DCT/DHT projection;
@@ -54,8 +59,8 @@ STEP3_BYTES_PER_CELL = 312
 # HBM bytes per stage-2 launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by the
 # same counters on a 512 MiB device copy (fetch x2.000, write x1.000):
 # profiles/stage_kernels_hbm_counters.md
-TRAFFIC_STAGE_2D = 28.93e6
-TRAFFIC_STAGE_3D = 184.16e6
+TRAFFIC_STAGE_2D = 28.87e6
+TRAFFIC_STAGE_3D = 183.90e6
 
 
 def log(msg: str) -> None:
@@ -415,6 +420,49 @@ def cpu_baseline(block_steps: int = 60):
     return cpu_report(step, NX * NY, block_steps, "the 1024x512 heated-airfoil case")
 
 
+CNN_GRID = 64
+CNN_CHANNELS = (9, 32, 32, 32, 32, 1)
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def gpu_run_cnn(grid: int = CNN_GRID, calls: int = 10) -> dict:
+    """The closure network at grid^3: ms per inference / training forward / backward."""
+    import torch
+    from aerolume_amd.closure.network import CNN3D
+    net = CNN3D((grid, grid, grid), CNN_CHANNELS, seed=1)
+    g = torch.Generator(device="cuda").manual_seed(505)
+    x = torch.randn(CNN_CHANNELS[0], grid, grid, grid, device="cuda", generator=g)
+    dy = torch.randn(CNN_CHANNELS[-1], grid, grid, grid, device="cuda", generator=g)
+    y = torch.empty(CNN_CHANNELS[-1], grid, grid, grid, device="cuda")
+
+    def best(fn) -> float:
+        fn()
+        torch.cuda.synchronize()
+        out = []
+        for _ in range(TIMED_BLOCKS):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(calls):
+                fn()
+            e1.record()
+            torch.cuda.synchronize()
+            out.append(e0.elapsed_time(e1) / calls)
+        return min(out)
+
+    infer = best(lambda: net.infer(x, out=y))
+    fwd = best(lambda: net.forward(x))
+    net.forward(x)
+    bwd = best(lambda: net.backward(dy))
+    cells = grid ** 3
+    # useful flops of the forward pass: 2 x 27 x cin x cout per cell per layer
+    flops = sum(2 * 27 * ci * co for ci, co in zip(CNN_CHANNELS[:-1], CNN_CHANNELS[1:])) * cells
+    return {"grid": [grid] * 3, "channels": list(CNN_CHANNELS), "infer_ms": round(infer, 4),
+            "forward_ms": round(fwd, 4), "backward_ms": round(bwd, 4),
+            "forward_gflop": round(flops / 1e9, 2), "infer_tflops": round(flops / (infer * 1e-3) / 1e12, 1),
+            "infer_frac_of_bf16_peak": round(flops / (infer * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, 4),
+            "dtype": "bf16 weights, f32 data", "data": "synthetic"}
+
+
 def run_only(args) -> int:
     """One workload alone, for profiler runs: prints a short JSON summary."""
     import torch
@@ -429,6 +477,8 @@ def run_only(args) -> int:
         out.update(ms_per_step=min(walls) / args.steps3d * 1e3, stage_us=stage_ms * 1e3)
     elif args.only == "ensemble":
         out.update(gpu_run_ensemble(args.ensemble_steps, 20))
+    elif args.only == "cnn":
+        out.update(gpu_run_cnn())
     else:
         n = 512 * 2**20 // 4
         x = torch.rand(n, device="cuda")
@@ -451,10 +501,11 @@ def main() -> int:
     ap.add_argument("--warmup3d", type=int, default=20)
     ap.add_argument("--no-3d", action="store_true", help="skip the 3D car workload")
     ap.add_argument("--ensemble-steps", type=int, default=1000)
-    ap.add_argument("--only", choices=("2d", "3d", "ensemble", "copy"), default=None,
+    ap.add_argument("--only", choices=("2d", "3d", "ensemble", "cnn", "copy"), default=None,
                     help="run one workload and print its figures (profiling runs); 'copy' times a plain "
                          "512 MiB device copy (the HBM counter calibration)")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble workload")
+    ap.add_argument("--no-cnn", action="store_true", help="skip the closure-network workload")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -520,6 +571,8 @@ def main() -> int:
         line.setdefault("workloads", {})["3d_car"] = w3
     if not args.no_ensemble:
         line.setdefault("workloads", {})["ensemble"] = gpu_run_ensemble(args.ensemble_steps, 20)
+    if not args.no_cnn:
+        line.setdefault("workloads", {})["closure_cnn"] = gpu_run_cnn()
     print(json.dumps(line), flush=True)
     return 0
 

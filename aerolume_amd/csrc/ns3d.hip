@@ -231,9 +231,10 @@ __device__ __forceinline__ int src_axis(int g, int ext, int n, int per) {
 // Pass 1 of the tile load for component C: the stage input at every LDS position (clamped
 // or wrapped source), u^ - grad psi with the outlet shift when c.psi is set (the previous
 // stage's projected field, core.py:281-297, 582-612). The tile is walked row by row: each
-// half-wave takes one (z, y) row of the tile, its lanes the row's x positions (32 at a time),
-// so the source column of each lane and the source row of each half-wave are worked out once
-// (shifts and masks, no per-element division) and reused across the row's elements.
+// half-wave takes one (z, y) row of the tile, its lanes the row's first 32 x positions, so the
+// source column of each lane and the source row of each half-wave are worked out once; the
+// few positions past 32 of every row are loaded together, one per thread. Element offsets are
+// 32-bit, as in the rest of the tile code.
 template <typename T, int C, int EX, int EY, int EZ>
 __device__ __forceinline__ void load_comp(const Dev3<T> &d, const Call3<T> &c, T *lds, int k0, int j0, int i0, T corr) {
   const int nx = d.nx, ny = d.ny, nz = d.nz;
@@ -246,50 +247,68 @@ __device__ __forceinline__ void load_comp(const Dev3<T> &d, const Call3<T> &c, T
   const T inv_h = d.inv_h;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int xl = lane & 31;
-  constexpr int NXP = (EX + 31) / 32;                        // 32-wide passes over a row
-  int sx[NXP], fcx[NXP], fmx[NXP];
-  bool xin[NXP], pvx[NXP], shx[NXP];
-#pragma unroll
-  for (int q = 0; q < NXP; ++q) {
-    const int x = xl + 32 * q;
-    xin[q] = x < EX;
-    const int i = src_axis(i0 - 2 + x, ex, nx, d.per[0]);
-    sx[q] = i;
-    const int fc = i == pn ? 0 : i;
-    fcx[q] = fc;
-    fmx[q] = fc == 0 ? pn - 1 : fc - 1;
-    pvx[q] = C == 0 && (pper || (i > 0 && i < pn));
-    shx[q] = shift && i == nx;
-  }
+  // column part of tile position x: source column, psi columns, gradient and outlet flags
+  struct Col {
+    int i, fc, fm;
+    bool pv, sh;
+  };
+  auto col_of = [&](int x) {
+    Col cl;
+    cl.i = src_axis(i0 - 2 + x, ex, nx, d.per[0]);
+    cl.fc = cl.i == pn ? 0 : cl.i;
+    cl.fm = cl.fc == 0 ? pn - 1 : cl.fc - 1;
+    cl.pv = C == 0 && (pper || (cl.i > 0 && cl.i < pn));
+    cl.sh = shift && cl.i == nx;
+    return cl;
+  };
+  // row part of tile row (y, z): source row and plane, and for v (w) the row-uniform normal face
+  struct Row {
+    int j, k, fc, fm;
+    bool pv;
+  };
+  auto row_of = [&](int y, int z) {
+    Row rw;
+    rw.j = src_axis(j0 - 2 + y, ey, ny, d.per[1]);
+    rw.k = src_axis(k0 - 2 + z, ez, nz, d.per[2]);
+    const int f = C == 1 ? rw.j : rw.k;
+    rw.pv = C != 0 && (pper || (f > 0 && f < pn));
+    rw.fc = f == pn ? 0 : f;
+    rw.fm = rw.fc == 0 ? pn - 1 : rw.fc - 1;
+    return rw;
+  };
+  // one tile element: the stage input, u^ - grad psi when psi is set, the outlet shift
+  auto value = [&](const Col &cl, const Row &rw) {
+    T val = src[(unsigned)((rw.k * ey + rw.j) * ex + cl.i)];
+    if (psi) {
+      if (C == 0) {
+        if (cl.pv) {
+          const T *pb = psi + (unsigned)((rw.k * ny + rw.j) * nx);
+          val = val - (pb[cl.fc] - pb[cl.fm]) * inv_h;
+        }
+      } else if (rw.pv) {
+        const unsigned pstride = C == 1 ? (unsigned)nx : (unsigned)(nx * ny);
+        const T *pb = psi + (C == 1 ? (unsigned)(rw.k * ny * nx) : (unsigned)(rw.j * nx)) + cl.i;
+        val = val - (pb[rw.fc * pstride] - pb[rw.fm * pstride]) * inv_h;
+      }
+    }
+    if (cl.sh) val = val + corr;
+    return val;
+  };
+  // positions x < 32 of every row: one half-wave per row, the lane's column worked out once
+  constexpr int XM = EX < 32 ? EX : 32, XR = EX - XM;
+  const bool xin = xl < XM;
+  const Col cm = col_of(xin ? xl : 0);
   for (int row = 2 * wave + (lane >> 5); row < EY * EZ; row += 2 * kTileWaves) {
     const int y = row % EY, z = row / EY;
-    const int j = src_axis(j0 - 2 + y, ey, ny, d.per[1]);
-    const int k = src_axis(k0 - 2 + z, ez, nz, d.per[2]);
-    const T *srow = src + (size_t)(k * ey + j) * ex;
-    T *lrow = lds + (z * EY + y) * EX;
-    // the normal-axis face index of a v (w) row is its y (z) position: row-uniform
-    const int f = C == 1 ? j : k;
-    const bool pv_row = C != 0 && (pper || (f > 0 && f < pn));
-    const int fc = f == pn ? 0 : f;
-    const int fm = fc == 0 ? pn - 1 : fc - 1;
-#pragma unroll
-    for (int q = 0; q < NXP; ++q) {
-      if (!xin[q]) continue;
-      T val = srow[sx[q]];
-      if (psi) {
-        if (C == 0) {
-          if (pvx[q]) {
-            const T *pb = psi + (size_t)(k * ny + j) * nx;
-            val = val - (pb[fcx[q]] - pb[fmx[q]]) * inv_h;
-          }
-        } else if (pv_row) {
-          const size_t pstride = C == 1 ? (size_t)nx : (size_t)nx * ny;
-          const T *pb = psi + (C == 1 ? (size_t)k * ny * nx : (size_t)j * nx) + sx[q];
-          val = val - (pb[fc * pstride] - pb[fm * pstride]) * inv_h;
-        }
-      }
-      if (shx[q]) val = val + corr;
-      lrow[xl + 32 * q] = val;
+    const Row rw = row_of(y, z);
+    if (xin) lds[(z * EY + y) * EX + xl] = value(cm, rw);
+  }
+  // the remaining XR columns of all rows together (so no half-wave pass runs mostly idle)
+  if constexpr (XR > 0) {
+    for (int e = threadIdx.x; e < EY * EZ * XR; e += kTileThreads) {
+      const int xr = e % XR, row = e / XR;
+      const int y = row % EY, z = row / EY;
+      lds[(z * EY + y) * EX + XM + xr] = value(col_of(XM + xr), row_of(y, z));
     }
   }
 }

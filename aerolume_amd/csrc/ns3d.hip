@@ -313,40 +313,6 @@ __device__ __forceinline__ void load_comp(const Dev3<T> &d, const Call3<T> &c, T
   }
 }
 
-// The same values element by element (the index split per element): fewer registers than the
-// row walk, which the final kernel needs at its register budget.
-template <typename T, int C, int EX, int EY, int EZ>
-__device__ __forceinline__ void load_comp_cells(const Dev3<T> &d, const Call3<T> &c, T *lds, int k0, int j0, int i0, T corr) {
-  const int nx = d.nx, ny = d.ny, nz = d.nz;
-  const int ex = nx + (C == 0), ey = ny + (C == 1), ez = nz + (C == 2);
-  const T *src = C == 0 ? c.up : (C == 1 ? c.vp : c.wp);
-  const T *psi = c.psi;
-  const int pn = C == 0 ? nx : (C == 1 ? ny : nz);         // cells along the normal axis
-  const int pper = d.per[C];
-  const int pstride = C == 0 ? 1 : (C == 1 ? nx : nx * ny);
-  const bool shift = c.corr_stage >= 0 && C == 0 && d.outlet;
-  const T inv_h = d.inv_h;
-  for (int e = threadIdx.x; e < EX * EY * EZ; e += kTileThreads) {
-    const int x = e % EX, t = e / EX, y = t % EY, z = t / EY;
-    const int i = src_axis(i0 - 2 + x, ex, nx, d.per[0]);
-    const int j = src_axis(j0 - 2 + y, ey, ny, d.per[1]);
-    const int k = src_axis(k0 - 2 + z, ez, nz, d.per[2]);
-    T val = src[(unsigned)((k * ey + j) * ex + i)];
-    if (psi) {
-      const int f = C == 0 ? i : (C == 1 ? j : k);
-      if (pper || (f > 0 && f < pn)) {
-        const int fc = f == pn ? 0 : f;
-        const int fm = fc == 0 ? pn - 1 : fc - 1;
-        const int ci = C == 0 ? 0 : i, cj = C == 1 ? 0 : j, ck = C == 2 ? 0 : k;
-        const T *pb = psi + (unsigned)((ck * ny + cj) * nx + ci);
-        val = val - (pb[(unsigned)(fc * pstride)] - pb[(unsigned)(fm * pstride)]) * inv_h;
-      }
-    }
-    if (shift && i == nx) val = val + corr;
-    lds[e] = val;
-  }
-}
-
 // Pass 2 along one axis: ghost positions of a non-periodic side from their mirror source in
 // LDS (ns3d/core.py:223-272), for every position of the other two axes. Runs after the
 // lower axes' pass, so corner values compose as pad3 does (x rule, then y, then z).
@@ -388,18 +354,12 @@ __device__ __forceinline__ void ghost_fill(const Dev3<T> &d, T *lds, int comp, i
   }
 }
 
-template <typename T, bool ROWS = true>
+template <typename T>
 __device__ __forceinline__ void load_tile(const Dev3<T> &d, const Call3<T> &c, Lds3<T> &L, int k0, int j0, int i0, T corr) {
   using S = Lds3<T>;
-  if constexpr (ROWS) {
-    load_comp<T, 0, S::UX, S::UY, S::UZ>(d, c, &L.u[0][0][0], k0, j0, i0, corr);
-    load_comp<T, 1, S::VX, S::VY, S::VZ>(d, c, &L.v[0][0][0], k0, j0, i0, corr);
-    load_comp<T, 2, S::WX, S::WY, S::WZ>(d, c, &L.w[0][0][0], k0, j0, i0, corr);
-  } else {
-    load_comp_cells<T, 0, S::UX, S::UY, S::UZ>(d, c, &L.u[0][0][0], k0, j0, i0, corr);
-    load_comp_cells<T, 1, S::VX, S::VY, S::VZ>(d, c, &L.v[0][0][0], k0, j0, i0, corr);
-    load_comp_cells<T, 2, S::WX, S::WY, S::WZ>(d, c, &L.w[0][0][0], k0, j0, i0, corr);
-  }
+  load_comp<T, 0, S::UX, S::UY, S::UZ>(d, c, &L.u[0][0][0], k0, j0, i0, corr);
+  load_comp<T, 1, S::VX, S::VY, S::VZ>(d, c, &L.v[0][0][0], k0, j0, i0, corr);
+  load_comp<T, 2, S::WX, S::WY, S::WZ>(d, c, &L.w[0][0][0], k0, j0, i0, corr);
   __syncthreads();
   // boundary tiles: ghost layers of non-periodic sides, axis by axis
   const bool bx = !d.per[0] && (i0 == 0 || i0 + S::TX + 3 > d.nx);
@@ -848,7 +808,7 @@ __global__ void __launch_bounds__(kTileThreads, Tile<T>::WAVES_PER_SIMD) k_final
   const int i0 = bx * S::TX, j0 = by * S::TY, k0 = bz * S::TZ;
   const int nx = d.nx, ny = d.ny, nz = d.nz;
   const T corr = c.corr_stage >= 0 ? d.red->corr[c.corr_stage] : (T)0;
-  load_tile<T, false>(d, c, L, k0, j0, i0, corr);
+  load_tile(d, c, L, k0, j0, i0, corr);
   T mnu = 0;
   if (d.turb) mnu = nu_tile(d, c, L, k0, j0, i0, false, true);
   const int ie = min(i0 + S::TX, nx), je = min(j0 + S::TY, ny), ke = min(k0 + S::TZ, nz);

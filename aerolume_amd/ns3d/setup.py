@@ -351,16 +351,18 @@ def body_window3d(vertex_sets, ops: Ops3D, margin_cells: float | None = None, *,
                   yaw_deg: float = 0.0) -> tuple:
     """Cell window covering every vertex plus a margin of mask_width + 3 + |shift| cells,
     clipped to the domain (reference ``ns3d/geometry.py:323-356``)."""
-    sets = [np.asarray(v, dtype=np.float64).reshape(-1, 3) for v in vertex_sets if len(v)]
-    if not sets:
+    clouds = [c for c in (np.asarray(v, dtype=np.float64).reshape(-1, 3) for v in vertex_sets) if c.size]
+    if not clouds:
         return (0, 0, 0, 0, 0, 0)
-    pts = np.vstack(sets)
+    pts = np.concatenate(clouds)
     if pitch_deg > 0 or yaw_deg > 0:
+        # the bodies at every attitude a live edit may give them: 7 samples of each swept angle
+        def sweep(amp):
+            return np.linspace(-amp, amp, 7) if amp > 0 else np.zeros(1)
+
         ref = np.asarray(ops.moment_ref, dtype=np.float64)
-        rot = [(pts - ref) @ rotation_matrix(pa, ya).T + ref
-               for pa in np.linspace(-pitch_deg, pitch_deg, 7 if pitch_deg > 0 else 1)
-               for ya in np.linspace(-yaw_deg, yaw_deg, 7 if yaw_deg > 0 else 1)]
-        pts = np.vstack(rot)
+        arm = pts - ref
+        pts = np.concatenate([arm @ rotation_matrix(a, b).T + ref for a in sweep(pitch_deg) for b in sweep(yaw_deg)])
     margin = (ops.mask_width + 3.0 + abs(ops.mask_shift)) if margin_cells is None else margin_cells
     h = ops.h
     lo, hi = pts.min(axis=0), pts.max(axis=0)
@@ -493,12 +495,12 @@ def build_geometry3d(bodies, ops: Ops3D, *, window=None) -> Geometry3D:
         a_u = a_u + cu[i] * (om * (ay * (zu - cz) - az * (yu - cy)))
         a_v = a_v + cv[i] * (om * (az * (xv - cx) - ax * (zv - cz)))
         a_w = a_w + cw[i] * (om * (ax * (yw - cy) - ay * (xw - cx)))
-    if nb == 1:
-        wu, wv, ww = torch.ones_like(cu), torch.ones_like(cv), torch.ones_like(cw)
-    else:
-        wu = cu / (cu.sum(0, keepdim=True) + 1e-30)
-        wv = cv / (cv.sum(0, keepdim=True) + 1e-30)
-        ww = cw / (cw.sum(0, keepdim=True) + 1e-30)
+    # each body's share of a face's penalisation force: its mask over the summed masks (a lone
+    # body takes the whole force)
+    def share(c):
+        return torch.ones_like(c) if nb == 1 else c / (c.sum(0, keepdim=True) + 1e-30)
+
+    wu, wv, ww = share(cu), share(cv), share(cw)
     xr, yr, zr = ops.moment_ref
     # M = r x F: Fx at u-faces gives My += (z - zr) Fx, Mz -= (y - yr) Fx; Fy at v-faces gives
     # Mx -= (z - zr) Fy, Mz += (x - xr) Fy; Fz at w-faces gives Mx += (y - yr) Fz, My -= (x - xr) Fz

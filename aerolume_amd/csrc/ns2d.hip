@@ -135,6 +135,14 @@ struct DevK : DevLT<T, LES, TH> {
 template <typename T, int LES, int TH>
 struct DevK<T, LES, TH, 0> : DevLT<T, LES, TH> {};
 
+// a plan view of d (the same values; the view's static members replace some flags)
+template <typename D, typename T>
+__host__ __device__ D view_of(const Dev<T> &d) {
+  D k;
+  static_cast<Dev<T> &>(k) = d;
+  return k;
+}
+
 template <typename T>
 struct Stage {
   int s;
@@ -1898,12 +1906,15 @@ struct BRun {
   int nslotB, mode;
 };
 
-template <typename T, int R>
+// D: the tile kernels' view of every run's plan (Dev<T> itself: generic); the run's plan is
+// copied into it, and only the fields the view's code reads stay live
+template <typename T, int R, typename D = Dev<T>>
 __global__ void __launch_bounds__(256) kb_pre(const Dev<T> *__restrict__ devs, const BRun<T> *__restrict__ runs,
                                              const int4 *__restrict__ items) {
   const int4 it = items[blockIdx.x];
   const BRun<T> &r = runs[it.x];
-  pre_body<T, R>(devs[it.x], it.y, it.z, r.u0, r.v0, r.th0);
+  const D d = view_of<D>(devs[it.x]);
+  pre_body<T, R>(d, it.y, it.z, r.u0, r.v0, r.th0);
 }
 
 template <typename T>
@@ -1911,11 +1922,17 @@ __global__ void __launch_bounds__(1024) kb_dt(const Dev<T> *__restrict__ devs) {
   dt_body<T>(devs[blockIdx.x]);
 }
 
-template <typename T, int R>
+template <typename T, int R, typename D = Dev<T>>
 __global__ void __launch_bounds__(256) kb_stage(const Dev<T> *__restrict__ devs, const BRun<T> *__restrict__ runs,
                                                const int4 *__restrict__ items, int stage) {
   const int4 it = items[blockIdx.x];
-  stage_body<T, R>(devs[it.x], runs[it.x].st[stage], it.y, it.z);
+  const D d = view_of<D>(devs[it.x]);
+  // tiles clear of the run's body window (2 cells around it) run without any window code
+  const int tj0 = it.z * R, ti0 = it.y * TX;
+  const bool touch = d.win && (d.periodic || (tj0 - 2 <= d.j1 && tj0 + R + 2 >= d.j0 && ti0 - 2 <= d.i1 &&
+                                              ti0 + TX + 2 >= d.i0));
+  if (touch) stage_body<T, R, true>(d, runs[it.x].st[stage], it.y, it.z);
+  else stage_body<T, R, false>(d, runs[it.x].st[stage], it.y, it.z);
 }
 
 template <typename T, int NT>
@@ -1942,13 +1959,21 @@ __global__ void __launch_bounds__(NT) kb_poisC(const Dev<T> *__restrict__ devs, 
   poisC_body<T, NT>(d, it.y, d.psi, write_p ? runs[it.x].p_out : (T *)nullptr, stage);
 }
 
-template <typename T, int R>
+template <typename T, int R, typename D = Dev<T>>
 __global__ void __launch_bounds__(256) kb_final(const Dev<T> *__restrict__ devs, const BRun<T> *__restrict__ runs,
                                                const int4 *__restrict__ items) {
   const int4 it = items[blockIdx.x];
   const BRun<T> &r = runs[it.x];
-  final_body<T, R>(devs[it.x], it.y, it.z, r.uC, r.vC, r.u_out, r.v_out, r.th_new, (const T *)nullptr, (T *)nullptr,
-                r.t_in);
+  const D d = view_of<D>(devs[it.x]);
+  const int tj0 = it.z * R, ti0 = it.y * TX;
+  const bool touch = d.win && (d.periodic || (tj0 - 2 <= d.j1 && tj0 + R + 2 >= d.j0 && ti0 - 2 <= d.i1 &&
+                                              ti0 + TX + 2 >= d.i0));
+  if (touch)
+    final_body<T, R, true>(d, it.y, it.z, r.uC, r.vC, r.u_out, r.v_out, r.th_new, (const T *)nullptr, (T *)nullptr,
+                           r.t_in);
+  else
+    final_body<T, R, false>(d, it.y, it.z, r.uC, r.vC, r.u_out, r.v_out, r.th_new, (const T *)nullptr,
+                            (T *)nullptr, r.t_in);
 }
 
 template <typename T>
@@ -2032,13 +2057,6 @@ void by_tile(const aero2d_plan *p, F &&f) {
   }
 }
 
-// a plan view of d (the same values; the view's static members replace some flags)
-template <typename D, typename T>
-D view_of(const Dev<T> &d) {
-  D k;
-  static_cast<Dev<T> &>(k) = d;
-  return k;
-}
 
 // calls f(std::integral_constant<int, NT>) for the transform block size NT = nt (256, 512, 1024)
 template <typename F>
@@ -2641,6 +2659,7 @@ struct aero2d_ensemble {
   int ty = kDefaultRows;   // rows per tile of the batched cell launches
   int nslot = 2;           // column pairs per block of the batched pass-B launch (capped per run)
   int ntrow = kFftThreads; // threads per block of the batched row passes (A, C): 256 or 512
+  bool plain_laminar = false;   // every run a plain wind tunnel without LES or heat transfer
   std::vector<aero2d_plan *> plans;
   std::vector<aero2d_fields> fields;
   std::vector<aero2d_geom> geoms;
@@ -2679,6 +2698,19 @@ int run_slots(const aero2d_plan *p, int want) {
                    (size_t)s * ny * (2 * sizeof(cplx<T>) + 2 * sizeof(T)) + ny * sizeof(cplx<T>) > 65536))
     s /= 2;
   return s;
+}
+
+// calls f(Tag<D>) with the view of the ensemble's tile kernels: the plain laminar view when every
+// run is a plain wind tunnel without LES or heat transfer, the generic Dev<T> otherwise
+template <typename T, typename F>
+void by_ens_view(const aero2d_ensemble *e, F &&f) {
+  if (e->plain_laminar) f(Tag<DevK<T, 0, 0, 1>>{});
+  else f(Tag<Dev<T>>{});
+}
+
+bool plain_laminar(const aero2d_desc &s) {
+  return s.turbulence == 0 && !s.thermal && !s.periodic && !s.closed && s.advection == 1 && !s.wall_model &&
+         !s.has_heat;
 }
 
 // (re)builds the device run tables for the tile rows e->ty and column slots e->nslot
@@ -2756,14 +2788,16 @@ int ensemble_step(aero2d_ensemble *e, hipStream_t st, bool chained) {
   const BRun<T> *runs = (const BRun<T> *)e->runs;
   const int4 *tiles = (const int4 *)e->tiles, *rows = (const int4 *)e->rows, *cols = (const int4 *)e->cols;
   const dim3 blk(64, 4);
+  by_ens_view<T>(e, [&](auto tag) {
+  using D = typename decltype(tag)::type;
   by_rows(e->ty, [&](auto rr) {
     constexpr int R = decltype(rr)::value;
     if (!chained) {
-      hipLaunchKernelGGL((kb_pre<T, R>), dim3(e->ntiles), blk, 0, st, devs, runs, tiles);
+      hipLaunchKernelGGL((kb_pre<T, R, D>), dim3(e->ntiles), blk, 0, st, devs, runs, tiles);
       hipLaunchKernelGGL(kb_dt<T>, dim3(e->n), dim3(1024), 0, st, devs);
     }
     for (int k = 0; k < 3; ++k) {
-      hipLaunchKernelGGL((kb_stage<T, R>), dim3(e->ntiles), blk, 0, st, devs, runs, tiles, k);
+      hipLaunchKernelGGL((kb_stage<T, R, D>), dim3(e->ntiles), blk, 0, st, devs, runs, tiles, k);
       by_threads(e->ntrow, [&](auto nt) {
         constexpr int NT = decltype(nt)::value;
         hipLaunchKernelGGL((kb_poisA<T, NT>), dim3(e->nrows), dim3(NT), e->ldsA, st, devs, runs, rows, k);
@@ -2774,7 +2808,8 @@ int ensemble_step(aero2d_ensemble *e, hipStream_t st, bool chained) {
         hipLaunchKernelGGL((kb_poisC<T, NT>), dim3(e->nrows), dim3(NT), e->ldsA, st, devs, runs, rows, k, 1);
       });
     }
-    hipLaunchKernelGGL((kb_final<T, R>), dim3(e->ntiles), blk, 0, st, devs, runs, tiles);
+    hipLaunchKernelGGL((kb_final<T, R, D>), dim3(e->ntiles), blk, 0, st, devs, runs, tiles);
+  });
   });
   hipLaunchKernelGGL(kb_finalize<T>, dim3(e->n), dim3(1024), 0, st, devs, runs);
   AERO_HIP_CHECK(hipGetLastError());
@@ -2793,12 +2828,14 @@ int ensemble_time(aero2d_ensemble *e, hipStream_t st, int iters, float *ms) {
   hipEvent_t e0, e1;
   AERO_HIP_CHECK(hipEventCreate(&e0));
   AERO_HIP_CHECK(hipEventCreate(&e1));
+  by_ens_view<T>(e, [&](auto tag) {
+  using D = typename decltype(tag)::type;
   by_rows(e->ty, [&](auto rr) {
     constexpr int R = decltype(rr)::value;
     for (int it = 0; it <= iters; ++it) {
       if (it == 1) (void)hipEventRecord(e0, st);   // the first round warms up
       for (int k = 0; k < 2; ++k) {
-        hipLaunchKernelGGL((kb_stage<T, R>), dim3(e->ntiles), blk, 0, st, devs, runs, tiles, k);
+        hipLaunchKernelGGL((kb_stage<T, R, D>), dim3(e->ntiles), blk, 0, st, devs, runs, tiles, k);
         by_threads(e->ntrow, [&](auto nt) {
           constexpr int NT = decltype(nt)::value;
           hipLaunchKernelGGL((kb_poisA<T, NT>), dim3(e->nrows), dim3(NT), e->ldsA, st, devs, runs, rows, k);
@@ -2810,6 +2847,7 @@ int ensemble_time(aero2d_ensemble *e, hipStream_t st, int iters, float *ms) {
         });
       }
     }
+  });
   });
   (void)hipEventRecord(e1, st);
   hipError_t err = hipEventSynchronize(e1);
@@ -2894,6 +2932,8 @@ int aero2d_ensemble_create(int32_t n, aero2d_plan *const *plans, const aero2d_fi
   e->geoms.assign(geoms, geoms + n);
   e->params.assign(params, params + n);
   if (outputs) e->outputs.assign(outputs, outputs + n);
+  e->plain_laminar = true;
+  for (int r = 0; r < n; ++r) e->plain_laminar = e->plain_laminar && plain_laminar(plans[r]->desc);
   int rc = dbl ? ensemble_build<double>(e) : ensemble_build<float>(e);
   if (rc) {
     aero2d_ensemble_destroy(e);

@@ -166,7 +166,7 @@ __device__ __forceinline__ void stockham_stage(const cplx<T> *__restrict__ src, 
 // buffer a (b is a scratch buffer of the same size). Returns the buffer holding the
 // result in natural order. All threads of the block must call it.
 template <typename T>
-__device__ cplx<T> *fft_lds(cplx<T> *a, cplx<T> *b, const FftPlan &pl, const cplx<T> *__restrict__ tw,
+__device__ __forceinline__ cplx<T> *fft_lds(cplx<T> *a, cplx<T> *b, const FftPlan &pl, const cplx<T> *__restrict__ tw,
                             int nslot, int slot_stride) {
   const int tid = threadIdx.x + threadIdx.y * blockDim.x;
   const int nthr = blockDim.x * blockDim.y;
@@ -244,7 +244,7 @@ __device__ __forceinline__ cplx<T> *pow2_stages(cplx<T> *src, cplx<T> *dst, cons
 // Forward transform of nslot sequences, as fft_lds: the power-of-two lengths 64 .. 2048 take
 // their compile-time stage sequence, other lengths the runtime plan.
 template <typename T>
-__device__ cplx<T> *fft_run(cplx<T> *a, cplx<T> *b, const FftPlan &pl, const cplx<T> *__restrict__ tw, int nslot,
+__device__ __forceinline__ cplx<T> *fft_run(cplx<T> *a, cplx<T> *b, const FftPlan &pl, const cplx<T> *__restrict__ tw, int nslot,
                             int slot_stride) {
   const int tid = threadIdx.x + threadIdx.y * blockDim.x;
   const int nthr = blockDim.x * blockDim.y;

@@ -1029,7 +1029,7 @@ __device__ __forceinline__ int makhoul_src(int n, int len) { return n < (len + 1
 
 // Outlet correction of core.py:337-357, computed identically by every block.
 template <typename T>
-__device__ T outlet_corr(const Dev<T> &d, double *lds) {
+__device__ __forceinline__ T outlet_corr(const Dev<T> &d, double *lds) {
   double v[2] = {0, 0};
   const int tid = threadIdx.x;
   for (int j = tid; j < d.ny; j += blockDim.x) {

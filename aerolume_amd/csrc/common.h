@@ -162,6 +162,23 @@ __device__ __forceinline__ void stockham_stage(const cplx<T> *__restrict__ src, 
   }
 }
 
+// Pointers typed as global memory: loads and stores through them compile to global instead of
+// flat instructions. For device allocations only: the batched kernels read their plans from a
+// device table, which leaves the compiler unsure where the plans' pointers point.
+// (the host pass of hipcc keeps the plain pointer: it compiles these bodies but never runs them,
+// and its address space 1 is not the device's global memory)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename X>
+using gptr = __attribute__((address_space(1))) X *;
+#else
+template <typename X>
+using gptr = X *;
+#endif
+template <typename X>
+__device__ __forceinline__ gptr<X> global_ptr(X *p) {
+  return (gptr<X>)p;
+}
+
 // Forward complex FFT of `nslot` length-n sequences stored slot_stride apart in LDS
 // buffer a (b is a scratch buffer of the same size). Returns the buffer holding the
 // result in natural order. All threads of the block must call it.

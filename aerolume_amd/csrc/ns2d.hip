@@ -1032,9 +1032,11 @@ template <typename T>
 __device__ __forceinline__ T outlet_corr(const Dev<T> &d, double *lds) {
   double v[2] = {0, 0};
   const int tid = threadIdx.x;
+  const auto u_inlet = global_ptr(d.u_inlet);
+  const auto outcol = global_ptr(d.outcol);
   for (int j = tid; j < d.ny; j += blockDim.x) {
-    v[0] += (double)d.u_inlet[j];
-    v[1] += (double)d.outcol[j];
+    v[0] += (double)u_inlet[j];
+    v[1] += (double)outcol[j];
   }
   block_sum4<2>(v, lds);
   __syncthreads();
@@ -1049,16 +1051,16 @@ __device__ __forceinline__ T outlet_corr(const Dev<T> &d, double *lds) {
 }
 
 // divergence at cell (j, i) with the outlet face shifted by corr (core.py:621-623)
-template <typename T>
-__device__ __forceinline__ T div_at(const Dev<T> &d, const T *u, const T *v, int j, int i, T corr) {
+template <typename T, typename P>
+__device__ __forceinline__ T div_at(const Dev<T> &d, P u, P v, int j, int i, T corr) {
   const int nx = d.nx;
   T ur = u[j * (nx + 1) + i + 1];
   if (i + 1 == nx) ur = ur + corr;
   return (((ur - u[j * (nx + 1) + i]) + v[(j + 1) * nx + i]) - v[j * nx + i]) * d.inv_h;
 }
 
-template <typename T>
-__device__ __forceinline__ void load_twiddles(cplx<T> *dst, const cplx<T> *__restrict__ src, int n) {
+template <typename T, typename P>
+__device__ __forceinline__ void load_twiddles(cplx<T> *dst, P src, int n) {
   for (int m = threadIdx.x; m < n; m += blockDim.x) dst[m] = src[m];
 }
 
@@ -1075,7 +1077,10 @@ __device__ __forceinline__ void poisA_body(const Dev<T> &d, int bx, const T *__r
   double *red = reinterpret_cast<double *>(TW + nx);
   const int ja = 2 * bx, jb = ja + 1;
   const bool hasb = jb < d.ny;
-  load_twiddles(TW, d.tw_x, nx);
+  const auto F = global_ptr(d.F);
+  const auto cs_x = global_ptr(d.cs_x);
+  const auto gu = global_ptr(u), gv = global_ptr(v), gf = global_ptr(f_in);
+  load_twiddles(TW, global_ptr(d.tw_x), nx);
   // every load of this thread is issued before the outlet correction's reduction waits:
   // the row elements (the face values of the divergence, or the right-hand side) and the DCT
   // twiddles of output frequency k = q, which this thread finishes below
@@ -1086,7 +1091,7 @@ __device__ __forceinline__ void poisA_body(const Dev<T> &d, int bx, const T *__r
 #pragma unroll
   for (int r = 0; r < MP; ++r) {
     const int q = threadIdx.x + NT * r;
-    if (q < nx && d.dct_x) csr[r] = d.cs_x[q];
+    if (q < nx && d.dct_x) csr[r] = cs_x[q];
   }
 #pragma unroll
   for (int r = 0; r < PF; ++r) {
@@ -1098,12 +1103,12 @@ __device__ __forceinline__ void poisA_body(const Dev<T> &d, int bx, const T *__r
         ur[r][h] = ul[r][h] = vt[r][h] = vb[r][h] = (T)0;
         if (h == 1 && !hasb) continue;
         if (mode == 0) {
-          ur[r][h] = f_in[j * nx + q];
+          ur[r][h] = gf[j * nx + q];
         } else {
-          ur[r][h] = u[j * (nx + 1) + q + 1];
-          ul[r][h] = u[j * (nx + 1) + q];
-          vt[r][h] = v[(j + 1) * nx + q];
-          vb[r][h] = v[j * nx + q];
+          ur[r][h] = gu[j * (nx + 1) + q + 1];
+          ul[r][h] = gu[j * (nx + 1) + q];
+          vt[r][h] = gv[(j + 1) * nx + q];
+          vb[r][h] = gv[j * nx + q];
         }
       }
     }
@@ -1111,7 +1116,7 @@ __device__ __forceinline__ void poisA_body(const Dev<T> &d, int bx, const T *__r
   T corr = 0;
   if (mode == 2) {
     corr = outlet_corr(d, red);
-    if (bx == 0 && threadIdx.x == 0) d.red->corr = corr;
+    if (bx == 0 && threadIdx.x == 0) global_ptr(d.red)->corr = corr;
   }
 #pragma unroll
   for (int r = 0; r < MP; ++r) {
@@ -1123,7 +1128,7 @@ __device__ __forceinline__ void poisA_body(const Dev<T> &d, int bx, const T *__r
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         if (r >= PF) {
-          x[h] = (h == 1 && !hasb) ? (T)0 : (mode == 0 ? f_in[(ja + h) * nx + q] : div_at(d, u, v, ja + h, q, corr));
+          x[h] = (h == 1 && !hasb) ? (T)0 : (mode == 0 ? gf[(ja + h) * nx + q] : div_at(d, gu, gv, ja + h, q, corr));
         } else if (mode == 0) {
           x[h] = ur[r < PF ? r : 0][h];
         } else {
@@ -1152,8 +1157,8 @@ __device__ __forceinline__ void poisA_body(const Dev<T> &d, int bx, const T *__r
       xa = va.x - va.y;
       xb = vb.x - vb.y;
     }
-    d.F[ja * nx + k] = xa;
-    if (hasb) d.F[jb * nx + k] = xb;
+    F[ja * nx + k] = xa;
+    if (hasb) F[jb * nx + k] = xb;
   }
 }
 template <typename T, int NT>
@@ -1189,23 +1194,26 @@ __device__ __forceinline__ void poisB_body(const Dev<T> &d, int bx, int nslot) {
   T *X = reinterpret_cast<T *>(TW + ny);   // 2 * nslot * ny reals
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int c0 = 2 * nslot * bx;
+  const auto F = global_ptr(d.F);
+  const auto inv_lam = global_ptr(d.inv_lam);
+  const auto tw_y = global_ptr(d.tw_y), cs_y = global_ptr(d.cs_y);
   cplx<T> csr[MP];
   T iar[MP], ibr[MP];
   // twiddles, column elements, DCT twiddles and spectral scalings in one pass of loads
 #pragma unroll
   for (int r = 0; r < MP; ++r) {
     const int w = tid + nthr * r;
-    if (w < ny) TW[w] = d.tw_y[w];
+    if (w < ny) TW[w] = tw_y[w];
     if (w < nslot * ny) {
       const int f = fdiv(w, ny, log2_or_neg(ny)), n = w - f * ny;
       const int ca = c0 + 2 * f, cb = ca + 1;
       const int q = d.dct_y ? makhoul_src(n, ny) : n;
-      const T xa = ca < nx ? d.F[q * nx + ca] : (T)0;
-      const T xb = cb < nx ? d.F[q * nx + cb] : (T)0;
+      const T xa = ca < nx ? F[q * nx + ca] : (T)0;
+      const T xb = cb < nx ? F[q * nx + cb] : (T)0;
       A[f * ny + n] = {xa, xb};
-      csr[r] = d.cs_y[n];
-      iar[r] = ca < nx ? d.inv_lam[(size_t)ca * ny + n] : (T)0;
-      ibr[r] = cb < nx ? d.inv_lam[(size_t)cb * ny + n] : (T)0;
+      csr[r] = cs_y[n];
+      iar[r] = ca < nx ? inv_lam[(size_t)ca * ny + n] : (T)0;
+      ibr[r] = cb < nx ? inv_lam[(size_t)cb * ny + n] : (T)0;
     }
   }
   __syncthreads();
@@ -1256,8 +1264,8 @@ __device__ __forceinline__ void poisB_body(const Dev<T> &d, int bx, int nslot) {
       vb = (hb.x - hb.y) * inv_n;
     }
     const int q = d.dct_y ? makhoul_src(m, ny) : m;
-    if (ca < nx) d.F[q * nx + ca] = va;
-    if (cb < nx) d.F[q * nx + cb] = vb;
+    if (ca < nx) F[q * nx + ca] = va;
+    if (cb < nx) F[q * nx + cb] = vb;
   }
 }
 template <typename T, int NT>
@@ -1286,18 +1294,22 @@ __device__ __forceinline__ void poisC_body(const Dev<T> &d, int bx, T *__restric
   // previous pressure accumulator values (stages 1, 2), at the positions this thread writes
   T pa[MP], pb[MP];
   const bool acc_p = p != nullptr && stage > 0;
+  const auto F = global_ptr(d.F);
+  const auto tw_x = global_ptr(d.tw_x), cs_x = global_ptr(d.cs_x);
+  const auto gpsi = global_ptr(psi), gp = global_ptr(p);
   // twiddles, row elements, DCT twiddles and the accumulator in one pass of loads
 #pragma unroll
   for (int r = 0; r < MP; ++r) {
     const int k = threadIdx.x + NT * r;
     if (k < nx) {
-      TW[k] = d.tw_x[k];
-      X[k] = d.F[ja * nx + k];
-      X[nx + k] = hasb ? d.F[jb * nx + k] : (T)0;
-      csr[r] = d.dct_x ? d.cs_x[k] : cplx<T>{0, 0};
+      TW[k] = tw_x[k];
+      X[k] = F[ja * nx + k];
+      X[nx + k] = hasb ? F[jb * nx + k] : (T)0;
+      csr[r] = cplx<T>{0, 0};
+      if (d.dct_x) csr[r] = cs_x[k];
       if (acc_p) {
-        pa[r] = p[ja * nx + k];
-        pb[r] = hasb ? p[jb * nx + k] : (T)0;
+        pa[r] = gp[ja * nx + k];
+        pb[r] = hasb ? gp[jb * nx + k] : (T)0;
       }
     }
   }
@@ -1311,7 +1323,7 @@ __device__ __forceinline__ void poisC_body(const Dev<T> &d, int bx, T *__restric
   cplx<T> *R = fft_run(A, B, d.fx, TW, 1, nx);
   const T inv_n = (T)(1.0 / (double)nx);
   T rdt = 0;
-  if (p != nullptr && stage == 2) rdt = (T)1 / d.red->dt;
+  if (p != nullptr && stage == 2) rdt = (T)1 / global_ptr(d.red)->dt;
   const T w = (T)kForceW[stage];
 #pragma unroll
   for (int rr = 0; rr < MP; ++rr) {
@@ -1329,18 +1341,18 @@ __device__ __forceinline__ void poisC_body(const Dev<T> &d, int bx, T *__restric
       va = (ha.x - ha.y) * inv_n;
       vb = (hb.x - hb.y) * inv_n;
     }
-    psi[ja * nx + q] = va;
-    if (hasb) psi[jb * nx + q] = vb;
+    gpsi[ja * nx + q] = va;
+    if (hasb) gpsi[jb * nx + q] = vb;
     if (p != nullptr) {
       if (stage == 0) {
-        p[ja * nx + q] = va * w;
-        if (hasb) p[jb * nx + q] = vb * w;
+        gp[ja * nx + q] = va * w;
+        if (hasb) gp[jb * nx + q] = vb * w;
       } else if (stage == 1) {
-        p[ja * nx + q] = pa[rr] + w * va;
-        if (hasb) p[jb * nx + q] = pb[rr] + w * vb;
+        gp[ja * nx + q] = pa[rr] + w * va;
+        if (hasb) gp[jb * nx + q] = pb[rr] + w * vb;
       } else {
-        p[ja * nx + q] = (pa[rr] + va) * rdt;
-        if (hasb) p[jb * nx + q] = (pb[rr] + vb) * rdt;
+        gp[ja * nx + q] = (pa[rr] + va) * rdt;
+        if (hasb) gp[jb * nx + q] = (pb[rr] + vb) * rdt;
       }
     }
   }

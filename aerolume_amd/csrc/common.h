@@ -174,8 +174,17 @@ using gptr = __attribute__((address_space(1))) X *;
 template <typename X>
 using gptr = X *;
 #endif
+// X in a parameter that takes no part in template argument deduction (a global pointer then
+// converts to the parameter's plain pointer type instead of failing the deduction)
 template <typename X>
-__device__ __forceinline__ gptr<X> global_ptr(X *p) {
+struct nd_ident {
+  using type = X;
+};
+template <typename X>
+using nd = typename nd_ident<X>::type;
+
+template <typename X>
+__host__ __device__ __forceinline__ gptr<X> global_ptr(X *p) {
   return (gptr<X>)p;
 }
 

@@ -80,9 +80,9 @@ struct Dev {
   int fixed_dt;
   int wallm;         // Werner-Wengle wall shear on
   int pid;           // adaptive_dt = "pid"
-  const T *dt_in;    // the step-start state's dt and controller memory (PID step)
-  const T *ctrl_in;
-  T *ctrl_out;       // the end-of-step state's controller memory
+  gptr<const T> dt_in;    // the step-start state's dt and controller memory (PID step)
+  gptr<const T> ctrl_in;
+  gptr<T> ctrl_out;       // the end-of-step state's controller memory
   double hd, cfl, force_ref, prt, diff_coef, les_c, gx, gy;
   T h, inv_h, kadv, inv_h2, ground;
   // double-built constants of the per-cell paths, rounded once (the values the kernels formed
@@ -90,20 +90,20 @@ struct Dev {
   T inv_prt, two_h, les_k2, les_kv;
   T wall_y;          // distance of the first face row from a wall, h / 2
   // geometry
-  const T *chi_u, *chi_v, *chi_c, *a_u, *a_v, *k_u, *k_v;
-  const T *chi_t, *a_t, *w_t, *vol_u, *vol_v, *ins_u, *ins_v;
-  const T *q_skin, *w_f;   // heat-flux bodies: skin source per unit kappa, skin weights
+  gptr<const T> chi_u, chi_v, chi_c, a_u, a_v, k_u, k_v;
+  gptr<const T> chi_t, a_t, w_t, vol_u, vol_v, ins_u, ins_v;
+  gptr<const T> q_skin, w_f;   // heat-flux bodies: skin source per unit kappa, skin weights
   // params
-  const T *nu, *eta, *u_inlet, *dt_fixed, *kappa, *ri, *eta_t, *theta_in, *wall_values, *heat, *heat_max;
-  const T *dye_inflow, *g_weight, *g_target, *g_timing;
+  gptr<const T> nu, eta, u_inlet, dt_fixed, kappa, ri, eta_t, theta_in, wall_values, heat, heat_max;
+  gptr<const T> dye_inflow, g_weight, g_target, g_timing;
   // scratch
-  T *F, *psi, *acc_u, *acc_v, *acc_t, *outcol;
-  double *pmax;    // per block: max |u|, |v|, |theta|, nu_t, sum theta
-  double *pforce;  // per block: 4 force sums per body, heat per temperature body
-  Red<T> *red;
-  const cplx<T> *tw_x, *tw_y, *cs_x, *cs_y;
-  const double *lam_x, *lam_y;
-  const T *inv_lam;   // (nx, ny), y fastest: 1 / (lam_x + lam_y) formed in float64, 0 at the zero mode
+  gptr<T> F, psi, acc_u, acc_v, acc_t, outcol;
+  gptr<double> pmax;    // per block: max |u|, |v|, |theta|, nu_t, sum theta
+  gptr<double> pforce;  // per block: 4 force sums per body, heat per temperature body
+  gptr<Red<T>> red;
+  gptr<const cplx<T>> tw_x, tw_y, cs_x, cs_y;
+  gptr<const double> lam_x, lam_y;
+  gptr<const T> inv_lam;   // (nx, ny), y fastest: 1 / (lam_x + lam_y) formed in float64, 0 at the zero mode
   FftPlan fx, fy;
   int dct_x, dct_y;   // 1: Neumann (DCT-II), 0: periodic (DHT)
 };
@@ -147,10 +147,10 @@ template <typename T>
 struct Stage {
   int s;
   double a, b, w;
-  const T *up, *vp, *psi_p;  // stage input velocity (+ the previous stage's potential)
-  const T *thp;              // stage input temperature
-  const T *u0, *v0, *th0;    // step start
-  T *uh, *vh, *thh;          // stage output (before projection)
+  gptr<const T> up, vp, psi_p;  // stage input velocity (+ the previous stage's potential)
+  gptr<const T> thp;              // stage input temperature
+  gptr<const T> u0, v0, th0;    // step start
+  gptr<T> uh, vh, thh;          // stage output (before projection)
 };
 
 template <typename T, int R>
@@ -176,7 +176,7 @@ __device__ __forceinline__ T wall_ghost(int kind, T val, T speed) {
 // u at face (j, i) of the stage input: u^ - grad psi when psi is given (core.py:645-660),
 // with the outlet face shifted by the mass correction (core.py:337-357).
 template <typename T>
-__device__ __forceinline__ T u_in(const Dev<T> &d, const T *__restrict__ u, const T *__restrict__ psi, T corr, int j,
+__device__ __forceinline__ T u_in(const Dev<T> &d, const nd<T> *__restrict__ u, const nd<T> *__restrict__ psi, T corr, int j,
                                   int i) {
   const int nx = d.nx;
   T val = u[j * (nx + 1) + i];
@@ -193,7 +193,7 @@ __device__ __forceinline__ T u_in(const Dev<T> &d, const T *__restrict__ u, cons
   return val;
 }
 template <typename T>
-__device__ __forceinline__ T v_in(const Dev<T> &d, const T *__restrict__ v, const T *__restrict__ psi, int j, int i) {
+__device__ __forceinline__ T v_in(const Dev<T> &d, const nd<T> *__restrict__ v, const nd<T> *__restrict__ psi, int j, int i) {
   const int nx = d.nx, ny = d.ny;
   T val = v[j * nx + i];
   if (psi) {
@@ -236,7 +236,7 @@ __device__ __forceinline__ T finish_v(const Dev<T> &d, T x, T q1, T q0, int flag
 // c + 64), each wave the row part of its rows, and every element then costs its loads and
 // finish_u / finish_v.
 template <typename T, int R, typename D>
-__device__ __forceinline__ void load_uv(const D &d, Lds<T, R> &L, int tj0, int ti0, const T *u, const T *v, const T *psi, T corr) {
+__device__ __forceinline__ void load_uv(const D &d, Lds<T, R> &L, int tj0, int ti0, const nd<T> *u, const nd<T> *v, const nd<T> *psi, T corr) {
   const int nx = d.nx, ny = d.ny;
   const bool hp = psi != nullptr;
   const int lane = threadIdx.x, wave = threadIdx.y;
@@ -370,7 +370,7 @@ __device__ __forceinline__ void load_uv(const D &d, Lds<T, R> &L, int tj0, int t
 // replicate elsewhere, wrap when periodic; use_inflow 2 is the smoke comb's rule (inflow also
 // in a closed box, core.py:718-723)
 template <typename T, int R, typename D>
-__device__ __forceinline__ void load_c(const D &d, Lds<T, R> &L, int tj0, int ti0, const T *c, const T *inflow_col, T inflow0,
+__device__ __forceinline__ void load_c(const D &d, Lds<T, R> &L, int tj0, int ti0, const nd<T> *c, const nd<T> *inflow_col, T inflow0,
                        int use_inflow) {
   const int nx = d.nx, ny = d.ny;
   const int lane = threadIdx.x, wave = threadIdx.y;
@@ -566,7 +566,7 @@ __device__ __forceinline__ T wall_dflux(const Dev<T> &d, int mode, T k, T inner,
 
 // emitter clock (physics.py:798-802)
 template <typename T>
-__device__ __forceinline__ T emitter_on(T t, const T *tm) {
+__device__ __forceinline__ T emitter_on(T t, const nd<T> *tm) {
   const T start = tm[0], stop = tm[1], period = tm[2], duty = tm[3];
   const bool active = (t >= start) && (t < stop);
   const T safe = period > (T)0 ? period : (T)1;
@@ -632,7 +632,7 @@ __device__ T compute_dt(const Dev<T> &d, const DtIn<T> &in, T umax, T vmax, T th
 // more than 3 % per step; the first step (dt_prev = 0) takes the target. ctrl_prev / ctrl_new
 // hold the last two errors.
 template <typename T>
-__device__ T pid_step(T limit, T dt_prev, const T *ctrl_prev, T *ctrl_new) {
+__device__ T pid_step(T limit, T dt_prev, const nd<T> *ctrl_prev, nd<T> *ctrl_new) {
   const T target = (T)0.9 * limit;
   const bool first = dt_prev <= (T)0;
   const T safe_prev = first ? target : dt_prev;
@@ -1032,8 +1032,8 @@ template <typename T>
 __device__ __forceinline__ T outlet_corr(const Dev<T> &d, double *lds) {
   double v[2] = {0, 0};
   const int tid = threadIdx.x;
-  const auto u_inlet = global_ptr(d.u_inlet);
-  const auto outcol = global_ptr(d.outcol);
+  const auto u_inlet = d.u_inlet;
+  const auto outcol = d.outcol;
   for (int j = tid; j < d.ny; j += blockDim.x) {
     v[0] += (double)u_inlet[j];
     v[1] += (double)outcol[j];
@@ -1077,10 +1077,10 @@ __device__ __forceinline__ void poisA_body(const Dev<T> &d, int bx, const T *__r
   double *red = reinterpret_cast<double *>(TW + nx);
   const int ja = 2 * bx, jb = ja + 1;
   const bool hasb = jb < d.ny;
-  const auto F = global_ptr(d.F);
-  const auto cs_x = global_ptr(d.cs_x);
+  const auto F = d.F;
+  const auto cs_x = d.cs_x;
   const auto gu = global_ptr(u), gv = global_ptr(v), gf = global_ptr(f_in);
-  load_twiddles(TW, global_ptr(d.tw_x), nx);
+  load_twiddles(TW, d.tw_x, nx);
   // every load of this thread is issued before the outlet correction's reduction waits:
   // the row elements (the face values of the divergence, or the right-hand side) and the DCT
   // twiddles of output frequency k = q, which this thread finishes below
@@ -1116,7 +1116,7 @@ __device__ __forceinline__ void poisA_body(const Dev<T> &d, int bx, const T *__r
   T corr = 0;
   if (mode == 2) {
     corr = outlet_corr(d, red);
-    if (bx == 0 && threadIdx.x == 0) global_ptr(d.red)->corr = corr;
+    if (bx == 0 && threadIdx.x == 0) d.red->corr = corr;
   }
 #pragma unroll
   for (int r = 0; r < MP; ++r) {
@@ -1194,9 +1194,9 @@ __device__ __forceinline__ void poisB_body(const Dev<T> &d, int bx, int nslot) {
   T *X = reinterpret_cast<T *>(TW + ny);   // 2 * nslot * ny reals
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int c0 = 2 * nslot * bx;
-  const auto F = global_ptr(d.F);
-  const auto inv_lam = global_ptr(d.inv_lam);
-  const auto tw_y = global_ptr(d.tw_y), cs_y = global_ptr(d.cs_y);
+  const auto F = d.F;
+  const auto inv_lam = d.inv_lam;
+  const auto tw_y = d.tw_y, cs_y = d.cs_y;
   cplx<T> csr[MP];
   T iar[MP], ibr[MP];
   // twiddles, column elements, DCT twiddles and spectral scalings in one pass of loads
@@ -1294,8 +1294,8 @@ __device__ __forceinline__ void poisC_body(const Dev<T> &d, int bx, T *__restric
   // previous pressure accumulator values (stages 1, 2), at the positions this thread writes
   T pa[MP], pb[MP];
   const bool acc_p = p != nullptr && stage > 0;
-  const auto F = global_ptr(d.F);
-  const auto tw_x = global_ptr(d.tw_x), cs_x = global_ptr(d.cs_x);
+  const auto F = d.F;
+  const auto tw_x = d.tw_x, cs_x = d.cs_x;
   const auto gpsi = global_ptr(psi), gp = global_ptr(p);
   // twiddles, row elements, DCT twiddles and the accumulator in one pass of loads
 #pragma unroll
@@ -1323,7 +1323,7 @@ __device__ __forceinline__ void poisC_body(const Dev<T> &d, int bx, T *__restric
   cplx<T> *R = fft_run(A, B, d.fx, TW, 1, nx);
   const T inv_n = (T)(1.0 / (double)nx);
   T rdt = 0;
-  if (p != nullptr && stage == 2) rdt = (T)1 / global_ptr(d.red)->dt;
+  if (p != nullptr && stage == 2) rdt = (T)1 / d.red->dt;
   const T w = (T)kForceW[stage];
 #pragma unroll
   for (int rr = 0; rr < MP; ++rr) {
@@ -1779,7 +1779,7 @@ struct Diag2 {
 // The discrete Laplacian of the Poisson solve at cell (j, i): Neumann walls mirror the edge
 // cell, periodic boxes wrap (ns2d/poisson.py:136-144).
 template <typename T>
-__device__ __forceinline__ T lap_at(const Dev<T> &d, const T *__restrict__ psi, int j, int i) {
+__device__ __forceinline__ T lap_at(const Dev<T> &d, const nd<T> *__restrict__ psi, int j, int i) {
   const int nx = d.nx, ny = d.ny;
   int ie, iw, jn, js;
   if (d.periodic) {
@@ -1908,12 +1908,12 @@ __global__ void __launch_bounds__(64) k_diag2_fin(Dev<T> d, Diag2<T> g) {
 template <typename T>
 struct BRun {
   Stage<T> st[3];
-  const T *u0, *v0, *th0;        // step start (in place: also the step's output)
-  T *p_out;
-  const T *uC, *vC;              // last stage's u^
-  T *u_out, *v_out;
-  const T *th_new, *t_in;
-  T *t_out, *dt_out;
+  gptr<const T> u0, v0, th0;        // step start (in place: also the step's output)
+  gptr<T> p_out;
+  gptr<const T> uC, vC;              // last stage's u^
+  gptr<T> u_out, v_out;
+  gptr<const T> th_new, t_in;
+  gptr<T> t_out, dt_out;
   aero2d_outputs o;
   int nslotB, mode;
 };
@@ -2230,29 +2230,29 @@ Dev<T> make_dev(const aero2d_plan *p, const aero2d_geom *g, const aero2d_params 
   d.inv_h2 = (T)(1.0 / (s.h * s.h));
   d.ground = (T)s.ground_speed;
   if (g) {
-    d.chi_u = (const T *)g->chi_u; d.chi_v = (const T *)g->chi_v; d.chi_c = (const T *)g->chi_c;
-    d.a_u = (const T *)g->a_u; d.a_v = (const T *)g->a_v; d.k_u = (const T *)g->k_u; d.k_v = (const T *)g->k_v;
-    d.chi_t = (const T *)g->chi_t; d.a_t = (const T *)g->a_t; d.w_t = (const T *)g->w_t;
-    d.vol_u = (const T *)g->vol_u; d.vol_v = (const T *)g->vol_v;
-    d.ins_u = (const T *)g->ins_u; d.ins_v = (const T *)g->ins_v;
-    d.q_skin = (const T *)g->q_skin; d.w_f = (const T *)g->w_f;
+    d.chi_u = global_ptr((const T *)g->chi_u); d.chi_v = global_ptr((const T *)g->chi_v); d.chi_c = global_ptr((const T *)g->chi_c);
+    d.a_u = global_ptr((const T *)g->a_u); d.a_v = global_ptr((const T *)g->a_v); d.k_u = global_ptr((const T *)g->k_u); d.k_v = global_ptr((const T *)g->k_v);
+    d.chi_t = global_ptr((const T *)g->chi_t); d.a_t = global_ptr((const T *)g->a_t); d.w_t = global_ptr((const T *)g->w_t);
+    d.vol_u = global_ptr((const T *)g->vol_u); d.vol_v = global_ptr((const T *)g->vol_v);
+    d.ins_u = global_ptr((const T *)g->ins_u); d.ins_v = global_ptr((const T *)g->ins_v);
+    d.q_skin = global_ptr((const T *)g->q_skin); d.w_f = global_ptr((const T *)g->w_f);
   }
   if (pr) {
-    d.nu = (const T *)pr->nu; d.eta = (const T *)pr->eta; d.u_inlet = (const T *)pr->u_inlet;
-    d.dt_fixed = (const T *)pr->dt_fixed; d.kappa = (const T *)pr->kappa; d.ri = (const T *)pr->ri;
-    d.eta_t = (const T *)pr->eta_t; d.theta_in = (const T *)pr->theta_in;
-    d.wall_values = (const T *)pr->wall_values; d.heat = (const T *)pr->heat; d.heat_max = (const T *)pr->heat_max;
-    d.dye_inflow = (const T *)pr->dye_inflow; d.g_weight = (const T *)pr->group_weight;
-    d.g_target = (const T *)pr->group_target; d.g_timing = (const T *)pr->group_timing;
+    d.nu = global_ptr((const T *)pr->nu); d.eta = global_ptr((const T *)pr->eta); d.u_inlet = global_ptr((const T *)pr->u_inlet);
+    d.dt_fixed = global_ptr((const T *)pr->dt_fixed); d.kappa = global_ptr((const T *)pr->kappa); d.ri = global_ptr((const T *)pr->ri);
+    d.eta_t = global_ptr((const T *)pr->eta_t); d.theta_in = global_ptr((const T *)pr->theta_in);
+    d.wall_values = global_ptr((const T *)pr->wall_values); d.heat = global_ptr((const T *)pr->heat); d.heat_max = global_ptr((const T *)pr->heat_max);
+    d.dye_inflow = global_ptr((const T *)pr->dye_inflow); d.g_weight = global_ptr((const T *)pr->group_weight);
+    d.g_target = global_ptr((const T *)pr->group_target); d.g_timing = global_ptr((const T *)pr->group_timing);
   }
-  d.F = (T *)p->F; d.psi = (T *)p->psi;
-  d.acc_u = (T *)p->acc_u; d.acc_v = (T *)p->acc_v; d.acc_t = (T *)p->acc_t; d.outcol = (T *)p->outcol;
-  d.pmax = p->pmax; d.pforce = p->pforce;
-  d.red = (Red<T> *)p->red;
-  d.tw_x = (const cplx<T> *)p->tw_x; d.tw_y = (const cplx<T> *)p->tw_y;
-  d.cs_x = (const cplx<T> *)p->cs_x; d.cs_y = (const cplx<T> *)p->cs_y;
-  d.lam_x = p->lam_x; d.lam_y = p->lam_y;
-  d.inv_lam = (const T *)p->inv_lam;
+  d.F = global_ptr((T *)p->F); d.psi = global_ptr((T *)p->psi);
+  d.acc_u = global_ptr((T *)p->acc_u); d.acc_v = global_ptr((T *)p->acc_v); d.acc_t = global_ptr((T *)p->acc_t); d.outcol = global_ptr((T *)p->outcol);
+  d.pmax = global_ptr(p->pmax); d.pforce = global_ptr(p->pforce);
+  d.red = global_ptr((Red<T> *)p->red);
+  d.tw_x = global_ptr((const cplx<T> *)p->tw_x); d.tw_y = global_ptr((const cplx<T> *)p->tw_y);
+  d.cs_x = global_ptr((const cplx<T> *)p->cs_x); d.cs_y = global_ptr((const cplx<T> *)p->cs_y);
+  d.lam_x = global_ptr(p->lam_x); d.lam_y = global_ptr(p->lam_y);
+  d.inv_lam = global_ptr((const T *)p->inv_lam);
   d.fx = p->fx; d.fy = p->fy;
   d.dct_x = s.periodic ? 0 : 1;
   d.dct_y = s.periodic ? 0 : 1;
@@ -2293,9 +2293,9 @@ int step_impl(aero2d_plan *p, hipStream_t st, const aero2d_fields *in, aero2d_fi
               const aero2d_params *pr, aero2d_outputs *o, bool chained, const aero2d_trace *tr = nullptr) {
   const aero2d_desc &s = p->desc;
   Dev<T> d = make_dev<T>(p, g, pr);
-  d.dt_in = (const T *)in->dt;
-  d.ctrl_in = (const T *)in->ctrl;
-  d.ctrl_out = (T *)out->ctrl;
+  d.dt_in = global_ptr((const T *)in->dt);
+  d.ctrl_in = global_ptr((const T *)in->ctrl);
+  d.ctrl_out = global_ptr((T *)out->ctrl);
   const int nx = s.nx, ny = s.ny;
   const dim3 blk(64, 4);
   const dim3 tiles(p->nbx, p->nby);
@@ -2329,9 +2329,9 @@ int step_impl(aero2d_plan *p, hipStream_t st, const aero2d_fields *in, aero2d_fi
     sg.a = kSsp[k][0];
     sg.b = kSsp[k][1];
     sg.w = kForceW[k];
-    sg.up = up; sg.vp = vp; sg.psi_p = psi_p; sg.thp = thp;
-    sg.u0 = u0; sg.v0 = v0; sg.th0 = th0;
-    sg.uh = ubuf[k]; sg.vh = vbuf[k]; sg.thh = s.thermal ? tbuf[k] : nullptr;
+    sg.up = global_ptr(up); sg.vp = global_ptr(vp); sg.psi_p = global_ptr(psi_p); sg.thp = global_ptr(thp);
+    sg.u0 = global_ptr(u0); sg.v0 = global_ptr(v0); sg.th0 = global_ptr(th0);
+    sg.uh = global_ptr(ubuf[k]); sg.vh = global_ptr(vbuf[k]); sg.thh = global_ptr(s.thermal ? tbuf[k] : nullptr);
     by_tile<T>(p, [&](auto r, auto tag) {
       using D = typename decltype(tag)::type;
       hipLaunchKernelGGL((k_stage<T, decltype(r)::value, D>), tiles, blk, 0, st, view_of<D>(d), sg);
@@ -2419,12 +2419,12 @@ int time_stage_impl(aero2d_plan *p, hipStream_t st, const aero2d_fields *f, cons
   sg.b = kSsp[stage][1];
   sg.w = kForceW[stage];
   // stage input = the previous stage's unprojected buffer and potential (as in a step)
-  sg.up = stage > 0 ? (const T *)p->uB : (const T *)f->u;
-  sg.vp = stage > 0 ? (const T *)p->vB : (const T *)f->v;
-  sg.psi_p = stage > 0 ? (const T *)p->psi : nullptr;
-  sg.thp = (const T *)f->theta;
-  sg.u0 = (const T *)f->u; sg.v0 = (const T *)f->v; sg.th0 = (const T *)f->theta;
-  sg.uh = (T *)p->uA; sg.vh = (T *)p->vA; sg.thh = s.thermal ? (T *)p->thA : nullptr;
+  sg.up = global_ptr(stage > 0 ? (const T *)p->uB : (const T *)f->u);
+  sg.vp = global_ptr(stage > 0 ? (const T *)p->vB : (const T *)f->v);
+  sg.psi_p = global_ptr(stage > 0 ? (const T *)p->psi : nullptr);
+  sg.thp = global_ptr((const T *)f->theta);
+  sg.u0 = global_ptr((const T *)f->u); sg.v0 = global_ptr((const T *)f->v); sg.th0 = global_ptr((const T *)f->theta);
+  sg.uh = global_ptr((T *)p->uA); sg.vh = global_ptr((T *)p->vA); sg.thh = global_ptr(s.thermal ? (T *)p->thA : nullptr);
   const dim3 blk(64, 4);
   const dim3 tiles(p->nbx, p->nby);
   hipEvent_t e0, e1;
@@ -2740,9 +2740,9 @@ int ensemble_build(aero2d_ensemble *e) {
     aero2d_plan *p = e->plans[r];
     const aero2d_desc &s = p->desc;
     devs[r] = make_dev<T>(p, &geoms[r], &params[r]);
-    devs[r].dt_in = (const T *)fields[r].dt;
-    devs[r].ctrl_in = (const T *)fields[r].ctrl;
-    devs[r].ctrl_out = (T *)fields[r].ctrl;
+    devs[r].dt_in = global_ptr((const T *)fields[r].dt);
+    devs[r].ctrl_in = global_ptr((const T *)fields[r].ctrl);
+    devs[r].ctrl_out = global_ptr((T *)fields[r].ctrl);
     const int nby = (s.ny + e->ty - 1) / e->ty;
     devs[r].ty = e->ty;
     devs[r].nby = nby;
@@ -2760,17 +2760,17 @@ int ensemble_build(aero2d_ensemble *e) {
       sg.a = kSsp[k][0];
       sg.b = kSsp[k][1];
       sg.w = kForceW[k];
-      sg.up = up; sg.vp = vp; sg.psi_p = psi_p; sg.thp = thp;
-      sg.u0 = u0; sg.v0 = v0; sg.th0 = th0;
-      sg.uh = ubuf[k]; sg.vh = vbuf[k]; sg.thh = s.thermal ? tbuf[k] : nullptr;
+      sg.up = global_ptr(up); sg.vp = global_ptr(vp); sg.psi_p = global_ptr(psi_p); sg.thp = global_ptr(thp);
+      sg.u0 = global_ptr(u0); sg.v0 = global_ptr(v0); sg.th0 = global_ptr(th0);
+      sg.uh = global_ptr(ubuf[k]); sg.vh = global_ptr(vbuf[k]); sg.thh = global_ptr(s.thermal ? tbuf[k] : nullptr);
       up = sg.uh; vp = sg.vh; thp = sg.thh; psi_p = (const T *)p->psi;
     }
-    b.u0 = u0; b.v0 = v0; b.th0 = th0;
-    b.p_out = (T *)f.p;
-    b.uC = (const T *)p->uC; b.vC = (const T *)p->vC;
-    b.u_out = (T *)f.u; b.v_out = (T *)f.v;
-    b.th_new = (const T *)f.theta;
-    b.t_in = (const T *)f.t; b.t_out = (T *)f.t; b.dt_out = (T *)f.dt;
+    b.u0 = global_ptr(u0); b.v0 = global_ptr(v0); b.th0 = global_ptr(th0);
+    b.p_out = global_ptr((T *)f.p);
+    b.uC = global_ptr((const T *)p->uC); b.vC = global_ptr((const T *)p->vC);
+    b.u_out = global_ptr((T *)f.u); b.v_out = global_ptr((T *)f.v);
+    b.th_new = global_ptr((const T *)f.theta);
+    b.t_in = global_ptr((const T *)f.t); b.t_out = global_ptr((T *)f.t); b.dt_out = global_ptr((T *)f.dt);
     if (outputs) b.o = outputs[r];
     b.nslotB = run_slots<T>(p, e->nslot);
     b.mode = (s.periodic || s.closed) ? 1 : 2;
@@ -3106,7 +3106,7 @@ int static_solve(unsigned char *base, hipStream_t st, int nx, int ny, double h, 
     off += align_up(bytes);
     return q;
   };
-  d.F = (T *)take(cells * sizeof(T));
+  d.F = global_ptr((T *)take(cells * sizeof(T)));
   T *inv = (T *)take(cells * sizeof(T));
   cplx<T> *tw_x = (cplx<T> *)take(nx * sizeof(cplx<T>)), *tw_y = (cplx<T> *)take(ny * sizeof(cplx<T>));
   cplx<T> *cs_x = (cplx<T> *)take(nx * sizeof(cplx<T>)), *cs_y = (cplx<T> *)take(ny * sizeof(cplx<T>));
@@ -3115,8 +3115,8 @@ int static_solve(unsigned char *base, hipStream_t st, int nx, int ny, double h, 
     set_error("aero2d_poisson_solve_static: scratch layout exceeds the static array");
     return AERO_EUNSUPPORTED;
   }
-  d.inv_lam = inv;
-  d.tw_x = tw_x; d.tw_y = tw_y; d.cs_x = cs_x; d.cs_y = cs_y; d.lam_x = lam_x; d.lam_y = lam_y;
+  d.inv_lam = global_ptr(inv);
+  d.tw_x = global_ptr(tw_x); d.tw_y = global_ptr(tw_y); d.cs_x = global_ptr(cs_x); d.cs_y = global_ptr(cs_y); d.lam_x = global_ptr(lam_x); d.lam_y = global_ptr(lam_y);
   const double hh = h * h;
   hipLaunchKernelGGL(k_static_axis<T>, dim3((nx + 255) / 256), dim3(256), 0, st, d.fx, hh, periodic, tw_x, cs_x, lam_x);
   hipLaunchKernelGGL(k_static_axis<T>, dim3((ny + 255) / 256), dim3(256), 0, st, d.fy, hh, periodic, tw_y, cs_y, lam_y);

@@ -43,6 +43,7 @@ namespace ns2d {
 
 constexpr int kFftThreads = 256;          // threads of the batched transform passes
 constexpr int kMaxLen = 4096;              // longest transform
+constexpr int kRowFill = 1024;             // row elements per batched row-pass block when filled
 constexpr int TX = 64;                      // tile width: one wave across (blocks of 64 x 4 threads)
 constexpr int HW = TX + 4, NW = TX + 2;     // halo-2 rows of u, v, scalar; eddy-viscosity rows (halo 1)
 constexpr int kDefaultRows = 4;             // tile height unless a plan or an ensemble picks another
@@ -1064,19 +1065,24 @@ __device__ __forceinline__ void load_twiddles(cplx<T> *dst, P src, int n) {
   for (int m = threadIdx.x; m < n; m += blockDim.x) dst[m] = src[m];
 }
 
-// Pass A: divergence of rows (2b, 2b+1) -> forward transform along x -> F rows.
-template <typename T, int NT>
-__device__ __forceinline__ void poisA_body(const Dev<T> &d, int bx, const T *__restrict__ u, const T *__restrict__ v, const T *__restrict__ f_in, int mode) {
+// Pass A: divergence of the nslot row pairs (2p, 2p+1), p = b nslot .. b nslot + nslot - 1 ->
+// forward transform along x -> F rows. The block's rows sit one after another in LDS, each pair
+// transformed as one complex sequence; pairs past the last row are skipped. ONE: nslot is 1
+// (single runs), known at compile time.
+template <typename T, int NT, bool ONE = false>
+__device__ __forceinline__ void poisA_body(const Dev<T> &d, int bx, int nslot, const T *__restrict__ u,
+                                           const T *__restrict__ v, const T *__restrict__ f_in, int mode) {
   constexpr int MP = kMaxLen / NT;   // row elements per thread
   // mode 0: f_in is the right-hand side itself; 1: divergence of (u, v); 2: + outlet correction
   extern __shared__ unsigned char smem[];
-  const int nx = d.nx;
+  if (ONE) nslot = 1;
+  const int nx = d.nx, ny = d.ny, lx = ONE ? 0 : log2_or_neg(nx);
+  const int nel = nslot * nx;        // elements of the block's row pairs
   cplx<T> *A = reinterpret_cast<cplx<T> *>(smem);
-  cplx<T> *B = A + nx;
-  cplx<T> *TW = B + nx;
+  cplx<T> *B = A + nel;
+  cplx<T> *TW = B + nel;
   double *red = reinterpret_cast<double *>(TW + nx);
-  const int ja = 2 * bx, jb = ja + 1;
-  const bool hasb = jb < d.ny;
+  const int j0 = 2 * bx * nslot;     // first row of the block
   const auto F = d.F;
   const auto cs_x = d.cs_x;
   const auto gu = global_ptr(u), gv = global_ptr(v), gf = global_ptr(f_in);
@@ -1090,18 +1096,19 @@ __device__ __forceinline__ void poisA_body(const Dev<T> &d, int bx, const T *__r
   T ur[PF][2], ul[PF][2], vt[PF][2], vb[PF][2];
 #pragma unroll
   for (int r = 0; r < MP; ++r) {
-    const int q = threadIdx.x + NT * r;
-    if (q < nx && d.dct_x) csr[r] = cs_x[q];
+    const int w = threadIdx.x + NT * r;
+    if (w < nel && d.dct_x) csr[r] = cs_x[w - (ONE ? 0 : fdiv(w, nx, lx)) * nx];
   }
 #pragma unroll
   for (int r = 0; r < PF; ++r) {
-    const int q = threadIdx.x + NT * r;
-    if (q < nx) {
+    const int w = threadIdx.x + NT * r;
+    if (w < nel) {
+      const int f = (ONE ? 0 : fdiv(w, nx, lx)), q = w - f * nx;
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        const int j = ja + h;
+        const int j = j0 + 2 * f + h;
         ur[r][h] = ul[r][h] = vt[r][h] = vb[r][h] = (T)0;
-        if (h == 1 && !hasb) continue;
+        if (j >= ny) continue;
         if (mode == 0) {
           ur[r][h] = gf[j * nx + q];
         } else {
@@ -1120,15 +1127,19 @@ __device__ __forceinline__ void poisA_body(const Dev<T> &d, int bx, const T *__r
   }
 #pragma unroll
   for (int r = 0; r < MP; ++r) {
-    // natural column q into transform position m; the divergence as div_at forms it
-    const int q = threadIdx.x + NT * r;
-    if (q < nx) {
+    // natural column q of pair f into transform position m; the divergence as div_at forms it
+    const int w = threadIdx.x + NT * r;
+    if (w < nel) {
+      const int f = (ONE ? 0 : fdiv(w, nx, lx)), q = w - f * nx;
+      const int ja = j0 + 2 * f;
       const int m = d.dct_x ? makhoul_pos(q, nx) : q;
       T x[2];
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        if (r >= PF) {
-          x[h] = (h == 1 && !hasb) ? (T)0 : (mode == 0 ? gf[(ja + h) * nx + q] : div_at(d, gu, gv, ja + h, q, corr));
+        if (ja + h >= ny) {
+          x[h] = (T)0;
+        } else if (r >= PF) {
+          x[h] = mode == 0 ? gf[(ja + h) * nx + q] : div_at(d, gu, gv, ja + h, q, corr);
         } else if (mode == 0) {
           x[h] = ur[r < PF ? r : 0][h];
         } else {
@@ -1137,17 +1148,20 @@ __device__ __forceinline__ void poisA_body(const Dev<T> &d, int bx, const T *__r
           x[h] = (((right - ul[rp][h]) + vt[rp][h]) - vb[rp][h]) * d.inv_h;
         }
       }
-      A[m] = {x[0], hasb ? x[1] : (T)0};
+      A[f * nx + m] = {x[0], x[1]};
     }
   }
   __syncthreads();
-  cplx<T> *R = fft_run(A, B, d.fx, TW, 1, nx);
+  cplx<T> *R = fft_run(A, B, d.fx, TW, nslot, nx);
 #pragma unroll
   for (int rr = 0; rr < MP; ++rr) {
-    const int k = threadIdx.x + NT * rr;
-    if (k >= nx) continue;
+    const int w = threadIdx.x + NT * rr;
+    if (w >= nel) continue;
+    const int f = (ONE ? 0 : fdiv(w, nx, lx)), k = w - f * nx;
+    const int ja = j0 + 2 * f;
+    if (ja >= ny) continue;
     cplx<T> va, vb;
-    split_pair(R, nx, k, va, vb);
+    split_pair(R + f * nx, nx, k, va, vb);
     T xa, xb;
     if (d.dct_x) {
       const cplx<T> cs = csr[rr];
@@ -1158,13 +1172,13 @@ __device__ __forceinline__ void poisA_body(const Dev<T> &d, int bx, const T *__r
       xb = vb.x - vb.y;
     }
     F[ja * nx + k] = xa;
-    if (hasb) F[jb * nx + k] = xb;
+    if (ja + 1 < ny) F[(ja + 1) * nx + k] = xb;
   }
 }
 template <typename T, int NT>
 __global__ void __launch_bounds__(NT) k_poisA(Dev<T> d, const T *__restrict__ u, const T *__restrict__ v,
                                               const T *__restrict__ f_in, int mode) {
-  poisA_body<T, NT>(d, blockIdx.x, u, v, f_in, mode);
+  poisA_body<T, NT, true>(d, blockIdx.x, 1, u, v, f_in, mode);
 }
 
 // Inverse real transform packing (see common.h): conj(Z) for the DCT-III, plain pair for DHT.
@@ -1277,19 +1291,22 @@ __global__ void __launch_bounds__(NT) k_poisB(Dev<T> d, int nslot) {
   poisB_body<T, NT>(d, g, nslot);
 }
 
-// Pass C: inverse along x of rows (2b, 2b+1) -> psi; accumulate the step-averaged pressure
-// p (core.py:935-942): stage 0 p = w psi, stage 1 p += w psi, stage 2 p = (p + psi) / dt.
-template <typename T, int NT>
-__device__ __forceinline__ void poisC_body(const Dev<T> &d, int bx, T *__restrict__ psi, T *__restrict__ p, int stage) {
+// Pass C: inverse along x of the nslot row pairs of block b (as pass A) -> psi; accumulate the
+// step-averaged pressure p (core.py:935-942): stage 0 p = w psi, stage 1 p += w psi, stage 2
+// p = (p + psi) / dt.
+template <typename T, int NT, bool ONE = false>
+__device__ __forceinline__ void poisC_body(const Dev<T> &d, int bx, int nslot, T *__restrict__ psi, T *__restrict__ p,
+                                           int stage) {
   constexpr int MP = kMaxLen / NT;   // row elements per thread
   extern __shared__ unsigned char smem[];
-  const int nx = d.nx;
+  if (ONE) nslot = 1;
+  const int nx = d.nx, ny = d.ny, lx = ONE ? 0 : log2_or_neg(nx);
+  const int nel = nslot * nx;        // elements of the block's row pairs
   cplx<T> *A = reinterpret_cast<cplx<T> *>(smem);
-  cplx<T> *B = A + nx;
-  cplx<T> *TW = B + nx;
-  T *X = reinterpret_cast<T *>(TW + nx);
-  const int ja = 2 * bx, jb = ja + 1;
-  const bool hasb = jb < d.ny;
+  cplx<T> *B = A + nel;
+  cplx<T> *TW = B + nel;
+  T *X = reinterpret_cast<T *>(TW + nx);   // 2 nx reals per pair: row 2p, then row 2p + 1
+  const int j0 = 2 * bx * nslot;     // first row of the block
   cplx<T> csr[MP];
   // previous pressure accumulator values (stages 1, 2), at the positions this thread writes
   T pa[MP], pb[MP];
@@ -1300,27 +1317,32 @@ __device__ __forceinline__ void poisC_body(const Dev<T> &d, int bx, T *__restric
   // twiddles, row elements, DCT twiddles and the accumulator in one pass of loads
 #pragma unroll
   for (int r = 0; r < MP; ++r) {
-    const int k = threadIdx.x + NT * r;
-    if (k < nx) {
-      TW[k] = tw_x[k];
-      X[k] = F[ja * nx + k];
-      X[nx + k] = hasb ? F[jb * nx + k] : (T)0;
+    const int w = threadIdx.x + NT * r;
+    if (w < nel) {
+      const int f = (ONE ? 0 : fdiv(w, nx, lx)), k = w - f * nx;
+      const int ja = j0 + 2 * f;
+      if (w < nx) TW[w] = tw_x[w];
+      X[2 * f * nx + k] = ja < ny ? F[ja * nx + k] : (T)0;
+      X[2 * f * nx + nx + k] = ja + 1 < ny ? F[(ja + 1) * nx + k] : (T)0;
       csr[r] = cplx<T>{0, 0};
       if (d.dct_x) csr[r] = cs_x[k];
       if (acc_p) {
-        pa[r] = gp[ja * nx + k];
-        pb[r] = hasb ? gp[jb * nx + k] : (T)0;
+        pa[r] = ja < ny ? gp[ja * nx + k] : (T)0;
+        pb[r] = ja + 1 < ny ? gp[(ja + 1) * nx + k] : (T)0;
       }
     }
   }
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < MP; ++r) {
-    const int k = threadIdx.x + NT * r;
-    if (k < nx) pack_inverse(X, X + nx, A, nx, d.dct_x, csr[r], k);
+    const int w = threadIdx.x + NT * r;
+    if (w < nel) {
+      const int f = (ONE ? 0 : fdiv(w, nx, lx)), k = w - f * nx;
+      pack_inverse(X + 2 * f * nx, X + 2 * f * nx + nx, A + f * nx, nx, d.dct_x, csr[r], k);
+    }
   }
   __syncthreads();
-  cplx<T> *R = fft_run(A, B, d.fx, TW, 1, nx);
+  cplx<T> *R = fft_run(A, B, d.fx, TW, nslot, nx);
   const T inv_n = (T)(1.0 / (double)nx);
   T rdt = 0;
   if (p != nullptr && stage == 2) rdt = (T)1 / d.red->dt;
@@ -1328,16 +1350,21 @@ __device__ __forceinline__ void poisC_body(const Dev<T> &d, int bx, T *__restric
 #pragma unroll
   for (int rr = 0; rr < MP; ++rr) {
     // natural column q (coalesced stores) from transform position m
-    const int q = threadIdx.x + NT * rr;
-    if (q >= nx) continue;
+    const int e = threadIdx.x + NT * rr;
+    if (e >= nel) continue;
+    const int f = (ONE ? 0 : fdiv(e, nx, lx)), q = e - f * nx;
+    const int ja = j0 + 2 * f, jb = ja + 1;
+    if (ja >= ny) continue;
+    const bool hasb = jb < ny;
+    const cplx<T> *Rf = R + f * nx;
     const int m = d.dct_x ? makhoul_pos(q, nx) : q;
     T va, vb;
     if (d.dct_x) {
-      va = R[m].x * inv_n;
-      vb = -R[m].y * inv_n;
+      va = Rf[m].x * inv_n;
+      vb = -Rf[m].y * inv_n;
     } else {
       cplx<T> ha, hb;
-      split_pair(R, nx, m, ha, hb);
+      split_pair(Rf, nx, m, ha, hb);
       va = (ha.x - ha.y) * inv_n;
       vb = (hb.x - hb.y) * inv_n;
     }
@@ -1359,7 +1386,7 @@ __device__ __forceinline__ void poisC_body(const Dev<T> &d, int bx, T *__restric
 }
 template <typename T, int NT>
 __global__ void __launch_bounds__(NT) k_poisC(Dev<T> d, T *__restrict__ psi, T *__restrict__ p, int stage) {
-  poisC_body<T, NT>(d, blockIdx.x, psi, p, stage);
+  poisC_body<T, NT, true>(d, blockIdx.x, 1, psi, p, stage);
 }
 
 // u -= grad psi (standalone projection, core.py:626-660).
@@ -1643,12 +1670,19 @@ __device__ __forceinline__ void finalize_body(const Dev<T> &d, const T *__restri
   constexpr int kW = kMaxWin + 5 + 6;   // window sums, dt maxima + sum theta, wall sums
   // only the slots in use (4 per body, 1 per temperature or heat-flux body) are reduced
   const int nq = 4 * d.nbody + (d.thermal ? d.n_temp + (d.win ? d.n_flux : 0) : 0);
+  // each wave reduces only the group its lanes loaded; the other groups are zero in every lane
+  // of it, which is what their reduction would give
+  const int t0 = tid & ~63;
+  if (t0 + 63 >= r1 && t0 < r2) {
 #pragma unroll
-  for (int q = 0; q < kMaxWin; ++q)
-    if (q < nq) acc[q] = wave_sum(acc[q]);
-  for (int q = 0; q < 4; ++q) pm[q] = wave_max(pm[q]);
-  psum = wave_sum(psum);
-  if (d.thermal)
+    for (int q = 0; q < kMaxWin; ++q)
+      if (q < nq) acc[q] = wave_sum(acc[q]);
+  }
+  if (t0 + 63 >= r2) {
+    for (int q = 0; q < 4; ++q) pm[q] = wave_max(pm[q]);
+    psum = wave_sum(psum);
+  }
+  if (d.thermal && t0 < r1)
     for (int q = 0; q < 6; ++q) ws[q] = wave_sum(ws[q]);
   if ((tid & 63) == 0) {
     for (int q = 0; q < kMaxWin; ++q) lds[w * kW + q] = acc[q];
@@ -1915,7 +1949,7 @@ struct BRun {
   gptr<const T> th_new, t_in;
   gptr<T> t_out, dt_out;
   aero2d_outputs o;
-  int nslotB, mode;
+  int nslotA, nslotB, mode;   // row pairs per pass A / C block, column pairs per pass B block
 };
 
 // D: the tile kernels' view of every run's plan (Dev<T> itself: generic); the run's plan is
@@ -1952,7 +1986,7 @@ __global__ void __launch_bounds__(NT) kb_poisA(const Dev<T> *__restrict__ devs, 
                                                const int4 *__restrict__ items, int stage) {
   const int4 it = items[blockIdx.x];
   const BRun<T> &r = runs[it.x];
-  poisA_body<T, NT>(devs[it.x], it.y, r.st[stage].uh, r.st[stage].vh, (const T *)nullptr, r.mode);
+  poisA_body<T, NT>(devs[it.x], it.y, r.nslotA, r.st[stage].uh, r.st[stage].vh, (const T *)nullptr, r.mode);
 }
 
 template <typename T>
@@ -1968,7 +2002,7 @@ __global__ void __launch_bounds__(NT) kb_poisC(const Dev<T> *__restrict__ devs, 
                                                const int4 *__restrict__ items, int stage, int write_p) {
   const int4 it = items[blockIdx.x];
   const Dev<T> &d = devs[it.x];
-  poisC_body<T, NT>(d, it.y, d.psi, write_p ? runs[it.x].p_out : (T *)nullptr, stage);
+  poisC_body<T, NT>(d, it.y, runs[it.x].nslotA, d.psi, write_p ? runs[it.x].p_out : (T *)nullptr, stage);
 }
 
 template <typename T, int R, typename D = Dev<T>>
@@ -2259,8 +2293,14 @@ Dev<T> make_dev(const aero2d_plan *p, const aero2d_geom *g, const aero2d_params 
   return d;
 }
 
+// LDS of a pass A / C block over nslot row pairs of length nx: two work buffers of the pairs, the
+// twiddles, then pass C's real rows (pass A's reduction scratch fits in their place)
 template <typename T>
-size_t lds_A(const aero2d_plan *p) { return 3 * p->desc.nx * sizeof(cplx<T>) + 2 * p->desc.nx * sizeof(T) + 64 * 8; }
+size_t lds_rows(size_t nx, size_t nslot) {
+  return (2 * nslot + 1) * nx * sizeof(cplx<T>) + 2 * nslot * nx * sizeof(T) + 64 * 8;
+}
+template <typename T>
+size_t lds_A(const aero2d_plan *p) { return lds_rows<T>(p->desc.nx, 1); }
 template <typename T>
 size_t lds_B(const aero2d_plan *p) {
   return p->nslotB * p->desc.ny * (2 * sizeof(cplx<T>) + 2 * sizeof(T)) + p->desc.ny * sizeof(cplx<T>);
@@ -2671,6 +2711,7 @@ struct aero2d_ensemble {
   int ty = kDefaultRows;   // rows per tile of the batched cell launches
   int nslot = 2;           // column pairs per block of the batched pass-B launch (capped per run)
   int ntrow = kFftThreads; // threads per block of the batched row passes (A, C): 256 or 512
+  int rowfill = 1;         // 1: row passes take as many row pairs per block as fill kRowFill elements
   bool plain_laminar = false;   // every run a plain wind tunnel without LES or heat transfer
   std::vector<aero2d_plan *> plans;
   std::vector<aero2d_fields> fields;
@@ -2709,6 +2750,16 @@ int run_slots(const aero2d_plan *p, int want) {
   while (s > 1 && ((size_t)s * ny > (size_t)kMaxLen ||
                    (size_t)s * ny * (2 * sizeof(cplx<T>) + 2 * sizeof(T)) + ny * sizeof(cplx<T>) > 65536))
     s /= 2;
+  return s;
+}
+
+// row pairs per batched pass A / C block: one, or (fill) the most that keep the block's rows
+// within kRowFill elements (a power of two, at least one)
+template <typename T>
+int run_row_slots(const aero2d_plan *p, int fill) {
+  const size_t nx = p->desc.nx;
+  int s = 1;
+  while (fill && (size_t)(2 * s) * nx <= (size_t)kRowFill && lds_rows<T>(nx, 2 * s) <= 65536) s *= 2;
   return s;
 }
 
@@ -2773,13 +2824,15 @@ int ensemble_build(aero2d_ensemble *e) {
     b.t_in = global_ptr((const T *)f.t); b.t_out = global_ptr((T *)f.t); b.dt_out = global_ptr((T *)f.dt);
     if (outputs) b.o = outputs[r];
     b.nslotB = run_slots<T>(p, e->nslot);
+    b.nslotA = run_row_slots<T>(p, e->rowfill);
     b.mode = (s.periodic || s.closed) ? 1 : 2;
     for (int by = 0; by < nby; ++by)
       for (int bx = 0; bx < p->nbx; ++bx) tiles.push_back(make_int4(r, bx, by, 0));
-    for (int bx = 0; bx < (s.ny + 1) / 2; ++bx) rows.push_back(make_int4(r, bx, 0, 0));
+    const int npairs = (s.ny + 1) / 2;
+    for (int bx = 0; bx < (npairs + b.nslotA - 1) / b.nslotA; ++bx) rows.push_back(make_int4(r, bx, 0, 0));
     const int cpb = 2 * b.nslotB;
     for (int bx = 0; bx < (s.nx + cpb - 1) / cpb; ++bx) cols.push_back(make_int4(r, bx, 0, 0));
-    e->ldsA = std::max(e->ldsA, lds_A<T>(p));
+    e->ldsA = std::max(e->ldsA, lds_rows<T>(s.nx, b.nslotA));
     e->ldsB = std::max(e->ldsB, b.nslotB * s.ny * (2 * sizeof(cplx<T>) + 2 * sizeof(T)) + s.ny * sizeof(cplx<T>));
   }
   e->ntiles = (int)tiles.size();
@@ -2874,36 +2927,42 @@ int ensemble_time(aero2d_ensemble *e, hipStream_t st, int iters, float *ms) {
   return AERO_OK;
 }
 
-// Times every (tile rows, column slots) candidate and keeps the fastest; choice[0..1]
-// receives the rows and slots chosen, choice[2] the candidates timed.
+// Times every (tile rows, column slots, row fill, row threads) candidate and keeps the fastest;
+// choice[0..1] receives the rows and slots chosen, choice[2] the candidates timed, choice[3] the
+// row-pass threads, choice[4] the row fill (1: row pairs per block fill kRowFill elements).
 template <typename T>
 int ensemble_tune(aero2d_ensemble *e, hipStream_t st, int iters, int32_t *choice) {
   static const int kRows[] = {4, 8, 16};
   static const int kSlots[] = {1, 2, 4};
+  static const int kFill[] = {0, 1};
   static const int kRowThreads[] = {256, 512};
   float best = 0.f;
-  int best_ty = e->ty, best_slot = e->nslot, best_nt = e->ntrow, tried = 0;
+  int best_ty = e->ty, best_slot = e->nslot, best_fill = e->rowfill, best_nt = e->ntrow, tried = 0;
   for (int ty : kRows)
-    for (int sl : kSlots) {
-      e->ty = ty;
-      e->nslot = sl;
-      int rc = ensemble_build<T>(e);
-      for (int nt : kRowThreads) {
-        e->ntrow = nt;   // a launch shape only: the work tables do not depend on it
-        float ms = 0.f;
-        if (!rc) rc = ensemble_time<T>(e, st, iters, &ms);
-        if (rc) return rc;
-        ++tried;
-        if (tried == 1 || ms < best) {
-          best = ms;
-          best_ty = ty;
-          best_slot = sl;
-          best_nt = nt;
+    for (int sl : kSlots)
+      for (int fill : kFill) {
+        e->ty = ty;
+        e->nslot = sl;
+        e->rowfill = fill;
+        int rc = ensemble_build<T>(e);
+        for (int nt : kRowThreads) {
+          e->ntrow = nt;   // a launch shape only: the work tables do not depend on it
+          float ms = 0.f;
+          if (!rc) rc = ensemble_time<T>(e, st, iters, &ms);
+          if (rc) return rc;
+          ++tried;
+          if (tried == 1 || ms < best) {
+            best = ms;
+            best_ty = ty;
+            best_slot = sl;
+            best_fill = fill;
+            best_nt = nt;
+          }
         }
       }
-    }
   e->ty = best_ty;
   e->nslot = best_slot;
+  e->rowfill = best_fill;
   e->ntrow = best_nt;
   const int rc = ensemble_build<T>(e);
   if (choice) {
@@ -2911,6 +2970,7 @@ int ensemble_tune(aero2d_ensemble *e, hipStream_t st, int iters, int32_t *choice
     choice[1] = best_slot;
     choice[2] = tried;
     choice[3] = best_nt;
+    choice[4] = best_fill;
   }
   return rc;
 }

@@ -437,13 +437,14 @@ int aero2d_ensemble_create(int32_t n, aero2d_plan *const *plans, const aero2d_fi
 int aero2d_ensemble_step(aero2d_ensemble *ensemble, void *stream, int32_t chained);
 /* Autotuner of the batched launches (no reference counterpart): times every candidate tile
  * shape (4, 8 or 16 rows per 64-wide cell tile) x pass-B column slots (1, 2 or 4 column pairs
- * per block, capped per run by LDS) x threads of the row-transform blocks (256 or 512) on RK
- * stages 0 and 1 of every run, `iters` rounds each, and rebuilds the work tables with the
- * fastest. The timed launches write only plan scratch,
+ * per block, capped per run by LDS) x row pairs per row-transform block (one, or as many as
+ * fill 1024 elements) x threads of the row-transform blocks (256 or 512) on RK stages 0 and 1
+ * of every run, `iters` rounds each, and rebuilds the work tables with the fastest. The timed
+ * launches write only plan scratch,
  * so the runs' state and outputs are untouched (the potential buffer aside); fields, u and v
  * of later steps do not depend on the choice, force sums round differently with the tile
- * rows. choice[0..3] (may be NULL) receives the rows, the slots, the candidates timed and the
- * row-transform threads.
+ * rows. choice[0..4] (may be NULL) receives the rows, the slots, the candidates timed, the
+ * row-transform threads and the row fill (0: one row pair per block, 1: filled).
  * Synchronises the stream; call outside graph capture. */
 int aero2d_ensemble_tune(aero2d_ensemble *ensemble, void *stream, int32_t iters, int32_t *choice);
 int aero2d_ensemble_destroy(aero2d_ensemble *ensemble);

@@ -2709,7 +2709,8 @@ int aero2d_poisson_solve(aero2d_plan *p, void *stream, const void *f, void *psi)
 struct aero2d_ensemble {
   int n = 0, dbl = 0;
   int ty = kDefaultRows;   // rows per tile of the batched cell launches
-  int nslot = 2;           // column pairs per block of the batched pass-B launch (capped per run)
+  int nslot = 2;           // column pairs per block of the batched pass-B launch (capped per run;
+                           // 0: as many as fill kRowFill elements)
   int ntrow = kFftThreads; // threads per block of the batched row passes (A, C): 256 or 512
   int rowfill = 1;         // 1: row passes take as many row pairs per block as fill kRowFill elements
   bool plain_laminar = false;   // every run a plain wind tunnel without LES or heat transfer
@@ -2742,11 +2743,16 @@ void ens_release(aero2d_ensemble *e) {
 }
 
 // column pairs per pass-B block for run p: the largest of want, want / 2, ..., 1 whose
-// columns fit the per-thread element count and 64 KB of LDS
+// columns fit the per-thread element count and 64 KB of LDS; want = 0: the most that keep the
+// block's columns within kRowFill elements (a power of two)
 template <typename T>
 int run_slots(const aero2d_plan *p, int want) {
   const size_t ny = p->desc.ny;
   int s = want;
+  if (want == 0) {
+    s = 1;
+    while ((size_t)(2 * s) * ny <= (size_t)kRowFill) s *= 2;
+  }
   while (s > 1 && ((size_t)s * ny > (size_t)kMaxLen ||
                    (size_t)s * ny * (2 * sizeof(cplx<T>) + 2 * sizeof(T)) + ny * sizeof(cplx<T>) > 65536))
     s /= 2;
@@ -2933,7 +2939,7 @@ int ensemble_time(aero2d_ensemble *e, hipStream_t st, int iters, float *ms) {
 template <typename T>
 int ensemble_tune(aero2d_ensemble *e, hipStream_t st, int iters, int32_t *choice) {
   static const int kRows[] = {4, 8, 16};
-  static const int kSlots[] = {1, 2, 4};
+  static const int kSlots[] = {1, 2, 4, 0};   // 0: filled to kRowFill elements
   static const int kFill[] = {0, 1};
   static const int kRowThreads[] = {256, 512};
   float best = 0.f;

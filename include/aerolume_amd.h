@@ -437,7 +437,7 @@ int aero2d_ensemble_create(int32_t n, aero2d_plan *const *plans, const aero2d_fi
 int aero2d_ensemble_step(aero2d_ensemble *ensemble, void *stream, int32_t chained);
 /* Autotuner of the batched launches (no reference counterpart): times every candidate tile
  * shape (4, 8 or 16 rows per 64-wide cell tile) x pass-B column slots (1, 2 or 4 column pairs
- * per block, capped per run by LDS) x row pairs per row-transform block (one, or as many as
+ * per block, capped per run by LDS, or 0: as many as fill 1024 elements) x row pairs per row-transform block (one, or as many as
  * fill 1024 elements) x threads of the row-transform blocks (256 or 512) on RK stages 0 and 1
  * of every run, `iters` rounds each, and rebuilds the work tables with the fastest. The timed
  * launches write only plan scratch,

@@ -1701,7 +1701,7 @@ __global__ void __launch_bounds__(kThreads) k_p1(const Dev3<T> *__restrict__ dp,
     A[sl * nx + n] = {xa, xb};
   }
   __syncthreads();
-  const cplx<T> *R = fft_lds(A, B, d.fp[0], TW, ns, nx);
+  const cplx<T> *R = fft_run(A, B, d.fp[0], TW, ns, nx);
   for (int e = threadIdx.x; e < ns * nx; e += kThreads) {
     const int sl = fdiv(e, nx, log2_or_neg(nx)), k = e - sl * nx;
     const int ra = r0 + 2 * sl, rb = ra + 1;
@@ -1734,7 +1734,7 @@ __global__ void __launch_bounds__(kThreads) k_py(const Dev3<T> *__restrict__ dp,
       reinterpret_cast<T *>(&A[(col >> 1) * st + n])[col & 1] = val;
     }
     __syncthreads();
-    const cplx<T> *R = fft_lds(A, B, d.fp[1], TW, ns, st);
+    const cplx<T> *R = fft_run(A, B, d.fp[1], TW, ns, st);
     for (int e = threadIdx.x; e < ns * ny; e += kThreads) {
       const int sl = fdiv(e, ny, log2_or_neg(ny)), kk = e - sl * ny;
       T xa, xb;
@@ -1759,7 +1759,7 @@ __global__ void __launch_bounds__(kThreads) k_py(const Dev3<T> *__restrict__ dp,
     A[sl * st + kk] = pack_inv(X + (2 * sl) * st, X + (2 * sl + 1) * st, ny, dct, dct ? d.cs[1][kk] : cplx<T>{0, 0}, kk);
   }
   __syncthreads();
-  const cplx<T> *R = fft_lds(A, B, d.fp[1], TW, ns, st);
+  const cplx<T> *R = fft_run(A, B, d.fp[1], TW, ns, st);
   const T inv_n = (T)(1.0 / (double)ny);
   for (int e = threadIdx.x; e < ns * ny; e += kThreads) {
     const int sl = fdiv(e, ny, log2_or_neg(ny)), m = e - sl * ny;
@@ -1798,7 +1798,7 @@ __global__ void __launch_bounds__(kThreads) k_pz(const Dev3<T> *__restrict__ dp)
     reinterpret_cast<T *>(&A[(col >> 1) * st + n])[col & 1] = val;
   }
   __syncthreads();
-  const cplx<T> *R = fft_lds(A, B, d.fp[2], TW, ns, st);
+  const cplx<T> *R = fft_run(A, B, d.fp[2], TW, ns, st);
   const T *inv = d.inv_lam + (size_t)j * nx * nz;
   for (int e = threadIdx.x; e < ns * nz; e += kThreads) {
     const int sl = fdiv(e, nz, log2_or_neg(nz)), kz = e - sl * nz;
@@ -1817,7 +1817,7 @@ __global__ void __launch_bounds__(kThreads) k_pz(const Dev3<T> *__restrict__ dp)
     Z[sl * st + kz] = pack_inv(X + (2 * sl) * st, X + (2 * sl + 1) * st, nz, dct, dct ? d.cs[2][kz] : cplx<T>{0, 0}, kz);
   }
   __syncthreads();
-  const cplx<T> *R2 = fft_lds(Z, (Z == A) ? B : A, d.fp[2], TW, ns, st);
+  const cplx<T> *R2 = fft_run(Z, (Z == A) ? B : A, d.fp[2], TW, ns, st);
   const T inv_n = (T)(1.0 / (double)nz);
   for (int e = threadIdx.x; e < ns * nz; e += kThreads) {
     const int sl = fdiv(e, nz, log2_or_neg(nz)), m = e - sl * nz;
@@ -1859,7 +1859,7 @@ __global__ void __launch_bounds__(kThreads) k_p5(const Dev3<T> *__restrict__ dp,
     A[sl * st + k] = pack_inv(X + (2 * sl) * st, X + (2 * sl + 1) * st, nx, dct, dct ? d.cs[0][k] : cplx<T>{0, 0}, k);
   }
   __syncthreads();
-  const cplx<T> *R = fft_lds(A, B, d.fp[0], TW, ns, st);
+  const cplx<T> *R = fft_run(A, B, d.fp[0], TW, ns, st);
   const T inv_n = (T)(1.0 / (double)nx);
   T rdt = 0;
   if (p && stage == 2) rdt = (T)1 / step_dt(d, c);

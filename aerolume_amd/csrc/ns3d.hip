@@ -46,6 +46,7 @@ namespace aero {
 namespace ns3d {
 
 constexpr int kThreads = 256;        // Poisson passes
+constexpr int kPlaneThreads = 1024;  // the fused (z, y) plane pass
 constexpr int kTileThreads = 1024;   // stage / final tiles (16 waves)
 constexpr int kTileWaves = kTileThreads / 64;
 constexpr int kFinThreads = 1024;
@@ -105,10 +106,13 @@ struct Dev3 {
   const cplx<T> *tw[3];
   const cplx<T> *cs[3];
   const double *lam[3];
-  const T *inv_lam;          // (ny, nx, nz), z fastest: 1 / (lam_x + lam_y + lam_z) formed in float64
+  const T *inv_lam;          // 1 / (lam_x + lam_y + lam_z) formed in float64, z fastest: (ny, nx, nz),
+                             // or (nx, ny, nz) with the plane pass
   FftPlan fp[3];
   int dct[3];
   int ns1, ns2, ns3, ns5;
+  int planes;                // 1: y and z transforms in one pass over (z, y) planes, spectrum (nx, nz, ny)
+  int plane_buf;             // bytes of each of the plane pass's two LDS work buffers
   // heat transfer (ns3d/physics.py:494-830)
   int theta, buoy, tside[6], tqk0, tqk1, tqj0, tqj1, tqi0, tqi1, tqwx, tqwy, tnt, tbody[kMaxT], tadia, tnskin;
   T ri, half_ri, ri_w[3], abs_ri, prt, kc, cadv, cfl, c6, h3;
@@ -1660,10 +1664,10 @@ __global__ void __launch_bounds__(kThreads) k_p1(const Dev3<T> *__restrict__ dp,
                                                  const T *__restrict__ f, int mode, int stage) {
   extern __shared__ unsigned char smem[];
   const Dev3<T> &d = *dp;
-  const int nx = d.nx, ns = d.ns1, nrows = d.ny * d.nz;
+  const int nx = d.nx, ns = d.ns1, nrows = d.ny * d.nz, st = nx + 1;
   cplx<T> *A = reinterpret_cast<cplx<T> *>(smem);
-  cplx<T> *B = A + ns * nx;
-  cplx<T> *TW = B + ns * nx;
+  cplx<T> *B = A + ns * st;
+  cplx<T> *TW = B + ns * st;
   double *red = reinterpret_cast<double *>(TW + nx);
   for (int m = threadIdx.x; m < nx; m += kThreads) TW[m] = d.tw[0][m];
   T corr = 0;
@@ -1698,15 +1702,29 @@ __global__ void __launch_bounds__(kThreads) k_p1(const Dev3<T> *__restrict__ dp,
       if (ra < nrows) xa = div_at(d, u, v, w, ra / d.ny, ra % d.ny, q, corr);
       if (rb < nrows) xb = div_at(d, u, v, w, rb / d.ny, rb % d.ny, q, corr);
     }
-    A[sl * nx + n] = {xa, xb};
+    A[sl * st + n] = {xa, xb};
   }
   __syncthreads();
-  const cplx<T> *R = fft_run(A, B, d.fp[0], TW, ns, nx);
+  const cplx<T> *R = fft_run(A, B, d.fp[0], TW, ns, st);
+  if (d.planes) {
+    // spectrum (nx, rows): wavenumber k of row r at k nrows + r; lanes walk the block's rows
+    // first (runs of 2 ns consecutive rows per wavenumber)
+    const int lns = log2_or_neg(ns);
+    for (int e = threadIdx.x; e < ns * nx; e += kThreads) {
+      const int k = fdiv(e, ns, lns), sl = e - k * ns;
+      const int ra = r0 + 2 * sl, rb = ra + 1;
+      T xa, xb;
+      fwd_split(R + sl * st, nx, k, d.dct[0], d.dct[0] ? d.cs[0][k] : cplx<T>{0, 0}, xa, xb);
+      if (ra < nrows) d.F[(size_t)k * nrows + ra] = xa;
+      if (rb < nrows) d.F[(size_t)k * nrows + rb] = xb;
+    }
+    return;
+  }
   for (int e = threadIdx.x; e < ns * nx; e += kThreads) {
     const int sl = fdiv(e, nx, log2_or_neg(nx)), k = e - sl * nx;
     const int ra = r0 + 2 * sl, rb = ra + 1;
     T xa, xb;
-    fwd_split(R + sl * nx, nx, k, d.dct[0], d.dct[0] ? d.cs[0][k] : cplx<T>{0, 0}, xa, xb);
+    fwd_split(R + sl * st, nx, k, d.dct[0], d.dct[0] ? d.cs[0][k] : cplx<T>{0, 0}, xa, xb);
     if (ra < nrows) d.F[(size_t)ra * nx + k] = xa;
     if (rb < nrows) d.F[(size_t)rb * nx + k] = xb;
   }
@@ -1834,6 +1852,103 @@ __global__ void __launch_bounds__(kThreads) k_pz(const Dev3<T> *__restrict__ dp)
   }
 }
 
+// P2-P4 in one pass (planes that fit one block's LDS): the (z, y) plane of x wavenumber
+// blockIdx.x, contiguous in the spectrum layout (nx, nz, ny) that P1 writes. Forward along y
+// (pairs of z rows as one complex sequence), forward along z (pairs of y columns), x 1/lambda,
+// inverse along z, inverse along y, all in LDS: one read and one write of the plane instead of
+// three of each over the whole field. Two work buffers take turns: the complex sequences of a
+// transform, its Stockham partner, then the real rows the next transform packs.
+template <typename T>
+__global__ void __launch_bounds__(kPlaneThreads) k_planes(const Dev3<T> *__restrict__ dp) {
+  extern __shared__ unsigned char smem[];
+  const Dev3<T> &d = *dp;
+  const int ny = d.ny, nz = d.nz, sy = ny + 1, sz = nz + 1, hz = nz / 2, hy = ny / 2;
+  const int ly = log2_or_neg(ny), lz = log2_or_neg(nz), tid = threadIdx.x;
+  cplx<T> *const W0 = reinterpret_cast<cplx<T> *>(smem);
+  cplx<T> *const W1 = reinterpret_cast<cplx<T> *>(smem + d.plane_buf);
+  cplx<T> *TWy = reinterpret_cast<cplx<T> *>(smem + 2 * (size_t)d.plane_buf);
+  cplx<T> *TWz = TWy + ny;
+  auto other = [&](const void *b) { return b == (const void *)W0 ? W1 : W0; };
+  for (int m = tid; m < ny; m += kPlaneThreads) TWy[m] = d.tw[1][m];
+  for (int m = tid; m < nz; m += kPlaneThreads) TWz[m] = d.tw[2][m];
+  const int dy = d.dct[1], dz = d.dct[2];
+  T *plane = d.F + (size_t)blockIdx.x * nz * ny;
+  // z row k, y position j -> sequence k / 2 (component k & 1) at the transform position of j
+  for (int e = tid; e < nz * ny; e += kPlaneThreads) {
+    const int k = fdiv(e, ny, ly), j = e - k * ny;
+    const int n = dy ? makhoul_pos(j, ny) : j;
+    reinterpret_cast<T *>(&W0[(k >> 1) * sy + n])[k & 1] = plane[e];
+  }
+  __syncthreads();
+  cplx<T> *R = fft_run(W0, W1, d.fp[1], TWy, hz, sy);
+  // y spectra of z rows 2s, 2s + 1 at ky -> z sequences: y column pair ky / 2 (component
+  // ky & 1), at the transform position of z row k
+  cplx<T> *Z = other(R);
+  for (int e = tid; e < hz * ny; e += kPlaneThreads) {
+    const int sl = fdiv(e, ny, ly), ky = e - sl * ny;
+    T xa, xb;
+    fwd_split(R + sl * sy, ny, ky, dy, dy ? d.cs[1][ky] : cplx<T>{0, 0}, xa, xb);
+    const int ka = 2 * sl, kb = ka + 1;
+    T *zs = reinterpret_cast<T *>(Z + (ky >> 1) * sz);
+    zs[2 * (dz ? makhoul_pos(ka, nz) : ka) + (ky & 1)] = xa;
+    zs[2 * (dz ? makhoul_pos(kb, nz) : kb) + (ky & 1)] = xb;
+  }
+  __syncthreads();
+  R = fft_run(Z, other(Z), d.fp[2], TWz, hy, sz);
+  // z spectra of y columns 2t, 2t + 1, x 1/lambda -> real rows X[ky][kz]
+  T *X = reinterpret_cast<T *>(other(R));
+  const T *inv = d.inv_lam + (size_t)blockIdx.x * ny * nz;
+  for (int e = tid; e < hy * nz; e += kPlaneThreads) {
+    const int t = fdiv(e, nz, lz), kz = e - t * nz;
+    T xa, xb;
+    fwd_split(R + t * sz, nz, kz, dz, dz ? d.cs[2][kz] : cplx<T>{0, 0}, xa, xb);
+    X[(2 * t) * sz + kz] = xa * inv[(size_t)(2 * t) * nz + kz];
+    X[(2 * t + 1) * sz + kz] = xb * inv[(size_t)(2 * t + 1) * nz + kz];
+  }
+  __syncthreads();
+  Z = R;
+  for (int e = tid; e < hy * nz; e += kPlaneThreads) {
+    const int t = fdiv(e, nz, lz), kz = e - t * nz;
+    Z[t * sz + kz] = pack_inv(X + (2 * t) * sz, X + (2 * t + 1) * sz, nz, dz, dz ? d.cs[2][kz] : cplx<T>{0, 0}, kz);
+  }
+  __syncthreads();
+  R = fft_run(Z, other(Z), d.fp[2], TWz, hy, sz);
+  // values at z row q of y columns 2t, 2t + 1 -> real z rows X[q][ky]
+  X = reinterpret_cast<T *>(other(R));
+  const T inv_nz = (T)(1.0 / (double)nz), inv_ny = (T)(1.0 / (double)ny);
+  for (int e = tid; e < hy * nz; e += kPlaneThreads) {
+    const int t = fdiv(e, nz, lz), m = e - t * nz;
+    T va, vb;
+    inv_unpack(R + t * sz, nz, m, dz, inv_nz, va, vb);
+    const int q = dz ? makhoul_src(m, nz) : m;
+    X[q * sy + 2 * t] = va;
+    X[q * sy + 2 * t + 1] = vb;
+  }
+  __syncthreads();
+  Z = R;
+  for (int e = tid; e < hz * ny; e += kPlaneThreads) {
+    const int sl = fdiv(e, ny, ly), ky = e - sl * ny;
+    Z[sl * sy + ky] = pack_inv(X + (2 * sl) * sy, X + (2 * sl + 1) * sy, ny, dy, dy ? d.cs[1][ky] : cplx<T>{0, 0}, ky);
+  }
+  __syncthreads();
+  R = fft_run(Z, other(Z), d.fp[1], TWy, hz, sy);
+  // values at y position q of z rows 2s, 2s + 1 -> the plane, then stored in natural order
+  X = reinterpret_cast<T *>(other(R));
+  for (int e = tid; e < hz * ny; e += kPlaneThreads) {
+    const int sl = fdiv(e, ny, ly), m = e - sl * ny;
+    T va, vb;
+    inv_unpack(R + sl * sy, ny, m, dy, inv_ny, va, vb);
+    const int q = dy ? makhoul_src(m, ny) : m;
+    X[(2 * sl) * sy + q] = va;
+    X[(2 * sl + 1) * sy + q] = vb;
+  }
+  __syncthreads();
+  for (int e = tid; e < nz * ny; e += kPlaneThreads) {
+    const int k = fdiv(e, ny, ly), j = e - k * ny;
+    plane[e] = X[k * sy + j];
+  }
+}
+
 // P5: rows (pairs) along x, inverse -> psi; the step-averaged pressure p (core.py:874-875):
 // stage 0 p = w psi, stage 1 p = p + w psi, stage 2 p = (p + psi) (1 / dt).
 template <typename T>
@@ -1849,9 +1964,18 @@ __global__ void __launch_bounds__(kThreads) k_p5(const Dev3<T> *__restrict__ dp,
   for (int m = threadIdx.x; m < nx; m += kThreads) TW[m] = d.tw[0][m];
   const int r0 = 2 * ns * blockIdx.x;
   const int dct = d.dct[0];
-  for (int e = threadIdx.x; e < 2 * ns * nx; e += kThreads) {
-    const int rr = fdiv(e, nx, log2_or_neg(nx)), k = e - rr * nx;
-    X[rr * st + k] = r0 + rr < nrows ? d.F[(size_t)(r0 + rr) * nx + k] : (T)0;
+  if (d.planes) {
+    // spectrum (nx, rows): lanes walk the block's rows first
+    const int l2 = log2_or_neg(2 * ns);
+    for (int e = threadIdx.x; e < 2 * ns * nx; e += kThreads) {
+      const int k = fdiv(e, 2 * ns, l2), rr = e - k * 2 * ns;
+      X[rr * st + k] = r0 + rr < nrows ? d.F[(size_t)k * nrows + r0 + rr] : (T)0;
+    }
+  } else {
+    for (int e = threadIdx.x; e < 2 * ns * nx; e += kThreads) {
+      const int rr = fdiv(e, nx, log2_or_neg(nx)), k = e - rr * nx;
+      X[rr * st + k] = r0 + rr < nrows ? d.F[(size_t)(r0 + rr) * nx + k] : (T)0;
+    }
   }
   __syncthreads();
   for (int e = threadIdx.x; e < ns * nx; e += kThreads) {
@@ -2020,6 +2144,8 @@ struct aero3d_plan {
   int nbx, nby, nbz, nblk, nwt, nq;
   int ns1, ns2, ns3, ns5;
   size_t lds1, lds2, lds3, lds5;
+  int planes;           // the fused (z, y) plane pass (FP32 planes that fit one block's LDS)
+  size_t plane_buf, lds_planes;
   int nwt_alloc;
 };
 
@@ -2078,7 +2204,8 @@ int upload3(aero3d_plan *p) {
       for (int i = 0; i < nx; ++i)
         for (int k = 0; k < nz; ++k) {
           const double la = (lams[2][k] + lams[1][j]) + lams[0][i];
-          inv[((size_t)j * nx + i) * nz + k] = (T)((la != 0.0 && !(i == 0 && j == 0 && k == 0)) ? 1.0 / la : 0.0);
+          const size_t at = p->planes ? ((size_t)i * ny + j) * nz + k : ((size_t)j * nx + i) * nz + k;
+          inv[at] = (T)((la != 0.0 && !(i == 0 && j == 0 && k == 0)) ? 1.0 / la : 0.0);
         }
     AERO_HIP_CHECK(hipMemcpy(p->inv_lam, inv.data(), inv.size() * sizeof(T), hipMemcpyHostToDevice));
   }
@@ -2173,6 +2300,8 @@ int upload3(aero3d_plan *p) {
   d.ns2 = p->ns2;
   d.ns3 = p->ns3;
   d.ns5 = p->ns5;
+  d.planes = p->planes;
+  d.plane_buf = (int)p->plane_buf;
   if (s.thermal) {
     d.theta = 1;
     d.buoy = s.richardson != 0.0;
@@ -2226,6 +2355,9 @@ int upload3(aero3d_plan *p) {
   if (sizeof(LdsT<T>) > 64 * 1024)
     AERO_HIP_CHECK(hipFuncSetAttribute((const void *)k_theta<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)sizeof(LdsT<T>)));
+  if (p->planes)
+    AERO_HIP_CHECK(hipFuncSetAttribute((const void *)k_planes<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)p->lds_planes));
   return AERO_OK;
 }
 
@@ -2300,11 +2432,15 @@ int poisson_passes(aero3d_plan *p, hipStream_t st, const Dev3<T> *dev, Call3<T> 
   const int rows = s.ny * s.nz;
   const int g1 = (rows + 2 * p->ns1 - 1) / (2 * p->ns1);
   hipLaunchKernelGGL(k_p1<T>, dim3(g1), dim3(kThreads), p->lds1, st, dev, c.uo, c.vo, c.wo, f, mode, stage);
-  const dim3 g2((s.nx + 2 * p->ns2 - 1) / (2 * p->ns2), s.nz);
-  hipLaunchKernelGGL(k_py<T>, g2, dim3(kThreads), p->lds2, st, dev, 0);
-  const dim3 g3((s.nx + 2 * p->ns3 - 1) / (2 * p->ns3), s.ny);
-  hipLaunchKernelGGL(k_pz<T>, g3, dim3(kThreads), p->lds3, st, dev);
-  hipLaunchKernelGGL(k_py<T>, g2, dim3(kThreads), p->lds2, st, dev, 1);
+  if (p->planes) {
+    hipLaunchKernelGGL(k_planes<T>, dim3(s.nx), dim3(kPlaneThreads), p->lds_planes, st, dev);
+  } else {
+    const dim3 g2((s.nx + 2 * p->ns2 - 1) / (2 * p->ns2), s.nz);
+    hipLaunchKernelGGL(k_py<T>, g2, dim3(kThreads), p->lds2, st, dev, 0);
+    const dim3 g3((s.nx + 2 * p->ns3 - 1) / (2 * p->ns3), s.ny);
+    hipLaunchKernelGGL(k_pz<T>, g3, dim3(kThreads), p->lds3, st, dev);
+    hipLaunchKernelGGL(k_py<T>, g2, dim3(kThreads), p->lds2, st, dev, 1);
+  }
   const int g5 = (rows + 2 * p->ns5 - 1) / (2 * p->ns5);
   hipLaunchKernelGGL(k_p5<T>, dim3(g5), dim3(kThreads), p->lds5, st, dev, c, psi, pr, stage);
   AERO_HIP_CHECK(hipGetLastError());
@@ -2686,6 +2822,18 @@ int aero3d_plan_create(const aero3d_desc *desc, aero3d_plan **out) {
     const int nwby = std::min(desc->win_j1 / ty, p->nby - 1) - desc->win_j0 / ty + 1;
     const int nwbz = std::min(desc->win_k1 / tz, p->nbz - 1) - desc->win_k0 / tz + 1;
     p->nwt = nwbx * nwby * nwbz;
+  }
+  // the fused plane pass: FP32, even ny and nz, two work buffers and the twiddles in one
+  // block's LDS
+  {
+    const size_t c = 2 * es;
+    p->plane_buf = std::max((nz / 2) * (ny + 1), (ny / 2) * (nz + 1)) * c;
+    p->lds_planes = 2 * p->plane_buf + (ny + nz) * c;
+    int dev_id = 0, max_lds = 0;
+    if (hipGetDevice(&dev_id) != hipSuccess ||
+        hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev_id) != hipSuccess)
+      max_lds = 64 * 1024;
+    p->planes = !p->dbl && ny % 2 == 0 && nz % 2 == 0 && p->lds_planes <= (size_t)max_lds;
   }
   p->lds1 = pick_slots((int)nx, es, 0, 4, p->ns1);
   p->lds2 = pick_slots((int)ny, es, 2, 8, p->ns2);

@@ -54,10 +54,13 @@ def _cfg(nx, ny, nz, dtype, **kw):
 
 @pytest.mark.parametrize("shape,kw,dtype", [((24, 16, 16), {}, "float64"), ((30, 20, 12), {"bc_y": "periodic"}, "float64"),
                                             ((16, 16, 16), {"periodic": True}, "float64"),
-                                            ((40, 24, 20), {"bc_y": "periodic"}, "float32")])
+                                            ((40, 24, 20), {"bc_y": "periodic"}, "float32"),
+                                            ((32, 16, 24), {}, "float32"),
+                                            ((16, 16, 16), {"periodic": True}, "float32")])
 def test_poisson_solve_matches_oracle(shape, kw, dtype):
     """Exact DCT / DHT solve on the GPU equals the oracle's dense-basis solve and satisfies
-    L psi = f - mean(f)."""
+    L psi = f - mean(f). The float32 cases take the fused (z, y) plane pass (even ny and nz),
+    float64 the separate y and z passes."""
     _need_gpu()
     from oracle import ns3d as O
     S = _S()

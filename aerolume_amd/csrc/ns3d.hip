@@ -47,6 +47,7 @@ namespace ns3d {
 
 constexpr int kThreads = 256;        // Poisson passes
 constexpr int kPlaneThreads = 1024;  // the fused (z, y) plane pass
+constexpr int kPlaneMax = 20480;     // longest plane (ny nz) of the fused pass: 20 values per thread
 constexpr int kTileThreads = 1024;   // stage / final tiles (16 waves)
 constexpr int kTileWaves = kTileThreads / 64;
 constexpr int kFinThreads = 1024;
@@ -1860,24 +1861,55 @@ __global__ void __launch_bounds__(kThreads) k_pz(const Dev3<T> *__restrict__ dp)
 // transform, its Stockham partner, then the real rows the next transform packs.
 template <typename T>
 __global__ void __launch_bounds__(kPlaneThreads) k_planes(const Dev3<T> *__restrict__ dp) {
+  constexpr int MP = kPlaneMax / kPlaneThreads;   // plane values per thread
   extern __shared__ unsigned char smem[];
   const Dev3<T> &d = *dp;
-  const int ny = d.ny, nz = d.nz, sy = ny + 1, sz = nz + 1, hz = nz / 2, hy = ny / 2;
+  const int ny = d.ny, nz = d.nz, sy = ny + 1, sz = nz + 1, hz = nz / 2, hy = ny / 2, np = ny * nz;
   const int ly = log2_or_neg(ny), lz = log2_or_neg(nz), tid = threadIdx.x;
   cplx<T> *const W0 = reinterpret_cast<cplx<T> *>(smem);
   cplx<T> *const W1 = reinterpret_cast<cplx<T> *>(smem + d.plane_buf);
   cplx<T> *TWy = reinterpret_cast<cplx<T> *>(smem + 2 * (size_t)d.plane_buf);
   cplx<T> *TWz = TWy + ny;
+  cplx<T> *CSy = TWz + nz;
+  cplx<T> *CSz = CSy + ny;
   auto other = [&](const void *b) { return b == (const void *)W0 ? W1 : W0; };
-  for (int m = tid; m < ny; m += kPlaneThreads) TWy[m] = d.tw[1][m];
-  for (int m = tid; m < nz; m += kPlaneThreads) TWz[m] = d.tw[2][m];
   const int dy = d.dct[1], dz = d.dct[2];
-  T *plane = d.F + (size_t)blockIdx.x * nz * ny;
+  T *plane = d.F + (size_t)blockIdx.x * np;
+  const T *inv = d.inv_lam + (size_t)blockIdx.x * np;   // [ky][kz]
+  // every global load of the pass is issued here, before anything waits on one: the plane,
+  // the spectral scalings this thread applies, the tables
+  T pv[MP], sa[MP / 2], sb[MP / 2];
+#pragma unroll
+  for (int r = 0; r < MP; ++r) {
+    const int e = tid + kPlaneThreads * r;
+    if (e < np) pv[r] = plane[e];
+  }
+#pragma unroll
+  for (int r = 0; r < MP / 2; ++r) {
+    const int e = tid + kPlaneThreads * r;
+    if (e < hy * nz) {
+      const int t = fdiv(e, nz, lz), kz = e - t * nz;
+      sa[r] = inv[(size_t)(2 * t) * nz + kz];
+      sb[r] = inv[(size_t)(2 * t + 1) * nz + kz];
+    }
+  }
+  for (int m = tid; m < ny; m += kPlaneThreads) {
+    TWy[m] = d.tw[1][m];
+    CSy[m] = dy ? d.cs[1][m] : cplx<T>{0, 0};
+  }
+  for (int m = tid; m < nz; m += kPlaneThreads) {
+    TWz[m] = d.tw[2][m];
+    CSz[m] = dz ? d.cs[2][m] : cplx<T>{0, 0};
+  }
   // z row k, y position j -> sequence k / 2 (component k & 1) at the transform position of j
-  for (int e = tid; e < nz * ny; e += kPlaneThreads) {
-    const int k = fdiv(e, ny, ly), j = e - k * ny;
-    const int n = dy ? makhoul_pos(j, ny) : j;
-    reinterpret_cast<T *>(&W0[(k >> 1) * sy + n])[k & 1] = plane[e];
+#pragma unroll
+  for (int r = 0; r < MP; ++r) {
+    const int e = tid + kPlaneThreads * r;
+    if (e < np) {
+      const int k = fdiv(e, ny, ly), j = e - k * ny;
+      const int n = dy ? makhoul_pos(j, ny) : j;
+      reinterpret_cast<T *>(&W0[(k >> 1) * sy + n])[k & 1] = pv[r];
+    }
   }
   __syncthreads();
   cplx<T> *R = fft_run(W0, W1, d.fp[1], TWy, hz, sy);
@@ -1887,7 +1919,7 @@ __global__ void __launch_bounds__(kPlaneThreads) k_planes(const Dev3<T> *__restr
   for (int e = tid; e < hz * ny; e += kPlaneThreads) {
     const int sl = fdiv(e, ny, ly), ky = e - sl * ny;
     T xa, xb;
-    fwd_split(R + sl * sy, ny, ky, dy, dy ? d.cs[1][ky] : cplx<T>{0, 0}, xa, xb);
+    fwd_split(R + sl * sy, ny, ky, dy, CSy[ky], xa, xb);
     const int ka = 2 * sl, kb = ka + 1;
     T *zs = reinterpret_cast<T *>(Z + (ky >> 1) * sz);
     zs[2 * (dz ? makhoul_pos(ka, nz) : ka) + (ky & 1)] = xa;
@@ -1897,19 +1929,22 @@ __global__ void __launch_bounds__(kPlaneThreads) k_planes(const Dev3<T> *__restr
   R = fft_run(Z, other(Z), d.fp[2], TWz, hy, sz);
   // z spectra of y columns 2t, 2t + 1, x 1/lambda -> real rows X[ky][kz]
   T *X = reinterpret_cast<T *>(other(R));
-  const T *inv = d.inv_lam + (size_t)blockIdx.x * ny * nz;
-  for (int e = tid; e < hy * nz; e += kPlaneThreads) {
-    const int t = fdiv(e, nz, lz), kz = e - t * nz;
-    T xa, xb;
-    fwd_split(R + t * sz, nz, kz, dz, dz ? d.cs[2][kz] : cplx<T>{0, 0}, xa, xb);
-    X[(2 * t) * sz + kz] = xa * inv[(size_t)(2 * t) * nz + kz];
-    X[(2 * t + 1) * sz + kz] = xb * inv[(size_t)(2 * t + 1) * nz + kz];
+#pragma unroll
+  for (int r = 0; r < MP / 2; ++r) {
+    const int e = tid + kPlaneThreads * r;
+    if (e < hy * nz) {
+      const int t = fdiv(e, nz, lz), kz = e - t * nz;
+      T xa, xb;
+      fwd_split(R + t * sz, nz, kz, dz, CSz[kz], xa, xb);
+      X[(2 * t) * sz + kz] = xa * sa[r];
+      X[(2 * t + 1) * sz + kz] = xb * sb[r];
+    }
   }
   __syncthreads();
   Z = R;
   for (int e = tid; e < hy * nz; e += kPlaneThreads) {
     const int t = fdiv(e, nz, lz), kz = e - t * nz;
-    Z[t * sz + kz] = pack_inv(X + (2 * t) * sz, X + (2 * t + 1) * sz, nz, dz, dz ? d.cs[2][kz] : cplx<T>{0, 0}, kz);
+    Z[t * sz + kz] = pack_inv(X + (2 * t) * sz, X + (2 * t + 1) * sz, nz, dz, CSz[kz], kz);
   }
   __syncthreads();
   R = fft_run(Z, other(Z), d.fp[2], TWz, hy, sz);
@@ -1928,7 +1963,7 @@ __global__ void __launch_bounds__(kPlaneThreads) k_planes(const Dev3<T> *__restr
   Z = R;
   for (int e = tid; e < hz * ny; e += kPlaneThreads) {
     const int sl = fdiv(e, ny, ly), ky = e - sl * ny;
-    Z[sl * sy + ky] = pack_inv(X + (2 * sl) * sy, X + (2 * sl + 1) * sy, ny, dy, dy ? d.cs[1][ky] : cplx<T>{0, 0}, ky);
+    Z[sl * sy + ky] = pack_inv(X + (2 * sl) * sy, X + (2 * sl + 1) * sy, ny, dy, CSy[ky], ky);
   }
   __syncthreads();
   R = fft_run(Z, other(Z), d.fp[1], TWy, hz, sy);
@@ -1943,9 +1978,13 @@ __global__ void __launch_bounds__(kPlaneThreads) k_planes(const Dev3<T> *__restr
     X[(2 * sl + 1) * sy + q] = vb;
   }
   __syncthreads();
-  for (int e = tid; e < nz * ny; e += kPlaneThreads) {
-    const int k = fdiv(e, ny, ly), j = e - k * ny;
-    plane[e] = X[k * sy + j];
+#pragma unroll
+  for (int r = 0; r < MP; ++r) {
+    const int e = tid + kPlaneThreads * r;
+    if (e < np) {
+      const int k = fdiv(e, ny, ly), j = e - k * ny;
+      plane[e] = X[k * sy + j];
+    }
   }
 }
 
@@ -2828,14 +2867,18 @@ int aero3d_plan_create(const aero3d_desc *desc, aero3d_plan **out) {
   {
     const size_t c = 2 * es;
     p->plane_buf = std::max((nz / 2) * (ny + 1), (ny / 2) * (nz + 1)) * c;
-    p->lds_planes = 2 * p->plane_buf + (ny + nz) * c;
+    p->lds_planes = 2 * p->plane_buf + 2 * (ny + nz) * c;
     int dev_id = 0, max_lds = 0;
     if (hipGetDevice(&dev_id) != hipSuccess ||
         hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev_id) != hipSuccess)
       max_lds = 64 * 1024;
-    p->planes = !p->dbl && ny % 2 == 0 && nz % 2 == 0 && p->lds_planes <= (size_t)max_lds;
+    p->planes = !p->dbl && ny % 2 == 0 && nz % 2 == 0 && ny * nz <= (size_t)kPlaneMax &&
+                p->lds_planes <= (size_t)max_lds;
   }
-  p->lds1 = pick_slots((int)nx, es, 0, 4, p->ns1);
+  // the forward x pass takes 8 row pairs per block with the plane pass (64-byte runs of each
+  // wavenumber in the (x, z, y) spectrum: 29.4 -> 26.8 us at 256x128x128; the inverse pass is
+  // slower so, 32.2 -> 40.8 us), 4 otherwise
+  p->lds1 = pick_slots((int)nx, es, 0, p->planes ? 8 : 4, p->ns1);
   p->lds2 = pick_slots((int)ny, es, 2, 8, p->ns2);
   p->lds3 = pick_slots((int)nz, es, 2, 8, p->ns3);
   p->lds5 = pick_slots((int)nx, es, 2, 4, p->ns5);

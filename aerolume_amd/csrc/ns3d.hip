@@ -1999,7 +1999,9 @@ __global__ void __launch_bounds__(kThreads) k_p5(const Dev3<T> *__restrict__ dp,
   cplx<T> *A = reinterpret_cast<cplx<T> *>(smem);
   cplx<T> *B = A + ns * st;
   cplx<T> *TW = B + ns * st;
-  T *X = reinterpret_cast<T *>(TW + nx);
+  // the real rows (2 ns of st values) share the transform's second buffer, which is free before
+  // the transform; after it they move to whichever buffer does not hold the result
+  T *X = reinterpret_cast<T *>(B);
   for (int m = threadIdx.x; m < nx; m += kThreads) TW[m] = d.tw[0][m];
   const int r0 = 2 * ns * blockIdx.x;
   const int dct = d.dct[0];
@@ -2023,6 +2025,7 @@ __global__ void __launch_bounds__(kThreads) k_p5(const Dev3<T> *__restrict__ dp,
   }
   __syncthreads();
   const cplx<T> *R = fft_run(A, B, d.fp[0], TW, ns, st);
+  X = reinterpret_cast<T *>(R == A ? B : A);
   const T inv_n = (T)(1.0 / (double)nx);
   T rdt = 0;
   if (p && stage == 2) rdt = (T)1 / step_dt(d, c);
@@ -2881,7 +2884,7 @@ int aero3d_plan_create(const aero3d_desc *desc, aero3d_plan **out) {
   p->lds1 = pick_slots((int)nx, es, 0, p->planes ? 8 : 4, p->ns1);
   p->lds2 = pick_slots((int)ny, es, 2, 8, p->ns2);
   p->lds3 = pick_slots((int)nz, es, 2, 8, p->ns3);
-  p->lds5 = pick_slots((int)nx, es, 2, 4, p->ns5);
+  p->lds5 = pick_slots((int)nx, es, 0, 4, p->ns5);   // its real rows share a transform buffer
   const size_t nu_ = nz * ny * (nx + 1), nv_ = nz * (ny + 1) * nx, nw_ = (nz + 1) * ny * nx, nc = nz * ny * nx;
   struct { void **ptr; size_t n; } tab[] = {
       {&p->dev, sizeof(Dev3<double>)},

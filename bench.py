@@ -53,9 +53,10 @@ SEED3 = 303
 # 3D stage kernel at stages 2-3 (FP32): read u^, v^, w^ of the previous stage, psi, and u0,
 # v0, w0 of the step start (7 x 4 B), write u^, v^, w^ (3 x 4 B)
 STAGE3_BYTES_PER_CELL = 40
-# whole step: first stage 28 + two stages 40 + three projections (16 + 8 + 8 + 8 + 16) + final
-# 28 + nu_t saved by the final kernel and read by the next first stage 8
-STEP3_BYTES_PER_CELL = 312
+# whole step: first stage 28 + two stages 40 + three projections (x pass 16, the fused (z, y)
+# plane pass 8 + its 1/lambda table 4, inverse x pass 16) + final 28 + nu_t saved by the final
+# kernel and read by the next first stage 8
+STEP3_BYTES_PER_CELL = 276
 # HBM bytes per stage-2 launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by the
 # same counters on a 512 MiB device copy (fetch x2.000, write x1.000):
 # profiles/stage_kernels_hbm_counters.md

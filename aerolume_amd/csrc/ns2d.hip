@@ -1197,15 +1197,20 @@ __device__ __forceinline__ void pack_inverse(const T *xa, const T *xb, cplx<T> *
 }
 
 // Pass B: column pairs, nslot per block: forward along y, x 1/lambda (FP64), inverse along y.
+// The global loads and stores walk the block's column pairs at one row (2 nslot consecutive
+// values of a row per segment); the spectral side keeps one column's elements per run of lanes.
+// Each column sits one element apart from the next in LDS (slot stride ny + 1), so the
+// row-walking accesses fall in different banks.
 template <typename T, int NT>
 __device__ __forceinline__ void poisB_body(const Dev<T> &d, int bx, int nslot) {
   constexpr int MP = kMaxLen / NT;   // column elements per thread
   extern __shared__ unsigned char smem[];
-  const int ny = d.ny, nx = d.nx;
+  const int ny = d.ny, nx = d.nx, st = ny + 1, nel = nslot * ny;
+  const int ly = log2_or_neg(ny), ls = log2_or_neg(nslot);
   cplx<T> *A = reinterpret_cast<cplx<T> *>(smem);
-  cplx<T> *B = A + nslot * ny;
-  cplx<T> *TW = B + nslot * ny;
-  T *X = reinterpret_cast<T *>(TW + ny);   // 2 * nslot * ny reals
+  cplx<T> *B = A + nslot * st;
+  cplx<T> *TW = B + nslot * st;
+  T *X = reinterpret_cast<T *>(TW + ny);   // 2 nslot rows of st reals
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int c0 = 2 * nslot * bx;
   const auto F = d.F;
@@ -1218,28 +1223,29 @@ __device__ __forceinline__ void poisB_body(const Dev<T> &d, int bx, int nslot) {
   for (int r = 0; r < MP; ++r) {
     const int w = tid + nthr * r;
     if (w < ny) TW[w] = tw_y[w];
-    if (w < nslot * ny) {
-      const int f = fdiv(w, ny, log2_or_neg(ny)), n = w - f * ny;
+    if (w < nel) {
+      const int n = fdiv(w, nslot, ls), f = w - n * nslot;   // row-walking: pair f of row n
       const int ca = c0 + 2 * f, cb = ca + 1;
       const int q = d.dct_y ? makhoul_src(n, ny) : n;
       const T xa = ca < nx ? F[q * nx + ca] : (T)0;
       const T xb = cb < nx ? F[q * nx + cb] : (T)0;
-      A[f * ny + n] = {xa, xb};
-      csr[r] = cs_y[n];
-      iar[r] = ca < nx ? inv_lam[(size_t)ca * ny + n] : (T)0;
-      ibr[r] = cb < nx ? inv_lam[(size_t)cb * ny + n] : (T)0;
+      A[f * st + n] = {xa, xb};
+      const int g = fdiv(w, ny, ly), k = w - g * ny;          // column-walking: element k of pair g
+      const int ga = c0 + 2 * g, gb = ga + 1;
+      csr[r] = cs_y[k];
+      iar[r] = ga < nx ? inv_lam[(size_t)ga * ny + k] : (T)0;
+      ibr[r] = gb < nx ? inv_lam[(size_t)gb * ny + k] : (T)0;
     }
   }
   __syncthreads();
-  cplx<T> *R = fft_run(A, B, d.fy, TW, nslot, ny);
+  cplx<T> *R = fft_run(A, B, d.fy, TW, nslot, st);
 #pragma unroll
   for (int rr = 0; rr < MP; ++rr) {
     const int w = tid + nthr * rr;
-    if (w >= nslot * ny) continue;
-    const int f = fdiv(w, ny, log2_or_neg(ny)), k = w - f * ny;
-    const int ca = c0 + 2 * f, cb = ca + 1;
+    if (w >= nel) continue;
+    const int f = fdiv(w, ny, ly), k = w - f * ny;
     cplx<T> va, vb;
-    split_pair(R + f * ny, ny, k, va, vb);
+    split_pair(R + f * st, ny, k, va, vb);
     T xa, xb;
     if (d.dct_y) {
       const cplx<T> cs = csr[rr];
@@ -1249,31 +1255,32 @@ __device__ __forceinline__ void poisB_body(const Dev<T> &d, int bx, int nslot) {
       xa = va.x - va.y;
       xb = vb.x - vb.y;
     }
-    X[(2 * f) * ny + k] = xa * iar[rr];
-    X[(2 * f + 1) * ny + k] = xb * ibr[rr];
+    X[(2 * f) * st + k] = xa * iar[rr];
+    X[(2 * f + 1) * st + k] = xb * ibr[rr];
   }
   __syncthreads();
   cplx<T> *Z = (R == A) ? B : A;
 #pragma unroll
   for (int rr = 0; rr < MP; ++rr) {
     const int w = tid + nthr * rr;
-    if (w >= nslot * ny) continue;
-    const int f = fdiv(w, ny, log2_or_neg(ny)), k = w - f * ny;
-    pack_inverse(X + (2 * f) * ny, X + (2 * f + 1) * ny, Z + f * ny, ny, d.dct_y, csr[rr], k);
+    if (w >= nel) continue;
+    const int f = fdiv(w, ny, ly), k = w - f * ny;
+    pack_inverse(X + (2 * f) * st, X + (2 * f + 1) * st, Z + f * st, ny, d.dct_y, csr[rr], k);
   }
   __syncthreads();
-  cplx<T> *R2 = fft_run(Z, (Z == A) ? B : A, d.fy, TW, nslot, ny);
+  cplx<T> *R2 = fft_run(Z, (Z == A) ? B : A, d.fy, TW, nslot, st);
   const T inv_n = (T)(1.0 / (double)ny);
-  for (int w = tid; w < nslot * ny; w += nthr) {
-    const int f = fdiv(w, ny, log2_or_neg(ny)), m = w - f * ny;
+  for (int w = tid; w < nel; w += nthr) {
+    const int m = fdiv(w, nslot, ls), f = w - m * nslot;    // row-walking stores
     const int ca = c0 + 2 * f, cb = ca + 1;
+    const cplx<T> *Rf = R2 + f * st;
     T va, vb;
     if (d.dct_y) {
-      va = R2[f * ny + m].x * inv_n;
-      vb = -R2[f * ny + m].y * inv_n;
+      va = Rf[m].x * inv_n;
+      vb = -Rf[m].y * inv_n;
     } else {
       cplx<T> ha, hb;
-      split_pair(R2 + f * ny, ny, m, ha, hb);
+      split_pair(Rf, ny, m, ha, hb);
       va = (ha.x - ha.y) * inv_n;
       vb = (hb.x - hb.y) * inv_n;
     }
@@ -2301,10 +2308,14 @@ size_t lds_rows(size_t nx, size_t nslot) {
 }
 template <typename T>
 size_t lds_A(const aero2d_plan *p) { return lds_rows<T>(p->desc.nx, 1); }
+// LDS of a pass B block over nslot column pairs of length ny: two work buffers and 2 nslot real
+// rows at slot stride ny + 1, the twiddles
 template <typename T>
-size_t lds_B(const aero2d_plan *p) {
-  return p->nslotB * p->desc.ny * (2 * sizeof(cplx<T>) + 2 * sizeof(T)) + p->desc.ny * sizeof(cplx<T>);
+size_t lds_cols(size_t ny, size_t nslot) {
+  return nslot * (ny + 1) * (2 * sizeof(cplx<T>) + 2 * sizeof(T)) + ny * sizeof(cplx<T>);
 }
+template <typename T>
+size_t lds_B(const aero2d_plan *p) { return lds_cols<T>(p->desc.ny, p->nslotB); }
 
 // right-hand side -> psi (passes A, B, C) on `stream`
 template <typename T>
@@ -2754,7 +2765,7 @@ int run_slots(const aero2d_plan *p, int want) {
     while ((size_t)(2 * s) * ny <= (size_t)kRowFill) s *= 2;
   }
   while (s > 1 && ((size_t)s * ny > (size_t)kMaxLen ||
-                   (size_t)s * ny * (2 * sizeof(cplx<T>) + 2 * sizeof(T)) + ny * sizeof(cplx<T>) > 65536))
+                   lds_cols<T>(ny, s) > 65536))
     s /= 2;
   return s;
 }
@@ -2839,7 +2850,7 @@ int ensemble_build(aero2d_ensemble *e) {
     const int cpb = 2 * b.nslotB;
     for (int bx = 0; bx < (s.nx + cpb - 1) / cpb; ++bx) cols.push_back(make_int4(r, bx, 0, 0));
     e->ldsA = std::max(e->ldsA, lds_rows<T>(s.nx, b.nslotA));
-    e->ldsB = std::max(e->ldsB, b.nslotB * s.ny * (2 * sizeof(cplx<T>) + 2 * sizeof(T)) + s.ny * sizeof(cplx<T>));
+    e->ldsB = std::max(e->ldsB, lds_cols<T>(s.ny, b.nslotB));
   }
   e->ntiles = (int)tiles.size();
   e->nrows = (int)rows.size();
@@ -3190,7 +3201,7 @@ int static_solve(unsigned char *base, hipStream_t st, int nx, int ny, double h, 
                      (const double *)lam_x, (const double *)lam_y, inv);
   const int ntA = transform_threads(nx), ntB = transform_threads(ny);
   const size_t ldsA = 3 * nx * sizeof(cplx<T>) + 2 * nx * sizeof(T) + 64 * 8;
-  const size_t ldsB = ny * (2 * sizeof(cplx<T>) + 2 * sizeof(T)) + ny * sizeof(cplx<T>);
+  const size_t ldsB = lds_cols<T>(ny, 1);
   by_threads(ntA, [&](auto nt) {
     constexpr int NT = decltype(nt)::value;
     hipLaunchKernelGGL((k_poisA<T, NT>), dim3((ny + 1) / 2), dim3(NT), ldsA, st, d, (const T *)nullptr,

@@ -1519,14 +1519,15 @@ __device__ __forceinline__ void inv_unpack(const cplx<T> *R, int n, int m, int d
 // (core.py:577-579)
 template <typename T>
 __device__ __forceinline__ T div_at(const Dev3<T> &d, const T *u, const T *v, const T *w, int k, int j, int i, T corr) {
-  const int nx = d.nx, ny = d.ny;
-  const T *ur = u + ((size_t)k * ny + j) * (nx + 1);
-  T u1 = ur[i + 1];
-  if (i + 1 == nx) u1 = u1 + corr;
-  const T *vr = v + ((size_t)k * (ny + 1) + j) * nx + i;
-  const size_t pl = (size_t)ny * nx;
-  const T *wr = w + ((size_t)k * ny + j) * nx + i;
-  return (((u1 - ur[i]) + (vr[nx] - vr[0])) + (wr[pl] - wr[0])) * d.inv_h;
+  // element offsets in 32 bits (the face arrays hold fewer than 2^32 values)
+  const unsigned nx = d.nx, ny = d.ny, kk = k, jj = j, ii = i;
+  const T *ur = u + (kk * ny + jj) * (nx + 1);
+  T u1 = ur[ii + 1];
+  if (i + 1 == d.nx) u1 = u1 + corr;
+  const T *vr = v + (kk * (ny + 1) + jj) * nx + ii;
+  const unsigned pl = ny * nx;
+  const T *wr = w + (kk * ny + jj) * nx + ii;
+  return (((u1 - ur[ii]) + (vr[nx] - vr[0])) + (wr[pl] - wr[0])) * d.inv_h;
 }
 
 // ------------------------------------------------------------------ diagnostics
@@ -1700,8 +1701,10 @@ __global__ void __launch_bounds__(kThreads) k_p1(const Dev3<T> *__restrict__ dp,
       if (ra < nrows) xa = f[(size_t)ra * nx + q];
       if (rb < nrows) xb = f[(size_t)rb * nx + q];
     } else {
-      if (ra < nrows) xa = div_at(d, u, v, w, ra / d.ny, ra % d.ny, q, corr);
-      if (rb < nrows) xb = div_at(d, u, v, w, rb / d.ny, rb % d.ny, q, corr);
+      const int ly = log2_or_neg(d.ny);
+      const int ka = fdiv(ra, d.ny, ly), kb = fdiv(rb, d.ny, ly);
+      if (ra < nrows) xa = div_at(d, u, v, w, ka, ra - ka * d.ny, q, corr);
+      if (rb < nrows) xb = div_at(d, u, v, w, kb, rb - kb * d.ny, q, corr);
     }
     A[sl * st + n] = {xa, xb};
   }

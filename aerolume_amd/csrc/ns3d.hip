@@ -46,6 +46,7 @@ namespace aero {
 namespace ns3d {
 
 constexpr int kThreads = 256;        // Poisson passes
+constexpr int kP5Threads = 512;     // the inverse x pass (8 row pairs per block)
 constexpr int kPlaneThreads = 1024;  // the fused (z, y) plane pass
 constexpr int kPlaneMax = 20480;     // longest plane (ny nz) of the fused pass: 20 values per thread
 constexpr int kTileThreads = 1024;   // stage / final tiles (16 waves)
@@ -1994,7 +1995,7 @@ __global__ void __launch_bounds__(kPlaneThreads) k_planes(const Dev3<T> *__restr
 // P5: rows (pairs) along x, inverse -> psi; the step-averaged pressure p (core.py:874-875):
 // stage 0 p = w psi, stage 1 p = p + w psi, stage 2 p = (p + psi) (1 / dt).
 template <typename T>
-__global__ void __launch_bounds__(kThreads) k_p5(const Dev3<T> *__restrict__ dp, Call3<T> c, T *__restrict__ psi,
+__global__ void __launch_bounds__(kP5Threads) k_p5(const Dev3<T> *__restrict__ dp, Call3<T> c, T *__restrict__ psi,
                                                  T *__restrict__ p, int stage) {
   extern __shared__ unsigned char smem[];
   const Dev3<T> &d = *dp;
@@ -2005,24 +2006,24 @@ __global__ void __launch_bounds__(kThreads) k_p5(const Dev3<T> *__restrict__ dp,
   // the real rows (2 ns of st values) share the transform's second buffer, which is free before
   // the transform; after it they move to whichever buffer does not hold the result
   T *X = reinterpret_cast<T *>(B);
-  for (int m = threadIdx.x; m < nx; m += kThreads) TW[m] = d.tw[0][m];
+  for (int m = threadIdx.x; m < nx; m += kP5Threads) TW[m] = d.tw[0][m];
   const int r0 = 2 * ns * blockIdx.x;
   const int dct = d.dct[0];
   if (d.planes) {
     // spectrum (nx, rows): lanes walk the block's rows first
     const int l2 = log2_or_neg(2 * ns);
-    for (int e = threadIdx.x; e < 2 * ns * nx; e += kThreads) {
+    for (int e = threadIdx.x; e < 2 * ns * nx; e += kP5Threads) {
       const int k = fdiv(e, 2 * ns, l2), rr = e - k * 2 * ns;
       X[rr * st + k] = r0 + rr < nrows ? d.F[(size_t)k * nrows + r0 + rr] : (T)0;
     }
   } else {
-    for (int e = threadIdx.x; e < 2 * ns * nx; e += kThreads) {
+    for (int e = threadIdx.x; e < 2 * ns * nx; e += kP5Threads) {
       const int rr = fdiv(e, nx, log2_or_neg(nx)), k = e - rr * nx;
       X[rr * st + k] = r0 + rr < nrows ? d.F[(size_t)(r0 + rr) * nx + k] : (T)0;
     }
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < ns * nx; e += kThreads) {
+  for (int e = threadIdx.x; e < ns * nx; e += kP5Threads) {
     const int sl = fdiv(e, nx, log2_or_neg(nx)), k = e - sl * nx;
     A[sl * st + k] = pack_inv(X + (2 * sl) * st, X + (2 * sl + 1) * st, nx, dct, dct ? d.cs[0][k] : cplx<T>{0, 0}, k);
   }
@@ -2033,7 +2034,7 @@ __global__ void __launch_bounds__(kThreads) k_p5(const Dev3<T> *__restrict__ dp,
   T rdt = 0;
   if (p && stage == 2) rdt = (T)1 / step_dt(d, c);
   const T w = (T)kForceW[stage];
-  for (int e = threadIdx.x; e < ns * nx; e += kThreads) {
+  for (int e = threadIdx.x; e < ns * nx; e += kP5Threads) {
     const int sl = fdiv(e, nx, log2_or_neg(nx)), m = e - sl * nx;
     T va, vb;
     inv_unpack(R + sl * st, nx, m, dct, inv_n, va, vb);
@@ -2042,7 +2043,7 @@ __global__ void __launch_bounds__(kThreads) k_p5(const Dev3<T> *__restrict__ dp,
     X[(2 * sl + 1) * st + q] = vb;
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < 2 * ns * nx; e += kThreads) {
+  for (int e = threadIdx.x; e < 2 * ns * nx; e += kP5Threads) {
     const int rr = fdiv(e, nx, log2_or_neg(nx)), q = e - rr * nx;
     if (r0 + rr >= nrows) continue;
     const size_t idx = (size_t)(r0 + rr) * nx + q;
@@ -2487,7 +2488,7 @@ int poisson_passes(aero3d_plan *p, hipStream_t st, const Dev3<T> *dev, Call3<T> 
     hipLaunchKernelGGL(k_py<T>, g2, dim3(kThreads), p->lds2, st, dev, 1);
   }
   const int g5 = (rows + 2 * p->ns5 - 1) / (2 * p->ns5);
-  hipLaunchKernelGGL(k_p5<T>, dim3(g5), dim3(kThreads), p->lds5, st, dev, c, psi, pr, stage);
+  hipLaunchKernelGGL(k_p5<T>, dim3(g5), dim3(kP5Threads), p->lds5, st, dev, c, psi, pr, stage);
   AERO_HIP_CHECK(hipGetLastError());
   return AERO_OK;
 }
@@ -2881,13 +2882,13 @@ int aero3d_plan_create(const aero3d_desc *desc, aero3d_plan **out) {
     p->planes = !p->dbl && ny % 2 == 0 && nz % 2 == 0 && ny * nz <= (size_t)kPlaneMax &&
                 p->lds_planes <= (size_t)max_lds;
   }
-  // the forward x pass takes 8 row pairs per block with the plane pass (64-byte runs of each
-  // wavenumber in the (x, z, y) spectrum: 29.4 -> 26.8 us at 256x128x128; the inverse pass is
-  // slower so, 32.2 -> 40.8 us), 4 otherwise
+  // the forward x pass takes 8 row pairs per 256-thread block with the plane pass (64-byte runs of
+  // each wavenumber in the (x, z, y) spectrum: 29.4 -> 26.8 us at 256x128x128), 4 otherwise; the
+  // inverse pass 8 per 512-thread block (31.8 -> 27.4 us; on 256 threads 8 pairs lose, 40.8 us)
   p->lds1 = pick_slots((int)nx, es, 0, p->planes ? 8 : 4, p->ns1);
   p->lds2 = pick_slots((int)ny, es, 2, 8, p->ns2);
   p->lds3 = pick_slots((int)nz, es, 2, 8, p->ns3);
-  p->lds5 = pick_slots((int)nx, es, 0, 4, p->ns5);   // its real rows share a transform buffer
+  p->lds5 = pick_slots((int)nx, es, 0, 8, p->ns5);   // its real rows share a transform buffer
   const size_t nu_ = nz * ny * (nx + 1), nv_ = nz * (ny + 1) * nx, nw_ = (nz + 1) * ny * nx, nc = nz * ny * nx;
   struct { void **ptr; size_t n; } tab[] = {
       {&p->dev, sizeof(Dev3<double>)},

@@ -174,6 +174,10 @@ using gptr = __attribute__((address_space(1))) X *;
 template <typename X>
 using gptr = X *;
 #endif
+// structs holding gptr members are filled on the host and read on the device: both passes must
+// lay them out alike
+static_assert(sizeof(gptr<float>) == sizeof(float *) && alignof(gptr<float>) == alignof(float *),
+              "global pointers must have the size and alignment of plain pointers");
 // X in a parameter that takes no part in template argument deduction (a global pointer then
 // converts to the parameter's plain pointer type instead of failing the deduction)
 template <typename X>
